@@ -1,0 +1,207 @@
+/*
+ * qdimsrg.h -- C ABI of the MI355X (gfx950) IM-SRG(2) commutator / BCH engine.
+ *
+ * Plain C, no GPU or framework types in any signature.  Every entry point
+ * returns QD_OK (0) or a negative QD_E* code; qd_last_error() gives the text
+ * of the most recent failure on the calling thread.
+ *
+ * Data model (the reference's operator API, src/ME_general.f90:26-62):
+ *   qd_context  one channel structure: N electrons in R oscillator shells,
+ *               M = R(R+1) spin-orbitals, (M_L, M_S) two-body blocks and the
+ *               cross-coupled (Pandya) blocks; owns one HIP stream and the
+ *               device workspaces.  Replaces reindex / build_cross_coupled
+ *               (src/ME_general.f90:956-1116, 363-448).
+ *   qd_op       one device-resident type(full_ham): neq doubles in the
+ *               reference's vectorize order (src/ME_general.f90:1871-1989) --
+ *               E0, fhh, fpp, fph, then per block Vhhhh, Vpppp, Vpphh, Vphph,
+ *               Vppph, Vphhh, column-major -- plus the integer herm flag.
+ *   qd_cc       one device-resident type(cc_mat): per CC block an r x nb
+ *               column-major matrix, plus the real herm flag
+ *               (src/ME_general.f90:13-24).
+ * Host vectors passed to upload/download are exactly the reference's
+ * vectorize/repackage vectors, so the ODE state of src/adams_ode.f90 and a
+ * Fortran full_ham round-trip through them unchanged.
+ *
+ * All work of one context runs in order on its stream; entry points that
+ * return a host scalar synchronise that stream.  A context is not reentrant:
+ * calls on one context from several threads must be serialised by the caller
+ * (the reference's callers are single-threaded).
+ */
+#ifndef QDIMSRG_H
+#define QDIMSRG_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define QD_OK 0
+#define QD_E_ARG (-1)      /* bad argument (null handle, size mismatch, bad herm) */
+#define QD_E_HIP (-2)      /* HIP runtime error (allocation, launch, copy)       */
+#define QD_E_LAYOUT (-3)   /* channel structure not supported                    */
+#define QD_E_NUMERIC (-4)  /* non-finite value met where the reference would      */
+                           /* silently propagate it (flow drivers only)           */
+
+typedef struct qd_context qd_context;
+typedef struct qd_op qd_op;
+typedef struct qd_cc qd_cc;
+
+typedef struct {
+  int64_t nbody, nshell, msp, nblock, ncc;
+  int64_t neq;      /* doubles per operator (get_num_elements, src/ME_general.f90:1755) */
+  int64_t cc_size;  /* doubles per CC matrix set (sum of r*nb)                          */
+  int64_t n2b;      /* two-body doubles per operator                                    */
+} qd_info;
+
+const char *qd_last_error(void);
+
+/* ---- context: channel structure + device tables -------------------------
+ * Replaces arrange_states / reindex / build_cross_coupled
+ * (src/ME_general.f90:688-744, 956-1116, 363-448).  Holes are the first
+ * nbody states in states order (the reference's energy->states map is the
+ * identity, convertF src/ME_general.f90:1527-1536); nbody must fill shells. */
+int qd_context_create(int nbody, int nshell, qd_context **out);
+int qd_context_destroy(qd_context *ctx);
+int qd_context_info(const qd_context *ctx, qd_info *out);
+/* Export one host table for inspection: which = 0 block dims (nblock x 3:
+ * nhh,npp,nph), 1 flat offsets (nblock x 6), 2 i_array (M(M-1)/2 x 3: type
+ * 1..3, block, index; 0-based), 3 CC map (M*M x 2: block,row), 4 CC dims
+ * (ncc x 2: r, nb), 5 pair lists (hh, ph, pp of every block, concatenated,
+ * x 2: (i, j) with i < j).  len is the capacity of out in int64 elements; returns the count. */
+int64_t qd_context_table(const qd_context *ctx, int which, int64_t *out, int64_t len);
+/* The same tables without a context or a GPU (host-only channel builder);
+ * which = 6 gives (neq, cc_size, nblock, ncc). */
+int64_t qd_layout_table(int nbody, int nshell, int which, int64_t *out, int64_t len);
+/* Block until all work queued on the context's stream is done. */
+int qd_context_sync(qd_context *ctx);
+
+/* ---- operators: type(full_ham), allocate_everything --------------------
+ * src/IMSRG_tools.f90:465-537 (allocate_everything: zeroed, herm = +1). */
+int qd_op_create(qd_context *ctx, int herm, qd_op **out);
+int qd_op_destroy(qd_op *op);
+int qd_op_set_herm(qd_op *op, int herm);  /* full_ham%herm, +1 or -1 */
+int qd_op_get_herm(const qd_op *op, int *herm);
+/* repackage (src/ME_general.f90:1992-2109): host vectorize-order -> device */
+int qd_op_upload(qd_op *op, const double *vec, int64_t neq);
+/* vectorize (src/ME_general.f90:1871-1989): device -> host */
+int qd_op_download(const qd_op *op, double *vec, int64_t neq);
+/* Device address of the operator's neq doubles (for callers that keep their
+ * own ODE state on the GPU); valid until qd_op_destroy. */
+double *qd_op_device_ptr(qd_op *op);
+
+/* ---- cross-coupled matrices: type(cc_mat), copy_cc ---------------------
+ * src/IMSRG_tools.f90:437-461 (copy_cc: zeroed X). */
+int qd_cc_create(qd_context *ctx, double herm, qd_cc **out);
+int qd_cc_destroy(qd_cc *cc);
+int qd_cc_set_herm(qd_cc *cc, double herm);  /* cc_mat%herm (real) */
+int qd_cc_upload(qd_cc *cc, const double *x, int64_t cc_size);
+int qd_cc_download(const qd_cc *cc, double *x, int64_t cc_size);
+int qd_cc_zero(qd_context *ctx, qd_cc *cc);
+
+/* ---- commutator entry points (module commutators) ----------------------
+ * [r1, r2] pieces with the reference's accumulation contract: 111 overwrites
+ * the one-body part of r3, 121 and 221 add to it; 122 overwrites the
+ * two-body part, 222 adds to it.  herm flags are read from r1/r2. */
+/* commutator_110(m,n,r1,r2,C), src/commutators.f90:12-33 */
+int qd_commutator_110(qd_context *ctx, const qd_op *r1, const qd_op *r2, double *c);
+/* commutator_220(r1,r2,c), src/commutators.f90:36-70 */
+int qd_commutator_220(qd_context *ctx, const qd_op *r1, const qd_op *r2, double *c);
+/* commutator_111(r1,r2,r3), src/commutators.f90:137-195 */
+int qd_commutator_111(qd_context *ctx, const qd_op *r1, const qd_op *r2, qd_op *r3);
+/* commutator_121(r1,r2,r3), src/commutators.f90:73-134 */
+int qd_commutator_121(qd_context *ctx, const qd_op *r1, const qd_op *r2, qd_op *r3);
+/* commutator_221(r1,r2,r3,w1,w2), src/commutators.f90:198-352; w1/w2 are
+ * context workspace */
+int qd_commutator_221(qd_context *ctx, const qd_op *r1, const qd_op *r2, qd_op *r3);
+/* commutator_122(r1,r2,r3), src/commutators.f90:355-676 */
+int qd_commutator_122(qd_context *ctx, const qd_op *r1, const qd_op *r2, qd_op *r3);
+/* commutator_222(r1,r2,r3,w1,HCC,ETACC), src/commutators.f90:679-1132 with
+ * construct_cc_ints (src/ME_general.f90:923-952): hcc = CC of r2, etacc = CC
+ * of r1, both supplied by the caller (zero in the reference's Magnus driver,
+ * calc_cc-filled in its ODE RHS). */
+int qd_commutator_222(qd_context *ctx, const qd_op *r1, const qd_op *r2, qd_op *r3,
+                      const qd_cc *hcc, const qd_cc *etacc);
+/* calc_cc(H,HCC), src/commutators.f90:1135-1159 */
+int qd_calc_cc(qd_context *ctx, const qd_op *h, qd_cc *hcc);
+/* build_white(gen,rec), src/IMSRG_tools.f90:377-434: writes gen's fph and
+ * Vpphh only (the caller sets gen herm = -1). */
+int qd_build_white(qd_context *ctx, qd_op *gen, const qd_op *rec);
+
+/* ---- operator algebra (src/ME_general.f90:592-686, 2112-2138) ---------- */
+/* add_arrays(r1,s1,r2,s2,r3): r3 = s1*r1 + s2*r2 (r3 may alias r1 or r2) */
+int qd_add_arrays(qd_context *ctx, const qd_op *r1, double s1, const qd_op *r2, double s2, qd_op *r3);
+/* copy_arrays(r1,r2): r2 = r1 (values only; herm is not copied, as in the reference) */
+int qd_copy_arrays(qd_context *ctx, const qd_op *r1, qd_op *r2);
+int qd_set_to_zero(qd_context *ctx, qd_op *r);
+/* mat_2_norm(r): Frobenius norm of every entry except E0 */
+int qd_mat_2_norm(qd_context *ctx, const qd_op *r, double *out);
+/* MBPT2(rec), src/HFQDmod.f90:325-362 */
+int qd_mbpt2(qd_context *ctx, const qd_op *h, double *out);
+
+/* ---- whole commutator and flow-level entry points ----------------------
+ * Full [r1,r2] -> r3 (0b in r3's E0, also returned in *e0 if non-null), in
+ * the call order of BCH_EXPAND (src/magnus_IMSRG.f90:297-311) and dGam
+ * (src/gs_IMSRG.f90:256-268).  hcc/etacc as for qd_commutator_222; NULL
+ * means an all-zero CC (the reference's Magnus driver). */
+int qd_commutator(qd_context *ctx, const qd_op *r1, const qd_op *r2, qd_op *r3,
+                  const qd_cc *hcc, const qd_cc *etacc, double *e0);
+/* BCH_EXPAND(HS,G,H,...), src/magnus_IMSRG.f90:253-318: hs = sum_k cof(k)
+ * ad_G^k H with the reference's literal coefficients and its 1e-5 (REAL(4))
+ * relative truncation; *nterms = nested commutators taken (1..11). */
+int qd_bch_expand(qd_context *ctx, qd_op *hs, const qd_op *g, const qd_op *h,
+                  const qd_cc *hcc, const qd_cc *occ, int *nterms);
+/* MAGNUS_EXPAND(DG,G,ETA,...), src/magnus_IMSRG.f90:321-369, as shipped:
+ * dOmega = eta (the routine returns at :345 before its nested sum). */
+int qd_magnus_expand(qd_context *ctx, qd_op *dg, const qd_op *g, const qd_op *eta);
+/* euler_step(G,DG,s,stp), src/magnus_IMSRG.f90:372-384: G += stp*DG, s += stp */
+int qd_euler_step(qd_context *ctx, qd_op *g, const qd_op *dg, double *s, double stp);
+/* dGam(t,yp,H,HCC,ETACC), src/gs_IMSRG.f90:223-273: der = [eta_White(H), H]
+ * with calc_cc of H into hcc and of eta into etacc (both overwritten). */
+int qd_dgam(qd_context *ctx, const qd_op *h, qd_op *der, qd_cc *hcc, qd_cc *etacc);
+
+/* s_evolve, src/magnus_IMSRG.f90:194-248: the Magnus flow with the White
+ * generator from H(0) = h0, Euler steps on Omega, BCH for H(s), reject-and-
+ * halve when ||eta|| grows past 1.5x its initial value, stop when
+ * |E_MBPT2(H(s))| <= crit.  Literal constants are the reference's REAL(4)
+ * values widened to REAL(8).  pandya = 0 reproduces the reference as shipped
+ * (zero CC inputs); 1 fills the CC matrices with calc_cc every commutator. */
+typedef struct {
+  double stp;        /* initial step (reference: REAL(4) 0.7)                      */
+  double conv;       /* stop when |MBPT2(H(s))| <= conv (reference: REAL(4) 1e-9)   */
+  int max_iter;      /* safety cap on accepted + rejected steps                     */
+  int pandya;        /* 0 as shipped, 1 with calc_cc-filled CC inputs                */
+  int verbose;       /* 1: print the reference's per-iteration line to stdout       */
+  int warm_start;    /* 1: start from the Omega passed in (H(s) = BCH(Omega, H(0)))  */
+} qd_flow_params;
+typedef struct {
+  double s, e0, crit;          /* final flow parameter, E0(s), |MBPT2(H(s))|          */
+  int iterations, rejected;    /* accepted and rejected Euler steps                  */
+  int64_t bch_terms;           /* nested commutators evaluated in all BCH expansions  */
+  int converged;
+  int trace_rows;              /* rows written to trace                              */
+} qd_flow_result;
+/* Fills *p with the reference's constants. */
+void qd_flow_params_default(qd_flow_params *p);
+/* Runs the flow.  omega on entry is the starting Omega (zero for a cold
+ * start, a predicted Omega for a warm start); on exit the final Omega.
+ * hs receives H(s).  trace (optional, capacity trace_len rows of 6 doubles:
+ * iteration, s, E0, nrm1, nrm2, crit) receives the per-step log. */
+int qd_magnus_flow(qd_context *ctx, const qd_op *h0, qd_op *hs, qd_op *omega,
+                   const qd_flow_params *p, qd_flow_result *res, double *trace, int trace_len);
+
+/* ---- timing of the dominant kernels (HIP events on the context stream) --
+ * Accumulates per-kernel-class device time while enabled.  Classes: 0 the
+ * product GEMM launch (ladder intermediates + Pandya products), 1 the
+ * accumulating ladder GEMM launch, 2 two-body gather (122 + ladder combine +
+ * Pandya scatter), 3 one-body, 4 calc_cc, 5 other. */
+int qd_timing_enable(qd_context *ctx, int on);
+int qd_timing_read(qd_context *ctx, double *ms_per_class, int64_t *launches_per_class, int nclass);
+/* Algorithmic FP64 flops of one full commutator's GEMM launches: the product
+ * launch without / with its Pandya part, and the accumulating launch. */
+int qd_gemm_flops(const qd_context *ctx, double *ladder, double *pandya, double *acc);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* QDIMSRG_H */

@@ -1,0 +1,281 @@
+"""MI355X IM-SRG(2) commutator / BCH engine -- Python host mirror of the C ABI.
+
+Mirrors the reference's operator API (src/commutators.f90, src/IMSRG_tools.f90,
+src/magnus_IMSRG.f90, src/gs_IMSRG.f90): ``Context`` is the channel structure,
+``Operator`` a device-resident ``type(full_ham)``, ``CCMat`` a device-resident
+``type(cc_mat)``; the module functions keep the reference's procedure names and
+argument order.  Host vectors are the reference's ``vectorize`` vectors.
+All compute runs in libqdimsrg.so on the GPU; nothing here computes physics.
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+
+from . import _lib
+from ._lib import FlowParams, FlowResult, QDError, check
+
+__all__ = ["Context", "Operator", "CCMat", "QDError", "commutator_110", "commutator_220", "commutator_111",
+           "commutator_121", "commutator_221", "commutator_122", "commutator_222", "calc_cc", "build_white",
+           "add_arrays", "copy_arrays", "set_to_zero", "mat_2_norm", "MBPT2", "commutator", "BCH_EXPAND",
+           "MAGNUS_EXPAND", "euler_step", "dGam", "magnus_flow"]
+
+
+def _dp(a: np.ndarray):
+    return a.ctypes.data_as(_lib.dp)
+
+
+class Context:
+    """Channel structure of N electrons in R shells (reindex + build_cross_coupled)."""
+
+    def __init__(self, nbody: int, nshell: int):
+        self.L = _lib.load()
+        h = C.c_void_p()
+        check(self.L.qd_context_create(int(nbody), int(nshell), C.byref(h)))
+        self.h = h
+        info = _lib.Info()
+        check(self.L.qd_context_info(self.h, C.byref(info)))
+        self.nbody, self.nshell, self.msp = info.nbody, info.nshell, info.msp
+        self.nblock, self.ncc, self.neq = info.nblock, info.ncc, info.neq
+        self.cc_size, self.n2b = info.cc_size, info.n2b
+
+    def table(self, which: int) -> np.ndarray:
+        n = self.L.qd_context_table(self.h, which, None, 0)
+        if n < 0:
+            check(int(n))
+        out = np.zeros(max(int(n), 1), dtype=np.int64)
+        check(int(self.L.qd_context_table(self.h, which, out.ctypes.data_as(_lib.i64p), int(n))) - int(n))
+        return out[:n]
+
+    def sync(self):
+        check(self.L.qd_context_sync(self.h))
+
+    def operator(self, herm: int = 1, vec: np.ndarray | None = None) -> "Operator":
+        op = Operator(self, herm)
+        if vec is not None:
+            op.upload(vec)
+        return op
+
+    def cc(self, herm: float = 1.0) -> "CCMat":
+        return CCMat(self, herm)
+
+    def timing(self, on: bool):
+        check(self.L.qd_timing_enable(self.h, int(bool(on))))
+
+    def timing_read(self):
+        ms = np.zeros(6)
+        n = np.zeros(6, dtype=np.int64)
+        check(self.L.qd_timing_read(self.h, _dp(ms), n.ctypes.data_as(_lib.i64p), 6))
+        return ms, n
+
+    def gemm_flops(self):
+        a, b, c = C.c_double(), C.c_double(), C.c_double()
+        check(self.L.qd_gemm_flops(self.h, C.byref(a), C.byref(b), C.byref(c)))
+        return dict(products=a.value, pandya=b.value, accumulate=c.value)
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.L.qd_context_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Operator:
+    """Device-resident full_ham (allocate_everything: zeroed)."""
+
+    def __init__(self, ctx: Context, herm: int = 1):
+        self.ctx = ctx
+        h = C.c_void_p()
+        check(ctx.L.qd_op_create(ctx.h, int(herm), C.byref(h)))
+        self.h = h
+
+    @property
+    def herm(self) -> int:
+        x = C.c_int()
+        check(self.ctx.L.qd_op_get_herm(self.h, C.byref(x)))
+        return x.value
+
+    @herm.setter
+    def herm(self, v: int):
+        check(self.ctx.L.qd_op_set_herm(self.h, int(v)))
+
+    def upload(self, vec: np.ndarray):
+        v = np.ascontiguousarray(vec, dtype=np.float64)
+        check(self.ctx.L.qd_op_upload(self.h, _dp(v), v.size))
+
+    def download(self) -> np.ndarray:
+        v = np.empty(self.ctx.neq)
+        check(self.ctx.L.qd_op_download(self.h, _dp(v), v.size))
+        return v
+
+    def close(self):
+        if getattr(self, "h", None) and self.ctx.h:
+            self.ctx.L.qd_op_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class CCMat:
+    """Device-resident cc_mat (copy_cc: zeroed)."""
+
+    def __init__(self, ctx: Context, herm: float = 1.0):
+        self.ctx = ctx
+        h = C.c_void_p()
+        check(ctx.L.qd_cc_create(ctx.h, float(herm), C.byref(h)))
+        self.h = h
+
+    def set_herm(self, herm: float):
+        check(self.ctx.L.qd_cc_set_herm(self.h, float(herm)))
+
+    def upload(self, x: np.ndarray):
+        v = np.ascontiguousarray(x, dtype=np.float64)
+        check(self.ctx.L.qd_cc_upload(self.h, _dp(v), v.size))
+
+    def download(self) -> np.ndarray:
+        v = np.empty(self.ctx.cc_size)
+        check(self.ctx.L.qd_cc_download(self.h, _dp(v), v.size))
+        return v
+
+    def zero(self):
+        check(self.ctx.L.qd_cc_zero(self.ctx.h, self.h))
+
+    def close(self):
+        if getattr(self, "h", None) and self.ctx.h:
+            self.ctx.L.qd_cc_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+# ---- reference procedure names (src/commutators.f90) ----------------------
+def _L(op):
+    return op.ctx.L, op.ctx.h
+
+
+def commutator_110(r1: Operator, r2: Operator) -> float:
+    L, h = _L(r1)
+    c = C.c_double()
+    check(L.qd_commutator_110(h, r1.h, r2.h, C.byref(c)))
+    return c.value
+
+
+def commutator_220(r1: Operator, r2: Operator) -> float:
+    L, h = _L(r1)
+    c = C.c_double()
+    check(L.qd_commutator_220(h, r1.h, r2.h, C.byref(c)))
+    return c.value
+
+
+def commutator_111(r1, r2, r3):
+    L, h = _L(r1); check(L.qd_commutator_111(h, r1.h, r2.h, r3.h))
+
+
+def commutator_121(r1, r2, r3):
+    L, h = _L(r1); check(L.qd_commutator_121(h, r1.h, r2.h, r3.h))
+
+
+def commutator_221(r1, r2, r3):
+    L, h = _L(r1); check(L.qd_commutator_221(h, r1.h, r2.h, r3.h))
+
+
+def commutator_122(r1, r2, r3):
+    L, h = _L(r1); check(L.qd_commutator_122(h, r1.h, r2.h, r3.h))
+
+
+def commutator_222(r1, r2, r3, HCC: CCMat, ETACC: CCMat):
+    L, h = _L(r1); check(L.qd_commutator_222(h, r1.h, r2.h, r3.h, HCC.h, ETACC.h))
+
+
+def calc_cc(H: Operator, HCC: CCMat):
+    L, h = _L(H); check(L.qd_calc_cc(h, H.h, HCC.h))
+
+
+def build_white(gen: Operator, rec: Operator):
+    L, h = _L(gen); check(L.qd_build_white(h, gen.h, rec.h))
+
+
+def add_arrays(r1, s1, r2, s2, r3):
+    L, h = _L(r1); check(L.qd_add_arrays(h, r1.h, float(s1), r2.h, float(s2), r3.h))
+
+
+def copy_arrays(r1, r2):
+    L, h = _L(r1); check(L.qd_copy_arrays(h, r1.h, r2.h))
+
+
+def set_to_zero(r):
+    L, h = _L(r); check(L.qd_set_to_zero(h, r.h))
+
+
+def mat_2_norm(r) -> float:
+    L, h = _L(r)
+    x = C.c_double()
+    check(L.qd_mat_2_norm(h, r.h, C.byref(x)))
+    return x.value
+
+
+def MBPT2(r) -> float:
+    L, h = _L(r)
+    x = C.c_double()
+    check(L.qd_mbpt2(h, r.h, C.byref(x)))
+    return x.value
+
+
+def commutator(r1, r2, r3, HCC: CCMat | None = None, ETACC: CCMat | None = None) -> float:
+    """Full [r1, r2] -> r3 in one pass; CC sets None = zero (Magnus as shipped)."""
+    L, h = _L(r1)
+    e0 = C.c_double()
+    check(L.qd_commutator(h, r1.h, r2.h, r3.h, HCC.h if HCC else None, ETACC.h if ETACC else None,
+                          C.byref(e0)))
+    return e0.value
+
+
+def BCH_EXPAND(HS, G, H, HCC=None, OCC=None) -> int:
+    L, h = _L(HS)
+    nt = C.c_int()
+    check(L.qd_bch_expand(h, HS.h, G.h, H.h, HCC.h if HCC else None, OCC.h if OCC else None, C.byref(nt)))
+    return nt.value
+
+
+def MAGNUS_EXPAND(DG, G, ETA):
+    L, h = _L(DG); check(L.qd_magnus_expand(h, DG.h, G.h, ETA.h))
+
+
+def euler_step(G, DG, s: float, stp: float) -> float:
+    L, h = _L(G)
+    x = C.c_double(s)
+    check(L.qd_euler_step(h, G.h, DG.h, C.byref(x), float(stp)))
+    return x.value
+
+
+def dGam(H, DER, HCC, ETACC):
+    L, h = _L(H); check(L.qd_dgam(h, H.h, DER.h, HCC.h, ETACC.h))
+
+
+def magnus_flow(ctx: Context, H0: Operator, HS: Operator, OMEGA: Operator, pandya: bool = False,
+                warm_start: bool = False, verbose: bool = False, max_iter: int = 100000,
+                conv: float | None = None, trace_len: int = 4096):
+    """s_evolve (src/magnus_IMSRG.f90:194-248) on the GPU; returns (result, trace)."""
+    p = FlowParams()
+    ctx.L.qd_flow_params_default(C.byref(p))
+    p.pandya, p.warm_start, p.verbose, p.max_iter = int(pandya), int(warm_start), int(verbose), int(max_iter)
+    if conv is not None:
+        p.conv = float(conv)
+    res = FlowResult()
+    trace = np.zeros((trace_len, 6))
+    check(ctx.L.qd_magnus_flow(ctx.h, H0.h, HS.h, OMEGA.h, C.byref(p), C.byref(res), _dp(trace), trace_len))
+    return res, trace[:res.trace_rows]

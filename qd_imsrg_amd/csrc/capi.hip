@@ -1,0 +1,895 @@
+// C ABI of the engine: contexts, device-resident operators, the commutator
+// pipeline, BCH_EXPAND and the Magnus flow (s_evolve).
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "qd_kernels.h"
+
+using namespace qd;
+
+namespace {
+thread_local std::string g_err;
+
+int fail(int code, const char *fmt, ...) {
+  char buf[512];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof buf, fmt, ap);
+  va_end(ap);
+  g_err = buf;
+  return code;
+}
+
+#define HIPC(expr)                                                                     \
+  do {                                                                                 \
+    hipError_t e_ = (expr);                                                            \
+    if (e_ != hipSuccess) return fail(QD_E_HIP, "%s: %s", #expr, hipGetErrorString(e_)); \
+  } while (0)
+
+// BCH_EXPAND coefficients exactly as written at src/magnus_IMSRG.f90:265-267
+const double BCH_COF[11] = {1.0, 0.5, 0.166666666666666666, 0.04166666666666666, 0.0083333333333333333,
+                            .001388888888888, 1.984126984e-4, 2.48015873e-5, 2.755731922e-6,
+                            2.755731922e-7, 2.505210839e-8};
+// REAL(4) literals of the reference widened to REAL(8)
+const double R4_1EM5 = 9.999999747378752e-06;   // conv, src/magnus_IMSRG.f90:259
+const double R4_1EM9 = 9.999999717180685e-10;   // conv_criteria, src/magnus_IMSRG.f90:10
+const double R4_0P7 = 0.699999988079071;        // stp = .7, src/magnus_IMSRG.f90:124
+const double R4_1EM3 = 0.0010000000474974513;   // + 1e-3, src/magnus_IMSRG.f90:226
+
+constexpr int NCLASS = 6;
+enum { TC_PRODUCTS = 0, TC_ACCUM = 1, TC_TWOBODY = 2, TC_ONEBODY = 3, TC_CALCCC = 4, TC_OTHER = 5 };
+
+struct GemmList {
+  GemmProb *probs = nullptr;
+  GemmSeg *segs = nullptr;
+  GemmTile *tiles = nullptr;
+  int ntiles = 0;
+  double flops = 0;  // algorithmic 2*M*N*K summed over segments
+};
+}  // namespace
+
+struct qd_context {
+  Channels ch;
+  DevTab dt{};
+  hipStream_t stream = nullptr;
+  std::vector<void *> allocs;
+  double *w1 = nullptr, *w2 = nullptr, *w3 = nullptr, *P = nullptr;
+  double *partials = nullptr, *scal = nullptr;  // reduction partials, 8 device scalars
+  double *host_scal = nullptr;                  // pinned mirror of scal
+  Seg2 *segs2 = nullptr; Tile2 *tiles2 = nullptr; int ntiles2 = 0;
+  Seg2 *pphh_segs = nullptr; Tile2 *pphh_tiles = nullptr; int npphh_segs = 0, npphh_tiles = 0;
+  Seg2 *cc_segs = nullptr; Tile2 *cc_tiles = nullptr; int ncc_tiles = 0;
+  GemmList g_all, g_acc;  // products (G1 + Pandya, Pandya tiles last) and accumulating products
+  int g1_ntiles = 0;      // leading G1-only tiles of g_all
+  double pandya_flops = 0;
+  // scratch operators and CC sets of the flow drivers (created on first use)
+  qd_op *bch_a = nullptr, *bch_b = nullptr;
+  qd_cc *cc_t1 = nullptr, *cc_t2 = nullptr;
+  // timing
+  bool timing = false;
+  struct Ev { hipEvent_t a, b; int cls; };
+  std::vector<Ev> events;
+  double t_ms[NCLASS] = {};
+  int64_t t_n[NCLASS] = {};
+};
+
+struct qd_op {
+  qd_context *ctx;
+  double *d;
+  int herm;
+};
+struct qd_cc {
+  qd_context *ctx;
+  double *d;
+  double herm;
+  bool zero;  // known to be all zeros (skips the Pandya work, exactly)
+};
+
+namespace {
+
+template <class T>
+int dev_upload(qd_context *c, const std::vector<T> &h, T **out) {
+  size_t bytes = std::max<size_t>(h.size(), 1) * sizeof(T);
+  void *p = nullptr;
+  HIPC(hipMalloc(&p, bytes));
+  c->allocs.push_back(p);
+  if (!h.empty()) HIPC(hipMemcpy(p, h.data(), h.size() * sizeof(T), hipMemcpyHostToDevice));
+  *out = (T *)p;
+  return QD_OK;
+}
+
+int dev_alloc(qd_context *c, size_t bytes, void **out) {
+  void *p = nullptr;
+  HIPC(hipMalloc(&p, std::max<size_t>(bytes, 8)));
+  c->allocs.push_back(p);
+  HIPC(hipMemset(p, 0, std::max<size_t>(bytes, 8)));
+  *out = p;
+  return QD_OK;
+}
+
+struct Timer {  // brackets one launch with events when timing is on
+  qd_context *c;
+  int cls;
+  hipEvent_t a{}, b{};
+  Timer(qd_context *c_, int cls_) : c(c_), cls(cls_) {
+    if (c->timing) {
+      hipEventCreate(&a);
+      hipEventCreate(&b);
+      hipEventRecord(a, c->stream);
+    }
+  }
+  ~Timer() {
+    if (c->timing) {
+      hipEventRecord(b, c->stream);
+      c->events.push_back({a, b, cls});
+    }
+  }
+};
+
+void add_tiles(std::vector<Tile2> &tiles, int seg, int size) {
+  for (int s = 0; s < size; s += 256) tiles.push_back({seg, s});
+}
+
+struct GemmBuilder {
+  std::vector<GemmProb> probs;
+  std::vector<GemmSeg> segs;
+  std::vector<GemmTile> tiles;
+  double flops = 0;
+  void begin(int c_slot, int64_t c_off, int ldc, int M, int N, int beta1) {
+    GemmProb p{};
+    p.c_off = c_off; p.c_slot = c_slot; p.ldc = std::max(ldc, 1); p.M = M; p.N = N; p.beta1 = beta1;
+    p.seg0 = (int)segs.size(); p.nseg = 0;
+    probs.push_back(p);
+  }
+  void seg(int a_slot, int64_t a_off, int lda, int ta, int b_slot, int64_t b_off, int ldb, int tb, int K,
+           double alpha) {
+    GemmSeg s{};
+    s.a_off = a_off; s.b_off = b_off; s.a_slot = a_slot; s.b_slot = b_slot;
+    s.lda = std::max(lda, 1); s.ldb = std::max(ldb, 1); s.ta = ta; s.tb = tb; s.K = K; s.alpha = alpha;
+    segs.push_back(s);
+    probs.back().nseg++;
+    flops += 2.0 * probs.back().M * probs.back().N * K;
+  }
+  void end() {
+    const GemmProb &p = probs.back();
+    if (p.M <= 0 || p.N <= 0) { segs.resize(p.seg0); probs.pop_back(); return; }
+    int pi = (int)probs.size() - 1;
+    for (int tm = 0; tm * GEMM_TM < p.M; tm++)
+      for (int tn = 0; tn * GEMM_TN < p.N; tn++) tiles.push_back({pi, tm, tn});
+  }
+};
+
+int upload_gemm(qd_context *c, GemmBuilder &b, GemmList &g) {
+  int rc;
+  if ((rc = dev_upload(c, b.probs, &g.probs))) return rc;
+  if ((rc = dev_upload(c, b.segs, &g.segs))) return rc;
+  if ((rc = dev_upload(c, b.tiles, &g.tiles))) return rc;
+  g.ntiles = (int)b.tiles.size();
+  g.flops = b.flops;
+  return QD_OK;
+}
+
+int build_worklists(qd_context *c) {
+  const Channels &ch = c->ch;
+  int rc;
+  // ---- two-body segments and tiles
+  std::vector<Seg2> segs, pseg;
+  std::vector<Tile2> tiles, ptiles;
+  for (int q = 0; q < ch.nblock; q++) {
+    const int nh = ch.nh[q], np = ch.np[q], nb = ch.nb[q];
+    const int H = ch.hh_start[q], X = ch.ph_start[q], Pp = ch.pp_start[q];
+    const int rows[6] = {nh, np, np, nb, np, nb}, cols[6] = {nh, np, nh, nb, nb, nh};
+    const int bra[6] = {H, Pp, Pp, X, Pp, X}, ket[6] = {H, Pp, H, X, X, H};
+    for (int s = 0; s < 6; s++) {
+      if (rows[s] * cols[s] == 0) continue;
+      Seg2 sg{ch.off[(size_t)q * 6 + s], rows[s], rows[s] * cols[s], q, s, bra[s], ket[s]};
+      add_tiles(tiles, (int)segs.size(), sg.size);
+      segs.push_back(sg);
+      if (s == PPHH) { add_tiles(ptiles, (int)pseg.size(), sg.size); pseg.push_back(sg); }
+    }
+  }
+  if ((rc = dev_upload(c, segs, &c->segs2))) return rc;
+  if ((rc = dev_upload(c, tiles, &c->tiles2))) return rc;
+  c->ntiles2 = (int)tiles.size();
+  if ((rc = dev_upload(c, pseg, &c->pphh_segs))) return rc;
+  if ((rc = dev_upload(c, ptiles, &c->pphh_tiles))) return rc;
+  c->npphh_segs = (int)pseg.size();
+  c->npphh_tiles = (int)ptiles.size();
+  // ---- CC segments
+  std::vector<Seg2> csegs;
+  std::vector<Tile2> ctiles;
+  for (int q = 0; q < ch.ncc; q++) {
+    const int r = ch.cc_r[q], nb = ch.cc_nb[q];
+    if (r * nb == 0) continue;
+    Seg2 sg{ch.cc_off[q], r, r * nb, q, 0, ch.cc_ab_start[q], ch.cc_hp_start[q]};
+    add_tiles(ctiles, (int)csegs.size(), sg.size);
+    csegs.push_back(sg);
+  }
+  if ((rc = dev_upload(c, csegs, &c->cc_segs))) return rc;
+  if ((rc = dev_upload(c, ctiles, &c->cc_tiles))) return rc;
+  c->ncc_tiles = (int)ctiles.size();
+
+  // ---- GEMM products (matrix_mult222 / commutator_221 intermediates)
+  GemmBuilder g1, g2;
+  for (int q = 0; q < ch.nblock; q++) {
+    const int nh = ch.nh[q], np = ch.np[q], nb = ch.nb[q];
+    const int64_t *o = &ch.off[(size_t)q * 6];
+    // w1.hhhh = r2.pphh^T r1.pphh
+    g1.begin(S_W1, o[HHHH], nh, nh, nh, 0); g1.seg(S_R2, o[PPHH], np, 1, S_R1, o[PPHH], np, 0, np, 1); g1.end();
+    // w1.pppp = r1.pphh r2.pphh^T
+    g1.begin(S_W1, o[PPPP], np, np, np, 0); g1.seg(S_R1, o[PPHH], np, 0, S_R2, o[PPHH], np, 1, nh, 1); g1.end();
+    // w1.phph = r2.ppph^T r1.ppph
+    g1.begin(S_W1, o[PHPH], nb, nb, nb, 0); g1.seg(S_R2, o[PPPH], np, 1, S_R1, o[PPPH], np, 0, np, 1); g1.end();
+    // w1.phhh = r2.ppph^T r1.pphh
+    g1.begin(S_W1, o[PHHH], nb, nb, nh, 0); g1.seg(S_R2, o[PPPH], np, 1, S_R1, o[PPHH], np, 0, np, 1); g1.end();
+    // w1.ppph = r1.pphh r2.phhh^T
+    g1.begin(S_W1, o[PPPH], np, np, nb, 0); g1.seg(S_R1, o[PPHH], np, 0, S_R2, o[PHHH], nb, 1, nh, 1); g1.end();
+    // w2.phph = r1.phhh r2.phhh^T
+    g1.begin(S_W2, o[PHPH], nb, nb, nb, 0); g1.seg(S_R1, o[PHHH], nb, 0, S_R2, o[PHHH], nb, 1, nh, 1); g1.end();
+    // w2.phhh = r1.ppph^T r2.pphh
+    g1.begin(S_W2, o[PHHH], nb, nb, nh, 0); g1.seg(S_R1, o[PPPH], np, 1, S_R2, o[PPHH], np, 0, np, 1); g1.end();
+    // w2.ppph = r2.pphh r1.phhh^T
+    g1.begin(S_W2, o[PPPH], np, np, nb, 0); g1.seg(S_R2, o[PPHH], np, 0, S_R1, o[PHHH], nb, 1, nh, 1); g1.end();
+    // w3.hhhh = r1.hhhh r2.hhhh ; w3.pppp = r2.pppp r1.pppp
+    g1.begin(S_W3, o[HHHH], nh, nh, nh, 0); g1.seg(S_R1, o[HHHH], nh, 0, S_R2, o[HHHH], nh, 0, nh, 1); g1.end();
+    g1.begin(S_W3, o[PPPP], np, np, np, 0); g1.seg(S_R2, o[PPPP], np, 0, S_R1, o[PPPP], np, 0, np, 1); g1.end();
+    // accumulate: Vpphh += r1.pppp r2.pphh - r2.pppp r1.pphh - r1.pphh r2.hhhh + r2.pphh r1.hhhh
+    g2.begin(S_OUT, o[PPHH], np, np, nh, 1);
+    g2.seg(S_R1, o[PPPP], np, 0, S_R2, o[PPHH], np, 0, np, 1);
+    g2.seg(S_R2, o[PPPP], np, 0, S_R1, o[PPHH], np, 0, np, -1);
+    g2.seg(S_R1, o[PPHH], np, 0, S_R2, o[HHHH], nh, 0, nh, -1);
+    g2.seg(S_R2, o[PPHH], np, 0, S_R1, o[HHHH], nh, 0, nh, 1);
+    g2.end();
+    // Vphhh += -r1.phhh r2.hhhh + r2.phhh r1.hhhh
+    g2.begin(S_OUT, o[PHHH], nb, nb, nh, 1);
+    g2.seg(S_R1, o[PHHH], nb, 0, S_R2, o[HHHH], nh, 0, nh, -1);
+    g2.seg(S_R2, o[PHHH], nb, 0, S_R1, o[HHHH], nh, 0, nh, 1);
+    g2.end();
+    // Vppph += r1.pppp r2.ppph - r2.pppp r1.ppph
+    g2.begin(S_OUT, o[PPPH], np, np, nb, 1);
+    g2.seg(S_R1, o[PPPP], np, 0, S_R2, o[PPPH], np, 0, np, 1);
+    g2.seg(S_R2, o[PPPP], np, 0, S_R1, o[PPPH], np, 0, np, -1);
+    g2.end();
+  }
+  c->g1_ntiles = (int)g1.tiles.size();
+  double f1 = g1.flops;
+  // Pandya: P(q) = ETACC(q) HCC(q)^T (r x r, K = nb); appended after G1
+  for (int q = 0; q < ch.ncc; q++) {
+    const int r = ch.cc_r[q], nb = ch.cc_nb[q];
+    if (r * nb == 0) continue;
+    g1.begin(S_P, ch.cc_poff[q], r, r, r, 0);
+    g1.seg(S_CC1, ch.cc_off[q], r, 0, S_CC2, ch.cc_off[q], r, 1, nb, 1);
+    g1.end();
+  }
+  c->pandya_flops = g1.flops - f1;
+  if ((rc = upload_gemm(c, g1, c->g_all))) return rc;
+  if ((rc = upload_gemm(c, g2, c->g_acc))) return rc;
+  return QD_OK;
+}
+
+int check_op(const qd_context *c, const qd_op *o, const char *what) {
+  if (!o) return fail(QD_E_ARG, "%s is null", what);
+  if (o->ctx != c) return fail(QD_E_ARG, "%s belongs to another context", what);
+  return QD_OK;
+}
+
+Slots make_slots(qd_context *c, const qd_op *r1, const qd_op *r2, qd_op *out, const qd_cc *ecc,
+                 const qd_cc *hcc) {
+  Slots s{};
+  s.p[S_R1] = r1 ? r1->d : nullptr;
+  s.p[S_R2] = r2 ? r2->d : nullptr;
+  s.p[S_OUT] = out ? out->d : nullptr;
+  s.p[S_W1] = c->w1; s.p[S_W2] = c->w2; s.p[S_W3] = c->w3;
+  s.p[S_CC1] = ecc ? ecc->d : nullptr;
+  s.p[S_CC2] = hcc ? hcc->d : nullptr;
+  s.p[S_P] = c->P;
+  return s;
+}
+
+bool pandya_on(const qd_cc *hcc, const qd_cc *ecc) {
+  return hcc && ecc && !hcc->zero && !ecc->zero;
+}
+
+// products phase: G1 (+ Pandya when both CC sets are non-zero)
+int run_products(qd_context *c, const qd_op *r1, const qd_op *r2, qd_op *out, const qd_cc *hcc, const qd_cc *ecc) {
+  const bool pd = pandya_on(hcc, ecc);
+  Slots s = make_slots(c, r1, r2, out, ecc, hcc);
+  Timer tm(c, TC_PRODUCTS);
+  launch_gemm(c->stream, c->g_all.probs, c->g_all.segs, c->g_all.tiles, pd ? c->g_all.ntiles : c->g1_ntiles, s);
+  HIPC(hipGetLastError());
+  return QD_OK;
+}
+
+int run_acc(qd_context *c, const qd_op *r1, const qd_op *r2, qd_op *out) {
+  Slots s = make_slots(c, r1, r2, out, nullptr, nullptr);
+  Timer tm(c, TC_ACCUM);
+  launch_gemm(c->stream, c->g_acc.probs, c->g_acc.segs, c->g_acc.tiles, c->g_acc.ntiles, s);
+  HIPC(hipGetLastError());
+  return QD_OK;
+}
+
+int run_twobody(qd_context *c, const qd_op *r1, const qd_op *r2, qd_op *out, int flags, const qd_cc *hcc,
+                const qd_cc *ecc) {
+  Ops o{r1->d, r2->d, r1->herm, r2->herm};
+  double hh = hcc ? hcc->herm : 1.0, eh = ecc ? ecc->herm : -1.0;
+  Timer tm(c, TC_TWOBODY);
+  launch_twobody(c->stream, c->dt, o, out->d, c->segs2, c->tiles2, c->ntiles2, flags, c->w1, c->w2, c->w3, c->P, hh, eh);
+  HIPC(hipGetLastError());
+  return QD_OK;
+}
+
+int run_onebody(qd_context *c, const qd_op *r1, const qd_op *r2, qd_op *out, int flags) {
+  Ops o{r1->d, r2->d, r1->herm, r2->herm};
+  Timer tm(c, TC_ONEBODY);
+  launch_onebody(c->stream, c->dt, o, out->d, flags, c->w1, c->w2);
+  HIPC(hipGetLastError());
+  return QD_OK;
+}
+
+int read_scalars(qd_context *c, int n, double *out) {
+  HIPC(hipMemcpyAsync(c->host_scal, c->scal, sizeof(double) * n, hipMemcpyDeviceToHost, c->stream));
+  HIPC(hipStreamSynchronize(c->stream));
+  for (int i = 0; i < n; i++) out[i] = c->host_scal[i];
+  return QD_OK;
+}
+
+int full_commutator(qd_context *c, const qd_op *r1, const qd_op *r2, qd_op *r3, const qd_cc *hcc,
+                    const qd_cc *ecc) {
+  int rc;
+  const bool pd = pandya_on(hcc, ecc);
+  if ((rc = run_products(c, r1, r2, r3, hcc, ecc))) return rc;
+  if ((rc = run_twobody(c, r1, r2, r3, TB_122 | TB_LADDER | (pd ? TB_PANDYA : 0), hcc, ecc))) return rc;
+  if ((rc = run_acc(c, r1, r2, r3))) return rc;
+  if ((rc = run_onebody(c, r1, r2, r3, OB_111 | OB_121 | OB_221))) return rc;
+  {
+    Ops o{r1->d, r2->d, r1->herm, r2->herm};
+    Timer tm(c, TC_OTHER);
+    launch_zerobody(c->stream, c->dt, o, c->pphh_segs, c->npphh_segs, r3->d, 3);
+    HIPC(hipGetLastError());
+  }
+  return QD_OK;
+}
+
+int norm_into(qd_context *c, const qd_op *r, int slot) {
+  Timer tm(c, TC_OTHER);
+  launch_norm2(c->stream, r->d, c->ch.neq, c->partials, c->scal + slot);
+  HIPC(hipGetLastError());
+  return QD_OK;
+}
+
+int ensure_scratch(qd_context *c) {
+  int rc;
+  if (!c->bch_a) {
+    if ((rc = qd_op_create(c, 1, &c->bch_a))) return rc;
+    if ((rc = qd_op_create(c, 1, &c->bch_b))) return rc;
+    if ((rc = qd_cc_create(c, 1.0, &c->cc_t1))) return rc;
+    if ((rc = qd_cc_create(c, -1.0, &c->cc_t2))) return rc;
+  }
+  return QD_OK;
+}
+
+int64_t layout_table(const Channels &ch, int which, int64_t *out, int64_t len) {
+  std::vector<int64_t> v;
+  switch (which) {
+    case 0: for (int q = 0; q < ch.nblock; q++) { v.push_back(ch.nh[q]); v.push_back(ch.np[q]); v.push_back(ch.nb[q]); } break;
+    case 1: v.assign(ch.off.begin(), ch.off.end()); break;
+    case 2: for (uint32_t a : ch.iarr) { v.push_back(ia_type(a)); v.push_back(ia_blk(a)); v.push_back(ia_idx(a)); } break;
+    case 3: v.assign(ch.cc_map.begin(), ch.cc_map.end()); break;
+    case 4: for (int q = 0; q < ch.ncc; q++) { v.push_back(ch.cc_r[q]); v.push_back(ch.cc_nb[q]); } break;
+    case 5: for (uint32_t pr : ch.pairs) { v.push_back(lo16(pr)); v.push_back(hi16(pr)); } break;
+    case 6: v.push_back(ch.neq); v.push_back(ch.cc_size); v.push_back(ch.nblock); v.push_back(ch.ncc); break;
+    default: return fail(QD_E_ARG, "unknown table %d", which);
+  }
+  if (out) {
+    if (len < (int64_t)v.size()) return fail(QD_E_ARG, "table needs %lld entries", (long long)v.size());
+    memcpy(out, v.data(), v.size() * sizeof(int64_t));
+  }
+  return (int64_t)v.size();
+}
+
+}  // namespace
+
+// ===========================================================================
+extern "C" {
+
+const char *qd_last_error(void) { return g_err.c_str(); }
+
+int qd_context_create(int nbody, int nshell, qd_context **out) {
+  if (!out) return fail(QD_E_ARG, "out is null");
+  *out = nullptr;
+  qd_context *c = new qd_context();
+  std::string why = build_channels(nbody, nshell, c->ch);
+  if (!why.empty()) { delete c; return fail(QD_E_LAYOUT, "%s", why.c_str()); }
+  int rc = QD_OK;
+  auto bail = [&](int r) { qd_context_destroy(c); return r; };
+  if (hipStreamCreate(&c->stream) != hipSuccess) { delete c; return fail(QD_E_HIP, "hipStreamCreate failed"); }
+  const Channels &ch = c->ch;
+  DevTab &t = c->dt;
+  t.n = ch.n; t.M = ch.M; t.mp = ch.mp; t.nblock = ch.nblock; t.ncc = ch.ncc; t.nch = ch.nch;
+  t.ml0 = ch.ml0; t.mlspan = ch.mlspan; t.neq = ch.neq;
+  t.off_fhh = ch.off_fhh; t.off_fpp = ch.off_fpp; t.off_fph = ch.off_fph;
+  std::vector<int2> ccmap((size_t)ch.M * ch.M);
+  for (size_t i = 0; i < ccmap.size(); i++) ccmap[i] = make_int2(ch.cc_map[2 * i], ch.cc_map[2 * i + 1]);
+#define UP(vec, field) if ((rc = dev_upload(c, vec, (decltype(vec)::value_type **)&t.field))) return bail(rc)
+  UP(ch.nh, nh); UP(ch.np, np); UP(ch.nb, nb); UP(ch.off, off); UP(ch.pairs, pairs);
+  UP(ch.hh_start, hh_start); UP(ch.ph_start, ph_start); UP(ch.pp_start, pp_start); UP(ch.iarr, iarr);
+  UP(ch.spch, spch); UP(ch.ch_start, ch_start); UP(ch.ch_states, ch_states); UP(ch.ch_lookup, ch_lookup);
+  UP(ch.ml, ml); UP(ch.ms, ms); UP(ch.cc_r, cc_r); UP(ch.cc_nb, cc_nb); UP(ch.cc_off, cc_off);
+  UP(ch.cc_poff, cc_poff); UP(ch.cc_ab, cc_ab); UP(ch.cc_hp, cc_hp); UP(ch.cc_ab_start, cc_ab_start);
+  UP(ch.cc_hp_start, cc_hp_start); UP(ccmap, cc_map);
+#undef UP
+  void *p;
+  if ((rc = dev_alloc(c, sizeof(double) * ch.neq, &p))) return bail(rc); c->w1 = (double *)p;
+  if ((rc = dev_alloc(c, sizeof(double) * ch.neq, &p))) return bail(rc); c->w2 = (double *)p;
+  if ((rc = dev_alloc(c, sizeof(double) * ch.neq, &p))) return bail(rc); c->w3 = (double *)p;
+  if ((rc = dev_alloc(c, sizeof(double) * ch.p_size, &p))) return bail(rc); c->P = (double *)p;
+  if ((rc = dev_alloc(c, sizeof(double) * 1024 * 64, &p))) return bail(rc); c->partials = (double *)p;
+  if ((rc = dev_alloc(c, sizeof(double) * 8, &p))) return bail(rc); c->scal = (double *)p;
+  if (hipHostMalloc((void **)&c->host_scal, sizeof(double) * 8, hipHostMallocDefault) != hipSuccess)
+    return bail(fail(QD_E_HIP, "hipHostMalloc failed"));
+  if ((rc = build_worklists(c))) return bail(rc);
+  if (c->npphh_tiles > 1024 * 64) return bail(fail(QD_E_LAYOUT, "too many pphh tiles for the reduction buffer"));
+  *out = c;
+  return QD_OK;
+}
+
+int qd_context_destroy(qd_context *c) {
+  if (!c) return QD_OK;
+  if (c->stream) hipStreamSynchronize(c->stream);
+  if (c->bch_a) { qd_op_destroy(c->bch_a); qd_op_destroy(c->bch_b); qd_cc_destroy(c->cc_t1); qd_cc_destroy(c->cc_t2); }
+  for (auto &e : c->events) { hipEventDestroy(e.a); hipEventDestroy(e.b); }
+  for (void *p : c->allocs) hipFree(p);
+  if (c->host_scal) hipHostFree(c->host_scal);
+  if (c->stream) hipStreamDestroy(c->stream);
+  delete c;
+  return QD_OK;
+}
+
+int qd_context_info(const qd_context *c, qd_info *o) {
+  if (!c || !o) return fail(QD_E_ARG, "null argument");
+  const Channels &ch = c->ch;
+  o->nbody = ch.n; o->nshell = ch.R; o->msp = ch.M; o->nblock = ch.nblock; o->ncc = ch.ncc;
+  o->neq = ch.neq; o->cc_size = ch.cc_size;
+  o->n2b = ch.neq - ch.off_2b;
+  return QD_OK;
+}
+
+int64_t qd_context_table(const qd_context *c, int which, int64_t *out, int64_t len) {
+  if (!c) return fail(QD_E_ARG, "null context");
+  return layout_table(c->ch, which, out, len);
+}
+
+int64_t qd_layout_table(int nbody, int nshell, int which, int64_t *out, int64_t len) {
+  Channels ch;
+  std::string why = build_channels(nbody, nshell, ch);
+  if (!why.empty()) return fail(QD_E_LAYOUT, "%s", why.c_str());
+  return layout_table(ch, which, out, len);
+}
+
+int qd_context_sync(qd_context *c) {
+  if (!c) return fail(QD_E_ARG, "null context");
+  HIPC(hipStreamSynchronize(c->stream));
+  return QD_OK;
+}
+
+// ---- operators -----------------------------------------------------------
+int qd_op_create(qd_context *c, int herm, qd_op **out) {
+  if (!c || !out) return fail(QD_E_ARG, "null argument");
+  if (herm != 1 && herm != -1) return fail(QD_E_ARG, "herm must be +1 or -1");
+  void *p = nullptr;
+  HIPC(hipMalloc(&p, sizeof(double) * c->ch.neq));
+  HIPC(hipMemsetAsync(p, 0, sizeof(double) * c->ch.neq, c->stream));
+  *out = new qd_op{c, (double *)p, herm};
+  return QD_OK;
+}
+int qd_op_destroy(qd_op *o) {
+  if (!o) return QD_OK;
+  hipStreamSynchronize(o->ctx->stream);
+  hipFree(o->d);
+  delete o;
+  return QD_OK;
+}
+int qd_op_set_herm(qd_op *o, int herm) {
+  if (!o || (herm != 1 && herm != -1)) return fail(QD_E_ARG, "bad operator or herm");
+  o->herm = herm;
+  return QD_OK;
+}
+int qd_op_get_herm(const qd_op *o, int *herm) {
+  if (!o || !herm) return fail(QD_E_ARG, "null argument");
+  *herm = o->herm;
+  return QD_OK;
+}
+int qd_op_upload(qd_op *o, const double *vec, int64_t neq) {
+  if (!o || !vec) return fail(QD_E_ARG, "null argument");
+  if (neq != o->ctx->ch.neq) return fail(QD_E_ARG, "neq %lld != %lld", (long long)neq, (long long)o->ctx->ch.neq);
+  HIPC(hipMemcpyAsync(o->d, vec, sizeof(double) * neq, hipMemcpyHostToDevice, o->ctx->stream));
+  HIPC(hipStreamSynchronize(o->ctx->stream));
+  return QD_OK;
+}
+int qd_op_download(const qd_op *o, double *vec, int64_t neq) {
+  if (!o || !vec) return fail(QD_E_ARG, "null argument");
+  if (neq != o->ctx->ch.neq) return fail(QD_E_ARG, "neq %lld != %lld", (long long)neq, (long long)o->ctx->ch.neq);
+  HIPC(hipMemcpyAsync(vec, o->d, sizeof(double) * neq, hipMemcpyDeviceToHost, o->ctx->stream));
+  HIPC(hipStreamSynchronize(o->ctx->stream));
+  return QD_OK;
+}
+double *qd_op_device_ptr(qd_op *o) { return o ? o->d : nullptr; }
+
+// ---- CC sets ---------------------------------------------------------------
+int qd_cc_create(qd_context *c, double herm, qd_cc **out) {
+  if (!c || !out) return fail(QD_E_ARG, "null argument");
+  void *p = nullptr;
+  size_t bytes = sizeof(double) * std::max<int64_t>(c->ch.cc_size, 1);
+  HIPC(hipMalloc(&p, bytes));
+  HIPC(hipMemsetAsync(p, 0, bytes, c->stream));
+  *out = new qd_cc{c, (double *)p, herm, true};
+  return QD_OK;
+}
+int qd_cc_destroy(qd_cc *cc) {
+  if (!cc) return QD_OK;
+  hipStreamSynchronize(cc->ctx->stream);
+  hipFree(cc->d);
+  delete cc;
+  return QD_OK;
+}
+int qd_cc_set_herm(qd_cc *cc, double herm) {
+  if (!cc) return fail(QD_E_ARG, "null argument");
+  cc->herm = herm;
+  return QD_OK;
+}
+int qd_cc_upload(qd_cc *cc, const double *x, int64_t n) {
+  if (!cc || !x) return fail(QD_E_ARG, "null argument");
+  if (n != cc->ctx->ch.cc_size) return fail(QD_E_ARG, "cc_size mismatch");
+  HIPC(hipMemcpyAsync(cc->d, x, sizeof(double) * n, hipMemcpyHostToDevice, cc->ctx->stream));
+  HIPC(hipStreamSynchronize(cc->ctx->stream));
+  cc->zero = false;
+  return QD_OK;
+}
+int qd_cc_download(const qd_cc *cc, double *x, int64_t n) {
+  if (!cc || !x) return fail(QD_E_ARG, "null argument");
+  if (n != cc->ctx->ch.cc_size) return fail(QD_E_ARG, "cc_size mismatch");
+  HIPC(hipMemcpyAsync(x, cc->d, sizeof(double) * n, hipMemcpyDeviceToHost, cc->ctx->stream));
+  HIPC(hipStreamSynchronize(cc->ctx->stream));
+  return QD_OK;
+}
+int qd_cc_zero(qd_context *c, qd_cc *cc) {
+  if (!c || !cc) return fail(QD_E_ARG, "null argument");
+  HIPC(hipMemsetAsync(cc->d, 0, sizeof(double) * std::max<int64_t>(c->ch.cc_size, 1), c->stream));
+  cc->zero = true;
+  return QD_OK;
+}
+
+// ---- commutator pieces -------------------------------------------------------
+#define CHK3(a, b, d)                                 \
+  do {                                                \
+    int rc_;                                          \
+    if (!c) return fail(QD_E_ARG, "null context");    \
+    if ((rc_ = check_op(c, a, "r1"))) return rc_;     \
+    if ((rc_ = check_op(c, b, "r2"))) return rc_;     \
+    if (d && (rc_ = check_op(c, d, "r3"))) return rc_; \
+  } while (0)
+
+int qd_commutator_110(qd_context *c, const qd_op *r1, const qd_op *r2, double *out) {
+  CHK3(r1, r2, (const qd_op *)nullptr);
+  Ops o{r1->d, r2->d, r1->herm, r2->herm};
+  launch_zerobody(c->stream, c->dt, o, c->pphh_segs, c->npphh_segs, c->scal, 1);
+  HIPC(hipGetLastError());
+  return read_scalars(c, 1, out);
+}
+int qd_commutator_220(qd_context *c, const qd_op *r1, const qd_op *r2, double *out) {
+  CHK3(r1, r2, (const qd_op *)nullptr);
+  Ops o{r1->d, r2->d, r1->herm, r2->herm};
+  launch_zerobody(c->stream, c->dt, o, c->pphh_segs, c->npphh_segs, c->scal, 2);
+  HIPC(hipGetLastError());
+  return read_scalars(c, 1, out);
+}
+int qd_commutator_111(qd_context *c, const qd_op *r1, const qd_op *r2, qd_op *r3) {
+  CHK3(r1, r2, r3);
+  return run_onebody(c, r1, r2, r3, OB_111);
+}
+int qd_commutator_121(qd_context *c, const qd_op *r1, const qd_op *r2, qd_op *r3) {
+  CHK3(r1, r2, r3);
+  return run_onebody(c, r1, r2, r3, OB_121 | OB_ACCUM);
+}
+int qd_commutator_221(qd_context *c, const qd_op *r1, const qd_op *r2, qd_op *r3) {
+  CHK3(r1, r2, r3);
+  int rc;
+  if ((rc = run_products(c, r1, r2, r3, nullptr, nullptr))) return rc;
+  return run_onebody(c, r1, r2, r3, OB_221 | OB_ACCUM);
+}
+int qd_commutator_122(qd_context *c, const qd_op *r1, const qd_op *r2, qd_op *r3) {
+  CHK3(r1, r2, r3);
+  return run_twobody(c, r1, r2, r3, TB_122, nullptr, nullptr);
+}
+int qd_commutator_222(qd_context *c, const qd_op *r1, const qd_op *r2, qd_op *r3, const qd_cc *hcc,
+                      const qd_cc *etacc) {
+  CHK3(r1, r2, r3);
+  int rc;
+  const bool pd = pandya_on(hcc, etacc);
+  if ((rc = run_products(c, r1, r2, r3, hcc, etacc))) return rc;
+  if ((rc = run_twobody(c, r1, r2, r3, TB_LADDER | TB_ACCUM | (pd ? TB_PANDYA : 0), hcc, etacc))) return rc;
+  return run_acc(c, r1, r2, r3);
+}
+int qd_calc_cc(qd_context *c, const qd_op *h, qd_cc *cc) {
+  if (!c || !h || !cc) return fail(QD_E_ARG, "null argument");
+  {
+    Timer tm(c, TC_CALCCC);
+    launch_calc_cc(c->stream, c->dt, h->d, h->herm, cc->d, c->cc_segs, c->cc_tiles, c->ncc_tiles);
+    HIPC(hipGetLastError());
+  }
+  cc->zero = false;
+  return QD_OK;
+}
+int qd_build_white(qd_context *c, qd_op *gen, const qd_op *rec) {
+  if (!c || !gen || !rec) return fail(QD_E_ARG, "null argument");
+  Timer tm(c, TC_OTHER);
+  launch_white(c->stream, c->dt, rec->d, rec->herm, gen->d, c->pphh_segs, c->pphh_tiles, c->npphh_tiles);
+  HIPC(hipGetLastError());
+  return QD_OK;
+}
+
+// ---- operator algebra ---------------------------------------------------------
+int qd_add_arrays(qd_context *c, const qd_op *r1, double s1, const qd_op *r2, double s2, qd_op *r3) {
+  CHK3(r1, r2, r3);
+  Timer tm(c, TC_OTHER);
+  launch_axpby(c->stream, c->ch.neq, s1, r1->d, s2, r2->d, r3->d);
+  HIPC(hipGetLastError());
+  return QD_OK;
+}
+int qd_copy_arrays(qd_context *c, const qd_op *r1, qd_op *r2) {
+  CHK3(r1, r2, (const qd_op *)nullptr);
+  if (r1 == r2) return QD_OK;
+  HIPC(hipMemcpyAsync(r2->d, r1->d, sizeof(double) * c->ch.neq, hipMemcpyDeviceToDevice, c->stream));
+  return QD_OK;
+}
+int qd_set_to_zero(qd_context *c, qd_op *r) {
+  if (!c || !r) return fail(QD_E_ARG, "null argument");
+  HIPC(hipMemsetAsync(r->d, 0, sizeof(double) * c->ch.neq, c->stream));
+  return QD_OK;
+}
+int qd_mat_2_norm(qd_context *c, const qd_op *r, double *out) {
+  if (!c || !r || !out) return fail(QD_E_ARG, "null argument");
+  int rc;
+  if ((rc = norm_into(c, r, 0))) return rc;
+  double s;
+  if ((rc = read_scalars(c, 1, &s))) return rc;
+  *out = std::sqrt(s);
+  return QD_OK;
+}
+int qd_mbpt2(qd_context *c, const qd_op *h, double *out) {
+  if (!c || !h || !out) return fail(QD_E_ARG, "null argument");
+  {
+    Timer tm(c, TC_OTHER);
+    launch_mbpt2(c->stream, c->dt, h->d, c->pphh_segs, c->pphh_tiles, c->npphh_tiles, c->partials, c->scal);
+    HIPC(hipGetLastError());
+  }
+  return read_scalars(c, 1, out);
+}
+
+// ---- whole commutator, BCH, Magnus ----------------------------------------------
+int qd_commutator(qd_context *c, const qd_op *r1, const qd_op *r2, qd_op *r3, const qd_cc *hcc,
+                  const qd_cc *etacc, double *e0) {
+  CHK3(r1, r2, r3);
+  if (r3 == r1 || r3 == r2) return fail(QD_E_ARG, "r3 must not alias r1 or r2");
+  int rc;
+  if ((rc = full_commutator(c, r1, r2, r3, hcc, etacc))) return rc;
+  if (e0) {
+    HIPC(hipMemcpyAsync(c->host_scal, r3->d, sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    HIPC(hipStreamSynchronize(c->stream));
+    *e0 = c->host_scal[0];
+  }
+  return QD_OK;
+}
+
+int qd_bch_expand(qd_context *c, qd_op *hs, const qd_op *g, const qd_op *h, const qd_cc *hcc, const qd_cc *occ,
+                  int *nterms) {
+  CHK3(g, h, hs);
+  int rc;
+  if ((rc = ensure_scratch(c))) return rc;
+  // Pandya inputs: the caller's CC sets are used when non-zero; the
+  // reference's Magnus driver passes zero sets (no Pandya term).
+  const bool pd = pandya_on(hcc, occ);
+  qd_cc *ccg = c->cc_t2, *cca = c->cc_t1;
+  if (pd) {  // fill CC of G (r1 slot) and of each AD (r2 slot) as calc_cc would
+    if ((rc = qd_calc_cc(c, g, ccg))) return rc;
+    ccg->herm = occ->herm;
+    cca->herm = hcc->herm;
+  }
+  // HS = H ; AD_0 = H ; then AD_k = [G, AD_{k-1}], HS += cof(k) AD_k
+  if ((rc = qd_copy_arrays(c, h, hs))) return rc;
+  const qd_op *prev = h;
+  qd_op *buf[2] = {c->bch_a, c->bch_b};
+  int k, taken = 0;
+  for (k = 2; k <= 12; k++) {
+    qd_op *next = buf[k & 1];
+    next->herm = 1;
+    qd_op adh = *prev;  // AD carries herm = +1 (src/magnus_IMSRG.f90:273-275)
+    adh.herm = 1;
+    if (pd) {
+      if ((rc = qd_calc_cc(c, &adh, cca))) return rc;
+    }
+    if ((rc = full_commutator(c, g, &adh, next, pd ? cca : nullptr, pd ? ccg : nullptr))) return rc;
+    taken++;
+    // ||INT1|| with INT1 = HS before the update, ||INT2||
+    if ((rc = norm_into(c, hs, 0))) return rc;
+    if ((rc = norm_into(c, next, 1))) return rc;
+    {
+      Timer tm(c, TC_OTHER);
+      launch_axpby(c->stream, c->ch.neq, 1.0, hs->d, BCH_COF[k - 2], next->d, hs->d);
+      HIPC(hipGetLastError());
+    }
+    double nn[2];
+    if ((rc = read_scalars(c, 2, nn))) return rc;
+    if (std::sqrt(nn[1]) * BCH_COF[k - 2] / std::sqrt(nn[0]) < R4_1EM5) break;
+    prev = next;
+  }
+  if (nterms) *nterms = taken;
+  return QD_OK;
+}
+
+int qd_magnus_expand(qd_context *c, qd_op *dg, const qd_op *g, const qd_op *eta) {
+  (void)g;
+  if (!c || !dg || !eta) return fail(QD_E_ARG, "null argument");
+  return qd_copy_arrays(c, eta, dg);
+}
+
+int qd_euler_step(qd_context *c, qd_op *g, const qd_op *dg, double *s, double stp) {
+  if (!c || !g || !dg || !s) return fail(QD_E_ARG, "null argument");
+  int rc = qd_add_arrays(c, g, 1.0, dg, stp, g);
+  if (rc) return rc;
+  *s += stp;
+  return QD_OK;
+}
+
+int qd_dgam(qd_context *c, const qd_op *h, qd_op *der, qd_cc *hcc, qd_cc *etacc) {
+  if (!c || !h || !der || !hcc || !etacc) return fail(QD_E_ARG, "null argument");
+  int rc;
+  if ((rc = ensure_scratch(c))) return rc;
+  qd_op *eta = c->bch_a;
+  if ((rc = qd_calc_cc(c, h, hcc))) return rc;
+  hcc->herm = 1.0;
+  if ((rc = qd_set_to_zero(c, eta))) return rc;
+  eta->herm = -1;
+  if ((rc = qd_build_white(c, eta, h))) return rc;
+  if ((rc = qd_calc_cc(c, eta, etacc))) return rc;
+  etacc->herm = -1.0;
+  return full_commutator(c, eta, h, der, hcc, etacc);
+}
+
+void qd_flow_params_default(qd_flow_params *p) {
+  if (!p) return;
+  p->stp = R4_0P7;
+  p->conv = R4_1EM9;
+  p->max_iter = 100000;
+  p->pandya = 0;
+  p->verbose = 0;
+  p->warm_start = 0;
+}
+
+int qd_magnus_flow(qd_context *c, const qd_op *h0, qd_op *hs, qd_op *omega, const qd_flow_params *p,
+                   qd_flow_result *res, double *trace, int trace_len) {
+  if (!c || !h0 || !hs || !omega || !p || !res) return fail(QD_E_ARG, "null argument");
+  int rc;
+  qd_op *eta, *eta0, *g0, *hs0, *dg;
+  if ((rc = qd_op_create(c, -1, &eta))) return rc;
+  qd_op_create(c, -1, &eta0);
+  qd_op_create(c, -1, &g0);
+  qd_op_create(c, 1, &hs0);
+  qd_op_create(c, -1, &dg);
+  qd_cc *hcc = nullptr, *occ = nullptr;
+  qd_cc_create(c, 1.0, &hcc);
+  qd_cc_create(c, -1.0, &occ);
+  if (p->pandya) { hcc->zero = false; occ->zero = false; }  // request calc_cc-filled inputs
+  omega->herm = -1;
+  memset(res, 0, sizeof *res);
+  double s = 0.0, stp = p->stp, nrm1 = 0, nrm2 = 0, crit = 0, e0 = 0;
+  int iteration = 0, rejected = 0, nt = 0, rows = 0;
+  int64_t terms = 0;
+  auto done = [&](int r) {
+    qd_op_destroy(eta); qd_op_destroy(eta0); qd_op_destroy(g0); qd_op_destroy(hs0); qd_op_destroy(dg);
+    qd_cc_destroy(hcc); qd_cc_destroy(occ);
+    return r;
+  };
+  // HS = H(0), or BCH(Omega, H(0)) for a warm start
+  if (p->warm_start) {
+    if ((rc = qd_bch_expand(c, hs, omega, h0, hcc, occ, &nt))) return done(rc);
+    terms += nt;
+  } else {
+    if ((rc = qd_set_to_zero(c, omega))) return done(rc);
+    if ((rc = qd_copy_arrays(c, h0, hs))) return done(rc);
+  }
+  if ((rc = qd_build_white(c, eta, hs))) return done(rc);
+  if ((rc = qd_mat_2_norm(c, eta, &nrm1))) return done(rc);
+  double m2;
+  if ((rc = qd_mbpt2(c, p->warm_start ? hs : h0, &m2))) return done(rc);
+  crit = std::fabs(m2);
+  int steps = 0;
+  while (crit > p->conv && steps < p->max_iter) {
+    steps++;
+    if ((rc = qd_copy_arrays(c, omega, g0))) return done(rc);
+    if ((rc = qd_magnus_expand(c, dg, omega, eta))) return done(rc);
+    if ((rc = qd_euler_step(c, omega, dg, &s, stp))) return done(rc);
+    if ((rc = qd_copy_arrays(c, hs, hs0))) return done(rc);
+    if ((rc = qd_bch_expand(c, hs, omega, h0, hcc, occ, &nt))) return done(rc);
+    terms += nt;
+    if ((rc = qd_copy_arrays(c, eta, eta0))) return done(rc);
+    if ((rc = qd_build_white(c, eta, hs))) return done(rc);
+    if ((rc = qd_mat_2_norm(c, eta, &nrm2))) return done(rc);
+    HIPC(hipMemcpyAsync(c->host_scal, hs->d, sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    HIPC(hipStreamSynchronize(c->stream));
+    e0 = c->host_scal[0];
+    if (!std::isfinite(e0) || !std::isfinite(nrm2)) return done(fail(QD_E_NUMERIC, "non-finite E0 or ||eta|| at s=%g", s));
+    if (p->verbose) printf("%5d%18.9f%18.9f%18.9f%18.9f%18.9f\n", iteration, s, e0, nrm1, nrm2, crit);
+    if (trace && rows < trace_len) {
+      double *r = trace + 6 * (size_t)rows++;
+      r[0] = iteration; r[1] = s; r[2] = e0; r[3] = nrm1; r[4] = nrm2; r[5] = crit;
+    }
+    if (nrm1 * 1.5 < nrm2) {
+      if (p->verbose) printf(" faillll\n");
+      s -= stp;
+      qd_copy_arrays(c, g0, omega);
+      qd_copy_arrays(c, hs0, hs);
+      qd_copy_arrays(c, eta0, eta);
+      stp = stp / 2.0 + R4_1EM3;
+      rejected++;
+      continue;
+    }
+    if ((rc = qd_mbpt2(c, hs, &m2))) return done(rc);
+    crit = std::fabs(m2);
+    iteration++;
+  }
+  HIPC(hipMemcpyAsync(c->host_scal, hs->d, sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  HIPC(hipStreamSynchronize(c->stream));
+  res->s = s;
+  res->e0 = c->host_scal[0];
+  res->crit = crit;
+  res->iterations = iteration;
+  res->rejected = rejected;
+  res->bch_terms = terms;
+  res->converged = crit <= p->conv;
+  res->trace_rows = rows;
+  return done(QD_OK);
+}
+
+// ---- timing ---------------------------------------------------------------------
+int qd_timing_enable(qd_context *c, int on) {
+  if (!c) return fail(QD_E_ARG, "null context");
+  HIPC(hipStreamSynchronize(c->stream));
+  for (auto &e : c->events) { hipEventDestroy(e.a); hipEventDestroy(e.b); }
+  c->events.clear();
+  for (int i = 0; i < NCLASS; i++) { c->t_ms[i] = 0; c->t_n[i] = 0; }
+  c->timing = on != 0;
+  return QD_OK;
+}
+int qd_timing_read(qd_context *c, double *ms, int64_t *nl, int nclass) {
+  if (!c) return fail(QD_E_ARG, "null context");
+  HIPC(hipStreamSynchronize(c->stream));
+  for (auto &e : c->events) {
+    float t = 0;
+    HIPC(hipEventElapsedTime(&t, e.a, e.b));
+    c->t_ms[e.cls] += t;
+    c->t_n[e.cls] += 1;
+    hipEventDestroy(e.a);
+    hipEventDestroy(e.b);
+  }
+  c->events.clear();
+  for (int i = 0; i < nclass && i < NCLASS; i++) {
+    if (ms) ms[i] = c->t_ms[i];
+    if (nl) nl[i] = c->t_n[i];
+  }
+  return QD_OK;
+}
+
+int qd_gemm_flops(const qd_context *c, double *ladder, double *pandya, double *acc) {
+  if (!c) return fail(QD_E_ARG, "null context");
+  if (ladder) *ladder = c->g_all.flops - c->pandya_flops;
+  if (pandya) *pandya = c->pandya_flops;
+  if (acc) *acc = c->g_acc.flops;
+  return QD_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,145 @@
+// Host channel builder: the (M_L, M_S) two-body blocks, their flat offsets,
+// the pair -> (type, block, index) map and the cross-coupled blocks.
+//
+// Semantics follow src/ME_general.f90: arrange_states (:688-744), reindex
+// (:956-1116), get_num_elements (:1755-1778), vectorize (:1871-1989) and
+// build_cross_coupled (:363-448).  Built with one pass over all pairs into
+// (M_S, M_L) buckets instead of the reference's scan per block.
+#include <algorithm>
+#include <map>
+
+#include "qd_internal.h"
+
+namespace qd {
+
+std::string build_channels(int nbody, int nshell, Channels &c) {
+  if (nshell < 1 || nshell > 40) return "nshell out of range";
+  c.n = nbody;
+  c.R = nshell;
+  // states order: shell, then m_l ascending, then m_s = -1, +1
+  c.ml.clear();
+  c.ms.clear();
+  std::vector<int> shell;
+  for (int R = 0; R < nshell; R++)
+    for (int ML = -R; ML <= R; ML += 2)
+      for (int MS = -1; MS <= 1; MS += 2) {
+        c.ml.push_back(ML);
+        c.ms.push_back(MS);
+        shell.push_back(R);
+      }
+  const int M = (int)c.ml.size();
+  c.M = M;
+  c.mp = M - nbody;
+  if (nbody < 0 || nbody > M) return "nbody out of range";
+  // holes = first N states: requires N to fill whole shells (closed shell),
+  // which is what makes the reference's Fermi-level split equal to index < N
+  if (nbody > 0 && nbody < M && shell[nbody - 1] == shell[nbody])
+    return "nbody must fill complete oscillator shells (2, 6, 12, 20, 30, 42, ...)";
+  const int n = nbody;
+
+  // ---- two-body blocks
+  std::map<std::pair<int, int>, std::vector<std::pair<int, int>>> bucket;
+  for (int i = 0; i < M - 1; i++)
+    for (int j = i + 1; j < M; j++) bucket[{c.ms[i] + c.ms[j], c.ml[i] + c.ml[j]}].push_back({i, j});
+  c.iarr.assign((size_t)M * (M - 1) / 2, 0u);
+  c.nh.clear(); c.np.clear(); c.nb.clear();
+  c.pairs.clear(); c.hh_start.clear(); c.ph_start.clear(); c.pp_start.clear();
+  int q = 0;
+  for (int MS = -2; MS <= 2; MS += 2)
+    for (int ML = 2 - 2 * nshell; ML <= 2 * nshell - 2; ML++, q++) {
+      std::vector<uint32_t> hh, ph, pp;
+      auto it = bucket.find({MS, ML});
+      if (it != bucket.end())
+        for (auto &pr : it->second) {
+          int i = pr.first, j = pr.second;
+          size_t II = (size_t)i * M - (size_t)i * (i + 1) / 2 + (j - i - 1);
+          if (j < n) { c.iarr[II] = pack_ia(T_HH, q, (int)hh.size()); hh.push_back(pack2(i, j)); }
+          else if (i < n) { c.iarr[II] = pack_ia(T_PH, q, (int)ph.size()); ph.push_back(pack2(i, j)); }
+          else { c.iarr[II] = pack_ia(T_PP, q, (int)pp.size()); pp.push_back(pack2(i, j)); }
+        }
+      c.nh.push_back((int)hh.size());
+      c.nb.push_back((int)ph.size());
+      c.np.push_back((int)pp.size());
+      c.hh_start.push_back((int)c.pairs.size());
+      c.pairs.insert(c.pairs.end(), hh.begin(), hh.end());
+      c.ph_start.push_back((int)c.pairs.size());
+      c.pairs.insert(c.pairs.end(), ph.begin(), ph.end());
+      c.pp_start.push_back((int)c.pairs.size());
+      c.pairs.insert(c.pairs.end(), pp.begin(), pp.end());
+    }
+  c.nblock = q;
+  if (c.nblock >= 1024) return "too many blocks for the packed pair map";
+
+  // ---- flat offsets in vectorize order
+  c.off_fhh = 1;
+  c.off_fpp = 1 + (int64_t)n * n;
+  c.off_fph = c.off_fpp + (int64_t)c.mp * c.mp;
+  c.off_2b = c.off_fph + (int64_t)c.mp * n;
+  c.off.assign((size_t)c.nblock * 6, 0);
+  int64_t k = c.off_2b;
+  for (int b = 0; b < c.nblock; b++) {
+    int64_t h = c.nh[b], p = c.np[b], x = c.nb[b];
+    int64_t sz[6] = {h * h, p * p, p * h, x * x, p * x, x * h};
+    for (int s = 0; s < 6; s++) { c.off[(size_t)b * 6 + s] = k; k += sz[s]; }
+  }
+  c.neq = k;
+
+  // ---- single-particle (m_l, m_s) channels
+  int mlmax = *std::max_element(c.ml.begin(), c.ml.end());
+  c.ml0 = mlmax;
+  c.mlspan = 2 * mlmax + 1;
+  c.ch_lookup.assign((size_t)c.mlspan * 2, -1);
+  c.spch.assign(M, -1);
+  c.nch = 0;
+  for (int i = 0; i < M; i++) {
+    int key = (c.ml[i] + c.ml0) * 2 + (c.ms[i] > 0);
+    if (c.ch_lookup[key] < 0) c.ch_lookup[key] = c.nch++;
+    c.spch[i] = c.ch_lookup[key];
+  }
+  c.ch_start.assign(c.nch + 1, 0);
+  for (int i = 0; i < M; i++) c.ch_start[c.spch[i] + 1]++;
+  for (int h = 0; h < c.nch; h++) c.ch_start[h + 1] += c.ch_start[h];
+  c.ch_states.assign(M, 0);
+  {
+    std::vector<int> fill(c.ch_start.begin(), c.ch_start.end() - 1);
+    for (int i = 0; i < M; i++) c.ch_states[fill[c.spch[i]]++] = i;
+  }
+
+  // ---- cross-coupled blocks: ordered pairs (i,j), M_L/M_S differences
+  int MLmax = 2 * mlmax;
+  std::map<std::pair<int, int>, std::vector<std::pair<int, int>>> ccb;
+  for (int i = 0; i < M; i++)
+    for (int j = 0; j < M; j++) ccb[{c.ml[i] - c.ml[j], c.ms[i] - c.ms[j]}].push_back({i, j});
+  c.cc_map.assign((size_t)M * M * 2, -1);
+  c.cc_r.clear(); c.cc_nb.clear(); c.cc_off.clear(); c.cc_poff.clear();
+  c.cc_ab.clear(); c.cc_hp.clear(); c.cc_ab_start.clear(); c.cc_hp_start.clear();
+  int64_t coff = 0, poff = 0;
+  q = 0;
+  for (int ML = -MLmax; ML <= MLmax; ML++)
+    for (int MS = -2; MS <= 2; MS++, q++) {
+      c.cc_ab_start.push_back((int)c.cc_ab.size());
+      c.cc_hp_start.push_back((int)c.cc_hp.size());
+      int r = 0, nbb = 0;
+      auto it = ccb.find({ML, MS});
+      if (it != ccb.end())
+        for (auto &pr : it->second) {
+          int i = pr.first, j = pr.second;
+          c.cc_map[((size_t)M * i + j) * 2] = q;
+          c.cc_map[((size_t)M * i + j) * 2 + 1] = r++;
+          c.cc_ab.push_back(pack2(i, j));
+          if (i < n && j >= n) { c.cc_hp.push_back(pack2(i, j)); nbb++; }
+        }
+      c.cc_r.push_back(r);
+      c.cc_nb.push_back(nbb);
+      c.cc_off.push_back(coff);
+      c.cc_poff.push_back(poff);
+      coff += (int64_t)r * nbb;
+      poff += (nbb > 0) ? (int64_t)r * r : 0;
+    }
+  c.ncc = q;
+  c.cc_size = coff;
+  c.p_size = poff;
+  return "";
+}
+
+}  // namespace qd

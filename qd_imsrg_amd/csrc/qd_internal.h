@@ -1,0 +1,74 @@
+// Internal declarations shared by the host channel builder, the kernels and
+// the C ABI.  Not installed; include/qdimsrg.h is the public surface.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "../../include/qdimsrg.h"
+
+namespace qd {
+
+// sub-block order inside one two-body block = the reference's vectorize order
+enum Sub { HHHH = 0, PPPP = 1, PPHH = 2, PHPH = 3, PPPH = 4, PHHH = 5 };
+enum PairType { T_HH = 1, T_PH = 2, T_PP = 3 };
+
+// Packed state pair: i in the low 16 bits, j in the high 16 bits (M <= 65535).
+__host__ __device__ inline uint32_t pack2(int i, int j) { return (uint32_t)i | ((uint32_t)j << 16); }
+__host__ __device__ inline int lo16(uint32_t p) { return (int)(p & 0xffffu); }
+__host__ __device__ inline int hi16(uint32_t p) { return (int)(p >> 16); }
+// Packed i_array entry: type (2 bits) | block (10 bits) | index within its class (20 bits).
+__host__ __device__ inline uint32_t pack_ia(int type, int blk, int idx) {
+  return ((uint32_t)type << 30) | ((uint32_t)blk << 20) | (uint32_t)idx;
+}
+__host__ __device__ inline int ia_type(uint32_t a) { return (int)(a >> 30); }
+__host__ __device__ inline int ia_blk(uint32_t a) { return (int)((a >> 20) & 0x3ffu); }
+__host__ __device__ inline int ia_idx(uint32_t a) { return (int)(a & 0xfffffu); }
+
+// Host-side channel structure (built once per context).
+struct Channels {
+  int n = 0, R = 0, M = 0, mp = 0, nblock = 0, ncc = 0;
+  int64_t neq = 0, cc_size = 0, p_size = 0;  // p_size: sum r*r (Pandya product)
+  int64_t off_fhh = 0, off_fpp = 0, off_fph = 0, off_2b = 0;
+  std::vector<int> ml, ms;                  // per state
+  std::vector<int> nh, np, nb;              // per block
+  std::vector<int64_t> off;                 // per block x 6
+  std::vector<uint32_t> pairs;              // all pair lists: per block hh, ph, pp
+  std::vector<int> hh_start, ph_start, pp_start;
+  std::vector<uint32_t> iarr;               // M(M-1)/2
+  // single-particle channels (m_l, m_s): states in ascending order
+  int nch = 0;
+  std::vector<int> spch, ch_start, ch_states, ch_lookup;  // ch_lookup[(ml+ML0)*2 + (ms>0)]
+  int ml0 = 0, mlspan = 0;
+  // cross-coupled blocks
+  std::vector<int> cc_r, cc_nb;
+  std::vector<int64_t> cc_off, cc_poff;
+  std::vector<uint32_t> cc_ab, cc_hp;
+  std::vector<int> cc_ab_start, cc_hp_start;
+  std::vector<int> cc_map;                  // M*M x 2 (block,row), -1 if none
+};
+
+// Builds the tables; returns an empty string on success, else the reason.
+std::string build_channels(int nbody, int nshell, Channels &c);
+
+// Device copy of the tables, passed by value to kernels.
+struct DevTab {
+  int n, M, mp, nblock, ncc, nch, ml0, mlspan;
+  int64_t neq, off_fhh, off_fpp, off_fph;
+  const int *nh, *np, *nb;
+  const int64_t *off;
+  const uint32_t *pairs;
+  const int *hh_start, *ph_start, *pp_start;
+  const uint32_t *iarr;
+  const int *spch, *ch_start, *ch_states, *ch_lookup;
+  const int *ml, *ms;
+  const int *cc_r, *cc_nb;
+  const int64_t *cc_off, *cc_poff;
+  const uint32_t *cc_ab, *cc_hp;
+  const int *cc_ab_start, *cc_hp_start;
+  const int2 *cc_map;
+};
+
+}  // namespace qd

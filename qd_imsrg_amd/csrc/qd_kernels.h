@@ -1,0 +1,64 @@
+// Kernel launchers and the work-list records they consume.
+#pragma once
+#include "qd_internal.h"
+
+namespace qd {
+
+// ---- two-body element tiles: one record per run of <= 256 consecutive
+// elements of one stored sub-block
+struct Seg2 {
+  int64_t off;      // flat offset of the sub-block
+  int rows, size;   // rows (ld) and rows*cols
+  int block, sub;
+  int bra0, ket0;   // start of the bra / ket pair lists in DevTab::pairs
+};
+struct Tile2 { int seg, start; };
+
+// ---- grouped FP64 GEMM work lists
+// Operand slots resolved at launch time (the same lists serve every call).
+enum Slot { S_R1 = 0, S_R2, S_OUT, S_W1, S_W2, S_W3, S_CC1, S_CC2, S_P, NSLOT };
+struct Slots { double *p[NSLOT]; };
+struct GemmSeg {   // one product op(A)*op(B) of a problem, alpha = +-1
+  int64_t a_off, b_off;
+  int a_slot, b_slot, lda, ldb, ta, tb, K;
+  double alpha;
+};
+struct GemmProb {  // C(M,N) (+)= sum over segments
+  int64_t c_off;
+  int c_slot, ldc, M, N, beta1, seg0, nseg;
+};
+struct GemmTile { int prob, tm, tn; };
+constexpr int GEMM_TM = 64, GEMM_TN = 64, GEMM_TK = 16;
+
+struct Ops {  // two operators and their hermiticity
+  const double *v1, *v2;
+  int h1, h2;
+};
+
+// flags of the fused two-body kernel
+enum { TB_122 = 1, TB_LADDER = 2, TB_PANDYA = 4, TB_ACCUM = 8 };
+// flags of the one-body kernel
+enum { OB_111 = 1, OB_121 = 2, OB_221 = 4, OB_ACCUM = 8 };
+
+void launch_gemm(hipStream_t s, const GemmProb *probs, const GemmSeg *segs, const GemmTile *tiles,
+                 int ntiles, const Slots &slots);
+void launch_twobody(hipStream_t s, const DevTab &t, const Ops &o, double *out, const Seg2 *segs,
+                    const Tile2 *tiles, int ntiles, int flags, const double *w1, const double *w2,
+                    const double *w3, const double *P, double hh, double eh);
+void launch_onebody(hipStream_t s, const DevTab &t, const Ops &o, double *out, int flags,
+                    const double *w1, const double *w2);
+// zero-body part (commutator_110 + commutator_220) into *dst (device scalar)
+void launch_zerobody(hipStream_t s, const DevTab &t, const Ops &o, const Seg2 *pphh_segs, int npphh,
+                     double *dst, int which /*1: 110, 2: 220, 3: both*/);
+void launch_calc_cc(hipStream_t s, const DevTab &t, const double *v, int herm, double *cc,
+                    const Seg2 *cc_segs, const Tile2 *cc_tiles, int ntiles);
+void launch_white(hipStream_t s, const DevTab &t, const double *rec, int hrec, double *gen,
+                  const Seg2 *pphh_segs, const Tile2 *pphh_tiles, int ntiles);
+void launch_axpby(hipStream_t s, int64_t n, double s1, const double *x, double s2, const double *y, double *z);
+// sum of squares of v[1..neq) -> *dst, via partials (deterministic)
+void launch_norm2(hipStream_t s, const double *v, int64_t neq, double *partials, double *dst);
+void launch_mbpt2(hipStream_t s, const DevTab &t, const double *v, const Seg2 *pphh_segs,
+                  const Tile2 *pphh_tiles, int ntiles, double *partials, double *dst);
+void launch_scale_add_scalar(hipStream_t s, double *dst, const double *a, const double *b);
+
+}  // namespace qd

@@ -1,0 +1,55 @@
+"""C-ABI library checks that need no GPU: it loads, exports every symbol the
+header declares, and its host channel builder equals the oracle's."""
+import os
+import re
+
+import numpy as np
+import pytest
+
+from oracle.channels import Channels
+from qd_imsrg_amd import _lib
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def header_functions():
+    txt = open(os.path.join(ROOT, "include", "qdimsrg.h")).read()
+    txt = re.sub(r"/\*.*?\*/", "", txt, flags=re.S)
+    return sorted(set(re.findall(r"\b(qd_[a-z0-9_]+)\s*\(", txt)))
+
+
+def test_library_loads_and_exports_header():
+    L = _lib.load()
+    names = header_functions()
+    assert len(names) >= 40
+    for name in names:
+        assert hasattr(L, name), name
+    assert set(names) == set(_lib.SIGNATURES), set(names) ^ set(_lib.SIGNATURES)
+
+
+def _table(N, R, which):
+    L = _lib.load()
+    n = L.qd_layout_table(N, R, which, None, 0)
+    assert n >= 0, L.qd_last_error()
+    out = np.zeros(n, dtype=np.int64)
+    assert L.qd_layout_table(N, R, which, out.ctypes.data_as(_lib.i64p), n) == n
+    return out
+
+
+@pytest.mark.parametrize("N,R", [(2, 3), (6, 6), (12, 8), (20, 14)])
+def test_layout_matches_oracle(N, R):
+    ch = Channels(N, R)
+    assert list(_table(N, R, 6)) == [ch.neq, ch.cc_size, ch.nblock, ch.ncc]
+    assert np.array_equal(_table(N, R, 0).reshape(-1, 3), np.stack([ch.nh, ch.np_, ch.nb], 1))
+    assert np.array_equal(_table(N, R, 1).reshape(-1, 6), ch.off)
+    assert np.array_equal(_table(N, R, 2).reshape(-1, 3), ch.iarr)
+    assert np.array_equal(_table(N, R, 3).reshape(-1, 2), ch.cc_map)
+    assert np.array_equal(_table(N, R, 4).reshape(-1, 2), np.stack([ch.cc_r, ch.cc_nb], 1))
+    pairs = np.concatenate([np.concatenate([b["qnhh"], b["qnph"], b["qnpp"]]) for b in ch.blocks])
+    assert np.array_equal(_table(N, R, 5).reshape(-1, 2), pairs)
+
+
+def test_layout_rejects_open_shell():
+    L = _lib.load()
+    assert L.qd_layout_table(5, 4, 6, None, 0) == -3
+    assert b"shell" in L.qd_last_error()

@@ -1,0 +1,152 @@
+"""GPU parity: every commutator entry point of the HIP library against the CPU
+oracle on the same seeded inputs.  Tolerance: max |gpu - oracle| <=
+1e-12 * max |oracle| (FP64; the MFMA accumulation order differs from the
+oracle's).  Sizes: N=6/R=6 (the CPU-runnable configuration) and N=20/R=14
+(the commutator microbenchmark's channel structure)."""
+import sys
+
+import numpy as np
+import pytest
+
+import qd_imsrg_amd as qd
+from oracle.channels import Channels
+from oracle.oracle import Oracle
+from tests.pins.layout import random_flat
+
+pytestmark = pytest.mark.gpu
+TOL = 1e-12
+
+
+def rel(a, b):
+    return np.abs(a - b).max() / max(np.abs(b).max(), 1e-300)
+
+
+class Case:
+    def __init__(self, N, R):
+        print(f"[case] N={N} R={R}", flush=True)
+        self.ch = Channels(N, R)
+        self.o = Oracle(self.ch)
+        self.ctx = qd.Context(N, R)
+        assert self.ctx.neq == self.ch.neq and self.ctx.cc_size == self.ch.cc_size
+
+    def ops(self, seed, h1, h2, dense_f=False):
+        rng = np.random.default_rng(seed)
+        x = random_flat(self.ch, rng, h1, dense_f=dense_f)
+        y = random_flat(self.ch, rng, h2, dense_f=dense_f)
+        return x, y, self.ctx.operator(h1, x), self.ctx.operator(h2, y)
+
+
+@pytest.fixture(scope="module", params=[(6, 6), (20, 14)], ids=["N6R6", "N20R14"])
+def case(request):
+    c = Case(*request.param)
+    yield c
+    c.ctx.close()
+
+
+@pytest.mark.parametrize("h1,h2", [(-1, 1), (1, 1), (1, -1)])
+def test_zero_body(case, h1, h2):
+    x, y, X, Y = case.ops(1, h1, h2)
+    a, b = qd.commutator_110(X, Y), qd.commutator_220(X, Y)
+    ra, rb = case.o.commutator_110(x, h1, y, h2), case.o.commutator_220(x, h1, y, h2)
+    assert abs(a - ra) <= TOL * max(abs(ra), 1) and abs(b - rb) <= TOL * max(abs(rb), 1)
+
+
+@pytest.mark.parametrize("h1,h2", [(-1, 1), (1, 1)])
+def test_one_body_pieces(case, h1, h2):
+    x, y, X, Y = case.ops(2, h1, h2)
+    o, ctx = case.o, case.ctx
+    # 111 overwrites, 121 and 221 add (BCH_EXPAND call order)
+    Z = ctx.operator(1)
+    ref = o.zeros()
+    qd.commutator_111(X, Y, Z); o.commutator_111(x, h1, y, h2, ref)
+    one = slice(1, case.ch.off[0, 0])
+    assert rel(Z.download()[one], ref[one]) <= TOL
+    qd.commutator_121(X, Y, Z); o.commutator_121(x, h1, y, h2, ref)
+    assert rel(Z.download()[one], ref[one]) <= TOL
+    qd.commutator_221(X, Y, Z); o.commutator_221(x, h1, y, h2, ref)
+    assert rel(Z.download()[one], ref[one]) <= TOL
+
+
+@pytest.mark.parametrize("h1,h2", [(-1, 1), (1, -1)])
+def test_two_body_pieces(case, h1, h2):
+    x, y, X, Y = case.ops(3, h1, h2)
+    o, ctx = case.o, case.ctx
+    two = slice(case.ch.off[0, 0], None)
+    Z = ctx.operator(1)
+    ref = o.zeros()
+    qd.commutator_122(X, Y, Z); o.commutator_122(x, h1, y, h2, ref)
+    assert rel(Z.download()[two], ref[two]) <= TOL
+    # 222 with caller-supplied CC sets (oracle calc_cc uploaded)
+    hcc, ecc = o.calc_cc(y, h2), o.calc_cc(x, h1)
+    HCC, ECC = ctx.cc(float(h2)), ctx.cc(float(h1))
+    HCC.upload(hcc[:case.ch.cc_size]); ECC.upload(ecc[:case.ch.cc_size])
+    qd.commutator_222(X, Y, Z, HCC, ECC); o.commutator_222(x, h1, y, h2, ref, hcc, h2, ecc, h1)
+    assert rel(Z.download()[two], ref[two]) <= TOL
+
+
+def test_calc_cc(case):
+    x, _, X, _ = case.ops(4, -1, 1)
+    C = case.ctx.cc(-1.0)
+    qd.calc_cc(X, C)
+    assert np.array_equal(C.download(), case.o.calc_cc(x, -1)[:case.ch.cc_size])
+
+
+@pytest.mark.parametrize("pandya", [True, False])
+@pytest.mark.parametrize("h1,h2", [(-1, 1), (1, 1)])
+def test_full_commutator(case, h1, h2, pandya):
+    x, y, X, Y = case.ops(5, h1, h2)
+    ctx = case.ctx
+    Z = ctx.operator(-h1 * h2)
+    if pandya:
+        HCC, ECC = ctx.cc(float(h2)), ctx.cc(float(h1))
+        qd.calc_cc(Y, HCC); qd.calc_cc(X, ECC)
+        e0 = qd.commutator(X, Y, Z, HCC, ECC)
+    else:
+        e0 = qd.commutator(X, Y, Z)
+    ref = case.o.commutator(x, h1, y, h2, pandya=pandya)
+    got = Z.download()
+    assert rel(got, ref) <= TOL
+    assert e0 == got[0]
+
+
+def test_dense_one_body(case):
+    x, y, X, Y = case.ops(6, -1, 1, dense_f=True)
+    Z = case.ctx.operator(1)
+    HCC, ECC = case.ctx.cc(1.0), case.ctx.cc(-1.0)
+    qd.calc_cc(Y, HCC); qd.calc_cc(X, ECC)
+    qd.commutator(X, Y, Z, HCC, ECC)
+    assert rel(Z.download(), case.o.commutator(x, -1, y, 1)) <= TOL
+
+
+def test_deterministic(case):
+    _, _, X, Y = case.ops(7, -1, 1)
+    HCC, ECC = case.ctx.cc(1.0), case.ctx.cc(-1.0)
+    qd.calc_cc(Y, HCC); qd.calc_cc(X, ECC)
+    Z1, Z2 = case.ctx.operator(1), case.ctx.operator(1)
+    qd.commutator(X, Y, Z1, HCC, ECC)
+    qd.commutator(X, Y, Z2, HCC, ECC)
+    assert np.array_equal(Z1.download(), Z2.download())
+
+
+def test_white_mbpt2_norm_add(case):
+    x, y, X, Y = case.ops(8, 1, 1)
+    o, ctx = case.o, case.ctx
+    E = ctx.operator(-1)
+    qd.build_white(E, X)
+    assert rel(E.download(), o.build_white(x, 1)) <= 1e-15
+    assert abs(qd.MBPT2(X) - o.mbpt2(x)) <= 1e-12 * abs(o.mbpt2(x))
+    assert abs(qd.mat_2_norm(X) - o.norm(x)) <= 1e-13 * o.norm(x)
+    Z = ctx.operator(1)
+    qd.add_arrays(X, 0.5, Y, -2.0, Z)
+    assert np.array_equal(Z.download(), o.add(x, 0.5, y, -2.0))
+
+
+def test_bch_expand(case):
+    rng = np.random.default_rng(9)
+    g = random_flat(case.ch, rng, -1, scale=0.02)
+    h = random_flat(case.ch, rng, 1)
+    G, H, HS = case.ctx.operator(-1, g), case.ctx.operator(1, h), case.ctx.operator(1)
+    nt = qd.BCH_EXPAND(HS, G, H)
+    ref, nref = case.o.bch_expand(g, -1, h)
+    assert nt == nref
+    assert rel(HS.download(), ref) <= 1e-11
