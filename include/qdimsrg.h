@@ -190,6 +190,19 @@ void qd_flow_params_default(qd_flow_params *p);
 int qd_magnus_flow(qd_context *ctx, const qd_op *h0, qd_op *hs, qd_op *omega,
                    const qd_flow_params *p, qd_flow_result *res, double *trace, int trace_len);
 
+/* ---- input side (host FP64, no GPU) --------------------------------------
+ * v_int(qi,qj,qk,ql,hw), src/ME_general.f90:2205-2276: the analytic 2D
+ * oscillator Coulomb element <i j|V|l k> for (n, m_l, m_s) triples. */
+double qd_v_int(const int *qi, const int *qj, const int *qk, const int *ql, double hw);
+/* construct_two_particle_HF_BASIS + HF_E2 (src/HFQDmod.f90:9-254, 292-323):
+ * H(0) in the Hartree-Fock basis, written to h0 (neq doubles, vectorize
+ * order; E0 = E_HF).  The interaction is built from v_int as get_ints
+ * (src/ME_general.f90:746-798) does instead of being read from ME.bin.  eig
+ * (optional, M doubles) receives the HF single-particle energies in states
+ * order, iters the SCF iteration count.  Eigenvector phases: the
+ * largest-magnitude component of each HF orbital is positive. */
+int qd_hf_hamiltonian(int nbody, int nshell, double hw, double *h0, int64_t neq, double *eig, int *iters);
+
 /* ---- timing of the dominant kernels (HIP events on the context stream) --
  * Accumulates per-kernel-class device time while enabled.  Classes: 0 the
  * product GEMM launch (ladder intermediates + Pandya products), 1 the

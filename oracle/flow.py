@@ -15,9 +15,31 @@ R4_1EM9 = 9.999999717180685e-10   # conv_criteria = 1e-9 src/magnus_IMSRG.f90:10
 R4_1EM3 = 0.0010000000474974513   # stp/2 + 1e-3         src/magnus_IMSRG.f90:226
 
 
+BCH_COF = (1.0, 0.5, 0.166666666666666666, 0.04166666666666666, 0.0083333333333333333,
+           .001388888888888, 1.984126984e-4, 2.48015873e-5, 2.755731922e-6, 2.755731922e-7,
+           2.505210839e-8)          # src/magnus_IMSRG.f90:265-267
+R4_1EM5 = 9.999999747378752e-06   # conv = 1e-5 (REAL(4)), src/magnus_IMSRG.f90:259
+
+
+def bch_expand_pandya(o: Oracle, g: np.ndarray, h: np.ndarray):
+    """BCH_EXPAND with every nested commutator's CC inputs filled by calc_cc
+    (the Pandya term the shipped Magnus driver leaves out)."""
+    hs = h.copy()
+    ad = h.copy()
+    for k in range(2, 13):
+        int1 = hs
+        nxt = o.commutator(g, -1, ad, 1, pandya=True)
+        hs = o.add(int1, 1.0, nxt, BCH_COF[k - 2])
+        if o.norm(nxt) * BCH_COF[k - 2] / o.norm(int1) < R4_1EM5:
+            return hs, k - 1
+        ad = nxt
+    return hs, 11
+
+
 def magnus_flow(o: Oracle, h0: np.ndarray, stp: float = R4_0P7, conv: float = R4_1EM9,
-                max_iter: int = 100000, omega0: np.ndarray | None = None):
+                max_iter: int = 100000, omega0: np.ndarray | None = None, pandya: bool = False):
     """Returns dict(hs, omega, s, e0, iterations, rejected, bch_terms, trace)."""
+    bch = (lambda g, h: bch_expand_pandya(o, g, h)) if pandya else (lambda g, h: o.bch_expand(g, -1, h))
     if omega0 is None:
         g = o.zeros()
         hs = h0.copy()
@@ -25,7 +47,7 @@ def magnus_flow(o: Oracle, h0: np.ndarray, stp: float = R4_0P7, conv: float = R4
         crit = abs(o.mbpt2(h0))
     else:
         g = omega0.copy()
-        hs, terms = o.bch_expand(g, -1, h0)
+        hs, terms = bch(g, h0)
         crit = abs(o.mbpt2(hs))
     eta = o.build_white(hs, 1)
     nrm1 = o.norm(eta)
@@ -39,7 +61,7 @@ def magnus_flow(o: Oracle, h0: np.ndarray, stp: float = R4_0P7, conv: float = R4
         g = o.add(g, 1.0, dg, stp)            # euler_step
         s += stp
         hs0 = hs.copy()
-        hs, nt = o.bch_expand(g, -1, h0)
+        hs, nt = bch(g, h0)
         terms += nt
         eta0 = eta.copy()
         eta = o.build_white(hs, 1, gen=eta.copy())

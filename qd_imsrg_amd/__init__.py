@@ -279,3 +279,26 @@ def magnus_flow(ctx: Context, H0: Operator, HS: Operator, OMEGA: Operator, pandy
     trace = np.zeros((trace_len, 6))
     check(ctx.L.qd_magnus_flow(ctx.h, H0.h, HS.h, OMEGA.h, C.byref(p), C.byref(res), _dp(trace), trace_len))
     return res, trace[:res.trace_rows]
+
+
+# ---- input side (host) ------------------------------------------------------
+def v_int(qi, qj, qk, ql, hw: float = 1.0) -> float:
+    """<i j|V|l k> of the analytic oscillator Coulomb interaction (host)."""
+    L = _lib.load()
+    arr = [(C.c_int * 3)(*map(int, q)) for q in (qi, qj, qk, ql)]
+    return L.qd_v_int(*[C.cast(a, _lib.ip) for a in arr], float(hw))
+
+
+def hf_hamiltonian(nbody: int, nshell: int, hw: float):
+    """H(0) in the HF basis (vectorize order) and the HF energies (host)."""
+    L = _lib.load()
+    n = L.qd_layout_table(int(nbody), int(nshell), 6, None, 0)
+    if n < 0:
+        check(int(n))
+    tab = np.zeros(n, dtype=np.int64)
+    L.qd_layout_table(int(nbody), int(nshell), 6, tab.ctypes.data_as(_lib.i64p), n)
+    neq = int(tab[0])
+    M = nshell * (nshell + 1)
+    h0, eig, it = np.zeros(neq), np.zeros(M), C.c_int()
+    check(L.qd_hf_hamiltonian(int(nbody), int(nshell), float(hw), _dp(h0), neq, _dp(eig), C.byref(it)))
+    return h0, eig, it.value

@@ -78,6 +78,8 @@ SIGNATURES = {
     "qd_timing_enable": (C.c_int, [vp, C.c_int]),
     "qd_timing_read": (C.c_int, [vp, dp, i64p, C.c_int]),
     "qd_gemm_flops": (C.c_int, [vp, dp, dp, dp]),
+    "qd_v_int": (C.c_double, [ip, ip, ip, ip, C.c_double]),
+    "qd_hf_hamiltonian": (C.c_int, [C.c_int, C.c_int, C.c_double, dp, C.c_int64, dp, ip]),
 }
 
 _LIB = None

@@ -348,7 +348,7 @@ int full_commutator(qd_context *c, const qd_op *r1, const qd_op *r2, qd_op *r3, 
   {
     Ops o{r1->d, r2->d, r1->herm, r2->herm};
     Timer tm(c, TC_OTHER);
-    launch_zerobody(c->stream, c->dt, o, c->pphh_segs, c->npphh_segs, r3->d, 3);
+    launch_zerobody(c->stream, c->dt, o, c->pphh_segs, c->npphh_segs, c->partials, r3->d, 3);
     HIPC(hipGetLastError());
   }
   return QD_OK;
@@ -432,7 +432,8 @@ int qd_context_create(int nbody, int nshell, qd_context **out) {
   if (hipHostMalloc((void **)&c->host_scal, sizeof(double) * 8, hipHostMallocDefault) != hipSuccess)
     return bail(fail(QD_E_HIP, "hipHostMalloc failed"));
   if ((rc = build_worklists(c))) return bail(rc);
-  if (c->npphh_tiles > 1024 * 64) return bail(fail(QD_E_LAYOUT, "too many pphh tiles for the reduction buffer"));
+  if (c->npphh_tiles > 1024 * 64 || c->npphh_segs + 1 > 1024 * 64)
+    return bail(fail(QD_E_LAYOUT, "too many pphh tiles for the reduction buffer"));
   *out = c;
   return QD_OK;
 }
@@ -576,14 +577,14 @@ int qd_cc_zero(qd_context *c, qd_cc *cc) {
 int qd_commutator_110(qd_context *c, const qd_op *r1, const qd_op *r2, double *out) {
   CHK3(r1, r2, (const qd_op *)nullptr);
   Ops o{r1->d, r2->d, r1->herm, r2->herm};
-  launch_zerobody(c->stream, c->dt, o, c->pphh_segs, c->npphh_segs, c->scal, 1);
+  launch_zerobody(c->stream, c->dt, o, c->pphh_segs, c->npphh_segs, c->partials, c->scal, 1);
   HIPC(hipGetLastError());
   return read_scalars(c, 1, out);
 }
 int qd_commutator_220(qd_context *c, const qd_op *r1, const qd_op *r2, double *out) {
   CHK3(r1, r2, (const qd_op *)nullptr);
   Ops o{r1->d, r2->d, r1->herm, r2->herm};
-  launch_zerobody(c->stream, c->dt, o, c->pphh_segs, c->npphh_segs, c->scal, 2);
+  launch_zerobody(c->stream, c->dt, o, c->pphh_segs, c->npphh_segs, c->partials, c->scal, 2);
   HIPC(hipGetLastError());
   return read_scalars(c, 1, out);
 }

@@ -11,7 +11,7 @@
 // sub-tile = 2x2 MFMA 16x16 accumulators.  K is staged through LDS in chunks
 // of 16 (4 MFMA k-steps).  MFMA operand layout: A(16x4) lane l holds
 // A[l%16][l/16], B(4x16) lane l holds B[l/16][l%16], D(16x16) lane l holds
-// D[4*(l/16)+v][l%16] in component v.
+// D[4*v + l/16][l%16] in component v (FP64: one row per group, four groups).
 #include "qd_kernels.h"
 
 namespace qd {
@@ -80,7 +80,7 @@ __global__ void __launch_bounds__(256) k_gemm(const GemmProb *__restrict__ probs
     const int c = col0 + wn * 32 + ni * 16 + lr;
 #pragma unroll
     for (int v = 0; v < 4; v++) {
-      const int r = row0 + wm * 32 + mi * 16 + 4 * lk + v;
+      const int r = row0 + wm * 32 + mi * 16 + 4 * v + lk;
       if (r < pr.M && c < pr.N) {
         double *p = C + r + (int64_t)c * pr.ldc;
         *p = pr.beta1 ? *p + a[v] : a[v];

@@ -1,0 +1,333 @@
+// Input side of the flow: analytic oscillator Coulomb integrals and the
+// Hartree-Fock H(0) in the reference's flat layout (host FP64).
+//
+// v_int restates src/ME_general.f90:2205-2276 (Anisimovas & Matulis, J. Phys.:
+// Condens. Matter 10, 601 (1998)); binomials are exact integers and the sums
+// run in long double.  The HF driver follows construct_two_particle_HF_BASIS
+// (src/HFQDmod.f90:9-254) with the interaction built from v_int the way
+// get_ints (src/ME_general.f90:746-798) does, instead of reading ME.bin.
+// Eigenvector phases: largest-magnitude component positive.
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <map>
+#include <unordered_map>
+#include <vector>
+
+#include "qd_internal.h"
+
+namespace qd {
+namespace {
+
+typedef long double ld;
+
+ld binom(int n, int k) {
+  if (k < 0 || k > n) return 0;
+  ld r = 1;
+  for (int i = 1; i <= k; i++) r = r * (n - k + i) / i;
+  return std::round(r);
+}
+ld fact(int n) {
+  ld r = 1;
+  for (int i = 2; i <= n; i++) r *= i;
+  return r;
+}
+// Gamma(x/2) for integer x >= 1
+ld gamma_half(int twice) {
+  if (twice % 2 == 0) return fact(twice / 2 - 1);
+  const int xc = (twice - 1) / 2;
+  return fact(2 * xc) / (std::pow((ld)4, xc) * fact(xc)) * 1.772453850905516027298167483341145182L;
+}
+
+struct InnerCache {
+  std::unordered_map<uint64_t, ld> memo;
+  ld get(int g1, int g2, int g3, int g4) {
+    const uint64_t key = ((uint64_t)g1 << 48) | ((uint64_t)g2 << 32) | ((uint64_t)g3 << 16) | (uint64_t)g4;
+    auto it = memo.find(key);
+    if (it != memo.end()) return it->second;
+    const int G = g1 + g2 + g3 + g4;
+    ld s = 0;
+    for (int l1 = 0; l1 <= g1; l1++)
+      for (int l2 = 0; l2 <= g2; l2++)
+        for (int l3 = 0; l3 <= g3; l3++) {
+          const int l4 = l1 + l2 - l3;
+          if (l4 < 0 || l4 > g4) continue;
+          const int lam = l1 + l2 + l3 + l4;
+          const ld sg = ((g2 + g3 - l2 - l3) & 1) ? -1 : 1;
+          s += sg * binom(g1, l1) * binom(g2, l2) * binom(g3, l3) * binom(g4, l4) * gamma_half(2 + lam) *
+               gamma_half(G - lam + 1);
+        }
+    memo.emplace(key, s);
+    return s;
+  }
+};
+
+InnerCache &cache() {
+  static thread_local InnerCache c;
+  return c;
+}
+
+double v_int_impl(const int *qi, const int *qj, const int *qk, const int *ql, double hw) {
+  if (qi[2] != ql[2] || qk[2] != qj[2]) return 0.0;
+  const int ai = std::abs(qi[1]), aj = std::abs(qj[1]), ak = std::abs(qk[1]), al = std::abs(ql[1]);
+  ld sm = 0;
+  for (int j1 = 0; j1 <= qi[0]; j1++)
+    for (int j2 = 0; j2 <= qj[0]; j2++)
+      for (int j3 = 0; j3 <= qk[0]; j3++)
+        for (int j4 = 0; j4 <= ql[0]; j4++) {
+          const int g1 = j1 + j4 + (ai + qi[1]) / 2 + (al - ql[1]) / 2;
+          const int g4 = j1 + j4 + (ai - qi[1]) / 2 + (al + ql[1]) / 2;
+          const int g2 = j2 + j3 + (aj + qj[1]) / 2 + (ak - qk[1]) / 2;
+          const int g3 = j2 + j3 + (aj - qj[1]) / 2 + (ak + qk[1]) / 2;
+          const int G = g1 + g2 + g3 + g4;
+          const ld sg = ((j1 + j2 + j3 + j4) & 1) ? -1 : 1;
+          sm += sg / (fact(j1) * fact(j2) * fact(j3) * fact(j4)) * binom(qi[0] + ai, qi[0] - j1) *
+                binom(qj[0] + aj, qj[0] - j2) * binom(qk[0] + ak, qk[0] - j3) * binom(ql[0] + al, ql[0] - j4) *
+                std::pow((ld)0.5, (ld)(G + 1) * 0.5L) * cache().get(g1, g2, g3, g4);
+        }
+  const ld nrm = fact(qi[0]) / fact(qi[0] + ai) * fact(qj[0]) / fact(qj[0] + aj) * fact(qk[0]) /
+                 fact(qk[0] + ak) * fact(ql[0]) / fact(ql[0] + al);
+  return (double)(std::sqrt((ld)hw) * sm * std::sqrt(nrm));
+}
+
+// cyclic Jacobi eigensolver for a small symmetric matrix (row-major a[n*n]);
+// eigenvalues ascending, eigenvectors in the columns of v
+void jacobi(int n, std::vector<double> a, std::vector<double> &w, std::vector<double> &v) {
+  v.assign((size_t)n * n, 0.0);
+  for (int i = 0; i < n; i++) v[(size_t)i * n + i] = 1.0;
+  for (int sweep = 0; sweep < 100; sweep++) {
+    double off = 0;
+    for (int p = 0; p < n; p++)
+      for (int q = p + 1; q < n; q++) off += a[(size_t)p * n + q] * a[(size_t)p * n + q];
+    if (off < 1e-300) break;
+    for (int p = 0; p < n; p++)
+      for (int q = p + 1; q < n; q++) {
+        const double apq = a[(size_t)p * n + q];
+        if (std::fabs(apq) < 1e-300) continue;
+        const double app = a[(size_t)p * n + p], aqq = a[(size_t)q * n + q];
+        const double th = (aqq - app) / (2 * apq);
+        const double t = (th >= 0 ? 1.0 : -1.0) / (std::fabs(th) + std::sqrt(th * th + 1));
+        const double c = 1 / std::sqrt(t * t + 1), s = t * c;
+        for (int k = 0; k < n; k++) {
+          const double akp = a[(size_t)k * n + p], akq = a[(size_t)k * n + q];
+          a[(size_t)k * n + p] = c * akp - s * akq;
+          a[(size_t)k * n + q] = s * akp + c * akq;
+        }
+        for (int k = 0; k < n; k++) {
+          const double apk = a[(size_t)p * n + k], aqk = a[(size_t)q * n + k];
+          a[(size_t)p * n + k] = c * apk - s * aqk;
+          a[(size_t)q * n + k] = s * apk + c * aqk;
+        }
+        for (int k = 0; k < n; k++) {
+          const double vkp = v[(size_t)k * n + p], vkq = v[(size_t)k * n + q];
+          v[(size_t)k * n + p] = c * vkp - s * vkq;
+          v[(size_t)k * n + q] = s * vkp + c * vkq;
+        }
+      }
+  }
+  std::vector<int> ord(n);
+  for (int i = 0; i < n; i++) ord[i] = i;
+  std::sort(ord.begin(), ord.end(), [&](int x, int y) { return a[(size_t)x * n + x] < a[(size_t)y * n + y]; });
+  std::vector<double> vs((size_t)n * n);
+  w.resize(n);
+  for (int k = 0; k < n; k++) {
+    w[k] = a[(size_t)ord[k] * n + ord[k]];
+    int big = 0;
+    for (int i = 0; i < n; i++)
+      if (std::fabs(v[(size_t)i * n + ord[k]]) > std::fabs(v[(size_t)big * n + ord[k]])) big = i;
+    const double sg = v[(size_t)big * n + ord[k]] < 0 ? -1.0 : 1.0;
+    for (int i = 0; i < n; i++) vs[(size_t)i * n + k] = sg * v[(size_t)i * n + ord[k]];
+  }
+  v.swap(vs);
+}
+
+struct Spatial {  // <ab|V|cd> for m_a + m_b = m_c + m_d, stored by (a, b, c, d)
+  int N = 0;
+  std::vector<int> n, m;
+  std::map<int, std::vector<int>> by_m;
+  std::vector<double> tab;  // N^4 (small R only)
+  double get(int a, int b, int c, int d) const {
+    return tab[(((size_t)a * N + b) * N + c) * N + d];
+  }
+};
+
+int build_spatial(int nshell, double hw, Spatial &S) {
+  for (int R = 0; R < nshell; R++)
+    for (int m = -R; m <= R; m += 2) { S.n.push_back((R - std::abs(m)) / 2); S.m.push_back(m); }
+  S.N = (int)S.n.size();
+  const int N = S.N;
+  if ((double)N * N * N * N > 2.0e9 / 8) return -1;
+  S.tab.assign((size_t)N * N * N * N, 0.0);
+  for (int a = 0; a < N; a++)
+    for (int b = 0; b < N; b++)
+      for (int c = 0; c < N; c++)
+        for (int d = 0; d < N; d++) {
+          if (S.m[a] + S.m[b] != S.m[c] + S.m[d]) continue;
+          // <ab|V|cd> = v_int(a, b, d, c) with equal spins
+          const int qa[3] = {S.n[a], S.m[a], 1}, qb[3] = {S.n[b], S.m[b], 1};
+          const int qc[3] = {S.n[c], S.m[c], 1}, qd[3] = {S.n[d], S.m[d], 1};
+          S.tab[(((size_t)a * N + b) * N + c) * N + d] = v_int_impl(qa, qb, qd, qc, hw);
+        }
+  return 0;
+}
+
+}  // namespace
+}  // namespace qd
+
+using namespace qd;
+
+extern "C" {
+
+double qd_v_int(const int *qi, const int *qj, const int *qk, const int *ql, double hw) {
+  return v_int_impl(qi, qj, qk, ql, hw);
+}
+
+int qd_hf_hamiltonian(int nbody, int nshell, double hw, double *h0, int64_t neq, double *eig_out, int *iters) {
+  Channels ch;
+  std::string why = build_channels(nbody, nshell, ch);
+  if (!why.empty() || !h0) return QD_E_LAYOUT;
+  if (neq != ch.neq) return QD_E_ARG;
+  const int M = ch.M, n = ch.n;
+  Spatial S;
+  if (build_spatial(nshell, hw, S)) return QD_E_LAYOUT;
+  // spin-orbital -> spatial orbital (nm_index order), spin
+  std::vector<int> sp(M), spin(M);
+  {
+    int k = 0;
+    for (int R = 0; R < nshell; R++)
+      for (int m = -R; m <= R; m += 2) {
+        const int a = R * (R + 1) / 2 + (R + m) / 2;  // nm_index - 1
+        for (int s = -1; s <= 1; s += 2) { sp[k] = a; spin[k] = s; k++; }
+      }
+  }
+  auto asym = [&](int p, int q, int r, int s) -> double {  // <pq|V|rs>_A
+    double v = 0;
+    if (spin[p] == spin[r] && spin[q] == spin[s]) v += S.get(sp[p], sp[q], sp[r], sp[s]);
+    if (spin[p] == spin[s] && spin[q] == spin[r]) v -= S.get(sp[p], sp[q], sp[s], sp[r]);
+    return v;
+  };
+  // single-particle channels
+  std::vector<std::vector<int>> chan(ch.nch);
+  for (int i = 0; i < M; i++) chan[ch.spch[i]].push_back(i);
+  std::vector<double> Tm(M), C((size_t)M * M, 0.0), eig(M, 0.0), F((size_t)M * M, 0.0);
+  for (int i = 0; i < M; i++) {
+    const int R = 2 * S.n[sp[i]] + std::abs(S.m[sp[i]]);
+    Tm[i] = hw * (R + 1);
+    C[(size_t)i * M + i] = 1.0;  // C[ho*M + hf]
+  }
+  double crit = 1.0;
+  int it = 0;
+  while (crit > 1e-6 && it < 1000) {
+    it++;
+    // rho_lj = sum_{h<n} C[l,h] C[j,h] (block-diagonal)
+    std::vector<double> rho((size_t)M * M, 0.0);
+    for (auto &cs : chan)
+      for (int l : cs)
+        for (int j : cs) {
+          double s = 0;
+          for (int h = 0; h < n; h++) s += C[(size_t)l * M + h] * C[(size_t)j * M + h];
+          rho[(size_t)l * M + j] = s;
+        }
+    std::fill(F.begin(), F.end(), 0.0);
+    for (auto &ci : chan)
+      for (int i : ci)
+        for (int k : ci) {
+          double v = 0;
+          for (auto &cj : chan)
+            for (int j : cj)
+              for (int l : cj) {
+                const double r = rho[(size_t)l * M + j];
+                if (r != 0.0) v += r * asym(j, i, l, k);
+              }
+          F[(size_t)i * M + k] = (i == k ? Tm[i] : 0.0) + v;
+        }
+    std::vector<double> eold = eig;
+    std::fill(C.begin(), C.end(), 0.0);
+    for (auto &cs : chan) {
+      const int d = (int)cs.size();
+      std::vector<double> a((size_t)d * d), w, v;
+      for (int x = 0; x < d; x++)
+        for (int y = 0; y < d; y++) a[(size_t)x * d + y] = F[(size_t)cs[x] * M + cs[y]];
+      jacobi(d, a, w, v);
+      for (int k = 0; k < d; k++) {
+        eig[cs[k]] = w[k];
+        for (int x = 0; x < d; x++) C[(size_t)cs[x] * M + cs[k]] = v[(size_t)x * d + k];
+      }
+    }
+    crit = 0;
+    for (int i = 0; i < n; i++) crit += std::fabs(eold[i] - eig[i]);
+  }
+  // F_HF = C^T F C (F from the last Fock build)
+  std::vector<double> FC((size_t)M * M, 0.0), FHF((size_t)M * M, 0.0);
+  for (int i = 0; i < M; i++)
+    for (int b = 0; b < M; b++) {
+      double s = 0;
+      for (int k = 0; k < M; k++) s += F[(size_t)i * M + k] * C[(size_t)k * M + b];
+      FC[(size_t)i * M + b] = s;
+    }
+  for (int a = 0; a < M; a++)
+    for (int b = 0; b < M; b++) {
+      double s = 0;
+      for (int i = 0; i < M; i++) s += C[(size_t)i * M + a] * FC[(size_t)i * M + b];
+      FHF[(size_t)a * M + b] = s;
+    }
+  // pack: E0 set below, one-body parts column-major
+  std::memset(h0, 0, sizeof(double) * neq);
+  const int mp = ch.mp;
+  for (int j = 0; j < n; j++)
+    for (int i = 0; i < n; i++) h0[ch.off_fhh + i + (int64_t)j * n] = FHF[(size_t)i * M + j];
+  for (int j = 0; j < mp; j++)
+    for (int i = 0; i < mp; i++) h0[ch.off_fpp + i + (int64_t)j * mp] = FHF[(size_t)(n + i) * M + n + j];
+  for (int j = 0; j < n; j++)
+    for (int i = 0; i < mp; i++) h0[ch.off_fph + i + (int64_t)j * mp] = FHF[(size_t)(n + i) * M + j];
+  // two-body: per block HFV = T V_HO T^T over the block's pairs (hh | ph | pp)
+  double vsum = 0;
+  for (int q = 0; q < ch.nblock; q++) {
+    const int nh = ch.nh[q], nb = ch.nb[q], np = ch.np[q], nt = nh + nb + np;
+    if (nt == 0) continue;
+    const uint32_t *P = &ch.pairs[ch.hh_start[q]];  // hh, ph, pp are contiguous
+    std::vector<double> V((size_t)nt * nt), T((size_t)nt * nt), TV((size_t)nt * nt, 0.0), H((size_t)nt * nt, 0.0);
+    for (int x = 0; x < nt; x++)
+      for (int y = 0; y < nt; y++) V[(size_t)x * nt + y] = asym(lo16(P[x]), hi16(P[x]), lo16(P[y]), hi16(P[y]));
+    for (int x = 0; x < nt; x++) {  // HF pair x, HO pair y
+      const int a1 = lo16(P[x]), a2 = hi16(P[x]);
+      for (int y = 0; y < nt; y++) {
+        const int b1 = lo16(P[y]), b2 = hi16(P[y]);
+        T[(size_t)x * nt + y] = C[(size_t)b1 * M + a1] * C[(size_t)b2 * M + a2] - C[(size_t)b2 * M + a1] * C[(size_t)b1 * M + a2];
+      }
+    }
+    for (int x = 0; x < nt; x++)
+      for (int z = 0; z < nt; z++) {
+        const double t = T[(size_t)x * nt + z];
+        if (t == 0.0) continue;
+        for (int y = 0; y < nt; y++) TV[(size_t)x * nt + y] += t * V[(size_t)z * nt + y];
+      }
+    for (int x = 0; x < nt; x++)
+      for (int y = 0; y < nt; y++) {
+        double s = 0;
+        for (int z = 0; z < nt; z++) s += TV[(size_t)x * nt + z] * T[(size_t)y * nt + z];
+        H[(size_t)x * nt + y] = s;
+      }
+    const int64_t *o = &ch.off[(size_t)q * 6];
+    auto put = [&](int sub, int r0, int rows, int c0, int cols) {
+      for (int c = 0; c < cols; c++)
+        for (int r = 0; r < rows; r++) h0[o[sub] + r + (int64_t)c * rows] = H[(size_t)(r0 + r) * nt + c0 + c];
+    };
+    const int h = 0, x = nh, p = nh + nb;
+    put(HHHH, h, nh, h, nh);
+    put(PPPP, p, np, p, np);
+    put(PPHH, p, np, h, nh);
+    put(PHPH, x, nb, x, nb);
+    put(PPPH, p, np, x, nb);
+    put(PHHH, x, nb, h, nh);
+    for (int j = 0; j < nh; j++) vsum += H[(size_t)j * nt + j];
+  }
+  double eh = 0;
+  for (int i = 0; i < n; i++) eh += eig[i];
+  h0[0] = eh - vsum;  // HF_E2 (src/HFQDmod.f90:292-323)
+  if (eig_out) std::memcpy(eig_out, eig.data(), sizeof(double) * M);
+  if (iters) *iters = it;
+  return QD_OK;
+}
+
+}  // extern "C"

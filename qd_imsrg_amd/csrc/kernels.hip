@@ -139,13 +139,20 @@ void launch_twobody(hipStream_t s, const DevTab &t, const Ops &o, double *out, c
 
 // ---------------------------------------------------------------------------
 // One-body part: commutator_111 (src/commutators.f90:137-195), _121 (:73-134)
-// and the partial traces of _221 (:278-348), one thread per output element of
-// fhh, fpp, fph (contiguous in the flat layout).
-__device__ double c121(const DevTab &t, const Ops &o, int x, int y) {
+// and the partial traces of _221 (:278-348).  One wave per output element of
+// fhh, fpp, fph (contiguous in the flat layout from offset 1); the lanes split
+// every summation and a fixed xor-shuffle tree reduces them (deterministic).
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+  return v;
+}
+
+__device__ double c121_lane(const DevTab &t, const Ops &o, int x, int y, int lane) {
   const int n = t.n;
   const int dl = t.ml[y] - t.ml[x], ds = t.ms[y] - t.ms[x];
   double r = 0.0;
-  for (int i = 0; i < n; i++) {
+  for (int i = lane; i < n; i += 64) {
     // particles a with (ml_a - ml_i, ms_a - ms_i) = +(dl,ds) or -(dl,ds)
     for (int sgn = 0; sgn < 2; sgn++) {
       if (sgn == 1 && dl == 0 && ds == 0) break;
@@ -170,8 +177,9 @@ __global__ void __launch_bounds__(256) k_onebody(DevTab t, Ops o, double *__rest
                                                  const double *__restrict__ w1, const double *__restrict__ w2) {
   const int n = t.n, mp = t.mp;
   const int64_t nhh = (int64_t)n * n, npp = (int64_t)mp * mp, nph = (int64_t)mp * n;
-  const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (e >= nhh + npp + nph) return;
+  const int lane = threadIdx.x & 63;
+  const int64_t e = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+  if (e >= nhh + npp + nph) return;  // whole wave exits together
   int kind, i, j, x, y;
   if (e < nhh) { kind = 0; i = (int)(e % n); j = (int)(e / n); x = i; y = j; }
   else if (e < nhh + npp) { kind = 1; int64_t f = e - nhh; i = (int)(f % mp); j = (int)(f / mp); x = n + i; y = n + j; }
@@ -182,55 +190,54 @@ __global__ void __launch_bounds__(256) k_onebody(DevTab t, Ops o, double *__rest
 #define FHH(v, a, b) (v)[ohh + (a) + (int64_t)(b) * n]
 #define FPP(v, a, b) (v)[opp + (a) + (int64_t)(b) * mp]
 #define FPH(v, a, b) (v)[oph + (a) + (int64_t)(b) * mp]
-  const int64_t dst = 1 + e;  // fhh, fpp, fph are contiguous from offset 1
+  const int64_t dst = 1 + e;
   double acc = (flags & OB_ACCUM) ? out[dst] : 0.0;
   if (flags & OB_111) {
-    double val;
+    double q1 = 0, q1t = 0, q2 = 0, q2t = 0, q3 = 0, q4 = 0;
     if (kind == 0) {
-      double q1 = 0, q1t = 0, q2 = 0, q2t = 0;
-      for (int k = 0; k < n; k++) { q1 += FHH(A, i, k) * FHH(B, k, j); q1t += FHH(A, j, k) * FHH(B, k, i); }
-      for (int k = 0; k < mp; k++) { q2 += FPH(B, k, i) * FPH(A, k, j); q2t += FPH(B, k, j) * FPH(A, k, i); }
-      val = q1 - h2 * q2 + h1 * (q2t - h2 * q1t);
+      for (int k = lane; k < n; k += 64) { q1 += FHH(A, i, k) * FHH(B, k, j); q1t += FHH(A, j, k) * FHH(B, k, i); }
+      for (int k = lane; k < mp; k += 64) { q2 += FPH(B, k, i) * FPH(A, k, j); q2t += FPH(B, k, j) * FPH(A, k, i); }
     } else if (kind == 1) {
-      double q1 = 0, q1t = 0, q2 = 0, q2t = 0;
-      for (int k = 0; k < mp; k++) { q1 += FPP(B, i, k) * FPP(A, k, j); q1t += FPP(B, j, k) * FPP(A, k, i); }
-      for (int k = 0; k < n; k++) { q2 += FPH(A, i, k) * FPH(B, j, k); q2t += FPH(A, j, k) * FPH(B, i, k); }
-      val = h2 * q2 + h1 * (h2 * q1t - q2t) - q1;
+      for (int k = lane; k < mp; k += 64) { q1 += FPP(B, i, k) * FPP(A, k, j); q1t += FPP(B, j, k) * FPP(A, k, i); }
+      for (int k = lane; k < n; k += 64) { q2 += FPH(A, i, k) * FPH(B, j, k); q2t += FPH(A, j, k) * FPH(B, i, k); }
     } else {
-      double q1 = 0, q2 = 0, q3 = 0, q4 = 0;
-      for (int k = 0; k < n; k++) { q1 += FPH(A, i, k) * FHH(B, k, j); q2 += FPH(B, i, k) * FHH(A, k, j); }
-      for (int k = 0; k < mp; k++) { q3 += FPP(A, i, k) * FPH(B, k, j); q4 += FPP(B, i, k) * FPH(A, k, j); }
-      val = q1 - q2 + q3 - q4;
+      for (int k = lane; k < n; k += 64) { q1 += FPH(A, i, k) * FHH(B, k, j); q2 += FPH(B, i, k) * FHH(A, k, j); }
+      for (int k = lane; k < mp; k += 64) { q3 += FPP(A, i, k) * FPH(B, k, j); q4 += FPP(B, i, k) * FPH(A, k, j); }
     }
+    q1 = wave_sum(q1); q1t = wave_sum(q1t); q2 = wave_sum(q2); q2t = wave_sum(q2t);
+    q3 = wave_sum(q3); q4 = wave_sum(q4);
+    double val;
+    if (kind == 0) val = q1 - h2 * q2 + h1 * (q2t - h2 * q1t);
+    else if (kind == 1) val = h2 * q2 + h1 * (h2 * q1t - q2t) - q1;
+    else val = q1 - q2 + q3 - q4;
     acc = (flags & OB_ACCUM) ? acc + val : val;
   }
 #undef FHH
 #undef FPP
 #undef FPH
-  if (flags & OB_121) acc += c121(t, o, x, y);
+  if (flags & OB_121) acc += wave_sum(c121_lane(t, o, x, y, lane));
   if ((flags & OB_221) && t.spch[x] == t.spch[y]) {
     // partial traces of the 221 intermediates (w1, w2 have herm = +1)
     double r = 0.0;
-    const double *wh = (kind == 1) ? w1 : (kind == 0 ? w1 : w2);  // hole-sum operand (first term)
-    for (int a = 0; a < t.M; a++) {
+    for (int a = lane; a < t.M; a += 64) {
       if (a < n) {
         if (kind == 2) r += h1 * velem(t, w2, 1, a, x, a, y) - h2 * velem(t, w1, 1, a, x, a, y);
-        else r += h1 * velem(t, wh, 1, a, y, a, x) - h2 * velem(t, wh, 1, a, x, a, y);
+        else r += h1 * velem(t, w1, 1, a, y, a, x) - h2 * velem(t, w1, 1, a, x, a, y);
       } else {
         if (kind == 1) r += -h1 * velem(t, w1, 1, a, y, a, x) + h2 * velem(t, w1, 1, a, x, a, y);
         else if (kind == 0) r += -h1 * velem(t, w2, 1, a, y, a, x) + h2 * velem(t, w2, 1, a, x, a, y);
         else r += -h1 * velem(t, w2, 1, a, x, a, y) + h2 * velem(t, w1, 1, a, x, a, y);
       }
     }
-    acc += r;
+    acc += wave_sum(r);
   }
-  out[dst] = acc;
+  if (lane == 0) out[dst] = acc;
 }
 
 void launch_onebody(hipStream_t s, const DevTab &t, const Ops &o, double *out, int flags, const double *w1,
                     const double *w2) {
   const int64_t tot = (int64_t)t.n * t.n + (int64_t)t.mp * t.mp + (int64_t)t.mp * t.n;
-  hipLaunchKernelGGL(k_onebody, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, s, t, o, out, flags, w1, w2);
+  hipLaunchKernelGGL(k_onebody, dim3((unsigned)((tot + 3) / 4)), dim3(256), 0, s, t, o, out, flags, w1, w2);
 }
 
 // ---------------------------------------------------------------------------
@@ -249,27 +256,35 @@ __device__ __forceinline__ double block_sum256(double v) {
 }
 
 // commutator_110 + commutator_220 (src/commutators.f90:12-70): (h1 - h2) *
-// (sum fph1 .* fph2 + sum over blocks Vpphh1 .* Vpphh2); one work-group.
-__global__ void __launch_bounds__(256) k_zerobody(DevTab t, Ops o, const Seg2 *__restrict__ segs, int nseg,
-                                                  double *dst, int which) {
-  double a = 0.0, b = 0.0;
-  if (which & 1) {
-    const int64_t m = (int64_t)t.mp * t.n;
-    for (int64_t e = threadIdx.x; e < m; e += 256) a += o.v1[t.off_fph + e] * o.v2[t.off_fph + e];
-  }
-  if (which & 2)
-    for (int sgi = 0; sgi < nseg; sgi++) {
-      const Seg2 sg = segs[sgi];
-      for (int e = threadIdx.x; e < sg.size; e += 256) b += o.v1[sg.off + e] * o.v2[sg.off + e];
+// (sum fph1 .* fph2 + sum over blocks Vpphh1 .* Vpphh2).  Work-group 0 takes
+// the fph product, work-group g >= 1 the pphh segment g-1; k_zero_final adds
+// the partials in a fixed order.
+__global__ void __launch_bounds__(256) k_zerobody_part(DevTab t, Ops o, const Seg2 *__restrict__ segs, int which,
+                                                       double *part) {
+  double a = 0.0;
+  if (blockIdx.x == 0) {
+    if (which & 1) {
+      const int64_t m = (int64_t)t.mp * t.n;
+      for (int64_t e = threadIdx.x; e < m; e += 256) a += o.v1[t.off_fph + e] * o.v2[t.off_fph + e];
     }
+  } else if (which & 2) {
+    const Seg2 sg = segs[blockIdx.x - 1];
+    for (int e = threadIdx.x; e < sg.size; e += 256) a += o.v1[sg.off + e] * o.v2[sg.off + e];
+  }
   a = block_sum256(a);
+  if (threadIdx.x == 0) part[blockIdx.x] = a;
+}
+__global__ void __launch_bounds__(256) k_zero_final(Ops o, const double *__restrict__ part, int np, double *dst) {
+  double b = 0.0;
+  for (int k = 1 + threadIdx.x; k < np; k += 256) b += part[k];
   b = block_sum256(b);
-  if (threadIdx.x == 0) *dst = (double)(o.h1 - o.h2) * a + (double)(o.h1 - o.h2) * b;
+  if (threadIdx.x == 0) *dst = (double)(o.h1 - o.h2) * part[0] + (double)(o.h1 - o.h2) * b;
 }
 
 void launch_zerobody(hipStream_t s, const DevTab &t, const Ops &o, const Seg2 *pphh_segs, int npphh,
-                     double *dst, int which) {
-  hipLaunchKernelGGL(k_zerobody, dim3(1), dim3(256), 0, s, t, o, pphh_segs, npphh, dst, which);
+                     double *part, double *dst, int which) {
+  hipLaunchKernelGGL(k_zerobody_part, dim3(1 + npphh), dim3(256), 0, s, t, o, pphh_segs, which, part);
+  hipLaunchKernelGGL(k_zero_final, dim3(1), dim3(256), 0, s, o, part, 1 + npphh, dst);
 }
 
 // ---------------------------------------------------------------------------

@@ -49,7 +49,7 @@ void launch_onebody(hipStream_t s, const DevTab &t, const Ops &o, double *out, i
                     const double *w1, const double *w2);
 // zero-body part (commutator_110 + commutator_220) into *dst (device scalar)
 void launch_zerobody(hipStream_t s, const DevTab &t, const Ops &o, const Seg2 *pphh_segs, int npphh,
-                     double *dst, int which /*1: 110, 2: 220, 3: both*/);
+                     double *part, double *dst, int which /*1: 110, 2: 220, 3: both*/);
 void launch_calc_cc(hipStream_t s, const DevTab &t, const double *v, int herm, double *cc,
                     const Seg2 *cc_segs, const Tile2 *cc_tiles, int ntiles);
 void launch_white(hipStream_t s, const DevTab &t, const double *rec, int hrec, double *gen,
