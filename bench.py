@@ -54,6 +54,32 @@ def cpu_baseline(x, y, ch, flops_ref, budget_s=20.0):
                        f"{dt / n * 1e3:.1f} ms each, OMP_NUM_THREADS={threads}")
 
 
+def breakdown(ctx, X, Y, HCC, ECC, reps=10):
+    """Device time of each reference entry point on its own (HIP events)."""
+    import qd_imsrg_amd as qd
+    Z = ctx.operator(-1)
+    pieces = {
+        "calc_cc": lambda: qd.calc_cc(Y, HCC),
+        "commutator_111": lambda: qd.commutator_111(X, Y, Z),
+        "commutator_121": lambda: qd.commutator_121(X, Y, Z),
+        "commutator_221": lambda: qd.commutator_221(X, Y, Z),
+        "commutator_122": lambda: qd.commutator_122(X, Y, Z),
+        "commutator_222": lambda: qd.commutator_222(X, Y, Z, HCC, ECC),
+        "commutator_222_no_pandya": lambda: qd.commutator_222(X, Y, Z, None, None),
+    }
+    res = {}
+    for name, fn in pieces.items():
+        fn()
+        ctx.sync()
+        ctx.timing(True)
+        for _ in range(reps):
+            fn()
+        ms, n = ctx.timing_read()
+        ctx.timing(False)
+        res[name] = {CLASS_NAMES[i]: ms[i] / reps for i in range(6) if n[i]}
+    return res
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -63,6 +89,8 @@ def main():
     ap.add_argument("--nshell", type=int, default=14)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--seed", type=int, default=20261203)
+    ap.add_argument("--breakdown", action="store_true",
+                    help="also time each reference entry point (110..222, calc_cc) on its own")
     a = ap.parse_args()
     if a.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {a.gpus} is not supported "
@@ -134,8 +162,11 @@ def main():
                                f"N={a.nbody} R={a.nshell} (M={ch.M}, neq={ch.neq})",
                    "flops_2b2b_reference_count": fl["total"], "parallelism": "single GPU"},
         "kernel_ms_per_step": {CLASS_NAMES[i]: ms_cls[i] / a.steps for i in range(6)},
+        "gemm_flops_per_step": gf,
         "roofline": roof,
     }
+    if a.breakdown:
+        out["breakdown_ms"] = breakdown(ctx, X, Y, HCC, ECC)
     if not a.no_cpu:
         out["cpu_baseline"] = cpu_baseline(x, y, ch, fl["total"])
     ctx.close()

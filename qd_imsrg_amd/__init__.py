@@ -197,8 +197,9 @@ def commutator_122(r1, r2, r3):
     L, h = _L(r1); check(L.qd_commutator_122(h, r1.h, r2.h, r3.h))
 
 
-def commutator_222(r1, r2, r3, HCC: CCMat, ETACC: CCMat):
-    L, h = _L(r1); check(L.qd_commutator_222(h, r1.h, r2.h, r3.h, HCC.h, ETACC.h))
+def commutator_222(r1, r2, r3, HCC: CCMat | None, ETACC: CCMat | None):
+    L, h = _L(r1)
+    check(L.qd_commutator_222(h, r1.h, r2.h, r3.h, HCC.h if HCC else None, ETACC.h if ETACC else None))
 
 
 def calc_cc(H: Operator, HCC: CCMat):

@@ -1,5 +1,6 @@
 // C ABI of the engine: contexts, device-resident operators, the commutator
 // pipeline, BCH_EXPAND and the Magnus flow (s_evolve).
+#include <algorithm>
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
@@ -58,9 +59,11 @@ struct qd_context {
   hipStream_t stream = nullptr;
   std::vector<void *> allocs;
   double *w1 = nullptr, *w2 = nullptr, *w3 = nullptr, *P = nullptr;
+  double *u1 = nullptr, *u2 = nullptr;  // cross-coupled mat-vecs of commutator_121 (M*M each)
   double *partials = nullptr, *scal = nullptr;  // reduction partials, 8 device scalars
   double *host_scal = nullptr;                  // pinned mirror of scal
-  Seg2 *segs2 = nullptr; Tile2 *tiles2 = nullptr; int ntiles2 = 0;
+  Task2 *tasks2 = nullptr; int ntasks2 = 0;
+  TilePair *ptp = nullptr; int nptp = 0;  // tile pairs of the Pandya products
   Seg2 *pphh_segs = nullptr; Tile2 *pphh_tiles = nullptr; int npphh_segs = 0, npphh_tiles = 0;
   Seg2 *cc_segs = nullptr; Tile2 *cc_tiles = nullptr; int ncc_tiles = 0;
   GemmList g_all, g_acc;  // products (G1 + Pandya, Pandya tiles last) and accumulating products
@@ -176,25 +179,51 @@ int upload_gemm(qd_context *c, GemmBuilder &b, GemmList &g) {
 int build_worklists(qd_context *c) {
   const Channels &ch = c->ch;
   int rc;
-  // ---- two-body segments and tiles
-  std::vector<Seg2> segs, pseg;
-  std::vector<Tile2> tiles, ptiles;
+  // ---- two-body tasks (block, ket type, ket tile), grouped per XCD: all tasks
+  // of one block run on one XCD so its operands stay in that XCD's L2; blocks
+  // are assigned to XCDs largest-first (LPT on the element count)
+  std::vector<Seg2> pseg;
+  std::vector<Tile2> ptiles;
+  std::vector<std::vector<Task2>> per_block(ch.nblock);
+  std::vector<double> work(ch.nblock, 0.0);
   for (int q = 0; q < ch.nblock; q++) {
-    const int nh = ch.nh[q], np = ch.np[q], nb = ch.nb[q];
-    const int H = ch.hh_start[q], X = ch.ph_start[q], Pp = ch.pp_start[q];
-    const int rows[6] = {nh, np, np, nb, np, nb}, cols[6] = {nh, np, nh, nb, nb, nh};
-    const int bra[6] = {H, Pp, Pp, X, Pp, X}, ket[6] = {H, Pp, H, X, X, H};
-    for (int s = 0; s < 6; s++) {
-      if (rows[s] * cols[s] == 0) continue;
-      Seg2 sg{ch.off[(size_t)q * 6 + s], rows[s], rows[s] * cols[s], q, s, bra[s], ket[s]};
-      add_tiles(tiles, (int)segs.size(), sg.size);
-      segs.push_back(sg);
-      if (s == PPHH) { add_tiles(ptiles, (int)pseg.size(), sg.size); pseg.push_back(sg); }
+    const int nh = ch.nh[q], np = ch.np[q], nb = ch.nb[q], ntot = nh + nb + np;
+    if (ntot == 0) continue;
+    if (2 * ntot > TB_LDS_DOUBLES) return fail(QD_E_LAYOUT, "two-body block of %d pairs exceeds the LDS tile", ntot);
+    const int tj = std::min(64, std::max(1, TB_LDS_DOUBLES / (2 * ntot)));
+    const int ncol[3] = {nh, nb, np}, start[3] = {0, nh, nh + nb};
+    for (int tau = 0; tau < 3; tau++) {
+      const int nrows = ntot - start[tau];
+      if (ncol[tau] == 0 || nrows == 0) continue;
+      for (int j0 = 0; j0 < ncol[tau]; j0 += tj) per_block[q].push_back({q, tau, j0, std::min(tj, ncol[tau] - j0)});
+      work[q] += (double)nrows * ncol[tau];
+    }
+    if (np * nh) {
+      Seg2 sg{ch.off[(size_t)q * 6 + PPHH], np, np * nh, q, PPHH, ch.pp_start[q], ch.hh_start[q]};
+      add_tiles(ptiles, (int)pseg.size(), sg.size);
+      pseg.push_back(sg);
     }
   }
-  if ((rc = dev_upload(c, segs, &c->segs2))) return rc;
-  if ((rc = dev_upload(c, tiles, &c->tiles2))) return rc;
-  c->ntiles2 = (int)tiles.size();
+  {
+    std::vector<int> order(ch.nblock);
+    for (int q = 0; q < ch.nblock; q++) order[q] = q;
+    std::sort(order.begin(), order.end(), [&](int a, int b) { return work[a] > work[b]; });
+    std::vector<std::vector<Task2>> xcd(NXCD);
+    std::vector<double> load(NXCD, 0.0);
+    for (int q : order) {
+      if (per_block[q].empty()) continue;
+      int best = (int)(std::min_element(load.begin(), load.end()) - load.begin());
+      load[best] += work[q];
+      xcd[best].insert(xcd[best].end(), per_block[q].begin(), per_block[q].end());
+    }
+    size_t longest = 0;
+    for (auto &l : xcd) longest = std::max(longest, l.size());
+    std::vector<Task2> tasks(longest * NXCD, Task2{-1, 0, 0, 0});
+    for (int x = 0; x < NXCD; x++)
+      for (size_t k = 0; k < xcd[x].size(); k++) tasks[k * NXCD + x] = xcd[x][k];
+    if ((rc = dev_upload(c, tasks, &c->tasks2))) return rc;
+    c->ntasks2 = (int)tasks.size();
+  }
   if ((rc = dev_upload(c, pseg, &c->pphh_segs))) return rc;
   if ((rc = dev_upload(c, ptiles, &c->pphh_tiles))) return rc;
   c->npphh_segs = (int)pseg.size();
@@ -266,6 +295,18 @@ int build_worklists(qd_context *c) {
     g1.end();
   }
   c->pandya_flops = g1.flops - f1;
+  {
+    std::vector<TilePair> tp;
+    for (int q = 0; q < ch.ncc; q++) {
+      const int r = ch.cc_r[q], nb = ch.cc_nb[q];
+      if (r * nb == 0) continue;
+      const int nt = (r + 31) / 32;
+      for (int ti = 0; ti < nt; ti++)
+        for (int tj = ti; tj < nt; tj++) tp.push_back({q, ti, tj});
+    }
+    if ((rc = dev_upload(c, tp, &c->ptp))) return rc;
+    c->nptp = (int)tp.size();
+  }
   if ((rc = upload_gemm(c, g1, c->g_all))) return rc;
   if ((rc = upload_gemm(c, g2, c->g_acc))) return rc;
   return QD_OK;
@@ -298,9 +339,16 @@ bool pandya_on(const qd_cc *hcc, const qd_cc *ecc) {
 int run_products(qd_context *c, const qd_op *r1, const qd_op *r2, qd_op *out, const qd_cc *hcc, const qd_cc *ecc) {
   const bool pd = pandya_on(hcc, ecc);
   Slots s = make_slots(c, r1, r2, out, ecc, hcc);
-  Timer tm(c, TC_PRODUCTS);
-  launch_gemm(c->stream, c->g_all.probs, c->g_all.segs, c->g_all.tiles, pd ? c->g_all.ntiles : c->g1_ntiles, s);
-  HIPC(hipGetLastError());
+  {
+    Timer tm(c, TC_PRODUCTS);
+    launch_gemm(c->stream, c->g_all.probs, c->g_all.segs, c->g_all.tiles, pd ? c->g_all.ntiles : c->g1_ntiles, s);
+    HIPC(hipGetLastError());
+  }
+  if (pd) {
+    Timer tc(c, TC_OTHER);
+    launch_pandya_combine(c->stream, c->dt, c->P, c->ptp, c->nptp, hcc->herm, ecc->herm);
+    HIPC(hipGetLastError());
+  }
   return QD_OK;
 }
 
@@ -317,7 +365,7 @@ int run_twobody(qd_context *c, const qd_op *r1, const qd_op *r2, qd_op *out, int
   Ops o{r1->d, r2->d, r1->herm, r2->herm};
   double hh = hcc ? hcc->herm : 1.0, eh = ecc ? ecc->herm : -1.0;
   Timer tm(c, TC_TWOBODY);
-  launch_twobody(c->stream, c->dt, o, out->d, c->segs2, c->tiles2, c->ntiles2, flags, c->w1, c->w2, c->w3, c->P, hh, eh);
+  launch_twobody(c->stream, c->dt, o, out->d, c->tasks2, c->ntasks2, flags, c->w1, c->w2, c->w3, c->P, hh, eh);
   HIPC(hipGetLastError());
   return QD_OK;
 }
@@ -325,7 +373,7 @@ int run_twobody(qd_context *c, const qd_op *r1, const qd_op *r2, qd_op *out, int
 int run_onebody(qd_context *c, const qd_op *r1, const qd_op *r2, qd_op *out, int flags) {
   Ops o{r1->d, r2->d, r1->herm, r2->herm};
   Timer tm(c, TC_ONEBODY);
-  launch_onebody(c->stream, c->dt, o, out->d, flags, c->w1, c->w2);
+  launch_onebody(c->stream, c->dt, o, out->d, flags, c->w1, c->w2, c->u1, c->u2);
   HIPC(hipGetLastError());
   return QD_OK;
 }
@@ -420,13 +468,15 @@ int qd_context_create(int nbody, int nshell, qd_context **out) {
   UP(ch.spch, spch); UP(ch.ch_start, ch_start); UP(ch.ch_states, ch_states); UP(ch.ch_lookup, ch_lookup);
   UP(ch.ml, ml); UP(ch.ms, ms); UP(ch.cc_r, cc_r); UP(ch.cc_nb, cc_nb); UP(ch.cc_off, cc_off);
   UP(ch.cc_poff, cc_poff); UP(ch.cc_ab, cc_ab); UP(ch.cc_hp, cc_hp); UP(ch.cc_ab_start, cc_ab_start);
-  UP(ch.cc_hp_start, cc_hp_start); UP(ccmap, cc_map);
+  UP(ch.cc_hp_start, cc_hp_start); UP(ccmap, cc_map); UP(ch.sub_start, sub_start); UP(ch.sub_list, sub_list);
 #undef UP
   void *p;
   if ((rc = dev_alloc(c, sizeof(double) * ch.neq, &p))) return bail(rc); c->w1 = (double *)p;
   if ((rc = dev_alloc(c, sizeof(double) * ch.neq, &p))) return bail(rc); c->w2 = (double *)p;
   if ((rc = dev_alloc(c, sizeof(double) * ch.neq, &p))) return bail(rc); c->w3 = (double *)p;
   if ((rc = dev_alloc(c, sizeof(double) * ch.p_size, &p))) return bail(rc); c->P = (double *)p;
+  if ((rc = dev_alloc(c, sizeof(double) * ch.M * ch.M, &p))) return bail(rc); c->u1 = (double *)p;
+  if ((rc = dev_alloc(c, sizeof(double) * ch.M * ch.M, &p))) return bail(rc); c->u2 = (double *)p;
   if ((rc = dev_alloc(c, sizeof(double) * 1024 * 64, &p))) return bail(rc); c->partials = (double *)p;
   if ((rc = dev_alloc(c, sizeof(double) * 8, &p))) return bail(rc); c->scal = (double *)p;
   if (hipHostMalloc((void **)&c->host_scal, sizeof(double) * 8, hipHostMallocDefault) != hipSuccess)

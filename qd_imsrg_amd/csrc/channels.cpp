@@ -139,6 +139,39 @@ std::string build_channels(int nbody, int nshell, Channels &c) {
   c.ncc = q;
   c.cc_size = coff;
   c.p_size = poff;
+
+  // ---- substitution lists: pair (i,j) with i or j replaced by a state t of
+  // the same (m_l, m_s) channel stays in the same two-body block
+  c.sub_start.assign(c.pairs.size() + 1, 0);
+  c.sub_list.clear();
+  if (M > 2047) return "too many states for the substitution lists";
+  for (int b = 0; b < c.nblock; b++) {
+    const int base = c.hh_start[b];
+    const int ntot = c.nh[b] + c.nb[b] + c.np[b];
+    if (ntot > 65535) return "block too large for the substitution lists";
+    for (int cidx = 0; cidx < ntot; cidx++) {
+      const uint32_t pr = c.pairs[base + cidx];
+      const int i = lo16(pr), j = hi16(pr);
+      c.sub_start[base + cidx] = (int)c.sub_list.size();
+      for (int pos = 0; pos < 2; pos++) {
+        const int x = pos ? j : i, other = pos ? i : j;
+        const int ch = c.spch[x];
+        for (int s = c.ch_start[ch]; s < c.ch_start[ch + 1]; s++) {
+          const int t = c.ch_states[s];
+          if (t == other) continue;
+          int a, bb, neg;
+          if (pos == 0) { neg = t > other; a = neg ? other : t; bb = neg ? t : other; }
+          else { neg = other > t; a = neg ? t : other; bb = neg ? other : t; }
+          const uint32_t ia = c.iarr[(size_t)a * M - (size_t)a * (a + 1) / 2 + (bb - a - 1)];
+          if (ia_blk(ia) != b) return "internal error: substitution left its block";
+          const int ty = ia_type(ia);
+          const int comb = ty == T_HH ? ia_idx(ia) : ty == T_PH ? c.nh[b] + ia_idx(ia) : c.nh[b] + c.nb[b] + ia_idx(ia);
+          c.sub_list.push_back((uint32_t)comb | ((uint32_t)t << 16) | ((uint32_t)pos << 27) | ((uint32_t)neg << 28));
+        }
+      }
+    }
+  }
+  c.sub_start[c.pairs.size()] = (int)c.sub_list.size();
   return "";
 }
 

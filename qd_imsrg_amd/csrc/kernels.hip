@@ -37,6 +37,12 @@ __device__ __forceinline__ double velem(const DevTab &t, const double *__restric
   }
 }
 
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+  return v;
+}
+
 // f_elem (src/ME_general.f90:1780-1813)
 __device__ __forceinline__ double felem(const DevTab &t, const double *__restrict__ v, int herm, int a, int b) {
   const int n = t.n;
@@ -45,96 +51,205 @@ __device__ __forceinline__ double felem(const DevTab &t, const double *__restric
 }
 
 // ---------------------------------------------------------------------------
-// commutator_122 (src/commutators.f90:355-676) for one output element with
-// indices in storage order.  For index position k the reference sums over
-// holes t with f(x_k,t) and over particles t with f(t,x_k) (times herm of
-// the one-body operand on the bra side / -1 and -herm on the ket side); only
-// t in the (m_l, m_s) channel of x_k can give a non-zero two-body element.
-__device__ double c122_half(const DevTab &t, const double *__restrict__ fa, int ha,
-                            const double *__restrict__ vb, int hb, const int x[4]) {
+// Element of one two-body block by combined (hh | ph | pp) row/column index,
+// with the hermiticity rule of v_elem for the sub-blocks stored transposed.
+__device__ __forceinline__ double vfull(const DevTab &t, const double *__restrict__ v, int herm, int q, int nh,
+                                        int nb, int np, int P, int Q) {
+  const int64_t *o = t.off + 6 * q;
+  const int tp = P < nh ? 0 : (P < nh + nb ? 1 : 2), tq = Q < nh ? 0 : (Q < nh + nb ? 1 : 2);
+  const int ip = P - (tp == 0 ? 0 : (tp == 1 ? nh : nh + nb));
+  const int iq = Q - (tq == 0 ? 0 : (tq == 1 ? nh : nh + nb));
+  switch (tp * 3 + tq) {
+    case 0: return v[o[HHHH] + ip + (int64_t)iq * nh];
+    case 1: return herm * v[o[PHHH] + iq + (int64_t)ip * nb];
+    case 2: return herm * v[o[PPHH] + iq + (int64_t)ip * np];
+    case 3: return v[o[PHHH] + ip + (int64_t)iq * nb];
+    case 4: return v[o[PHPH] + ip + (int64_t)iq * nb];
+    case 5: return herm * v[o[PPPH] + iq + (int64_t)ip * np];
+    case 6: return v[o[PPHH] + ip + (int64_t)iq * np];
+    case 7: return v[o[PPPH] + ip + (int64_t)iq * np];
+    default: return v[o[PPPP] + ip + (int64_t)iq * np];
+  }
+}
+
+// Offset of block element (P, Q) (combined indices) in the flat layout; *tr is
+// set when the element is stored transposed (the herm factor of v_elem applies).
+__device__ __forceinline__ int64_t vidx(const DevTab &t, int q, int nh, int nb, int np, int P, int Q, bool *tr) {
+  const int64_t *o = t.off + 6 * q;
+  const int tp = P < nh ? 0 : (P < nh + nb ? 1 : 2), tq = Q < nh ? 0 : (Q < nh + nb ? 1 : 2);
+  const int ip = P - (tp == 0 ? 0 : (tp == 1 ? nh : nh + nb));
+  const int iq = Q - (tq == 0 ? 0 : (tq == 1 ? nh : nh + nb));
+  *tr = tp < tq;
+  switch (tp * 3 + tq) {
+    case 0: return o[HHHH] + ip + (int64_t)iq * nh;
+    case 1: return o[PHHH] + iq + (int64_t)ip * nb;
+    case 2: return o[PPHH] + iq + (int64_t)ip * np;
+    case 3: return o[PHHH] + ip + (int64_t)iq * nb;
+    case 4: return o[PHPH] + ip + (int64_t)iq * nb;
+    case 5: return o[PPPH] + iq + (int64_t)ip * np;
+    case 6: return o[PPHH] + ip + (int64_t)iq * np;
+    case 7: return o[PPPH] + ip + (int64_t)iq * np;
+    default: return o[PPPP] + ip + (int64_t)iq * np;
+  }
+}
+
+// commutator_122 (src/commutators.f90:355-676) for the output <x1 x2|V|x3 x4>
+// in storage order: [f1, V2] - [f2, V1] in one pass over the substitution
+// lists.  For the replaced index x the reference sums over holes t with f(x,t)
+// and over particles t with f(t,x), times herm of the one-body operand on the
+// bra side and -1 / -herm on the ket side; only t in the (m_l, m_s) channel of
+// x can give a non-zero element, which is what the precomputed lists
+// enumerate (with the pair-sorting sign).  Bra substitutions read the LDS
+// tiles (column j of the block), ket substitutions read block row Ic.
+__device__ __forceinline__ double c122(const DevTab &t, const Ops &o, const double *__restrict__ tile1,
+                                       const double *__restrict__ tile2, int ntot, int j, int q, int nh, int nb,
+                                       int np, int gI, int gJ, int Ic, int x1, int x2, int x3, int x4) {
   double r = 0.0;
-  for (int k = 0; k < 4; k++) {
-    const int xk = x[k];
-    const int c = t.spch[xk];
-    const int s0 = t.ch_start[c], s1 = t.ch_start[c + 1];
-    for (int s = s0; s < s1; s++) {
-      const int tt = t.ch_states[s];
-      double coef;
-      if (k < 2) coef = (tt < t.n) ? felem(t, fa, ha, xk, tt) : ha * felem(t, fa, ha, tt, xk);
-      else coef = (tt < t.n) ? -ha * felem(t, fa, ha, xk, tt) : -felem(t, fa, ha, tt, xk);
-      int y0 = x[0], y1 = x[1], y2 = x[2], y3 = x[3];
-      if (k == 0) y0 = tt; else if (k == 1) y1 = tt; else if (k == 2) y2 = tt; else y3 = tt;
-      r += coef * velem(t, vb, hb, y0, y1, y2, y3);
-    }
+  const int n = t.n;
+  const double h1 = o.h1, h2 = o.h2;
+  for (int k = t.sub_start[gI], ke = t.sub_start[gI + 1]; k < ke; k++) {
+    const uint32_t en = t.sub_list[k];
+    const int Pp = (int)(en & 0xffffu), tt = (int)((en >> 16) & 0x7ffu);
+    const int xk = ((en >> 27) & 1u) ? x2 : x1;
+    double c1, c2;
+    if (tt < n) { c1 = felem(t, o.v1, o.h1, xk, tt); c2 = felem(t, o.v2, o.h2, xk, tt); }
+    else { c1 = h1 * felem(t, o.v1, o.h1, tt, xk); c2 = h2 * felem(t, o.v2, o.h2, tt, xk); }
+    const double d = c1 * tile2[j * ntot + Pp] - c2 * tile1[j * ntot + Pp];
+    r += ((en >> 28) & 1u) ? -d : d;
+  }
+  for (int k = t.sub_start[gJ], ke = t.sub_start[gJ + 1]; k < ke; k++) {
+    const uint32_t en = t.sub_list[k];
+    const int Qp = (int)(en & 0xffffu), tt = (int)((en >> 16) & 0x7ffu);
+    const int xk = ((en >> 27) & 1u) ? x4 : x3;
+    double c1, c2;
+    if (tt < n) { c1 = -h1 * felem(t, o.v1, o.h1, xk, tt); c2 = -h2 * felem(t, o.v2, o.h2, xk, tt); }
+    else { c1 = -felem(t, o.v1, o.h1, tt, xk); c2 = -felem(t, o.v2, o.h2, tt, xk); }
+    bool tr;
+    const int64_t a = vidx(t, q, nh, nb, np, Ic, Qp, &tr);
+    const double v2 = tr ? h2 * o.v2[a] : o.v2[a], v1 = tr ? h1 * o.v1[a] : o.v1[a];
+    const double d = c1 * v2 - c2 * v1;
+    r += ((en >> 28) & 1u) ? -d : d;
   }
   return r;
 }
 
-// Pandya difference D[(a,b),(c,d)] = hh * P[(a,b),(c,d)] - eh * P[(c,d),(a,b)]
-// with P = ETACC * HCC^T per CC block: XCC - YCC of construct_cc_ints
-// (src/ME_general.f90:948-949), since YCC = eh * HCC ETACC^T = eh * P^T.
-__device__ __forceinline__ double pandya_d(const DevTab &t, const double *__restrict__ P, double hh,
-                                           double eh, int a, int b, int c, int d) {
+// Pandya difference D = XCC - YCC of construct_cc_ints (src/ME_general.f90:
+// 948-949) at CC rows (a,b), (c,d); k_pandya_combine has already turned the
+// product P = ETACC HCC^T into D = hh P - eh P^T in place (YCC = eh P^T).
+__device__ __forceinline__ double pandya_d(const DevTab &t, const double *__restrict__ D, int a, int b, int c,
+                                           int d) {
   const int2 m1 = t.cc_map[a * t.M + b], m2 = t.cc_map[c * t.M + d];
   const int q = m1.x;
   if (t.cc_nb[q] == 0) return 0.0;
-  const int64_t r = t.cc_r[q];
-  const double *B = P + t.cc_poff[q];
-  return hh * B[m1.y + m2.y * r] - eh * B[m2.y + m1.y * r];
+  return D[t.cc_poff[q] + m1.y + (int64_t)m2.y * t.cc_r[q]];
 }
 
+// In place D = hh * P - eh * P^T on every r x r CC product, one 32x32 tile
+// pair (ti <= tj) per work-group through LDS.
+__global__ void __launch_bounds__(256) k_pandya_combine(DevTab t, double *__restrict__ P,
+                                                        const TilePair *__restrict__ tp, double hh, double eh) {
+  __shared__ double A[32][33], B[32][33];
+  const TilePair p = tp[blockIdx.x];
+  const int r = t.cc_r[p.blk];
+  double *base = P + t.cc_poff[p.blk];
+  const int i0 = p.ti * 32, j0 = p.tj * 32;
+  for (int e = threadIdx.x; e < 1024; e += 256) {
+    const int li = e & 31, lj = e >> 5;
+    A[li][lj] = (i0 + li < r && j0 + lj < r) ? base[(i0 + li) + (int64_t)(j0 + lj) * r] : 0.0;
+    B[li][lj] = (j0 + li < r && i0 + lj < r) ? base[(j0 + li) + (int64_t)(i0 + lj) * r] : 0.0;
+  }
+  __syncthreads();
+  for (int e = threadIdx.x; e < 1024; e += 256) {
+    const int li = e & 31, lj = e >> 5;
+    if (i0 + li < r && j0 + lj < r) base[(i0 + li) + (int64_t)(j0 + lj) * r] = hh * A[li][lj] - eh * B[lj][li];
+    if (p.ti != p.tj && j0 + li < r && i0 + lj < r)
+      base[(j0 + li) + (int64_t)(i0 + lj) * r] = hh * B[li][lj] - eh * A[lj][li];
+  }
+}
+
+void launch_pandya_combine(hipStream_t s, const DevTab &t, double *P, const TilePair *tp, int ntp, double hh,
+                           double eh) {
+  if (ntp <= 0) return;
+  hipLaunchKernelGGL(k_pandya_combine, dim3(ntp), dim3(256), 0, s, t, P, tp, hh, eh);
+}
+
+// All stored two-body outputs of one task: rows of every pair type that is
+// stored against this ket type (hh kets: hh, ph, pp rows; ph kets: ph, pp
+// rows; pp kets: pp rows), i.e. the six sub-blocks of the flat layout.
 __global__ void __launch_bounds__(256) k_twobody(DevTab t, Ops o, double *__restrict__ out,
-                                                 const Seg2 *__restrict__ segs,
-                                                 const Tile2 *__restrict__ tiles, int flags,
+                                                 const Task2 *__restrict__ tasks, int flags,
                                                  const double *__restrict__ w1, const double *__restrict__ w2,
                                                  const double *__restrict__ w3, const double *__restrict__ P,
                                                  double hh, double eh) {
-  const Tile2 tl = tiles[blockIdx.x];
-  const Seg2 sg = segs[tl.seg];
-  const int e = tl.start + threadIdx.x;
-  if (e >= sg.size) return;
-  const int I = e % sg.rows, J = e / sg.rows;
-  const uint32_t bra = t.pairs[sg.bra0 + I], ket = t.pairs[sg.ket0 + J];
-  const int x[4] = {lo16(bra), hi16(bra), lo16(ket), hi16(ket)};
-  const int64_t idx = sg.off + e;
-  double acc = (flags & TB_ACCUM) ? out[idx] : 0.0;
-  if (flags & TB_122)
-    acc += c122_half(t, o.v1, o.h1, o.v2, o.h2, x) - c122_half(t, o.v2, o.h2, o.v1, o.h1, x);
-  if (flags & TB_LADDER) {
-    // transposed / signed combinations of the products of matrix_mult222
-    // (src/commutators.f90:999-1127) that cannot be accumulated by the GEMM
-    const double h1 = o.h1, h2 = o.h2;
-    const int64_t tr = sg.off + J + (int64_t)I * sg.rows;  // (J,I) of a square sub-block
-    switch (sg.sub) {
-      case HHHH:
-      case PPPP:
-        acc += h1 * w1[tr] - h2 * w1[idx];
-        acc += h1 * h2 * w3[tr] - w3[idx];
-        break;
-      case PHHH: acc += h1 * w2[idx] - h2 * w1[idx]; break;
-      case PPPH: acc += -h2 * w1[idx] + h1 * w2[idx]; break;
-      case PHPH:
-        acc += h1 * w2[tr] - h2 * w2[idx];
-        acc += h1 * w1[tr] - h2 * w1[idx];
-        break;
-      default: break;
+  __shared__ double lds[TB_LDS_DOUBLES];
+  const Task2 tk = tasks[blockIdx.x];
+  if (tk.block < 0) return;
+  const int q = tk.block, tau = tk.tau, j0 = tk.j0, tj = tk.tj;
+  const int nh = t.nh[q], nb = t.nb[q], np = t.np[q], ntot = nh + nb + np;
+  const int base = t.hh_start[q];
+  const int tstart = tau == 0 ? 0 : (tau == 1 ? nh : nh + nb);
+  const bool do122 = (flags & TB_122) != 0;
+  if (do122) {
+    for (int e = threadIdx.x; e < ntot * tj; e += 256) {
+      const int r = e % ntot, j = e / ntot;
+      lds[e] = vfull(t, o.v1, o.h1, q, nh, nb, np, r, tstart + j0 + j);
+      lds[ntot * tj + e] = vfull(t, o.v2, o.h2, q, nh, nb, np, r, tstart + j0 + j);
     }
+    __syncthreads();
   }
-  if (flags & TB_PANDYA) {
-    // src/commutators.f90:746-751; storage-order indices carry the ph sign
-    const int x1 = x[0], x2 = x[1], x3 = x[2], x4 = x[3];
-    acc += pandya_d(t, P, hh, eh, x3, x1, x2, x4) - pandya_d(t, P, hh, eh, x3, x2, x1, x4) +
-           pandya_d(t, P, hh, eh, x4, x2, x1, x3) - pandya_d(t, P, hh, eh, x4, x1, x2, x3);
+  const int nrows = ntot - tstart;  // row types stored against this ket type
+  const int64_t *off = t.off + 6 * q;
+  for (int e = threadIdx.x; e < nrows * tj; e += 256) {
+    const int Ic = tstart + e % nrows, j = e / nrows;
+    const int Jl = j0 + j, Jc = tstart + Jl;
+    const int rho = Ic < nh ? 0 : (Ic < nh + nb ? 1 : 2);
+    const int Il = Ic - (rho == 0 ? 0 : (rho == 1 ? nh : nh + nb));
+    const int rows = rho == 0 ? nh : (rho == 1 ? nb : np);
+    int sub;
+    if (tau == 0) sub = rho == 0 ? HHHH : (rho == 1 ? PHHH : PPHH);
+    else if (tau == 1) sub = rho == 1 ? PHPH : PPPH;
+    else sub = PPPP;
+    const int64_t idx = off[sub] + Il + (int64_t)Jl * rows;
+    const uint32_t bra = t.pairs[base + Ic], ket = t.pairs[base + Jc];
+    const int x1 = lo16(bra), x2 = hi16(bra), x3 = lo16(ket), x4 = hi16(ket);
+    double acc = (flags & TB_ACCUM) ? out[idx] : 0.0;
+    if (do122) {
+      acc += c122(t, o, lds, lds + ntot * tj, ntot, j, q, nh, nb, np, base + Ic, base + Jc, Ic, x1, x2, x3, x4);
+    }
+    if (flags & TB_LADDER) {
+      // transposed / signed combinations of the products of matrix_mult222
+      // (src/commutators.f90:999-1127) that cannot be accumulated by the GEMM
+      const double h1 = o.h1, h2 = o.h2;
+      const int64_t tr = off[sub] + Jl + (int64_t)Il * rows;  // (J,I) of a square sub-block
+      switch (sub) {
+        case HHHH:
+        case PPPP:
+          acc += h1 * w1[tr] - h2 * w1[idx];
+          acc += h1 * h2 * w3[tr] - w3[idx];
+          break;
+        case PHHH: acc += h1 * w2[idx] - h2 * w1[idx]; break;
+        case PPPH: acc += -h2 * w1[idx] + h1 * w2[idx]; break;
+        case PHPH:
+          acc += h1 * w2[tr] - h2 * w2[idx];
+          acc += h1 * w1[tr] - h2 * w1[idx];
+          break;
+        default: break;
+      }
+    }
+    if (flags & TB_PANDYA) {
+      // src/commutators.f90:746-751; storage-order indices carry the ph sign
+      acc += pandya_d(t, P, x3, x1, x2, x4) - pandya_d(t, P, x3, x2, x1, x4) + pandya_d(t, P, x4, x2, x1, x3) -
+             pandya_d(t, P, x4, x1, x2, x3);
+    }
+    out[idx] = acc;
   }
-  out[idx] = acc;
 }
 
-void launch_twobody(hipStream_t s, const DevTab &t, const Ops &o, double *out, const Seg2 *segs,
-                    const Tile2 *tiles, int ntiles, int flags, const double *w1, const double *w2,
-                    const double *w3, const double *P, double hh, double eh) {
-  if (ntiles <= 0) return;
-  hipLaunchKernelGGL(k_twobody, dim3(ntiles), dim3(256), 0, s, t, o, out, segs, tiles, flags, w1, w2, w3, P,
-                     hh, eh);
+void launch_twobody(hipStream_t s, const DevTab &t, const Ops &o, double *out, const Task2 *tasks, int ntasks,
+                    int flags, const double *w1, const double *w2, const double *w3, const double *P, double hh,
+                    double eh) {
+  if (ntasks <= 0) return;
+  hipLaunchKernelGGL(k_twobody, dim3(ntasks), dim3(256), 0, s, t, o, out, tasks, flags, w1, w2, w3, P, hh, eh);
 }
 
 // ---------------------------------------------------------------------------
@@ -142,39 +257,38 @@ void launch_twobody(hipStream_t s, const DevTab &t, const Ops &o, double *out, c
 // and the partial traces of _221 (:278-348).  One wave per output element of
 // fhh, fpp, fph (contiguous in the flat layout from offset 1); the lanes split
 // every summation and a fixed xor-shuffle tree reduces them (deterministic).
-__device__ __forceinline__ double wave_sum(double v) {
-#pragma unroll
-  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
-  return v;
+
+// commutator_121 through the cross-coupled matrices: with
+// CC_k[(p,q),(i,a)] = <q i|V_k|p a> (calc_cc), v(a,x,i,y) = h * CC[(x,y),(i,a)]
+// and v(i,x,a,y) = CC[(y,x),(i,a)], so the 121 sum over (i,a) of
+// src/commutators.f90:83-132 becomes u_k[(p,q)] = sum_(i,a) CC_k[(p,q),(i,a)]
+// f_other(a,i): one thread per ordered pair (p,q).
+__global__ void __launch_bounds__(256) k_cc_matvec(DevTab t, const double *__restrict__ v, int herm,
+                                                   const double *__restrict__ f, double *__restrict__ u) {
+  const int64_t e = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);  // one wave per ordered pair
+  const int lane = threadIdx.x & 63;
+  if (e >= (int64_t)t.M * t.M) return;
+  const int p = (int)(e / t.M), q = (int)(e % t.M);
+  const int blk = t.cc_map[e].x;
+  const int nb = t.cc_nb[blk];
+  const uint32_t *hp = t.cc_hp + t.cc_hp_start[blk];
+  double s = 0.0;
+  for (int c = lane; c < nb; c += 64) {
+    const int i = lo16(hp[c]), a = hi16(hp[c]);
+    s += velem(t, v, herm, q, i, p, a) * f[t.off_fph + (a - t.n) + (int64_t)i * t.mp];
+  }
+  s = wave_sum(s);
+  if (lane == 0) u[e] = s;
 }
 
-__device__ double c121_lane(const DevTab &t, const Ops &o, int x, int y, int lane) {
-  const int n = t.n;
-  const int dl = t.ml[y] - t.ml[x], ds = t.ms[y] - t.ms[x];
-  double r = 0.0;
-  for (int i = lane; i < n; i += 64) {
-    // particles a with (ml_a - ml_i, ms_a - ms_i) = +(dl,ds) or -(dl,ds)
-    for (int sgn = 0; sgn < 2; sgn++) {
-      if (sgn == 1 && dl == 0 && ds == 0) break;
-      const int mla = t.ml[i] + (sgn ? -dl : dl), msa = t.ms[i] + (sgn ? -ds : ds);
-      if (mla < -t.ml0 || mla > t.ml0 || (msa != 1 && msa != -1)) continue;
-      const int c = t.ch_lookup[(mla + t.ml0) * 2 + (msa > 0)];
-      if (c < 0) continue;
-      for (int s = t.ch_start[c]; s < t.ch_start[c + 1]; s++) {
-        const int a = t.ch_states[s];
-        if (a < n) continue;
-        const double f1 = o.v1[t.off_fph + (a - n) + (int64_t)i * t.mp];
-        const double f2 = o.v2[t.off_fph + (a - n) + (int64_t)i * t.mp];
-        r += f1 * (o.h1 * velem(t, o.v2, o.h2, a, x, i, y) - velem(t, o.v2, o.h2, i, x, a, y));
-        r -= f2 * (o.h2 * velem(t, o.v1, o.h1, a, x, i, y) - velem(t, o.v1, o.h1, i, x, a, y));
-      }
-    }
-  }
-  return r;
+void launch_cc_matvec(hipStream_t s, const DevTab &t, const double *v, int herm, const double *f, double *u) {
+  const int64_t m = (int64_t)t.M * t.M;
+  hipLaunchKernelGGL(k_cc_matvec, dim3((unsigned)((m + 3) / 4)), dim3(256), 0, s, t, v, herm, f, u);
 }
 
 __global__ void __launch_bounds__(256) k_onebody(DevTab t, Ops o, double *__restrict__ out, int flags,
-                                                 const double *__restrict__ w1, const double *__restrict__ w2) {
+                                                 const double *__restrict__ w1, const double *__restrict__ w2,
+                                                 const double *__restrict__ u1, const double *__restrict__ u2) {
   const int n = t.n, mp = t.mp;
   const int64_t nhh = (int64_t)n * n, npp = (int64_t)mp * mp, nph = (int64_t)mp * n;
   const int lane = threadIdx.x & 63;
@@ -215,7 +329,11 @@ __global__ void __launch_bounds__(256) k_onebody(DevTab t, Ops o, double *__rest
 #undef FHH
 #undef FPP
 #undef FPH
-  if (flags & OB_121) acc += wave_sum(c121_lane(t, o, x, y, lane));
+  if (flags & OB_121) {
+    // u2 = CC(r2) f(r1), u1 = CC(r1) f(r2)
+    const int64_t xy = (int64_t)x * t.M + y, yx = (int64_t)y * t.M + x;
+    acc += (double)(o.h1 * o.h2) * u2[xy] - u2[yx] - ((double)(o.h1 * o.h2) * u1[xy] - u1[yx]);
+  }
   if ((flags & OB_221) && t.spch[x] == t.spch[y]) {
     // partial traces of the 221 intermediates (w1, w2 have herm = +1)
     double r = 0.0;
@@ -235,9 +353,13 @@ __global__ void __launch_bounds__(256) k_onebody(DevTab t, Ops o, double *__rest
 }
 
 void launch_onebody(hipStream_t s, const DevTab &t, const Ops &o, double *out, int flags, const double *w1,
-                    const double *w2) {
+                    const double *w2, double *u1, double *u2) {
+  if (flags & OB_121) {
+    launch_cc_matvec(s, t, o.v2, o.h2, o.v1, u2);
+    launch_cc_matvec(s, t, o.v1, o.h1, o.v2, u1);
+  }
   const int64_t tot = (int64_t)t.n * t.n + (int64_t)t.mp * t.mp + (int64_t)t.mp * t.n;
-  hipLaunchKernelGGL(k_onebody, dim3((unsigned)((tot + 3) / 4)), dim3(256), 0, s, t, o, out, flags, w1, w2);
+  hipLaunchKernelGGL(k_onebody, dim3((unsigned)((tot + 3) / 4)), dim3(256), 0, s, t, o, out, flags, w1, w2, u1, u2);
 }
 
 // ---------------------------------------------------------------------------

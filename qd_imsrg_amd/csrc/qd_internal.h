@@ -48,6 +48,11 @@ struct Channels {
   std::vector<uint32_t> cc_ab, cc_hp;
   std::vector<int> cc_ab_start, cc_hp_start;
   std::vector<int> cc_map;                  // M*M x 2 (block,row), -1 if none
+  // one-index substitutions of every pair (for the 1b x 2b commutator):
+  // entry = P' (combined hh|ph|pp index in the block, 16 bits) | t << 16
+  // (11 bits) | pos << 27 (which index was replaced) | neg << 28 (sort sign)
+  std::vector<int> sub_start;               // per global pair index, size npairs + 1
+  std::vector<uint32_t> sub_list;
 };
 
 // Builds the tables; returns an empty string on success, else the reason.
@@ -69,6 +74,8 @@ struct DevTab {
   const uint32_t *cc_ab, *cc_hp;
   const int *cc_ab_start, *cc_hp_start;
   const int2 *cc_map;
+  const int *sub_start;
+  const uint32_t *sub_list;
 };
 
 }  // namespace qd

@@ -14,6 +14,13 @@ struct Seg2 {
 };
 struct Tile2 { int seg, start; };
 
+// ---- two-body tasks: one (block, ket type, ket tile) per work-group.  The
+// work-group stages both operands' full block columns of its kets in LDS.
+struct Task2 { int block, tau, j0, tj; };  // tau: 0 hh, 1 ph, 2 pp kets; block < 0: idle
+constexpr int TB_LDS_DOUBLES = 4096;        // 32 KB: two tiles of ntot * tj doubles
+constexpr int NXCD = 8;                     // MI355X: work-group i runs on XCD i % 8
+struct TilePair { int blk, ti, tj; };       // 32x32 tiles (ti, tj) and (tj, ti) of one CC block
+
 // ---- grouped FP64 GEMM work lists
 // Operand slots resolved at launch time (the same lists serve every call).
 enum Slot { S_R1 = 0, S_R2, S_OUT, S_W1, S_W2, S_W3, S_CC1, S_CC2, S_P, NSLOT };
@@ -42,11 +49,11 @@ enum { OB_111 = 1, OB_121 = 2, OB_221 = 4, OB_ACCUM = 8 };
 
 void launch_gemm(hipStream_t s, const GemmProb *probs, const GemmSeg *segs, const GemmTile *tiles,
                  int ntiles, const Slots &slots);
-void launch_twobody(hipStream_t s, const DevTab &t, const Ops &o, double *out, const Seg2 *segs,
-                    const Tile2 *tiles, int ntiles, int flags, const double *w1, const double *w2,
-                    const double *w3, const double *P, double hh, double eh);
+void launch_twobody(hipStream_t s, const DevTab &t, const Ops &o, double *out, const Task2 *tasks,
+                    int ntasks, int flags, const double *w1, const double *w2, const double *w3,
+                    const double *P, double hh, double eh);
 void launch_onebody(hipStream_t s, const DevTab &t, const Ops &o, double *out, int flags,
-                    const double *w1, const double *w2);
+                    const double *w1, const double *w2, double *u1, double *u2);
 // zero-body part (commutator_110 + commutator_220) into *dst (device scalar)
 void launch_zerobody(hipStream_t s, const DevTab &t, const Ops &o, const Seg2 *pphh_segs, int npphh,
                      double *part, double *dst, int which /*1: 110, 2: 220, 3: both*/);
@@ -54,6 +61,8 @@ void launch_calc_cc(hipStream_t s, const DevTab &t, const double *v, int herm, d
                     const Seg2 *cc_segs, const Tile2 *cc_tiles, int ntiles);
 void launch_white(hipStream_t s, const DevTab &t, const double *rec, int hrec, double *gen,
                   const Seg2 *pphh_segs, const Tile2 *pphh_tiles, int ntiles);
+void launch_pandya_combine(hipStream_t s, const DevTab &t, double *P, const TilePair *tp, int ntp, double hh,
+                           double eh);
 void launch_axpby(hipStream_t s, int64_t n, double s1, const double *x, double s2, const double *y, double *z);
 // sum of squares of v[1..neq) -> *dst, via partials (deterministic)
 void launch_norm2(hipStream_t s, const double *v, int64_t neq, double *partials, double *dst);

@@ -1,0 +1,221 @@
+! iso_c_binding interface to libqdimsrg.so (include/qdimsrg.h).
+!
+! A Fortran caller of the reference's operator API (src/commutators.f90,
+! src/magnus_IMSRG.f90, src/gs_IMSRG.f90) uses these procedures on opaque
+! device handles; full_ham data crosses as the reference's vectorize vector
+! (src/ME_general.f90:1871-1989) through qd_op_upload / qd_op_download.
+module qdimsrg_c
+  use iso_c_binding
+  implicit none
+
+  integer(c_int), parameter :: QD_OK = 0
+
+  type, bind(C) :: qd_info
+     integer(c_int64_t) :: nbody, nshell, msp, nblock, ncc, neq, cc_size, n2b
+  end type qd_info
+
+  type, bind(C) :: qd_flow_params
+     real(c_double) :: stp, conv
+     integer(c_int) :: max_iter, pandya, verbose, warm_start
+  end type qd_flow_params
+
+  type, bind(C) :: qd_flow_result
+     real(c_double) :: s, e0, crit
+     integer(c_int) :: iterations, rejected
+     integer(c_int64_t) :: bch_terms
+     integer(c_int) :: converged, trace_rows
+  end type qd_flow_result
+
+  interface
+     function qd_context_create(nbody, nshell, ctx) bind(C, name="qd_context_create")
+       import :: c_int, c_ptr
+       integer(c_int), value :: nbody, nshell
+       type(c_ptr) :: ctx
+       integer(c_int) :: qd_context_create
+     end function
+     function qd_context_destroy(ctx) bind(C, name="qd_context_destroy")
+       import :: c_int, c_ptr
+       type(c_ptr), value :: ctx
+       integer(c_int) :: qd_context_destroy
+     end function
+     function qd_context_info(ctx, info) bind(C, name="qd_context_info")
+       import :: c_int, c_ptr, qd_info
+       type(c_ptr), value :: ctx
+       type(qd_info) :: info
+       integer(c_int) :: qd_context_info
+     end function
+     function qd_op_create(ctx, herm, op) bind(C, name="qd_op_create")
+       import :: c_int, c_ptr
+       type(c_ptr), value :: ctx
+       integer(c_int), value :: herm
+       type(c_ptr) :: op
+       integer(c_int) :: qd_op_create
+     end function
+     function qd_op_destroy(op) bind(C, name="qd_op_destroy")
+       import :: c_int, c_ptr
+       type(c_ptr), value :: op
+       integer(c_int) :: qd_op_destroy
+     end function
+     function qd_op_set_herm(op, herm) bind(C, name="qd_op_set_herm")
+       import :: c_int, c_ptr
+       type(c_ptr), value :: op
+       integer(c_int), value :: herm
+       integer(c_int) :: qd_op_set_herm
+     end function
+     function qd_op_upload(op, vec, neq) bind(C, name="qd_op_upload")
+       import :: c_int, c_ptr, c_double, c_int64_t
+       type(c_ptr), value :: op
+       real(c_double) :: vec(*)
+       integer(c_int64_t), value :: neq
+       integer(c_int) :: qd_op_upload
+     end function
+     function qd_op_download(op, vec, neq) bind(C, name="qd_op_download")
+       import :: c_int, c_ptr, c_double, c_int64_t
+       type(c_ptr), value :: op
+       real(c_double) :: vec(*)
+       integer(c_int64_t), value :: neq
+       integer(c_int) :: qd_op_download
+     end function
+     function qd_cc_create(ctx, herm, cc) bind(C, name="qd_cc_create")
+       import :: c_int, c_ptr, c_double
+       type(c_ptr), value :: ctx
+       real(c_double), value :: herm
+       type(c_ptr) :: cc
+       integer(c_int) :: qd_cc_create
+     end function
+     function qd_cc_destroy(cc) bind(C, name="qd_cc_destroy")
+       import :: c_int, c_ptr
+       type(c_ptr), value :: cc
+       integer(c_int) :: qd_cc_destroy
+     end function
+     ! ---- commutators (src/commutators.f90) ----
+     function qd_commutator_110(ctx, r1, r2, c) bind(C, name="qd_commutator_110")
+       import :: c_int, c_ptr, c_double
+       type(c_ptr), value :: ctx, r1, r2
+       real(c_double) :: c
+       integer(c_int) :: qd_commutator_110
+     end function
+     function qd_commutator_220(ctx, r1, r2, c) bind(C, name="qd_commutator_220")
+       import :: c_int, c_ptr, c_double
+       type(c_ptr), value :: ctx, r1, r2
+       real(c_double) :: c
+       integer(c_int) :: qd_commutator_220
+     end function
+     function qd_commutator_111(ctx, r1, r2, r3) bind(C, name="qd_commutator_111")
+       import :: c_int, c_ptr
+       type(c_ptr), value :: ctx, r1, r2, r3
+       integer(c_int) :: qd_commutator_111
+     end function
+     function qd_commutator_121(ctx, r1, r2, r3) bind(C, name="qd_commutator_121")
+       import :: c_int, c_ptr
+       type(c_ptr), value :: ctx, r1, r2, r3
+       integer(c_int) :: qd_commutator_121
+     end function
+     function qd_commutator_221(ctx, r1, r2, r3) bind(C, name="qd_commutator_221")
+       import :: c_int, c_ptr
+       type(c_ptr), value :: ctx, r1, r2, r3
+       integer(c_int) :: qd_commutator_221
+     end function
+     function qd_commutator_122(ctx, r1, r2, r3) bind(C, name="qd_commutator_122")
+       import :: c_int, c_ptr
+       type(c_ptr), value :: ctx, r1, r2, r3
+       integer(c_int) :: qd_commutator_122
+     end function
+     function qd_commutator_222(ctx, r1, r2, r3, hcc, etacc) bind(C, name="qd_commutator_222")
+       import :: c_int, c_ptr
+       type(c_ptr), value :: ctx, r1, r2, r3, hcc, etacc
+       integer(c_int) :: qd_commutator_222
+     end function
+     function qd_calc_cc(ctx, h, hcc) bind(C, name="qd_calc_cc")
+       import :: c_int, c_ptr
+       type(c_ptr), value :: ctx, h, hcc
+       integer(c_int) :: qd_calc_cc
+     end function
+     function qd_build_white(ctx, gen, rec) bind(C, name="qd_build_white")
+       import :: c_int, c_ptr
+       type(c_ptr), value :: ctx, gen, rec
+       integer(c_int) :: qd_build_white
+     end function
+     function qd_add_arrays(ctx, r1, s1, r2, s2, r3) bind(C, name="qd_add_arrays")
+       import :: c_int, c_ptr, c_double
+       type(c_ptr), value :: ctx, r1, r2, r3
+       real(c_double), value :: s1, s2
+       integer(c_int) :: qd_add_arrays
+     end function
+     function qd_copy_arrays(ctx, r1, r2) bind(C, name="qd_copy_arrays")
+       import :: c_int, c_ptr
+       type(c_ptr), value :: ctx, r1, r2
+       integer(c_int) :: qd_copy_arrays
+     end function
+     function qd_mat_2_norm(ctx, r, x) bind(C, name="qd_mat_2_norm")
+       import :: c_int, c_ptr, c_double
+       type(c_ptr), value :: ctx, r
+       real(c_double) :: x
+       integer(c_int) :: qd_mat_2_norm
+     end function
+     function qd_mbpt2(ctx, h, x) bind(C, name="qd_mbpt2")
+       import :: c_int, c_ptr, c_double
+       type(c_ptr), value :: ctx, h
+       real(c_double) :: x
+       integer(c_int) :: qd_mbpt2
+     end function
+     ! ---- flow level (src/magnus_IMSRG.f90, src/gs_IMSRG.f90) ----
+     function qd_bch_expand(ctx, hs, g, h, hcc, occ, nterms) bind(C, name="qd_bch_expand")
+       import :: c_int, c_ptr
+       type(c_ptr), value :: ctx, hs, g, h, hcc, occ
+       integer(c_int) :: nterms
+       integer(c_int) :: qd_bch_expand
+     end function
+     function qd_euler_step(ctx, g, dg, s, stp) bind(C, name="qd_euler_step")
+       import :: c_int, c_ptr, c_double
+       type(c_ptr), value :: ctx, g, dg
+       real(c_double) :: s
+       real(c_double), value :: stp
+       integer(c_int) :: qd_euler_step
+     end function
+     function qd_dgam(ctx, h, der, hcc, etacc) bind(C, name="qd_dgam")
+       import :: c_int, c_ptr
+       type(c_ptr), value :: ctx, h, der, hcc, etacc
+       integer(c_int) :: qd_dgam
+     end function
+     subroutine qd_flow_params_default(p) bind(C, name="qd_flow_params_default")
+       import :: qd_flow_params
+       type(qd_flow_params) :: p
+     end subroutine
+     function qd_magnus_flow(ctx, h0, hs, omega, p, res, trace, trace_len) bind(C, name="qd_magnus_flow")
+       import :: c_int, c_ptr, c_double, qd_flow_params, qd_flow_result
+       type(c_ptr), value :: ctx, h0, hs, omega
+       type(qd_flow_params) :: p
+       type(qd_flow_result) :: res
+       real(c_double) :: trace(*)
+       integer(c_int), value :: trace_len
+       integer(c_int) :: qd_magnus_flow
+     end function
+     ! ---- input side ----
+     function qd_hf_hamiltonian(nbody, nshell, hw, h0, neq, eig, iters) bind(C, name="qd_hf_hamiltonian")
+       import :: c_int, c_double, c_int64_t
+       integer(c_int), value :: nbody, nshell
+       real(c_double), value :: hw
+       real(c_double) :: h0(*), eig(*)
+       integer(c_int64_t), value :: neq
+       integer(c_int) :: iters
+       integer(c_int) :: qd_hf_hamiltonian
+     end function
+     function qd_last_error() bind(C, name="qd_last_error")
+       import :: c_ptr
+       type(c_ptr) :: qd_last_error
+     end function
+  end interface
+
+contains
+
+  subroutine qd_check(rc, what)
+    integer(c_int), intent(in) :: rc
+    character(*), intent(in) :: what
+    if (rc /= QD_OK) then
+       print *, 'qdimsrg error in ', what, ': code ', rc
+       stop 1
+    end if
+  end subroutine qd_check
+
+end module qdimsrg_c
