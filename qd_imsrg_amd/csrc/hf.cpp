@@ -11,6 +11,7 @@
 #include <cmath>
 #include <cstring>
 #include <map>
+#include <mutex>
 #include <unordered_map>
 #include <vector>
 
@@ -141,34 +142,73 @@ void jacobi(int n, std::vector<double> a, std::vector<double> &w, std::vector<do
   v.swap(vs);
 }
 
-struct Spatial {  // <ab|V|cd> for m_a + m_b = m_c + m_d, stored by (a, b, c, d)
-  int N = 0;
+// <ab|V|cd> at hw = 1 for m_a + m_b = m_c + m_d: for each (a, b, c) the d run
+// over the orbitals with m_d = m_a + m_b - m_c (compressed, N^3 x ~R/2)
+struct Spatial {
+  int N = 0, R = 0, mmax = 0;
   std::vector<int> n, m;
-  std::map<int, std::vector<int>> by_m;
-  std::vector<double> tab;  // N^4 (small R only)
+  std::vector<std::vector<int>> by_m;   // orbitals with m = mm - mmax
+  std::vector<int> pos_in_m;            // position of an orbital in its m list
+  std::vector<int64_t> start;           // N^3
+  std::vector<double> tab;
   double get(int a, int b, int c, int d) const {
-    return tab[(((size_t)a * N + b) * N + c) * N + d];
+    const int md = m[a] + m[b] - m[c];
+    if (md != m[d]) return 0.0;
+    return tab[start[((size_t)a * N + b) * N + c] + pos_in_m[d]];
   }
 };
 
-int build_spatial(int nshell, double hw, Spatial &S) {
+void build_spatial(int nshell, Spatial &S) {
+  S.R = nshell;
   for (int R = 0; R < nshell; R++)
     for (int m = -R; m <= R; m += 2) { S.n.push_back((R - std::abs(m)) / 2); S.m.push_back(m); }
-  S.N = (int)S.n.size();
-  const int N = S.N;
-  if ((double)N * N * N * N > 2.0e9 / 8) return -1;
-  S.tab.assign((size_t)N * N * N * N, 0.0);
+  const int N = S.N = (int)S.n.size();
+  S.mmax = nshell - 1;
+  S.by_m.assign(2 * S.mmax + 1, {});
+  S.pos_in_m.assign(N, 0);
+  for (int a = 0; a < N; a++) {
+    S.pos_in_m[a] = (int)S.by_m[S.m[a] + S.mmax].size();
+    S.by_m[S.m[a] + S.mmax].push_back(a);
+  }
+  S.start.assign((size_t)N * N * N, 0);
+  int64_t k = 0;
   for (int a = 0; a < N; a++)
     for (int b = 0; b < N; b++)
-      for (int c = 0; c < N; c++)
-        for (int d = 0; d < N; d++) {
-          if (S.m[a] + S.m[b] != S.m[c] + S.m[d]) continue;
+      for (int c = 0; c < N; c++) {
+        S.start[((size_t)a * N + b) * N + c] = k;
+        const int md = S.m[a] + S.m[b] - S.m[c];
+        if (md >= -S.mmax && md <= S.mmax) k += (int64_t)S.by_m[md + S.mmax].size();
+      }
+  S.tab.assign(k, 0.0);
+#pragma omp parallel for schedule(dynamic, 1) collapse(2)
+  for (int a = 0; a < N; a++)
+    for (int b = 0; b < N; b++)
+      for (int c = 0; c < N; c++) {
+        const int md = S.m[a] + S.m[b] - S.m[c];
+        if (md < -S.mmax || md > S.mmax) continue;
+        const std::vector<int> &ds = S.by_m[md + S.mmax];
+        for (size_t kd = 0; kd < ds.size(); kd++) {
+          const int d = ds[kd];
           // <ab|V|cd> = v_int(a, b, d, c) with equal spins
           const int qa[3] = {S.n[a], S.m[a], 1}, qb[3] = {S.n[b], S.m[b], 1};
           const int qc[3] = {S.n[c], S.m[c], 1}, qd[3] = {S.n[d], S.m[d], 1};
-          S.tab[(((size_t)a * N + b) * N + c) * N + d] = v_int_impl(qa, qb, qd, qc, hw);
+          S.tab[S.start[((size_t)a * N + b) * N + c] + kd] = v_int_impl(qa, qb, qd, qc, 1.0);
         }
-  return 0;
+      }
+}
+
+// one table per shell count per process (hw enters as sqrt(hw), applied on use)
+const Spatial &spatial_table(int nshell) {
+  static std::mutex mu;
+  static std::map<int, Spatial> cache;
+  std::lock_guard<std::mutex> lock(mu);
+  auto it = cache.find(nshell);
+  if (it == cache.end()) {
+    Spatial S;
+    build_spatial(nshell, S);
+    it = cache.emplace(nshell, std::move(S)).first;
+  }
+  return it->second;
 }
 
 }  // namespace
@@ -188,8 +228,8 @@ int qd_hf_hamiltonian(int nbody, int nshell, double hw, double *h0, int64_t neq,
   if (!why.empty() || !h0) return QD_E_LAYOUT;
   if (neq != ch.neq) return QD_E_ARG;
   const int M = ch.M, n = ch.n;
-  Spatial S;
-  if (build_spatial(nshell, hw, S)) return QD_E_LAYOUT;
+  const Spatial &S = spatial_table(nshell);
+  const double shw = std::sqrt(hw);
   // spin-orbital -> spatial orbital (nm_index order), spin
   std::vector<int> sp(M), spin(M);
   {
@@ -204,7 +244,7 @@ int qd_hf_hamiltonian(int nbody, int nshell, double hw, double *h0, int64_t neq,
     double v = 0;
     if (spin[p] == spin[r] && spin[q] == spin[s]) v += S.get(sp[p], sp[q], sp[r], sp[s]);
     if (spin[p] == spin[s] && spin[q] == spin[r]) v -= S.get(sp[p], sp[q], sp[s], sp[r]);
-    return v;
+    return shw * v;
   };
   // single-particle channels
   std::vector<std::vector<int>> chan(ch.nch);
@@ -229,9 +269,10 @@ int qd_hf_hamiltonian(int nbody, int nshell, double hw, double *h0, int64_t neq,
           rho[(size_t)l * M + j] = s;
         }
     std::fill(F.begin(), F.end(), 0.0);
-    for (auto &ci : chan)
-      for (int i : ci)
-        for (int k : ci) {
+#pragma omp parallel for schedule(dynamic, 1)
+    for (int cc = 0; cc < (int)chan.size(); cc++)
+      for (int i : chan[cc])
+        for (int k : chan[cc]) {
           double v = 0;
           for (auto &cj : chan)
             for (int j : cj)
@@ -281,7 +322,7 @@ int qd_hf_hamiltonian(int nbody, int nshell, double hw, double *h0, int64_t neq,
   for (int j = 0; j < n; j++)
     for (int i = 0; i < mp; i++) h0[ch.off_fph + i + (int64_t)j * mp] = FHF[(size_t)(n + i) * M + j];
   // two-body: per block HFV = T V_HO T^T over the block's pairs (hh | ph | pp)
-  double vsum = 0;
+#pragma omp parallel for schedule(dynamic, 1)
   for (int q = 0; q < ch.nblock; q++) {
     const int nh = ch.nh[q], nb = ch.nb[q], np = ch.np[q], nt = nh + nb + np;
     if (nt == 0) continue;
@@ -320,8 +361,10 @@ int qd_hf_hamiltonian(int nbody, int nshell, double hw, double *h0, int64_t neq,
     put(PHPH, x, nb, x, nb);
     put(PPPH, p, np, x, nb);
     put(PHHH, x, nb, h, nh);
-    for (int j = 0; j < nh; j++) vsum += H[(size_t)j * nt + j];
   }
+  double vsum = 0;  // sum over blocks of the Vhhhh diagonal, in block order
+  for (int q = 0; q < ch.nblock; q++)
+    for (int j = 0; j < ch.nh[q]; j++) vsum += h0[ch.off[(size_t)q * 6 + HHHH] + j + (int64_t)j * ch.nh[q]];
   double eh = 0;
   for (int i = 0; i < n; i++) eh += eig[i];
   h0[0] = eh - vsum;  // HF_E2 (src/HFQDmod.f90:292-323)
