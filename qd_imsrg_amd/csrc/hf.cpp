@@ -9,6 +9,8 @@
 // Eigenvector phases: largest-magnitude component positive.
 #include <algorithm>
 #include <cmath>
+#include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <map>
 #include <mutex>
@@ -40,6 +42,29 @@ ld gamma_half(int twice) {
   return fact(2 * xc) / (std::pow((ld)4, xc) * fact(xc)) * 1.772453850905516027298167483341145182L;
 }
 
+// The inner sum cancels heavily for large g; it is accumulated in binary128
+// (exact integer binomials and factorials up to ~2^113) and memoized per tuple.
+typedef __float128 qf;
+qf qbinom(int n, int k) {
+  if (k < 0 || k > n) return 0;
+  qf r = 1;
+  for (int i = 1; i <= k; i++) r = r * (n - k + i) / i;
+  return r;
+}
+qf qfact(int n) {
+  qf r = 1;
+  for (int i = 2; i <= n; i++) r *= i;
+  return r;
+}
+// Gamma(x/2) / sqrt(pi)^[x odd] as an exact binary128 rational value
+qf qgamma_half_rational(int twice) {
+  if (twice % 2 == 0) return qfact(twice / 2 - 1);
+  const int xc = (twice - 1) / 2;
+  qf p4 = 1;
+  for (int i = 0; i < xc; i++) p4 *= 4;
+  return qfact(2 * xc) / (p4 * qfact(xc));
+}
+
 struct InnerCache {
   std::unordered_map<uint64_t, ld> memo;
   ld get(int g1, int g2, int g3, int g4) {
@@ -47,19 +72,23 @@ struct InnerCache {
     auto it = memo.find(key);
     if (it != memo.end()) return it->second;
     const int G = g1 + g2 + g3 + g4;
-    ld s = 0;
+    // lam = 2(l1+l2) is even: Gamma(1+lam/2) is an integer and
+    // Gamma((G-lam+1)/2) carries sqrt(pi) exactly when G is even
+    qf s = 0;
     for (int l1 = 0; l1 <= g1; l1++)
       for (int l2 = 0; l2 <= g2; l2++)
         for (int l3 = 0; l3 <= g3; l3++) {
           const int l4 = l1 + l2 - l3;
           if (l4 < 0 || l4 > g4) continue;
           const int lam = l1 + l2 + l3 + l4;
-          const ld sg = ((g2 + g3 - l2 - l3) & 1) ? -1 : 1;
-          s += sg * binom(g1, l1) * binom(g2, l2) * binom(g3, l3) * binom(g4, l4) * gamma_half(2 + lam) *
-               gamma_half(G - lam + 1);
+          const qf term = qbinom(g1, l1) * qbinom(g2, l2) * qbinom(g3, l3) * qbinom(g4, l4) *
+                          qgamma_half_rational(2 + lam) * qgamma_half_rational(G - lam + 1);
+          if ((g2 + g3 - l2 - l3) & 1) s -= term; else s += term;
         }
-    memo.emplace(key, s);
-    return s;
+    ld r = (ld)s;
+    if ((G & 1) == 0) r *= 1.772453850905516027298167483341145182L;
+    memo.emplace(key, r);
+    return r;
   }
 };
 
@@ -255,10 +284,27 @@ int qd_hf_hamiltonian(int nbody, int nshell, double hw, double *h0, int64_t neq,
     Tm[i] = hw * (R + 1);
     C[(size_t)i * M + i] = 1.0;  // C[ho*M + hf]
   }
+  // Roothaan iteration as in the reference; if it has not converged after
+  // 100 iterations (it can settle into a 2-cycle, e.g. N=20, hw=0.5, R>=10,
+  // where the reference's loop never ends) restart from the oscillator basis
+  // with the density mixed half/half with the previous one, which has the
+  // same fixed point.
   double crit = 1.0;
-  int it = 0;
-  while (crit > 1e-6 && it < 1000) {
+  int it = 0, total = 0;
+  double mix = 0.0;
+  std::vector<double> rho_old;
+  while (crit > 1e-6) {
+    if (it == 100 && mix == 0.0) {
+      mix = 0.5;
+      it = 0;
+      rho_old.clear();
+      std::fill(C.begin(), C.end(), 0.0);
+      for (int i = 0; i < M; i++) C[(size_t)i * M + i] = 1.0;
+      std::fill(eig.begin(), eig.end(), 0.0);
+    }
+    if (it >= 2000) break;
     it++;
+    total++;
     // rho_lj = sum_{h<n} C[l,h] C[j,h] (block-diagonal)
     std::vector<double> rho((size_t)M * M, 0.0);
     for (auto &cs : chan)
@@ -268,6 +314,11 @@ int qd_hf_hamiltonian(int nbody, int nshell, double hw, double *h0, int64_t neq,
           for (int h = 0; h < n; h++) s += C[(size_t)l * M + h] * C[(size_t)j * M + h];
           rho[(size_t)l * M + j] = s;
         }
+    if (mix > 0.0) {
+      if (!rho_old.empty())
+        for (size_t e = 0; e < rho.size(); e++) rho[e] = (1.0 - mix) * rho[e] + mix * rho_old[e];
+      rho_old = rho;
+    }
     std::fill(F.begin(), F.end(), 0.0);
 #pragma omp parallel for schedule(dynamic, 1)
     for (int cc = 0; cc < (int)chan.size(); cc++)
@@ -297,6 +348,11 @@ int qd_hf_hamiltonian(int nbody, int nshell, double hw, double *h0, int64_t neq,
     }
     crit = 0;
     for (int i = 0; i < n; i++) crit += std::fabs(eold[i] - eig[i]);
+    if (std::getenv("QD_HF_TRACE")) {
+      double e = 0;
+      for (int i = 0; i < n; i++) e += eig[i];
+      fprintf(stderr, "hf iter %d crit %.3e sum_eocc %.10f\n", it, crit, e);
+    }
   }
   // F_HF = C^T F C (F from the last Fock build)
   std::vector<double> FC((size_t)M * M, 0.0), FHF((size_t)M * M, 0.0);
@@ -369,7 +425,8 @@ int qd_hf_hamiltonian(int nbody, int nshell, double hw, double *h0, int64_t neq,
   for (int i = 0; i < n; i++) eh += eig[i];
   h0[0] = eh - vsum;  // HF_E2 (src/HFQDmod.f90:292-323)
   if (eig_out) std::memcpy(eig_out, eig.data(), sizeof(double) * M);
-  if (iters) *iters = it;
+  if (iters) *iters = total;
+  if (crit > 1e-6) return QD_E_NUMERIC;
   return QD_OK;
 }
 
