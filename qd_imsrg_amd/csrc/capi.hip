@@ -903,6 +903,7 @@ int qd_magnus_flow(qd_context *c, const qd_op *h0, qd_op *hs, qd_op *omega, cons
   res->bch_terms = terms;
   res->converged = crit <= p->conv;
   res->trace_rows = rows;
+  if (p->verbose) fflush(stdout);
   return done(QD_OK);
 }
 
