@@ -190,6 +190,26 @@ void qd_flow_params_default(qd_flow_params *p);
 int qd_magnus_flow(qd_context *ctx, const qd_op *h0, qd_op *hs, qd_op *omega,
                    const qd_flow_params *p, qd_flow_result *res, double *trace, int trace_len);
 
+/* ---- batched form: launch strategy of the grouped GEMM ------------------
+ * The GEMM problems of all (M_L, M_S) blocks run as one grouped persistent
+ * launch (mode 0, default) or as one launch per block problem, the way the
+ * reference calls dgemm per block (mode 1).  No counterpart in the reference:
+ * this is the batching control of the MI355X engine. */
+int qd_set_launch_mode(qd_context *ctx, int mode);
+/* 1: qd_bch_expand captures each nested-commutator term (calc_cc of the
+ * term, the fused commutator, both norms) in a hipGraph on first use and
+ * replays it; graphs are keyed by the operand addresses.  0 (default): plain
+ * stream launches.  No counterpart in the reference. */
+int qd_set_graphs(qd_context *ctx, int on);
+/* Project autotuner: for every (M, N, K) size bucket of the grouped GEMM
+ * problems, times each tile shape (64x64, 128x64, 32x32, 128x16, 64x16,
+ * 16x64, 16x128) on scratch operands and keeps the fastest.  Reports the
+ * device time of one commutator's GEMM launches before and after (ms). */
+int qd_autotune(qd_context *ctx, int reps, double *ms_before, double *ms_after);
+/* Current tile shape of every GEMM problem (products, then accumulating);
+ * returns the count. */
+int qd_gemm_shapes(const qd_context *ctx, int *shapes, int64_t len);
+
 /* ---- input side (host FP64, no GPU) --------------------------------------
  * v_int(qi,qj,qk,ql,hw), src/ME_general.f90:2205-2276: the analytic 2D
  * oscillator Coulomb element <i j|V|l k> for (n, m_l, m_s) triples. */

@@ -74,6 +74,25 @@ class Context:
         check(self.L.qd_gemm_flops(self.h, C.byref(a), C.byref(b), C.byref(c)))
         return dict(products=a.value, pandya=b.value, accumulate=c.value)
 
+    def set_launch_mode(self, mode: int):
+        """0: grouped persistent GEMM launch; 1: one launch per block problem."""
+        check(self.L.qd_set_launch_mode(self.h, int(mode)))
+
+    def set_graphs(self, on: bool):
+        """Capture each BCH term in a hipGraph and replay it."""
+        check(self.L.qd_set_graphs(self.h, int(bool(on))))
+
+    def autotune(self, reps: int = 3):
+        a, b = C.c_double(), C.c_double()
+        check(self.L.qd_autotune(self.h, int(reps), C.byref(a), C.byref(b)))
+        return a.value, b.value
+
+    def gemm_shapes(self) -> np.ndarray:
+        n = self.L.qd_gemm_shapes(self.h, None, 0)
+        out = np.zeros(max(n, 1), dtype=np.int32)
+        self.L.qd_gemm_shapes(self.h, out.ctypes.data_as(_lib.ip), n)
+        return out[:n]
+
     def close(self):
         if getattr(self, "h", None):
             self.L.qd_context_destroy(self.h)

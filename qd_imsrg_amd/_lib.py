@@ -78,6 +78,10 @@ SIGNATURES = {
     "qd_timing_enable": (C.c_int, [vp, C.c_int]),
     "qd_timing_read": (C.c_int, [vp, dp, i64p, C.c_int]),
     "qd_gemm_flops": (C.c_int, [vp, dp, dp, dp]),
+    "qd_set_launch_mode": (C.c_int, [vp, C.c_int]),
+    "qd_set_graphs": (C.c_int, [vp, C.c_int]),
+    "qd_autotune": (C.c_int, [vp, C.c_int, dp, dp]),
+    "qd_gemm_shapes": (C.c_int, [vp, ip, C.c_int64]),
     "qd_v_int": (C.c_double, [ip, ip, ip, ip, C.c_double]),
     "qd_hf_hamiltonian": (C.c_int, [C.c_int, C.c_int, C.c_double, dp, C.c_int64, dp, ip]),
 }

@@ -5,6 +5,8 @@
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
+#include <map>
+#include <tuple>
 #include <string>
 #include <vector>
 
@@ -44,12 +46,20 @@ const double R4_1EM3 = 0.0010000000474974513;   // + 1e-3, src/magnus_IMSRG.f90:
 constexpr int NCLASS = 6;
 enum { TC_PRODUCTS = 0, TC_ACCUM = 1, TC_TWOBODY = 2, TC_ONEBODY = 3, TC_CALCCC = 4, TC_OTHER = 5 };
 
+// One grouped-GEMM work list.  Problems [0, npre) are the ladder products;
+// [npre, nprob) the Pandya products (empty for the accumulating list).
 struct GemmList {
+  std::vector<GemmProb> hprobs;
+  std::vector<GemmSeg> hsegs;
   GemmProb *probs = nullptr;
   GemmSeg *segs = nullptr;
-  GemmTile *tiles = nullptr;
-  int ntiles = 0;
-  double flops = 0;  // algorithmic 2*M*N*K summed over segments
+  int npre = 0;
+  GemmTile *tiles_pre = nullptr, *tiles_all = nullptr;  // sorted by decreasing work
+  int ntiles_pre = 0, ntiles_all = 0;
+  GemmTile *ptiles = nullptr;                            // grouped per problem
+  std::vector<int> ptile_off;
+  size_t cap = 0;                                        // capacity of each tile buffer
+  double flops = 0, flops_pre = 0;                       // algorithmic 2*M*N*K
 };
 }  // namespace
 
@@ -66,9 +76,20 @@ struct qd_context {
   TilePair *ptp = nullptr; int nptp = 0;  // tile pairs of the Pandya products
   Seg2 *pphh_segs = nullptr; Tile2 *pphh_tiles = nullptr; int npphh_segs = 0, npphh_tiles = 0;
   Seg2 *cc_segs = nullptr; Tile2 *cc_tiles = nullptr; int ncc_tiles = 0;
-  GemmList g_all, g_acc;  // products (G1 + Pandya, Pandya tiles last) and accumulating products
-  int g1_ntiles = 0;      // leading G1-only tiles of g_all
-  double pandya_flops = 0;
+  GemmList g_all, g_acc;  // products (ladder + Pandya) and accumulating products
+  int launch_mode = 0;    // 0 grouped persistent launch, 1 one launch per channel problem
+  bool use_graphs = false;  // capture each BCH term (commutator + norms) in a hipGraph
+  struct GraphKey {
+    const void *g, *prev, *next, *hs, *cca, *ccg;
+    int hg, pd;
+    double hh, eh;
+    bool operator<(const GraphKey &o) const {
+      return std::tie(g, prev, next, hs, cca, ccg, hg, pd, hh, eh) <
+             std::tie(o.g, o.prev, o.next, o.hs, o.cca, o.ccg, o.hg, o.pd, o.hh, o.eh);
+    }
+  };
+  std::map<GraphKey, hipGraphExec_t> graphs;
+  int gemm_grid = 1024;   // persistent grid of the grouped launch
   // scratch operators and CC sets of the flow drivers (created on first use)
   qd_op *bch_a = nullptr, *bch_b = nullptr;
   qd_cc *cc_t1 = nullptr, *cc_t2 = nullptr;
@@ -140,12 +161,11 @@ void add_tiles(std::vector<Tile2> &tiles, int seg, int size) {
 struct GemmBuilder {
   std::vector<GemmProb> probs;
   std::vector<GemmSeg> segs;
-  std::vector<GemmTile> tiles;
   double flops = 0;
   void begin(int c_slot, int64_t c_off, int ldc, int M, int N, int beta1) {
     GemmProb p{};
     p.c_off = c_off; p.c_slot = c_slot; p.ldc = std::max(ldc, 1); p.M = M; p.N = N; p.beta1 = beta1;
-    p.seg0 = (int)segs.size(); p.nseg = 0;
+    p.seg0 = (int)segs.size(); p.nseg = 0; p.shape = SH_64x64;
     probs.push_back(p);
   }
   void seg(int a_slot, int64_t a_off, int lda, int ta, int b_slot, int64_t b_off, int ldb, int tb, int K,
@@ -159,21 +179,82 @@ struct GemmBuilder {
   }
   void end() {
     const GemmProb &p = probs.back();
-    if (p.M <= 0 || p.N <= 0) { segs.resize(p.seg0); probs.pop_back(); return; }
-    int pi = (int)probs.size() - 1;
-    for (int tm = 0; tm * GEMM_TM < p.M; tm++)
-      for (int tn = 0; tn * GEMM_TN < p.N; tn++) tiles.push_back({pi, tm, tn});
+    if (p.M <= 0 || p.N <= 0) { segs.resize(p.seg0); probs.pop_back(); }
   }
 };
 
-int upload_gemm(qd_context *c, GemmBuilder &b, GemmList &g) {
-  int rc;
-  if ((rc = dev_upload(c, b.probs, &g.probs))) return rc;
-  if ((rc = dev_upload(c, b.segs, &g.segs))) return rc;
-  if ((rc = dev_upload(c, b.tiles, &g.tiles))) return rc;
-  g.ntiles = (int)b.tiles.size();
-  g.flops = b.flops;
+int ktot(const GemmList &g, const GemmProb &p) {
+  int k = 0;
+  for (int i = 0; i < p.nseg; i++) k += g.hsegs[p.seg0 + i].K;
+  return k;
+}
+
+// size-driven default before autotuning: skinny shapes for hole-pair sides
+int default_shape(int M, int N) {
+  if (N <= 16 && M > 16) return M >= 128 ? SH_128x16 : SH_64x16;
+  if (M <= 16 && N > 16) return N >= 128 ? SH_16x128 : SH_16x64;
+  if (M <= 32 && N <= 32) return SH_32x32;
+  if (M >= 256 && N >= 64) return SH_128x64;
+  return SH_64x64;
+}
+
+void tiles_of(const GemmList &g, int pi, std::vector<GemmTile> &out) {
+  const GemmProb &p = g.hprobs[pi];
+  const int TM = SHAPE_TM[p.shape], TN = SHAPE_TN[p.shape];
+  for (int tm = 0; tm * TM < p.M; tm++)
+    for (int tn = 0; tn * TN < p.N; tn++) out.push_back({pi, tm, tn});
+}
+
+// (re)builds the three tile lists from the problems' current shapes
+int upload_tiles(qd_context *c, GemmList &g) {
+  const int np = (int)g.hprobs.size();
+  auto work = [&](const GemmTile &t) {
+    const GemmProb &p = g.hprobs[t.prob];
+    return (double)SHAPE_TM[p.shape] * SHAPE_TN[p.shape] * (ktot(g, p) + GEMM_TK);
+  };
+  std::vector<GemmTile> pre, all, grouped;
+  g.ptile_off.assign(np + 1, 0);
+  for (int pi = 0; pi < np; pi++) {
+    g.ptile_off[pi] = (int)grouped.size();
+    tiles_of(g, pi, grouped);
+  }
+  g.ptile_off[np] = (int)grouped.size();
+  all = grouped;
+  for (int pi = 0; pi < g.npre; pi++) tiles_of(g, pi, pre);
+  auto bywork = [&](const GemmTile &a, const GemmTile &b) { return work(a) > work(b); };
+  std::stable_sort(pre.begin(), pre.end(), bywork);
+  std::stable_sort(all.begin(), all.end(), bywork);
+  if (!g.tiles_all) {  // capacity: every shape covers at least 16 x 64 outputs
+    size_t cap = 0;
+    for (auto &p : g.hprobs) cap += (size_t)((p.M + 15) / 16) * ((p.N + 15) / 16);
+    g.cap = std::max<size_t>(cap, 1);
+    void *q;
+    int rc;
+    if ((rc = dev_alloc(c, sizeof(GemmTile) * g.cap, &q))) return rc; g.tiles_all = (GemmTile *)q;
+    if ((rc = dev_alloc(c, sizeof(GemmTile) * g.cap, &q))) return rc; g.tiles_pre = (GemmTile *)q;
+    if ((rc = dev_alloc(c, sizeof(GemmTile) * g.cap, &q))) return rc; g.ptiles = (GemmTile *)q;
+  }
+  if (all.size() > g.cap) return fail(QD_E_LAYOUT, "tile list exceeds its capacity");
+  HIPC(hipMemcpy(g.tiles_all, all.data(), sizeof(GemmTile) * all.size(), hipMemcpyHostToDevice));
+  HIPC(hipMemcpy(g.tiles_pre, pre.data(), sizeof(GemmTile) * pre.size(), hipMemcpyHostToDevice));
+  HIPC(hipMemcpy(g.ptiles, grouped.data(), sizeof(GemmTile) * grouped.size(), hipMemcpyHostToDevice));
+  HIPC(hipMemcpy(g.probs, g.hprobs.data(), sizeof(GemmProb) * np, hipMemcpyHostToDevice));
+  g.ntiles_all = (int)all.size();
+  g.ntiles_pre = (int)pre.size();
   return QD_OK;
+}
+
+int upload_gemm(qd_context *c, GemmBuilder &b, int npre, double flops_pre, GemmList &g) {
+  int rc;
+  g.hprobs = b.probs;
+  g.hsegs = b.segs;
+  g.npre = npre;
+  g.flops = b.flops;
+  g.flops_pre = flops_pre;
+  for (auto &p : g.hprobs) p.shape = default_shape(p.M, p.N);
+  if ((rc = dev_upload(c, g.hprobs, &g.probs))) return rc;
+  if ((rc = dev_upload(c, g.hsegs, &g.segs))) return rc;
+  return upload_tiles(c, g);
 }
 
 int build_worklists(qd_context *c) {
@@ -284,8 +365,8 @@ int build_worklists(qd_context *c) {
     g2.seg(S_R2, o[PPPP], np, 0, S_R1, o[PPPH], np, 0, np, -1);
     g2.end();
   }
-  c->g1_ntiles = (int)g1.tiles.size();
-  double f1 = g1.flops;
+  const int npre = (int)g1.probs.size();
+  const double f1 = g1.flops;
   // Pandya: P(q) = ETACC(q) HCC(q)^T (r x r, K = nb); appended after G1
   for (int q = 0; q < ch.ncc; q++) {
     const int r = ch.cc_r[q], nb = ch.cc_nb[q];
@@ -294,7 +375,7 @@ int build_worklists(qd_context *c) {
     g1.seg(S_CC1, ch.cc_off[q], r, 0, S_CC2, ch.cc_off[q], r, 1, nb, 1);
     g1.end();
   }
-  c->pandya_flops = g1.flops - f1;
+
   {
     std::vector<TilePair> tp;
     for (int q = 0; q < ch.ncc; q++) {
@@ -307,8 +388,8 @@ int build_worklists(qd_context *c) {
     if ((rc = dev_upload(c, tp, &c->ptp))) return rc;
     c->nptp = (int)tp.size();
   }
-  if ((rc = upload_gemm(c, g1, c->g_all))) return rc;
-  if ((rc = upload_gemm(c, g2, c->g_acc))) return rc;
+  if ((rc = upload_gemm(c, g1, npre, f1, c->g_all))) return rc;
+  if ((rc = upload_gemm(c, g2, (int)g2.probs.size(), g2.flops, c->g_acc))) return rc;
   return QD_OK;
 }
 
@@ -335,14 +416,31 @@ bool pandya_on(const qd_cc *hcc, const qd_cc *ecc) {
   return hcc && ecc && !hcc->zero && !ecc->zero;
 }
 
+// One grouped persistent launch over the sorted tile list, or (per-channel
+// mode) one launch per problem, the way the reference calls dgemm per block.
+int run_gemm_list(qd_context *c, const GemmList &g, bool with_pandya, const Slots &s) {
+  if (c->launch_mode == 0) {
+    if (with_pandya) launch_gemm(c->stream, g.probs, g.segs, g.tiles_all, g.ntiles_all, s, c->gemm_grid);
+    else launch_gemm(c->stream, g.probs, g.segs, g.tiles_pre, g.ntiles_pre, s, c->gemm_grid);
+  } else {
+    const int np = with_pandya ? (int)g.hprobs.size() : g.npre;
+    for (int pi = 0; pi < np; pi++) {
+      const int n = g.ptile_off[pi + 1] - g.ptile_off[pi];
+      launch_gemm(c->stream, g.probs, g.segs, g.ptiles + g.ptile_off[pi], n, s, 0);
+    }
+  }
+  HIPC(hipGetLastError());
+  return QD_OK;
+}
+
 // products phase: G1 (+ Pandya when both CC sets are non-zero)
 int run_products(qd_context *c, const qd_op *r1, const qd_op *r2, qd_op *out, const qd_cc *hcc, const qd_cc *ecc) {
   const bool pd = pandya_on(hcc, ecc);
   Slots s = make_slots(c, r1, r2, out, ecc, hcc);
   {
     Timer tm(c, TC_PRODUCTS);
-    launch_gemm(c->stream, c->g_all.probs, c->g_all.segs, c->g_all.tiles, pd ? c->g_all.ntiles : c->g1_ntiles, s);
-    HIPC(hipGetLastError());
+    int rc = run_gemm_list(c, c->g_all, pd, s);
+    if (rc) return rc;
   }
   if (pd) {
     Timer tc(c, TC_OTHER);
@@ -355,9 +453,7 @@ int run_products(qd_context *c, const qd_op *r1, const qd_op *r2, qd_op *out, co
 int run_acc(qd_context *c, const qd_op *r1, const qd_op *r2, qd_op *out) {
   Slots s = make_slots(c, r1, r2, out, nullptr, nullptr);
   Timer tm(c, TC_ACCUM);
-  launch_gemm(c->stream, c->g_acc.probs, c->g_acc.segs, c->g_acc.tiles, c->g_acc.ntiles, s);
-  HIPC(hipGetLastError());
-  return QD_OK;
+  return run_gemm_list(c, c->g_acc, false, s);
 }
 
 int run_twobody(qd_context *c, const qd_op *r1, const qd_op *r2, qd_op *out, int flags, const qd_cc *hcc,
@@ -455,6 +551,12 @@ int qd_context_create(int nbody, int nshell, qd_context **out) {
   int rc = QD_OK;
   auto bail = [&](int r) { qd_context_destroy(c); return r; };
   if (hipStreamCreate(&c->stream) != hipSuccess) { delete c; return fail(QD_E_HIP, "hipStreamCreate failed"); }
+  {
+    int dev = 0, ncu = 0;
+    if (hipGetDevice(&dev) == hipSuccess &&
+        hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev) == hipSuccess && ncu > 0)
+      c->gemm_grid = 4 * ncu;
+  }
   const Channels &ch = c->ch;
   DevTab &t = c->dt;
   t.n = ch.n; t.M = ch.M; t.mp = ch.mp; t.nblock = ch.nblock; t.ncc = ch.ncc; t.nch = ch.nch;
@@ -493,6 +595,7 @@ int qd_context_destroy(qd_context *c) {
   if (c->stream) hipStreamSynchronize(c->stream);
   if (c->bch_a) { qd_op_destroy(c->bch_a); qd_op_destroy(c->bch_b); qd_cc_destroy(c->cc_t1); qd_cc_destroy(c->cc_t2); }
   for (auto &e : c->events) { hipEventDestroy(e.a); hipEventDestroy(e.b); }
+  for (auto &kv : c->graphs) hipGraphExecDestroy(kv.second);
   for (void *p : c->allocs) hipFree(p);
   if (c->host_scal) hipHostFree(c->host_scal);
   if (c->stream) hipStreamDestroy(c->stream);
@@ -760,14 +863,39 @@ int qd_bch_expand(qd_context *c, qd_op *hs, const qd_op *g, const qd_op *h, cons
     next->herm = 1;
     qd_op adh = *prev;  // AD carries herm = +1 (src/magnus_IMSRG.f90:273-275)
     adh.herm = 1;
-    if (pd) {
-      if ((rc = qd_calc_cc(c, &adh, cca))) return rc;
+    // one term: [G, AD] into next, ||INT1|| (= HS before the update) and ||INT2||
+    auto term = [&]() -> int {
+      if (pd) {
+        Timer tc(c, TC_CALCCC);
+        launch_calc_cc(c->stream, c->dt, adh.d, adh.herm, cca->d, c->cc_segs, c->cc_tiles, c->ncc_tiles);
+        HIPC(hipGetLastError());
+      }
+      int r;
+      if ((r = full_commutator(c, g, &adh, next, pd ? cca : nullptr, pd ? ccg : nullptr))) return r;
+      if ((r = norm_into(c, hs, 0))) return r;
+      return norm_into(c, next, 1);
+    };
+    if (c->use_graphs && !c->timing) {
+      qd_context::GraphKey key{g->d, prev->d, next->d, hs->d, pd ? cca->d : nullptr, pd ? ccg->d : nullptr,
+                               g->herm, pd ? 1 : 0, pd ? cca->herm : 0.0, pd ? ccg->herm : 0.0};
+      auto it = c->graphs.find(key);
+      if (it == c->graphs.end()) {
+        hipGraph_t graph;
+        HIPC(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+        rc = term();
+        hipError_t ec = hipStreamEndCapture(c->stream, &graph);
+        if (rc) return rc;
+        if (ec != hipSuccess) return fail(QD_E_HIP, "hipStreamEndCapture: %s", hipGetErrorString(ec));
+        hipGraphExec_t exec;
+        HIPC(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0));
+        HIPC(hipGraphDestroy(graph));
+        it = c->graphs.emplace(key, exec).first;
+      }
+      HIPC(hipGraphLaunch(it->second, c->stream));
+    } else if ((rc = term())) {
+      return rc;
     }
-    if ((rc = full_commutator(c, g, &adh, next, pd ? cca : nullptr, pd ? ccg : nullptr))) return rc;
     taken++;
-    // ||INT1|| with INT1 = HS before the update, ||INT2||
-    if ((rc = norm_into(c, hs, 0))) return rc;
-    if ((rc = norm_into(c, next, 1))) return rc;
     {
       Timer tm(c, TC_OTHER);
       launch_axpby(c->stream, c->ch.neq, 1.0, hs->d, BCH_COF[k - 2], next->d, hs->d);
@@ -936,10 +1064,127 @@ int qd_timing_read(qd_context *c, double *ms, int64_t *nl, int nclass) {
   return QD_OK;
 }
 
+int qd_set_graphs(qd_context *c, int on) {
+  if (!c) return fail(QD_E_ARG, "null context");
+  HIPC(hipStreamSynchronize(c->stream));
+  for (auto &kv : c->graphs) hipGraphExecDestroy(kv.second);
+  c->graphs.clear();
+  c->use_graphs = on != 0;
+  return QD_OK;
+}
+
+int qd_set_launch_mode(qd_context *c, int mode) {
+  if (!c || (mode != 0 && mode != 1)) return fail(QD_E_ARG, "launch mode must be 0 (grouped) or 1 (per channel)");
+  c->launch_mode = mode;
+  return QD_OK;
+}
+
+namespace {
+int size_class(int x) { return x <= 16 ? 0 : x <= 32 ? 1 : x <= 64 ? 2 : x <= 128 ? 3 : x <= 256 ? 4 : 5; }
+
+// device time (ms) of one grouped launch over the given problems with their current shapes
+int time_subset(qd_context *c, GemmList &g, const std::vector<int> &pis, const Slots &s, int reps, GemmTile *dtiles,
+                double *ms) {
+  std::vector<GemmTile> t;
+  for (int pi : pis) tiles_of(g, pi, t);
+  HIPC(hipMemcpy(g.probs, g.hprobs.data(), sizeof(GemmProb) * g.hprobs.size(), hipMemcpyHostToDevice));
+  HIPC(hipMemcpy(dtiles, t.data(), sizeof(GemmTile) * t.size(), hipMemcpyHostToDevice));
+  hipEvent_t a, b;
+  HIPC(hipEventCreate(&a));
+  HIPC(hipEventCreate(&b));
+  launch_gemm(c->stream, g.probs, g.segs, dtiles, (int)t.size(), s, c->gemm_grid);  // warm
+  HIPC(hipEventRecord(a, c->stream));
+  for (int r = 0; r < reps; r++) launch_gemm(c->stream, g.probs, g.segs, dtiles, (int)t.size(), s, c->gemm_grid);
+  HIPC(hipEventRecord(b, c->stream));
+  HIPC(hipEventSynchronize(b));
+  float f = 0;
+  HIPC(hipEventElapsedTime(&f, a, b));
+  hipEventDestroy(a);
+  hipEventDestroy(b);
+  *ms = f / reps;
+  return QD_OK;
+}
+
+int tune_list(qd_context *c, GemmList &g, const Slots &s, int reps, GemmTile *dtiles) {
+  std::map<int, std::vector<int>> buckets;
+  for (int pi = 0; pi < (int)g.hprobs.size(); pi++) {
+    const GemmProb &p = g.hprobs[pi];
+    buckets[size_class(p.M) * 100 + size_class(p.N) * 10 + size_class(ktot(g, p))].push_back(pi);
+  }
+  for (auto &kv : buckets) {
+    int best = -1;
+    double best_ms = 1e300;
+    for (int sh = 0; sh < NSHAPE; sh++) {
+      for (int pi : kv.second) g.hprobs[pi].shape = sh;
+      double ms;
+      int rc = time_subset(c, g, kv.second, s, reps, dtiles, &ms);
+      if (rc) return rc;
+      if (ms < best_ms) { best_ms = ms; best = sh; }
+    }
+    for (int pi : kv.second) g.hprobs[pi].shape = best;
+  }
+  return upload_tiles(c, g);
+}
+}  // namespace
+
+int qd_autotune(qd_context *c, int reps, double *ms_before, double *ms_after) {
+  if (!c) return fail(QD_E_ARG, "null context");
+  if (reps < 1) reps = 3;
+  int rc;
+  qd_op *a, *b, *o;
+  qd_cc *x, *y;
+  if ((rc = qd_op_create(c, -1, &a)) || (rc = qd_op_create(c, 1, &b)) || (rc = qd_op_create(c, 1, &o))) return rc;
+  if ((rc = qd_cc_create(c, -1.0, &x)) || (rc = qd_cc_create(c, 1.0, &y))) return rc;
+  x->zero = y->zero = false;
+  const Slots s1 = make_slots(c, a, b, o, x, y);
+  size_t cap = std::max(c->g_all.cap, c->g_acc.cap);
+  void *dt;
+  if ((rc = dev_alloc(c, sizeof(GemmTile) * cap, &dt))) return rc;
+  auto total = [&](double *ms) -> int {
+    hipEvent_t e0, e1;
+    HIPC(hipEventCreate(&e0));
+    HIPC(hipEventCreate(&e1));
+    HIPC(hipEventRecord(e0, c->stream));
+    for (int r = 0; r < reps; r++) {
+      if ((rc = run_gemm_list(c, c->g_all, true, s1))) return rc;
+      if ((rc = run_gemm_list(c, c->g_acc, false, s1))) return rc;
+    }
+    HIPC(hipEventRecord(e1, c->stream));
+    HIPC(hipEventSynchronize(e1));
+    float f = 0;
+    HIPC(hipEventElapsedTime(&f, e0, e1));
+    hipEventDestroy(e0);
+    hipEventDestroy(e1);
+    *ms = f / reps;
+    return QD_OK;
+  };
+  double before = 0, after = 0;
+  if ((rc = total(&before))) return rc;
+  if ((rc = tune_list(c, c->g_all, s1, reps, (GemmTile *)dt))) return rc;
+  if ((rc = tune_list(c, c->g_acc, s1, reps, (GemmTile *)dt))) return rc;
+  if ((rc = total(&after))) return rc;
+  if (ms_before) *ms_before = before;
+  if (ms_after) *ms_after = after;
+  qd_op_destroy(a); qd_op_destroy(b); qd_op_destroy(o); qd_cc_destroy(x); qd_cc_destroy(y);
+  return QD_OK;
+}
+
+int qd_gemm_shapes(const qd_context *c, int *shapes, int64_t len) {
+  if (!c) return fail(QD_E_ARG, "null context");
+  const int64_t n = (int64_t)(c->g_all.hprobs.size() + c->g_acc.hprobs.size());
+  if (shapes) {
+    if (len < n) return fail(QD_E_ARG, "needs %lld entries", (long long)n);
+    int64_t k = 0;
+    for (auto &p : c->g_all.hprobs) shapes[k++] = p.shape;
+    for (auto &p : c->g_acc.hprobs) shapes[k++] = p.shape;
+  }
+  return (int)n;
+}
+
 int qd_gemm_flops(const qd_context *c, double *ladder, double *pandya, double *acc) {
   if (!c) return fail(QD_E_ARG, "null context");
-  if (ladder) *ladder = c->g_all.flops - c->pandya_flops;
-  if (pandya) *pandya = c->pandya_flops;
+  if (ladder) *ladder = c->g_all.flops_pre;
+  if (pandya) *pandya = c->g_all.flops - c->g_all.flops_pre;
   if (acc) *acc = c->g_acc.flops;
   return QD_OK;
 }

@@ -32,10 +32,14 @@ struct GemmSeg {   // one product op(A)*op(B) of a problem, alpha = +-1
 };
 struct GemmProb {  // C(M,N) (+)= sum over segments
   int64_t c_off;
-  int c_slot, ldc, M, N, beta1, seg0, nseg;
+  int c_slot, ldc, M, N, beta1, seg0, nseg, shape;
 };
 struct GemmTile { int prob, tm, tn; };
-constexpr int GEMM_TM = 64, GEMM_TN = 64, GEMM_TK = 16;
+constexpr int GEMM_TK = 16;
+// tile shapes of the grouped GEMM (rows x cols of C per work-group)
+enum GemmShape { SH_64x64 = 0, SH_128x64, SH_32x32, SH_128x16, SH_64x16, SH_16x64, SH_16x128, NSHAPE };
+constexpr int SHAPE_TM[NSHAPE] = {64, 128, 32, 128, 64, 16, 16};
+constexpr int SHAPE_TN[NSHAPE] = {64, 64, 32, 16, 16, 64, 128};
 
 struct Ops {  // two operators and their hermiticity
   const double *v1, *v2;
@@ -47,8 +51,9 @@ enum { TB_122 = 1, TB_LADDER = 2, TB_PANDYA = 4, TB_ACCUM = 8 };
 // flags of the one-body kernel
 enum { OB_111 = 1, OB_121 = 2, OB_221 = 4, OB_ACCUM = 8 };
 
+// grid <= 0: one work-group per tile; else a persistent grid striding the list
 void launch_gemm(hipStream_t s, const GemmProb *probs, const GemmSeg *segs, const GemmTile *tiles,
-                 int ntiles, const Slots &slots);
+                 int ntiles, const Slots &slots, int grid);
 void launch_twobody(hipStream_t s, const DevTab &t, const Ops &o, double *out, const Task2 *tasks,
                     int ntasks, int flags, const double *w1, const double *w2, const double *w3,
                     const double *P, double hh, double eh);
