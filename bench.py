@@ -29,17 +29,20 @@ PEAK_HBM = 8.0e12          # MI355X HBM3E, B/s
 CLASS_NAMES = ["gemm_products", "gemm_accumulate", "twobody_gather", "onebody", "calc_cc", "other"]
 
 
-def random_operator(ch, rng, herm):
-    """Seeded random operator in the blocked layout (same recipe as the tests)."""
-    from tests.pins.layout import random_flat
-    return random_flat(ch, rng, herm)
+def progress(msg):
+    """One line on stderr per finished leg, so a long run shows where it is."""
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
-def cpu_baseline(x, y, ch, flops_ref, budget_s=20.0):
-    """The FP64 CPU oracle (C + OpenMP) on the host cores, one commutator at a
-    time until ~budget_s of CPU work; returns the same metric in TFLOP/s."""
+def cpu_baseline(x, y, nbody, nshell, flops_ref, flows, budget_s=20.0):
+    """The FP64 CPU oracle (C + OpenMP) on the host cores: one commutator at a
+    time until ~budget_s of CPU work (same metric, TFLOP/s), and the N=6 R=6
+    time-to-E_gs flows from the same H(0) as the GPU flows (the oracle's E_gs
+    is also the check of the GPU's)."""
+    from oracle.channels import Channels
+    from oracle.flow import magnus_flow
     from oracle.oracle import Oracle
-    o = Oracle(ch)
+    o = Oracle(Channels(nbody, nshell))
     threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
     o.commutator(x, 1, y, 1)  # warm
     n, t0 = 0, time.perf_counter()
@@ -49,9 +52,142 @@ def cpu_baseline(x, y, ch, flops_ref, budget_s=20.0):
         dt = time.perf_counter() - t0
         if dt > budget_s or n >= 20:
             break
-    return dict(value=flops_ref * n / dt / 1e12, unit="TFLOP/s", cores=threads, kind="port",
-                sample=f"{n} full [X,Y] at N=20 R=14 (calc_cc + all pieces) by the C oracle, "
-                       f"{dt / n * 1e3:.1f} ms each, OMP_NUM_THREADS={threads}")
+    out = dict(value=flops_ref * n / dt / 1e12, unit="TFLOP/s", cores=threads, kind="port",
+               sample=f"{n} full [X,Y] at N={nbody} R={nshell} (calc_cc + all pieces) by the C oracle, "
+                      f"{dt / n * 1e3:.1f} ms each, OMP_NUM_THREADS={threads}")
+    if flows:
+        o6 = Oracle(Channels(6, 6))
+        recs = []
+        for rec in flows["runs"]:
+            t0 = time.perf_counter()
+            ref = magnus_flow(o6, flows["h0"], pandya=rec["pandya"])
+            t = time.perf_counter() - t0
+            recs.append(dict(workload=rec["workload"], cpu_oracle_s=t, oracle_e_gs=ref["e0"],
+                             gpu_s=rec["gpu_s"], abs_dE_gpu_vs_oracle=abs(rec["e_gs"] - ref["e0"])))
+            progress(f"CPU oracle flow pandya={rec['pandya']}: {t:.2f} s")
+        out["time_to_egs"] = recs
+    return out
+
+
+def time_to_egs():
+    """IMSRG(2) time-to-E_gs at N=6, R=6, hw=1 (HF basis, Magnus, White):
+    wall time from H(0) resident on the GPU to convergence, as shipped (zero
+    CC inputs) and with calc_cc-filled CC inputs (the Pandya term)."""
+    import qd_imsrg_amd as qd
+    h0, _, _ = qd.hf_hamiltonian(6, 6, 1.0)
+    out = []
+    for pandya in (False, True):
+        ctx = qd.Context(6, 6)
+        ctx.set_graphs(True)
+        H0 = ctx.operator(1, h0)
+        best = None
+        for rep in range(3):  # first run captures the graphs
+            HS, OM = ctx.operator(1), ctx.operator(-1)
+            ctx.sync()
+            t0 = time.perf_counter()
+            res, _ = qd.magnus_flow(ctx, H0, HS, OM, pandya=pandya)
+            t = time.perf_counter() - t0
+            best = t if best is None else min(best, t)
+            progress(f"N=6 R=6 flow pandya={pandya} rep {rep}: {t:.3f} s, E={res.e0:.9f}")
+        out.append({"workload": f"N=6 R=6 hw=1.0 HF basis, Magnus IM-SRG(2) White, "
+                                f"{'calc_cc CC inputs (Pandya term)' if pandya else 'as shipped (zero CC inputs)'}",
+                    "pandya": pandya, "gpu_s": best, "e_gs": res.e0, "iterations": res.iterations,
+                    "bch_terms": int(res.bch_terms)})
+        ctx.close()
+    return {"runs": out, "h0": h0}
+
+
+def precision_form(nbody=42, nshell=20, hw=0.28, modes=("fp64", "bf16x3", "fp32", "bf16x2", "bf16x4")):
+    """Precision form at N=42, R=20, hw=0.28 (HF basis, Magnus, White): the
+    same flow with the 2b-2b GEMMs in each arithmetic, grouped + autotuned +
+    graphs throughout.  Per mode: time to E_gs, |dE_gs|, relative error of
+    ||Omega|| and of ||Gamma_od|| against the FP64 GPU flow, and the device
+    rate of one full commutator [Omega, H(s)] (FP64-equivalent reference
+    flops / time)."""
+    import qd_imsrg_amd as qd
+    from qd_imsrg_amd.layout import Layout
+    t0 = time.perf_counter()
+    h0, _, it = qd.hf_hamiltonian(nbody, nshell, hw)
+    t_hf = time.perf_counter() - t0
+    progress(f"N={nbody} R={nshell} HF done in {t_hf:.1f} s")
+    lay = Layout(nbody, nshell)
+    flops = lay.reference_gemm_flops()["total"]
+    ctx = qd.Context(nbody, nshell)
+    tune = ctx.autotune(reps=1)
+    progress(f"autotune GEMM ms {tune[0]:.2f} -> {tune[1]:.2f}")
+    ctx.set_graphs(True)
+    H0 = ctx.operator(1, h0)
+    HCC, ECC, Z = ctx.cc(1.0), ctx.cc(-1.0), ctx.operator(1)
+    res = {}
+    for p in modes:
+        ctx.set_precision(p)
+        HS, OM = ctx.operator(1), ctx.operator(-1)
+        ctx.sync()
+        t0 = time.perf_counter()
+        r, _ = qd.magnus_flow(ctx, H0, HS, OM, verbose=2)  # per-step lines on stderr
+        t = time.perf_counter() - t0
+        # device rate of the full commutator at this size and precision
+        qd.calc_cc(HS, HCC)
+        qd.calc_cc(OM, ECC)
+        qd.commutator(OM, HS, Z, HCC, ECC)
+        ctx.sync()
+        k, t1 = 3, time.perf_counter()
+        for _ in range(k):
+            qd.calc_cc(HS, HCC)
+            qd.calc_cc(OM, ECC)
+            qd.commutator(OM, HS, Z, HCC, ECC)
+        ctx.sync()
+        ms = (time.perf_counter() - t1) * 1e3 / k
+        res[p] = {"time_to_egs_s": t, "e_gs": r.e0, "iterations": r.iterations, "bch_terms": int(r.bch_terms),
+                  "converged": bool(r.converged), "omega_norm": qd.mat_2_norm(OM),
+                  "gamma_od_norm": lay.gamma_od_norm(HS.download()), "commutator_ms": ms,
+                  "commutator_tflops_fp64_equiv": flops / (ms * 1e-3) / 1e12}
+        progress(f"precision {p}: {t:.2f} s, E={r.e0:.10f}, {r.iterations} steps, commutator {ms:.2f} ms")
+        HS.close(), OM.close()
+    ctx.close()
+    ref = res.get("fp64")
+    if ref:
+        for p, v in res.items():
+            v["abs_dE"] = abs(v["e_gs"] - ref["e_gs"])
+            v["rel_err_omega_norm"] = abs(v["omega_norm"] - ref["omega_norm"]) / ref["omega_norm"]
+            v["rel_err_gamma_od_norm"] = abs(v["gamma_od_norm"] - ref["gamma_od_norm"]) / ref["gamma_od_norm"]
+            v["speedup_vs_fp64"] = ref["time_to_egs_s"] / v["time_to_egs_s"]
+    res["workload"] = (f"N={nbody} R={nshell} hw={hw} HF basis (E_HF={h0[0]:.9f}, {it} SCF iterations, "
+                       f"{t_hf:.1f} s host), Magnus IM-SRG(2) White, as shipped; F_2b2b={flops / 1e9:.1f} GF")
+    return res
+
+
+def batched_flow(nbody=20, nshell=16, hw=0.5):
+    """Batched form at N=20, R=16, hw=0.5: the Magnus flow with the grouped
+    persistent GEMM launch, autotuned tiles and hipGraph-captured BCH terms,
+    against the per-channel plain launch path on the same H(0)."""
+    import qd_imsrg_amd as qd
+    t0 = time.perf_counter()
+    h0, _, it = qd.hf_hamiltonian(nbody, nshell, hw)
+    t_hf = time.perf_counter() - t0
+    progress(f"N={nbody} R={nshell} HF done in {t_hf:.1f} s")
+    res = {}
+    for mode in ("per_channel", "grouped_autotuned_graphs"):
+        ctx = qd.Context(nbody, nshell)
+        tune = None
+        if mode == "per_channel":
+            ctx.set_launch_mode(1)
+        else:
+            tune = ctx.autotune(reps=2)
+            ctx.set_graphs(True)
+        H0, HS, OM = ctx.operator(1, h0), ctx.operator(1), ctx.operator(-1)
+        ctx.sync()
+        t0 = time.perf_counter()
+        r, _ = qd.magnus_flow(ctx, H0, HS, OM)
+        t = time.perf_counter() - t0
+        progress(f"batched form {mode}: {t:.2f} s, E={r.e0:.9f}, {r.iterations} steps")
+        res[mode] = {"time_to_egs_s": t, "e_gs": r.e0, "iterations": r.iterations, "bch_terms": int(r.bch_terms),
+                     "autotune_gemm_ms_before_after": tune}
+        ctx.close()
+    res["abs_dE_between_paths"] = abs(res["per_channel"]["e_gs"] - res["grouped_autotuned_graphs"]["e_gs"])
+    res["speedup"] = res["per_channel"]["time_to_egs_s"] / res["grouped_autotuned_graphs"]["time_to_egs_s"]
+    res["workload"] = f"N={nbody} R={nshell} hw={hw} HF basis (E_HF={h0[0]:.9f}, {it} SCF iterations, {t_hf:.1f} s host)"
+    return res
 
 
 def breakdown(ctx, X, Y, HCC, ECC, reps=10):
@@ -80,6 +216,31 @@ def breakdown(ctx, X, Y, HCC, ECC, reps=10):
     return res
 
 
+def precision_micro(ctx, step, Z, flops, reps=10):
+    """The same microbenchmark step with the 2b-2b GEMMs in each split
+    arithmetic: device-side rate (FP64-equivalent reference flops / time) and
+    max |[X,Y] - [X,Y]_fp64| / max |[X,Y]_fp64|."""
+    ctx.set_precision("fp64")
+    step()
+    ref = Z.download()
+    out = {}
+    for p in ("fp64", "fp32", "bf16x2", "bf16x3", "bf16x4"):
+        ctx.set_precision(p)
+        step()
+        ctx.sync()
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            step()
+        ctx.sync()
+        ms = (time.perf_counter() - t0) * 1e3 / reps
+        z = Z.download()
+        out[p] = {"ms_per_step": ms, "tflops_fp64_equiv": flops / (ms * 1e-3) / 1e12,
+                  "max_rel_err": float(np.abs(z - ref).max() / np.abs(ref).max())}
+    ctx.set_precision("fp64")
+    progress("precision micro: " + ", ".join(f"{p} {v['ms_per_step']:.3f} ms" for p, v in out.items()))
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -91,6 +252,11 @@ def main():
     ap.add_argument("--seed", type=int, default=20261203)
     ap.add_argument("--breakdown", action="store_true",
                     help="also time each reference entry point (110..222, calc_cc) on its own")
+    ap.add_argument("--no-flow", action="store_true", help="skip the N=6 R=6 time-to-E_gs flows")
+    ap.add_argument("--batched", action="store_true",
+                    help="also run the N=20 R=16 batched-form flow against the per-channel path")
+    ap.add_argument("--precision", action="store_true",
+                    help="also run the N=42 R=20 precision-form flows (FP64, FP32, BF16 slices)")
     a = ap.parse_args()
     if a.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {a.gpus} is not supported "
@@ -98,15 +264,17 @@ def main():
         sys.exit(2)
 
     import qd_imsrg_amd as qd
-    from oracle.channels import Channels  # layout sizes and the reference flop count only
+    from qd_imsrg_amd.layout import Layout, random_operator
 
-    ch = Channels(a.nbody, a.nshell)
-    fl = ch.gemm_flops_2b2b()
+    ch = Layout(a.nbody, a.nshell)
+    fl = ch.reference_gemm_flops()
     rng = np.random.default_rng(a.seed)
     x = random_operator(ch, rng, 1)
     y = random_operator(ch, rng, 1)
 
     ctx = qd.Context(a.nbody, a.nshell)
+    tune = ctx.autotune(reps=3)  # tile shapes per size bucket (setup, not timed)
+    progress(f"autotune: GEMM ms {tune[0]:.3f} -> {tune[1]:.3f}")
     X, Y, Z = ctx.operator(1, x), ctx.operator(1, y), ctx.operator(-1)
     HCC, ECC = ctx.cc(1.0), ctx.cc(1.0)
 
@@ -127,6 +295,7 @@ def main():
     t1 = time.perf_counter()
     ms_cls, n_cls = ctx.timing_read()
     ctx.timing(False)
+    progress(f"commutator microbenchmark: {a.steps} steps timed")
     ms_step = (t1 - t0) * 1e3 / a.steps
     value = fl["total"] / (ms_step * 1e-3) / 1e12
 
@@ -165,11 +334,21 @@ def main():
         "gemm_flops_per_step": gf,
         "roofline": roof,
     }
+    out["autotune_gemm_ms_before_after"] = tune
     if a.breakdown:
         out["breakdown_ms"] = breakdown(ctx, X, Y, HCC, ECC)
-    if not a.no_cpu:
-        out["cpu_baseline"] = cpu_baseline(x, y, ch, fl["total"])
+    out["precision_micro"] = precision_micro(ctx, step, Z, fl["total"])
     ctx.close()
+    flows = None
+    if not a.no_flow:
+        flows = time_to_egs()
+        out["time_to_egs"] = flows["runs"]
+    if a.batched:
+        out["batched_form"] = batched_flow()
+    if a.precision:
+        out["precision_form"] = precision_form()
+    if not a.no_cpu:
+        out["cpu_baseline"] = cpu_baseline(x, y, a.nbody, a.nshell, fl["total"], flows)
     print(json.dumps(out))
 
 

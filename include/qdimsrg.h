@@ -68,7 +68,9 @@ int qd_context_info(const qd_context *ctx, qd_info *out);
  * nhh,npp,nph), 1 flat offsets (nblock x 6), 2 i_array (M(M-1)/2 x 3: type
  * 1..3, block, index; 0-based), 3 CC map (M*M x 2: block,row), 4 CC dims
  * (ncc x 2: r, nb), 5 pair lists (hh, ph, pp of every block, concatenated,
- * x 2: (i, j) with i < j).  len is the capacity of out in int64 elements; returns the count. */
+ * x 2: (i, j) with i < j), 6 sizes (neq, cc_size, nblock, ncc), 7 single-
+ * particle quantum numbers (M x 2: m_l, m_s as 2*s_z, states order).  len is
+ * the capacity of out in int64 elements; returns the count. */
 int64_t qd_context_table(const qd_context *ctx, int which, int64_t *out, int64_t len);
 /* The same tables without a context or a GPU (host-only channel builder);
  * which = 6 gives (neq, cc_size, nblock, ncc). */
@@ -171,7 +173,8 @@ typedef struct {
   double conv;       /* stop when |MBPT2(H(s))| <= conv (reference: REAL(4) 1e-9)   */
   int max_iter;      /* safety cap on accepted + rejected steps                     */
   int pandya;        /* 0 as shipped, 1 with calc_cc-filled CC inputs                */
-  int verbose;       /* 1: print the reference's per-iteration line to stdout       */
+  int verbose;       /* 1: print the reference's per-iteration line to stdout;
+                       2: the same line to stderr, flushed every iteration      */
   int warm_start;    /* 1: start from the Omega passed in (H(s) = BCH(Omega, H(0)))  */
 } qd_flow_params;
 typedef struct {
@@ -201,11 +204,63 @@ int qd_set_launch_mode(qd_context *ctx, int mode);
  * replays it; graphs are keyed by the operand addresses.  0 (default): plain
  * stream launches.  No counterpart in the reference. */
 int qd_set_graphs(qd_context *ctx, int on);
-/* Project autotuner: for every (M, N, K) size bucket of the grouped GEMM
- * problems, times each tile shape (64x64, 128x64, 32x32, 128x16, 64x16,
- * 16x64, 16x128) on scratch operands and keeps the fastest.  Reports the
- * device time of one commutator's GEMM launches before and after (ms). */
+/* Project autotuner: coordinate descent over the (M, N, K) size buckets of
+ * the grouped GEMM problems, largest work first.  For each bucket every tile
+ * shape (64x64, 128x64, 32x32, 128x16, 64x16, 16x64, 16x128) is tried, timing
+ * one commutator's whole GEMM work (both grouped lists) on scratch operands at
+ * the current precision, and the fastest is kept.  Reports that device time
+ * before and after (ms).  Drops captured BCH graphs. */
 int qd_autotune(qd_context *ctx, int reps, double *ms_before, double *ms_after);
+
+/* ---- learned warm start ----------------------------------------------
+ * No counterpart in the reference (s_evolve, src/magnus_IMSRG.f90:194-248,
+ * always starts from Omega = 0).  A small FP32 multilayer perceptron
+ * (nin -> nlayers x nhidden tanh -> nout linear, Glorot-uniform weights from
+ * seed, zero biases), trained and evaluated on the context's stream by the
+ * library's own kernels, maps frequency features to the coefficients of a PCA
+ * basis of converged Omegas; qd_basis_expand turns coefficients into an
+ * operator, the warm start of qd_magnus_flow (warm_start = 1).
+ * Parameters: one flat FP32 vector, per layer W (out x in, row-major) then b.
+ * x: batch x nin, t and y: batch x nout, row-major.  Loss: mean squared
+ * error over batch and outputs.  qd_mlp_train: full-batch Adam (beta1 0.9,
+ * beta2 0.999, eps 1e-8), optional per-step loss trace (length steps). */
+typedef struct qd_mlp qd_mlp;
+typedef struct qd_basis qd_basis;
+int qd_mlp_create(qd_context *ctx, int nin, int nhidden, int nlayers, int nout, uint64_t seed, qd_mlp **out);
+int qd_mlp_destroy(qd_mlp *m);
+int64_t qd_mlp_num_params(const qd_mlp *m);
+int qd_mlp_get_params(qd_mlp *m, float *p, int64_t n);
+int qd_mlp_set_params(qd_mlp *m, const float *p, int64_t n);  /* also resets the Adam state */
+int qd_mlp_forward(qd_mlp *m, const float *x, int batch, float *y);
+int qd_mlp_loss_grad(qd_mlp *m, const float *x, const float *t, int batch, float *loss, float *grad);
+int qd_mlp_train(qd_mlp *m, const float *x, const float *t, int batch, int steps, float lr, float *loss_trace);
+/* PCA basis of operators: mean (neq) and rank vectors (rank x neq, row-major),
+ * FP64, held on the device.  qd_basis_expand: out = mean + sum_r coef[r] vecs[r]. */
+int qd_basis_create(qd_context *ctx, int rank, const double *mean, const double *vecs, qd_basis **out);
+int qd_basis_destroy(qd_basis *b);
+int qd_basis_expand(qd_basis *b, const double *coef, qd_op *out);
+
+/* ---- precision form: arithmetic of the 2b-2b GEMMs ----------------------
+ * The GEMMs of commutator_222 (matrix_mult222 products and accumulations,
+ * the Pandya products of construct_cc_ints) and the commutator_221
+ * intermediates they share:
+ *   QD_PREC_FP64    FP64 MFMA (default; the plain form)
+ *   QD_PREC_FP32    operands rounded to FP32, FP32 MFMA
+ *   QD_PREC_BF16X2..4  operands cut into 2..4 BF16 mantissa slices; slice
+ *                   pairs (i, j) with i + j < S on BF16 MFMA
+ * Split modes accumulate in FP32 over K stages of 32 and in FP64 across
+ * stages and into the output.  Everything else stays FP64: the one-body
+ * pieces, 122, the Pandya gather, the zero-body traces, White denominators,
+ * the BCH sum, norms and the Omega update.  No counterpart in the reference
+ * (src/commutators.f90:999-1127 and src/ME_general.f90:948-949 call dgemm).
+ * Drops captured BCH graphs. */
+#define QD_PREC_FP64 0
+#define QD_PREC_FP32 1
+#define QD_PREC_BF16X2 2
+#define QD_PREC_BF16X3 3
+#define QD_PREC_BF16X4 4
+int qd_set_precision(qd_context *ctx, int prec);
+int qd_get_precision(qd_context *ctx, int *prec);
 /* Current tile shape of every GEMM problem (products, then accumulating);
  * returns the count. */
 int qd_gemm_shapes(const qd_context *ctx, int *shapes, int64_t len);

@@ -16,6 +16,9 @@ import numpy as np
 from . import _lib
 from ._lib import FlowParams, FlowResult, QDError, check
 
+# QD_PREC_* of qdimsrg.h: arithmetic of the 2b-2b GEMMs
+PRECISIONS = {"fp64": 0, "fp32": 1, "bf16x2": 2, "bf16x3": 3, "bf16x4": 4}
+
 __all__ = ["Context", "Operator", "CCMat", "QDError", "commutator_110", "commutator_220", "commutator_111",
            "commutator_121", "commutator_221", "commutator_122", "commutator_222", "calc_cc", "build_white",
            "add_arrays", "copy_arrays", "set_to_zero", "mat_2_norm", "MBPT2", "commutator", "BCH_EXPAND",
@@ -81,6 +84,17 @@ class Context:
     def set_graphs(self, on: bool):
         """Capture each BCH term in a hipGraph and replay it."""
         check(self.L.qd_set_graphs(self.h, int(bool(on))))
+
+    def set_precision(self, prec):
+        """Arithmetic of the 2b-2b GEMMs: "fp64" (default), "fp32", or
+        "bf16x2" / "bf16x3" / "bf16x4" (BF16 mantissa slices); or the
+        QD_PREC_* integer."""
+        check(self.L.qd_set_precision(self.h, PRECISIONS[prec] if isinstance(prec, str) else int(prec)))
+
+    def precision(self) -> str:
+        v = C.c_int()
+        check(self.L.qd_get_precision(self.h, C.byref(v)))
+        return {i: k for k, i in PRECISIONS.items()}[v.value]
 
     def autotune(self, reps: int = 3):
         a, b = C.c_double(), C.c_double()
@@ -287,9 +301,10 @@ def dGam(H, DER, HCC, ETACC):
 
 
 def magnus_flow(ctx: Context, H0: Operator, HS: Operator, OMEGA: Operator, pandya: bool = False,
-                warm_start: bool = False, verbose: bool = False, max_iter: int = 100000,
+                warm_start: bool = False, verbose: int = 0, max_iter: int = 100000,
                 conv: float | None = None, trace_len: int = 4096):
-    """s_evolve (src/magnus_IMSRG.f90:194-248) on the GPU; returns (result, trace)."""
+    """s_evolve (src/magnus_IMSRG.f90:194-248) on the GPU; returns (result, trace).
+    verbose: 1 prints the reference's per-iteration line to stdout, 2 to stderr."""
     p = FlowParams()
     ctx.L.qd_flow_params_default(C.byref(p))
     p.pandya, p.warm_start, p.verbose, p.max_iter = int(pandya), int(warm_start), int(verbose), int(max_iter)

@@ -14,6 +14,7 @@ vp = C.c_void_p
 dp = C.POINTER(C.c_double)
 i64p = C.POINTER(C.c_int64)
 ip = C.POINTER(C.c_int)
+fp = C.POINTER(C.c_float)
 
 
 class Info(C.Structure):
@@ -81,6 +82,19 @@ SIGNATURES = {
     "qd_set_launch_mode": (C.c_int, [vp, C.c_int]),
     "qd_set_graphs": (C.c_int, [vp, C.c_int]),
     "qd_autotune": (C.c_int, [vp, C.c_int, dp, dp]),
+    "qd_set_precision": (C.c_int, [vp, C.c_int]),
+    "qd_mlp_create": (C.c_int, [vp, C.c_int, C.c_int, C.c_int, C.c_int, C.c_uint64, C.POINTER(vp)]),
+    "qd_mlp_destroy": (C.c_int, [vp]),
+    "qd_mlp_num_params": (C.c_int64, [vp]),
+    "qd_mlp_get_params": (C.c_int, [vp, fp, C.c_int64]),
+    "qd_mlp_set_params": (C.c_int, [vp, fp, C.c_int64]),
+    "qd_mlp_forward": (C.c_int, [vp, fp, C.c_int, fp]),
+    "qd_mlp_loss_grad": (C.c_int, [vp, fp, fp, C.c_int, fp, fp]),
+    "qd_mlp_train": (C.c_int, [vp, fp, fp, C.c_int, C.c_int, C.c_float, fp]),
+    "qd_basis_create": (C.c_int, [vp, C.c_int, dp, dp, C.POINTER(vp)]),
+    "qd_basis_destroy": (C.c_int, [vp]),
+    "qd_basis_expand": (C.c_int, [vp, dp, vp]),
+    "qd_get_precision": (C.c_int, [vp, ip]),
     "qd_gemm_shapes": (C.c_int, [vp, ip, C.c_int64]),
     "qd_v_int": (C.c_double, [ip, ip, ip, ip, C.c_double]),
     "qd_hf_hamiltonian": (C.c_int, [C.c_int, C.c_int, C.c_double, dp, C.c_int64, dp, ip]),

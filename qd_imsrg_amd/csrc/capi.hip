@@ -16,8 +16,9 @@ using namespace qd;
 
 namespace {
 thread_local std::string g_err;
+}  // namespace
 
-int fail(int code, const char *fmt, ...) {
+int qd::set_error(int code, const char *fmt, ...) {
   char buf[512];
   va_list ap;
   va_start(ap, fmt);
@@ -25,6 +26,12 @@ int fail(int code, const char *fmt, ...) {
   va_end(ap);
   g_err = buf;
   return code;
+}
+
+namespace {
+template <class... A>
+int fail(int code, const char *fmt, A... a) {
+  return qd::set_error(code, fmt, a...);
 }
 
 #define HIPC(expr)                                                                     \
@@ -54,8 +61,9 @@ struct GemmList {
   GemmProb *probs = nullptr;
   GemmSeg *segs = nullptr;
   int npre = 0;
-  GemmTile *tiles_pre = nullptr, *tiles_all = nullptr;  // sorted by decreasing work
+  GemmTile *tiles_pre = nullptr, *tiles_all = nullptr;  // sorted by shape, then decreasing work
   int ntiles_pre = 0, ntiles_all = 0;
+  int shape_pre[NSHAPE + 1] = {}, shape_all[NSHAPE + 1] = {};  // per-shape ranges of the two lists
   GemmTile *ptiles = nullptr;                            // grouped per problem
   std::vector<int> ptile_off;
   size_t cap = 0;                                        // capacity of each tile buffer
@@ -78,6 +86,7 @@ struct qd_context {
   Seg2 *cc_segs = nullptr; Tile2 *cc_tiles = nullptr; int ncc_tiles = 0;
   GemmList g_all, g_acc;  // products (ladder + Pandya) and accumulating products
   int launch_mode = 0;    // 0 grouped persistent launch, 1 one launch per channel problem
+  int gemm_prec = GEMM_FP64;  // arithmetic of the 2b-2b GEMMs (GemmPrec)
   bool use_graphs = false;  // capture each BCH term (commutator + norms) in a hipGraph
   struct GraphKey {
     const void *g, *prev, *next, *hs, *cca, *ccg;
@@ -112,6 +121,11 @@ struct qd_cc {
   double herm;
   bool zero;  // known to be all zeros (skips the Pandya work, exactly)
 };
+
+hipStream_t qd::context_stream(qd_context *c) { return c->stream; }
+int64_t qd::context_neq(const qd_context *c) { return c->ch.neq; }
+qd_context *qd::op_context(qd_op *o) { return o->ctx; }
+double *qd::op_data(qd_op *o) { return o->d; }
 
 namespace {
 
@@ -190,13 +204,9 @@ int ktot(const GemmList &g, const GemmProb &p) {
 }
 
 // size-driven default before autotuning: skinny shapes for hole-pair sides
-int default_shape(int M, int N) {
-  if (N <= 16 && M > 16) return M >= 128 ? SH_128x16 : SH_64x16;
-  if (M <= 16 && N > 16) return N >= 128 ? SH_16x128 : SH_16x64;
-  if (M <= 32 && N <= 32) return SH_32x32;
-  if (M >= 256 && N >= 64) return SH_128x64;
-  return SH_64x64;
-}
+// 64x64 everywhere until the autotuner has measured something better: the
+// larger and the skinny shapes only pay off for some size buckets
+int default_shape(int, int) { return SH_64x64; }
 
 void tiles_of(const GemmList &g, int pi, std::vector<GemmTile> &out) {
   const GemmProb &p = g.hprobs[pi];
@@ -221,9 +231,20 @@ int upload_tiles(qd_context *c, GemmList &g) {
   g.ptile_off[np] = (int)grouped.size();
   all = grouped;
   for (int pi = 0; pi < g.npre; pi++) tiles_of(g, pi, pre);
-  auto bywork = [&](const GemmTile &a, const GemmTile &b) { return work(a) > work(b); };
+  auto bywork = [&](const GemmTile &a, const GemmTile &b) {
+    const int sa = g.hprobs[a.prob].shape, sb = g.hprobs[b.prob].shape;
+    if (sa != sb) return sa < sb;
+    return work(a) > work(b);
+  };
   std::stable_sort(pre.begin(), pre.end(), bywork);
   std::stable_sort(all.begin(), all.end(), bywork);
+  auto ranges = [&](const std::vector<GemmTile> &t, int *r) {
+    for (int sh = 0; sh <= NSHAPE; sh++) r[sh] = 0;
+    for (auto &x : t) r[g.hprobs[x.prob].shape + 1]++;
+    for (int sh = 0; sh < NSHAPE; sh++) r[sh + 1] += r[sh];
+  };
+  ranges(pre, g.shape_pre);
+  ranges(all, g.shape_all);
   if (!g.tiles_all) {  // capacity: every shape covers at least 16 x 64 outputs
     size_t cap = 0;
     for (auto &p : g.hprobs) cap += (size_t)((p.M + 15) / 16) * ((p.N + 15) / 16);
@@ -420,13 +441,15 @@ bool pandya_on(const qd_cc *hcc, const qd_cc *ecc) {
 // mode) one launch per problem, the way the reference calls dgemm per block.
 int run_gemm_list(qd_context *c, const GemmList &g, bool with_pandya, const Slots &s) {
   if (c->launch_mode == 0) {
-    if (with_pandya) launch_gemm(c->stream, g.probs, g.segs, g.tiles_all, g.ntiles_all, s, c->gemm_grid);
-    else launch_gemm(c->stream, g.probs, g.segs, g.tiles_pre, g.ntiles_pre, s, c->gemm_grid);
+    const int *r = with_pandya ? g.shape_all : g.shape_pre;
+    GemmTile *t = with_pandya ? g.tiles_all : g.tiles_pre;
+    for (int sh = 0; sh < NSHAPE; sh++)
+      launch_gemm(c->stream, g.probs, g.segs, t + r[sh], r[sh + 1] - r[sh], s, c->gemm_grid, sh, c->gemm_prec);
   } else {
     const int np = with_pandya ? (int)g.hprobs.size() : g.npre;
     for (int pi = 0; pi < np; pi++) {
       const int n = g.ptile_off[pi + 1] - g.ptile_off[pi];
-      launch_gemm(c->stream, g.probs, g.segs, g.ptiles + g.ptile_off[pi], n, s, 0);
+      launch_gemm(c->stream, g.probs, g.segs, g.ptiles + g.ptile_off[pi], n, s, 0, g.hprobs[pi].shape, c->gemm_prec);
     }
   }
   HIPC(hipGetLastError());
@@ -526,6 +549,7 @@ int64_t layout_table(const Channels &ch, int which, int64_t *out, int64_t len) {
     case 4: for (int q = 0; q < ch.ncc; q++) { v.push_back(ch.cc_r[q]); v.push_back(ch.cc_nb[q]); } break;
     case 5: for (uint32_t pr : ch.pairs) { v.push_back(lo16(pr)); v.push_back(hi16(pr)); } break;
     case 6: v.push_back(ch.neq); v.push_back(ch.cc_size); v.push_back(ch.nblock); v.push_back(ch.ncc); break;
+    case 7: for (int i = 0; i < ch.M; i++) { v.push_back(ch.ml[i]); v.push_back(ch.ms[i]); } break;
     default: return fail(QD_E_ARG, "unknown table %d", which);
   }
   if (out) {
@@ -1002,13 +1026,17 @@ int qd_magnus_flow(qd_context *c, const qd_op *h0, qd_op *hs, qd_op *omega, cons
     HIPC(hipStreamSynchronize(c->stream));
     e0 = c->host_scal[0];
     if (!std::isfinite(e0) || !std::isfinite(nrm2)) return done(fail(QD_E_NUMERIC, "non-finite E0 or ||eta|| at s=%g", s));
-    if (p->verbose) printf("%5d%18.9f%18.9f%18.9f%18.9f%18.9f\n", iteration, s, e0, nrm1, nrm2, crit);
+    if (p->verbose) {
+      FILE *vf = p->verbose == 2 ? stderr : stdout;
+      fprintf(vf, "%5d%18.9f%18.9f%18.9f%18.9f%18.9f\n", iteration, s, e0, nrm1, nrm2, crit);
+      if (p->verbose == 2) fflush(vf);
+    }
     if (trace && rows < trace_len) {
       double *r = trace + 6 * (size_t)rows++;
       r[0] = iteration; r[1] = s; r[2] = e0; r[3] = nrm1; r[4] = nrm2; r[5] = crit;
     }
     if (nrm1 * 1.5 < nrm2) {
-      if (p->verbose) printf(" faillll\n");
+      if (p->verbose) fprintf(p->verbose == 2 ? stderr : stdout, " faillll\n");
       s -= stp;
       qd_copy_arrays(c, g0, omega);
       qd_copy_arrays(c, hs0, hs);
@@ -1064,105 +1092,123 @@ int qd_timing_read(qd_context *c, double *ms, int64_t *nl, int nclass) {
   return QD_OK;
 }
 
-int qd_set_graphs(qd_context *c, int on) {
-  if (!c) return fail(QD_E_ARG, "null context");
+// captured BCH terms bake in kernels, tile lists and shapes: any change to
+// those drops them (they are captured again on next use)
+static int drop_graphs(qd_context *c) {
   HIPC(hipStreamSynchronize(c->stream));
   for (auto &kv : c->graphs) hipGraphExecDestroy(kv.second);
   c->graphs.clear();
-  c->use_graphs = on != 0;
   return QD_OK;
+}
+
+int qd_set_graphs(qd_context *c, int on) {
+  if (!c) return fail(QD_E_ARG, "null context");
+  int rc = drop_graphs(c);
+  c->use_graphs = on != 0;
+  return rc;
 }
 
 int qd_set_launch_mode(qd_context *c, int mode) {
   if (!c || (mode != 0 && mode != 1)) return fail(QD_E_ARG, "launch mode must be 0 (grouped) or 1 (per channel)");
   c->launch_mode = mode;
+  drop_graphs(c);
+  return QD_OK;
+}
+
+int qd_set_precision(qd_context *c, int prec) {
+  if (!c || prec < QD_PREC_FP64 || prec > QD_PREC_BF16X4)
+    return fail(QD_E_ARG, "precision must be QD_PREC_FP64, QD_PREC_FP32 or QD_PREC_BF16X2..4");
+  c->gemm_prec = prec;  // QD_PREC_* equal the GemmPrec values
+  drop_graphs(c);
+  return QD_OK;
+}
+
+int qd_get_precision(qd_context *c, int *prec) {
+  if (!c || !prec) return fail(QD_E_ARG, "null argument");
+  *prec = c->gemm_prec;
   return QD_OK;
 }
 
 namespace {
 int size_class(int x) { return x <= 16 ? 0 : x <= 32 ? 1 : x <= 64 ? 2 : x <= 128 ? 3 : x <= 256 ? 4 : 5; }
 
-// device time (ms) of one grouped launch over the given problems with their current shapes
-int time_subset(qd_context *c, GemmList &g, const std::vector<int> &pis, const Slots &s, int reps, GemmTile *dtiles,
-                double *ms) {
-  std::vector<GemmTile> t;
-  for (int pi : pis) tiles_of(g, pi, t);
-  HIPC(hipMemcpy(g.probs, g.hprobs.data(), sizeof(GemmProb) * g.hprobs.size(), hipMemcpyHostToDevice));
-  HIPC(hipMemcpy(dtiles, t.data(), sizeof(GemmTile) * t.size(), hipMemcpyHostToDevice));
-  hipEvent_t a, b;
-  HIPC(hipEventCreate(&a));
-  HIPC(hipEventCreate(&b));
-  launch_gemm(c->stream, g.probs, g.segs, dtiles, (int)t.size(), s, c->gemm_grid);  // warm
-  HIPC(hipEventRecord(a, c->stream));
-  for (int r = 0; r < reps; r++) launch_gemm(c->stream, g.probs, g.segs, dtiles, (int)t.size(), s, c->gemm_grid);
-  HIPC(hipEventRecord(b, c->stream));
-  HIPC(hipEventSynchronize(b));
-  float f = 0;
-  HIPC(hipEventElapsedTime(&f, a, b));
-  hipEventDestroy(a);
-  hipEventDestroy(b);
-  *ms = f / reps;
-  return QD_OK;
-}
-
-int tune_list(qd_context *c, GemmList &g, const Slots &s, int reps, GemmTile *dtiles) {
-  std::map<int, std::vector<int>> buckets;
+// problems of a list grouped by size bucket (M, N, K classes), largest total work first
+std::vector<std::vector<int>> size_buckets(const GemmList &g) {
+  std::map<int, std::vector<int>> m;
   for (int pi = 0; pi < (int)g.hprobs.size(); pi++) {
     const GemmProb &p = g.hprobs[pi];
-    buckets[size_class(p.M) * 100 + size_class(p.N) * 10 + size_class(ktot(g, p))].push_back(pi);
+    m[size_class(p.M) * 100 + size_class(p.N) * 10 + size_class(ktot(g, p))].push_back(pi);
   }
-  for (auto &kv : buckets) {
-    int best = -1;
-    double best_ms = 1e300;
-    for (int sh = 0; sh < NSHAPE; sh++) {
-      for (int pi : kv.second) g.hprobs[pi].shape = sh;
-      double ms;
-      int rc = time_subset(c, g, kv.second, s, reps, dtiles, &ms);
-      if (rc) return rc;
-      if (ms < best_ms) { best_ms = ms; best = sh; }
-    }
-    for (int pi : kv.second) g.hprobs[pi].shape = best;
+  std::vector<std::pair<double, std::vector<int>>> v;
+  for (auto &kv : m) {
+    double w = 0;
+    for (int pi : kv.second) w += (double)g.hprobs[pi].M * g.hprobs[pi].N * ktot(g, g.hprobs[pi]);
+    v.push_back({w, kv.second});
   }
-  return upload_tiles(c, g);
+  std::sort(v.begin(), v.end(), [](const auto &x, const auto &y) { return x.first > y.first; });
+  std::vector<std::vector<int>> out;
+  for (auto &x : v) out.push_back(x.second);
+  return out;
 }
 }  // namespace
 
+// Coordinate descent over size buckets: for each bucket (largest work first)
+// every tile shape is tried with all other buckets as they stand, and the one
+// giving the shortest device time of the whole GEMM work of a commutator (both
+// grouped lists, as launched) is kept.  Timing the whole launch rather than a
+// bucket alone prices a shape by what it adds to the grouped launch.
 int qd_autotune(qd_context *c, int reps, double *ms_before, double *ms_after) {
   if (!c) return fail(QD_E_ARG, "null context");
   if (reps < 1) reps = 3;
   int rc;
+  if ((rc = drop_graphs(c))) return rc;
   qd_op *a, *b, *o;
   qd_cc *x, *y;
   if ((rc = qd_op_create(c, -1, &a)) || (rc = qd_op_create(c, 1, &b)) || (rc = qd_op_create(c, 1, &o))) return rc;
   if ((rc = qd_cc_create(c, -1.0, &x)) || (rc = qd_cc_create(c, 1.0, &y))) return rc;
   x->zero = y->zero = false;
   const Slots s1 = make_slots(c, a, b, o, x, y);
-  size_t cap = std::max(c->g_all.cap, c->g_acc.cap);
-  void *dt;
-  if ((rc = dev_alloc(c, sizeof(GemmTile) * cap, &dt))) return rc;
+  hipEvent_t e0, e1;
+  HIPC(hipEventCreate(&e0));
+  HIPC(hipEventCreate(&e1));
   auto total = [&](double *ms) -> int {
-    hipEvent_t e0, e1;
-    HIPC(hipEventCreate(&e0));
-    HIPC(hipEventCreate(&e1));
+    int r2;
+    if ((r2 = run_gemm_list(c, c->g_all, true, s1)) || (r2 = run_gemm_list(c, c->g_acc, false, s1))) return r2;  // warm
     HIPC(hipEventRecord(e0, c->stream));
     for (int r = 0; r < reps; r++) {
-      if ((rc = run_gemm_list(c, c->g_all, true, s1))) return rc;
-      if ((rc = run_gemm_list(c, c->g_acc, false, s1))) return rc;
+      if ((r2 = run_gemm_list(c, c->g_all, true, s1))) return r2;
+      if ((r2 = run_gemm_list(c, c->g_acc, false, s1))) return r2;
     }
     HIPC(hipEventRecord(e1, c->stream));
     HIPC(hipEventSynchronize(e1));
     float f = 0;
     HIPC(hipEventElapsedTime(&f, e0, e1));
-    hipEventDestroy(e0);
-    hipEventDestroy(e1);
     *ms = f / reps;
     return QD_OK;
   };
-  double before = 0, after = 0;
+  double before = 0, best_total = 0;
   if ((rc = total(&before))) return rc;
-  if ((rc = tune_list(c, c->g_all, s1, reps, (GemmTile *)dt))) return rc;
-  if ((rc = tune_list(c, c->g_acc, s1, reps, (GemmTile *)dt))) return rc;
+  best_total = before;
+  for (GemmList *g : {&c->g_all, &c->g_acc}) {
+    for (const auto &bk : size_buckets(*g)) {
+      const int cur = g->hprobs[bk[0]].shape;
+      int best = cur;
+      for (int sh = 0; sh < NSHAPE; sh++) {
+        if (sh == cur) continue;
+        for (int pi : bk) g->hprobs[pi].shape = sh;
+        if ((rc = upload_tiles(c, *g))) return rc;
+        double ms;
+        if ((rc = total(&ms))) return rc;
+        if (ms < best_total) { best_total = ms; best = sh; }
+      }
+      for (int pi : bk) g->hprobs[pi].shape = best;
+      if ((rc = upload_tiles(c, *g))) return rc;
+    }
+  }
+  double after = 0;
   if ((rc = total(&after))) return rc;
+  hipEventDestroy(e0);
+  hipEventDestroy(e1);
   if (ms_before) *ms_before = before;
   if (ms_after) *ms_after = after;
   qd_op_destroy(a); qd_op_destroy(b); qd_op_destroy(o); qd_cc_destroy(x); qd_cc_destroy(y);

@@ -8,9 +8,10 @@
 // that add into one sub-block become one pass.
 //
 // Each problem carries its own tile shape (chosen per size bucket by the
-// autotuner in capi.hip); the kernel dispatches on it at run time.  The launch
-// is persistent: a fixed grid strides over the tile list, which is sorted by
-// decreasing work.  Work-group = 256 threads = 4 waves; K is staged through
+// autotuner in capi.hip); every shape is its own kernel instantiation (own
+// register budget), launched once over that shape's part of the tile list.
+// The launches are persistent: a fixed grid strides over the tiles, which are
+// sorted by decreasing work.  Work-group = 256 threads = 4 waves; K is staged through
 // LDS in chunks of 16 (4 MFMA k-steps).  MFMA operand layout: A(16x4) lane l
 // holds A[l%16][l/16], B(4x16) lane l holds B[l/16][l%16], D(16x16) lane l
 // holds D[4*v + l/16][l%16] in component v (FP64: one row per group).
@@ -102,30 +103,268 @@ __device__ __forceinline__ void gemm_tile(const GemmProb &pr, const GemmSeg *__r
     }
 }
 
+// ---- split-precision tiles: FP32, or the mantissa cut into S BF16 slices
+//
+// x = x_0 + x_1 + ... + x_{S-1} + r, each x_s a BF16 number taken by
+// truncation from what the previous slices left (|r| < 2^(1-8S) |x|).  The
+// product keeps the slice pairs (i, j) with i + j < S, smallest first, all on
+// v_mfma_f32_16x16x32_bf16 (BF16 products are exact in FP32) into one FP32
+// accumulator per K stage of 32; each stage's FP32 sum is then added into an
+// FP64 accumulator, and the output update (beta) stays FP64.  FP32 mode rounds
+// the operands to FP32 and uses v_mfma_f32_16x16x4_f32 with the same staging.
+// FP32 MFMA output layout: lane l, component v holds D[4*(l/16) + v][l%16].
+typedef float f4 __attribute__((ext_vector_type(4)));
+typedef __bf16 bf8 __attribute__((ext_vector_type(8)));
+typedef unsigned u4 __attribute__((ext_vector_type(4)));
+constexpr int SPLIT_TK = 32;                    // K per staging round
+constexpr int SPLIT_LDW = (SPLIT_TK + 8) / 2;   // 32-bit words per staged row (16-byte aligned, padded)
+
+template <int S>
+__device__ __forceinline__ void bf16_slices(double v, unsigned (&h)[S]) {
+  double r = v;
+#pragma unroll
+  for (int s = 0; s < S; s++) {
+    const unsigned u = __float_as_uint((float)r) & 0xffff0000u;
+    h[s] = u >> 16;
+    if (s + 1 < S) r -= (double)__uint_as_float(u);
+  }
+}
+
+__device__ __forceinline__ double load_a(const GemmSeg &sg, const double *__restrict__ A, const GemmProb &pr, int gi,
+                                         int gk) {
+  if (gi >= pr.M || gk >= sg.K) return 0.0;
+  return sg.ta ? A[gk + (int64_t)gi * sg.lda] : A[gi + (int64_t)gk * sg.lda];
+}
+__device__ __forceinline__ double load_b(const GemmSeg &sg, const double *__restrict__ B, const GemmProb &pr, int gj,
+                                         int gk) {
+  if (gj >= pr.N || gk >= sg.K) return 0.0;
+  return sg.tb ? B[gj + (int64_t)gk * sg.ldb] : B[gk + (int64_t)gj * sg.ldb];
+}
+
+// C tile update from an FP64 accumulator in the FP32 MFMA output layout
+template <int FM, int FN>
+__device__ __forceinline__ void store_f32_layout(const GemmProb &pr, const Slots &slots, int r0, int c0, int lr, int lk,
+                                                 const d4 (&acc)[FM][FN]) {
+  double *__restrict__ C = slots.p[pr.c_slot] + pr.c_off;
+#pragma unroll
+  for (int x = 0; x < FM; x++)
+#pragma unroll
+    for (int y = 0; y < FN; y++) {
+      const int c = c0 + y * 16 + lr;
+#pragma unroll
+      for (int v = 0; v < 4; v++) {
+        const int r = r0 + x * 16 + 4 * lk + v;
+        if (r < pr.M && c < pr.N) {
+          double *p = C + r + (int64_t)c * pr.ldc;
+          *p = pr.beta1 ? *p + acc[x][y][v] : acc[x][y][v];
+        }
+      }
+    }
+}
+
+template <int TM, int TN, int WM, int WN, int S>
+__device__ __forceinline__ void gemm_tile_bf16(const GemmProb &pr, const GemmSeg *__restrict__ segs,
+                                               const Slots &slots, int tm, int tn, unsigned *__restrict__ lds) {
+  constexpr int FM = TM / WM / 16, FN = TN / WN / 16, KP = SPLIT_TK / 2;
+  unsigned *As = lds;                        // [S][TM][SPLIT_LDW]: slice s of A(i, k), k pairs packed
+  unsigned *Bs = lds + S * TM * SPLIT_LDW;   // [S][TN][SPLIT_LDW]: slice s of B(k, j)
+  const int row0 = tm * TM, col0 = tn * TN;
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int wm = wave % WM, wn = wave / WM;
+  const int lr = lane & 15, lk = lane >> 4;
+  const int m0 = wm * (TM / WM), n0 = wn * (TN / WN);
+  d4 acc[FM][FN];
+#pragma unroll
+  for (int a = 0; a < FM; a++)
+#pragma unroll
+    for (int b = 0; b < FN; b++) acc[a][b] = d4{0, 0, 0, 0};
+
+  for (int si = 0; si < pr.nseg; si++) {
+    const GemmSeg sg = segs[pr.seg0 + si];
+    const double *__restrict__ A = slots.p[sg.a_slot] + sg.a_off;
+    const double *__restrict__ B = slots.p[sg.b_slot] + sg.b_off;
+    for (int k0 = 0; k0 < sg.K; k0 += SPLIT_TK) {
+#pragma unroll
+      for (int r = 0; r < (TM * KP + 255) / 256; r++) {
+        const int idx = tid + 256 * r;
+        if (idx < TM * KP) {
+          int i, kp;
+          if (!sg.ta) { i = idx % TM; kp = idx / TM; }
+          else { kp = idx % KP; i = idx / KP; }
+          const int gi = row0 + i, gk = k0 + 2 * kp;
+          unsigned h0[S], h1[S];
+          bf16_slices<S>(sg.alpha * load_a(sg, A, pr, gi, gk), h0);
+          bf16_slices<S>(sg.alpha * load_a(sg, A, pr, gi, gk + 1), h1);
+#pragma unroll
+          for (int s = 0; s < S; s++) As[(s * TM + i) * SPLIT_LDW + kp] = h0[s] | (h1[s] << 16);
+        }
+      }
+#pragma unroll
+      for (int r = 0; r < (TN * KP + 255) / 256; r++) {
+        const int idx = tid + 256 * r;
+        if (idx < TN * KP) {
+          int j, kp;
+          if (!sg.tb) { kp = idx % KP; j = idx / KP; }
+          else { j = idx % TN; kp = idx / TN; }
+          const int gj = col0 + j, gk = k0 + 2 * kp;
+          unsigned h0[S], h1[S];
+          bf16_slices<S>(load_b(sg, B, pr, gj, gk), h0);
+          bf16_slices<S>(load_b(sg, B, pr, gj, gk + 1), h1);
+#pragma unroll
+          for (int s = 0; s < S; s++) Bs[(s * TN + j) * SPLIT_LDW + kp] = h0[s] | (h1[s] << 16);
+        }
+      }
+      __syncthreads();
+      bf8 a[S][FM], b[S][FN];
+#pragma unroll
+      for (int s = 0; s < S; s++) {
+#pragma unroll
+        for (int x = 0; x < FM; x++)
+          a[s][x] = __builtin_bit_cast(bf8, *(const u4 *)(As + (s * TM + m0 + x * 16 + lr) * SPLIT_LDW + 4 * lk));
+#pragma unroll
+        for (int y = 0; y < FN; y++)
+          b[s][y] = __builtin_bit_cast(bf8, *(const u4 *)(Bs + (s * TN + n0 + y * 16 + lr) * SPLIT_LDW + 4 * lk));
+      }
+#pragma unroll
+      for (int x = 0; x < FM; x++)
+#pragma unroll
+        for (int y = 0; y < FN; y++) {
+          f4 c = f4{0, 0, 0, 0};
+#pragma unroll
+          for (int d = S - 1; d >= 0; d--)
+#pragma unroll
+            for (int i = 0; i <= d; i++) c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[i][x], b[d - i][y], c, 0, 0, 0);
+#pragma unroll
+          for (int v = 0; v < 4; v++) acc[x][y][v] += (double)c[v];
+        }
+      __syncthreads();
+    }
+  }
+  store_f32_layout<FM, FN>(pr, slots, row0 + m0, col0 + n0, lr, lk, acc);
+}
+
+template <int TM, int TN, int WM, int WN>
+__device__ __forceinline__ void gemm_tile_f32(const GemmProb &pr, const GemmSeg *__restrict__ segs, const Slots &slots,
+                                              int tm, int tn, float *__restrict__ lds) {
+  constexpr int FM = TM / WM / 16, FN = TN / WN / 16;
+  float *As = lds;                          // [SPLIT_TK][TM + 1]
+  float *Bs = lds + SPLIT_TK * (TM + 1);    // [SPLIT_TK][TN + 1]
+  const int row0 = tm * TM, col0 = tn * TN;
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int wm = wave % WM, wn = wave / WM;
+  const int lr = lane & 15, lk = lane >> 4;
+  const int m0 = wm * (TM / WM), n0 = wn * (TN / WN);
+  d4 acc[FM][FN];
+#pragma unroll
+  for (int a = 0; a < FM; a++)
+#pragma unroll
+    for (int b = 0; b < FN; b++) acc[a][b] = d4{0, 0, 0, 0};
+
+  for (int si = 0; si < pr.nseg; si++) {
+    const GemmSeg sg = segs[pr.seg0 + si];
+    const double *__restrict__ A = slots.p[sg.a_slot] + sg.a_off;
+    const double *__restrict__ B = slots.p[sg.b_slot] + sg.b_off;
+    for (int k0 = 0; k0 < sg.K; k0 += SPLIT_TK) {
+#pragma unroll
+      for (int r = 0; r < (TM * SPLIT_TK + 255) / 256; r++) {
+        const int idx = tid + 256 * r;
+        if (idx < TM * SPLIT_TK) {
+          int i, k;
+          if (!sg.ta) { i = idx % TM; k = idx / TM; }
+          else { k = idx % SPLIT_TK; i = idx / SPLIT_TK; }
+          As[k * (TM + 1) + i] = (float)(sg.alpha * load_a(sg, A, pr, row0 + i, k0 + k));
+        }
+      }
+#pragma unroll
+      for (int r = 0; r < (TN * SPLIT_TK + 255) / 256; r++) {
+        const int idx = tid + 256 * r;
+        if (idx < TN * SPLIT_TK) {
+          int j, k;
+          if (!sg.tb) { k = idx % SPLIT_TK; j = idx / SPLIT_TK; }
+          else { j = idx % TN; k = idx / TN; }
+          Bs[k * (TN + 1) + j] = (float)load_b(sg, B, pr, col0 + j, k0 + k);
+        }
+      }
+      __syncthreads();
+      f4 c[FM][FN];
+#pragma unroll
+      for (int x = 0; x < FM; x++)
+#pragma unroll
+        for (int y = 0; y < FN; y++) c[x][y] = f4{0, 0, 0, 0};
+#pragma unroll
+      for (int kk = 0; kk < SPLIT_TK; kk += 4) {
+        float a[FM], b[FN];
+#pragma unroll
+        for (int x = 0; x < FM; x++) a[x] = As[(kk + lk) * (TM + 1) + m0 + x * 16 + lr];
+#pragma unroll
+        for (int y = 0; y < FN; y++) b[y] = Bs[(kk + lk) * (TN + 1) + n0 + y * 16 + lr];
+#pragma unroll
+        for (int x = 0; x < FM; x++)
+#pragma unroll
+          for (int y = 0; y < FN; y++) c[x][y] = __builtin_amdgcn_mfma_f32_16x16x4f32(a[x], b[y], c[x][y], 0, 0, 0);
+      }
+#pragma unroll
+      for (int x = 0; x < FM; x++)
+#pragma unroll
+        for (int y = 0; y < FN; y++)
+#pragma unroll
+          for (int v = 0; v < 4; v++) acc[x][y][v] += (double)c[x][y][v];
+      __syncthreads();
+    }
+  }
+  store_f32_layout<FM, FN>(pr, slots, row0 + m0, col0 + n0, lr, lk, acc);
+}
+
+// LDS bytes of one work-group for shape (TM, TN) and precision P
+constexpr int gemm_lds_bytes(int TM, int TN, int P) {
+  return P == GEMM_FP64  ? GEMM_TK * (TM + 1 + TN + 1) * 8
+         : P == GEMM_FP32 ? SPLIT_TK * (TM + 1 + TN + 1) * 4
+                          : P * (TM + TN) * SPLIT_LDW * 4;
+}
+
+template <int SH, int P>
 __global__ void __launch_bounds__(256) k_gemm(const GemmProb *__restrict__ probs, const GemmSeg *__restrict__ segs,
                                               const GemmTile *__restrict__ tiles, int ntiles, Slots slots) {
-  __shared__ double lds[GEMM_TK * (128 + 1) + GEMM_TK * (64 + 1) + 64];
+  constexpr int TM = SHAPE_TM[SH], TN = SHAPE_TN[SH];
+  constexpr int WM = TN <= 16 ? 4 : (TM <= 16 ? 1 : 2), WN = 4 / WM;
+  __shared__ __align__(16) unsigned char lds[gemm_lds_bytes(TM, TN, P)];
   for (int w = blockIdx.x; w < ntiles; w += gridDim.x) {
     const GemmTile tl = tiles[w];
     const GemmProb pr = probs[tl.prob];
-    switch (pr.shape) {
-      case SH_64x64: gemm_tile<64, 64, 2, 2>(pr, segs, slots, tl.tm, tl.tn, lds); break;
-      case SH_128x64: gemm_tile<128, 64, 2, 2>(pr, segs, slots, tl.tm, tl.tn, lds); break;
-      case SH_32x32: gemm_tile<32, 32, 2, 2>(pr, segs, slots, tl.tm, tl.tn, lds); break;
-      case SH_128x16: gemm_tile<128, 16, 4, 1>(pr, segs, slots, tl.tm, tl.tn, lds); break;
-      case SH_64x16: gemm_tile<64, 16, 4, 1>(pr, segs, slots, tl.tm, tl.tn, lds); break;
-      case SH_16x64: gemm_tile<16, 64, 1, 4>(pr, segs, slots, tl.tm, tl.tn, lds); break;
-      default: gemm_tile<16, 128, 1, 4>(pr, segs, slots, tl.tm, tl.tn, lds); break;
-    }
+    if constexpr (P == GEMM_FP64) gemm_tile<TM, TN, WM, WN>(pr, segs, slots, tl.tm, tl.tn, (double *)lds);
+    else if constexpr (P == GEMM_FP32) gemm_tile_f32<TM, TN, WM, WN>(pr, segs, slots, tl.tm, tl.tn, (float *)lds);
+    else gemm_tile_bf16<TM, TN, WM, WN, P>(pr, segs, slots, tl.tm, tl.tn, (unsigned *)lds);
     __syncthreads();  // the next tile reuses the LDS staging area
   }
 }
 
+template <int P>
+static void launch_gemm_p(hipStream_t s, const GemmProb *probs, const GemmSeg *segs, const GemmTile *tiles,
+                          int ntiles, const Slots &slots, int grid, int shape) {
+  switch (shape) {
+    case SH_64x64: hipLaunchKernelGGL((k_gemm<SH_64x64, P>), dim3(grid), dim3(256), 0, s, probs, segs, tiles, ntiles, slots); break;
+    case SH_128x64: hipLaunchKernelGGL((k_gemm<SH_128x64, P>), dim3(grid), dim3(256), 0, s, probs, segs, tiles, ntiles, slots); break;
+    case SH_32x32: hipLaunchKernelGGL((k_gemm<SH_32x32, P>), dim3(grid), dim3(256), 0, s, probs, segs, tiles, ntiles, slots); break;
+    case SH_128x16: hipLaunchKernelGGL((k_gemm<SH_128x16, P>), dim3(grid), dim3(256), 0, s, probs, segs, tiles, ntiles, slots); break;
+    case SH_64x16: hipLaunchKernelGGL((k_gemm<SH_64x16, P>), dim3(grid), dim3(256), 0, s, probs, segs, tiles, ntiles, slots); break;
+    case SH_16x64: hipLaunchKernelGGL((k_gemm<SH_16x64, P>), dim3(grid), dim3(256), 0, s, probs, segs, tiles, ntiles, slots); break;
+    case SH_16x128: hipLaunchKernelGGL((k_gemm<SH_16x128, P>), dim3(grid), dim3(256), 0, s, probs, segs, tiles, ntiles, slots); break;
+    default: break;
+  }
+}
+
 void launch_gemm(hipStream_t s, const GemmProb *probs, const GemmSeg *segs, const GemmTile *tiles, int ntiles,
-                 const Slots &slots, int grid) {
+                 const Slots &slots, int grid, int shape, int prec) {
   if (ntiles <= 0) return;
   if (grid <= 0 || grid > ntiles) grid = ntiles;
-  hipLaunchKernelGGL(k_gemm, dim3(grid), dim3(256), 0, s, probs, segs, tiles, ntiles, slots);
+  switch (prec) {
+    case GEMM_FP64: launch_gemm_p<GEMM_FP64>(s, probs, segs, tiles, ntiles, slots, grid, shape); break;
+    case GEMM_FP32: launch_gemm_p<GEMM_FP32>(s, probs, segs, tiles, ntiles, slots, grid, shape); break;
+    case GEMM_BF16X2: launch_gemm_p<GEMM_BF16X2>(s, probs, segs, tiles, ntiles, slots, grid, shape); break;
+    case GEMM_BF16X3: launch_gemm_p<GEMM_BF16X3>(s, probs, segs, tiles, ntiles, slots, grid, shape); break;
+    case GEMM_BF16X4: launch_gemm_p<GEMM_BF16X4>(s, probs, segs, tiles, ntiles, slots, grid, shape); break;
+    default: break;
+  }
 }
 
 }  // namespace qd

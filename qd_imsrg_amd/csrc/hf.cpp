@@ -254,8 +254,10 @@ double qd_v_int(const int *qi, const int *qj, const int *qk, const int *ql, doub
 int qd_hf_hamiltonian(int nbody, int nshell, double hw, double *h0, int64_t neq, double *eig_out, int *iters) {
   Channels ch;
   std::string why = build_channels(nbody, nshell, ch);
-  if (!why.empty() || !h0) return QD_E_LAYOUT;
-  if (neq != ch.neq) return QD_E_ARG;
+  if (!why.empty()) return set_error(QD_E_LAYOUT, "%s", why.c_str());
+  if (!h0) return set_error(QD_E_ARG, "h0 is null");
+  if (neq != ch.neq) return set_error(QD_E_ARG, "h0 holds %lld elements, the layout needs %lld", (long long)neq,
+                                      (long long)ch.neq);
   const int M = ch.M, n = ch.n;
   const Spatial &S = spatial_table(nshell);
   const double shw = std::sqrt(hw);
@@ -426,7 +428,7 @@ int qd_hf_hamiltonian(int nbody, int nshell, double hw, double *h0, int64_t neq,
   h0[0] = eh - vsum;  // HF_E2 (src/HFQDmod.f90:292-323)
   if (eig_out) std::memcpy(eig_out, eig.data(), sizeof(double) * M);
   if (iters) *iters = total;
-  if (crit > 1e-6) return QD_E_NUMERIC;
+  if (crit > 1e-6) return set_error(QD_E_NUMERIC, "Hartree-Fock did not converge (crit %g after %d iterations)", crit, total);
   return QD_OK;
 }
 

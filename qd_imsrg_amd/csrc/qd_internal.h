@@ -58,6 +58,14 @@ struct Channels {
 // Builds the tables; returns an empty string on success, else the reason.
 std::string build_channels(int nbody, int nshell, Channels &c);
 
+// Error text for qd_last_error (thread-local); returns code.
+int set_error(int code, const char *fmt, ...);
+// Accessors for sources outside capi.hip
+hipStream_t context_stream(qd_context *c);
+int64_t context_neq(const qd_context *c);
+qd_context *op_context(qd_op *o);
+double *op_data(qd_op *o);
+
 // Device copy of the tables, passed by value to kernels.
 struct DevTab {
   int n, M, mp, nblock, ncc, nch, ml0, mlspan;

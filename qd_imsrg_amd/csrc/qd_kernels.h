@@ -40,6 +40,9 @@ constexpr int GEMM_TK = 16;
 enum GemmShape { SH_64x64 = 0, SH_128x64, SH_32x32, SH_128x16, SH_64x16, SH_16x64, SH_16x128, NSHAPE };
 constexpr int SHAPE_TM[NSHAPE] = {64, 128, 32, 128, 64, 16, 16};
 constexpr int SHAPE_TN[NSHAPE] = {64, 64, 32, 16, 16, 64, 128};
+// arithmetic of the grouped GEMM: FP64, FP32, or BF16 mantissa slices (the
+// value is the slice count)
+enum GemmPrec { GEMM_FP64 = 0, GEMM_FP32 = 1, GEMM_BF16X2 = 2, GEMM_BF16X3 = 3, GEMM_BF16X4 = 4 };
 
 struct Ops {  // two operators and their hermiticity
   const double *v1, *v2;
@@ -51,9 +54,10 @@ enum { TB_122 = 1, TB_LADDER = 2, TB_PANDYA = 4, TB_ACCUM = 8 };
 // flags of the one-body kernel
 enum { OB_111 = 1, OB_121 = 2, OB_221 = 4, OB_ACCUM = 8 };
 
-// grid <= 0: one work-group per tile; else a persistent grid striding the list
+// All tiles must have the given shape.  grid <= 0: one work-group per tile;
+// else a persistent grid striding the list.
 void launch_gemm(hipStream_t s, const GemmProb *probs, const GemmSeg *segs, const GemmTile *tiles,
-                 int ntiles, const Slots &slots, int grid);
+                 int ntiles, const Slots &slots, int grid, int shape, int prec);
 void launch_twobody(hipStream_t s, const DevTab &t, const Ops &o, double *out, const Task2 *tasks,
                     int ntasks, int flags, const double *w1, const double *w2, const double *w3,
                     const double *P, double hh, double eh);

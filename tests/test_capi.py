@@ -53,3 +53,23 @@ def test_layout_rejects_open_shell():
     L = _lib.load()
     assert L.qd_layout_table(5, 4, 6, None, 0) == -3
     assert b"shell" in L.qd_last_error()
+
+
+@pytest.mark.parametrize("N,R", [(6, 6), (20, 14)])
+def test_layout_view_matches_oracle(N, R):
+    """The package's host layout view (library tables) against the oracle's
+    channel builder: dims, offsets, single-particle quantum numbers, the
+    reference flop count (C2: 10.02 GF) and the synthetic operator recipe."""
+    from qd_imsrg_amd.layout import Layout, random_operator
+    from tests.pins.layout import random_flat
+    lay, ch = Layout(N, R), Channels(N, R)
+    assert lay.neq == ch.neq and lay.nblock == ch.nblock and lay.ncc == ch.ncc and lay.cc_size == ch.cc_size
+    assert np.array_equal(lay.off, ch.off) and np.array_equal(lay.np_, ch.np_)
+    assert np.array_equal(lay.nh, ch.nh) and np.array_equal(lay.nb, ch.nb)
+    assert np.array_equal(lay.ml, ch.ml) and np.array_equal(lay.ms, ch.ms)
+    assert lay.reference_gemm_flops() == ch.gemm_flops_2b2b() and lay.n2b() == ch.n2b()
+    if (N, R) == (20, 14):
+        assert lay.reference_gemm_flops()["total"] == 10019392114
+    a = random_operator(lay, np.random.default_rng(5), -1)
+    b = random_flat(ch, np.random.default_rng(5), -1)
+    assert np.array_equal(a, b)

@@ -1,0 +1,97 @@
+"""Precision form: the 2b-2b GEMMs in FP32 or in 2..4 BF16 mantissa slices
+(FP32 MFMA accumulation per K stage of 32, FP64 across stages), everything
+else FP64.  Errors are measured against the plain FP64 GPU form on the same
+inputs and must stay inside the stated bounds:
+
+  commutator at N=20 R=14 (random X, Y), max |err| / max |[X,Y]_fp64|:
+      fp32, bf16x3, bf16x4 <= 1e-6;  bf16x2 <= 1e-3
+  Magnus flow at N=6 R=6 hw=1 (HF basis): |E_gs - E_gs(fp64)| <= 1e-6 Ha for
+      fp32, bf16x3, bf16x4; relative error of ||Omega|| <= 1e-4 and
+      absolute error of ||Gamma_od|| (small at convergence) <= 1e-5.
+
+The FP64 mode must stay bitwise what it was, and the zero-body part (FP64
+traces) is unaffected by the GEMM precision."""
+import numpy as np
+import pytest
+
+import qd_imsrg_amd as qd
+from oracle.channels import SUB_PPHH, Channels
+from tests.pins.layout import random_flat
+
+pytestmark = pytest.mark.gpu
+
+BOUND_COMM = {"fp32": 1e-6, "bf16x2": 1e-3, "bf16x3": 1e-6, "bf16x4": 1e-6}
+BOUND_E = 1e-6
+
+
+def rel(a, b):
+    return np.abs(a - b).max() / np.abs(b).max()
+
+
+def gamma_od_norm(ch, v):
+    """Frobenius norm of the off-diagonal part: f_ph and every Vpphh block."""
+    s = float(np.sum(v[ch.off_fph:ch.off_fph + ch.mp * ch.n] ** 2))
+    for q in range(ch.nblock):
+        k = int(ch.off[q, SUB_PPHH])
+        s += float(np.sum(v[k:k + int(ch.np_[q]) * int(ch.nh[q])] ** 2))
+    return s ** 0.5
+
+
+def full(ctx, x, y):
+    X, Y, Z = ctx.operator(-1, x), ctx.operator(1, y), ctx.operator(1)
+    HCC, ECC = ctx.cc(1.0), ctx.cc(-1.0)
+    qd.calc_cc(Y, HCC)
+    qd.calc_cc(X, ECC)
+    e0 = qd.commutator(X, Y, Z, HCC, ECC)
+    return e0, Z.download()
+
+
+def test_commutator_errors_per_precision():
+    print("[case] precision N=20 R=14 commutator", flush=True)
+    ch = Channels(20, 14)
+    ctx = qd.Context(20, 14)
+    rng = np.random.default_rng(41)
+    x, y = random_flat(ch, rng, -1), random_flat(ch, rng, 1)
+    e_ref, ref = full(ctx, x, y)
+    errs = {}
+    for p in ("fp32", "bf16x2", "bf16x3", "bf16x4"):
+        ctx.set_precision(p)
+        assert ctx.precision() == p
+        e0, z = full(ctx, x, y)
+        errs[p] = rel(z, ref)
+        assert e0 == e_ref  # zero-body traces stay FP64 and do not use the GEMMs
+    ctx.set_precision("fp64")
+    e_again, again = full(ctx, x, y)
+    ctx.close()
+    print("commutator max-rel error vs FP64:", {k: f"{v:.2e}" for k, v in errs.items()})
+    assert np.array_equal(again, ref) and e_again == e_ref
+    for p, e in errs.items():
+        assert e <= BOUND_COMM[p], (p, e)
+    assert errs["bf16x3"] < errs["bf16x2"]
+
+
+def test_magnus_flow_errors_per_precision():
+    print("[case] precision N=6 R=6 Magnus flow", flush=True)
+    ch = Channels(6, 6)
+    h0, _, _ = qd.hf_hamiltonian(6, 6, 1.0)
+    ctx = qd.Context(6, 6)
+    H0 = ctx.operator(1, h0)
+
+    def flow(p):
+        ctx.set_precision(p)
+        HS, OM = ctx.operator(1), ctx.operator(-1)
+        res, _ = qd.magnus_flow(ctx, H0, HS, OM)
+        return res, qd.mat_2_norm(OM), gamma_od_norm(ch, HS.download())
+
+    r64, om64, od64 = flow("fp64")
+    out = {}
+    for p in ("fp32", "bf16x2", "bf16x3", "bf16x4"):
+        r, om, od = flow(p)
+        out[p] = (abs(r.e0 - r64.e0), abs(om - om64) / om64, abs(od - od64), r.iterations)
+    ctx.close()
+    print(f"E_gs(fp64) = {r64.e0:.10f} in {r64.iterations} steps, ||Gamma_od|| = {od64:.3e}")
+    for p, (de, dom, dod, it) in out.items():
+        print(f"  {p}: |dE| = {de:.2e} Ha, rel ||Omega|| {dom:.2e}, abs ||Gamma_od|| {dod:.2e}, {it} steps")
+    for p in ("fp32", "bf16x3", "bf16x4"):
+        de, dom, dod, _ = out[p]
+        assert de <= BOUND_E and dom <= 1e-4 and dod <= 1e-5, (p, out[p])
