@@ -175,7 +175,15 @@ typedef struct {
   int pandya;        /* 0 as shipped, 1 with calc_cc-filled CC inputs                */
   int verbose;       /* 1: print the reference's per-iteration line to stdout;
                        2: the same line to stderr, flushed every iteration      */
-  int warm_start;    /* 1: start from the Omega passed in (H(s) = BCH(Omega, H(0)))  */
+  int warm_start;    /* 1: start from the Omega passed in (H(s) = BCH(Omega, H(0)));
+                       the step-rejection scale ||eta_0|| is then taken from
+                       H(0), as in a cold start                                 */
+  int diis;          /* 0: the reference's iteration.  m > 0 (at most 6): Pulay
+                       (DIIS) acceleration of the same map Omega -> Omega +
+                       stp eta(Omega) over the last m iterates, same stopping
+                       rule, no step rejection.  No counterpart in the
+                       reference; it makes warm starts converge in a few
+                       iterations where the plain map has a slow tail.        */
 } qd_flow_params;
 typedef struct {
   double s, e0, crit;          /* final flow parameter, E0(s), |MBPT2(H(s))|          */

@@ -36,9 +36,29 @@ def bch_expand_pandya(o: Oracle, g: np.ndarray, h: np.ndarray):
     return hs, 11
 
 
+def diis_weights(B: np.ndarray) -> np.ndarray | None:
+    """Pulay weights: minimise |sum c_i r_i|^2 with sum c_i = 1 (bordered
+    system of the Gram matrix B, scaled by its largest diagonal)."""
+    n = B.shape[0]
+    A = np.zeros((n + 1, n + 1))
+    A[:n, :n] = B / B.diagonal().max()
+    A[:n, n] = A[n, :n] = 1.0
+    rhs = np.zeros(n + 1)
+    rhs[n] = 1.0
+    try:
+        c = np.linalg.solve(A, rhs)[:n]
+    except np.linalg.LinAlgError:
+        return None
+    return c if np.all(np.isfinite(c)) else None
+
+
 def magnus_flow(o: Oracle, h0: np.ndarray, stp: float = R4_0P7, conv: float = R4_1EM9,
-                max_iter: int = 100000, omega0: np.ndarray | None = None, pandya: bool = False):
-    """Returns dict(hs, omega, s, e0, iterations, rejected, bch_terms, trace)."""
+                max_iter: int = 100000, omega0: np.ndarray | None = None, pandya: bool = False,
+                diis: int = 0):
+    """Returns dict(hs, omega, s, e0, iterations, rejected, bch_terms, trace).
+    diis > 0: Pulay acceleration over the last ``diis`` (<= 6) iterates of the
+    map Omega -> Omega + stp eta(Omega), no step rejection (the library's
+    qd_magnus_flow with qd_flow_params.diis; not in the reference)."""
     bch = (lambda g, h: bch_expand_pandya(o, g, h)) if pandya else (lambda g, h: o.bch_expand(g, -1, h))
     if omega0 is None:
         g = o.zeros()
@@ -49,11 +69,38 @@ def magnus_flow(o: Oracle, h0: np.ndarray, stp: float = R4_0P7, conv: float = R4
         g = omega0.copy()
         hs, terms = bch(g, h0)
         crit = abs(o.mbpt2(hs))
+    # step-rejection scale: from H(0) also for a warm start (the library's
+    # qd_magnus_flow does the same; the reference always starts cold)
+    nrm1 = o.norm(o.build_white(h0, 1))
     eta = o.build_white(hs, 1)
-    nrm1 = o.norm(eta)
     s = 0.0
     it = rej = steps = 0
     trace = []
+    if diis > 0:
+        m = min(diis, 6)
+        HO, HR, B, count = [None] * m, [None] * m, np.zeros((m, m)), 0
+        while crit > conv and steps < max_iter:
+            steps += 1
+            slot = count % m
+            HO[slot], HR[slot] = g.copy(), eta.copy()
+            count += 1
+            n = min(count, m)
+            for i in range(n):
+                B[slot, i] = B[i, slot] = float(np.dot(HR[slot][1:], HR[i][1:]))
+            c = diis_weights(B[:n, :n]) if n >= 2 else None
+            if c is None:
+                g = o.add(g, 1.0, eta, stp)
+            else:
+                g = sum(ci * HO[i] + (ci * stp) * HR[i] for i, ci in enumerate(c))
+            s += stp
+            hs, nt = bch(g, h0)
+            terms += nt
+            eta = o.build_white(hs, 1)
+            trace.append((it, s, hs[0], nrm1, o.norm(eta), crit))
+            crit = abs(o.mbpt2(hs))
+            it += 1
+        return dict(hs=hs, omega=g, s=s, e0=hs[0], iterations=it, rejected=0, bch_terms=terms,
+                    crit=crit, trace=np.array(trace).reshape(-1, 6))
     while crit > conv and steps < max_iter:
         steps += 1
         g0 = g.copy()

@@ -302,12 +302,14 @@ def dGam(H, DER, HCC, ETACC):
 
 def magnus_flow(ctx: Context, H0: Operator, HS: Operator, OMEGA: Operator, pandya: bool = False,
                 warm_start: bool = False, verbose: int = 0, max_iter: int = 100000,
-                conv: float | None = None, trace_len: int = 4096):
+                conv: float | None = None, trace_len: int = 4096, diis: int = 0):
     """s_evolve (src/magnus_IMSRG.f90:194-248) on the GPU; returns (result, trace).
-    verbose: 1 prints the reference's per-iteration line to stdout, 2 to stderr."""
+    verbose: 1 prints the reference's per-iteration line to stdout, 2 to stderr.
+    diis: depth of the Pulay acceleration (0 = the reference's iteration)."""
     p = FlowParams()
     ctx.L.qd_flow_params_default(C.byref(p))
     p.pandya, p.warm_start, p.verbose, p.max_iter = int(pandya), int(warm_start), int(verbose), int(max_iter)
+    p.diis = int(diis)
     if conv is not None:
         p.conv = float(conv)
     res = FlowResult()

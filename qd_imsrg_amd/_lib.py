@@ -24,7 +24,7 @@ class Info(C.Structure):
 
 class FlowParams(C.Structure):
     _fields_ = [("stp", C.c_double), ("conv", C.c_double), ("max_iter", C.c_int), ("pandya", C.c_int),
-                ("verbose", C.c_int), ("warm_start", C.c_int)]
+                ("verbose", C.c_int), ("warm_start", C.c_int), ("diis", C.c_int)]
 
 
 class FlowResult(C.Structure):

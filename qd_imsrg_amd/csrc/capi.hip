@@ -521,6 +521,47 @@ int full_commutator(qd_context *c, const qd_op *r1, const qd_op *r2, qd_op *r3, 
   return QD_OK;
 }
 
+// Pulay (DIIS) weights: minimise |sum_i c_i r_i|^2 subject to sum_i c_i = 1,
+// from the Gram matrix B (n x n, row-major) of the residuals.  Gaussian
+// elimination with partial pivoting on the bordered (n+1) system, B scaled
+// by its largest diagonal.  Returns false if the system is singular.
+bool diis_weights(int n, const double *B, double *c) {
+  const int m = n + 1;
+  std::vector<double> A((size_t)m * (m + 1), 0.0);
+  double sc = 0;
+  for (int i = 0; i < n; i++) sc = std::max(sc, B[i * n + i]);
+  if (!(sc > 0)) return false;
+  for (int i = 0; i < n; i++) {
+    for (int j = 0; j < n; j++) A[(size_t)i * (m + 1) + j] = B[i * n + j] / sc;
+    A[(size_t)i * (m + 1) + n] = 1.0;
+    A[(size_t)n * (m + 1) + i] = 1.0;
+  }
+  A[(size_t)n * (m + 1) + m] = 1.0;  // right-hand side (0, ..., 0, 1)
+  for (int k = 0; k < m; k++) {
+    int piv = k;
+    for (int i = k + 1; i < m; i++)
+      if (std::fabs(A[(size_t)i * (m + 1) + k]) > std::fabs(A[(size_t)piv * (m + 1) + k])) piv = i;
+    if (std::fabs(A[(size_t)piv * (m + 1) + k]) < 1e-14) return false;
+    if (piv != k)
+      for (int j = 0; j <= m; j++) std::swap(A[(size_t)k * (m + 1) + j], A[(size_t)piv * (m + 1) + j]);
+    for (int i = k + 1; i < m; i++) {
+      const double f = A[(size_t)i * (m + 1) + k] / A[(size_t)k * (m + 1) + k];
+      for (int j = k; j <= m; j++) A[(size_t)i * (m + 1) + j] -= f * A[(size_t)k * (m + 1) + j];
+    }
+  }
+  std::vector<double> x(m);
+  for (int i = m - 1; i >= 0; i--) {
+    double r = A[(size_t)i * (m + 1) + m];
+    for (int j = i + 1; j < m; j++) r -= A[(size_t)i * (m + 1) + j] * x[j];
+    x[i] = r / A[(size_t)i * (m + 1) + i];
+  }
+  for (int i = 0; i < n; i++) {
+    if (!std::isfinite(x[i])) return false;
+    c[i] = x[i];
+  }
+  return true;
+}
+
 int norm_into(qd_context *c, const qd_op *r, int slot) {
   Timer tm(c, TC_OTHER);
   launch_norm2(c->stream, r->d, c->ch.neq, c->partials, c->scal + slot);
@@ -971,6 +1012,7 @@ void qd_flow_params_default(qd_flow_params *p) {
   p->pandya = 0;
   p->verbose = 0;
   p->warm_start = 0;
+  p->diis = 0;
 }
 
 int qd_magnus_flow(qd_context *c, const qd_op *h0, qd_op *hs, qd_op *omega, const qd_flow_params *p,
@@ -1005,13 +1047,90 @@ int qd_magnus_flow(qd_context *c, const qd_op *h0, qd_op *hs, qd_op *omega, cons
     if ((rc = qd_set_to_zero(c, omega))) return done(rc);
     if ((rc = qd_copy_arrays(c, h0, hs))) return done(rc);
   }
-  if ((rc = qd_build_white(c, eta, hs))) return done(rc);
+  // nrm1 is the step-rejection scale (never updated, src/magnus_IMSRG.f90:
+  // 205-231).  A warm start takes it from H(0), the cold start's scale: the
+  // warm H(s) is nearly decoupled, and a tiny ||eta|| there would turn every
+  // later step into a rejection and a halving of the step.
+  if ((rc = qd_build_white(c, eta, p->warm_start ? h0 : hs))) return done(rc);
   if ((rc = qd_mat_2_norm(c, eta, &nrm1))) return done(rc);
+  if (p->warm_start && (rc = qd_build_white(c, eta, hs))) return done(rc);
   double m2;
   if ((rc = qd_mbpt2(c, p->warm_start ? hs : h0, &m2))) return done(rc);
   crit = std::fabs(m2);
   int steps = 0;
-  while (crit > p->conv && steps < p->max_iter) {
+  if (p->diis > 0) {
+    // Pulay-accelerated iteration of the same map Omega -> Omega + stp eta(Omega)
+    // (no counterpart in the reference): keep the last m (Omega_i, eta_i), take
+    // the combination of residuals eta_i with least norm (sum of weights 1),
+    // and step from the same combination of iterates.  Same stopping rule.
+    const int m = std::min(p->diis, 6);
+    std::vector<qd_op *> HO(m, nullptr), HR(m, nullptr);
+    for (int i = 0; i < m; i++) {
+      if ((rc = qd_op_create(c, -1, &HO[i])) || (rc = qd_op_create(c, -1, &HR[i]))) {
+        for (auto *o : HO) qd_op_destroy(o);
+        for (auto *o : HR) qd_op_destroy(o);
+        return done(rc);
+      }
+    }
+    auto done2 = [&](int r) {
+      for (auto *o : HO) qd_op_destroy(o);
+      for (auto *o : HR) qd_op_destroy(o);
+      return done(r);
+    };
+    std::vector<double> B((size_t)m * m, 0.0);
+    int count = 0;
+    while (crit > p->conv && steps < p->max_iter) {
+      steps++;
+      const int slot = count % m;
+      if ((rc = qd_copy_arrays(c, omega, HO[slot])) || (rc = qd_copy_arrays(c, eta, HR[slot]))) return done2(rc);
+      count++;
+      const int n = std::min(count, m);
+      for (int i = 0; i < n; i++) launch_dot(c->stream, HR[slot]->d, HR[i]->d, c->ch.neq, c->partials, c->scal + i);
+      HIPC(hipGetLastError());
+      double d[8];
+      if ((rc = read_scalars(c, n, d))) return done2(rc);
+      for (int i = 0; i < n; i++) B[(size_t)slot * m + i] = B[(size_t)i * m + slot] = d[i];
+      std::vector<double> Bn((size_t)n * n), w(n);
+      for (int i = 0; i < n; i++)
+        for (int j = 0; j < n; j++) Bn[(size_t)i * n + j] = B[(size_t)i * m + j];
+      LinComb lc{};
+      if (n >= 2 && diis_weights(n, Bn.data(), w.data())) {
+        for (int i = 0; i < n; i++) {
+          lc.c[lc.k] = w[i]; lc.x[lc.k++] = HO[i]->d;
+          lc.c[lc.k] = w[i] * stp; lc.x[lc.k++] = HR[i]->d;
+        }
+      } else {
+        lc.c[0] = 1.0; lc.x[0] = omega->d; lc.c[1] = stp; lc.x[1] = eta->d; lc.k = 2;
+      }
+      launch_lincomb(c->stream, c->ch.neq, lc, omega->d);
+      HIPC(hipGetLastError());
+      s += stp;
+      if ((rc = qd_bch_expand(c, hs, omega, h0, hcc, occ, &nt))) return done2(rc);
+      terms += nt;
+      if ((rc = qd_build_white(c, eta, hs))) return done2(rc);
+      if ((rc = qd_mat_2_norm(c, eta, &nrm2))) return done2(rc);
+      if ((rc = qd_mbpt2(c, hs, &m2))) return done2(rc);
+      HIPC(hipMemcpyAsync(c->host_scal, hs->d, sizeof(double), hipMemcpyDeviceToHost, c->stream));
+      HIPC(hipStreamSynchronize(c->stream));
+      e0 = c->host_scal[0];
+      if (!std::isfinite(e0) || !std::isfinite(nrm2) || !std::isfinite(m2))
+        return done2(fail(QD_E_NUMERIC, "non-finite E0, ||eta|| or MBPT2 at iteration %d", iteration));
+      if (p->verbose) {
+        FILE *vf = p->verbose == 2 ? stderr : stdout;
+        fprintf(vf, "%5d%18.9f%18.9f%18.9f%18.9f%18.9f\n", iteration, s, e0, nrm1, nrm2, crit);
+        if (p->verbose == 2) fflush(vf);
+      }
+      if (trace && rows < trace_len) {
+        double *r = trace + 6 * (size_t)rows++;
+        r[0] = iteration; r[1] = s; r[2] = e0; r[3] = nrm1; r[4] = nrm2; r[5] = crit;
+      }
+      crit = std::fabs(m2);
+      iteration++;
+    }
+    for (auto *o : HO) qd_op_destroy(o);
+    for (auto *o : HR) qd_op_destroy(o);
+  }
+  while (p->diis <= 0 && crit > p->conv && steps < p->max_iter) {
     steps++;
     if ((rc = qd_copy_arrays(c, omega, g0))) return done(rc);
     if ((rc = qd_magnus_expand(c, dg, omega, eta))) return done(rc);

@@ -508,6 +508,38 @@ void launch_norm2(hipStream_t s, const double *v, int64_t neq, double *partials,
   hipLaunchKernelGGL(k_sum_final, dim3(1), dim3(256), 0, s, partials, NPART, dst);
 }
 
+// dot product over [1, n) (E0 excluded, as in mat_2_norm), same partition
+__global__ void __launch_bounds__(256) k_dot_part(const double *__restrict__ a, const double *__restrict__ b,
+                                                  int64_t n, double *part) {
+  const int64_t chunk = (n - 1 + NPART - 1) / NPART;
+  const int64_t lo = 1 + (int64_t)blockIdx.x * chunk, hi = lo + chunk < n ? lo + chunk : n;
+  double s = 0.0;
+  for (int64_t k = lo + threadIdx.x; k < hi; k += 256) s += a[k] * b[k];
+  s = block_sum256(s);
+  if (threadIdx.x == 0) part[blockIdx.x] = s;
+}
+
+void launch_dot(hipStream_t s, const double *a, const double *b, int64_t neq, double *partials, double *dst) {
+  hipLaunchKernelGGL(k_dot_part, dim3(NPART), dim3(256), 0, s, a, b, neq, partials);
+  hipLaunchKernelGGL(k_sum_final, dim3(1), dim3(256), 0, s, partials, NPART, dst);
+}
+
+// z = sum_i c_i x_i over up to LINCOMB_MAX vectors, one pass
+__global__ void __launch_bounds__(256) k_lincomb(int64_t n, LinComb lc, double *__restrict__ z) {
+  for (int64_t e = (int64_t)blockIdx.x * 256 + threadIdx.x; e < n; e += (int64_t)gridDim.x * 256) {
+    double acc = 0.0;
+    for (int i = 0; i < lc.k; i++) acc = fma(lc.c[i], lc.x[i][e], acc);
+    z[e] = acc;
+  }
+}
+
+void launch_lincomb(hipStream_t s, int64_t n, const LinComb &lc, double *z) {
+  int64_t blocks = (n + 255) / 256;
+  if (blocks > 8192) blocks = 8192;
+  if (blocks < 1) blocks = 1;
+  hipLaunchKernelGGL(k_lincomb, dim3((unsigned)blocks), dim3(256), 0, s, n, lc, z);
+}
+
 // MBPT2 (src/HFQDmod.f90:325-362): sum Vpphh^2 / (f_h1 + f_h2 - f_p1 - f_p2)
 __global__ void __launch_bounds__(256) k_mbpt2_part(DevTab t, const double *__restrict__ v,
                                                     const Seg2 *__restrict__ segs,

@@ -75,6 +75,15 @@ void launch_pandya_combine(hipStream_t s, const DevTab &t, double *P, const Tile
 void launch_axpby(hipStream_t s, int64_t n, double s1, const double *x, double s2, const double *y, double *z);
 // sum of squares of v[1..neq) -> *dst, via partials (deterministic)
 void launch_norm2(hipStream_t s, const double *v, int64_t neq, double *partials, double *dst);
+// sum of a[k] b[k] over [1, neq) -> *dst (deterministic)
+void launch_dot(hipStream_t s, const double *a, const double *b, int64_t neq, double *partials, double *dst);
+constexpr int LINCOMB_MAX = 32;
+struct LinComb {  // z = sum_i c[i] x[i]
+  int k;
+  double c[LINCOMB_MAX];
+  const double *x[LINCOMB_MAX];
+};
+void launch_lincomb(hipStream_t s, int64_t n, const LinComb &lc, double *z);
 void launch_mbpt2(hipStream_t s, const DevTab &t, const double *v, const Seg2 *pphh_segs,
                   const Tile2 *pphh_tiles, int ntiles, double *partials, double *dst);
 void launch_scale_add_scalar(hipStream_t s, double *dst, const double *a, const double *b);

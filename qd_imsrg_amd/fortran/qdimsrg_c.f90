@@ -16,7 +16,7 @@ module qdimsrg_c
 
   type, bind(C) :: qd_flow_params
      real(c_double) :: stp, conv
-     integer(c_int) :: max_iter, pandya, verbose, warm_start
+     integer(c_int) :: max_iter, pandya, verbose, warm_start, diis
   end type qd_flow_params
 
   type, bind(C) :: qd_flow_result

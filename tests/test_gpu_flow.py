@@ -48,3 +48,31 @@ def test_warm_start_from_converged_omega():
     assert warm.converged and warm.iterations <= 1
     assert abs(warm.e0 - e_cold) < 1e-8
     ctx.close()
+
+
+@pytest.mark.parametrize("warm", [False, True])
+def test_diis_flow_matches_oracle(warm):
+    """Pulay-accelerated flow (qd_flow_params.diis = 6) against the oracle's
+    restatement, cold and from a perturbed converged Omega: same iteration
+    count, E_gs to 1e-10 relative, H(s) and Omega to 1e-9 relative (the
+    weights come from a small solve on rounded dot products, so the
+    trajectories agree to rounding, not bitwise)."""
+    print(f"[case] diis flow N=6 R=6 warm={warm}", flush=True)
+    ch = Channels(6, 6)
+    o = Oracle(ch)
+    h0 = model_hamiltonian(ch, 1)
+    om0 = None
+    if warm:
+        om0 = magnus_flow(o, h0)["omega"]
+        om0 = om0 * (1 + 0.01 * np.random.default_rng(5).standard_normal(om0.size))
+    ref = magnus_flow(o, h0, omega0=om0, diis=6)
+    ctx = qd.Context(6, 6)
+    H0, HS = ctx.operator(1, h0), ctx.operator(1)
+    OM = ctx.operator(-1, om0) if warm else ctx.operator(-1)
+    res, _ = qd.magnus_flow(ctx, H0, HS, OM, warm_start=warm, diis=6)
+    print(f"iterations {res.iterations} (oracle {ref['iterations']}), E {res.e0:.12f}")
+    assert res.converged and res.iterations == ref["iterations"]
+    assert abs(res.e0 - ref["e0"]) <= 1e-10 * abs(ref["e0"])
+    assert np.abs(HS.download() - ref["hs"]).max() <= 1e-9 * np.abs(ref["hs"]).max()
+    assert np.abs(OM.download() - ref["omega"]).max() <= 1e-9 * np.abs(ref["omega"]).max()
+    ctx.close()
