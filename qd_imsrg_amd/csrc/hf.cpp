@@ -8,6 +8,8 @@
 // get_ints (src/ME_general.f90:746-798) does, instead of reading ME.bin.
 // Eigenvector phases: largest-magnitude component positive.
 #include <algorithm>
+#include <array>
+#include <chrono>
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
@@ -16,6 +18,8 @@
 #include <mutex>
 #include <unordered_map>
 #include <vector>
+
+#include <omp.h>
 
 #include "qd_internal.h"
 
@@ -65,28 +69,48 @@ qf qgamma_half_rational(int twice) {
   return qfact(2 * xc) / (p4 * qfact(xc));
 }
 
+// I(g1, g2, g3, g4): the innermost triple sum of v_int, in binary128
+ld inner_value(int g1, int g2, int g3, int g4) {
+  const int G = g1 + g2 + g3 + g4;
+  // lam = 2(l1+l2) is even: Gamma(1+lam/2) is an integer and
+  // Gamma((G-lam+1)/2) carries sqrt(pi) exactly when G is even
+  qf s = 0;
+  for (int l1 = 0; l1 <= g1; l1++)
+    for (int l2 = 0; l2 <= g2; l2++)
+      for (int l3 = 0; l3 <= g3; l3++) {
+        const int l4 = l1 + l2 - l3;
+        if (l4 < 0 || l4 > g4) continue;
+        const int lam = l1 + l2 + l3 + l4;
+        const qf term = qbinom(g1, l1) * qbinom(g2, l2) * qbinom(g3, l3) * qbinom(g4, l4) *
+                        qgamma_half_rational(2 + lam) * qgamma_half_rational(G - lam + 1);
+        if ((g2 + g3 - l2 - l3) & 1) s -= term; else s += term;
+      }
+  ld r = (ld)s;
+  if ((G & 1) == 0) r *= 1.772453850905516027298167483341145182L;
+  return r;
+}
+
+uint64_t inner_key(int g1, int g2, int g3, int g4) {
+  return ((uint64_t)g1 << 48) | ((uint64_t)g2 << 32) | ((uint64_t)g3 << 16) | (uint64_t)g4;
+}
+
+// read-only table of I values, keys sorted (filled before a parallel build)
+struct InnerTable {
+  std::vector<uint64_t> keys;
+  std::vector<ld> vals;
+  const ld *find(uint64_t k) const {
+    auto it = std::lower_bound(keys.begin(), keys.end(), k);
+    return (it != keys.end() && *it == k) ? &vals[it - keys.begin()] : nullptr;
+  }
+};
+
 struct InnerCache {
   std::unordered_map<uint64_t, ld> memo;
   ld get(int g1, int g2, int g3, int g4) {
-    const uint64_t key = ((uint64_t)g1 << 48) | ((uint64_t)g2 << 32) | ((uint64_t)g3 << 16) | (uint64_t)g4;
+    const uint64_t key = inner_key(g1, g2, g3, g4);
     auto it = memo.find(key);
     if (it != memo.end()) return it->second;
-    const int G = g1 + g2 + g3 + g4;
-    // lam = 2(l1+l2) is even: Gamma(1+lam/2) is an integer and
-    // Gamma((G-lam+1)/2) carries sqrt(pi) exactly when G is even
-    qf s = 0;
-    for (int l1 = 0; l1 <= g1; l1++)
-      for (int l2 = 0; l2 <= g2; l2++)
-        for (int l3 = 0; l3 <= g3; l3++) {
-          const int l4 = l1 + l2 - l3;
-          if (l4 < 0 || l4 > g4) continue;
-          const int lam = l1 + l2 + l3 + l4;
-          const qf term = qbinom(g1, l1) * qbinom(g2, l2) * qbinom(g3, l3) * qbinom(g4, l4) *
-                          qgamma_half_rational(2 + lam) * qgamma_half_rational(G - lam + 1);
-          if ((g2 + g3 - l2 - l3) & 1) s -= term; else s += term;
-        }
-    ld r = (ld)s;
-    if ((G & 1) == 0) r *= 1.772453850905516027298167483341145182L;
+    const ld r = inner_value(g1, g2, g3, g4);
     memo.emplace(key, r);
     return r;
   }
@@ -97,24 +121,50 @@ InnerCache &cache() {
   return c;
 }
 
-double v_int_impl(const int *qi, const int *qj, const int *qk, const int *ql, double hw) {
+// The quadruple sum over (j1, j2, j3, j4) of the reference's v_int
+// (src/ME_general.f90:2205-2396) depends on j1, j4 only through j1 + j4 (in
+// g1, g4 and the sign) and on j2, j3 only through j2 + j3, so it is summed as
+// sum_{J14, J23} A(J14) B(J23) 2^{-(G+1)/2} I(g1, g2, g3, g4) with
+// A(J) = sum_{j1+j4=J} (-1)^J binom(n1+|m1|, n1-j1) binom(n4+|m4|, n4-j4) / (j1! j4!)
+// and B likewise: (n1+n4+1)(n2+n3+1) terms instead of (n1+1)(n2+1)(n3+1)(n4+1).
+// Keys (g1, g2, g3, g4) the sum below visits for this quadruple
+void inner_keys(const int *qi, const int *qj, const int *qk, const int *ql, std::vector<uint64_t> &out) {
+  if (qi[2] != ql[2] || qk[2] != qj[2]) return;
+  const int ai = std::abs(qi[1]), aj = std::abs(qj[1]), ak = std::abs(qk[1]), al = std::abs(ql[1]);
+  for (int J14 = 0; J14 <= qi[0] + ql[0]; J14++)
+    for (int J23 = 0; J23 <= qj[0] + qk[0]; J23++)
+      out.push_back(inner_key(J14 + (ai + qi[1]) / 2 + (al - ql[1]) / 2, J23 + (aj + qj[1]) / 2 + (ak - qk[1]) / 2,
+                              J23 + (aj - qj[1]) / 2 + (ak + qk[1]) / 2, J14 + (ai - qi[1]) / 2 + (al + ql[1]) / 2));
+}
+
+double v_int_impl(const int *qi, const int *qj, const int *qk, const int *ql, double hw,
+                  const InnerTable *tab = nullptr) {
   if (qi[2] != ql[2] || qk[2] != qj[2]) return 0.0;
   const int ai = std::abs(qi[1]), aj = std::abs(qj[1]), ak = std::abs(qk[1]), al = std::abs(ql[1]);
-  ld sm = 0;
+  ld A[64], B[64];
+  const int n14 = qi[0] + ql[0], n23 = qj[0] + qk[0];
+  for (int J = 0; J <= n14; J++) A[J] = 0;
+  for (int J = 0; J <= n23; J++) B[J] = 0;
   for (int j1 = 0; j1 <= qi[0]; j1++)
-    for (int j2 = 0; j2 <= qj[0]; j2++)
-      for (int j3 = 0; j3 <= qk[0]; j3++)
-        for (int j4 = 0; j4 <= ql[0]; j4++) {
-          const int g1 = j1 + j4 + (ai + qi[1]) / 2 + (al - ql[1]) / 2;
-          const int g4 = j1 + j4 + (ai - qi[1]) / 2 + (al + ql[1]) / 2;
-          const int g2 = j2 + j3 + (aj + qj[1]) / 2 + (ak - qk[1]) / 2;
-          const int g3 = j2 + j3 + (aj - qj[1]) / 2 + (ak + qk[1]) / 2;
-          const int G = g1 + g2 + g3 + g4;
-          const ld sg = ((j1 + j2 + j3 + j4) & 1) ? -1 : 1;
-          sm += sg / (fact(j1) * fact(j2) * fact(j3) * fact(j4)) * binom(qi[0] + ai, qi[0] - j1) *
-                binom(qj[0] + aj, qj[0] - j2) * binom(qk[0] + ak, qk[0] - j3) * binom(ql[0] + al, ql[0] - j4) *
-                std::pow((ld)0.5, (ld)(G + 1) * 0.5L) * cache().get(g1, g2, g3, g4);
-        }
+    for (int j4 = 0; j4 <= ql[0]; j4++)
+      A[j1 + j4] += (((j1 + j4) & 1) ? -1 : 1) / (fact(j1) * fact(j4)) * binom(qi[0] + ai, qi[0] - j1) *
+                    binom(ql[0] + al, ql[0] - j4);
+  for (int j2 = 0; j2 <= qj[0]; j2++)
+    for (int j3 = 0; j3 <= qk[0]; j3++)
+      B[j2 + j3] += (((j2 + j3) & 1) ? -1 : 1) / (fact(j2) * fact(j3)) * binom(qj[0] + aj, qj[0] - j2) *
+                    binom(qk[0] + ak, qk[0] - j3);
+  ld sm = 0;
+  for (int J14 = 0; J14 <= n14; J14++) {
+    const int g1 = J14 + (ai + qi[1]) / 2 + (al - ql[1]) / 2;
+    const int g4 = J14 + (ai - qi[1]) / 2 + (al + ql[1]) / 2;
+    for (int J23 = 0; J23 <= n23; J23++) {
+      const int g2 = J23 + (aj + qj[1]) / 2 + (ak - qk[1]) / 2;
+      const int g3 = J23 + (aj - qj[1]) / 2 + (ak + qk[1]) / 2;
+      const int G = g1 + g2 + g3 + g4;
+      const ld *hit = tab ? tab->find(inner_key(g1, g2, g3, g4)) : nullptr;
+      sm += A[J14] * B[J23] * std::pow((ld)0.5, (ld)(G + 1) * 0.5L) * (hit ? *hit : cache().get(g1, g2, g3, g4));
+    }
+  }
   const ld nrm = fact(qi[0]) / fact(qi[0] + ai) * fact(qj[0]) / fact(qj[0] + aj) * fact(qk[0]) /
                  fact(qk[0] + ak) * fact(ql[0]) / fact(ql[0] + al);
   return (double)(std::sqrt((ld)hw) * sm * std::sqrt(nrm));
@@ -209,21 +259,84 @@ void build_spatial(int nshell, Spatial &S) {
         if (md >= -S.mmax && md <= S.mmax) k += (int64_t)S.by_m[md + S.mmax].size();
       }
   S.tab.assign(k, 0.0);
-#pragma omp parallel for schedule(dynamic, 1) collapse(2)
+  // <ab|V|cd> is invariant under (a,b,c,d) -> (b,a,d,c) (particle exchange),
+  // -> (c,d,a,b) (real and symmetric) and under m -> -m on all four orbitals
+  // (time reversal); each orbit of this group of 8 is evaluated once, at its
+  // lexicographically smallest member, and copied to the others.
+  std::vector<int> rev(N);
   for (int a = 0; a < N; a++)
     for (int b = 0; b < N; b++)
-      for (int c = 0; c < N; c++) {
-        const int md = S.m[a] + S.m[b] - S.m[c];
-        if (md < -S.mmax || md > S.mmax) continue;
-        const std::vector<int> &ds = S.by_m[md + S.mmax];
-        for (size_t kd = 0; kd < ds.size(); kd++) {
-          const int d = ds[kd];
-          // <ab|V|cd> = v_int(a, b, d, c) with equal spins
-          const int qa[3] = {S.n[a], S.m[a], 1}, qb[3] = {S.n[b], S.m[b], 1};
-          const int qc[3] = {S.n[c], S.m[c], 1}, qd[3] = {S.n[d], S.m[d], 1};
-          S.tab[S.start[((size_t)a * N + b) * N + c] + kd] = v_int_impl(qa, qb, qd, qc, 1.0);
+      if (S.n[b] == S.n[a] && S.m[b] == -S.m[a]) rev[a] = b;
+  auto canon = [&](int a, int b, int c, int d) {
+    std::array<int, 4> best{a, b, c, d};
+    const std::array<int, 4> base[2] = {{a, b, c, d}, {rev[a], rev[b], rev[c], rev[d]}};
+    for (const auto &t : base) {
+      const std::array<int, 4> im[4] = {{t[0], t[1], t[2], t[3]}, {t[1], t[0], t[3], t[2]},
+                                        {t[2], t[3], t[0], t[1]}, {t[3], t[2], t[1], t[0]}};
+      for (const auto &x : im) best = std::min(best, x);
+    }
+    return best;
+  };
+  auto at = [&](int a, int b, int c, int d) -> double & {
+    return S.tab[S.start[((size_t)a * N + b) * N + c] + S.pos_in_m[d]];
+  };
+  // every inner sum I(g1..g4) the owned elements need, computed once (in
+  // parallel) before the elements themselves
+  InnerTable itab;
+  {
+    std::vector<std::vector<uint64_t>> per(omp_get_max_threads());
+#pragma omp parallel for schedule(dynamic, 1) collapse(2)
+    for (int a = 0; a < N; a++)
+      for (int b = 0; b < N; b++)
+        for (int c = 0; c < N; c++) {
+          const int md = S.m[a] + S.m[b] - S.m[c];
+          if (md < -S.mmax || md > S.mmax) continue;
+          auto &out = per[omp_get_thread_num()];
+          for (int d : S.by_m[md + S.mmax]) {
+            if (canon(a, b, c, d) != std::array<int, 4>{a, b, c, d}) continue;
+            const int qa[3] = {S.n[a], S.m[a], 1}, qb[3] = {S.n[b], S.m[b], 1};
+            const int qc[3] = {S.n[c], S.m[c], 1}, qd[3] = {S.n[d], S.m[d], 1};
+            inner_keys(qa, qb, qd, qc, out);
+          }
+          if (out.size() > (1u << 20)) {  // keep the per-thread lists short
+            std::sort(out.begin(), out.end());
+            out.erase(std::unique(out.begin(), out.end()), out.end());
+          }
         }
-      }
+    for (auto &v : per) {
+      itab.keys.insert(itab.keys.end(), v.begin(), v.end());
+      std::vector<uint64_t>().swap(v);
+    }
+    std::sort(itab.keys.begin(), itab.keys.end());
+    itab.keys.erase(std::unique(itab.keys.begin(), itab.keys.end()), itab.keys.end());
+    itab.vals.resize(itab.keys.size());
+#pragma omp parallel for schedule(dynamic, 64)
+    for (size_t i = 0; i < itab.keys.size(); i++) {
+      const uint64_t k = itab.keys[i];
+      itab.vals[i] = inner_value((int)(k >> 48), (int)((k >> 32) & 0xffff), (int)((k >> 16) & 0xffff), (int)(k & 0xffff));
+    }
+  }
+  for (int pass = 0; pass < 2; pass++) {
+#pragma omp parallel for schedule(dynamic, 1) collapse(2)
+    for (int a = 0; a < N; a++)
+      for (int b = 0; b < N; b++)
+        for (int c = 0; c < N; c++) {
+          const int md = S.m[a] + S.m[b] - S.m[c];
+          if (md < -S.mmax || md > S.mmax) continue;
+          for (int d : S.by_m[md + S.mmax]) {
+            const std::array<int, 4> cn = canon(a, b, c, d);
+            const bool own = cn == std::array<int, 4>{a, b, c, d};
+            if (pass == 0 && own) {
+              // <ab|V|cd> = v_int(a, b, d, c) with equal spins
+              const int qa[3] = {S.n[a], S.m[a], 1}, qb[3] = {S.n[b], S.m[b], 1};
+              const int qc[3] = {S.n[c], S.m[c], 1}, qd[3] = {S.n[d], S.m[d], 1};
+              at(a, b, c, d) = v_int_impl(qa, qb, qd, qc, 1.0, &itab);
+            } else if (pass == 1 && !own) {
+              at(a, b, c, d) = at(cn[0], cn[1], cn[2], cn[3]);
+            }
+          }
+        }
+  }
 }
 
 // one table per shell count per process (hw enters as sqrt(hw), applied on use)
@@ -259,7 +372,12 @@ int qd_hf_hamiltonian(int nbody, int nshell, double hw, double *h0, int64_t neq,
   if (neq != ch.neq) return set_error(QD_E_ARG, "h0 holds %lld elements, the layout needs %lld", (long long)neq,
                                       (long long)ch.neq);
   const int M = ch.M, n = ch.n;
+  const bool trace = std::getenv("QD_HF_TRACE") != nullptr;
+  auto now = [] { return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count(); };
+  double t_mark = now();
   const Spatial &S = spatial_table(nshell);
+  if (trace) fprintf(stderr, "hf: Coulomb table (R=%d) %.2f s\n", nshell, now() - t_mark);
+  t_mark = now();
   const double shw = std::sqrt(hw);
   // spin-orbital -> spatial orbital (nm_index order), spin
   std::vector<int> sp(M), spin(M);
@@ -350,12 +468,14 @@ int qd_hf_hamiltonian(int nbody, int nshell, double hw, double *h0, int64_t neq,
     }
     crit = 0;
     for (int i = 0; i < n; i++) crit += std::fabs(eold[i] - eig[i]);
-    if (std::getenv("QD_HF_TRACE")) {
+    if (trace) {
       double e = 0;
       for (int i = 0; i < n; i++) e += eig[i];
       fprintf(stderr, "hf iter %d crit %.3e sum_eocc %.10f\n", it, crit, e);
     }
   }
+  if (trace) fprintf(stderr, "hf: %d SCF iterations %.2f s\n", total, now() - t_mark);
+  t_mark = now();
   // F_HF = C^T F C (F from the last Fock build)
   std::vector<double> FC((size_t)M * M, 0.0), FHF((size_t)M * M, 0.0);
   for (int i = 0; i < M; i++)
@@ -385,26 +505,40 @@ int qd_hf_hamiltonian(int nbody, int nshell, double hw, double *h0, int64_t neq,
     const int nh = ch.nh[q], nb = ch.nb[q], np = ch.np[q], nt = nh + nb + np;
     if (nt == 0) continue;
     const uint32_t *P = &ch.pairs[ch.hh_start[q]];  // hh, ph, pp are contiguous
-    std::vector<double> V((size_t)nt * nt), T((size_t)nt * nt), TV((size_t)nt * nt, 0.0), H((size_t)nt * nt, 0.0);
+    // T[x][z] = C[b1,a1] C[b2,a2] - C[b2,a1] C[b1,a2] for HF pair x = (a1,a2) and
+    // oscillator pair z = (b1,b2): non-zero only for b1, b2 in the channels of
+    // a1, a2 (either order), so each row is built from those channels alone
+    std::vector<double> V((size_t)nt * nt), TV((size_t)nt * nt, 0.0), H((size_t)nt * nt, 0.0);
+    std::vector<std::vector<std::pair<int, double>>> T(nt);
+    auto comb = [&](int i, int j) {  // combined (hh | ph | pp) index of pair i < j in block q
+      const uint32_t ia = ch.iarr[(size_t)i * M - (size_t)i * (i + 1) / 2 + (j - i - 1)];
+      const int t = ia_type(ia), k = ia_idx(ia);
+      return t == T_HH ? k : (t == T_PH ? nh + k : nh + nb + k);
+    };
     for (int x = 0; x < nt; x++)
       for (int y = 0; y < nt; y++) V[(size_t)x * nt + y] = asym(lo16(P[x]), hi16(P[x]), lo16(P[y]), hi16(P[y]));
-    for (int x = 0; x < nt; x++) {  // HF pair x, HO pair y
+    for (int x = 0; x < nt; x++) {
       const int a1 = lo16(P[x]), a2 = hi16(P[x]);
-      for (int y = 0; y < nt; y++) {
-        const int b1 = lo16(P[y]), b2 = hi16(P[y]);
-        T[(size_t)x * nt + y] = C[(size_t)b1 * M + a1] * C[(size_t)b2 * M + a2] - C[(size_t)b2 * M + a1] * C[(size_t)b1 * M + a2];
-      }
+      const auto &c1 = chan[ch.spch[a1]], &c2 = chan[ch.spch[a2]];
+      for (int b1 : c1)
+        for (int b2 : c2) {
+          if (b1 == b2) continue;
+          const int lo = std::min(b1, b2), hi = std::max(b1, b2);
+          if (ch.spch[b1] == ch.spch[b2] && b1 > b2) continue;  // same channel: each unordered pair once
+          const double t = C[(size_t)lo * M + a1] * C[(size_t)hi * M + a2] - C[(size_t)hi * M + a1] * C[(size_t)lo * M + a2];
+          if (t != 0.0) T[x].push_back({comb(lo, hi), t});
+        }
     }
     for (int x = 0; x < nt; x++)
-      for (int z = 0; z < nt; z++) {
-        const double t = T[(size_t)x * nt + z];
-        if (t == 0.0) continue;
-        for (int y = 0; y < nt; y++) TV[(size_t)x * nt + y] += t * V[(size_t)z * nt + y];
+      for (const auto &zt : T[x]) {
+        const double *v = &V[(size_t)zt.first * nt];
+        double *tv = &TV[(size_t)x * nt];
+        for (int y = 0; y < nt; y++) tv[y] += zt.second * v[y];
       }
     for (int x = 0; x < nt; x++)
       for (int y = 0; y < nt; y++) {
         double s = 0;
-        for (int z = 0; z < nt; z++) s += TV[(size_t)x * nt + z] * T[(size_t)y * nt + z];
+        for (const auto &zt : T[y]) s += TV[(size_t)x * nt + zt.first] * zt.second;
         H[(size_t)x * nt + y] = s;
       }
     const int64_t *o = &ch.off[(size_t)q * 6];
@@ -420,6 +554,7 @@ int qd_hf_hamiltonian(int nbody, int nshell, double hw, double *h0, int64_t neq,
     put(PPPH, p, np, x, nb);
     put(PHHH, x, nb, h, nh);
   }
+  if (trace) fprintf(stderr, "hf: basis transform %.2f s\n", now() - t_mark);
   double vsum = 0;  // sum over blocks of the Vhhhh diagonal, in block order
   for (int q = 0; q < ch.nblock; q++)
     for (int j = 0; j < ch.nh[q]; j++) vsum += h0[ch.off[(size_t)q * 6 + HHHH] + j + (int64_t)j * ch.nh[q]];
