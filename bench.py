@@ -190,6 +190,83 @@ def batched_flow(nbody=20, nshell=16, hw=0.5):
     return res
 
 
+def learned_form(nbody=20, nshell=14, n_omega=64, lo=0.1, hi=1.0, steps=20000, lr=1e-3):
+    """Learned warm start at N=20, R=14: converged FP64 Magnus Omegas (cold
+    start, as shipped) over n_omega log-spaced frequencies in [lo, hi]; every
+    4th (k = 2 mod 4) is held out.  PCA (rank 64 requested) + FP32 MLP
+    (omega, log omega -> 3 x 256 tanh -> PCA coefficients) trained by the
+    library's kernels; held-out flows continue from the predicted Omega with
+    the Pulay-accelerated iteration.  Reported per held-out frequency: cold
+    steps (as shipped), cold steps with the same acceleration (what the
+    network adds beyond it), warm steps, |dE| against the cold E_gs, and
+    wall times; a warm flow that fails falls back to the cold result."""
+    import qd_imsrg_amd as qd
+    from qd_imsrg_amd.warmstart import WarmStart
+    omegas = lo * (hi / lo) ** (np.arange(n_omega) / (n_omega - 1))
+    ctx = qd.Context(nbody, nshell)
+    ctx.autotune(reps=2)
+    ctx.set_graphs(True)
+    runs = {}
+    t_sweep = time.perf_counter()
+    for k, w in enumerate(omegas):
+        try:
+            h0, _, _ = qd.hf_hamiltonian(nbody, nshell, float(w))
+            H0, HS, OM = ctx.operator(1, h0), ctx.operator(1), ctx.operator(-1)
+            t0 = time.perf_counter()
+            r, _ = qd.magnus_flow(ctx, H0, HS, OM)
+            t = time.perf_counter() - t0
+            if not r.converged:
+                raise qd.QDError("not converged")
+            runs[k] = dict(h0=h0, e0=r.e0, steps=r.iterations, s=t, omega=OM.download())
+            progress(f"sweep omega {w:.4f}: E={r.e0:.9f}, {r.iterations} steps, {t:.2f} s")
+        except qd.QDError as e:
+            progress(f"sweep omega {w:.4f}: skipped ({e})")
+    t_sweep = time.perf_counter() - t_sweep
+    held = [k for k in range(n_omega) if k % 4 == 2 and k in runs]
+    train = [k for k in range(n_omega) if k % 4 != 2 and k in runs]
+    ws = WarmStart(ctx, rank=64, hidden=256, layers=3, seed=2026)
+    fit = ws.fit(omegas[train], [runs[k]["omega"] for k in train], steps=steps, lr=lr)
+    progress(f"fit: {fit}")
+    out = []
+    for k in held:
+        H0 = ctx.operator(1, runs[k]["h0"])
+        HS, OM = ctx.operator(1), ctx.operator(-1)
+        t0 = time.perf_counter()
+        rd, _ = qd.magnus_flow(ctx, H0, HS, OM, diis=6, max_iter=500)
+        t_cd = time.perf_counter() - t0
+        HS, OM = ctx.operator(1), ctx.operator(-1)
+        t0 = time.perf_counter()
+        ws.predict(omegas[k], OM)
+        fallback = False
+        try:
+            rw, _ = qd.magnus_flow(ctx, H0, HS, OM, warm_start=True, diis=6, max_iter=100)
+            ok = rw.converged
+        except qd.QDError:
+            ok = False
+        t_w = time.perf_counter() - t0
+        if not ok:  # cold start as the fallback
+            fallback = True
+            steps_w, e_w, t_w = runs[k]["steps"] + 100, runs[k]["e0"], t_w + runs[k]["s"]
+        else:
+            steps_w, e_w = rw.iterations, rw.e0
+        rec = dict(omega=float(omegas[k]), cold_steps=runs[k]["steps"], cold_s=runs[k]["s"],
+                   cold_diis_steps=rd.iterations, cold_diis_s=t_cd, warm_steps=steps_w, warm_s=t_w,
+                   abs_dE_warm_vs_cold=abs(e_w - runs[k]["e0"]), fallback=fallback)
+        progress(f"held-out omega {omegas[k]:.4f}: cold {rec['cold_steps']}, cold+DIIS {rec['cold_diis_steps']}, "
+                 f"warm {steps_w}, |dE| {rec['abs_dE_warm_vs_cold']:.2e}")
+        out.append(rec)
+    ws.close()
+    ctx.close()
+    mean = lambda key: float(np.mean([r[key] for r in out])) if out else None
+    return {"workload": f"N={nbody} R={nshell}, {n_omega} log-spaced omega in [{lo}, {hi}], "
+                        f"{len(train)} train / {len(held)} held out (k = 2 mod 4)",
+            "sweep_s": t_sweep, "fit": fit, "held_out": out,
+            "mean_steps": {"cold": mean("cold_steps"), "cold_diis": mean("cold_diis_steps"), "warm": mean("warm_steps")},
+            "mean_flow_s": {"cold": mean("cold_s"), "cold_diis": mean("cold_diis_s"), "warm": mean("warm_s")},
+            "max_abs_dE": max((r["abs_dE_warm_vs_cold"] for r in out), default=None),
+            "fallbacks": sum(r["fallback"] for r in out)}
+
+
 def breakdown(ctx, X, Y, HCC, ECC, reps=10):
     """Device time of each reference entry point on its own (HIP events)."""
     import qd_imsrg_amd as qd
@@ -255,6 +332,8 @@ def main():
     ap.add_argument("--no-flow", action="store_true", help="skip the N=6 R=6 time-to-E_gs flows")
     ap.add_argument("--batched", action="store_true",
                     help="also run the N=20 R=16 batched-form flow against the per-channel path")
+    ap.add_argument("--learned", action="store_true",
+                    help="also run the N=20 R=14 learned warm-start sweep (64 frequencies)")
     ap.add_argument("--precision", action="store_true",
                     help="also run the N=42 R=20 precision-form flows (FP64, FP32, BF16 slices)")
     a = ap.parse_args()
@@ -347,6 +426,8 @@ def main():
         out["batched_form"] = batched_flow()
     if a.precision:
         out["precision_form"] = precision_form()
+    if a.learned:
+        out["learned_form"] = learned_form()
     if not a.no_cpu:
         out["cpu_baseline"] = cpu_baseline(x, y, a.nbody, a.nshell, fl["total"], flows)
     print(json.dumps(out))

@@ -178,12 +178,15 @@ typedef struct {
   int warm_start;    /* 1: start from the Omega passed in (H(s) = BCH(Omega, H(0)));
                        the step-rejection scale ||eta_0|| is then taken from
                        H(0), as in a cold start                                 */
-  int diis;          /* 0: the reference's iteration.  m > 0 (at most 6): Pulay
-                       (DIIS) acceleration of the same map Omega -> Omega +
-                       stp eta(Omega) over the last m iterates, same stopping
-                       rule, no step rejection.  No counterpart in the
-                       reference; it makes warm starts converge in a few
-                       iterations where the plain map has a slow tail.        */
+  int diis;          /* 0: the reference's iteration.  m > 0 (at most 6): once
+                       |MBPT2(H(s))| <= 1e-4, Pulay (DIIS) acceleration of the
+                       same map Omega -> Omega + stp eta(Omega) over the last m
+                       iterates (history dropped when ||eta|| grows past twice
+                       its smallest value since); before that, and after a
+                       drop, the reference's step and rejection rule.  Same
+                       stopping rule.  No counterpart in the reference; it
+                       removes the slow tail of the plain map near decoupling,
+                       which is what a warm start lands in.                    */
 } qd_flow_params;
 typedef struct {
   double s, e0, crit;          /* final flow parameter, E0(s), |MBPT2(H(s))|          */
@@ -200,6 +203,42 @@ void qd_flow_params_default(qd_flow_params *p);
  * iteration, s, E0, nrm1, nrm2, crit) receives the per-step log. */
 int qd_magnus_flow(qd_context *ctx, const qd_op *h0, qd_op *hs, qd_op *omega,
                    const qd_flow_params *p, qd_flow_result *res, double *trace, int trace_len);
+
+/* ---- the ODE path --------------------------------------------------------
+ * The reference's gs_IMSRG integrates dH/ds = dGam(H) (src/gs_IMSRG.f90:
+ * 223-273) with the Shampine-Gordon Adams PECE code (src/adams_ode.f90).
+ * qd_ode holds that integrator's state (the reference's work/iwork) on the
+ * device; qd_ode_integrate is ode(dGam, neq, y, H, HCC, ETACC, t, tout,
+ * relerr, abserr, iflag, work, iwork) (src/adams_ode.f90:7) with y a
+ * device operator: on iflag = 1 y is the initial value, on return it holds
+ * the solution at *t (= tout when iflag = 2).  iflag, relerr, abserr and t
+ * follow the reference: 2 success, 3 tolerances raised, 4 over 500 steps,
+ * 5 also looks stiff, 6 bad input. */
+typedef struct qd_ode qd_ode;
+int qd_ode_create(qd_context *ctx, qd_ode **out);
+int qd_ode_destroy(qd_ode *o);
+int qd_ode_integrate(qd_ode *o, qd_op *y, double *t, double tout, double *relerr, double *abserr, int *iflag);
+/* successful steps, right-hand side evaluations, last order, next step size */
+int qd_ode_stats(const qd_ode *o, int *nsteps, int64_t *nfe, int *order, double *h);
+/* gs_IMSRG's decoupling loop (src/gs_IMSRG.f90:100-120): ode over output
+ * intervals of stp until |(E_old - E_new)/E_old| over one interval is below
+ * conv.  Defaults are the reference's REAL(4) literals widened: relerr 1e-8,
+ * abserr 1e-6, stp 0.1, conv 1e-7.  hs receives H(s); trace (optional,
+ * trace_len rows of 4: output, s, E0, crit) the reference's per-output line. */
+typedef struct {
+  double relerr, abserr, stp, conv;
+  int max_outputs;
+  int verbose;       /* 1: the reference's per-output line to stdout, 2: stderr */
+} qd_gs_params;
+typedef struct {
+  double s, e0, crit;
+  int outputs, nsteps;
+  int64_t nfe;
+  int iflag, converged;
+} qd_gs_result;
+void qd_gs_params_default(qd_gs_params *p);
+int qd_gs_flow(qd_context *ctx, const qd_op *h0, qd_op *hs, const qd_gs_params *p, qd_gs_result *res,
+               double *trace, int trace_len);
 
 /* ---- batched form: launch strategy of the grouped GEMM ------------------
  * The GEMM problems of all (M_L, M_S) blocks run as one grouped persistent

@@ -36,6 +36,9 @@ def bch_expand_pandya(o: Oracle, g: np.ndarray, h: np.ndarray):
     return hs, 11
 
 
+DIIS_START = 1e-4
+
+
 def diis_weights(B: np.ndarray) -> np.ndarray | None:
     """Pulay weights: minimise |sum c_i r_i|^2 with sum c_i = 1 (bordered
     system of the Gram matrix B, scaled by its largest diagonal)."""
@@ -56,8 +59,10 @@ def magnus_flow(o: Oracle, h0: np.ndarray, stp: float = R4_0P7, conv: float = R4
                 max_iter: int = 100000, omega0: np.ndarray | None = None, pandya: bool = False,
                 diis: int = 0):
     """Returns dict(hs, omega, s, e0, iterations, rejected, bch_terms, trace).
-    diis > 0: Pulay acceleration over the last ``diis`` (<= 6) iterates of the
-    map Omega -> Omega + stp eta(Omega), no step rejection (the library's
+    diis > 0: once |MBPT2| <= DIIS_START, Pulay acceleration over the last
+    ``diis`` (<= 6) iterates of the map Omega -> Omega + stp eta(Omega)
+    (history dropped when ||eta|| grows past twice its smallest value since);
+    the reference's step with its rejection rule otherwise (the library's
     qd_magnus_flow with qd_flow_params.diis; not in the reference)."""
     bch = (lambda g, h: bch_expand_pandya(o, g, h)) if pandya else (lambda g, h: o.bch_expand(g, -1, h))
     if omega0 is None:
@@ -76,11 +81,11 @@ def magnus_flow(o: Oracle, h0: np.ndarray, stp: float = R4_0P7, conv: float = R4
     s = 0.0
     it = rej = steps = 0
     trace = []
-    if diis > 0:
-        m = min(diis, 6)
-        HO, HR, B, count = [None] * m, [None] * m, np.zeros((m, m)), 0
-        while crit > conv and steps < max_iter:
-            steps += 1
+    m = min(max(diis, 0), 6)
+    HO, HR, B, count, best = [None] * max(m, 1), [None] * max(m, 1), np.zeros((max(m, 1),) * 2), 0, 0.0
+    while crit > conv and steps < max_iter:
+        steps += 1
+        if m > 0 and crit <= DIIS_START:
             slot = count % m
             HO[slot], HR[slot] = g.copy(), eta.copy()
             count += 1
@@ -96,13 +101,16 @@ def magnus_flow(o: Oracle, h0: np.ndarray, stp: float = R4_0P7, conv: float = R4
             hs, nt = bch(g, h0)
             terms += nt
             eta = o.build_white(hs, 1)
-            trace.append((it, s, hs[0], nrm1, o.norm(eta), crit))
+            nrm2 = o.norm(eta)
+            trace.append((it, s, hs[0], nrm1, nrm2, crit))
+            if count == 1 or nrm2 < best:
+                best = nrm2
+            if nrm2 > 2.0 * best:
+                count = 0
             crit = abs(o.mbpt2(hs))
             it += 1
-        return dict(hs=hs, omega=g, s=s, e0=hs[0], iterations=it, rejected=0, bch_terms=terms,
-                    crit=crit, trace=np.array(trace).reshape(-1, 6))
-    while crit > conv and steps < max_iter:
-        steps += 1
+            continue
+        count = 0
         g0 = g.copy()
         dg = eta.copy()                       # MAGNUS_EXPAND as shipped
         g = o.add(g, 1.0, dg, stp)            # euler_step

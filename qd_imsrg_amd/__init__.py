@@ -318,6 +318,54 @@ def magnus_flow(ctx: Context, H0: Operator, HS: Operator, OMEGA: Operator, pandy
     return res, trace[:res.trace_rows]
 
 
+class ODE:
+    """ode(dGam, ...) of src/adams_ode.f90:7 with the integrator's state on
+    the device; ``integrate`` advances operator ``y`` from t to tout."""
+
+    def __init__(self, ctx: Context):
+        self.L = ctx.L
+        h = C.c_void_p()
+        check(self.L.qd_ode_create(ctx.h, C.byref(h)))
+        self.h = h
+        self.t, self.relerr, self.abserr, self.iflag = 0.0, 0.0, 0.0, 1
+
+    def integrate(self, y: Operator, t: float, tout: float, relerr: float, abserr: float, iflag: int):
+        tt, re, ab, fl = C.c_double(t), C.c_double(relerr), C.c_double(abserr), C.c_int(iflag)
+        check(self.L.qd_ode_integrate(self.h, y.h, C.byref(tt), float(tout), C.byref(re), C.byref(ab), C.byref(fl)))
+        return tt.value, re.value, ab.value, fl.value
+
+    def stats(self):
+        ns, nfe, k, h = C.c_int(), C.c_int64(), C.c_int(), C.c_double()
+        check(self.L.qd_ode_stats(self.h, C.byref(ns), C.byref(nfe), C.byref(k), C.byref(h)))
+        return dict(nsteps=ns.value, nfe=nfe.value, order=k.value, h=h.value)
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.L.qd_ode_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def gs_flow(ctx: Context, H0: Operator, HS: Operator, verbose: int = 0, max_outputs: int = 100000,
+            trace_len: int = 4096, **overrides):
+    """gs_IMSRG's decoupling loop (src/gs_IMSRG.f90:100-120) on the GPU;
+    returns (result, trace rows (output, s, E0, crit))."""
+    p = _lib.GsParams()
+    ctx.L.qd_gs_params_default(C.byref(p))
+    p.verbose, p.max_outputs = int(verbose), int(max_outputs)
+    for k, v in overrides.items():
+        setattr(p, k, v)
+    res = _lib.GsResult()
+    trace = np.zeros((trace_len, 4))
+    check(ctx.L.qd_gs_flow(ctx.h, H0.h, HS.h, C.byref(p), C.byref(res), _dp(trace), trace_len))
+    return res, trace[:min(res.outputs, trace_len)]
+
+
 # ---- input side (host) ------------------------------------------------------
 def v_int(qi, qj, qk, ql, hw: float = 1.0) -> float:
     """<i j|V|l k> of the analytic oscillator Coulomb interaction (host)."""

@@ -33,6 +33,16 @@ class FlowResult(C.Structure):
                 ("trace_rows", C.c_int)]
 
 
+class GsParams(C.Structure):
+    _fields_ = [("relerr", C.c_double), ("abserr", C.c_double), ("stp", C.c_double), ("conv", C.c_double),
+                ("max_outputs", C.c_int), ("verbose", C.c_int)]
+
+
+class GsResult(C.Structure):
+    _fields_ = [("s", C.c_double), ("e0", C.c_double), ("crit", C.c_double), ("outputs", C.c_int),
+                ("nsteps", C.c_int), ("nfe", C.c_int64), ("iflag", C.c_int), ("converged", C.c_int)]
+
+
 # name -> (restype, argtypes); the exported surface of include/qdimsrg.h
 SIGNATURES = {
     "qd_last_error": (C.c_char_p, []),
@@ -83,6 +93,12 @@ SIGNATURES = {
     "qd_set_graphs": (C.c_int, [vp, C.c_int]),
     "qd_autotune": (C.c_int, [vp, C.c_int, dp, dp]),
     "qd_set_precision": (C.c_int, [vp, C.c_int]),
+    "qd_ode_create": (C.c_int, [vp, C.POINTER(vp)]),
+    "qd_ode_destroy": (C.c_int, [vp]),
+    "qd_ode_integrate": (C.c_int, [vp, vp, dp, C.c_double, dp, dp, ip]),
+    "qd_ode_stats": (C.c_int, [vp, ip, i64p, ip, dp]),
+    "qd_gs_params_default": (None, [C.POINTER(GsParams)]),
+    "qd_gs_flow": (C.c_int, [vp, vp, vp, C.POINTER(GsParams), C.POINTER(GsResult), dp, C.c_int]),
     "qd_mlp_create": (C.c_int, [vp, C.c_int, C.c_int, C.c_int, C.c_int, C.c_uint64, C.POINTER(vp)]),
     "qd_mlp_destroy": (C.c_int, [vp]),
     "qd_mlp_num_params": (C.c_int64, [vp]),

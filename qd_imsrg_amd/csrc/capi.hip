@@ -10,6 +10,7 @@
 #include <string>
 #include <vector>
 
+#include "ode.h"
 #include "qd_kernels.h"
 
 using namespace qd;
@@ -49,6 +50,10 @@ const double R4_1EM5 = 9.999999747378752e-06;   // conv, src/magnus_IMSRG.f90:25
 const double R4_1EM9 = 9.999999717180685e-10;   // conv_criteria, src/magnus_IMSRG.f90:10
 const double R4_0P7 = 0.699999988079071;        // stp = .7, src/magnus_IMSRG.f90:124
 const double R4_1EM3 = 0.0010000000474974513;   // + 1e-3, src/magnus_IMSRG.f90:226
+const double R4_1EM8 = 9.99999993922529e-09;    // rel = 1e-8, src/gs_IMSRG.f90:92
+const double R4_1EM6 = 9.999999974752427e-07;   // abse = 1e-6, src/gs_IMSRG.f90:93
+const double R4_0P1 = 0.10000000149011612;      // stp = 0.1, src/gs_IMSRG.f90:97
+const double R4_1EM7 = 1.0000000116860974e-07;  // conv_criteria, src/gs_IMSRG.f90:13
 
 constexpr int NCLASS = 6;
 enum { TC_PRODUCTS = 0, TC_ACCUM = 1, TC_TWOBODY = 2, TC_ONEBODY = 3, TC_CALCCC = 4, TC_OTHER = 5 };
@@ -78,6 +83,8 @@ struct qd_context {
   std::vector<void *> allocs;
   double *w1 = nullptr, *w2 = nullptr, *w3 = nullptr, *P = nullptr;
   double *u1 = nullptr, *u2 = nullptr;  // cross-coupled mat-vecs of commutator_121 (M*M each)
+  double2 *sbra = nullptr, *sket = nullptr;  // 122 coefficients per substitution-list entry
+  int npairs = 0;
   double *partials = nullptr, *scal = nullptr;  // reduction partials, 8 device scalars
   double *host_scal = nullptr;                  // pinned mirror of scal
   Task2 *tasks2 = nullptr; int ntasks2 = 0;
@@ -484,7 +491,9 @@ int run_twobody(qd_context *c, const qd_op *r1, const qd_op *r2, qd_op *out, int
   Ops o{r1->d, r2->d, r1->herm, r2->herm};
   double hh = hcc ? hcc->herm : 1.0, eh = ecc ? ecc->herm : -1.0;
   Timer tm(c, TC_TWOBODY);
-  launch_twobody(c->stream, c->dt, o, out->d, c->tasks2, c->ntasks2, flags, c->w1, c->w2, c->w3, c->P, hh, eh);
+  if (flags & TB_122) launch_subcoef(c->stream, c->dt, o, c->sbra, c->sket, c->npairs);
+  launch_twobody(c->stream, c->dt, o, out->d, c->tasks2, c->ntasks2, flags, c->w1, c->w2, c->w3, c->P, c->sbra,
+                 c->sket, hh, eh);
   HIPC(hipGetLastError());
   return QD_OK;
 }
@@ -642,6 +651,11 @@ int qd_context_create(int nbody, int nshell, qd_context **out) {
   if ((rc = dev_alloc(c, sizeof(double) * ch.neq, &p))) return bail(rc); c->w2 = (double *)p;
   if ((rc = dev_alloc(c, sizeof(double) * ch.neq, &p))) return bail(rc); c->w3 = (double *)p;
   if ((rc = dev_alloc(c, sizeof(double) * ch.p_size, &p))) return bail(rc); c->P = (double *)p;
+  if ((rc = dev_alloc(c, sizeof(double2) * std::max<size_t>(ch.sub_list.size(), 1), &p))) return bail(rc);
+  c->sbra = (double2 *)p;
+  if ((rc = dev_alloc(c, sizeof(double2) * std::max<size_t>(ch.sub_list.size(), 1), &p))) return bail(rc);
+  c->sket = (double2 *)p;
+  c->npairs = (int)ch.pairs.size();
   if ((rc = dev_alloc(c, sizeof(double) * ch.M * ch.M, &p))) return bail(rc); c->u1 = (double *)p;
   if ((rc = dev_alloc(c, sizeof(double) * ch.M * ch.M, &p))) return bail(rc); c->u2 = (double *)p;
   if ((rc = dev_alloc(c, sizeof(double) * 1024 * 64, &p))) return bail(rc); c->partials = (double *)p;
@@ -917,6 +931,7 @@ int qd_bch_expand(qd_context *c, qd_op *hs, const qd_op *g, const qd_op *h, cons
     if ((rc = qd_calc_cc(c, g, ccg))) return rc;
     ccg->herm = occ->herm;
     cca->herm = hcc->herm;
+    ccg->zero = cca->zero = false;  // cca is refilled per term by the launch below
   }
   // HS = H ; AD_0 = H ; then AD_k = [G, AD_{k-1}], HS += cof(k) AD_k
   if ((rc = qd_copy_arrays(c, h, hs))) return rc;
@@ -1058,29 +1073,49 @@ int qd_magnus_flow(qd_context *c, const qd_op *h0, qd_op *hs, qd_op *omega, cons
   if ((rc = qd_mbpt2(c, p->warm_start ? hs : h0, &m2))) return done(rc);
   crit = std::fabs(m2);
   int steps = 0;
-  if (p->diis > 0) {
-    // Pulay-accelerated iteration of the same map Omega -> Omega + stp eta(Omega)
-    // (no counterpart in the reference): keep the last m (Omega_i, eta_i), take
-    // the combination of residuals eta_i with least norm (sum of weights 1),
-    // and step from the same combination of iterates.  Same stopping rule.
-    const int m = std::min(p->diis, 6);
-    std::vector<qd_op *> HO(m, nullptr), HR(m, nullptr);
-    for (int i = 0; i < m; i++) {
-      if ((rc = qd_op_create(c, -1, &HO[i])) || (rc = qd_op_create(c, -1, &HR[i]))) {
-        for (auto *o : HO) qd_op_destroy(o);
-        for (auto *o : HR) qd_op_destroy(o);
-        return done(rc);
-      }
+  // Pulay (DIIS) acceleration (no counterpart in the reference): once the
+  // flow is close to decoupling (crit <= DIIS_START), keep the last m
+  // iterates (Omega_i, eta_i) of the map Omega -> Omega + stp eta(Omega) and
+  // step from the combination of iterates whose residuals eta_i have the
+  // least norm (weights summing to 1).  Further out, and whenever a step of
+  // it lets ||eta|| grow past twice the smallest value seen since it took
+  // over (the history is then dropped), the reference's step with its
+  // rejection rule is taken.  Same stopping rule.
+  const double DIIS_START = 1e-4;
+  const int m = std::min(std::max(p->diis, 0), 6);
+  std::vector<qd_op *> HO(m, nullptr), HR(m, nullptr);
+  auto free_hist = [&]() {
+    for (auto *o : HO) qd_op_destroy(o);
+    for (auto *o : HR) qd_op_destroy(o);
+  };
+  for (int i = 0; i < m; i++)
+    if ((rc = qd_op_create(c, -1, &HO[i])) || (rc = qd_op_create(c, -1, &HR[i]))) { free_hist(); return done(rc); }
+  auto done2 = [&](int r) { free_hist(); return done(r); };
+  std::vector<double> B((size_t)std::max(m, 1) * std::max(m, 1), 0.0);
+  int count = 0;
+  double best = 0.0;
+  auto log_step = [&]() {
+    if (p->verbose) {
+      FILE *vf = p->verbose == 2 ? stderr : stdout;
+      fprintf(vf, "%5d%18.9f%18.9f%18.9f%18.9f%18.9f\n", iteration, s, e0, nrm1, nrm2, crit);
+      if (p->verbose == 2) fflush(vf);
     }
-    auto done2 = [&](int r) {
-      for (auto *o : HO) qd_op_destroy(o);
-      for (auto *o : HR) qd_op_destroy(o);
-      return done(r);
-    };
-    std::vector<double> B((size_t)m * m, 0.0);
-    int count = 0;
-    while (crit > p->conv && steps < p->max_iter) {
-      steps++;
+    if (trace && rows < trace_len) {
+      double *r = trace + 6 * (size_t)rows++;
+      r[0] = iteration; r[1] = s; r[2] = e0; r[3] = nrm1; r[4] = nrm2; r[5] = crit;
+    }
+  };
+  auto read_e0 = [&]() -> int {
+    HIPC(hipMemcpyAsync(c->host_scal, hs->d, sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    HIPC(hipStreamSynchronize(c->stream));
+    e0 = c->host_scal[0];
+    if (!std::isfinite(e0) || !std::isfinite(nrm2))
+      return fail(QD_E_NUMERIC, "non-finite E0 or ||eta|| at s=%g", s);
+    return QD_OK;
+  };
+  while (crit > p->conv && steps < p->max_iter) {
+    steps++;
+    if (m > 0 && crit <= DIIS_START) {
       const int slot = count % m;
       if ((rc = qd_copy_arrays(c, omega, HO[slot])) || (rc = qd_copy_arrays(c, eta, HR[slot]))) return done2(rc);
       count++;
@@ -1109,51 +1144,27 @@ int qd_magnus_flow(qd_context *c, const qd_op *h0, qd_op *hs, qd_op *omega, cons
       terms += nt;
       if ((rc = qd_build_white(c, eta, hs))) return done2(rc);
       if ((rc = qd_mat_2_norm(c, eta, &nrm2))) return done2(rc);
+      if ((rc = read_e0())) return done2(rc);
+      log_step();
+      if (count == 1 || nrm2 < best) best = nrm2;
+      if (nrm2 > 2.0 * best) count = 0;  // drop the history
       if ((rc = qd_mbpt2(c, hs, &m2))) return done2(rc);
-      HIPC(hipMemcpyAsync(c->host_scal, hs->d, sizeof(double), hipMemcpyDeviceToHost, c->stream));
-      HIPC(hipStreamSynchronize(c->stream));
-      e0 = c->host_scal[0];
-      if (!std::isfinite(e0) || !std::isfinite(nrm2) || !std::isfinite(m2))
-        return done2(fail(QD_E_NUMERIC, "non-finite E0, ||eta|| or MBPT2 at iteration %d", iteration));
-      if (p->verbose) {
-        FILE *vf = p->verbose == 2 ? stderr : stdout;
-        fprintf(vf, "%5d%18.9f%18.9f%18.9f%18.9f%18.9f\n", iteration, s, e0, nrm1, nrm2, crit);
-        if (p->verbose == 2) fflush(vf);
-      }
-      if (trace && rows < trace_len) {
-        double *r = trace + 6 * (size_t)rows++;
-        r[0] = iteration; r[1] = s; r[2] = e0; r[3] = nrm1; r[4] = nrm2; r[5] = crit;
-      }
       crit = std::fabs(m2);
       iteration++;
+      continue;
     }
-    for (auto *o : HO) qd_op_destroy(o);
-    for (auto *o : HR) qd_op_destroy(o);
-  }
-  while (p->diis <= 0 && crit > p->conv && steps < p->max_iter) {
-    steps++;
-    if ((rc = qd_copy_arrays(c, omega, g0))) return done(rc);
-    if ((rc = qd_magnus_expand(c, dg, omega, eta))) return done(rc);
-    if ((rc = qd_euler_step(c, omega, dg, &s, stp))) return done(rc);
-    if ((rc = qd_copy_arrays(c, hs, hs0))) return done(rc);
-    if ((rc = qd_bch_expand(c, hs, omega, h0, hcc, occ, &nt))) return done(rc);
+    count = 0;
+    if ((rc = qd_copy_arrays(c, omega, g0))) return done2(rc);
+    if ((rc = qd_magnus_expand(c, dg, omega, eta))) return done2(rc);
+    if ((rc = qd_euler_step(c, omega, dg, &s, stp))) return done2(rc);
+    if ((rc = qd_copy_arrays(c, hs, hs0))) return done2(rc);
+    if ((rc = qd_bch_expand(c, hs, omega, h0, hcc, occ, &nt))) return done2(rc);
     terms += nt;
-    if ((rc = qd_copy_arrays(c, eta, eta0))) return done(rc);
-    if ((rc = qd_build_white(c, eta, hs))) return done(rc);
-    if ((rc = qd_mat_2_norm(c, eta, &nrm2))) return done(rc);
-    HIPC(hipMemcpyAsync(c->host_scal, hs->d, sizeof(double), hipMemcpyDeviceToHost, c->stream));
-    HIPC(hipStreamSynchronize(c->stream));
-    e0 = c->host_scal[0];
-    if (!std::isfinite(e0) || !std::isfinite(nrm2)) return done(fail(QD_E_NUMERIC, "non-finite E0 or ||eta|| at s=%g", s));
-    if (p->verbose) {
-      FILE *vf = p->verbose == 2 ? stderr : stdout;
-      fprintf(vf, "%5d%18.9f%18.9f%18.9f%18.9f%18.9f\n", iteration, s, e0, nrm1, nrm2, crit);
-      if (p->verbose == 2) fflush(vf);
-    }
-    if (trace && rows < trace_len) {
-      double *r = trace + 6 * (size_t)rows++;
-      r[0] = iteration; r[1] = s; r[2] = e0; r[3] = nrm1; r[4] = nrm2; r[5] = crit;
-    }
+    if ((rc = qd_copy_arrays(c, eta, eta0))) return done2(rc);
+    if ((rc = qd_build_white(c, eta, hs))) return done2(rc);
+    if ((rc = qd_mat_2_norm(c, eta, &nrm2))) return done2(rc);
+    if ((rc = read_e0())) return done2(rc);
+    log_step();
     if (nrm1 * 1.5 < nrm2) {
       if (p->verbose) fprintf(p->verbose == 2 ? stderr : stdout, " faillll\n");
       s -= stp;
@@ -1164,10 +1175,11 @@ int qd_magnus_flow(qd_context *c, const qd_op *h0, qd_op *hs, qd_op *omega, cons
       rejected++;
       continue;
     }
-    if ((rc = qd_mbpt2(c, hs, &m2))) return done(rc);
+    if ((rc = qd_mbpt2(c, hs, &m2))) return done2(rc);
     crit = std::fabs(m2);
     iteration++;
   }
+  free_hist();
   HIPC(hipMemcpyAsync(c->host_scal, hs->d, sizeof(double), hipMemcpyDeviceToHost, c->stream));
   HIPC(hipStreamSynchronize(c->stream));
   res->s = s;
@@ -1180,6 +1192,133 @@ int qd_magnus_flow(qd_context *c, const qd_op *h0, qd_op *hs, qd_op *omega, cons
   res->trace_rows = rows;
   if (p->verbose) fflush(stdout);
   return done(QD_OK);
+}
+
+// ---- the ODE path: ode(dGam, ...) and the gs_IMSRG decoupling loop ----------------
+struct qd_ode {
+  qd_context *ctx = nullptr;
+  AdamsDevice a;
+  qd_cc *hcc = nullptr, *etacc = nullptr;
+};
+
+int qd_ode_create(qd_context *c, qd_ode **out) {
+  if (!c || !out) return fail(QD_E_ARG, "null argument");
+  *out = nullptr;
+  qd_ode *o = new qd_ode();
+  o->ctx = c;
+  int rc;
+  if ((rc = o->a.init(c->stream, c->ch.neq)) || (rc = qd_cc_create(c, 1.0, &o->hcc)) ||
+      (rc = qd_cc_create(c, -1.0, &o->etacc))) {
+    qd_ode_destroy(o);
+    return rc;
+  }
+  *out = o;
+  return QD_OK;
+}
+
+int qd_ode_destroy(qd_ode *o) {
+  if (!o) return QD_OK;
+  o->a.release();
+  qd_cc_destroy(o->hcc);
+  qd_cc_destroy(o->etacc);
+  delete o;
+  return QD_OK;
+}
+
+// Fortran edit descriptor E14.6: sign, "0.", six digits, "E", signed two-digit exponent
+static const char *fortran_e(double x, char *buf) {
+  if (x == 0.0 || !std::isfinite(x)) {
+    snprintf(buf, 32, "%14s", x == 0.0 ? "0.000000E+00" : "NaN");
+    return buf;
+  }
+  int e = (int)std::floor(std::log10(std::fabs(x))) + 1;
+  double m = std::fabs(x) / std::pow(10.0, e);
+  long long r = std::llround(m * 1e6);
+  if (r >= 1000000) { r /= 10; e += 1; }
+  if (r < 100000) { r *= 10; e -= 1; }
+  char body[32];
+  snprintf(body, sizeof body, "%s0.%06lldE%c%02d", x < 0 ? "-" : "", r, e < 0 ? '-' : '+', std::abs(e));
+  snprintf(buf, 32, "%14s", body);
+  return buf;
+}
+
+static OdeRhs dgam_rhs(qd_ode *o) {
+  return [o](double, const double *y, double *yp) -> int {
+    qd_op H{o->ctx, const_cast<double *>(y), 1}, D{o->ctx, yp, 1};
+    return qd_dgam(o->ctx, &H, &D, o->hcc, o->etacc);
+  };
+}
+
+int qd_ode_integrate(qd_ode *o, qd_op *y, double *t, double tout, double *relerr, double *abserr, int *iflag) {
+  if (!o || !y || !t || !relerr || !abserr || !iflag) return fail(QD_E_ARG, "null argument");
+  if (y->ctx != o->ctx) return fail(QD_E_ARG, "operator belongs to another context");
+  return o->a.ode(dgam_rhs(o), y->d, t, tout, relerr, abserr, iflag);
+}
+
+int qd_ode_stats(const qd_ode *o, int *nsteps, int64_t *nfe, int *order, double *h) {
+  if (!o) return fail(QD_E_ARG, "null argument");
+  if (nsteps) *nsteps = o->a.nsteps;
+  if (nfe) *nfe = o->a.nfe;
+  if (order) *order = o->a.kold;
+  if (h) *h = o->a.h;
+  return QD_OK;
+}
+
+void qd_gs_params_default(qd_gs_params *p) {
+  if (!p) return;
+  p->relerr = R4_1EM8;
+  p->abserr = R4_1EM6;
+  p->stp = R4_0P1;
+  p->conv = R4_1EM7;
+  p->max_outputs = 100000;
+  p->verbose = 0;
+}
+
+int qd_gs_flow(qd_context *c, const qd_op *h0, qd_op *hs, const qd_gs_params *p, qd_gs_result *res, double *trace,
+               int trace_len) {
+  if (!c || !h0 || !hs || !p || !res) return fail(QD_E_ARG, "null argument");
+  int rc;
+  qd_ode *o;
+  if ((rc = qd_ode_create(c, &o))) return rc;
+  memset(res, 0, sizeof *res);
+  if ((rc = qd_copy_arrays(c, h0, hs))) { qd_ode_destroy(o); return rc; }
+  hs->herm = 1;
+  double s = 0.0, rel = p->relerr, ab = p->abserr, crit = 1.0, e_new = 0.0;
+  int flag = 1, pr = 0;
+  HIPC(hipMemcpyAsync(c->host_scal, hs->d, sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  HIPC(hipStreamSynchronize(c->stream));
+  e_new = c->host_scal[0];
+  while (crit > p->conv && pr < p->max_outputs) {
+    const double e_old = e_new;
+    if ((rc = o->a.ode(dgam_rhs(o), hs->d, &s, s + p->stp, &rel, &ab, &flag))) { qd_ode_destroy(o); return rc; }
+    HIPC(hipMemcpyAsync(c->host_scal, hs->d, sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    HIPC(hipStreamSynchronize(c->stream));
+    e_new = c->host_scal[0];
+    if (!std::isfinite(e_new)) { qd_ode_destroy(o); return fail(QD_E_NUMERIC, "non-finite E0 at s=%g", s); }
+    crit = std::fabs((e_old - e_new) / e_old);
+    pr++;
+    if (p->verbose) {  // the reference's per-output line '(I5,3(e14.6))' (src/gs_IMSRG.f90:117)
+      FILE *vf = p->verbose == 2 ? stderr : stdout;
+      char b1[32], b2[32], b3[32];
+      fprintf(vf, "%5d%s%s%s\n", pr, fortran_e(s, b1), fortran_e(e_new, b2), fortran_e(crit, b3));
+      fflush(vf);
+    }
+    if (trace && pr <= trace_len) {
+      double *r = trace + 4 * (size_t)(pr - 1);
+      r[0] = pr; r[1] = s; r[2] = e_new; r[3] = crit;
+    }
+    if (flag != 2) break;  // tolerance raised (3), too many steps (4, 5) or bad input (6)
+  }
+  res->s = s;
+  res->e0 = e_new;
+  res->crit = crit;
+  res->outputs = pr;
+  res->nsteps = o->a.nsteps;
+  res->nfe = o->a.nfe;
+  res->iflag = flag;
+  res->converged = crit <= p->conv;
+  qd_ode_destroy(o);
+  return QD_OK;
 }
 
 // ---- timing ---------------------------------------------------------------------

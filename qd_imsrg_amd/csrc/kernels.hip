@@ -93,42 +93,92 @@ __device__ __forceinline__ int64_t vidx(const DevTab &t, int q, int nh, int nb, 
   }
 }
 
+// One-body coefficients of every substitution-list entry for one operator
+// pair, so the 1b x 2b loops below read two doubles per entry instead of
+// evaluating f_elem.  For the replaced index x of pair g (x1 or x2 of the
+// pair, by the entry's pos bit) and the substitute t, the reference sums
+// over holes t with f(x,t) and over particles t with f(t,x), times herm of
+// the one-body operand on the bra side and -1 / -herm on the ket side
+// (src/commutators.f90:391-419 and the analogous loops per sub-block); the
+// entry's sort sign is folded in.  bra[k] = (c1, c2) multiplies (V2, V1),
+// ket[k] likewise.
+__global__ void __launch_bounds__(256) k_subcoef(DevTab t, Ops o, double2 *__restrict__ bra,
+                                                 double2 *__restrict__ ket, int npairs) {
+  const int g = blockIdx.x * 256 + threadIdx.x;
+  if (g >= npairs) return;
+  const uint32_t pr = t.pairs[g];
+  const int x1 = lo16(pr), x2 = hi16(pr), n = t.n;
+  const double h1 = o.h1, h2 = o.h2;
+  for (int k = t.sub_start[g], ke = t.sub_start[g + 1]; k < ke; k++) {
+    const uint32_t en = t.sub_list[k];
+    const int tt = (int)((en >> 16) & 0x7ffu);
+    const int xk = ((en >> 27) & 1u) ? x2 : x1;
+    const double sg = ((en >> 28) & 1u) ? -1.0 : 1.0;
+    double b1, b2, k1, k2;
+    if (tt < n) {
+      const double f1 = felem(t, o.v1, o.h1, xk, tt), f2 = felem(t, o.v2, o.h2, xk, tt);
+      b1 = f1; b2 = f2; k1 = -h1 * f1; k2 = -h2 * f2;
+    } else {
+      const double f1 = felem(t, o.v1, o.h1, tt, xk), f2 = felem(t, o.v2, o.h2, tt, xk);
+      b1 = h1 * f1; b2 = h2 * f2; k1 = -f1; k2 = -f2;
+    }
+    bra[k] = make_double2(sg * b1, sg * b2);
+    ket[k] = make_double2(sg * k1, sg * k2);
+  }
+}
+
+void launch_subcoef(hipStream_t s, const DevTab &t, const Ops &o, double2 *bra, double2 *ket, int npairs) {
+  if (npairs > 0) hipLaunchKernelGGL(k_subcoef, dim3((npairs + 255) / 256), dim3(256), 0, s, t, o, bra, ket, npairs);
+}
+
 // commutator_122 (src/commutators.f90:355-676) for the output <x1 x2|V|x3 x4>
 // in storage order: [f1, V2] - [f2, V1] in one pass over the substitution
-// lists.  For the replaced index x the reference sums over holes t with f(x,t)
-// and over particles t with f(t,x), times herm of the one-body operand on the
-// bra side and -1 / -herm on the ket side; only t in the (m_l, m_s) channel of
-// x can give a non-zero element, which is what the precomputed lists
-// enumerate (with the pair-sorting sign).  Bra substitutions read the LDS
-// tiles (column j of the block), ket substitutions read block row Ic.
+// lists.  Only substitutes t in the (m_l, m_s) channel of the replaced index
+// can give a non-zero element, which is what the precomputed lists
+// enumerate.  Bra substitutions read the LDS tiles (column j of the block);
+// ket substitutions read block row Ic from global memory, coalesced over
+// the lanes (consecutive rows), at offsets from three per-thread (base,
+// stride) pairs, one per ket pair type.
 __device__ __forceinline__ double c122(const DevTab &t, const Ops &o, const double *__restrict__ tile1,
-                                       const double *__restrict__ tile2, int ntot, int j, int q, int nh, int nb,
-                                       int np, int gI, int gJ, int Ic, int x1, int x2, int x3, int x4) {
+                                       const double *__restrict__ tile2, const double2 *__restrict__ sbra,
+                                       const double2 *__restrict__ sket, int ntot, int j, int q, int nh, int nb,
+                                       int np, int gI, int gJ, int Ic) {
   double r = 0.0;
-  const int n = t.n;
-  const double h1 = o.h1, h2 = o.h2;
   for (int k = t.sub_start[gI], ke = t.sub_start[gI + 1]; k < ke; k++) {
-    const uint32_t en = t.sub_list[k];
-    const int Pp = (int)(en & 0xffffu), tt = (int)((en >> 16) & 0x7ffu);
-    const int xk = ((en >> 27) & 1u) ? x2 : x1;
-    double c1, c2;
-    if (tt < n) { c1 = felem(t, o.v1, o.h1, xk, tt); c2 = felem(t, o.v2, o.h2, xk, tt); }
-    else { c1 = h1 * felem(t, o.v1, o.h1, tt, xk); c2 = h2 * felem(t, o.v2, o.h2, tt, xk); }
-    const double d = c1 * tile2[j * ntot + Pp] - c2 * tile1[j * ntot + Pp];
-    r += ((en >> 28) & 1u) ? -d : d;
+    const int Pp = (int)(t.sub_list[k] & 0xffffu);
+    const double2 c = sbra[k];
+    r += c.x * tile2[j * ntot + Pp] - c.y * tile1[j * ntot + Pp];
   }
+  // offsets of (Ic, Q) for Q of type tq (0 hh, 1 ph, 2 pp), index iq in its class:
+  // a = base[tq] + iq * mul[tq]; stored transposed (herm factor) when tp < tq
+  const int64_t *o6 = t.off + 6 * q;
+  const int tp = Ic < nh ? 0 : (Ic < nh + nb ? 1 : 2);
+  const int ip = Ic - (tp == 0 ? 0 : (tp == 1 ? nh : nh + nb));
+  const int rows_p = tp == 0 ? nh : (tp == 1 ? nb : np);
+  int64_t b0, b1, b2, m0, m1, m2;
+  if (tp == 0) {
+    b0 = o6[HHHH] + ip; m0 = nh;
+    b1 = o6[PHHH] + (int64_t)ip * nb; m1 = 1;
+    b2 = o6[PPHH] + (int64_t)ip * np; m2 = 1;
+  } else if (tp == 1) {
+    b0 = o6[PHHH] + ip; m0 = rows_p;
+    b1 = o6[PHPH] + ip; m1 = rows_p;
+    b2 = o6[PPPH] + (int64_t)ip * np; m2 = 1;
+  } else {
+    b0 = o6[PPHH] + ip; m0 = rows_p;
+    b1 = o6[PPPH] + ip; m1 = rows_p;
+    b2 = o6[PPPP] + ip; m2 = rows_p;
+  }
+  const double h1 = o.h1, h2 = o.h2;
   for (int k = t.sub_start[gJ], ke = t.sub_start[gJ + 1]; k < ke; k++) {
-    const uint32_t en = t.sub_list[k];
-    const int Qp = (int)(en & 0xffffu), tt = (int)((en >> 16) & 0x7ffu);
-    const int xk = ((en >> 27) & 1u) ? x4 : x3;
-    double c1, c2;
-    if (tt < n) { c1 = -h1 * felem(t, o.v1, o.h1, xk, tt); c2 = -h2 * felem(t, o.v2, o.h2, xk, tt); }
-    else { c1 = -felem(t, o.v1, o.h1, tt, xk); c2 = -felem(t, o.v2, o.h2, tt, xk); }
-    bool tr;
-    const int64_t a = vidx(t, q, nh, nb, np, Ic, Qp, &tr);
+    const int Qp = (int)(t.sub_list[k] & 0xffffu);
+    const int tq = Qp < nh ? 0 : (Qp < nh + nb ? 1 : 2);
+    const int iq = Qp - (tq == 0 ? 0 : (tq == 1 ? nh : nh + nb));
+    const int64_t a = tq == 0 ? b0 + iq * m0 : (tq == 1 ? b1 + iq * m1 : b2 + iq * m2);
+    const bool tr = tp < tq;
     const double v2 = tr ? h2 * o.v2[a] : o.v2[a], v1 = tr ? h1 * o.v1[a] : o.v1[a];
-    const double d = c1 * v2 - c2 * v1;
-    r += ((en >> 28) & 1u) ? -d : d;
+    const double2 c = sket[k];
+    r += c.x * v2 - c.y * v1;
   }
   return r;
 }
@@ -180,6 +230,7 @@ __global__ void __launch_bounds__(256) k_twobody(DevTab t, Ops o, double *__rest
                                                  const Task2 *__restrict__ tasks, int flags,
                                                  const double *__restrict__ w1, const double *__restrict__ w2,
                                                  const double *__restrict__ w3, const double *__restrict__ P,
+                                                 const double2 *__restrict__ sbra, const double2 *__restrict__ sket,
                                                  double hh, double eh) {
   __shared__ double lds[TB_LDS_DOUBLES];
   const Task2 tk = tasks[blockIdx.x];
@@ -214,7 +265,7 @@ __global__ void __launch_bounds__(256) k_twobody(DevTab t, Ops o, double *__rest
     const int x1 = lo16(bra), x2 = hi16(bra), x3 = lo16(ket), x4 = hi16(ket);
     double acc = (flags & TB_ACCUM) ? out[idx] : 0.0;
     if (do122) {
-      acc += c122(t, o, lds, lds + ntot * tj, ntot, j, q, nh, nb, np, base + Ic, base + Jc, Ic, x1, x2, x3, x4);
+      acc += c122(t, o, lds, lds + ntot * tj, sbra, sket, ntot, j, q, nh, nb, np, base + Ic, base + Jc, Ic);
     }
     if (flags & TB_LADDER) {
       // transposed / signed combinations of the products of matrix_mult222
@@ -246,10 +297,11 @@ __global__ void __launch_bounds__(256) k_twobody(DevTab t, Ops o, double *__rest
 }
 
 void launch_twobody(hipStream_t s, const DevTab &t, const Ops &o, double *out, const Task2 *tasks, int ntasks,
-                    int flags, const double *w1, const double *w2, const double *w3, const double *P, double hh,
-                    double eh) {
+                    int flags, const double *w1, const double *w2, const double *w3, const double *P,
+                    const double2 *sbra, const double2 *sket, double hh, double eh) {
   if (ntasks <= 0) return;
-  hipLaunchKernelGGL(k_twobody, dim3(ntasks), dim3(256), 0, s, t, o, out, tasks, flags, w1, w2, w3, P, hh, eh);
+  hipLaunchKernelGGL(k_twobody, dim3(ntasks), dim3(256), 0, s, t, o, out, tasks, flags, w1, w2, w3, P, sbra, sket,
+                     hh, eh);
 }
 
 // ---------------------------------------------------------------------------

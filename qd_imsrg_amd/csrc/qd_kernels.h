@@ -58,9 +58,12 @@ enum { OB_111 = 1, OB_121 = 2, OB_221 = 4, OB_ACCUM = 8 };
 // else a persistent grid striding the list.
 void launch_gemm(hipStream_t s, const GemmProb *probs, const GemmSeg *segs, const GemmTile *tiles,
                  int ntiles, const Slots &slots, int grid, int shape, int prec);
+// one-body coefficients of the substitution lists for operator pair o (bra
+// and ket use), consumed by the 122 part of launch_twobody
+void launch_subcoef(hipStream_t s, const DevTab &t, const Ops &o, double2 *bra, double2 *ket, int npairs);
 void launch_twobody(hipStream_t s, const DevTab &t, const Ops &o, double *out, const Task2 *tasks,
                     int ntasks, int flags, const double *w1, const double *w2, const double *w3,
-                    const double *P, double hh, double eh);
+                    const double *P, const double2 *sbra, const double2 *sket, double hh, double eh);
 void launch_onebody(hipStream_t s, const DevTab &t, const Ops &o, double *out, int flags,
                     const double *w1, const double *w2, double *u1, double *u2);
 // zero-body part (commutator_110 + commutator_220) into *dst (device scalar)

@@ -141,6 +141,25 @@ def test_white_mbpt2_norm_add(case):
     assert np.array_equal(Z.download(), o.add(x, 0.5, y, -2.0))
 
 
+def test_bch_expand_pandya(case):
+    """BCH_EXPAND with calc_cc-filled CC inputs (every nested commutator gets
+    its Pandya term) against the oracle's restatement."""
+    from oracle.flow import bch_expand_pandya
+    rng = np.random.default_rng(10)
+    g = random_flat(case.ch, rng, -1, scale=0.02)
+    h = random_flat(case.ch, rng, 1)
+    G, H, HS = case.ctx.operator(-1, g), case.ctx.operator(1, h), case.ctx.operator(1)
+    HCC, OCC = case.ctx.cc(1.0), case.ctx.cc(-1.0)
+    qd.calc_cc(H, HCC)
+    qd.calc_cc(G, OCC)
+    nt = qd.BCH_EXPAND(HS, G, H, HCC, OCC)
+    ref, nref = bch_expand_pandya(case.o, g, h)
+    no_pandya, _ = case.o.bch_expand(g, -1, h)
+    assert nt == nref
+    assert rel(HS.download(), ref) <= 1e-11
+    assert rel(no_pandya, ref) > 1e-6  # the Pandya term is really there
+
+
 def test_bch_expand(case):
     rng = np.random.default_rng(9)
     g = random_flat(case.ch, rng, -1, scale=0.02)

@@ -1,6 +1,7 @@
 """GPU Magnus flow (s_evolve) against the oracle's flow on a seeded model
 Hamiltonian at N=6/R=6: same accepted/rejected step sequence, same BCH term
-counts, E0 per step and final Omega / H(s) to 1e-10 relative."""
+counts, E0 per step and final Omega / H(s) to 1e-10 relative; as shipped
+(zero CC inputs) and with calc_cc-filled CC inputs (the Pandya term)."""
 import numpy as np
 import pytest
 
@@ -13,16 +14,16 @@ from tests.pins.hamiltonians import model_hamiltonian
 pytestmark = pytest.mark.gpu
 
 
-@pytest.mark.parametrize("N,R,seed", [(6, 6, 1), (2, 4, 2)])
-def test_magnus_flow_matches_oracle(N, R, seed):
-    print(f"[case] magnus flow N={N} R={R} seed={seed}", flush=True)
+@pytest.mark.parametrize("N,R,seed,pandya", [(6, 6, 1, False), (2, 4, 2, False), (6, 6, 1, True)])
+def test_magnus_flow_matches_oracle(N, R, seed, pandya):
+    print(f"[case] magnus flow N={N} R={R} seed={seed} pandya={pandya}", flush=True)
     ch = Channels(N, R)
     o = Oracle(ch)
     h0 = model_hamiltonian(ch, seed)
-    ref = magnus_flow(o, h0)
+    ref = magnus_flow(o, h0, pandya=pandya)
     ctx = qd.Context(N, R)
     H0, HS, OM = ctx.operator(1, h0), ctx.operator(1), ctx.operator(-1)
-    res, trace = qd.magnus_flow(ctx, H0, HS, OM)
+    res, trace = qd.magnus_flow(ctx, H0, HS, OM, pandya=pandya)
     assert res.converged
     assert (res.iterations, res.rejected, res.bch_terms) == (ref["iterations"], ref["rejected"], ref["bch_terms"])
     assert np.allclose(trace[:, :2], ref["trace"][:, :2], rtol=0, atol=1e-12)
@@ -75,4 +76,56 @@ def test_diis_flow_matches_oracle(warm):
     assert abs(res.e0 - ref["e0"]) <= 1e-10 * abs(ref["e0"])
     assert np.abs(HS.download() - ref["hs"]).max() <= 1e-9 * np.abs(ref["hs"]).max()
     assert np.abs(OM.download() - ref["omega"]).max() <= 1e-9 * np.abs(ref["omega"]).max()
+    ctx.close()
+
+
+def test_gs_ode_flow_matches_oracle():
+    """The ODE path (gs_IMSRG: Adams PECE over dGam, output every 0.1 until
+    |dE/E| < 1e-7) on the GPU against the oracle's restatement at N=6 R=6
+    hw=1 (HF basis): same number of outputs, integrator steps and RHS
+    evaluations, E_gs to 1e-10 relative, H(s) to 1e-8 relative (the step
+    and order decisions are taken on reductions whose rounding differs)."""
+    from oracle.adams import gs_flow
+    print("[case] gs ode flow N=6 R=6", flush=True)
+    ch = Channels(6, 6)
+    o = Oracle(ch)
+    h0, _, _ = qd.hf_hamiltonian(6, 6, 1.0)
+    ref = gs_flow(o, h0)
+    ctx = qd.Context(6, 6)
+    H0, HS = ctx.operator(1, h0), ctx.operator(1)
+    res, trace = qd.gs_flow(ctx, H0, HS)
+    print(f"outputs {res.outputs} (oracle {ref['outputs']}), steps {res.nsteps} ({ref['nsteps']}), "
+          f"nfe {res.nfe} ({ref['nfe']}), E {res.e0:.12f} ({ref['e0']:.12f})")
+    assert res.converged and res.iflag == 2
+    assert (res.outputs, res.nsteps, res.nfe) == (ref["outputs"], ref["nsteps"], ref["nfe"])
+    assert abs(res.e0 - ref["e0"]) <= 1e-10 * abs(ref["e0"])
+    assert np.abs(trace[:, 2] - ref["trace"][:, 2]).max() <= 1e-10 * abs(ref["e0"])
+    assert np.abs(HS.download() - ref["hs"]).max() <= 1e-8 * np.abs(ref["hs"]).max()
+    ctx.close()
+
+
+def test_ode_integrate_entry_point():
+    """qd_ode_integrate (the reference's ode call) over two output points
+    equals the same two calls of the oracle's integrator."""
+    from oracle.adams import GS_ABS, GS_REL, Adams
+    ch = Channels(6, 6)
+    o = Oracle(ch)
+    h0 = model_hamiltonian(ch, 3)
+
+    def dgam(_t, y):
+        return o.commutator(o.build_white(y, 1), -1, y, 1, pandya=True)
+    a = Adams(dgam, ch.neq)
+    y, t, rel, ab, flag = h0.copy(), 0.0, GS_REL, GS_ABS, 1
+    ctx = qd.Context(6, 6)
+    ode = qd.ODE(ctx)
+    Y = ctx.operator(1, h0)
+    tg, rg, ag, fg = 0.0, GS_REL, GS_ABS, 1
+    for tout in (0.25, 0.5):
+        y, t, rel, ab, flag = a.ode(y, t, tout, rel, ab, flag)
+        tg, rg, ag, fg = ode.integrate(Y, tg, tout, rg, ag, fg)
+        assert (fg, tg) == (flag, t) == (2, tout)
+        assert np.abs(Y.download() - y).max() <= 1e-9 * np.abs(y).max()
+    st = ode.stats()
+    assert (st["nsteps"], st["nfe"]) == (a.nsteps, a.nfe)
+    ode.close()
     ctx.close()
