@@ -201,7 +201,7 @@ def learned_form(nbody=20, nshell=14, n_omega=64, lo=0.1, hi=1.0, steps=20000, l
     network adds beyond it), warm steps, |dE| against the cold E_gs, and
     wall times; a warm flow that fails falls back to the cold result."""
     import qd_imsrg_amd as qd
-    from qd_imsrg_amd.warmstart import WarmStart
+    from qd_imsrg_amd.warmstart import WarmStart, warm_flow
     omegas = lo * (hi / lo) ** (np.arange(n_omega) / (n_omega - 1))
     ctx = qd.Context(nbody, nshell)
     ctx.autotune(reps=2)
@@ -236,19 +236,10 @@ def learned_form(nbody=20, nshell=14, n_omega=64, lo=0.1, hi=1.0, steps=20000, l
         t_cd = time.perf_counter() - t0
         HS, OM = ctx.operator(1), ctx.operator(-1)
         t0 = time.perf_counter()
-        ws.predict(omegas[k], OM)
-        fallback = False
-        try:
-            rw, _ = qd.magnus_flow(ctx, H0, HS, OM, warm_start=True, diis=6, max_iter=100)
-            ok = rw.converged
-        except qd.QDError:
-            ok = False
+        rw, fallback, spent = warm_flow(ctx, ws, float(omegas[k]), H0, HS, OM)
         t_w = time.perf_counter() - t0
-        if not ok:  # cold start as the fallback
-            fallback = True
-            steps_w, e_w, t_w = runs[k]["steps"] + 100, runs[k]["e0"], t_w + runs[k]["s"]
-        else:
-            steps_w, e_w = rw.iterations, rw.e0
+        steps_w = rw.iterations + (spent if fallback else 0)
+        e_w = rw.e0
         rec = dict(omega=float(omegas[k]), cold_steps=runs[k]["steps"], cold_s=runs[k]["s"],
                    cold_diis_steps=rd.iterations, cold_diis_s=t_cd, warm_steps=steps_w, warm_s=t_w,
                    abs_dE_warm_vs_cold=abs(e_w - runs[k]["e0"]), fallback=fallback)
@@ -258,10 +249,13 @@ def learned_form(nbody=20, nshell=14, n_omega=64, lo=0.1, hi=1.0, steps=20000, l
     ws.close()
     ctx.close()
     mean = lambda key: float(np.mean([r[key] for r in out])) if out else None
+    median = lambda key: float(np.median([r[key] for r in out])) if out else None
     return {"workload": f"N={nbody} R={nshell}, {n_omega} log-spaced omega in [{lo}, {hi}], "
                         f"{len(train)} train / {len(held)} held out (k = 2 mod 4)",
             "sweep_s": t_sweep, "fit": fit, "held_out": out,
             "mean_steps": {"cold": mean("cold_steps"), "cold_diis": mean("cold_diis_steps"), "warm": mean("warm_steps")},
+            "median_steps": {"cold": median("cold_steps"), "cold_diis": median("cold_diis_steps"),
+                             "warm": median("warm_steps")},
             "mean_flow_s": {"cold": mean("cold_s"), "cold_diis": mean("cold_diis_s"), "warm": mean("warm_s")},
             "max_abs_dE": max((r["abs_dE_warm_vs_cold"] for r in out), default=None),
             "fallbacks": sum(r["fallback"] for r in out)}
@@ -332,6 +326,9 @@ def main():
     ap.add_argument("--no-flow", action="store_true", help="skip the N=6 R=6 time-to-E_gs flows")
     ap.add_argument("--batched", action="store_true",
                     help="also run the N=20 R=16 batched-form flow against the per-channel path")
+    ap.add_argument("--no-autotune", action="store_true", help="keep the default 64x64 GEMM tiles")
+    ap.add_argument("--no-precision-micro", action="store_true",
+                    help="skip the per-precision repeat of the microbenchmark step")
     ap.add_argument("--learned", action="store_true",
                     help="also run the N=20 R=14 learned warm-start sweep (64 frequencies)")
     ap.add_argument("--precision", action="store_true",
@@ -352,8 +349,10 @@ def main():
     y = random_operator(ch, rng, 1)
 
     ctx = qd.Context(a.nbody, a.nshell)
-    tune = ctx.autotune(reps=3)  # tile shapes per size bucket (setup, not timed)
-    progress(f"autotune: GEMM ms {tune[0]:.3f} -> {tune[1]:.3f}")
+    tune = None
+    if not a.no_autotune:
+        tune = ctx.autotune(reps=3)  # tile shapes per size bucket (setup, not timed)
+        progress(f"autotune: GEMM ms {tune[0]:.3f} -> {tune[1]:.3f}")
     X, Y, Z = ctx.operator(1, x), ctx.operator(1, y), ctx.operator(-1)
     HCC, ECC = ctx.cc(1.0), ctx.cc(1.0)
 
@@ -416,7 +415,8 @@ def main():
     out["autotune_gemm_ms_before_after"] = tune
     if a.breakdown:
         out["breakdown_ms"] = breakdown(ctx, X, Y, HCC, ECC)
-    out["precision_micro"] = precision_micro(ctx, step, Z, fl["total"])
+    if not a.no_precision_micro:
+        out["precision_micro"] = precision_micro(ctx, step, Z, fl["total"])
     ctx.close()
     flows = None
     if not a.no_flow:

@@ -159,7 +159,10 @@ class WarmStart:
         t = (coef - self.c_mu) / self.c_sd
         self.mlp = MLP(self.ctx, 2, self.hidden, self.layers, self.rank, seed=self.seed)
         t0 = time.perf_counter()
-        trace = self.mlp.train(x, t, steps, lr)
+        # Adam with a step-decayed rate: half the steps at lr, then a quarter
+        # each at lr/3 and lr/10 (the moments carry over between phases)
+        parts = [(steps - 2 * (steps // 4), lr), (steps // 4, lr / 3), (steps // 4, lr / 10)]
+        trace = np.concatenate([self.mlp.train(x, t, n, r) for n, r in parts if n > 0] or [np.zeros(0)])
         t_train = time.perf_counter() - t0
         # how well the basis reproduces the training set and the fit its coefficients
         fit = self.coefficients(omegas)
@@ -181,3 +184,22 @@ class WarmStart:
         for o in (self.mlp, self.basis):
             if o is not None:
                 o.close()
+
+
+def warm_flow(ctx, ws: WarmStart, omega: float, H0, HS, OMEGA, diis: int = 6, max_iter: int = 100):
+    """Magnus flow for one frequency from the network's predicted Omega (with
+    the Pulay-accelerated continuation); the cold start (as shipped) is the
+    fallback when the warm flow fails or has not converged within max_iter.
+    Returns (result, fallback, iterations spent in the warm attempt)."""
+    from . import QDError, magnus_flow
+    ws.predict(omega, OMEGA)
+    spent = 0
+    try:
+        r, _ = magnus_flow(ctx, H0, HS, OMEGA, warm_start=True, diis=diis, max_iter=max_iter)
+        spent = r.iterations
+        if r.converged:
+            return r, False, spent
+    except QDError:
+        spent = max_iter
+    r, _ = magnus_flow(ctx, H0, HS, OMEGA)  # cold: Omega is reset to zero
+    return r, True, spent

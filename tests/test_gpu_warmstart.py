@@ -13,7 +13,7 @@ import pytest
 import qd_imsrg_amd as qd
 from oracle.mlp import forward as o_forward
 from oracle.mlp import loss_grad as o_loss_grad
-from qd_imsrg_amd.warmstart import MLP, Basis, WarmStart
+from qd_imsrg_amd.warmstart import MLP, Basis, WarmStart, warm_flow
 
 pytestmark = pytest.mark.gpu
 
@@ -109,6 +109,14 @@ def test_warm_start_saves_steps(ctx):
         dE.append(abs(r.e0 - cold[k][0]))
         print(f"omega {omegas[k]:.4f}: cold {cold[k][1]} steps, cold+DIIS {rd.iterations}, "
               f"learned warm+DIIS {r.iterations}, |dE| {dE[-1]:.2e}")
+    # the packaged driver: same warm result, and the cold start as the fallback
+    # when the warm attempt is cut off before converging
+    k = held[0]
+    H0 = ctx.operator(1, h0s[k])
+    r, fb, spent = warm_flow(ctx, ws, omegas[k], H0, ctx.operator(1), ctx.operator(-1))
+    assert not fb and r.iterations == n_warm[0]
+    r, fb, spent = warm_flow(ctx, ws, omegas[k], H0, ctx.operator(1), ctx.operator(-1), max_iter=1)
+    assert fb and spent == 1 and r.iterations == cold[k][1] and r.e0 == cold[k][0]
     ws.close()
     assert max(dE) <= 1e-6
     assert np.mean(n_warm) < np.mean(n_cold) and np.mean(n_warm) < np.mean(n_cold_diis)
