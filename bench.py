@@ -327,6 +327,9 @@ def main():
     ap.add_argument("--batched", action="store_true",
                     help="also run the N=20 R=16 batched-form flow against the per-channel path")
     ap.add_argument("--no-autotune", action="store_true", help="keep the default 64x64 GEMM tiles")
+    ap.add_argument("--calibrate", action="store_true",
+                    help="after the timed steps, run add_arrays (k_axpby: reads 16, writes 8 bytes per "
+                         "element of the operator) 3 times, the reference copy for HBM counter scaling")
     ap.add_argument("--no-precision-micro", action="store_true",
                     help="skip the per-precision repeat of the microbenchmark step")
     ap.add_argument("--learned", action="store_true",
@@ -374,6 +377,10 @@ def main():
     ms_cls, n_cls = ctx.timing_read()
     ctx.timing(False)
     progress(f"commutator microbenchmark: {a.steps} steps timed")
+    if a.calibrate:
+        for _ in range(3):
+            qd.add_arrays(X, 1.0, Y, 0.0, Z)
+        ctx.sync()
     ms_step = (t1 - t0) * 1e3 / a.steps
     value = fl["total"] / (ms_step * 1e-3) / 1e12
 

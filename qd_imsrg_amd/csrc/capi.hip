@@ -489,11 +489,16 @@ int run_acc(qd_context *c, const qd_op *r1, const qd_op *r2, qd_op *out) {
 int run_twobody(qd_context *c, const qd_op *r1, const qd_op *r2, qd_op *out, int flags, const qd_cc *hcc,
                 const qd_cc *ecc) {
   Ops o{r1->d, r2->d, r1->herm, r2->herm};
-  double hh = hcc ? hcc->herm : 1.0, eh = ecc ? ecc->herm : -1.0;
   Timer tm(c, TC_TWOBODY);
   if (flags & TB_122) launch_subcoef(c->stream, c->dt, o, c->sbra, c->sket, c->npairs);
-  launch_twobody(c->stream, c->dt, o, out->d, c->tasks2, c->ntasks2, flags, c->w1, c->w2, c->w3, c->P, c->sbra,
-                 c->sket, hh, eh);
+  const int other = flags & (TB_122 | TB_LADDER);
+  if (other)
+    launch_twobody(c->stream, c->dt, o, out->d, c->tasks2, c->ntasks2, flags & ~TB_PANDYA, c->w1, c->w2, c->w3,
+                   c->sbra, c->sket);
+  // the Pandya gather is its own pass, in the row order that keeps its reads
+  // of the combined product compact; it adds to what the pass above wrote
+  if (flags & TB_PANDYA)
+    launch_pandya_gather(c->stream, c->dt, out->d, c->tasks2, c->ntasks2, c->P, other || (flags & TB_ACCUM));
   HIPC(hipGetLastError());
   return QD_OK;
 }
@@ -645,6 +650,7 @@ int qd_context_create(int nbody, int nshell, qd_context **out) {
   UP(ch.ml, ml); UP(ch.ms, ms); UP(ch.cc_r, cc_r); UP(ch.cc_nb, cc_nb); UP(ch.cc_off, cc_off);
   UP(ch.cc_poff, cc_poff); UP(ch.cc_ab, cc_ab); UP(ch.cc_hp, cc_hp); UP(ch.cc_ab_start, cc_ab_start);
   UP(ch.cc_hp_start, cc_hp_start); UP(ccmap, cc_map); UP(ch.sub_start, sub_start); UP(ch.sub_list, sub_list);
+  UP(ch.rowperm, rowperm);
 #undef UP
   void *p;
   if ((rc = dev_alloc(c, sizeof(double) * ch.neq, &p))) return bail(rc); c->w1 = (double *)p;

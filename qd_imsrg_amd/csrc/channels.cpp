@@ -172,6 +172,27 @@ std::string build_channels(int nbody, int nshell, Channels &c) {
     }
   }
   c.sub_start[c.pairs.size()] = (int)c.sub_list.size();
+
+  // ---- row order of the Pandya gather: inside each pair class of a block,
+  // rows sorted by (channel of i, channel of j, i, j), so consecutive rows
+  // form compact (i, j) grids whose cross-coupled partners are adjacent rows
+  // of the Pandya product
+  c.rowperm.assign(c.pairs.size(), 0);
+  for (int b = 0; b < c.nblock; b++) {
+    const int base = c.hh_start[b];
+    const int bounds[4] = {0, c.nh[b], c.nh[b] + c.nb[b], c.nh[b] + c.nb[b] + c.np[b]};
+    for (int cl = 0; cl < 3; cl++) {
+      std::vector<int> idx;
+      for (int x = bounds[cl]; x < bounds[cl + 1]; x++) idx.push_back(x);
+      std::stable_sort(idx.begin(), idx.end(), [&](int u, int v) {
+        const uint32_t pu = c.pairs[base + u], pv = c.pairs[base + v];
+        const int ku[4] = {c.spch[lo16(pu)], c.spch[hi16(pu)], lo16(pu), hi16(pu)};
+        const int kv[4] = {c.spch[lo16(pv)], c.spch[hi16(pv)], lo16(pv), hi16(pv)};
+        return std::lexicographical_compare(ku, ku + 4, kv, kv + 4);
+      });
+      for (size_t k = 0; k < idx.size(); k++) c.rowperm[base + bounds[cl] + k] = idx[k];
+    }
+  }
   return "";
 }
 

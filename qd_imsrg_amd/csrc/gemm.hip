@@ -21,6 +21,42 @@ namespace qd {
 
 typedef double d4 __attribute__((ext_vector_type(4)));
 
+// Walk of K in chunks of KC over all segments of a problem, with the next
+// chunk's global loads issued into registers (load) before the current
+// chunk's MFMAs (compute) and written to LDS (store) after the next barrier.
+template <int KC, class Load, class Store, class Compute>
+__device__ __forceinline__ void chunk_walk(const GemmProb &pr, const GemmSeg *__restrict__ segs, Load load,
+                                           Store store, Compute compute) {
+  if (pr.nseg <= 0) return;
+  int si = 0, k0 = 0;
+  GemmSeg sg = segs[pr.seg0];
+  load(sg, 0);
+  int cta = sg.ta, ctb = sg.tb;
+  while (true) {
+    int nsi = si, nk0 = k0 + KC;
+    GemmSeg nsg = sg;
+    if (nk0 >= sg.K) {
+      nsi = si + 1;
+      nk0 = 0;
+      if (nsi < pr.nseg) nsg = segs[pr.seg0 + nsi];
+    }
+    const bool more = nsi < pr.nseg;
+    __syncthreads();
+    store(cta, ctb);
+    __syncthreads();
+    if (more) {
+      load(nsg, nk0);
+      cta = nsg.ta;
+      ctb = nsg.tb;
+    }
+    compute();
+    if (!more) break;
+    si = nsi;
+    k0 = nk0;
+    sg = nsg;
+  }
+}
+
 // shape table: TM x TN tile, WM x WN waves, each wave FM x FN MFMA tiles
 template <int TM, int TN, int WM, int WN>
 __device__ __forceinline__ void gemm_tile(const GemmProb &pr, const GemmSeg *__restrict__ segs, const Slots &slots,
@@ -39,53 +75,69 @@ __device__ __forceinline__ void gemm_tile(const GemmProb &pr, const GemmSeg *__r
 #pragma unroll
     for (int b = 0; b < FN; b++) acc[a][b] = d4{0, 0, 0, 0};
 
-  for (int si = 0; si < pr.nseg; si++) {
-    const GemmSeg sg = segs[pr.seg0 + si];
+  constexpr int NA = (TM * GEMM_TK + 255) / 256, NB = (TN * GEMM_TK + 255) / 256;
+  double ra[NA], rb[NB];
+  auto load = [&](const GemmSeg &sg, int k0) {
     const double *__restrict__ A = slots.p[sg.a_slot] + sg.a_off;
     const double *__restrict__ B = slots.p[sg.b_slot] + sg.b_off;
-    for (int k0 = 0; k0 < sg.K; k0 += GEMM_TK) {
 #pragma unroll
-      for (int r = 0; r < (TM * GEMM_TK + 255) / 256; r++) {
-        const int idx = tid + 256 * r;
-        if (idx < TM * GEMM_TK) {
-          int i, k;
-          if (!sg.ta) { i = idx % TM; k = idx / TM; }  // columns of A contiguous in i
-          else { k = idx % GEMM_TK; i = idx / GEMM_TK; }  // A stored transposed
-          const int gi = row0 + i, gk = k0 + k;
-          double v = 0.0;
-          if (gi < pr.M && gk < sg.K) v = sg.ta ? A[gk + (int64_t)gi * sg.lda] : A[gi + (int64_t)gk * sg.lda];
-          As[k * (TM + 1) + i] = sg.alpha * v;
-        }
-      }
-#pragma unroll
-      for (int r = 0; r < (TN * GEMM_TK + 255) / 256; r++) {
-        const int idx = tid + 256 * r;
-        if (idx < TN * GEMM_TK) {
-          int j, k;
-          if (!sg.tb) { k = idx % GEMM_TK; j = idx / GEMM_TK; }  // columns of B contiguous in k
-          else { j = idx % TN; k = idx / TN; }                   // B stored transposed
-          const int gj = col0 + j, gk = k0 + k;
-          double v = 0.0;
-          if (gj < pr.N && gk < sg.K) v = sg.tb ? B[gj + (int64_t)gk * sg.ldb] : B[gk + (int64_t)gj * sg.ldb];
-          Bs[k * (TN + 1) + j] = v;
-        }
-      }
-      __syncthreads();
-#pragma unroll
-      for (int kk = 0; kk < GEMM_TK; kk += 4) {
-        double a[FM], b[FN];
-#pragma unroll
-        for (int x = 0; x < FM; x++) a[x] = As[(kk + lk) * (TM + 1) + m0 + x * 16 + lr];
-#pragma unroll
-        for (int y = 0; y < FN; y++) b[y] = Bs[(kk + lk) * (TN + 1) + n0 + y * 16 + lr];
-#pragma unroll
-        for (int x = 0; x < FM; x++)
-#pragma unroll
-          for (int y = 0; y < FN; y++) acc[x][y] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[x], b[y], acc[x][y], 0, 0, 0);
-      }
-      __syncthreads();
+    for (int r = 0; r < NA; r++) {
+      const int idx = tid + 256 * r;
+      int i, k;
+      if (!sg.ta) { i = idx % TM; k = idx / TM; }        // columns of A contiguous in i
+      else { k = idx % GEMM_TK; i = idx / GEMM_TK; }     // A stored transposed
+      const int gi = row0 + i, gk = k0 + k;
+      double v = 0.0;
+      if (idx < TM * GEMM_TK && gi < pr.M && gk < sg.K)
+        v = sg.ta ? A[gk + (int64_t)gi * sg.lda] : A[gi + (int64_t)gk * sg.lda];
+      ra[r] = sg.alpha * v;
     }
-  }
+#pragma unroll
+    for (int r = 0; r < NB; r++) {
+      const int idx = tid + 256 * r;
+      int j, k;
+      if (!sg.tb) { k = idx % GEMM_TK; j = idx / GEMM_TK; }  // columns of B contiguous in k
+      else { j = idx % TN; k = idx / TN; }                   // B stored transposed
+      const int gj = col0 + j, gk = k0 + k;
+      double v = 0.0;
+      if (idx < TN * GEMM_TK && gj < pr.N && gk < sg.K)
+        v = sg.tb ? B[gj + (int64_t)gk * sg.ldb] : B[gk + (int64_t)gj * sg.ldb];
+      rb[r] = v;
+    }
+  };
+  auto store = [&](int ta, int tb) {
+#pragma unroll
+    for (int r = 0; r < NA; r++) {
+      const int idx = tid + 256 * r;
+      if (idx < TM * GEMM_TK) {
+        const int i = ta ? idx / GEMM_TK : idx % TM, k = ta ? idx % GEMM_TK : idx / TM;
+        As[k * (TM + 1) + i] = ra[r];
+      }
+    }
+#pragma unroll
+    for (int r = 0; r < NB; r++) {
+      const int idx = tid + 256 * r;
+      if (idx < TN * GEMM_TK) {
+        const int j = tb ? idx % TN : idx / GEMM_TK, k = tb ? idx / TN : idx % GEMM_TK;
+        Bs[k * (TN + 1) + j] = rb[r];
+      }
+    }
+  };
+  auto compute = [&]() {
+#pragma unroll
+    for (int kk = 0; kk < GEMM_TK; kk += 4) {
+      double a[FM], b[FN];
+#pragma unroll
+      for (int x = 0; x < FM; x++) a[x] = As[(kk + lk) * (TM + 1) + m0 + x * 16 + lr];
+#pragma unroll
+      for (int y = 0; y < FN; y++) b[y] = Bs[(kk + lk) * (TN + 1) + n0 + y * 16 + lr];
+#pragma unroll
+      for (int x = 0; x < FM; x++)
+#pragma unroll
+        for (int y = 0; y < FN; y++) acc[x][y] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[x], b[y], acc[x][y], 0, 0, 0);
+    }
+  };
+  chunk_walk<GEMM_TK>(pr, segs, load, store, compute);
   double *__restrict__ C = slots.p[pr.c_slot] + pr.c_off;
 #pragma unroll
   for (int x = 0; x < FM; x++)
@@ -166,6 +218,7 @@ template <int TM, int TN, int WM, int WN, int S>
 __device__ __forceinline__ void gemm_tile_bf16(const GemmProb &pr, const GemmSeg *__restrict__ segs,
                                                const Slots &slots, int tm, int tn, unsigned *__restrict__ lds) {
   constexpr int FM = TM / WM / 16, FN = TN / WN / 16, KP = SPLIT_TK / 2;
+  constexpr int NA = (TM * KP + 255) / 256, NB = (TN * KP + 255) / 256;
   unsigned *As = lds;                        // [S][TM][SPLIT_LDW]: slice s of A(i, k), k pairs packed
   unsigned *Bs = lds + S * TM * SPLIT_LDW;   // [S][TN][SPLIT_LDW]: slice s of B(k, j)
   const int row0 = tm * TM, col0 = tn * TN;
@@ -178,68 +231,78 @@ __device__ __forceinline__ void gemm_tile_bf16(const GemmProb &pr, const GemmSeg
   for (int a = 0; a < FM; a++)
 #pragma unroll
     for (int b = 0; b < FN; b++) acc[a][b] = d4{0, 0, 0, 0};
-
-  for (int si = 0; si < pr.nseg; si++) {
-    const GemmSeg sg = segs[pr.seg0 + si];
+  double ra[NA][2], rb[NB][2];  // pairs of consecutive k
+  auto load = [&](const GemmSeg &sg, int k0) {
     const double *__restrict__ A = slots.p[sg.a_slot] + sg.a_off;
     const double *__restrict__ B = slots.p[sg.b_slot] + sg.b_off;
-    for (int k0 = 0; k0 < sg.K; k0 += SPLIT_TK) {
 #pragma unroll
-      for (int r = 0; r < (TM * KP + 255) / 256; r++) {
-        const int idx = tid + 256 * r;
-        if (idx < TM * KP) {
-          int i, kp;
-          if (!sg.ta) { i = idx % TM; kp = idx / TM; }
-          else { kp = idx % KP; i = idx / KP; }
-          const int gi = row0 + i, gk = k0 + 2 * kp;
-          unsigned h0[S], h1[S];
-          bf16_slices<S>(sg.alpha * load_a(sg, A, pr, gi, gk), h0);
-          bf16_slices<S>(sg.alpha * load_a(sg, A, pr, gi, gk + 1), h1);
+    for (int r = 0; r < NA; r++) {
+      const int idx = tid + 256 * r;
+      const int i = sg.ta ? idx / KP : idx % TM, kp = sg.ta ? idx % KP : idx / TM;
+      const bool in = idx < TM * KP;
+      ra[r][0] = in ? sg.alpha * load_a(sg, A, pr, row0 + i, k0 + 2 * kp) : 0.0;
+      ra[r][1] = in ? sg.alpha * load_a(sg, A, pr, row0 + i, k0 + 2 * kp + 1) : 0.0;
+    }
 #pragma unroll
-          for (int s = 0; s < S; s++) As[(s * TM + i) * SPLIT_LDW + kp] = h0[s] | (h1[s] << 16);
-        }
+    for (int r = 0; r < NB; r++) {
+      const int idx = tid + 256 * r;
+      const int j = sg.tb ? idx % TN : idx / KP, kp = sg.tb ? idx / TN : idx % KP;
+      const bool in = idx < TN * KP;
+      rb[r][0] = in ? load_b(sg, B, pr, col0 + j, k0 + 2 * kp) : 0.0;
+      rb[r][1] = in ? load_b(sg, B, pr, col0 + j, k0 + 2 * kp + 1) : 0.0;
+    }
+  };
+  auto store = [&](int ta, int tb) {
+#pragma unroll
+    for (int r = 0; r < NA; r++) {
+      const int idx = tid + 256 * r;
+      if (idx < TM * KP) {
+        const int i = ta ? idx / KP : idx % TM, kp = ta ? idx % KP : idx / TM;
+        unsigned h0[S], h1[S];
+        bf16_slices<S>(ra[r][0], h0);
+        bf16_slices<S>(ra[r][1], h1);
+#pragma unroll
+        for (int s = 0; s < S; s++) As[(s * TM + i) * SPLIT_LDW + kp] = h0[s] | (h1[s] << 16);
       }
+    }
 #pragma unroll
-      for (int r = 0; r < (TN * KP + 255) / 256; r++) {
-        const int idx = tid + 256 * r;
-        if (idx < TN * KP) {
-          int j, kp;
-          if (!sg.tb) { kp = idx % KP; j = idx / KP; }
-          else { j = idx % TN; kp = idx / TN; }
-          const int gj = col0 + j, gk = k0 + 2 * kp;
-          unsigned h0[S], h1[S];
-          bf16_slices<S>(load_b(sg, B, pr, gj, gk), h0);
-          bf16_slices<S>(load_b(sg, B, pr, gj, gk + 1), h1);
+    for (int r = 0; r < NB; r++) {
+      const int idx = tid + 256 * r;
+      if (idx < TN * KP) {
+        const int j = tb ? idx % TN : idx / KP, kp = tb ? idx / TN : idx % KP;
+        unsigned h0[S], h1[S];
+        bf16_slices<S>(rb[r][0], h0);
+        bf16_slices<S>(rb[r][1], h1);
 #pragma unroll
-          for (int s = 0; s < S; s++) Bs[(s * TN + j) * SPLIT_LDW + kp] = h0[s] | (h1[s] << 16);
-        }
+        for (int s = 0; s < S; s++) Bs[(s * TN + j) * SPLIT_LDW + kp] = h0[s] | (h1[s] << 16);
       }
-      __syncthreads();
-      bf8 a[S][FM], b[S][FN];
+    }
+  };
+  auto compute = [&]() {
+    bf8 a[S][FM], b[S][FN];
 #pragma unroll
-      for (int s = 0; s < S; s++) {
-#pragma unroll
-        for (int x = 0; x < FM; x++)
-          a[s][x] = __builtin_bit_cast(bf8, *(const u4 *)(As + (s * TM + m0 + x * 16 + lr) * SPLIT_LDW + 4 * lk));
-#pragma unroll
-        for (int y = 0; y < FN; y++)
-          b[s][y] = __builtin_bit_cast(bf8, *(const u4 *)(Bs + (s * TN + n0 + y * 16 + lr) * SPLIT_LDW + 4 * lk));
-      }
+    for (int s = 0; s < S; s++) {
 #pragma unroll
       for (int x = 0; x < FM; x++)
+        a[s][x] = __builtin_bit_cast(bf8, *(const u4 *)(As + (s * TM + m0 + x * 16 + lr) * SPLIT_LDW + 4 * lk));
 #pragma unroll
-        for (int y = 0; y < FN; y++) {
-          f4 c = f4{0, 0, 0, 0};
-#pragma unroll
-          for (int d = S - 1; d >= 0; d--)
-#pragma unroll
-            for (int i = 0; i <= d; i++) c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[i][x], b[d - i][y], c, 0, 0, 0);
-#pragma unroll
-          for (int v = 0; v < 4; v++) acc[x][y][v] += (double)c[v];
-        }
-      __syncthreads();
+      for (int y = 0; y < FN; y++)
+        b[s][y] = __builtin_bit_cast(bf8, *(const u4 *)(Bs + (s * TN + n0 + y * 16 + lr) * SPLIT_LDW + 4 * lk));
     }
-  }
+#pragma unroll
+    for (int x = 0; x < FM; x++)
+#pragma unroll
+      for (int y = 0; y < FN; y++) {
+        f4 c = f4{0, 0, 0, 0};
+#pragma unroll
+        for (int d = S - 1; d >= 0; d--)
+#pragma unroll
+          for (int i = 0; i <= d; i++) c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[i][x], b[d - i][y], c, 0, 0, 0);
+#pragma unroll
+        for (int v = 0; v < 4; v++) acc[x][y][v] += (double)c[v];
+      }
+  };
+  chunk_walk<SPLIT_TK>(pr, segs, load, store, compute);
   store_f32_layout<FM, FN>(pr, slots, row0 + m0, col0 + n0, lr, lk, acc);
 }
 
@@ -247,6 +310,7 @@ template <int TM, int TN, int WM, int WN>
 __device__ __forceinline__ void gemm_tile_f32(const GemmProb &pr, const GemmSeg *__restrict__ segs, const Slots &slots,
                                               int tm, int tn, float *__restrict__ lds) {
   constexpr int FM = TM / WM / 16, FN = TN / WN / 16;
+  constexpr int NA = (TM * SPLIT_TK + 255) / 256, NB = (TN * SPLIT_TK + 255) / 256;
   float *As = lds;                          // [SPLIT_TK][TM + 1]
   float *Bs = lds + SPLIT_TK * (TM + 1);    // [SPLIT_TK][TN + 1]
   const int row0 = tm * TM, col0 = tn * TN;
@@ -259,59 +323,67 @@ __device__ __forceinline__ void gemm_tile_f32(const GemmProb &pr, const GemmSeg 
   for (int a = 0; a < FM; a++)
 #pragma unroll
     for (int b = 0; b < FN; b++) acc[a][b] = d4{0, 0, 0, 0};
-
-  for (int si = 0; si < pr.nseg; si++) {
-    const GemmSeg sg = segs[pr.seg0 + si];
+  float ra[NA], rb[NB];
+  auto load = [&](const GemmSeg &sg, int k0) {
     const double *__restrict__ A = slots.p[sg.a_slot] + sg.a_off;
     const double *__restrict__ B = slots.p[sg.b_slot] + sg.b_off;
-    for (int k0 = 0; k0 < sg.K; k0 += SPLIT_TK) {
 #pragma unroll
-      for (int r = 0; r < (TM * SPLIT_TK + 255) / 256; r++) {
-        const int idx = tid + 256 * r;
-        if (idx < TM * SPLIT_TK) {
-          int i, k;
-          if (!sg.ta) { i = idx % TM; k = idx / TM; }
-          else { k = idx % SPLIT_TK; i = idx / SPLIT_TK; }
-          As[k * (TM + 1) + i] = (float)(sg.alpha * load_a(sg, A, pr, row0 + i, k0 + k));
-        }
-      }
-#pragma unroll
-      for (int r = 0; r < (TN * SPLIT_TK + 255) / 256; r++) {
-        const int idx = tid + 256 * r;
-        if (idx < TN * SPLIT_TK) {
-          int j, k;
-          if (!sg.tb) { k = idx % SPLIT_TK; j = idx / SPLIT_TK; }
-          else { j = idx % TN; k = idx / TN; }
-          Bs[k * (TN + 1) + j] = (float)load_b(sg, B, pr, col0 + j, k0 + k);
-        }
-      }
-      __syncthreads();
-      f4 c[FM][FN];
-#pragma unroll
-      for (int x = 0; x < FM; x++)
-#pragma unroll
-        for (int y = 0; y < FN; y++) c[x][y] = f4{0, 0, 0, 0};
-#pragma unroll
-      for (int kk = 0; kk < SPLIT_TK; kk += 4) {
-        float a[FM], b[FN];
-#pragma unroll
-        for (int x = 0; x < FM; x++) a[x] = As[(kk + lk) * (TM + 1) + m0 + x * 16 + lr];
-#pragma unroll
-        for (int y = 0; y < FN; y++) b[y] = Bs[(kk + lk) * (TN + 1) + n0 + y * 16 + lr];
-#pragma unroll
-        for (int x = 0; x < FM; x++)
-#pragma unroll
-          for (int y = 0; y < FN; y++) c[x][y] = __builtin_amdgcn_mfma_f32_16x16x4f32(a[x], b[y], c[x][y], 0, 0, 0);
-      }
-#pragma unroll
-      for (int x = 0; x < FM; x++)
-#pragma unroll
-        for (int y = 0; y < FN; y++)
-#pragma unroll
-          for (int v = 0; v < 4; v++) acc[x][y][v] += (double)c[x][y][v];
-      __syncthreads();
+    for (int r = 0; r < NA; r++) {
+      const int idx = tid + 256 * r;
+      const int i = sg.ta ? idx / SPLIT_TK : idx % TM, k = sg.ta ? idx % SPLIT_TK : idx / TM;
+      ra[r] = idx < TM * SPLIT_TK ? (float)(sg.alpha * load_a(sg, A, pr, row0 + i, k0 + k)) : 0.0f;
     }
-  }
+#pragma unroll
+    for (int r = 0; r < NB; r++) {
+      const int idx = tid + 256 * r;
+      const int j = sg.tb ? idx % TN : idx / SPLIT_TK, k = sg.tb ? idx / TN : idx % SPLIT_TK;
+      rb[r] = idx < TN * SPLIT_TK ? (float)load_b(sg, B, pr, col0 + j, k0 + k) : 0.0f;
+    }
+  };
+  auto store = [&](int ta, int tb) {
+#pragma unroll
+    for (int r = 0; r < NA; r++) {
+      const int idx = tid + 256 * r;
+      if (idx < TM * SPLIT_TK) {
+        const int i = ta ? idx / SPLIT_TK : idx % TM, k = ta ? idx % SPLIT_TK : idx / TM;
+        As[k * (TM + 1) + i] = ra[r];
+      }
+    }
+#pragma unroll
+    for (int r = 0; r < NB; r++) {
+      const int idx = tid + 256 * r;
+      if (idx < TN * SPLIT_TK) {
+        const int j = tb ? idx % TN : idx / SPLIT_TK, k = tb ? idx / TN : idx % SPLIT_TK;
+        Bs[k * (TN + 1) + j] = rb[r];
+      }
+    }
+  };
+  auto compute = [&]() {
+    f4 c[FM][FN];
+#pragma unroll
+    for (int x = 0; x < FM; x++)
+#pragma unroll
+      for (int y = 0; y < FN; y++) c[x][y] = f4{0, 0, 0, 0};
+#pragma unroll
+    for (int kk = 0; kk < SPLIT_TK; kk += 4) {
+      float a[FM], b[FN];
+#pragma unroll
+      for (int x = 0; x < FM; x++) a[x] = As[(kk + lk) * (TM + 1) + m0 + x * 16 + lr];
+#pragma unroll
+      for (int y = 0; y < FN; y++) b[y] = Bs[(kk + lk) * (TN + 1) + n0 + y * 16 + lr];
+#pragma unroll
+      for (int x = 0; x < FM; x++)
+#pragma unroll
+        for (int y = 0; y < FN; y++) c[x][y] = __builtin_amdgcn_mfma_f32_16x16x4f32(a[x], b[y], c[x][y], 0, 0, 0);
+    }
+#pragma unroll
+    for (int x = 0; x < FM; x++)
+#pragma unroll
+      for (int y = 0; y < FN; y++)
+#pragma unroll
+        for (int v = 0; v < 4; v++) acc[x][y][v] += (double)c[x][y][v];
+  };
+  chunk_walk<SPLIT_TK>(pr, segs, load, store, compute);
   store_f32_layout<FM, FN>(pr, slots, row0 + m0, col0 + n0, lr, lk, acc);
 }
 

@@ -229,9 +229,8 @@ void launch_pandya_combine(hipStream_t s, const DevTab &t, double *P, const Tile
 __global__ void __launch_bounds__(256) k_twobody(DevTab t, Ops o, double *__restrict__ out,
                                                  const Task2 *__restrict__ tasks, int flags,
                                                  const double *__restrict__ w1, const double *__restrict__ w2,
-                                                 const double *__restrict__ w3, const double *__restrict__ P,
-                                                 const double2 *__restrict__ sbra, const double2 *__restrict__ sket,
-                                                 double hh, double eh) {
+                                                 const double *__restrict__ w3,
+                                                 const double2 *__restrict__ sbra, const double2 *__restrict__ sket) {
   __shared__ double lds[TB_LDS_DOUBLES];
   const Task2 tk = tasks[blockIdx.x];
   if (tk.block < 0) return;
@@ -287,21 +286,58 @@ __global__ void __launch_bounds__(256) k_twobody(DevTab t, Ops o, double *__rest
         default: break;
       }
     }
-    if (flags & TB_PANDYA) {
-      // src/commutators.f90:746-751; storage-order indices carry the ph sign
-      acc += pandya_d(t, P, x3, x1, x2, x4) - pandya_d(t, P, x3, x2, x1, x4) + pandya_d(t, P, x4, x2, x1, x3) -
-             pandya_d(t, P, x4, x1, x2, x3);
-    }
     out[idx] = acc;
   }
 }
 
 void launch_twobody(hipStream_t s, const DevTab &t, const Ops &o, double *out, const Task2 *tasks, int ntasks,
-                    int flags, const double *w1, const double *w2, const double *w3, const double *P,
-                    const double2 *sbra, const double2 *sket, double hh, double eh) {
+                    int flags, const double *w1, const double *w2, const double *w3, const double2 *sbra,
+                    const double2 *sket) {
   if (ntasks <= 0) return;
-  hipLaunchKernelGGL(k_twobody, dim3(ntasks), dim3(256), 0, s, t, o, out, tasks, flags, w1, w2, w3, P, sbra, sket,
-                     hh, eh);
+  hipLaunchKernelGGL(k_twobody, dim3(ntasks), dim3(256), 0, s, t, o, out, tasks, flags, w1, w2, w3, sbra, sket);
+}
+
+// The Pandya part of commutator_222 (src/commutators.f90:746-751 and the
+// analogous lines per sub-block): every stored output <x1 x2|V|x3 x4> adds
+// D(x3,x1|x2,x4) - D(x3,x2|x1,x4) + D(x4,x2|x1,x3) - D(x4,x1|x2,x3) of the
+// combined product D (storage-order indices carry the ph sign).  Same tasks
+// as the two-body pass; rows are visited in the block's Pandya row order
+// (grouped by the channels of x1 and x2), so the lanes of a wave cover a
+// compact (x1, x2) grid and each D read touches a few adjacent rows.
+__global__ void __launch_bounds__(256) k_pandya_gather(DevTab t, double *__restrict__ out,
+                                                       const Task2 *__restrict__ tasks,
+                                                       const double *__restrict__ P, int accumulate) {
+  const Task2 tk = tasks[blockIdx.x];
+  if (tk.block < 0) return;
+  const int q = tk.block, tau = tk.tau, j0 = tk.j0, tj = tk.tj;
+  const int nh = t.nh[q], nb = t.nb[q], np = t.np[q], ntot = nh + nb + np;
+  const int base = t.hh_start[q];
+  const int tstart = tau == 0 ? 0 : (tau == 1 ? nh : nh + nb);
+  const int nrows = ntot - tstart;
+  const int64_t *off = t.off + 6 * q;
+  for (int e = threadIdx.x; e < nrows * tj; e += 256) {
+    const int Ic = t.rowperm[base + tstart + e % nrows], j = e / nrows;
+    const int Jl = j0 + j, Jc = tstart + Jl;
+    const int rho = Ic < nh ? 0 : (Ic < nh + nb ? 1 : 2);
+    const int Il = Ic - (rho == 0 ? 0 : (rho == 1 ? nh : nh + nb));
+    const int rows = rho == 0 ? nh : (rho == 1 ? nb : np);
+    int sub;
+    if (tau == 0) sub = rho == 0 ? HHHH : (rho == 1 ? PHHH : PPHH);
+    else if (tau == 1) sub = rho == 1 ? PHPH : PPPH;
+    else sub = PPPP;
+    const int64_t idx = off[sub] + Il + (int64_t)Jl * rows;
+    const uint32_t bra = t.pairs[base + Ic], ket = t.pairs[base + Jc];
+    const int x1 = lo16(bra), x2 = hi16(bra), x3 = lo16(ket), x4 = hi16(ket);
+    const double d = pandya_d(t, P, x3, x1, x2, x4) - pandya_d(t, P, x3, x2, x1, x4) +
+                     pandya_d(t, P, x4, x2, x1, x3) - pandya_d(t, P, x4, x1, x2, x3);
+    out[idx] = accumulate ? out[idx] + d : d;
+  }
+}
+
+void launch_pandya_gather(hipStream_t s, const DevTab &t, double *out, const Task2 *tasks, int ntasks,
+                          const double *P, bool accumulate) {
+  if (ntasks <= 0) return;
+  hipLaunchKernelGGL(k_pandya_gather, dim3(ntasks), dim3(256), 0, s, t, out, tasks, P, accumulate ? 1 : 0);
 }
 
 // ---------------------------------------------------------------------------

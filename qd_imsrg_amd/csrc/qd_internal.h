@@ -53,6 +53,7 @@ struct Channels {
   // (11 bits) | pos << 27 (which index was replaced) | neg << 28 (sort sign)
   std::vector<int> sub_start;               // per global pair index, size npairs + 1
   std::vector<uint32_t> sub_list;
+  std::vector<int> rowperm;                 // per global pair index: row order of the Pandya gather
 };
 
 // Builds the tables; returns an empty string on success, else the reason.
@@ -84,6 +85,7 @@ struct DevTab {
   const int2 *cc_map;
   const int *sub_start;
   const uint32_t *sub_list;
+  const int *rowperm;
 };
 
 }  // namespace qd

@@ -63,7 +63,11 @@ void launch_gemm(hipStream_t s, const GemmProb *probs, const GemmSeg *segs, cons
 void launch_subcoef(hipStream_t s, const DevTab &t, const Ops &o, double2 *bra, double2 *ket, int npairs);
 void launch_twobody(hipStream_t s, const DevTab &t, const Ops &o, double *out, const Task2 *tasks,
                     int ntasks, int flags, const double *w1, const double *w2, const double *w3,
-                    const double *P, const double2 *sbra, const double2 *sket, double hh, double eh);
+                    const double2 *sbra, const double2 *sket);
+// Pandya part of commutator_222 from the combined product P (D = XCC - YCC),
+// added to out (or written, accumulate = false)
+void launch_pandya_gather(hipStream_t s, const DevTab &t, double *out, const Task2 *tasks, int ntasks,
+                          const double *P, bool accumulate);
 void launch_onebody(hipStream_t s, const DevTab &t, const Ops &o, double *out, int flags,
                     const double *w1, const double *w2, double *u1, double *u2);
 // zero-body part (commutator_110 + commutator_220) into *dst (device scalar)
