@@ -26,7 +26,26 @@ sys.path.insert(0, ROOT)
 
 PEAK_FP64_MFMA = 78.6e12   # MI355X dense FP64 matrix, FLOP/s
 PEAK_HBM = 8.0e12          # MI355X HBM3E, B/s
-CLASS_NAMES = ["gemm_products", "gemm_accumulate", "twobody_gather", "onebody", "calc_cc", "other"]
+CLASS_NAMES = ["gemm_products", "gemm_accumulate", "twobody_gather", "onebody", "calc_cc", "other", "pandya_gather"]
+NCLS = len(CLASS_NAMES)
+# kernels behind each timing class (names as rocprofv3 reports them)
+CLASS_KERNELS = {0: ["k_gemm"], 1: ["k_gemm"], 2: ["k_subcoef", "k_twobody"], 3: ["k_cc_matvec", "k_onebody"],
+                 4: ["k_calc_cc"], 6: ["k_pandya_gather"]}
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "hbm_traffic_n20r14.json")
+
+
+def measured_traffic(cls: int, nbody: int, nshell: int):
+    """HBM bytes per launch of a timing class from the committed rocprofv3
+    FETCH_SIZE / WRITE_SIZE runs of this workload (scaled by the counters'
+    reading of a copy of known size), or None when none were taken."""
+    try:
+        with open(TRAFFIC_FILE) as f:
+            d = json.load(f)
+    except (OSError, ValueError):
+        return None
+    if (d.get("nbody"), d.get("nshell")) != (nbody, nshell):
+        return None
+    return d.get("class_bytes_per_launch", {}).get(CLASS_NAMES[cls])
 
 
 def progress(msg):
@@ -283,7 +302,7 @@ def breakdown(ctx, X, Y, HCC, ECC, reps=10):
             fn()
         ms, n = ctx.timing_read()
         ctx.timing(False)
-        res[name] = {CLASS_NAMES[i]: ms[i] / reps for i in range(6) if n[i]}
+        res[name] = {CLASS_NAMES[i]: ms[i] / reps for i in range(NCLS) if n[i]}
     return res
 
 
@@ -385,7 +404,7 @@ def main():
     value = fl["total"] / (ms_step * 1e-3) / 1e12
 
     gf = ctx.gemm_flops()
-    per_launch = {CLASS_NAMES[i]: (ms_cls[i] / n_cls[i] if n_cls[i] else 0.0) for i in range(6)}
+    per_launch = {CLASS_NAMES[i]: (ms_cls[i] / n_cls[i] if n_cls[i] else 0.0) for i in range(NCLS)}
     # dominant kernel class by device time
     dom = int(np.argmax(ms_cls))
     n2b = ch.n2b()
@@ -398,14 +417,22 @@ def main():
         roof = dict(kernel="k_gemm (accumulating ladder products)", bound="mfma",
                     achieved=achieved / 1e12, peak=PEAK_FP64_MFMA / 1e12, unit="TFLOP/s")
     else:
-        # two-body gather: one write and two operand reads per output element,
-        # plus its ladder-intermediate and Pandya-product reads (8 + 8*6 bytes)
-        byts = {2: 8.0 * n2b * (3 + 4 + 8), 3: 8.0 * 3 * (ch.neq - n2b), 4: 8.0 * 2 * ch.cc_size}.get(dom, 0.0)
+        # algorithmic bytes per launch: two-body pass = per output element one
+        # write, two operand reads (the 1b x 2b term) and four reads of ladder
+        # intermediates; Pandya gather = one read-modify-write and four reads
+        # of the combined product; one-body = per one-body element one write
+        # and two operand reads; calc_cc = one gathered read and one write
+        byts = {2: 8.0 * n2b * (3 + 4), 6: 8.0 * n2b * (2 + 4), 3: 8.0 * 3 * (ch.neq - n2b),
+                4: 8.0 * 2 * ch.cc_size}.get(dom, 0.0)
         t = per_launch[CLASS_NAMES[dom]] * 1e-3
-        roof = dict(kernel=CLASS_NAMES[dom], bound="hbm", achieved=byts / t / 1e9 if t else 0.0,
-                    peak=PEAK_HBM / 1e9, unit="GB/s")
+        roof = dict(kernel=f"{CLASS_NAMES[dom]} ({' + '.join(CLASS_KERNELS.get(dom, []))})", bound="hbm",
+                    achieved=byts / t / 1e9 if t else 0.0, peak=PEAK_HBM / 1e9, unit="GB/s",
+                    algorithmic_bytes_per_launch=byts)
     roof["frac"] = roof["achieved"] / roof["peak"]
-    roof["traffic"] = None
+    tr = measured_traffic(dom, a.nbody, a.nshell)
+    roof["traffic"] = tr
+    if tr is not None:
+        roof["traffic_source"] = os.path.relpath(TRAFFIC_FILE, ROOT)
 
     out = {
         "metric": "IMSRG(2) time-to-E_gs (|dE|<=1e-6 Ha vs FP64) and 2b-2b commutator TFLOP/s",
@@ -415,7 +442,7 @@ def main():
         "config": {"workload": f"commutator microbenchmark: full FP64 [X,Y] incl. calc_cc, channel structure "
                                f"N={a.nbody} R={a.nshell} (M={ch.M}, neq={ch.neq})",
                    "flops_2b2b_reference_count": fl["total"], "parallelism": "single GPU"},
-        "kernel_ms_per_step": {CLASS_NAMES[i]: ms_cls[i] / a.steps for i in range(6)},
+        "kernel_ms_per_step": {CLASS_NAMES[i]: ms_cls[i] / a.steps for i in range(NCLS)},
         "gemm_flops_per_step": gf,
         "roofline": roof,
     }

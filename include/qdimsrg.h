@@ -328,8 +328,8 @@ int qd_hf_hamiltonian(int nbody, int nshell, double hw, double *h0, int64_t neq,
 /* ---- timing of the dominant kernels (HIP events on the context stream) --
  * Accumulates per-kernel-class device time while enabled.  Classes: 0 the
  * product GEMM launch (ladder intermediates + Pandya products), 1 the
- * accumulating ladder GEMM launch, 2 two-body gather (122 + ladder combine +
- * Pandya scatter), 3 one-body, 4 calc_cc, 5 other. */
+ * accumulating ladder GEMM launch, 2 two-body pass (122 + ladder combine),
+ * 3 one-body, 4 calc_cc, 5 other, 6 Pandya gather. */
 int qd_timing_enable(qd_context *ctx, int on);
 int qd_timing_read(qd_context *ctx, double *ms_per_class, int64_t *launches_per_class, int nclass);
 /* Algorithmic FP64 flops of one full commutator's GEMM launches: the product

@@ -67,9 +67,9 @@ class Context:
         check(self.L.qd_timing_enable(self.h, int(bool(on))))
 
     def timing_read(self):
-        ms = np.zeros(6)
-        n = np.zeros(6, dtype=np.int64)
-        check(self.L.qd_timing_read(self.h, _dp(ms), n.ctypes.data_as(_lib.i64p), 6))
+        ms = np.zeros(7)
+        n = np.zeros(7, dtype=np.int64)
+        check(self.L.qd_timing_read(self.h, _dp(ms), n.ctypes.data_as(_lib.i64p), 7))
         return ms, n
 
     def gemm_flops(self):

@@ -40,19 +40,20 @@ BIN = os.path.join(HERE, "bin")
 
 
 def build_fortran(verbose: bool = False) -> str:
-    """The iso_c_binding module and the run_magnus driver (amdflang)."""
+    """The iso_c_binding module and the run_magnus / run_gs drivers (amdflang)."""
     os.makedirs(BIN, exist_ok=True)
     mod = [FLANG, "-O2", "-c", os.path.join(FORTRAN, "qdimsrg_c.f90"), "-module-dir", OBJ,
            "-o", os.path.join(OBJ, "qdimsrg_c.o")]
     r = subprocess.run(mod, capture_output=True, text=True)
     if r.returncode != 0:
         raise RuntimeError(f"amdflang failed on qdimsrg_c.f90:\n{r.stderr}")
-    exe = os.path.join(BIN, "run_magnus")
-    cmd = [FLANG, "-O2", os.path.join(FORTRAN, "run_magnus.f90"), "-I", OBJ, os.path.join(OBJ, "qdimsrg_c.o"),
-           "-o", exe, "-L", HERE, "-lqdimsrg", "-Wl,-rpath,$ORIGIN/..", "-Wl,-rpath,/opt/rocm/lib"]
-    r = subprocess.run(cmd, capture_output=True, text=True)
-    if r.returncode != 0:
-        raise RuntimeError(f"amdflang failed on run_magnus.f90:\n{r.stderr}")
+    for prog in ("run_magnus", "run_gs"):
+        exe = os.path.join(BIN, prog)
+        cmd = [FLANG, "-O2", os.path.join(FORTRAN, prog + ".f90"), "-I", OBJ, os.path.join(OBJ, "qdimsrg_c.o"),
+               "-o", exe, "-L", HERE, "-lqdimsrg", "-Wl,-rpath,$ORIGIN/..", "-Wl,-rpath,/opt/rocm/lib"]
+        r = subprocess.run(cmd, capture_output=True, text=True)
+        if r.returncode != 0:
+            raise RuntimeError(f"amdflang failed on {prog}.f90:\n{r.stderr}")
     if verbose:
         print("built", exe)
     return exe

@@ -55,8 +55,8 @@ const double R4_1EM6 = 9.999999974752427e-07;   // abse = 1e-6, src/gs_IMSRG.f90
 const double R4_0P1 = 0.10000000149011612;      // stp = 0.1, src/gs_IMSRG.f90:97
 const double R4_1EM7 = 1.0000000116860974e-07;  // conv_criteria, src/gs_IMSRG.f90:13
 
-constexpr int NCLASS = 6;
-enum { TC_PRODUCTS = 0, TC_ACCUM = 1, TC_TWOBODY = 2, TC_ONEBODY = 3, TC_CALCCC = 4, TC_OTHER = 5 };
+constexpr int NCLASS = 7;
+enum { TC_PRODUCTS = 0, TC_ACCUM = 1, TC_TWOBODY = 2, TC_ONEBODY = 3, TC_CALCCC = 4, TC_OTHER = 5, TC_PANDYA = 6 };
 
 // One grouped-GEMM work list.  Problems [0, npre) are the ladder products;
 // [npre, nprob) the Pandya products (empty for the accumulating list).
@@ -489,17 +489,21 @@ int run_acc(qd_context *c, const qd_op *r1, const qd_op *r2, qd_op *out) {
 int run_twobody(qd_context *c, const qd_op *r1, const qd_op *r2, qd_op *out, int flags, const qd_cc *hcc,
                 const qd_cc *ecc) {
   Ops o{r1->d, r2->d, r1->herm, r2->herm};
-  Timer tm(c, TC_TWOBODY);
-  if (flags & TB_122) launch_subcoef(c->stream, c->dt, o, c->sbra, c->sket, c->npairs);
   const int other = flags & (TB_122 | TB_LADDER);
-  if (other)
+  if (other) {
+    Timer tm(c, TC_TWOBODY);
+    if (flags & TB_122) launch_subcoef(c->stream, c->dt, o, c->sbra, c->sket, c->npairs);
     launch_twobody(c->stream, c->dt, o, out->d, c->tasks2, c->ntasks2, flags & ~TB_PANDYA, c->w1, c->w2, c->w3,
                    c->sbra, c->sket);
+    HIPC(hipGetLastError());
+  }
   // the Pandya gather is its own pass, in the row order that keeps its reads
   // of the combined product compact; it adds to what the pass above wrote
-  if (flags & TB_PANDYA)
+  if (flags & TB_PANDYA) {
+    Timer tm(c, TC_PANDYA);
     launch_pandya_gather(c->stream, c->dt, out->d, c->tasks2, c->ntasks2, c->P, other || (flags & TB_ACCUM));
-  HIPC(hipGetLastError());
+    HIPC(hipGetLastError());
+  }
   return QD_OK;
 }
 

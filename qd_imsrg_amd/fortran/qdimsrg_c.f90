@@ -19,6 +19,18 @@ module qdimsrg_c
      integer(c_int) :: max_iter, pandya, verbose, warm_start, diis
   end type qd_flow_params
 
+  type, bind(C) :: qd_gs_params
+     real(c_double) :: relerr, abserr, stp, conv
+     integer(c_int) :: max_outputs, verbose
+  end type qd_gs_params
+
+  type, bind(C) :: qd_gs_result
+     real(c_double) :: s, e0, crit
+     integer(c_int) :: outputs, nsteps
+     integer(c_int64_t) :: nfe
+     integer(c_int) :: iflag, converged
+  end type qd_gs_result
+
   type, bind(C) :: qd_flow_result
      real(c_double) :: s, e0, crit
      integer(c_int) :: iterations, rejected
@@ -190,6 +202,59 @@ module qdimsrg_c
        real(c_double) :: trace(*)
        integer(c_int), value :: trace_len
        integer(c_int) :: qd_magnus_flow
+     end function
+     ! ---- the ODE path (src/adams_ode.f90:7, src/gs_IMSRG.f90:100-120) ----
+     function qd_ode_create(ctx, ode) bind(C, name="qd_ode_create")
+       import :: c_int, c_ptr
+       type(c_ptr), value :: ctx
+       type(c_ptr) :: ode
+       integer(c_int) :: qd_ode_create
+     end function
+     function qd_ode_destroy(ode) bind(C, name="qd_ode_destroy")
+       import :: c_int, c_ptr
+       type(c_ptr), value :: ode
+       integer(c_int) :: qd_ode_destroy
+     end function
+     function qd_ode_integrate(ode, y, t, tout, relerr, abserr, iflag) bind(C, name="qd_ode_integrate")
+       import :: c_int, c_ptr, c_double
+       type(c_ptr), value :: ode, y
+       real(c_double) :: t, relerr, abserr
+       real(c_double), value :: tout
+       integer(c_int) :: iflag
+       integer(c_int) :: qd_ode_integrate
+     end function
+     subroutine qd_gs_params_default(p) bind(C, name="qd_gs_params_default")
+       import :: qd_gs_params
+       type(qd_gs_params) :: p
+     end subroutine
+     function qd_gs_flow(ctx, h0, hs, p, res, trace, trace_len) bind(C, name="qd_gs_flow")
+       import :: c_int, c_ptr, c_double, qd_gs_params, qd_gs_result
+       type(c_ptr), value :: ctx, h0, hs
+       type(qd_gs_params) :: p
+       type(qd_gs_result) :: res
+       real(c_double) :: trace(*)
+       integer(c_int), value :: trace_len
+       integer(c_int) :: qd_gs_flow
+     end function
+     ! ---- batched and precision forms ----
+     function qd_set_graphs(ctx, on) bind(C, name="qd_set_graphs")
+       import :: c_int, c_ptr
+       type(c_ptr), value :: ctx
+       integer(c_int), value :: on
+       integer(c_int) :: qd_set_graphs
+     end function
+     function qd_autotune(ctx, reps, ms_before, ms_after) bind(C, name="qd_autotune")
+       import :: c_int, c_ptr, c_double
+       type(c_ptr), value :: ctx
+       integer(c_int), value :: reps
+       real(c_double) :: ms_before, ms_after
+       integer(c_int) :: qd_autotune
+     end function
+     function qd_set_precision(ctx, prec) bind(C, name="qd_set_precision")
+       import :: c_int, c_ptr
+       type(c_ptr), value :: ctx
+       integer(c_int), value :: prec
+       integer(c_int) :: qd_set_precision
      end function
      ! ---- input side ----
      function qd_hf_hamiltonian(nbody, nshell, hw, h0, neq, eig, iters) bind(C, name="qd_hf_hamiltonian")
