@@ -135,47 +135,48 @@ void launch_subcoef(hipStream_t s, const DevTab &t, const Ops &o, double2 *bra, 
 // in storage order: [f1, V2] - [f2, V1] in one pass over the substitution
 // lists.  Only substitutes t in the (m_l, m_s) channel of the replaced index
 // can give a non-zero element, which is what the precomputed lists
-// enumerate.  Bra substitutions read the LDS tiles (column j of the block);
-// ket substitutions read block row Ic from global memory, coalesced over
-// the lanes (consecutive rows), at offsets from three per-thread (base,
-// stride) pairs, one per ket pair type.
-__device__ __forceinline__ double c122(const DevTab &t, const Ops &o, const double *__restrict__ tile1,
-                                       const double *__restrict__ tile2, const double2 *__restrict__ sbra,
-                                       const double2 *__restrict__ sket, int ntot, int j, int q, int nh, int nb,
-                                       int np, int gI, int gJ, int Ic) {
-  double r = 0.0;
-  for (int k = t.sub_start[gI], ke = t.sub_start[gI + 1]; k < ke; k++) {
-    const int Pp = (int)(t.sub_list[k] & 0xffffu);
-    const double2 c = sbra[k];
-    r += c.x * tile2[j * ntot + Pp] - c.y * tile1[j * ntot + Pp];
-  }
-  // offsets of (Ic, Q) for Q of type tq (0 hh, 1 ph, 2 pp), index iq in its class:
-  // a = base[tq] + iq * mul[tq]; stored transposed (herm factor) when tp < tq
-  const int64_t *o6 = t.off + 6 * q;
-  const int tp = Ic < nh ? 0 : (Ic < nh + nb ? 1 : 2);
-  const int ip = Ic - (tp == 0 ? 0 : (tp == 1 ? nh : nh + nb));
-  const int rows_p = tp == 0 ? nh : (tp == 1 ? nb : np);
+// enumerate.  Bra substitutions read the LDS tiles (columns of the block);
+// one thread applies each bra entry to JB consecutive kets of the tile.  Ket
+// substitutions read block row Ic from global memory, coalesced over the
+// lanes (consecutive rows), at offsets from three per-thread (base, stride)
+// pairs, one per ket pair type.
+struct RowAddr {  // offsets of (Ic, Q) for Q of type 0 hh, 1 ph, 2 pp: base + iq * mul
   int64_t b0, b1, b2, m0, m1, m2;
-  if (tp == 0) {
-    b0 = o6[HHHH] + ip; m0 = nh;
-    b1 = o6[PHHH] + (int64_t)ip * nb; m1 = 1;
-    b2 = o6[PPHH] + (int64_t)ip * np; m2 = 1;
-  } else if (tp == 1) {
-    b0 = o6[PHHH] + ip; m0 = rows_p;
-    b1 = o6[PHPH] + ip; m1 = rows_p;
-    b2 = o6[PPPH] + (int64_t)ip * np; m2 = 1;
+  int tp;
+};
+
+__device__ __forceinline__ RowAddr row_addr(const DevTab &t, int q, int nh, int nb, int np, int Ic) {
+  const int64_t *o6 = t.off + 6 * q;
+  RowAddr r;
+  r.tp = Ic < nh ? 0 : (Ic < nh + nb ? 1 : 2);
+  const int ip = Ic - (r.tp == 0 ? 0 : (r.tp == 1 ? nh : nh + nb));
+  const int rows_p = r.tp == 0 ? nh : (r.tp == 1 ? nb : np);
+  if (r.tp == 0) {
+    r.b0 = o6[HHHH] + ip; r.m0 = nh;
+    r.b1 = o6[PHHH] + (int64_t)ip * nb; r.m1 = 1;
+    r.b2 = o6[PPHH] + (int64_t)ip * np; r.m2 = 1;
+  } else if (r.tp == 1) {
+    r.b0 = o6[PHHH] + ip; r.m0 = rows_p;
+    r.b1 = o6[PHPH] + ip; r.m1 = rows_p;
+    r.b2 = o6[PPPH] + (int64_t)ip * np; r.m2 = 1;
   } else {
-    b0 = o6[PPHH] + ip; m0 = rows_p;
-    b1 = o6[PPPH] + ip; m1 = rows_p;
-    b2 = o6[PPPP] + ip; m2 = rows_p;
+    r.b0 = o6[PPHH] + ip; r.m0 = rows_p;
+    r.b1 = o6[PPPH] + ip; r.m1 = rows_p;
+    r.b2 = o6[PPPP] + ip; r.m2 = rows_p;
   }
+  return r;
+}
+
+__device__ __forceinline__ double c122_ket(const DevTab &t, const Ops &o, const double2 *__restrict__ sket,
+                                           const RowAddr &ra, int nh, int nb, int gJ) {
+  double r = 0.0;
   const double h1 = o.h1, h2 = o.h2;
   for (int k = t.sub_start[gJ], ke = t.sub_start[gJ + 1]; k < ke; k++) {
     const int Qp = (int)(t.sub_list[k] & 0xffffu);
     const int tq = Qp < nh ? 0 : (Qp < nh + nb ? 1 : 2);
     const int iq = Qp - (tq == 0 ? 0 : (tq == 1 ? nh : nh + nb));
-    const int64_t a = tq == 0 ? b0 + iq * m0 : (tq == 1 ? b1 + iq * m1 : b2 + iq * m2);
-    const bool tr = tp < tq;
+    const int64_t a = tq == 0 ? ra.b0 + iq * ra.m0 : (tq == 1 ? ra.b1 + iq * ra.m1 : ra.b2 + iq * ra.m2);
+    const bool tr = ra.tp < tq;
     const double v2 = tr ? h2 * o.v2[a] : o.v2[a], v1 = tr ? h1 * o.v1[a] : o.v1[a];
     const double2 c = sket[k];
     r += c.x * v2 - c.y * v1;
@@ -249,9 +250,11 @@ __global__ void __launch_bounds__(256) k_twobody(DevTab t, Ops o, double *__rest
   }
   const int nrows = ntot - tstart;  // row types stored against this ket type
   const int64_t *off = t.off + 6 * q;
-  for (int e = threadIdx.x; e < nrows * tj; e += 256) {
-    const int Ic = tstart + e % nrows, j = e / nrows;
-    const int Jl = j0 + j, Jc = tstart + Jl;
+  constexpr int JB = 4;  // kets per thread
+  const int njb = (tj + JB - 1) / JB;
+  const double *tile1 = lds, *tile2 = lds + ntot * tj;
+  for (int e = threadIdx.x; e < nrows * njb; e += 256) {
+    const int Ic = tstart + e % nrows, jb = (e / nrows) * JB;
     const int rho = Ic < nh ? 0 : (Ic < nh + nb ? 1 : 2);
     const int Il = Ic - (rho == 0 ? 0 : (rho == 1 ? nh : nh + nb));
     const int rows = rho == 0 ? nh : (rho == 1 ? nb : np);
@@ -259,34 +262,49 @@ __global__ void __launch_bounds__(256) k_twobody(DevTab t, Ops o, double *__rest
     if (tau == 0) sub = rho == 0 ? HHHH : (rho == 1 ? PHHH : PPHH);
     else if (tau == 1) sub = rho == 1 ? PHPH : PPPH;
     else sub = PPPP;
-    const int64_t idx = off[sub] + Il + (int64_t)Jl * rows;
-    const uint32_t bra = t.pairs[base + Ic], ket = t.pairs[base + Jc];
-    const int x1 = lo16(bra), x2 = hi16(bra), x3 = lo16(ket), x4 = hi16(ket);
-    double acc = (flags & TB_ACCUM) ? out[idx] : 0.0;
+    const int nj = tj - jb < JB ? tj - jb : JB;
+    double bsum[JB] = {0.0, 0.0, 0.0, 0.0};
+    RowAddr ra{};
     if (do122) {
-      acc += c122(t, o, lds, lds + ntot * tj, sbra, sket, ntot, j, q, nh, nb, np, base + Ic, base + Jc, Ic);
-    }
-    if (flags & TB_LADDER) {
-      // transposed / signed combinations of the products of matrix_mult222
-      // (src/commutators.f90:999-1127) that cannot be accumulated by the GEMM
-      const double h1 = o.h1, h2 = o.h2;
-      const int64_t tr = off[sub] + Jl + (int64_t)Il * rows;  // (J,I) of a square sub-block
-      switch (sub) {
-        case HHHH:
-        case PPPP:
-          acc += h1 * w1[tr] - h2 * w1[idx];
-          acc += h1 * h2 * w3[tr] - w3[idx];
-          break;
-        case PHHH: acc += h1 * w2[idx] - h2 * w1[idx]; break;
-        case PPPH: acc += -h2 * w1[idx] + h1 * w2[idx]; break;
-        case PHPH:
-          acc += h1 * w2[tr] - h2 * w2[idx];
-          acc += h1 * w1[tr] - h2 * w1[idx];
-          break;
-        default: break;
+      const int gI = base + Ic;
+      for (int k = t.sub_start[gI], ke = t.sub_start[gI + 1]; k < ke; k++) {
+        const int Pp = (int)(t.sub_list[k] & 0xffffu);
+        const double2 c = sbra[k];
+#pragma unroll
+        for (int b = 0; b < JB; b++)
+          if (b < nj) bsum[b] += c.x * tile2[(jb + b) * ntot + Pp] - c.y * tile1[(jb + b) * ntot + Pp];
       }
+      ra = row_addr(t, q, nh, nb, np, Ic);
     }
-    out[idx] = acc;
+#pragma unroll
+    for (int b = 0; b < JB; b++) {
+      if (b >= nj) break;
+      const int Jl = j0 + jb + b, Jc = tstart + Jl;
+      const int64_t idx = off[sub] + Il + (int64_t)Jl * rows;
+      double acc = (flags & TB_ACCUM) ? out[idx] : 0.0;
+      if (do122) acc += bsum[b] + c122_ket(t, o, sket, ra, nh, nb, base + Jc);
+      if (flags & TB_LADDER) {
+        // transposed / signed combinations of the products of matrix_mult222
+        // (src/commutators.f90:999-1127) that cannot be accumulated by the GEMM
+        const double h1 = o.h1, h2 = o.h2;
+        const int64_t tr = off[sub] + Jl + (int64_t)Il * rows;  // (J,I) of a square sub-block
+        switch (sub) {
+          case HHHH:
+          case PPPP:
+            acc += h1 * w1[tr] - h2 * w1[idx];
+            acc += h1 * h2 * w3[tr] - w3[idx];
+            break;
+          case PHHH: acc += h1 * w2[idx] - h2 * w1[idx]; break;
+          case PPPH: acc += -h2 * w1[idx] + h1 * w2[idx]; break;
+          case PHPH:
+            acc += h1 * w2[tr] - h2 * w2[idx];
+            acc += h1 * w1[tr] - h2 * w1[idx];
+            break;
+          default: break;
+        }
+      }
+      out[idx] = acc;
+    }
   }
 }
 

@@ -40,6 +40,7 @@ program run_gs
   call qd_check(qd_hf_hamiltonian(int(n, c_int), int(emax, c_int), real(hw, c_double), vec, info%neq, eig, it), &
        'Hartree-Fock')
   print *, 'Hartree-Fock Energy: ', vec(1)
+  flush(6)
   call qd_check(qd_op_create(ctx, 1_c_int, h0), 'op')
   call qd_check(qd_op_create(ctx, 1_c_int, hs), 'op')
   call qd_check(qd_op_upload(h0, vec, info%neq), 'upload')
