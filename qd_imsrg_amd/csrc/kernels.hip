@@ -250,7 +250,7 @@ __global__ void __launch_bounds__(256) k_twobody(DevTab t, Ops o, double *__rest
   }
   const int nrows = ntot - tstart;  // row types stored against this ket type
   const int64_t *off = t.off + 6 * q;
-  constexpr int JB = 4;  // kets per thread
+  constexpr int JB = 8;  // kets per thread
   const int njb = (tj + JB - 1) / JB;
   const double *tile1 = lds, *tile2 = lds + ntot * tj;
   for (int e = threadIdx.x; e < nrows * njb; e += 256) {
@@ -263,7 +263,7 @@ __global__ void __launch_bounds__(256) k_twobody(DevTab t, Ops o, double *__rest
     else if (tau == 1) sub = rho == 1 ? PHPH : PPPH;
     else sub = PPPP;
     const int nj = tj - jb < JB ? tj - jb : JB;
-    double bsum[JB] = {0.0, 0.0, 0.0, 0.0};
+    double bsum[JB] = {};
     RowAddr ra{};
     if (do122) {
       const int gI = base + Ic;
