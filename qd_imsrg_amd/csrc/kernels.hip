@@ -171,6 +171,7 @@ __device__ __forceinline__ double c122_ket(const DevTab &t, const Ops &o, const 
                                            const RowAddr &ra, int nh, int nb, int gJ) {
   double r = 0.0;
   const double h1 = o.h1, h2 = o.h2;
+#pragma unroll 4
   for (int k = t.sub_start[gJ], ke = t.sub_start[gJ + 1]; k < ke; k++) {
     const int Qp = (int)(t.sub_list[k] & 0xffffu);
     const int tq = Qp < nh ? 0 : (Qp < nh + nb ? 1 : 2);
@@ -267,6 +268,7 @@ __global__ void __launch_bounds__(256) k_twobody(DevTab t, Ops o, double *__rest
     RowAddr ra{};
     if (do122) {
       const int gI = base + Ic;
+#pragma unroll 4
       for (int k = t.sub_start[gI], ke = t.sub_start[gI + 1]; k < ke; k++) {
         const int Pp = (int)(t.sub_list[k] & 0xffffu);
         const double2 c = sbra[k];
