@@ -35,7 +35,7 @@ struct GemmProb {  // C(M,N) (+)= sum over segments
   int c_slot, ldc, M, N, beta1, seg0, nseg, shape;
 };
 struct GemmTile { int prob, tm, tn; };
-constexpr int GEMM_TK = 16;
+constexpr int GEMM_TK = 32;
 // tile shapes of the grouped GEMM (rows x cols of C per work-group)
 enum GemmShape { SH_64x64 = 0, SH_128x64, SH_32x32, SH_128x16, SH_64x16, SH_16x64, SH_16x128, NSHAPE };
 constexpr int SHAPE_TM[NSHAPE] = {64, 128, 32, 128, 64, 16, 16};
