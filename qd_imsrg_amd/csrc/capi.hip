@@ -88,7 +88,8 @@ struct qd_context {
   double *partials = nullptr, *scal = nullptr;  // reduction partials, 8 device scalars
   double *host_scal = nullptr;                  // pinned mirror of scal
   Task2 *tasks2 = nullptr; int ntasks2 = 0;
-  TilePair *ptp = nullptr; int nptp = 0;  // tile pairs of the Pandya products
+  TilePair *ptp = nullptr; int nptp = 0;  // 32x32 tile pairs of the Pandya combine
+  TilePair *ftp = nullptr; int nftp = 0;  // 64x64 tile pairs of the fused Pandya product + combine
   Seg2 *pphh_segs = nullptr; Tile2 *pphh_tiles = nullptr; int npphh_segs = 0, npphh_tiles = 0;
   Seg2 *cc_segs = nullptr; Tile2 *cc_tiles = nullptr; int ncc_tiles = 0;
   GemmList g_all, g_acc;  // products (ladder + Pandya) and accumulating products
@@ -415,6 +416,19 @@ int build_worklists(qd_context *c) {
     }
     if ((rc = dev_upload(c, tp, &c->ptp))) return rc;
     c->nptp = (int)tp.size();
+    // the fused pass: largest blocks first, so the persistent tail is short
+    std::vector<std::pair<int, int>> order;
+    for (int q = 0; q < ch.ncc; q++)
+      if (ch.cc_r[q] * ch.cc_nb[q]) order.push_back({ch.cc_r[q] * ch.cc_r[q] * ch.cc_nb[q], q});
+    std::stable_sort(order.begin(), order.end(), [](auto &a, auto &b) { return a.first > b.first; });
+    std::vector<TilePair> ftp;
+    for (auto &oq : order) {
+      const int q = oq.second, nt = (ch.cc_r[q] + 63) / 64;
+      for (int ti = 0; ti < nt; ti++)
+        for (int tj = ti; tj < nt; tj++) ftp.push_back({q, ti, tj});
+    }
+    if ((rc = dev_upload(c, ftp, &c->ftp))) return rc;
+    c->nftp = (int)ftp.size();
   }
   if ((rc = upload_gemm(c, g1, npre, f1, c->g_all))) return rc;
   if ((rc = upload_gemm(c, g2, (int)g2.probs.size(), g2.flops, c->g_acc))) return rc;
@@ -467,12 +481,19 @@ int run_gemm_list(qd_context *c, const GemmList &g, bool with_pandya, const Slot
 int run_products(qd_context *c, const qd_op *r1, const qd_op *r2, qd_op *out, const qd_cc *hcc, const qd_cc *ecc) {
   const bool pd = pandya_on(hcc, ecc);
   Slots s = make_slots(c, r1, r2, out, ecc, hcc);
+  // grouped FP64: the Pandya product and its combine run fused; per-channel
+  // launches and the split precisions keep product GEMM + combine pass
+  const bool fused = pd && c->launch_mode == 0 && c->gemm_prec == GEMM_FP64;
   {
     Timer tm(c, TC_PRODUCTS);
-    int rc = run_gemm_list(c, c->g_all, pd, s);
+    int rc = run_gemm_list(c, c->g_all, pd && !fused, s);
     if (rc) return rc;
+    if (fused) {
+      launch_pandya_fused(c->stream, c->dt, ecc->d, hcc->d, c->P, c->ftp, c->nftp, hcc->herm, ecc->herm);
+      HIPC(hipGetLastError());
+    }
   }
-  if (pd) {
+  if (pd && !fused) {
     Timer tc(c, TC_OTHER);
     launch_pandya_combine(c->stream, c->dt, c->P, c->ptp, c->nptp, hcc->herm, ecc->herm);
     HIPC(hipGetLastError());

@@ -439,4 +439,120 @@ void launch_gemm(hipStream_t s, const GemmProb *probs, const GemmSeg *segs, cons
   }
 }
 
+// ---- Pandya product and combine in one pass (FP64)
+//
+// construct_cc_ints (src/ME_general.f90:923-952) needs D = XCC - YCC =
+// h_H P - h_eta P^T with P = ETACC HCC^T per CC block.  One work-group takes
+// a pair of 64x64 tiles (ti <= tj) of a block: it computes P_ij = E_i H_j^T
+// and P_ji = E_j H_i^T (K staged through LDS in chunks of 16, the next chunk
+// prefetched into registers, k ascending as in the grouped GEMM, so P is the
+// same to the bit), then writes D_ij = h_H P_ij - h_eta P_ji^T and
+// D_ji = h_H P_ji - h_eta P_ij^T, exchanging the transposed tile through LDS.
+// The product matrix P is never written.
+__global__ void __launch_bounds__(256) k_pandya_fused(DevTab t, const double *__restrict__ E,
+                                                      const double *__restrict__ H, double *__restrict__ D,
+                                                      const TilePair *__restrict__ tp, double hh, double eh) {
+  constexpr int T = 64, TK = 16, LD = T + 1;
+  __shared__ double lds[4 * TK * LD];  // E_i, E_j, H_i, H_j as [TK][LD]; later one [T][LD] tile
+  const TilePair pr = tp[blockIdx.x];
+  const int q = pr.blk, r = t.cc_r[q], K = t.cc_nb[q];
+  const double *__restrict__ Eq = E + t.cc_off[q];
+  const double *__restrict__ Hq = H + t.cc_off[q];
+  double *__restrict__ Dq = D + t.cc_poff[q];
+  const bool diag = pr.ti == pr.tj;
+  const int i0 = pr.ti * T, j0 = pr.tj * T;
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int lr = lane & 15, lk = lane >> 4;
+  const int m0 = (wave & 1) * 32, n0 = (wave >> 1) * 32;
+  double *Ei = lds, *Ej = lds + TK * LD, *Hi = lds + 2 * TK * LD, *Hj = lds + 3 * TK * LD;
+  d4 a1[2][2], a2[2][2];
+#pragma unroll
+  for (int x = 0; x < 2; x++)
+#pragma unroll
+    for (int y = 0; y < 2; y++) a1[x][y] = a2[x][y] = d4{0, 0, 0, 0};
+  // staging: element (row x, k) of a panel is M[x + k r] (column-major r x K)
+  double rg[4][4];
+  auto load = [&](int k0) {
+#pragma unroll
+    for (int u = 0; u < 4; u++) {
+      const int idx = tid + 256 * u, x = idx % T, k = idx / T, gk = k0 + k;
+      const bool kin = gk < K;
+      rg[0][u] = (kin && i0 + x < r) ? Eq[(i0 + x) + (int64_t)gk * r] : 0.0;
+      rg[1][u] = (kin && j0 + x < r) ? Eq[(j0 + x) + (int64_t)gk * r] : 0.0;
+      rg[2][u] = (kin && i0 + x < r) ? Hq[(i0 + x) + (int64_t)gk * r] : 0.0;
+      rg[3][u] = (kin && j0 + x < r) ? Hq[(j0 + x) + (int64_t)gk * r] : 0.0;
+    }
+  };
+  load(0);
+  for (int k0 = 0; k0 < K; k0 += TK) {
+    __syncthreads();
+#pragma unroll
+    for (int u = 0; u < 4; u++) {
+      const int idx = tid + 256 * u, x = idx % T, k = idx / T;
+      Ei[k * LD + x] = rg[0][u];
+      Ej[k * LD + x] = rg[1][u];
+      Hi[k * LD + x] = rg[2][u];
+      Hj[k * LD + x] = rg[3][u];
+    }
+    __syncthreads();
+    if (k0 + TK < K) load(k0 + TK);
+#pragma unroll
+    for (int kk = 0; kk < TK; kk += 4) {
+      double ei[2], hj[2], ej[2], hi[2];
+#pragma unroll
+      for (int x = 0; x < 2; x++) {
+        ei[x] = Ei[(kk + lk) * LD + m0 + x * 16 + lr];
+        hj[x] = Hj[(kk + lk) * LD + n0 + x * 16 + lr];
+        ej[x] = Ej[(kk + lk) * LD + m0 + x * 16 + lr];
+        hi[x] = Hi[(kk + lk) * LD + n0 + x * 16 + lr];
+      }
+#pragma unroll
+      for (int x = 0; x < 2; x++)
+#pragma unroll
+        for (int y = 0; y < 2; y++) {
+          a1[x][y] = __builtin_amdgcn_mfma_f64_16x16x4f64(ei[x], hj[y], a1[x][y], 0, 0, 0);
+          if (!diag) a2[x][y] = __builtin_amdgcn_mfma_f64_16x16x4f64(ej[x], hi[y], a2[x][y], 0, 0, 0);
+        }
+    }
+  }
+  // epilogue; FP64 MFMA output: lane l, component v holds row 4 v + l / 16, column l % 16
+  double *Tt = lds;  // [T][LD]
+  auto put = [&](const d4 (&a)[2][2]) {
+#pragma unroll
+    for (int x = 0; x < 2; x++)
+#pragma unroll
+      for (int y = 0; y < 2; y++)
+#pragma unroll
+        for (int v = 0; v < 4; v++) Tt[(m0 + x * 16 + 4 * v + lk) * LD + n0 + y * 16 + lr] = a[x][y][v];
+  };
+  auto out = [&](const d4 (&a)[2][2], int r0, int c0) {  // D[r0 + x][c0 + y] = hh a[x][y] - eh Tt[y][x]
+#pragma unroll
+    for (int x = 0; x < 2; x++)
+#pragma unroll
+      for (int y = 0; y < 2; y++)
+#pragma unroll
+        for (int v = 0; v < 4; v++) {
+          const int lx = m0 + x * 16 + 4 * v + lk, ly = n0 + y * 16 + lr;
+          if (r0 + lx < r && c0 + ly < r)
+            Dq[(r0 + lx) + (int64_t)(c0 + ly) * r] = hh * a[x][y][v] - eh * Tt[ly * LD + lx];
+        }
+  };
+  __syncthreads();
+  put(diag ? a1 : a2);
+  __syncthreads();
+  out(a1, i0, j0);
+  if (!diag) {
+    __syncthreads();
+    put(a1);
+    __syncthreads();
+    out(a2, j0, i0);
+  }
+}
+
+void launch_pandya_fused(hipStream_t s, const DevTab &t, const double *etacc, const double *hcc, double *D,
+                         const TilePair *tp, int ntp, double hh, double eh) {
+  if (ntp <= 0) return;
+  hipLaunchKernelGGL(k_pandya_fused, dim3(ntp), dim3(256), 0, s, t, etacc, hcc, D, tp, hh, eh);
+}
+
 }  // namespace qd

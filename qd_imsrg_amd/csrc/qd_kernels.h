@@ -79,6 +79,10 @@ void launch_white(hipStream_t s, const DevTab &t, const double *rec, int hrec, d
                   const Seg2 *pphh_segs, const Tile2 *pphh_tiles, int ntiles);
 void launch_pandya_combine(hipStream_t s, const DevTab &t, double *P, const TilePair *tp, int ntp, double hh,
                            double eh);
+// D = hh * E H^T - eh * (E H^T)^T per CC block straight from the CC sets
+// (E = ETACC, H = HCC), over 64x64 tile pairs (ti <= tj)
+void launch_pandya_fused(hipStream_t s, const DevTab &t, const double *etacc, const double *hcc, double *D,
+                         const TilePair *tp, int ntp, double hh, double eh);
 void launch_axpby(hipStream_t s, int64_t n, double s1, const double *x, double s2, const double *y, double *z);
 // sum of squares of v[1..neq) -> *dst, via partials (deterministic)
 void launch_norm2(hipStream_t s, const double *v, int64_t neq, double *partials, double *dst);
