@@ -88,6 +88,7 @@ struct qd_context {
   double *partials = nullptr, *scal = nullptr;  // reduction partials, 8 device scalars
   double *host_scal = nullptr;                  // pinned mirror of scal
   Task2 *tasks2 = nullptr; int ntasks2 = 0;
+  PTask *ptasks = nullptr; int nptasks = 0; int4 *pgroups = nullptr;  // Pandya gather work
   TilePair *ptp = nullptr; int nptp = 0;  // 32x32 tile pairs of the Pandya combine
   TilePair *ftp = nullptr; int nftp = 0;  // 64x64 tile pairs of the fused Pandya product + combine
   Seg2 *pphh_segs = nullptr; Tile2 *pphh_tiles = nullptr; int npphh_segs = 0, npphh_tiles = 0;
@@ -295,6 +296,8 @@ int build_worklists(qd_context *c) {
   std::vector<Seg2> pseg;
   std::vector<Tile2> ptiles;
   std::vector<std::vector<Task2>> per_block(ch.nblock);
+  std::vector<std::vector<PTask>> pg_block(ch.nblock);
+  std::vector<int4> groups;
   std::vector<double> work(ch.nblock, 0.0);
   for (int q = 0; q < ch.nblock; q++) {
     const int nh = ch.nh[q], np = ch.np[q], nb = ch.nb[q], ntot = nh + nb + np;
@@ -307,6 +310,38 @@ int build_worklists(qd_context *c) {
       if (ncol[tau] == 0 || nrows == 0) continue;
       for (int j0 = 0; j0 < ncol[tau]; j0 += tj) per_block[q].push_back({q, tau, j0, std::min(tj, ncol[tau] - j0)});
       work[q] += (double)nrows * ncol[tau];
+    }
+    // Pandya gather: groups of equal (pair class, channel of i, channel of j)
+    // along the rowperm order; every ket group is paired with bundles of the
+    // row groups of classes >= its own
+    {
+      const int base = ch.hh_start[q], g_first = (int)groups.size();
+      int gstart[4] = {0, 0, 0, 0};  // first group of each pair class, relative to g_first
+      for (int cl = 0, pos = 0; cl < 3; cl++) {
+        gstart[cl] = (int)groups.size() - g_first;
+        for (; pos < start[cl] + ncol[cl];) {
+          const uint32_t p0 = ch.pairs[base + ch.rowperm[base + pos]];
+          int len = 1;
+          while (pos + len < start[cl] + ncol[cl]) {
+            const uint32_t p1 = ch.pairs[base + ch.rowperm[base + pos + len]];
+            if (ch.spch[lo16(p1)] != ch.spch[lo16(p0)] || ch.spch[hi16(p1)] != ch.spch[hi16(p0)]) break;
+            len++;
+          }
+          groups.push_back(make_int4(pos, len, ch.spch[lo16(p0)], ch.spch[hi16(p0)]));
+          pos += len;
+        }
+      }
+      gstart[3] = (int)groups.size() - g_first;
+      for (int tau = 0; tau < 3; tau++)
+        for (int gc = gstart[tau]; gc < gstart[tau + 1]; gc++) {
+          const int4 col = groups[g_first + gc];
+          for (int g = gstart[tau]; g < gstart[3];) {
+            int ng = 0, elems = 0;
+            while (g + ng < gstart[3] && ng < PG_MAX && elems < PG_TARGET) elems += groups[g_first + g + ng++].y * col.y;
+            pg_block[q].push_back(PTask{q, tau, g_first + gc, g_first + g, ng});
+            g += ng;
+          }
+        }
     }
     if (np * nh) {
       Seg2 sg{ch.off[(size_t)q * 6 + PPHH], np, np * nh, q, PPHH, ch.pp_start[q], ch.hh_start[q]};
@@ -333,6 +368,24 @@ int build_worklists(qd_context *c) {
       for (size_t k = 0; k < xcd[x].size(); k++) tasks[k * NXCD + x] = xcd[x][k];
     if ((rc = dev_upload(c, tasks, &c->tasks2))) return rc;
     c->ntasks2 = (int)tasks.size();
+    // the same block -> XCD assignment for the Pandya gather
+    std::vector<std::vector<PTask>> pxcd(NXCD);
+    std::fill(load.begin(), load.end(), 0.0);
+    for (int q : order) {
+      if (per_block[q].empty()) continue;
+      int best = (int)(std::min_element(load.begin(), load.end()) - load.begin());
+      load[best] += work[q];
+      pxcd[best].insert(pxcd[best].end(), pg_block[q].begin(), pg_block[q].end());
+    }
+    longest = 0;
+    for (auto &l : pxcd) longest = std::max(longest, l.size());
+    std::vector<PTask> ptasks(longest * NXCD, PTask{-1, 0, 0, 0, 0});
+    for (int x = 0; x < NXCD; x++)
+      for (size_t k = 0; k < pxcd[x].size(); k++) ptasks[k * NXCD + x] = pxcd[x][k];
+    if (groups.empty()) groups.push_back(make_int4(0, 0, 0, 0));
+    if ((rc = dev_upload(c, ptasks, &c->ptasks))) return rc;
+    if ((rc = dev_upload(c, groups, &c->pgroups))) return rc;
+    c->nptasks = (int)ptasks.size();
   }
   if ((rc = dev_upload(c, pseg, &c->pphh_segs))) return rc;
   if ((rc = dev_upload(c, ptiles, &c->pphh_tiles))) return rc;
@@ -522,7 +575,8 @@ int run_twobody(qd_context *c, const qd_op *r1, const qd_op *r2, qd_op *out, int
   // of the combined product compact; it adds to what the pass above wrote
   if (flags & TB_PANDYA) {
     Timer tm(c, TC_PANDYA);
-    launch_pandya_gather(c->stream, c->dt, out->d, c->tasks2, c->ntasks2, c->P, other || (flags & TB_ACCUM));
+    launch_pandya_gather(c->stream, c->dt, out->d, c->ptasks, c->nptasks, c->pgroups, c->P,
+                         other || (flags & TB_ACCUM));
     HIPC(hipGetLastError());
   }
   return QD_OK;
@@ -667,15 +721,18 @@ int qd_context_create(int nbody, int nshell, qd_context **out) {
   t.ml0 = ch.ml0; t.mlspan = ch.mlspan; t.neq = ch.neq;
   t.off_fhh = ch.off_fhh; t.off_fpp = ch.off_fpp; t.off_fph = ch.off_fph;
   std::vector<int2> ccmap((size_t)ch.M * ch.M);
-  for (size_t i = 0; i < ccmap.size(); i++) ccmap[i] = make_int2(ch.cc_map[2 * i], ch.cc_map[2 * i + 1]);
+  for (size_t i = 0; i < ccmap.size(); i++) {
+    const int blk = ch.cc_map[2 * i], row = ch.cc_map[2 * i + 1];
+    ccmap[i] = make_int2(blk, blk < 0 ? -1 : ch.cc_dev_row[ch.cc_ab_start[blk] + row]);
+  }
 #define UP(vec, field) if ((rc = dev_upload(c, vec, (decltype(vec)::value_type **)&t.field))) return bail(rc)
   UP(ch.nh, nh); UP(ch.np, np); UP(ch.nb, nb); UP(ch.off, off); UP(ch.pairs, pairs);
   UP(ch.hh_start, hh_start); UP(ch.ph_start, ph_start); UP(ch.pp_start, pp_start); UP(ch.iarr, iarr);
   UP(ch.spch, spch); UP(ch.ch_start, ch_start); UP(ch.ch_states, ch_states); UP(ch.ch_lookup, ch_lookup);
   UP(ch.ml, ml); UP(ch.ms, ms); UP(ch.cc_r, cc_r); UP(ch.cc_nb, cc_nb); UP(ch.cc_off, cc_off);
-  UP(ch.cc_poff, cc_poff); UP(ch.cc_ab, cc_ab); UP(ch.cc_hp, cc_hp); UP(ch.cc_ab_start, cc_ab_start);
+  UP(ch.cc_poff, cc_poff); UP(ch.cc_ab_dev, cc_ab); UP(ch.cc_hp, cc_hp); UP(ch.cc_ab_start, cc_ab_start);
   UP(ch.cc_hp_start, cc_hp_start); UP(ccmap, cc_map); UP(ch.sub_start, sub_start); UP(ch.sub_list, sub_list);
-  UP(ch.rowperm, rowperm);
+  UP(ch.rowperm, rowperm); UP(ch.pinfo, pinfo); UP(ch.cc_rs, cc_rs);
 #undef UP
   void *p;
   if ((rc = dev_alloc(c, sizeof(double) * ch.neq, &p))) return bail(rc); c->w1 = (double *)p;
@@ -805,10 +862,27 @@ int qd_cc_set_herm(qd_cc *cc, double herm) {
   cc->herm = herm;
   return QD_OK;
 }
+// Row permutation between the reference's CC layout (host) and the device
+// row order (every block r x nb column-major at cc_off).
+static void cc_rows(const Channels &ch, const double *from, double *to, bool to_device) {
+  for (int b = 0; b < ch.ncc; b++) {
+    const int r = ch.cc_r[b], nb = ch.cc_nb[b], s0 = ch.cc_ab_start[b];
+    const int64_t o = ch.cc_off[b];
+    for (int col = 0; col < nb; col++)
+      for (int a = 0; a < r; a++) {
+        const int64_t ref = o + a + (int64_t)col * r, dev = o + ch.cc_dev_row[s0 + a] + (int64_t)col * r;
+        if (to_device) to[dev] = from[ref];
+        else to[ref] = from[dev];
+      }
+  }
+}
+
 int qd_cc_upload(qd_cc *cc, const double *x, int64_t n) {
   if (!cc || !x) return fail(QD_E_ARG, "null argument");
   if (n != cc->ctx->ch.cc_size) return fail(QD_E_ARG, "cc_size mismatch");
-  HIPC(hipMemcpyAsync(cc->d, x, sizeof(double) * n, hipMemcpyHostToDevice, cc->ctx->stream));
+  std::vector<double> dev((size_t)n);
+  cc_rows(cc->ctx->ch, x, dev.data(), true);
+  HIPC(hipMemcpyAsync(cc->d, dev.data(), sizeof(double) * n, hipMemcpyHostToDevice, cc->ctx->stream));
   HIPC(hipStreamSynchronize(cc->ctx->stream));
   cc->zero = false;
   return QD_OK;
@@ -816,8 +890,10 @@ int qd_cc_upload(qd_cc *cc, const double *x, int64_t n) {
 int qd_cc_download(const qd_cc *cc, double *x, int64_t n) {
   if (!cc || !x) return fail(QD_E_ARG, "null argument");
   if (n != cc->ctx->ch.cc_size) return fail(QD_E_ARG, "cc_size mismatch");
-  HIPC(hipMemcpyAsync(x, cc->d, sizeof(double) * n, hipMemcpyDeviceToHost, cc->ctx->stream));
+  std::vector<double> dev((size_t)n);
+  HIPC(hipMemcpyAsync(dev.data(), cc->d, sizeof(double) * n, hipMemcpyDeviceToHost, cc->ctx->stream));
   HIPC(hipStreamSynchronize(cc->ctx->stream));
+  cc_rows(cc->ctx->ch, dev.data(), x, false);
   return QD_OK;
 }
 int qd_cc_zero(qd_context *c, qd_cc *cc) {

@@ -139,6 +139,35 @@ std::string build_channels(int nbody, int nshell, Channels &c) {
   c.ncc = q;
   c.cc_size = coff;
   c.p_size = poff;
+  // device row order: stable sort of each block's rows by the channel of i
+  // (the reference order is i-outer, j-inner, both ascending)
+  c.cc_dev_row.assign(c.cc_ab.size(), 0);
+  c.cc_ab_dev.assign(c.cc_ab.size(), 0u);
+  for (int b = 0; b < c.ncc; b++) {
+    const int s0 = c.cc_ab_start[b], r = c.cc_r[b];
+    std::vector<int> ord(r);
+    for (int a = 0; a < r; a++) ord[a] = a;
+    std::stable_sort(ord.begin(), ord.end(),
+                     [&](int u, int v) { return c.spch[lo16(c.cc_ab[s0 + u])] < c.spch[lo16(c.cc_ab[s0 + v])]; });
+    for (int d = 0; d < r; d++) {
+      c.cc_dev_row[s0 + ord[d]] = d;
+      c.cc_ab_dev[s0 + d] = c.cc_ab[s0 + ord[d]];
+    }
+  }
+  c.sp_pos.assign(M, 0);
+  for (int h = 0; h < c.nch; h++)
+    for (int s = c.ch_start[h]; s < c.ch_start[h + 1]; s++) c.sp_pos[c.ch_states[s]] = s - c.ch_start[h];
+  c.cc_rs.assign((size_t)c.ncc * c.nch, -1);
+  for (int b = 0; b < c.ncc; b++)
+    for (int d = c.cc_r[b] - 1; d >= 0; d--) c.cc_rs[(size_t)b * c.nch + c.spch[lo16(c.cc_ab_dev[c.cc_ab_start[b] + d])]] = d;
+  // the closed form of the device row, relied on by the Pandya gather
+  for (int i = 0; i < M; i++)
+    for (int j = 0; j < M; j++) {
+      const int b = c.cc_map[((size_t)M * i + j) * 2], row = c.cc_map[((size_t)M * i + j) * 2 + 1];
+      const int hj = c.spch[j];
+      const int want = c.cc_rs[(size_t)b * c.nch + c.spch[i]] + c.sp_pos[i] * (c.ch_start[hj + 1] - c.ch_start[hj]) + c.sp_pos[j];
+      if (c.cc_dev_row[c.cc_ab_start[b] + row] != want) return "internal error: CC device row order";
+    }
 
   // ---- substitution lists: pair (i,j) with i or j replaced by a state t of
   // the same (m_l, m_s) channel stays in the same two-body block
@@ -191,6 +220,16 @@ std::string build_channels(int nbody, int nshell, Channels &c) {
         return std::lexicographical_compare(ku, ku + 4, kv, kv + 4);
       });
       for (size_t k = 0; k < idx.size(); k++) c.rowperm[base + bounds[cl] + k] = idx[k];
+    }
+  }
+  c.pinfo.assign(c.pairs.size(), 0u);
+  for (int b = 0; b < c.nblock; b++) {
+    const int base = c.hh_start[b], ntot = c.nh[b] + c.nb[b] + c.np[b];
+    for (int pos = 0; pos < ntot; pos++) {
+      const int I = c.rowperm[base + pos];
+      const uint32_t pr = c.pairs[base + I];
+      if (c.sp_pos[lo16(pr)] > 255 || c.sp_pos[hi16(pr)] > 255) return "single-particle channel too large";
+      c.pinfo[base + pos] = (uint32_t)I | ((uint32_t)c.sp_pos[lo16(pr)] << 16) | ((uint32_t)c.sp_pos[hi16(pr)] << 24);
     }
   }
   return "";

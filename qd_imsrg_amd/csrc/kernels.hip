@@ -320,44 +320,95 @@ void launch_twobody(hipStream_t s, const DevTab &t, const Ops &o, double *out, c
 // The Pandya part of commutator_222 (src/commutators.f90:746-751 and the
 // analogous lines per sub-block): every stored output <x1 x2|V|x3 x4> adds
 // D(x3,x1|x2,x4) - D(x3,x2|x1,x4) + D(x4,x2|x1,x3) - D(x4,x1|x2,x3) of the
-// combined product D (storage-order indices carry the ph sign).  Same tasks
-// as the two-body pass; rows are visited in the block's Pandya row order
-// (grouped by the channels of x1 and x2), so the lanes of a wave cover a
-// compact (x1, x2) grid and each D read touches a few adjacent rows.
+// combined product D (storage-order indices carry the ph sign).  A task is
+// one ket group (x3, x4 in fixed single-particle channels) against a bundle
+// of row groups (x1, x2 in fixed channels).  The device rows of a CC block
+// are sorted by the channel of their first state, so for one (row group,
+// ket group) box each of the four terms reads a dense sub-block of D whose
+// offset is fixed per box: e.g. D(x3,x1|x2,x4) sits at
+// base + pos3 |ch x1| + pos1 + (pos2 |ch x4| + pos4) r, with pos = rank of
+// the state in its channel.  The per-box offsets are set up once in LDS.
 __global__ void __launch_bounds__(256) k_pandya_gather(DevTab t, double *__restrict__ out,
-                                                       const Task2 *__restrict__ tasks,
+                                                       const PTask *__restrict__ tasks,
+                                                       const int4 *__restrict__ groups,
                                                        const double *__restrict__ P, int accumulate) {
-  const Task2 tk = tasks[blockIdx.x];
+  __shared__ int4 gs[PG_MAX];          // row groups: first position, length, channels of x1, x2
+  __shared__ int pre[PG_MAX + 1];      // element offsets of the boxes
+  __shared__ int64_t bbase[PG_MAX][4]; // per box and term: D offset of the sub-block (-1: no D block)
+  __shared__ int bld[PG_MAX][4];       // per box and term: leading dimension r of the D block
+  __shared__ int64_t bout[PG_MAX];     // per box: offset of the output sub-block
+  __shared__ int4 binfo[PG_MAX];       // per box: output rows, first combined index of the rows, |ch x1|, |ch x2|
+  const PTask tk = tasks[blockIdx.x];
   if (tk.block < 0) return;
-  const int q = tk.block, tau = tk.tau, j0 = tk.j0, tj = tk.tj;
-  const int nh = t.nh[q], nb = t.nb[q], np = t.np[q], ntot = nh + nb + np;
+  const int q = tk.block, tau = tk.tau, ng = tk.ng;
+  const int nh = t.nh[q], nb = t.nb[q];
   const int base = t.hh_start[q];
   const int tstart = tau == 0 ? 0 : (tau == 1 ? nh : nh + nb);
-  const int nrows = ntot - tstart;
-  const int64_t *off = t.off + 6 * q;
-  for (int e = threadIdx.x; e < nrows * tj; e += 256) {
-    const int Ic = t.rowperm[base + tstart + e % nrows], j = e / nrows;
-    const int Jl = j0 + j, Jc = tstart + Jl;
-    const int rho = Ic < nh ? 0 : (Ic < nh + nb ? 1 : 2);
-    const int Il = Ic - (rho == 0 ? 0 : (rho == 1 ? nh : nh + nb));
-    const int rows = rho == 0 ? nh : (rho == 1 ? nb : np);
+  const int4 col = groups[tk.gc];
+  const int A3 = col.z, A4 = col.w;
+  const int a3 = t.ch_start[A3 + 1] - t.ch_start[A3], a4 = t.ch_start[A4 + 1] - t.ch_start[A4];
+  if ((int)threadIdx.x < ng) {
+    const int g = threadIdx.x;
+    const int4 rg = groups[tk.g0 + g];
+    gs[g] = rg;
+    const int A1 = rg.z, A2 = rg.w;
+    // (row pair | column pair) channels of D(x3,x1|x2,x4), D(x3,x2|x1,x4),
+    // D(x4,x2|x1,x3), D(x4,x1|x2,x3)
+    const int rc0[4] = {A3, A3, A4, A4}, rc1[4] = {A1, A2, A2, A1}, cc0[4] = {A2, A1, A1, A2};
+    for (int k = 0; k < 4; k++) {
+      const int s0 = t.ch_states[t.ch_start[rc0[k]]], s1 = t.ch_states[t.ch_start[rc1[k]]];
+      const int K = t.cc_map[s0 * t.M + s1].x;
+      bld[g][k] = t.cc_r[K];
+      bbase[g][k] = t.cc_nb[K] == 0 ? -1
+                                     : t.cc_poff[K] + t.cc_rs[K * t.nch + rc0[k]] +
+                                           (int64_t)t.cc_rs[K * t.nch + cc0[k]] * t.cc_r[K];
+    }
+    const int rho = rg.x < nh ? 0 : (rg.x < nh + nb ? 1 : 2);
     int sub;
     if (tau == 0) sub = rho == 0 ? HHHH : (rho == 1 ? PHHH : PPHH);
     else if (tau == 1) sub = rho == 1 ? PHPH : PPPH;
     else sub = PPPP;
-    const int64_t idx = off[sub] + Il + (int64_t)Jl * rows;
-    const uint32_t bra = t.pairs[base + Ic], ket = t.pairs[base + Jc];
-    const int x1 = lo16(bra), x2 = hi16(bra), x3 = lo16(ket), x4 = hi16(ket);
-    const double d = pandya_d(t, P, x3, x1, x2, x4) - pandya_d(t, P, x3, x2, x1, x4) +
-                     pandya_d(t, P, x4, x2, x1, x3) - pandya_d(t, P, x4, x1, x2, x3);
+    bout[g] = t.off[6 * q + sub];
+    binfo[g] = make_int4(rho == 0 ? nh : (rho == 1 ? nb : t.np[q]), rho == 0 ? 0 : (rho == 1 ? nh : nh + nb),
+                         t.ch_start[A1 + 1] - t.ch_start[A1], t.ch_start[A2 + 1] - t.ch_start[A2]);
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    int acc = 0;
+    for (int g = 0; g < ng; g++) { pre[g] = acc; acc += gs[g].y * col.y; }
+    pre[ng] = acc;
+  }
+  __syncthreads();
+  const int total = pre[ng];
+  // elements of one (row group, ket group) box are consecutive, rows fastest
+  for (int e = threadIdx.x; e < total; e += 256) {
+    int lo = 0, hi = ng - 1;
+    while (lo < hi) {
+      const int mid = (lo + hi + 1) >> 1;
+      if (pre[mid] <= e) lo = mid;
+      else hi = mid - 1;
+    }
+    const int le = e - pre[lo], len = gs[lo].y;
+    const uint32_t pr = t.pinfo[base + gs[lo].x + le % len], pc = t.pinfo[base + col.x + le / len];
+    const int Ic = (int)(pr & 0xffffu), p1 = (int)((pr >> 16) & 0xffu), p2 = (int)(pr >> 24);
+    const int Jc = (int)(pc & 0xffffu), p3 = (int)((pc >> 16) & 0xffu), p4 = (int)(pc >> 24);
+    const int4 bi = binfo[lo];
+    const int a1 = bi.z, a2 = bi.w;
+    const int64_t b0 = bbase[lo][0], b1 = bbase[lo][1], b2 = bbase[lo][2], b3 = bbase[lo][3];
+    const double d0 = b0 < 0 ? 0.0 : P[b0 + p3 * a1 + p1 + (int64_t)(p2 * a4 + p4) * bld[lo][0]];
+    const double d1 = b1 < 0 ? 0.0 : P[b1 + p3 * a2 + p2 + (int64_t)(p1 * a4 + p4) * bld[lo][1]];
+    const double d2 = b2 < 0 ? 0.0 : P[b2 + p4 * a2 + p2 + (int64_t)(p1 * a3 + p3) * bld[lo][2]];
+    const double d3 = b3 < 0 ? 0.0 : P[b3 + p4 * a1 + p1 + (int64_t)(p2 * a3 + p3) * bld[lo][3]];
+    const double d = d0 - d1 + d2 - d3;
+    const int64_t idx = bout[lo] + (Ic - bi.y) + (int64_t)(Jc - tstart) * bi.x;
     out[idx] = accumulate ? out[idx] + d : d;
   }
 }
 
-void launch_pandya_gather(hipStream_t s, const DevTab &t, double *out, const Task2 *tasks, int ntasks,
-                          const double *P, bool accumulate) {
+void launch_pandya_gather(hipStream_t s, const DevTab &t, double *out, const PTask *tasks, int ntasks,
+                          const int4 *groups, const double *P, bool accumulate) {
   if (ntasks <= 0) return;
-  hipLaunchKernelGGL(k_pandya_gather, dim3(ntasks), dim3(256), 0, s, t, out, tasks, P, accumulate ? 1 : 0);
+  hipLaunchKernelGGL(k_pandya_gather, dim3(ntasks), dim3(256), 0, s, t, out, tasks, groups, P, accumulate ? 1 : 0);
 }
 
 // ---------------------------------------------------------------------------

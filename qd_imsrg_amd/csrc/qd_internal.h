@@ -48,12 +48,25 @@ struct Channels {
   std::vector<uint32_t> cc_ab, cc_hp;
   std::vector<int> cc_ab_start, cc_hp_start;
   std::vector<int> cc_map;                  // M*M x 2 (block,row), -1 if none
+  // Device row order of every CC block: rows sorted by (channel of i, i, j),
+  // so the rows (i, j) with i in one single-particle channel (and hence j in
+  // one channel) are contiguous.  cc_map, cc_ab and the qd_cc host transfers
+  // keep the reference's order; the device tables use this one.
+  std::vector<int> cc_dev_row;              // per reference row (cc_ab_start + row): device row
+  std::vector<uint32_t> cc_ab_dev;          // pairs in device row order
+  // cc_rs[K * nch + c]: first device row of block K whose first state is in
+  // channel c (-1 if none).  The device row of (i, j) is then
+  // cc_rs[K * nch + ch(i)] + pos(i) * |ch(j)| + pos(j), pos = rank in the channel.
+  std::vector<int> cc_rs;
+  std::vector<int> sp_pos;                  // per state: rank within its channel
   // one-index substitutions of every pair (for the 1b x 2b commutator):
   // entry = P' (combined hh|ph|pp index in the block, 16 bits) | t << 16
   // (11 bits) | pos << 27 (which index was replaced) | neg << 28 (sort sign)
   std::vector<int> sub_start;               // per global pair index, size npairs + 1
   std::vector<uint32_t> sub_list;
   std::vector<int> rowperm;                 // per global pair index: row order of the Pandya gather
+  // per position of that row order: combined index | pos(i) << 16 | pos(j) << 24
+  std::vector<uint32_t> pinfo;
 };
 
 // Builds the tables; returns an empty string on success, else the reason.
@@ -86,6 +99,8 @@ struct DevTab {
   const int *sub_start;
   const uint32_t *sub_list;
   const int *rowperm;
+  const uint32_t *pinfo;
+  const int *cc_rs;
 };
 
 }  // namespace qd

@@ -66,8 +66,15 @@ void launch_twobody(hipStream_t s, const DevTab &t, const Ops &o, double *out, c
                     const double2 *sbra, const double2 *sket);
 // Pandya part of commutator_222 from the combined product P (D = XCC - YCC),
 // added to out (or written, accumulate = false)
-void launch_pandya_gather(hipStream_t s, const DevTab &t, double *out, const Task2 *tasks, int ntasks,
-                          const double *P, bool accumulate);
+// Work of the Pandya gather: one ket group gc (of pair class tau) against
+// row groups g0 .. g0+ng-1 of the group list; a group = a run of positions of
+// the block's rowperm order with one (pair class, channel of i, channel of
+// j), stored as (first position, length, channel of i, channel of j).
+struct PTask { int block, tau, gc, g0, ng; };
+constexpr int PG_MAX = 64;        // row groups per task
+constexpr int PG_TARGET = 1024;   // output elements per task (bundling target)
+void launch_pandya_gather(hipStream_t s, const DevTab &t, double *out, const PTask *tasks, int ntasks,
+                          const int4 *groups, const double *P, bool accumulate);
 void launch_onebody(hipStream_t s, const DevTab &t, const Ops &o, double *out, int flags,
                     const double *w1, const double *w2, double *u1, double *u2);
 // zero-body part (commutator_110 + commutator_220) into *dst (device scalar)

@@ -91,6 +91,15 @@ def test_calc_cc(case):
     assert np.array_equal(C.download(), case.o.calc_cc(x, -1)[:case.ch.cc_size])
 
 
+def test_cc_transfer_round_trip(case):
+    # the device keeps CC rows in its own order; upload and download permute
+    # them, so a round trip returns the reference layout bit for bit
+    v = np.random.default_rng(11).standard_normal(case.ch.cc_size)
+    C = case.ctx.cc(1.0)
+    C.upload(v)
+    assert np.array_equal(C.download(), v)
+
+
 @pytest.mark.parametrize("pandya", [True, False])
 @pytest.mark.parametrize("h1,h2", [(-1, 1), (1, 1)])
 def test_full_commutator(case, h1, h2, pandya):
