@@ -1,0 +1,251 @@
+// Two-body part of the IM-SRG(2) commutator over class-pair boxes (gfx950,
+// FP64): commutator_122, the ladder combinations of matrix_mult222 and the
+// Pandya term of commutator_222, one write per stored output.
+//
+// Every substitution of the 1b x 2b commutator keeps the replaced state in
+// its single-particle (m_l, m_s) channel, so the stored outputs
+// <x1 x2|V|x3 x4> with x1, x2 in channels (A1, A2) and x3, x4 in (A3, A4)
+// only need V on the ordered grid A1 x A2 x A3 x A4.  On that grid
+// [f, V](x1..x4) = sum_t f(x1,t) V(t,x2,x3,x4) + f(x2,t) V(x1,t,x3,x4)
+//                  - herm(f) (f(x3,t) V(x1,x2,t,x4) + f(x4,t) V(x1,x2,x3,t))
+// (src/commutators.f90:355-676, which sums the same terms split by hole and
+// particle substitutes) is four dense contractions with |c| x |c| channel
+// blocks of f.  A work-group stages the grids of both operands and the
+// channel blocks in LDS, contracts, and writes the stored elements of the
+// grid with the sign of their canonical order.
+#include "qd_kernels.h"
+
+namespace qd {
+
+__global__ void __launch_bounds__(256) k_fclass(DevTab t, Ops o, double *__restrict__ F) {
+  const int e = blockIdx.x * 256 + threadIdx.x;
+  if (e >= t.fcls_size) return;
+  int lo = 0, hi = t.nch - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (t.fcls_off[mid] <= e) lo = mid;
+    else hi = mid - 1;
+  }
+  const int c0 = t.ch_start[lo], sz = t.ch_start[lo + 1] - c0, le = e - t.fcls_off[lo];
+  const int x = t.ch_states[c0 + le / sz], y = t.ch_states[c0 + le % sz];
+  F[e] = felem(t, o.v1, o.h1, x, y);
+  F[t.fcls_size + e] = felem(t, o.v2, o.h2, x, y);
+}
+
+void launch_fclass(hipStream_t s, const DevTab &t, const Ops &o, double *F) {
+  if (t.fcls_size > 0) hipLaunchKernelGGL(k_fclass, dim3((t.fcls_size + 255) / 256), dim3(256), 0, s, t, o, F);
+}
+
+__device__ __forceinline__ int box_of(const int *pre, int ng, int e) {
+  int lo = 0, hi = ng - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (pre[mid] <= e) lo = mid;
+    else hi = mid - 1;
+  }
+  return lo;
+}
+
+__global__ void __launch_bounds__(256) k_box2b(DevTab t, Ops o, double *__restrict__ out,
+                                               const BTask *__restrict__ tasks, const int *__restrict__ blist,
+                                               int flags, const double *__restrict__ w1,
+                                               const double *__restrict__ w2, const double *__restrict__ w3,
+                                               const double *__restrict__ F, const double *__restrict__ D) {
+  __shared__ double g1[BX_VCAP], g2[BX_VCAP];  // ordered grids of V1, V2: [row][ldc]
+  __shared__ double fl[BX_FCAP];               // channel blocks: f1 then f2, A3, A4, then A1, A2 per box
+  __shared__ int pre[BX_GMAX + 1];             // element offsets of the boxes
+  __shared__ int4 bx[BX_GMAX];                 // class pair of the rows, rows in the task, grid offset, F offset
+  __shared__ int64_t pb[BX_GMAX][4];           // Pandya: D offset of each term's sub-block (-1: no D block)
+  __shared__ int pl[BX_GMAX][4];               // Pandya: leading dimension of each term's D block
+  const BTask tk = tasks[blockIdx.x];
+  if (tk.block < 0) return;
+  const int q = tk.block, mode = tk.mode, ng = tk.ng;
+  const int nh = t.nh[q], nb = t.nb[q], np = t.np[q];
+  const int tstart[3] = {0, nh, nh + nb}, tcount[3] = {nh, nb, np};
+  const int A3 = t.cp_a[tk.gcol], A4 = t.cp_b[tk.gcol];
+  const int s3 = t.ch_start[A3 + 1] - t.ch_start[A3], s4 = t.ch_start[A4 + 1] - t.ch_start[A4];
+  const int nc = mode == 1 ? tk.cnt : s3 * s4, ldc = nc | 1;
+  const int c_off = mode == 1 ? tk.lo : 0, r_off = mode == 2 ? tk.lo : 0;
+  const uint32_t *cgrid = t.cp_grid + t.cp_grid_off[tk.gcol];
+  const bool do122 = (flags & TB_122) != 0, first = mode != 2;
+  const bool dopd = first && (flags & TB_PANDYA) && D;
+  if (threadIdx.x == 0) {
+    int acc = 0, goff = 0, foff = 2 * (s3 * s3 + s4 * s4);
+    for (int b = 0; b < ng; b++) {
+      const int g = blist[tk.first + b];
+      const int A1 = t.cp_a[g], A2 = t.cp_b[g];
+      const int s1 = t.ch_start[A1 + 1] - t.ch_start[A1], s2 = t.ch_start[A2 + 1] - t.ch_start[A2];
+      const int nr = mode == 2 ? tk.cnt : s1 * s2;
+      bx[b] = make_int4(g, nr, goff, foff);
+      pre[b] = acc;
+      acc += nr * nc;
+      goff += nr * ldc;
+      foff += 2 * (s1 * s1 + s2 * s2);
+    }
+    pre[ng] = acc;
+  }
+  if (dopd && (int)threadIdx.x < ng) {
+    // D(x3,x1|x2,x4), D(x3,x2|x1,x4), D(x4,x2|x1,x3), D(x4,x1|x2,x3): device
+    // rows of a CC block are ordered by the channel of the first state, so
+    // each term's box is a dense sub-block at a fixed offset
+    const int g = blist[tk.first + threadIdx.x];
+    const int A1 = t.cp_a[g], A2 = t.cp_b[g];
+    const int rc0[4] = {A3, A3, A4, A4}, rc1[4] = {A1, A2, A2, A1}, cc0[4] = {A2, A1, A1, A2};
+    for (int k = 0; k < 4; k++) {
+      const int x = t.ch_states[t.ch_start[rc0[k]]], y = t.ch_states[t.ch_start[rc1[k]]];
+      const int K = t.cc_map[x * t.M + y].x;
+      pl[threadIdx.x][k] = t.cc_r[K];
+      pb[threadIdx.x][k] = t.cc_nb[K] == 0 ? -1
+                                           : t.cc_poff[K] + t.cc_rs[K * t.nch + rc0[k]] +
+                                                 (int64_t)t.cc_rs[K * t.nch + cc0[k]] * t.cc_r[K];
+    }
+  }
+  __syncthreads();
+  const int total = pre[ng];
+  if (do122) {
+    // channel blocks of f1 and f2
+    const int fs = t.fcls_size;
+    for (int k = threadIdx.x; k < s3 * s3; k += 256) {
+      fl[k] = F[t.fcls_off[A3] + k];
+      fl[s3 * s3 + k] = F[fs + t.fcls_off[A3] + k];
+    }
+    for (int k = threadIdx.x; k < s4 * s4; k += 256) {
+      fl[2 * s3 * s3 + k] = F[t.fcls_off[A4] + k];
+      fl[2 * s3 * s3 + s4 * s4 + k] = F[fs + t.fcls_off[A4] + k];
+    }
+    for (int b = 0; b < ng; b++) {
+      const int4 B = bx[b];
+      const int A1 = t.cp_a[B.x], A2 = t.cp_b[B.x];
+      const int s1 = t.ch_start[A1 + 1] - t.ch_start[A1], s2 = t.ch_start[A2 + 1] - t.ch_start[A2];
+      for (int k = threadIdx.x; k < s1 * s1; k += 256) {
+        fl[B.w + k] = F[t.fcls_off[A1] + k];
+        fl[B.w + s1 * s1 + k] = F[fs + t.fcls_off[A1] + k];
+      }
+      for (int k = threadIdx.x; k < s2 * s2; k += 256) {
+        fl[B.w + 2 * s1 * s1 + k] = F[t.fcls_off[A2] + k];
+        fl[B.w + 2 * s1 * s1 + s2 * s2 + k] = F[fs + t.fcls_off[A2] + k];
+      }
+    }
+    // ordered grids: V(x1,x2,x3,x4) = sign * (herm if stored transposed) *
+    // stored element of the canonical pairs, 0 for x1 == x2 or x3 == x4
+    for (int e = threadIdx.x; e < total; e += 256) {
+      const int b = box_of(pre, ng, e);
+      const int4 B = bx[b];
+      const int le = e - pre[b], r = le % B.y, c = le / B.y;
+      const uint32_t er = t.cp_grid[t.cp_grid_off[B.x] + r_off + r], ec = cgrid[c_off + c];
+      double v1 = 0.0, v2 = 0.0;
+      if ((er >> 17) & (ec >> 17) & 1u) {
+        bool tr;
+        const int64_t a = vidx(t, q, nh, nb, np, (int)(er & 0xffffu), (int)(ec & 0xffffu), &tr);
+        const bool neg = ((er ^ ec) >> 16) & 1u;
+        v1 = tr ? o.h1 * o.v1[a] : o.v1[a];
+        v2 = tr ? o.h2 * o.v2[a] : o.v2[a];
+        if (neg) { v1 = -v1; v2 = -v2; }
+      }
+      g1[B.z + r * ldc + c] = v1;
+      g2[B.z + r * ldc + c] = v2;
+    }
+    __syncthreads();
+  }
+  const int64_t *off = t.off + 6 * q;
+  const double h1 = o.h1, h2 = o.h2;
+  for (int e = threadIdx.x; e < total; e += 256) {
+    const int b = box_of(pre, ng, e);
+    const int4 B = bx[b];
+    const int le = e - pre[b], r = le % B.y, c = le / B.y;
+    const int rr = r_off + r, cc = c_off + c;
+    const uint32_t er = t.cp_grid[t.cp_grid_off[B.x] + rr], ec = cgrid[cc];
+    if (!((er >> 17) & (ec >> 17) & 1u)) continue;
+    const int tyr = (int)((er >> 18) & 3u), tyc = (int)((ec >> 18) & 3u);
+    const bool sr = (er >> 16) & 1u, sc = (ec >> 16) & 1u;
+    const int A1 = t.cp_a[B.x], A2 = t.cp_b[B.x];
+    // stored: row type >= column type; one orientation of a same-channel pair
+    if (tyr < tyc || (A1 == A2 && sr) || (A3 == A4 && sc)) continue;
+    const int s1 = t.ch_start[A1 + 1] - t.ch_start[A1], s2 = t.ch_start[A2 + 1] - t.ch_start[A2];
+    const int i1 = rr / s2, i2 = rr % s2, i3 = cc / s4, i4 = cc % s4;
+    double val = 0.0;
+    if (do122) {
+      const double *G1 = g1 + B.z, *G2 = g2 + B.z;
+      if (first) {
+        const double *fa = fl + B.w, *fb = fl + B.w + 2 * s1 * s1;
+        double acc = 0.0;
+        for (int k = 0; k < s1; k++) {
+          const int gi = (k * s2 + i2) * ldc + c;
+          acc += fa[i1 * s1 + k] * G2[gi] - fa[s1 * s1 + i1 * s1 + k] * G1[gi];
+        }
+        for (int k = 0; k < s2; k++) {
+          const int gi = (i1 * s2 + k) * ldc + c;
+          acc += fb[i2 * s2 + k] * G2[gi] - fb[s2 * s2 + i2 * s2 + k] * G1[gi];
+        }
+        val += acc;
+      }
+      if (mode != 1) {
+        const double *fc = fl, *fd = fl + 2 * s3 * s3;
+        double k2 = 0.0, k1 = 0.0;
+        for (int k = 0; k < s3; k++) {
+          const int gi = r * ldc + k * s4 + i4;
+          k2 += fc[i3 * s3 + k] * G2[gi];
+          k1 += fc[s3 * s3 + i3 * s3 + k] * G1[gi];
+        }
+        for (int k = 0; k < s4; k++) {
+          const int gi = r * ldc + i3 * s4 + k;
+          k2 += fd[i4 * s4 + k] * G2[gi];
+          k1 += fd[s4 * s4 + i4 * s4 + k] * G1[gi];
+        }
+        val += h2 * k1 - h1 * k2;
+      }
+    }
+    if (dopd) {
+      const int64_t b0 = pb[b][0], b1 = pb[b][1], b2 = pb[b][2], b3 = pb[b][3];
+      const double d0 = b0 < 0 ? 0.0 : D[b0 + i3 * s1 + i1 + (int64_t)(i2 * s4 + i4) * pl[b][0]];
+      const double d1 = b1 < 0 ? 0.0 : D[b1 + i3 * s2 + i2 + (int64_t)(i1 * s4 + i4) * pl[b][1]];
+      const double d2 = b2 < 0 ? 0.0 : D[b2 + i4 * s2 + i2 + (int64_t)(i1 * s3 + i3) * pl[b][2]];
+      const double d3 = b3 < 0 ? 0.0 : D[b3 + i4 * s1 + i1 + (int64_t)(i2 * s3 + i3) * pl[b][3]];
+      val += d0 - d1 + d2 - d3;
+    }
+    if (sr != sc) val = -val;
+    // stored position (sub-block of the flat layout)
+    const int Il = (int)(er & 0xffffu) - tstart[tyr], Jl = (int)(ec & 0xffffu) - tstart[tyc];
+    const int rows = tcount[tyr];
+    int sub;
+    if (tyc == 0) sub = tyr == 0 ? HHHH : (tyr == 1 ? PHHH : PPHH);
+    else if (tyc == 1) sub = tyr == 1 ? PHPH : PPPH;
+    else sub = PPPP;
+    const int64_t idx = off[sub] + Il + (int64_t)Jl * rows;
+    if (!first) {
+      out[idx] += val;
+      continue;
+    }
+    double acc = (flags & TB_ACCUM) ? out[idx] : 0.0;
+    acc += val;
+    if (flags & TB_LADDER) {
+      // transposed / signed combinations of the products of matrix_mult222
+      // (src/commutators.f90:999-1127) that the GEMMs cannot accumulate
+      const int64_t tr = off[sub] + Jl + (int64_t)Il * rows;  // (J,I) of a square sub-block
+      switch (sub) {
+        case HHHH:
+        case PPPP:
+          acc += h1 * w1[tr] - h2 * w1[idx];
+          acc += h1 * h2 * w3[tr] - w3[idx];
+          break;
+        case PHHH: acc += h1 * w2[idx] - h2 * w1[idx]; break;
+        case PPPH: acc += -h2 * w1[idx] + h1 * w2[idx]; break;
+        case PHPH:
+          acc += h1 * w2[tr] - h2 * w2[idx];
+          acc += h1 * w1[tr] - h2 * w1[idx];
+          break;
+        default: break;
+      }
+    }
+    out[idx] = acc;
+  }
+}
+
+void launch_box2b(hipStream_t s, const DevTab &t, const Ops &o, double *out, const BTask *tasks, int ntasks,
+                  const int *blist, int flags, const double *w1, const double *w2, const double *w3,
+                  const double *F, const double *D) {
+  if (ntasks <= 0) return;
+  hipLaunchKernelGGL(k_box2b, dim3(ntasks), dim3(256), 0, s, t, o, out, tasks, blist, flags, w1, w2, w3, F, D);
+}
+
+}  // namespace qd

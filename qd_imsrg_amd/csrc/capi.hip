@@ -89,6 +89,10 @@ struct qd_context {
   double *host_scal = nullptr;                  // pinned mirror of scal
   Task2 *tasks2 = nullptr; int ntasks2 = 0;
   PTask *ptasks = nullptr; int nptasks = 0; int4 *pgroups = nullptr;  // Pandya gather work
+  BTask *bt1 = nullptr, *bt2 = nullptr; int nbt1 = 0, nbt2 = 0;      // class-pair box passes
+  int *blist = nullptr;                                              // class pairs of the box tasks
+  int box_cap = BX_VCAP;                                             // grid elements per box task
+  double *fcls = nullptr;                                            // channel blocks of f (2 operands)
   TilePair *ptp = nullptr; int nptp = 0;  // 32x32 tile pairs of the Pandya combine
   TilePair *ftp = nullptr; int nftp = 0;  // 64x64 tile pairs of the fused Pandya product + combine
   Seg2 *pphh_segs = nullptr; Tile2 *pphh_tiles = nullptr; int npphh_segs = 0, npphh_tiles = 0;
@@ -349,6 +353,58 @@ int build_worklists(qd_context *c) {
       pseg.push_back(sg);
     }
   }
+  // ---- class-pair box tasks: per block, every column class pair against
+  // bundles of row class pairs (a box only if it holds a stored element:
+  // row type >= column type somewhere); a box too large for the LDS grids is
+  // split into column slabs (bra terms, first launch) and row slabs (ket
+  // terms, second launch).  Blocks go to XCDs as for the two-body tasks.
+  std::vector<std::vector<BTask>> bt1(ch.nblock), bt2(ch.nblock);
+  std::vector<int> blist;
+  {
+    const int cap = c->box_cap;
+    auto csize = [&](int h) { return ch.ch_start[h + 1] - ch.ch_start[h]; };
+    for (int q = 0; q < ch.nblock; q++)
+      for (int gc = ch.cp_start[q]; gc < ch.cp_start[q + 1]; gc++) {
+        const int s3 = csize(ch.cp_a[gc]), s4 = csize(ch.cp_b[gc]);
+        const int nc = s3 * s4, ldc = nc | 1, fcol = 2 * (s3 * s3 + s4 * s4);
+        std::vector<int> cur;
+        int goff = 0, foff = fcol;
+        auto flush = [&]() {
+          if (!cur.empty()) {
+            bt1[q].push_back(BTask{q, gc, (int)blist.size(), (int)cur.size(), 0, 0, 0});
+            blist.insert(blist.end(), cur.begin(), cur.end());
+          }
+          cur.clear(); goff = 0; foff = fcol;
+        };
+        for (int gr = ch.cp_start[q]; gr < ch.cp_start[q + 1]; gr++) {
+          if (ch.cp_tmax[gr] < ch.cp_tmin[gc]) continue;
+          const int s1 = csize(ch.cp_a[gr]), s2 = csize(ch.cp_b[gr]);
+          const int nr = s1 * s2, fneed = 2 * (s1 * s1 + s2 * s2);
+          if (fcol + fneed > BX_FCAP || 3 * nr > cap || ldc > cap)
+            return fail(QD_E_LAYOUT, "single-particle channel of %d states too large for the two-body boxes",
+                        std::max(std::max(s1, s2), std::max(s3, s4)));
+          if (nr * ldc > cap) {
+            int w = cap / nr;                   // column slab: nr * (w | 1) <= cap
+            if ((w | 1) > cap / nr) w--;
+            for (int lo = 0; lo < nc; lo += w) {
+              bt1[q].push_back(BTask{q, gc, (int)blist.size(), 1, 1, lo, std::min(w, nc - lo)});
+              blist.push_back(gr);
+            }
+            const int h = cap / ldc;            // row slab: h * ldc <= cap
+            for (int lo = 0; lo < nr; lo += h) {
+              bt2[q].push_back(BTask{q, gc, (int)blist.size(), 1, 2, lo, std::min(h, nr - lo)});
+              blist.push_back(gr);
+            }
+            continue;
+          }
+          if (goff + nr * ldc > cap || foff + fneed > BX_FCAP || (int)cur.size() == BX_GMAX) flush();
+          cur.push_back(gr);
+          goff += nr * ldc;
+          foff += fneed;
+        }
+        flush();
+      }
+  }
   {
     std::vector<int> order(ch.nblock);
     for (int q = 0; q < ch.nblock; q++) order[q] = q;
@@ -382,6 +438,27 @@ int build_worklists(qd_context *c) {
     std::vector<PTask> ptasks(longest * NXCD, PTask{-1, 0, 0, 0, 0});
     for (int x = 0; x < NXCD; x++)
       for (size_t k = 0; k < pxcd[x].size(); k++) ptasks[k * NXCD + x] = pxcd[x][k];
+    for (int pass = 0; pass < 2; pass++) {
+      auto &src = pass == 0 ? bt1 : bt2;
+      std::vector<std::vector<BTask>> bx(NXCD);
+      std::fill(load.begin(), load.end(), 0.0);
+      for (int q : order) {
+        if (per_block[q].empty()) continue;
+        int best = (int)(std::min_element(load.begin(), load.end()) - load.begin());
+        load[best] += work[q];
+        bx[best].insert(bx[best].end(), src[q].begin(), src[q].end());
+      }
+      size_t lng = 0;
+      for (auto &l : bx) lng = std::max(lng, l.size());
+      std::vector<BTask> bt(lng * NXCD, BTask{-1, 0, 0, 0, 0, 0, 0});
+      for (int x = 0; x < NXCD; x++)
+        for (size_t k = 0; k < bx[x].size(); k++) bt[k * NXCD + x] = bx[x][k];
+      if (bt.empty()) continue;
+      if ((rc = dev_upload(c, bt, pass == 0 ? &c->bt1 : &c->bt2))) return rc;
+      (pass == 0 ? c->nbt1 : c->nbt2) = (int)bt.size();
+    }
+    if (blist.empty()) blist.push_back(0);
+    if ((rc = dev_upload(c, blist, &c->blist))) return rc;
     if (groups.empty()) groups.push_back(make_int4(0, 0, 0, 0));
     if ((rc = dev_upload(c, ptasks, &c->ptasks))) return rc;
     if ((rc = dev_upload(c, groups, &c->pgroups))) return rc;
@@ -563,22 +640,16 @@ int run_acc(qd_context *c, const qd_op *r1, const qd_op *r2, qd_op *out) {
 int run_twobody(qd_context *c, const qd_op *r1, const qd_op *r2, qd_op *out, int flags, const qd_cc *hcc,
                 const qd_cc *ecc) {
   Ops o{r1->d, r2->d, r1->herm, r2->herm};
-  const int other = flags & (TB_122 | TB_LADDER);
-  if (other) {
-    Timer tm(c, TC_TWOBODY);
-    if (flags & TB_122) launch_subcoef(c->stream, c->dt, o, c->sbra, c->sket, c->npairs);
-    launch_twobody(c->stream, c->dt, o, out->d, c->tasks2, c->ntasks2, flags & ~TB_PANDYA, c->w1, c->w2, c->w3,
-                   c->sbra, c->sket);
-    HIPC(hipGetLastError());
-  }
-  // the Pandya gather is its own pass, in the row order that keeps its reads
-  // of the combined product compact; it adds to what the pass above wrote
-  if (flags & TB_PANDYA) {
-    Timer tm(c, TC_PANDYA);
-    launch_pandya_gather(c->stream, c->dt, out->d, c->ptasks, c->nptasks, c->pgroups, c->P,
-                         other || (flags & TB_ACCUM));
-    HIPC(hipGetLastError());
-  }
+  Timer tm(c, TC_TWOBODY);
+  if (flags & TB_122) launch_fclass(c->stream, c->dt, o, c->fcls);
+  // one pass writes every stored output (1b x 2b, ladder and Pandya terms);
+  // a second adds the ket-side 1b x 2b terms of the boxes split into slabs
+  launch_box2b(c->stream, c->dt, o, out->d, c->bt1, c->nbt1, c->blist, flags, c->w1, c->w2, c->w3, c->fcls,
+               (flags & TB_PANDYA) ? c->P : nullptr);
+  if (flags & TB_122)
+    launch_box2b(c->stream, c->dt, o, out->d, c->bt2, c->nbt2, c->blist, flags, c->w1, c->w2, c->w3, c->fcls,
+                 nullptr);
+  HIPC(hipGetLastError());
   return QD_OK;
 }
 
@@ -733,6 +804,9 @@ int qd_context_create(int nbody, int nshell, qd_context **out) {
   UP(ch.cc_poff, cc_poff); UP(ch.cc_ab_dev, cc_ab); UP(ch.cc_hp, cc_hp); UP(ch.cc_ab_start, cc_ab_start);
   UP(ch.cc_hp_start, cc_hp_start); UP(ccmap, cc_map); UP(ch.sub_start, sub_start); UP(ch.sub_list, sub_list);
   UP(ch.rowperm, rowperm); UP(ch.pinfo, pinfo); UP(ch.cc_rs, cc_rs);
+  UP(ch.cp_a, cp_a); UP(ch.cp_b, cp_b); UP(ch.cp_grid_off, cp_grid_off); UP(ch.cp_grid, cp_grid);
+  UP(ch.fcls_off, fcls_off);
+  t.fcls_size = ch.fcls_size;
 #undef UP
   void *p;
   if ((rc = dev_alloc(c, sizeof(double) * ch.neq, &p))) return bail(rc); c->w1 = (double *)p;
@@ -744,6 +818,13 @@ int qd_context_create(int nbody, int nshell, qd_context **out) {
   if ((rc = dev_alloc(c, sizeof(double2) * std::max<size_t>(ch.sub_list.size(), 1), &p))) return bail(rc);
   c->sket = (double2 *)p;
   c->npairs = (int)ch.pairs.size();
+  if ((rc = dev_alloc(c, sizeof(double) * 2 * std::max(ch.fcls_size, 1), &p))) return bail(rc);
+  c->fcls = (double *)p;
+  if (const char *env = getenv("QD_BOX_CAP")) {
+    // grid elements per box task (tests use a small cap to exercise the slabs)
+    const int v = atoi(env);
+    if (v >= 16 && v <= BX_VCAP) c->box_cap = v;
+  }
   if ((rc = dev_alloc(c, sizeof(double) * ch.M * ch.M, &p))) return bail(rc); c->u1 = (double *)p;
   if ((rc = dev_alloc(c, sizeof(double) * ch.M * ch.M, &p))) return bail(rc); c->u2 = (double *)p;
   if ((rc = dev_alloc(c, sizeof(double) * 1024 * 64, &p))) return bail(rc); c->partials = (double *)p;

@@ -222,6 +222,53 @@ std::string build_channels(int nbody, int nshell, Channels &c) {
       for (size_t k = 0; k < idx.size(); k++) c.rowperm[base + bounds[cl] + k] = idx[k];
     }
   }
+  // ---- class pairs and their ordered grids
+  c.fcls_off.assign(c.nch, 0);
+  c.fcls_size = 0;
+  for (int h = 0; h < c.nch; h++) {
+    const int sz = c.ch_start[h + 1] - c.ch_start[h];
+    c.fcls_off[h] = c.fcls_size;
+    c.fcls_size += sz * sz;
+  }
+  c.cp_start.assign(c.nblock + 1, 0);
+  c.cp_a.clear(); c.cp_b.clear(); c.cp_grid_off.clear(); c.cp_tmin.clear(); c.cp_tmax.clear(); c.cp_grid.clear();
+  for (int b = 0; b < c.nblock; b++) {
+    c.cp_start[b] = (int)c.cp_a.size();
+    const int base = c.hh_start[b], ntot = c.nh[b] + c.nb[b] + c.np[b];
+    std::vector<std::pair<int, int>> keys;
+    for (int I = 0; I < ntot; I++) {
+      const uint32_t pr = c.pairs[base + I];
+      const int u = c.spch[lo16(pr)], v = c.spch[hi16(pr)];
+      keys.push_back({std::min(u, v), std::max(u, v)});
+    }
+    std::sort(keys.begin(), keys.end());
+    keys.erase(std::unique(keys.begin(), keys.end()), keys.end());
+    for (auto &k : keys) {
+      const int A = k.first, B = k.second;
+      const int sa = c.ch_start[A + 1] - c.ch_start[A], sb = c.ch_start[B + 1] - c.ch_start[B];
+      c.cp_a.push_back(A);
+      c.cp_b.push_back(B);
+      c.cp_grid_off.push_back((int)c.cp_grid.size());
+      int tmin = 3, tmax = -1;
+      for (int i1 = 0; i1 < sa; i1++)
+        for (int i2 = 0; i2 < sb; i2++) {
+          const int x1 = c.ch_states[c.ch_start[A] + i1], x2 = c.ch_states[c.ch_start[B] + i2];
+          if (x1 == x2) { c.cp_grid.push_back(0u); continue; }
+          const int lo = std::min(x1, x2), hi = std::max(x1, x2);
+          const uint32_t ia = c.iarr[(size_t)lo * M - (size_t)lo * (lo + 1) / 2 + (hi - lo - 1)];
+          if (ia_blk(ia) != b) return "internal error: class pair left its block";
+          const int ty = ia_type(ia) - 1;  // 0 hh, 1 ph, 2 pp
+          const int comb = ty == 0 ? ia_idx(ia) : ty == 1 ? c.nh[b] + ia_idx(ia) : c.nh[b] + c.nb[b] + ia_idx(ia);
+          tmin = std::min(tmin, ty);
+          tmax = std::max(tmax, ty);
+          c.cp_grid.push_back((uint32_t)comb | ((uint32_t)(x1 > x2) << 16) | (1u << 17) | ((uint32_t)ty << 18));
+        }
+      c.cp_tmin.push_back(tmin);
+      c.cp_tmax.push_back(tmax);
+    }
+  }
+  c.cp_start[c.nblock] = (int)c.cp_a.size();
+
   c.pinfo.assign(c.pairs.size(), 0u);
   for (int b = 0; b < c.nblock; b++) {
     const int base = c.hh_start[b], ntot = c.nh[b] + c.nb[b] + c.np[b];

@@ -67,6 +67,18 @@ struct Channels {
   std::vector<int> rowperm;                 // per global pair index: row order of the Pandya gather
   // per position of that row order: combined index | pos(i) << 16 | pos(j) << 24
   std::vector<uint32_t> pinfo;
+  // Class pairs of the two-body pass: per block, the unordered pairs {A, B}
+  // (A <= B) of single-particle channels whose state pairs lie in the block.
+  // Each has a grid of |A| x |B| ordered (x1, x2), x1 the pos-th state of A
+  // and x2 of B, row-major; an entry packs the combined index of the stored
+  // pair (min, max) | (x1 > x2) << 16 | (x1 != x2) << 17 | pair type << 18.
+  std::vector<int> cp_start;                // per block: first class pair (nblock + 1)
+  std::vector<int> cp_a, cp_b, cp_grid_off; // per class pair
+  std::vector<int> cp_tmin, cp_tmax;        // smallest / largest pair type (0 hh, 1 ph, 2 pp)
+  std::vector<uint32_t> cp_grid;
+  // one-body matrices blocked by channel: |c| x |c| row-major at fcls_off[c]
+  std::vector<int> fcls_off;
+  int fcls_size = 0;
 };
 
 // Builds the tables; returns an empty string on success, else the reason.
@@ -101,6 +113,10 @@ struct DevTab {
   const int *rowperm;
   const uint32_t *pinfo;
   const int *cc_rs;
+  const int *cp_a, *cp_b, *cp_grid_off;
+  const uint32_t *cp_grid;
+  const int *fcls_off;
+  int fcls_size;
 };
 
 }  // namespace qd

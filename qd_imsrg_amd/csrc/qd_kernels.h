@@ -49,6 +49,35 @@ struct Ops {  // two operators and their hermiticity
   int h1, h2;
 };
 
+// ---- device helpers shared by the kernel sources
+// f_elem (src/ME_general.f90:1780-1813)
+__device__ __forceinline__ double felem(const DevTab &t, const double *__restrict__ v, int herm, int a, int b) {
+  const int n = t.n;
+  if (a < n) return (b < n) ? v[t.off_fhh + a + (int64_t)b * n] : herm * v[t.off_fph + (b - n) + (int64_t)a * t.mp];
+  return (b < n) ? v[t.off_fph + (a - n) + (int64_t)b * t.mp] : v[t.off_fpp + (a - n) + (int64_t)(b - n) * t.mp];
+}
+
+// Offset of block element (P, Q) (combined indices) in the flat layout; *tr is
+// set when the element is stored transposed (the herm factor of v_elem applies).
+__device__ __forceinline__ int64_t vidx(const DevTab &t, int q, int nh, int nb, int np, int P, int Q, bool *tr) {
+  const int64_t *o = t.off + 6 * q;
+  const int tp = P < nh ? 0 : (P < nh + nb ? 1 : 2), tq = Q < nh ? 0 : (Q < nh + nb ? 1 : 2);
+  const int ip = P - (tp == 0 ? 0 : (tp == 1 ? nh : nh + nb));
+  const int iq = Q - (tq == 0 ? 0 : (tq == 1 ? nh : nh + nb));
+  *tr = tp < tq;
+  switch (tp * 3 + tq) {
+    case 0: return o[HHHH] + ip + (int64_t)iq * nh;
+    case 1: return o[PHHH] + iq + (int64_t)ip * nb;
+    case 2: return o[PPHH] + iq + (int64_t)ip * np;
+    case 3: return o[PHHH] + ip + (int64_t)iq * nb;
+    case 4: return o[PHPH] + ip + (int64_t)iq * nb;
+    case 5: return o[PPPH] + iq + (int64_t)ip * np;
+    case 6: return o[PPHH] + ip + (int64_t)iq * np;
+    case 7: return o[PPPH] + ip + (int64_t)iq * np;
+    default: return o[PPPP] + ip + (int64_t)iq * np;
+  }
+}
+
 // flags of the fused two-body kernel
 enum { TB_122 = 1, TB_LADDER = 2, TB_PANDYA = 4, TB_ACCUM = 8 };
 // flags of the one-body kernel
@@ -75,6 +104,26 @@ constexpr int PG_MAX = 64;        // row groups per task
 constexpr int PG_TARGET = 1024;   // output elements per task (bundling target)
 void launch_pandya_gather(hipStream_t s, const DevTab &t, double *out, const PTask *tasks, int ntasks,
                           const int4 *groups, const double *P, bool accumulate);
+
+// ---- class-pair boxes of the two-body pass.  A box = the stored outputs
+// whose bra lies in one class pair (rows) and whose ket in another (columns).
+// mode 0: whole boxes, the class pairs blist[first .. first+ng-1] against
+// column pair gcol; mode 1: one box, columns lo .. lo+cnt-1 of its grid, with
+// the bra-side 1b x 2b terms (plus ladder and Pandya terms); mode 2: one box,
+// rows lo .. lo+cnt-1, adding the ket-side 1b x 2b terms (second launch).
+struct BTask { int block, gcol, first, ng, mode, lo, cnt; };
+constexpr int BX_VCAP = 2048;  // grid elements per operand staged in LDS
+constexpr int BX_FCAP = 1024;  // doubles of one-body channel blocks staged in LDS
+constexpr int BX_GMAX = 32;    // boxes per task
+// one-body matrices of both operands blocked by channel: F[0, fcls_size) of
+// r1, F[fcls_size, 2 fcls_size) of r2
+void launch_fclass(hipStream_t s, const DevTab &t, const Ops &o, double *F);
+// all stored two-body outputs of the tasks: 1b x 2b (TB_122), ladder
+// combinations of the products w1..w3 (TB_LADDER), Pandya term from D
+// (TB_PANDYA); TB_ACCUM adds to out (mode 2 tasks always add)
+void launch_box2b(hipStream_t s, const DevTab &t, const Ops &o, double *out, const BTask *tasks, int ntasks,
+                  const int *blist, int flags, const double *w1, const double *w2, const double *w3,
+                  const double *F, const double *D);
 void launch_onebody(hipStream_t s, const DevTab &t, const Ops &o, double *out, int flags,
                     const double *w1, const double *w2, double *u1, double *u2);
 // zero-body part (commutator_110 + commutator_220) into *dst (device scalar)

@@ -178,3 +178,26 @@ def test_bch_expand(case):
     ref, nref = case.o.bch_expand(g, -1, h)
     assert nt == nref
     assert rel(HS.download(), ref) <= 1e-11
+
+
+@pytest.mark.parametrize("cap", [40, 100])
+def test_box_slabs(monkeypatch, cap):
+    # a small cap on the LDS grid of the two-body boxes splits most boxes of
+    # N=6/R=6 into column slabs (bra terms) and row slabs (ket terms)
+    monkeypatch.setenv("QD_BOX_CAP", str(cap))
+    c = Case(6, 6)
+    try:
+        for h1, h2 in [(-1, 1), (1, 1)]:
+            x, y, X, Y = c.ops(21, h1, h2)
+            Z = c.ctx.operator(-h1 * h2)
+            HCC, ECC = c.ctx.cc(float(h2)), c.ctx.cc(float(h1))
+            qd.calc_cc(Y, HCC); qd.calc_cc(X, ECC)
+            qd.commutator(X, Y, Z, HCC, ECC)
+            assert rel(Z.download(), c.o.commutator(x, h1, y, h2, pandya=True)) <= TOL
+            two = slice(c.ch.off[0, 0], None)
+            Z = c.ctx.operator(1)
+            ref = c.o.zeros()
+            qd.commutator_122(X, Y, Z); c.o.commutator_122(x, h1, y, h2, ref)
+            assert rel(Z.download()[two], ref[two]) <= TOL
+    finally:
+        c.ctx.close()
