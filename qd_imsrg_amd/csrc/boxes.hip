@@ -36,8 +36,9 @@ void launch_fclass(hipStream_t s, const DevTab &t, const Ops &o, double *F) {
   if (t.fcls_size > 0) hipLaunchKernelGGL(k_fclass, dim3((t.fcls_size + 255) / 256), dim3(256), 0, s, t, o, F);
 }
 
-__device__ __forceinline__ int box_of(const int *pre, int ng, int e) {
-  int lo = 0, hi = ng - 1;
+// last box b in [lo, ng) with pre[b] <= e
+__device__ __forceinline__ int box_of(const int *pre, int lo, int ng, int e) {
+  int hi = ng - 1;
   while (lo < hi) {
     const int mid = (lo + hi + 1) >> 1;
     if (pre[mid] <= e) lo = mid;
@@ -45,6 +46,11 @@ __device__ __forceinline__ int box_of(const int *pre, int ng, int e) {
   }
   return lo;
 }
+
+// floor(a / d) for 0 <= a < 2^12 and 1 <= d < 2^12 from rd = 1.0f / d: the
+// product's error (< 2^-11) stays below the distance 0.5 / d of (a + 0.5) / d
+// from the nearest integer
+__device__ __forceinline__ int qdiv(int a, float rd) { return (int)(((float)a + 0.5f) * rd); }
 
 __global__ void __launch_bounds__(256) k_box2b(DevTab t, Ops o, double *__restrict__ out,
                                                const BTask *__restrict__ tasks, const int *__restrict__ blist,
@@ -55,6 +61,8 @@ __global__ void __launch_bounds__(256) k_box2b(DevTab t, Ops o, double *__restri
   __shared__ double fl[BX_FCAP];               // channel blocks: f1 then f2, A3, A4, then A1, A2 per box
   __shared__ int pre[BX_GMAX + 1];             // element offsets of the boxes
   __shared__ int4 bx[BX_GMAX];                 // class pair of the rows, rows in the task, grid offset, F offset
+  __shared__ int4 bs[BX_GMAX];                 // |A1|, |A2|, A1 == A2, unused
+  __shared__ float2 brc[BX_GMAX];              // 1 / rows in the task, 1 / |A2|
   __shared__ int64_t pb[BX_GMAX][4];           // Pandya: D offset of each term's sub-block (-1: no D block)
   __shared__ int pl[BX_GMAX][4];               // Pandya: leading dimension of each term's D block
   const BTask tk = tasks[blockIdx.x];
@@ -77,6 +85,8 @@ __global__ void __launch_bounds__(256) k_box2b(DevTab t, Ops o, double *__restri
       const int s1 = t.ch_start[A1 + 1] - t.ch_start[A1], s2 = t.ch_start[A2 + 1] - t.ch_start[A2];
       const int nr = mode == 2 ? tk.cnt : s1 * s2;
       bx[b] = make_int4(g, nr, goff, foff);
+      bs[b] = make_int4(s1, s2, A1 == A2, 0);
+      brc[b] = make_float2(1.0f / (float)nr, 1.0f / (float)s2);
       pre[b] = acc;
       acc += nr * nc;
       goff += nr * ldc;
@@ -128,10 +138,10 @@ __global__ void __launch_bounds__(256) k_box2b(DevTab t, Ops o, double *__restri
     }
     // ordered grids: V(x1,x2,x3,x4) = sign * (herm if stored transposed) *
     // stored element of the canonical pairs, 0 for x1 == x2 or x3 == x4
-    for (int e = threadIdx.x; e < total; e += 256) {
-      const int b = box_of(pre, ng, e);
+    for (int e = threadIdx.x, b = 0; e < total; e += 256) {
+      if (b + 1 < ng && pre[b + 1] <= e) b = box_of(pre, b + 1, ng, e);
       const int4 B = bx[b];
-      const int le = e - pre[b], r = le % B.y, c = le / B.y;
+      const int le = e - pre[b], c = qdiv(le, brc[b].x), r = le - c * B.y;
       const uint32_t er = t.cp_grid[t.cp_grid_off[B.x] + r_off + r], ec = cgrid[c_off + c];
       double v1 = 0.0, v2 = 0.0;
       if ((er >> 17) & (ec >> 17) & 1u) {
@@ -149,20 +159,22 @@ __global__ void __launch_bounds__(256) k_box2b(DevTab t, Ops o, double *__restri
   }
   const int64_t *off = t.off + 6 * q;
   const double h1 = o.h1, h2 = o.h2;
-  for (int e = threadIdx.x; e < total; e += 256) {
-    const int b = box_of(pre, ng, e);
+  const float rs4 = 1.0f / (float)s4;
+  const bool same34 = A3 == A4;
+  for (int e = threadIdx.x, b = 0; e < total; e += 256) {
+    if (b + 1 < ng && pre[b + 1] <= e) b = box_of(pre, b + 1, ng, e);
     const int4 B = bx[b];
-    const int le = e - pre[b], r = le % B.y, c = le / B.y;
+    const int le = e - pre[b], c = qdiv(le, brc[b].x), r = le - c * B.y;
     const int rr = r_off + r, cc = c_off + c;
     const uint32_t er = t.cp_grid[t.cp_grid_off[B.x] + rr], ec = cgrid[cc];
     if (!((er >> 17) & (ec >> 17) & 1u)) continue;
     const int tyr = (int)((er >> 18) & 3u), tyc = (int)((ec >> 18) & 3u);
     const bool sr = (er >> 16) & 1u, sc = (ec >> 16) & 1u;
-    const int A1 = t.cp_a[B.x], A2 = t.cp_b[B.x];
+    const int4 S = bs[b];
     // stored: row type >= column type; one orientation of a same-channel pair
-    if (tyr < tyc || (A1 == A2 && sr) || (A3 == A4 && sc)) continue;
-    const int s1 = t.ch_start[A1 + 1] - t.ch_start[A1], s2 = t.ch_start[A2 + 1] - t.ch_start[A2];
-    const int i1 = rr / s2, i2 = rr % s2, i3 = cc / s4, i4 = cc % s4;
+    if (tyr < tyc || (S.z && sr) || (same34 && sc)) continue;
+    const int s1 = S.x, s2 = S.y;
+    const int i1 = qdiv(rr, brc[b].y), i2 = rr - i1 * s2, i3 = qdiv(cc, rs4), i4 = cc - i3 * s4;
     double val = 0.0;
     if (do122) {
       const double *G1 = g1 + B.z, *G2 = g2 + B.z;

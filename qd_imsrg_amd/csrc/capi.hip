@@ -653,10 +653,13 @@ int run_twobody(qd_context *c, const qd_op *r1, const qd_op *r2, qd_op *out, int
   return QD_OK;
 }
 
-int run_onebody(qd_context *c, const qd_op *r1, const qd_op *r2, qd_op *out, int flags) {
+// ecc / hcc: calc_cc of r1 / r2 (the Pandya inputs) or null
+int run_onebody(qd_context *c, const qd_op *r1, const qd_op *r2, qd_op *out, int flags,
+                const qd_cc *ecc = nullptr, const qd_cc *hcc = nullptr) {
   Ops o{r1->d, r2->d, r1->herm, r2->herm};
   Timer tm(c, TC_ONEBODY);
-  launch_onebody(c->stream, c->dt, o, out->d, flags, c->w1, c->w2, c->u1, c->u2);
+  launch_onebody(c->stream, c->dt, o, out->d, flags, c->w1, c->w2, c->u1, c->u2, ecc ? ecc->d : nullptr,
+                 hcc ? hcc->d : nullptr);
   HIPC(hipGetLastError());
   return QD_OK;
 }
@@ -675,7 +678,10 @@ int full_commutator(qd_context *c, const qd_op *r1, const qd_op *r2, qd_op *r3, 
   if ((rc = run_products(c, r1, r2, r3, hcc, ecc))) return rc;
   if ((rc = run_twobody(c, r1, r2, r3, TB_122 | TB_LADDER | (pd ? TB_PANDYA : 0), hcc, ecc))) return rc;
   if ((rc = run_acc(c, r1, r2, r3))) return rc;
-  if ((rc = run_onebody(c, r1, r2, r3, OB_111 | OB_121 | OB_221))) return rc;
+  // with the Pandya inputs on hand, the 121 mat-vecs read them (hcc and ecc
+  // are calc_cc of r2 and r1 by the entry point's contract)
+  if ((rc = run_onebody(c, r1, r2, r3, OB_111 | OB_121 | OB_221, pd ? ecc : nullptr, pd ? hcc : nullptr)))
+    return rc;
   {
     Ops o{r1->d, r2->d, r1->herm, r2->herm};
     Timer tm(c, TC_OTHER);

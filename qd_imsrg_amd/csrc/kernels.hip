@@ -483,9 +483,48 @@ __global__ void __launch_bounds__(256) k_onebody(DevTab t, Ops o, double *__rest
   if (lane == 0) out[dst] = acc;
 }
 
+// The same mat-vec from a calc_cc-filled CC set (element <q i|V|p a> at
+// device row (p,q), column (i,a)).  A work-group takes 32 consecutive device
+// rows; its 8 column groups stride the columns (coalesced over the rows, f(a,i)
+// uniform across them) and their partial sums are added in a fixed order.
+__global__ void __launch_bounds__(256) k_cc_gemv(DevTab t, const double *__restrict__ cc,
+                                                 const double *__restrict__ f, double *__restrict__ u) {
+  __shared__ double part[8][33];
+  const int rl = threadIdx.x & 31, cg = threadIdx.x >> 5;
+  const int g = blockIdx.x * 32 + rl;
+  double s = 0.0;
+  if (g < t.M * t.M) {
+    int lo = 0, hi = t.ncc - 1;  // block of device row g: last with cc_ab_start <= g
+    while (lo < hi) {
+      const int mid = (lo + hi + 1) >> 1;
+      if (t.cc_ab_start[mid] <= g) lo = mid;
+      else hi = mid - 1;
+    }
+    const int K = lo, d = g - t.cc_ab_start[K], r = t.cc_r[K], nb = t.cc_nb[K];
+    const double *col = cc + t.cc_off[K] + d;
+    const uint32_t *hp = t.cc_hp + t.cc_hp_start[K];
+    for (int c = cg; c < nb; c += 8) {
+      const int i = lo16(hp[c]), a = hi16(hp[c]);
+      s += col[(int64_t)c * r] * f[t.off_fph + (a - t.n) + (int64_t)i * t.mp];
+    }
+  }
+  part[cg][rl] = s;
+  __syncthreads();
+  if (cg == 0 && g < t.M * t.M) {
+    double v = part[0][rl];
+    for (int k = 1; k < 8; k++) v += part[k][rl];
+    const uint32_t pq = t.cc_ab[g];
+    u[(int64_t)lo16(pq) * t.M + hi16(pq)] = v;
+  }
+}
+
 void launch_onebody(hipStream_t s, const DevTab &t, const Ops &o, double *out, int flags, const double *w1,
-                    const double *w2, double *u1, double *u2) {
-  if (flags & OB_121) {
+                    const double *w2, double *u1, double *u2, const double *cc1, const double *cc2) {
+  if ((flags & OB_121) && cc1 && cc2) {
+    const int m = t.M * t.M;
+    hipLaunchKernelGGL(k_cc_gemv, dim3((m + 31) / 32), dim3(256), 0, s, t, cc2, o.v1, u2);
+    hipLaunchKernelGGL(k_cc_gemv, dim3((m + 31) / 32), dim3(256), 0, s, t, cc1, o.v2, u1);
+  } else if (flags & OB_121) {
     launch_cc_matvec(s, t, o.v2, o.h2, o.v1, u2);
     launch_cc_matvec(s, t, o.v1, o.h1, o.v2, u1);
   }

@@ -112,8 +112,10 @@ void launch_pandya_gather(hipStream_t s, const DevTab &t, double *out, const PTa
 // the bra-side 1b x 2b terms (plus ladder and Pandya terms); mode 2: one box,
 // rows lo .. lo+cnt-1, adding the ket-side 1b x 2b terms (second launch).
 struct BTask { int block, gcol, first, ng, mode, lo, cnt; };
-constexpr int BX_VCAP = 2048;  // grid elements per operand staged in LDS
-constexpr int BX_FCAP = 1024;  // doubles of one-body channel blocks staged in LDS
+// LDS per work-group stays under 32 KB (five per CU); channels of up to 10
+// states (R = 20 shells) fit the channel-block area
+constexpr int BX_VCAP = 1400;  // grid elements per operand staged in LDS
+constexpr int BX_FCAP = 800;   // doubles of one-body channel blocks staged in LDS
 constexpr int BX_GMAX = 32;    // boxes per task
 // one-body matrices of both operands blocked by channel: F[0, fcls_size) of
 // r1, F[fcls_size, 2 fcls_size) of r2
@@ -124,8 +126,11 @@ void launch_fclass(hipStream_t s, const DevTab &t, const Ops &o, double *F);
 void launch_box2b(hipStream_t s, const DevTab &t, const Ops &o, double *out, const BTask *tasks, int ntasks,
                   const int *blist, int flags, const double *w1, const double *w2, const double *w3,
                   const double *F, const double *D);
+// cc1 / cc2: calc_cc of r1 / r2 when the caller holds them (then the CC
+// mat-vecs of commutator_121 read them instead of gathering V), else null
 void launch_onebody(hipStream_t s, const DevTab &t, const Ops &o, double *out, int flags,
-                    const double *w1, const double *w2, double *u1, double *u2);
+                    const double *w1, const double *w2, double *u1, double *u2, const double *cc1,
+                    const double *cc2);
 // zero-body part (commutator_110 + commutator_220) into *dst (device scalar)
 void launch_zerobody(hipStream_t s, const DevTab &t, const Ops &o, const Seg2 *pphh_segs, int npphh,
                      double *part, double *dst, int which /*1: 110, 2: 220, 3: both*/);
