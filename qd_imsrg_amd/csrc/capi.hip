@@ -98,6 +98,9 @@ struct qd_context {
   Seg2 *pphh_segs = nullptr; Tile2 *pphh_tiles = nullptr; int npphh_segs = 0, npphh_tiles = 0;
   Seg2 *cc_segs = nullptr; Tile2 *cc_tiles = nullptr; int ncc_tiles = 0;
   GemmList g_all, g_acc;  // products (ladder + Pandya) and accumulating products
+  GemmList g_1b;          // one-body products of commutator_111 (always FP64)
+  GemmTile *ob_tiles = nullptr; int nob_tiles = 0;  // their 16 x 16 tiles
+  int *ob221 = nullptr; int nob221 = 0;  // one-body outputs whose states share a channel
   int launch_mode = 0;    // 0 grouped persistent launch, 1 one launch per channel problem
   int gemm_prec = GEMM_FP64;  // arithmetic of the 2b-2b GEMMs (GemmPrec)
   bool use_graphs = false;  // capture each BCH term (commutator + norms) in a hipGraph
@@ -289,6 +292,31 @@ int upload_gemm(qd_context *c, GemmBuilder &b, int npre, double flops_pre, GemmL
   if ((rc = dev_upload(c, g.hprobs, &g.probs))) return rc;
   if ((rc = dev_upload(c, g.hsegs, &g.segs))) return rc;
   return upload_tiles(c, g);
+}
+
+// One-body products of commutator_111 into the (otherwise unused) one-body
+// region of w1, w2, w3, with A = r1, B = r2: P0 = A_hh B_hh, Q0 = B_ph^T A_ph,
+// P1 = B_pp A_pp, Q1 = A_ph B_ph^T, and R2 = A_ph B_hh - B_ph A_hh + A_pp B_ph
+// - B_pp A_ph in three parts (w1: the two hole sums, w2: A_pp B_ph, w3: B_pp A_ph)
+void onebody_products(const Channels &ch, GemmBuilder &g) {
+  const int n = ch.n, mp = ch.mp;
+  const int64_t ohh = ch.off_fhh, opp = ch.off_fpp, oph = ch.off_fph;
+  if (n > 0) {
+    g.begin(S_W1, ohh, n, n, n, 0); g.seg(S_R1, ohh, n, 0, S_R2, ohh, n, 0, n, 1); g.end();
+    if (mp > 0) { g.begin(S_W2, ohh, n, n, n, 0); g.seg(S_R2, oph, mp, 1, S_R1, oph, mp, 0, mp, 1); g.end(); }
+  }
+  if (mp > 0) {
+    g.begin(S_W1, opp, mp, mp, mp, 0); g.seg(S_R2, opp, mp, 0, S_R1, opp, mp, 0, mp, 1); g.end();
+    if (n > 0) {
+      g.begin(S_W2, opp, mp, mp, mp, 0); g.seg(S_R1, oph, mp, 0, S_R2, oph, mp, 1, n, 1); g.end();
+      g.begin(S_W1, oph, mp, mp, n, 0);
+      g.seg(S_R1, oph, mp, 0, S_R2, ohh, n, 0, n, 1);
+      g.seg(S_R2, oph, mp, 0, S_R1, ohh, n, 0, n, -1);
+      g.end();
+      g.begin(S_W2, oph, mp, mp, n, 0); g.seg(S_R1, opp, mp, 0, S_R2, oph, mp, 0, mp, 1); g.end();
+      g.begin(S_W3, oph, mp, mp, n, 0); g.seg(S_R2, opp, mp, 0, S_R1, oph, mp, 0, mp, 1); g.end();
+    }
+  }
 }
 
 int build_worklists(qd_context *c) {
@@ -524,6 +552,9 @@ int build_worklists(qd_context *c) {
     g2.seg(S_R2, o[PPPP], np, 0, S_R1, o[PPPH], np, 0, np, -1);
     g2.end();
   }
+  // the one-body products ride along in the products launch (a few small
+  // tiles among the ladder tiles)
+  onebody_products(ch, g1);
   const int npre = (int)g1.probs.size();
   const double f1 = g1.flops;
   // Pandya: P(q) = ETACC(q) HCC(q)^T (r x r, K = nb); appended after G1
@@ -562,6 +593,33 @@ int build_worklists(qd_context *c) {
   }
   if ((rc = upload_gemm(c, g1, npre, f1, c->g_all))) return rc;
   if ((rc = upload_gemm(c, g2, (int)g2.probs.size(), g2.flops, c->g_acc))) return rc;
+  // ---- one-body products of commutator_111 on their own (16 x 16 FMA tiles,
+  // FP64 always), for the calls whose products pass did not compute them
+  {
+    GemmBuilder g3;
+    onebody_products(ch, g3);
+    const int n = ch.n, mp = ch.mp;
+    if ((rc = upload_gemm(c, g3, (int)g3.probs.size(), g3.flops, c->g_1b))) return rc;
+    std::vector<GemmTile> t16;
+    for (int pi = 0; pi < (int)g3.probs.size(); pi++)
+      for (int tm = 0; tm < (g3.probs[pi].M + 15) / 16; tm++)
+        for (int tn = 0; tn < (g3.probs[pi].N + 15) / 16; tn++) t16.push_back(GemmTile{pi, tm, tn});
+    c->nob_tiles = (int)t16.size();
+    if (t16.empty()) t16.push_back(GemmTile{0, 0, 0});
+    if ((rc = dev_upload(c, t16, &c->ob_tiles))) return rc;
+    std::vector<int> l221;
+    const int64_t tot = (int64_t)n * n + (int64_t)mp * mp + (int64_t)mp * n;
+    for (int64_t e = 0; e < tot; e++) {
+      int x, y;
+      if (e < (int64_t)n * n) { x = (int)(e % n); y = (int)(e / n); }
+      else if (e < (int64_t)n * n + (int64_t)mp * mp) { const int64_t f = e - (int64_t)n * n; x = n + (int)(f % mp); y = n + (int)(f / mp); }
+      else { const int64_t f = e - (int64_t)n * n - (int64_t)mp * mp; x = n + (int)(f % mp); y = (int)(f / mp); }
+      if (ch.spch[x] == ch.spch[y]) l221.push_back((int)e);
+    }
+    c->nob221 = (int)l221.size();
+    if (l221.empty()) l221.push_back(0);
+    if ((rc = dev_upload(c, l221, &c->ob221))) return rc;
+  }
   return QD_OK;
 }
 
@@ -590,17 +648,18 @@ bool pandya_on(const qd_cc *hcc, const qd_cc *ecc) {
 
 // One grouped persistent launch over the sorted tile list, or (per-channel
 // mode) one launch per problem, the way the reference calls dgemm per block.
-int run_gemm_list(qd_context *c, const GemmList &g, bool with_pandya, const Slots &s) {
+int run_gemm_list(qd_context *c, const GemmList &g, bool with_pandya, const Slots &s, int prec = -1) {
+  if (prec < 0) prec = c->gemm_prec;
   if (c->launch_mode == 0) {
     const int *r = with_pandya ? g.shape_all : g.shape_pre;
     GemmTile *t = with_pandya ? g.tiles_all : g.tiles_pre;
     for (int sh = 0; sh < NSHAPE; sh++)
-      launch_gemm(c->stream, g.probs, g.segs, t + r[sh], r[sh + 1] - r[sh], s, c->gemm_grid, sh, c->gemm_prec);
+      launch_gemm(c->stream, g.probs, g.segs, t + r[sh], r[sh + 1] - r[sh], s, c->gemm_grid, sh, prec);
   } else {
     const int np = with_pandya ? (int)g.hprobs.size() : g.npre;
     for (int pi = 0; pi < np; pi++) {
       const int n = g.ptile_off[pi + 1] - g.ptile_off[pi];
-      launch_gemm(c->stream, g.probs, g.segs, g.ptiles + g.ptile_off[pi], n, s, 0, g.hprobs[pi].shape, c->gemm_prec);
+      launch_gemm(c->stream, g.probs, g.segs, g.ptiles + g.ptile_off[pi], n, s, 0, g.hprobs[pi].shape, prec);
     }
   }
   HIPC(hipGetLastError());
@@ -653,13 +712,18 @@ int run_twobody(qd_context *c, const qd_op *r1, const qd_op *r2, qd_op *out, int
   return QD_OK;
 }
 
-// ecc / hcc: calc_cc of r1 / r2 (the Pandya inputs) or null
+// ecc / hcc: calc_cc of r1 / r2 (the Pandya inputs) or null; products: the
+// products pass of the same operands has run (it holds the one-body products,
+// in FP64 when the GEMM precision is FP64)
 int run_onebody(qd_context *c, const qd_op *r1, const qd_op *r2, qd_op *out, int flags,
-                const qd_cc *ecc = nullptr, const qd_cc *hcc = nullptr) {
+                const qd_cc *ecc = nullptr, const qd_cc *hcc = nullptr, bool products = false) {
   Ops o{r1->d, r2->d, r1->herm, r2->herm};
   Timer tm(c, TC_ONEBODY);
-  launch_onebody(c->stream, c->dt, o, out->d, flags, c->w1, c->w2, c->u1, c->u2, ecc ? ecc->d : nullptr,
-                 hcc ? hcc->d : nullptr);
+  if ((flags & OB_111) && !(products && c->gemm_prec == GEMM_FP64))
+    launch_gemm_small(c->stream, c->g_1b.probs, c->g_1b.segs, c->ob_tiles, c->nob_tiles,
+                      make_slots(c, r1, r2, out, nullptr, nullptr));
+  launch_onebody(c->stream, c->dt, o, out->d, flags, c->w1, c->w2, c->w3, c->u1, c->u2, ecc ? ecc->d : nullptr,
+                 hcc ? hcc->d : nullptr, c->ob221, c->nob221);
   HIPC(hipGetLastError());
   return QD_OK;
 }
@@ -680,7 +744,7 @@ int full_commutator(qd_context *c, const qd_op *r1, const qd_op *r2, qd_op *r3, 
   if ((rc = run_acc(c, r1, r2, r3))) return rc;
   // with the Pandya inputs on hand, the 121 mat-vecs read them (hcc and ecc
   // are calc_cc of r2 and r1 by the entry point's contract)
-  if ((rc = run_onebody(c, r1, r2, r3, OB_111 | OB_121 | OB_221, pd ? ecc : nullptr, pd ? hcc : nullptr)))
+  if ((rc = run_onebody(c, r1, r2, r3, OB_111 | OB_121 | OB_221, pd ? ecc : nullptr, pd ? hcc : nullptr, true)))
     return rc;
   {
     Ops o{r1->d, r2->d, r1->herm, r2->herm};

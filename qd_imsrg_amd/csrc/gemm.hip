@@ -555,4 +555,58 @@ void launch_pandya_fused(hipStream_t s, const DevTab &t, const double *etacc, co
   hipLaunchKernelGGL(k_pandya_fused, dim3(ntp), dim3(256), 0, s, t, etacc, hcc, D, tp, hh, eh);
 }
 
+// Small products (the one-body blocks of commutator_111, K <= a few hundred):
+// one 16 x 16 output tile per work-group and one output per thread, K staged
+// through LDS in chunks of 192 (one global round trip for K <= 192), plain
+// FP64 FMA in a fixed order.  Many small work-groups keep the latency of the K
+// walk short where the MFMA tiles of k_gemm would leave most of the chip idle.
+constexpr int SMALL_KC = 192;
+__global__ void __launch_bounds__(256) k_gemm_small(const GemmProb *__restrict__ probs,
+                                                    const GemmSeg *__restrict__ segs,
+                                                    const GemmTile *__restrict__ tiles, Slots slots) {
+  __shared__ double As[16][SMALL_KC + 1], Bs[SMALL_KC][17];
+  const GemmTile tl = tiles[blockIdx.x];
+  const GemmProb p = probs[tl.prob];
+  const int ti = threadIdx.x & 15, tj = threadIdx.x >> 4;
+  const int i0 = tl.tm * 16, j0 = tl.tn * 16;
+  double acc = 0.0;
+  for (int s = 0; s < p.nseg; s++) {
+    const GemmSeg sg = segs[p.seg0 + s];
+    const double *__restrict__ A = slots.p[sg.a_slot] + sg.a_off;
+    const double *__restrict__ B = slots.p[sg.b_slot] + sg.b_off;
+    double part = 0.0;
+    for (int k0 = 0; k0 < sg.K; k0 += SMALL_KC) {
+      for (int idx = threadIdx.x; idx < 16 * SMALL_KC; idx += 256) {
+        // the contiguous index of each operand runs fastest across the lanes
+        const int i = sg.ta ? idx / SMALL_KC : idx % 16, k = sg.ta ? idx % SMALL_KC : idx / 16;
+        const int gi = i0 + i, gk = k0 + k;
+        double v = 0.0;
+        if (gi < p.M && gk < sg.K) v = sg.ta ? A[gk + (int64_t)gi * sg.lda] : A[gi + (int64_t)gk * sg.lda];
+        As[i][k] = v;
+        const int j = sg.tb ? idx % 16 : idx / SMALL_KC, kb = sg.tb ? idx / 16 : idx % SMALL_KC;
+        const int gj = j0 + j, gkb = k0 + kb;
+        double w = 0.0;
+        if (gj < p.N && gkb < sg.K) w = sg.tb ? B[gj + (int64_t)gkb * sg.ldb] : B[gkb + (int64_t)gj * sg.ldb];
+        Bs[kb][j] = w;
+      }
+      __syncthreads();
+      const int kn = sg.K - k0 < SMALL_KC ? sg.K - k0 : SMALL_KC;
+      for (int k = 0; k < kn; k++) part += As[ti][k] * Bs[k][tj];
+      __syncthreads();
+    }
+    acc += sg.alpha * part;
+  }
+  const int gi = i0 + ti, gj = j0 + tj;
+  if (gi < p.M && gj < p.N) {
+    double *C = slots.p[p.c_slot] + p.c_off + gi + (int64_t)gj * p.ldc;
+    *C = p.beta1 ? *C + acc : acc;
+  }
+}
+
+void launch_gemm_small(hipStream_t s, const GemmProb *probs, const GemmSeg *segs, const GemmTile *tiles,
+                       int ntiles, const Slots &slots) {
+  if (ntiles <= 0) return;
+  hipLaunchKernelGGL(k_gemm_small, dim3(ntiles), dim3(256), 0, s, probs, segs, tiles, slots);
+}
+
 }  // namespace qd

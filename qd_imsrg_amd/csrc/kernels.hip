@@ -385,15 +385,15 @@ void launch_pandya_gather(hipStream_t s, const DevTab &t, double *out, const PTa
 
 // ---------------------------------------------------------------------------
 // One-body part: commutator_111 (src/commutators.f90:137-195), _121 (:73-134)
-// and the partial traces of _221 (:278-348).  One wave per output element of
-// fhh, fpp, fph (contiguous in the flat layout from offset 1); the lanes split
-// every summation and a fixed xor-shuffle tree reduces them (deterministic).
+// and the partial traces of _221 (:278-348) for the output elements of fhh,
+// fpp, fph (contiguous in the flat layout from offset 1).  All reductions
+// have a fixed order (deterministic).
 
 // commutator_121 through the cross-coupled matrices: with
 // CC_k[(p,q),(i,a)] = <q i|V_k|p a> (calc_cc), v(a,x,i,y) = h * CC[(x,y),(i,a)]
 // and v(i,x,a,y) = CC[(y,x),(i,a)], so the 121 sum over (i,a) of
 // src/commutators.f90:83-132 becomes u_k[(p,q)] = sum_(i,a) CC_k[(p,q),(i,a)]
-// f_other(a,i): one thread per ordered pair (p,q).
+// f_other(a,i): one wave per ordered pair (p,q), gathering V.
 __global__ void __launch_bounds__(256) k_cc_matvec(DevTab t, const double *__restrict__ v, int herm,
                                                    const double *__restrict__ f, double *__restrict__ u) {
   const int64_t e = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);  // one wave per ordered pair
@@ -415,72 +415,6 @@ __global__ void __launch_bounds__(256) k_cc_matvec(DevTab t, const double *__res
 void launch_cc_matvec(hipStream_t s, const DevTab &t, const double *v, int herm, const double *f, double *u) {
   const int64_t m = (int64_t)t.M * t.M;
   hipLaunchKernelGGL(k_cc_matvec, dim3((unsigned)((m + 3) / 4)), dim3(256), 0, s, t, v, herm, f, u);
-}
-
-__global__ void __launch_bounds__(256) k_onebody(DevTab t, Ops o, double *__restrict__ out, int flags,
-                                                 const double *__restrict__ w1, const double *__restrict__ w2,
-                                                 const double *__restrict__ u1, const double *__restrict__ u2) {
-  const int n = t.n, mp = t.mp;
-  const int64_t nhh = (int64_t)n * n, npp = (int64_t)mp * mp, nph = (int64_t)mp * n;
-  const int lane = threadIdx.x & 63;
-  const int64_t e = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
-  if (e >= nhh + npp + nph) return;  // whole wave exits together
-  int kind, i, j, x, y;
-  if (e < nhh) { kind = 0; i = (int)(e % n); j = (int)(e / n); x = i; y = j; }
-  else if (e < nhh + npp) { kind = 1; int64_t f = e - nhh; i = (int)(f % mp); j = (int)(f / mp); x = n + i; y = n + j; }
-  else { kind = 2; int64_t f = e - nhh - npp; i = (int)(f % mp); j = (int)(f / mp); x = n + i; y = j; }
-  const double *A = o.v1, *B = o.v2;
-  const double h1 = o.h1, h2 = o.h2;
-  const int64_t ohh = t.off_fhh, opp = t.off_fpp, oph = t.off_fph;
-#define FHH(v, a, b) (v)[ohh + (a) + (int64_t)(b) * n]
-#define FPP(v, a, b) (v)[opp + (a) + (int64_t)(b) * mp]
-#define FPH(v, a, b) (v)[oph + (a) + (int64_t)(b) * mp]
-  const int64_t dst = 1 + e;
-  double acc = (flags & OB_ACCUM) ? out[dst] : 0.0;
-  if (flags & OB_111) {
-    double q1 = 0, q1t = 0, q2 = 0, q2t = 0, q3 = 0, q4 = 0;
-    if (kind == 0) {
-      for (int k = lane; k < n; k += 64) { q1 += FHH(A, i, k) * FHH(B, k, j); q1t += FHH(A, j, k) * FHH(B, k, i); }
-      for (int k = lane; k < mp; k += 64) { q2 += FPH(B, k, i) * FPH(A, k, j); q2t += FPH(B, k, j) * FPH(A, k, i); }
-    } else if (kind == 1) {
-      for (int k = lane; k < mp; k += 64) { q1 += FPP(B, i, k) * FPP(A, k, j); q1t += FPP(B, j, k) * FPP(A, k, i); }
-      for (int k = lane; k < n; k += 64) { q2 += FPH(A, i, k) * FPH(B, j, k); q2t += FPH(A, j, k) * FPH(B, i, k); }
-    } else {
-      for (int k = lane; k < n; k += 64) { q1 += FPH(A, i, k) * FHH(B, k, j); q2 += FPH(B, i, k) * FHH(A, k, j); }
-      for (int k = lane; k < mp; k += 64) { q3 += FPP(A, i, k) * FPH(B, k, j); q4 += FPP(B, i, k) * FPH(A, k, j); }
-    }
-    q1 = wave_sum(q1); q1t = wave_sum(q1t); q2 = wave_sum(q2); q2t = wave_sum(q2t);
-    q3 = wave_sum(q3); q4 = wave_sum(q4);
-    double val;
-    if (kind == 0) val = q1 - h2 * q2 + h1 * (q2t - h2 * q1t);
-    else if (kind == 1) val = h2 * q2 + h1 * (h2 * q1t - q2t) - q1;
-    else val = q1 - q2 + q3 - q4;
-    acc = (flags & OB_ACCUM) ? acc + val : val;
-  }
-#undef FHH
-#undef FPP
-#undef FPH
-  if (flags & OB_121) {
-    // u2 = CC(r2) f(r1), u1 = CC(r1) f(r2)
-    const int64_t xy = (int64_t)x * t.M + y, yx = (int64_t)y * t.M + x;
-    acc += (double)(o.h1 * o.h2) * u2[xy] - u2[yx] - ((double)(o.h1 * o.h2) * u1[xy] - u1[yx]);
-  }
-  if ((flags & OB_221) && t.spch[x] == t.spch[y]) {
-    // partial traces of the 221 intermediates (w1, w2 have herm = +1)
-    double r = 0.0;
-    for (int a = lane; a < t.M; a += 64) {
-      if (a < n) {
-        if (kind == 2) r += h1 * velem(t, w2, 1, a, x, a, y) - h2 * velem(t, w1, 1, a, x, a, y);
-        else r += h1 * velem(t, w1, 1, a, y, a, x) - h2 * velem(t, w1, 1, a, x, a, y);
-      } else {
-        if (kind == 1) r += -h1 * velem(t, w1, 1, a, y, a, x) + h2 * velem(t, w1, 1, a, x, a, y);
-        else if (kind == 0) r += -h1 * velem(t, w2, 1, a, y, a, x) + h2 * velem(t, w2, 1, a, x, a, y);
-        else r += -h1 * velem(t, w2, 1, a, x, a, y) + h2 * velem(t, w1, 1, a, x, a, y);
-      }
-    }
-    acc += wave_sum(r);
-  }
-  if (lane == 0) out[dst] = acc;
 }
 
 // The same mat-vec from a calc_cc-filled CC set (element <q i|V|p a> at
@@ -518,8 +452,93 @@ __global__ void __launch_bounds__(256) k_cc_gemv(DevTab t, const double *__restr
   }
 }
 
+// Output element e of fhh, fpp, fph (flat offset 1 + e): kind 0 hh, 1 pp, 2 ph;
+// (i, j) within its block, (x, y) as states
+__device__ __forceinline__ void ob_decode(const DevTab &t, int64_t e, int &kind, int &i, int &j, int &x, int &y) {
+  const int n = t.n, mp = t.mp;
+  const int64_t nhh = (int64_t)n * n, npp = (int64_t)mp * mp;
+  if (e < nhh) { kind = 0; i = (int)(e % n); j = (int)(e / n); x = i; y = j; }
+  else if (e < nhh + npp) { kind = 1; int64_t f = e - nhh; i = (int)(f % mp); j = (int)(f / mp); x = n + i; y = n + j; }
+  else { kind = 2; int64_t f = e - nhh - npp; i = (int)(f % mp); j = (int)(f / mp); x = n + i; y = j; }
+}
+
+// commutator_111 (src/commutators.f90:137-195) from the one-body products that
+// the FP64 product list of the one-body pass leaves in the one-body region of w1,
+// w2, w3 (P0 = A_hh B_hh, Q0 = B_ph^T A_ph, P1 = B_pp A_pp, Q1 = A_ph B_ph^T,
+// R2 = A_ph B_hh - B_ph A_hh + A_pp B_ph - B_pp A_ph in three parts, for
+// A = r1, B = r2), and
+// commutator_121 (:73-134) from the CC mat-vecs u1, u2: one thread per output.
+__global__ void __launch_bounds__(256) k_onebody(DevTab t, Ops o, double *__restrict__ out, int flags,
+                                                 const double *__restrict__ w1, const double *__restrict__ w2,
+                                                 const double *__restrict__ w3, const double *__restrict__ u1,
+                                                 const double *__restrict__ u2) {
+  const int n = t.n, mp = t.mp;
+  const int64_t e = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  if (e >= (int64_t)n * n + (int64_t)mp * mp + (int64_t)mp * n) return;
+  int kind, i, j, x, y;
+  ob_decode(t, e, kind, i, j, x, y);
+  const double h1 = o.h1, h2 = o.h2;
+  const int64_t dst = 1 + e;
+  double acc = (flags & OB_ACCUM) ? out[dst] : 0.0;
+  if (flags & OB_111) {
+    double val;
+    if (kind == 0) {
+      const double *P = w1 + t.off_fhh, *Q = w2 + t.off_fhh;
+      const double q1 = P[i + (int64_t)j * n], q1t = P[j + (int64_t)i * n];
+      const double q2 = Q[i + (int64_t)j * n], q2t = Q[j + (int64_t)i * n];
+      val = q1 - h2 * q2 + h1 * (q2t - h2 * q1t);
+    } else if (kind == 1) {
+      const double *P = w1 + t.off_fpp, *Q = w2 + t.off_fpp;
+      const double q1 = P[i + (int64_t)j * mp], q1t = P[j + (int64_t)i * mp];
+      const double q2 = Q[i + (int64_t)j * mp], q2t = Q[j + (int64_t)i * mp];
+      val = h2 * q2 + h1 * (h2 * q1t - q2t) - q1;
+    } else {
+      const int64_t k = t.off_fph + i + (int64_t)j * mp;
+      val = w1[k] + w2[k] - w3[k];
+    }
+    acc = (flags & OB_ACCUM) ? acc + val : val;
+  }
+  if (flags & OB_121) {
+    // u2 = CC(r2) f(r1), u1 = CC(r1) f(r2)
+    const int64_t xy = (int64_t)x * t.M + y, yx = (int64_t)y * t.M + x;
+    acc += (double)(o.h1 * o.h2) * u2[xy] - u2[yx] - ((double)(o.h1 * o.h2) * u1[xy] - u1[yx]);
+  }
+  out[dst] = acc;
+}
+
+// partial traces of commutator_221 (src/commutators.f90:278-348) for the
+// outputs whose two states share a single-particle channel (the others are
+// zero): one wave per listed output, added to out
+__global__ void __launch_bounds__(256) k_onebody221(DevTab t, Ops o, double *__restrict__ out,
+                                                    const int *__restrict__ list, int nlist,
+                                                    const double *__restrict__ w1, const double *__restrict__ w2) {
+  const int lane = threadIdx.x & 63;
+  const int k = blockIdx.x * 4 + (threadIdx.x >> 6);
+  if (k >= nlist) return;  // whole wave exits together
+  const int64_t e = list[k];
+  int kind, i, j, x, y;
+  ob_decode(t, e, kind, i, j, x, y);
+  const int n = t.n;
+  const double h1 = o.h1, h2 = o.h2;
+  // w1, w2 have herm = +1
+  double r = 0.0;
+  for (int a = lane; a < t.M; a += 64) {
+    if (a < n) {
+      if (kind == 2) r += h1 * velem(t, w2, 1, a, x, a, y) - h2 * velem(t, w1, 1, a, x, a, y);
+      else r += h1 * velem(t, w1, 1, a, y, a, x) - h2 * velem(t, w1, 1, a, x, a, y);
+    } else {
+      if (kind == 1) r += -h1 * velem(t, w1, 1, a, y, a, x) + h2 * velem(t, w1, 1, a, x, a, y);
+      else if (kind == 0) r += -h1 * velem(t, w2, 1, a, y, a, x) + h2 * velem(t, w2, 1, a, x, a, y);
+      else r += -h1 * velem(t, w2, 1, a, x, a, y) + h2 * velem(t, w1, 1, a, x, a, y);
+    }
+  }
+  r = wave_sum(r);
+  if (lane == 0) out[1 + e] += r;
+}
+
 void launch_onebody(hipStream_t s, const DevTab &t, const Ops &o, double *out, int flags, const double *w1,
-                    const double *w2, double *u1, double *u2, const double *cc1, const double *cc2) {
+                    const double *w2, const double *w3, double *u1, double *u2, const double *cc1,
+                    const double *cc2, const int *list221, int n221) {
   if ((flags & OB_121) && cc1 && cc2) {
     const int m = t.M * t.M;
     hipLaunchKernelGGL(k_cc_gemv, dim3((m + 31) / 32), dim3(256), 0, s, t, cc2, o.v1, u2);
@@ -529,7 +548,11 @@ void launch_onebody(hipStream_t s, const DevTab &t, const Ops &o, double *out, i
     launch_cc_matvec(s, t, o.v1, o.h1, o.v2, u1);
   }
   const int64_t tot = (int64_t)t.n * t.n + (int64_t)t.mp * t.mp + (int64_t)t.mp * t.n;
-  hipLaunchKernelGGL(k_onebody, dim3((unsigned)((tot + 3) / 4)), dim3(256), 0, s, t, o, out, flags, w1, w2, u1, u2);
+  if ((flags & (OB_111 | OB_121)) || !(flags & OB_ACCUM))
+    hipLaunchKernelGGL(k_onebody, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, s, t, o, out, flags, w1, w2, w3,
+                       u1, u2);
+  if ((flags & OB_221) && n221 > 0)
+    hipLaunchKernelGGL(k_onebody221, dim3((n221 + 3) / 4), dim3(256), 0, s, t, o, out, list221, n221, w1, w2);
 }
 
 // ---------------------------------------------------------------------------

@@ -83,6 +83,9 @@ enum { TB_122 = 1, TB_LADDER = 2, TB_PANDYA = 4, TB_ACCUM = 8 };
 // flags of the one-body kernel
 enum { OB_111 = 1, OB_121 = 2, OB_221 = 4, OB_ACCUM = 8 };
 
+// Small products: 16 x 16 tiles (tm, tn in units of 16), FP64 FMA.
+void launch_gemm_small(hipStream_t s, const GemmProb *probs, const GemmSeg *segs, const GemmTile *tiles,
+                       int ntiles, const Slots &slots);
 // All tiles must have the given shape.  grid <= 0: one work-group per tile;
 // else a persistent grid striding the list.
 void launch_gemm(hipStream_t s, const GemmProb *probs, const GemmSeg *segs, const GemmTile *tiles,
@@ -128,9 +131,12 @@ void launch_box2b(hipStream_t s, const DevTab &t, const Ops &o, double *out, con
                   const double *F, const double *D);
 // cc1 / cc2: calc_cc of r1 / r2 when the caller holds them (then the CC
 // mat-vecs of commutator_121 read them instead of gathering V), else null
+// OB_111 needs the one-body products of the one-body product list in the
+// one-body region of w1, w2, w3; list221: the n221 outputs whose states share
+// a channel (221 traces)
 void launch_onebody(hipStream_t s, const DevTab &t, const Ops &o, double *out, int flags,
-                    const double *w1, const double *w2, double *u1, double *u2, const double *cc1,
-                    const double *cc2);
+                    const double *w1, const double *w2, const double *w3, double *u1, double *u2,
+                    const double *cc1, const double *cc2, const int *list221, int n221);
 // zero-body part (commutator_110 + commutator_220) into *dst (device scalar)
 void launch_zerobody(hipStream_t s, const DevTab &t, const Ops &o, const Seg2 *pphh_segs, int npphh,
                      double *part, double *dst, int which /*1: 110, 2: 220, 3: both*/);
