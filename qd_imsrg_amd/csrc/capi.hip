@@ -83,12 +83,8 @@ struct qd_context {
   std::vector<void *> allocs;
   double *w1 = nullptr, *w2 = nullptr, *w3 = nullptr, *P = nullptr;
   double *u1 = nullptr, *u2 = nullptr;  // cross-coupled mat-vecs of commutator_121 (M*M each)
-  double2 *sbra = nullptr, *sket = nullptr;  // 122 coefficients per substitution-list entry
-  int npairs = 0;
   double *partials = nullptr, *scal = nullptr;  // reduction partials, 8 device scalars
   double *host_scal = nullptr;                  // pinned mirror of scal
-  Task2 *tasks2 = nullptr; int ntasks2 = 0;
-  PTask *ptasks = nullptr; int nptasks = 0; int4 *pgroups = nullptr;  // Pandya gather work
   BTask *bt1 = nullptr, *bt2 = nullptr; int nbt1 = 0, nbt2 = 0;      // class-pair box passes
   int *blist = nullptr;                                              // class pairs of the box tasks
   int box_cap = BX_VCAP;                                             // grid elements per box task
@@ -322,61 +318,17 @@ void onebody_products(const Channels &ch, GemmBuilder &g) {
 int build_worklists(qd_context *c) {
   const Channels &ch = c->ch;
   int rc;
-  // ---- two-body tasks (block, ket type, ket tile), grouped per XCD: all tasks
-  // of one block run on one XCD so its operands stay in that XCD's L2; blocks
-  // are assigned to XCDs largest-first (LPT on the element count)
+  // ---- per-block work (stored two-body elements) for the XCD assignment:
+  // all tasks of one block run on one XCD so its operands stay in that XCD's
+  // L2; blocks are assigned to XCDs largest-first (LPT on the element count)
   std::vector<Seg2> pseg;
   std::vector<Tile2> ptiles;
-  std::vector<std::vector<Task2>> per_block(ch.nblock);
-  std::vector<std::vector<PTask>> pg_block(ch.nblock);
-  std::vector<int4> groups;
   std::vector<double> work(ch.nblock, 0.0);
   for (int q = 0; q < ch.nblock; q++) {
-    const int nh = ch.nh[q], np = ch.np[q], nb = ch.nb[q], ntot = nh + nb + np;
-    if (ntot == 0) continue;
-    if (2 * ntot > TB_LDS_DOUBLES) return fail(QD_E_LAYOUT, "two-body block of %d pairs exceeds the LDS tile", ntot);
-    const int tj = std::min(64, std::max(1, TB_LDS_DOUBLES / (2 * ntot)));
-    const int ncol[3] = {nh, nb, np}, start[3] = {0, nh, nh + nb};
-    for (int tau = 0; tau < 3; tau++) {
-      const int nrows = ntot - start[tau];
-      if (ncol[tau] == 0 || nrows == 0) continue;
-      for (int j0 = 0; j0 < ncol[tau]; j0 += tj) per_block[q].push_back({q, tau, j0, std::min(tj, ncol[tau] - j0)});
-      work[q] += (double)nrows * ncol[tau];
-    }
-    // Pandya gather: groups of equal (pair class, channel of i, channel of j)
-    // along the rowperm order; every ket group is paired with bundles of the
-    // row groups of classes >= its own
-    {
-      const int base = ch.hh_start[q], g_first = (int)groups.size();
-      int gstart[4] = {0, 0, 0, 0};  // first group of each pair class, relative to g_first
-      for (int cl = 0, pos = 0; cl < 3; cl++) {
-        gstart[cl] = (int)groups.size() - g_first;
-        for (; pos < start[cl] + ncol[cl];) {
-          const uint32_t p0 = ch.pairs[base + ch.rowperm[base + pos]];
-          int len = 1;
-          while (pos + len < start[cl] + ncol[cl]) {
-            const uint32_t p1 = ch.pairs[base + ch.rowperm[base + pos + len]];
-            if (ch.spch[lo16(p1)] != ch.spch[lo16(p0)] || ch.spch[hi16(p1)] != ch.spch[hi16(p0)]) break;
-            len++;
-          }
-          groups.push_back(make_int4(pos, len, ch.spch[lo16(p0)], ch.spch[hi16(p0)]));
-          pos += len;
-        }
-      }
-      gstart[3] = (int)groups.size() - g_first;
-      for (int tau = 0; tau < 3; tau++)
-        for (int gc = gstart[tau]; gc < gstart[tau + 1]; gc++) {
-          const int4 col = groups[g_first + gc];
-          for (int g = gstart[tau]; g < gstart[3];) {
-            int ng = 0, elems = 0;
-            while (g + ng < gstart[3] && ng < PG_MAX && elems < PG_TARGET) elems += groups[g_first + g + ng++].y * col.y;
-            pg_block[q].push_back(PTask{q, tau, g_first + gc, g_first + g, ng});
-            g += ng;
-          }
-        }
-    }
+    const double nh = ch.nh[q], np = ch.np[q], nb = ch.nb[q];
+    work[q] = nh * nh + np * np + np * nh + nb * nb + np * nb + nb * nh;
     if (np * nh) {
-      Seg2 sg{ch.off[(size_t)q * 6 + PPHH], np, np * nh, q, PPHH, ch.pp_start[q], ch.hh_start[q]};
+      Seg2 sg{ch.off[(size_t)q * 6 + PPHH], (int)np, (int)(np * nh), q, PPHH, ch.pp_start[q], ch.hh_start[q]};
       add_tiles(ptiles, (int)pseg.size(), sg.size);
       pseg.push_back(sg);
     }
@@ -437,41 +389,13 @@ int build_worklists(qd_context *c) {
     std::vector<int> order(ch.nblock);
     for (int q = 0; q < ch.nblock; q++) order[q] = q;
     std::sort(order.begin(), order.end(), [&](int a, int b) { return work[a] > work[b]; });
-    std::vector<std::vector<Task2>> xcd(NXCD);
     std::vector<double> load(NXCD, 0.0);
-    for (int q : order) {
-      if (per_block[q].empty()) continue;
-      int best = (int)(std::min_element(load.begin(), load.end()) - load.begin());
-      load[best] += work[q];
-      xcd[best].insert(xcd[best].end(), per_block[q].begin(), per_block[q].end());
-    }
-    size_t longest = 0;
-    for (auto &l : xcd) longest = std::max(longest, l.size());
-    std::vector<Task2> tasks(longest * NXCD, Task2{-1, 0, 0, 0});
-    for (int x = 0; x < NXCD; x++)
-      for (size_t k = 0; k < xcd[x].size(); k++) tasks[k * NXCD + x] = xcd[x][k];
-    if ((rc = dev_upload(c, tasks, &c->tasks2))) return rc;
-    c->ntasks2 = (int)tasks.size();
-    // the same block -> XCD assignment for the Pandya gather
-    std::vector<std::vector<PTask>> pxcd(NXCD);
-    std::fill(load.begin(), load.end(), 0.0);
-    for (int q : order) {
-      if (per_block[q].empty()) continue;
-      int best = (int)(std::min_element(load.begin(), load.end()) - load.begin());
-      load[best] += work[q];
-      pxcd[best].insert(pxcd[best].end(), pg_block[q].begin(), pg_block[q].end());
-    }
-    longest = 0;
-    for (auto &l : pxcd) longest = std::max(longest, l.size());
-    std::vector<PTask> ptasks(longest * NXCD, PTask{-1, 0, 0, 0, 0});
-    for (int x = 0; x < NXCD; x++)
-      for (size_t k = 0; k < pxcd[x].size(); k++) ptasks[k * NXCD + x] = pxcd[x][k];
     for (int pass = 0; pass < 2; pass++) {
       auto &src = pass == 0 ? bt1 : bt2;
       std::vector<std::vector<BTask>> bx(NXCD);
       std::fill(load.begin(), load.end(), 0.0);
       for (int q : order) {
-        if (per_block[q].empty()) continue;
+        if (work[q] == 0) continue;
         int best = (int)(std::min_element(load.begin(), load.end()) - load.begin());
         load[best] += work[q];
         bx[best].insert(bx[best].end(), src[q].begin(), src[q].end());
@@ -487,10 +411,6 @@ int build_worklists(qd_context *c) {
     }
     if (blist.empty()) blist.push_back(0);
     if ((rc = dev_upload(c, blist, &c->blist))) return rc;
-    if (groups.empty()) groups.push_back(make_int4(0, 0, 0, 0));
-    if ((rc = dev_upload(c, ptasks, &c->ptasks))) return rc;
-    if ((rc = dev_upload(c, groups, &c->pgroups))) return rc;
-    c->nptasks = (int)ptasks.size();
   }
   if ((rc = dev_upload(c, pseg, &c->pphh_segs))) return rc;
   if ((rc = dev_upload(c, ptiles, &c->pphh_tiles))) return rc;
@@ -872,8 +792,8 @@ int qd_context_create(int nbody, int nshell, qd_context **out) {
   UP(ch.spch, spch); UP(ch.ch_start, ch_start); UP(ch.ch_states, ch_states); UP(ch.ch_lookup, ch_lookup);
   UP(ch.ml, ml); UP(ch.ms, ms); UP(ch.cc_r, cc_r); UP(ch.cc_nb, cc_nb); UP(ch.cc_off, cc_off);
   UP(ch.cc_poff, cc_poff); UP(ch.cc_ab_dev, cc_ab); UP(ch.cc_hp, cc_hp); UP(ch.cc_ab_start, cc_ab_start);
-  UP(ch.cc_hp_start, cc_hp_start); UP(ccmap, cc_map); UP(ch.sub_start, sub_start); UP(ch.sub_list, sub_list);
-  UP(ch.rowperm, rowperm); UP(ch.pinfo, pinfo); UP(ch.cc_rs, cc_rs);
+  UP(ch.cc_hp_start, cc_hp_start); UP(ccmap, cc_map);
+  UP(ch.cc_rs, cc_rs);
   UP(ch.cp_a, cp_a); UP(ch.cp_b, cp_b); UP(ch.cp_grid_off, cp_grid_off); UP(ch.cp_grid, cp_grid);
   UP(ch.fcls_off, fcls_off);
   t.fcls_size = ch.fcls_size;
@@ -883,11 +803,6 @@ int qd_context_create(int nbody, int nshell, qd_context **out) {
   if ((rc = dev_alloc(c, sizeof(double) * ch.neq, &p))) return bail(rc); c->w2 = (double *)p;
   if ((rc = dev_alloc(c, sizeof(double) * ch.neq, &p))) return bail(rc); c->w3 = (double *)p;
   if ((rc = dev_alloc(c, sizeof(double) * ch.p_size, &p))) return bail(rc); c->P = (double *)p;
-  if ((rc = dev_alloc(c, sizeof(double2) * std::max<size_t>(ch.sub_list.size(), 1), &p))) return bail(rc);
-  c->sbra = (double2 *)p;
-  if ((rc = dev_alloc(c, sizeof(double2) * std::max<size_t>(ch.sub_list.size(), 1), &p))) return bail(rc);
-  c->sket = (double2 *)p;
-  c->npairs = (int)ch.pairs.size();
   if ((rc = dev_alloc(c, sizeof(double) * 2 * std::max(ch.fcls_size, 1), &p))) return bail(rc);
   c->fcls = (double *)p;
   if (const char *env = getenv("QD_BOX_CAP")) {

@@ -169,59 +169,6 @@ std::string build_channels(int nbody, int nshell, Channels &c) {
       if (c.cc_dev_row[c.cc_ab_start[b] + row] != want) return "internal error: CC device row order";
     }
 
-  // ---- substitution lists: pair (i,j) with i or j replaced by a state t of
-  // the same (m_l, m_s) channel stays in the same two-body block
-  c.sub_start.assign(c.pairs.size() + 1, 0);
-  c.sub_list.clear();
-  if (M > 2047) return "too many states for the substitution lists";
-  for (int b = 0; b < c.nblock; b++) {
-    const int base = c.hh_start[b];
-    const int ntot = c.nh[b] + c.nb[b] + c.np[b];
-    if (ntot > 65535) return "block too large for the substitution lists";
-    for (int cidx = 0; cidx < ntot; cidx++) {
-      const uint32_t pr = c.pairs[base + cidx];
-      const int i = lo16(pr), j = hi16(pr);
-      c.sub_start[base + cidx] = (int)c.sub_list.size();
-      for (int pos = 0; pos < 2; pos++) {
-        const int x = pos ? j : i, other = pos ? i : j;
-        const int ch = c.spch[x];
-        for (int s = c.ch_start[ch]; s < c.ch_start[ch + 1]; s++) {
-          const int t = c.ch_states[s];
-          if (t == other) continue;
-          int a, bb, neg;
-          if (pos == 0) { neg = t > other; a = neg ? other : t; bb = neg ? t : other; }
-          else { neg = other > t; a = neg ? t : other; bb = neg ? other : t; }
-          const uint32_t ia = c.iarr[(size_t)a * M - (size_t)a * (a + 1) / 2 + (bb - a - 1)];
-          if (ia_blk(ia) != b) return "internal error: substitution left its block";
-          const int ty = ia_type(ia);
-          const int comb = ty == T_HH ? ia_idx(ia) : ty == T_PH ? c.nh[b] + ia_idx(ia) : c.nh[b] + c.nb[b] + ia_idx(ia);
-          c.sub_list.push_back((uint32_t)comb | ((uint32_t)t << 16) | ((uint32_t)pos << 27) | ((uint32_t)neg << 28));
-        }
-      }
-    }
-  }
-  c.sub_start[c.pairs.size()] = (int)c.sub_list.size();
-
-  // ---- row order of the Pandya gather: inside each pair class of a block,
-  // rows sorted by (channel of i, channel of j, i, j), so consecutive rows
-  // form compact (i, j) grids whose cross-coupled partners are adjacent rows
-  // of the Pandya product
-  c.rowperm.assign(c.pairs.size(), 0);
-  for (int b = 0; b < c.nblock; b++) {
-    const int base = c.hh_start[b];
-    const int bounds[4] = {0, c.nh[b], c.nh[b] + c.nb[b], c.nh[b] + c.nb[b] + c.np[b]};
-    for (int cl = 0; cl < 3; cl++) {
-      std::vector<int> idx;
-      for (int x = bounds[cl]; x < bounds[cl + 1]; x++) idx.push_back(x);
-      std::stable_sort(idx.begin(), idx.end(), [&](int u, int v) {
-        const uint32_t pu = c.pairs[base + u], pv = c.pairs[base + v];
-        const int ku[4] = {c.spch[lo16(pu)], c.spch[hi16(pu)], lo16(pu), hi16(pu)};
-        const int kv[4] = {c.spch[lo16(pv)], c.spch[hi16(pv)], lo16(pv), hi16(pv)};
-        return std::lexicographical_compare(ku, ku + 4, kv, kv + 4);
-      });
-      for (size_t k = 0; k < idx.size(); k++) c.rowperm[base + bounds[cl] + k] = idx[k];
-    }
-  }
   // ---- class pairs and their ordered grids
   c.fcls_off.assign(c.nch, 0);
   c.fcls_size = 0;
@@ -235,6 +182,7 @@ std::string build_channels(int nbody, int nshell, Channels &c) {
   for (int b = 0; b < c.nblock; b++) {
     c.cp_start[b] = (int)c.cp_a.size();
     const int base = c.hh_start[b], ntot = c.nh[b] + c.nb[b] + c.np[b];
+    if (ntot > 65535) return "two-body block too large for the class-pair grids";
     std::vector<std::pair<int, int>> keys;
     for (int I = 0; I < ntot; I++) {
       const uint32_t pr = c.pairs[base + I];
@@ -269,16 +217,6 @@ std::string build_channels(int nbody, int nshell, Channels &c) {
   }
   c.cp_start[c.nblock] = (int)c.cp_a.size();
 
-  c.pinfo.assign(c.pairs.size(), 0u);
-  for (int b = 0; b < c.nblock; b++) {
-    const int base = c.hh_start[b], ntot = c.nh[b] + c.nb[b] + c.np[b];
-    for (int pos = 0; pos < ntot; pos++) {
-      const int I = c.rowperm[base + pos];
-      const uint32_t pr = c.pairs[base + I];
-      if (c.sp_pos[lo16(pr)] > 255 || c.sp_pos[hi16(pr)] > 255) return "single-particle channel too large";
-      c.pinfo[base + pos] = (uint32_t)I | ((uint32_t)c.sp_pos[lo16(pr)] << 16) | ((uint32_t)c.sp_pos[hi16(pr)] << 24);
-    }
-  }
   return "";
 }
 

@@ -43,131 +43,6 @@ __device__ __forceinline__ double wave_sum(double v) {
   return v;
 }
 
-// ---------------------------------------------------------------------------
-// Element of one two-body block by combined (hh | ph | pp) row/column index,
-// with the hermiticity rule of v_elem for the sub-blocks stored transposed.
-__device__ __forceinline__ double vfull(const DevTab &t, const double *__restrict__ v, int herm, int q, int nh,
-                                        int nb, int np, int P, int Q) {
-  const int64_t *o = t.off + 6 * q;
-  const int tp = P < nh ? 0 : (P < nh + nb ? 1 : 2), tq = Q < nh ? 0 : (Q < nh + nb ? 1 : 2);
-  const int ip = P - (tp == 0 ? 0 : (tp == 1 ? nh : nh + nb));
-  const int iq = Q - (tq == 0 ? 0 : (tq == 1 ? nh : nh + nb));
-  switch (tp * 3 + tq) {
-    case 0: return v[o[HHHH] + ip + (int64_t)iq * nh];
-    case 1: return herm * v[o[PHHH] + iq + (int64_t)ip * nb];
-    case 2: return herm * v[o[PPHH] + iq + (int64_t)ip * np];
-    case 3: return v[o[PHHH] + ip + (int64_t)iq * nb];
-    case 4: return v[o[PHPH] + ip + (int64_t)iq * nb];
-    case 5: return herm * v[o[PPPH] + iq + (int64_t)ip * np];
-    case 6: return v[o[PPHH] + ip + (int64_t)iq * np];
-    case 7: return v[o[PPPH] + ip + (int64_t)iq * np];
-    default: return v[o[PPPP] + ip + (int64_t)iq * np];
-  }
-}
-
-// One-body coefficients of every substitution-list entry for one operator
-// pair, so the 1b x 2b loops below read two doubles per entry instead of
-// evaluating f_elem.  For the replaced index x of pair g (x1 or x2 of the
-// pair, by the entry's pos bit) and the substitute t, the reference sums
-// over holes t with f(x,t) and over particles t with f(t,x), times herm of
-// the one-body operand on the bra side and -1 / -herm on the ket side
-// (src/commutators.f90:391-419 and the analogous loops per sub-block); the
-// entry's sort sign is folded in.  bra[k] = (c1, c2) multiplies (V2, V1),
-// ket[k] likewise.
-__global__ void __launch_bounds__(256) k_subcoef(DevTab t, Ops o, double2 *__restrict__ bra,
-                                                 double2 *__restrict__ ket, int npairs) {
-  const int g = blockIdx.x * 256 + threadIdx.x;
-  if (g >= npairs) return;
-  const uint32_t pr = t.pairs[g];
-  const int x1 = lo16(pr), x2 = hi16(pr), n = t.n;
-  const double h1 = o.h1, h2 = o.h2;
-  for (int k = t.sub_start[g], ke = t.sub_start[g + 1]; k < ke; k++) {
-    const uint32_t en = t.sub_list[k];
-    const int tt = (int)((en >> 16) & 0x7ffu);
-    const int xk = ((en >> 27) & 1u) ? x2 : x1;
-    const double sg = ((en >> 28) & 1u) ? -1.0 : 1.0;
-    double b1, b2, k1, k2;
-    if (tt < n) {
-      const double f1 = felem(t, o.v1, o.h1, xk, tt), f2 = felem(t, o.v2, o.h2, xk, tt);
-      b1 = f1; b2 = f2; k1 = -h1 * f1; k2 = -h2 * f2;
-    } else {
-      const double f1 = felem(t, o.v1, o.h1, tt, xk), f2 = felem(t, o.v2, o.h2, tt, xk);
-      b1 = h1 * f1; b2 = h2 * f2; k1 = -f1; k2 = -f2;
-    }
-    bra[k] = make_double2(sg * b1, sg * b2);
-    ket[k] = make_double2(sg * k1, sg * k2);
-  }
-}
-
-void launch_subcoef(hipStream_t s, const DevTab &t, const Ops &o, double2 *bra, double2 *ket, int npairs) {
-  if (npairs > 0) hipLaunchKernelGGL(k_subcoef, dim3((npairs + 255) / 256), dim3(256), 0, s, t, o, bra, ket, npairs);
-}
-
-// commutator_122 (src/commutators.f90:355-676) for the output <x1 x2|V|x3 x4>
-// in storage order: [f1, V2] - [f2, V1] in one pass over the substitution
-// lists.  Only substitutes t in the (m_l, m_s) channel of the replaced index
-// can give a non-zero element, which is what the precomputed lists
-// enumerate.  Bra substitutions read the LDS tiles (columns of the block);
-// one thread applies each bra entry to JB consecutive kets of the tile.  Ket
-// substitutions read block row Ic from global memory, coalesced over the
-// lanes (consecutive rows), at offsets from three per-thread (base, stride)
-// pairs, one per ket pair type.
-struct RowAddr {  // offsets of (Ic, Q) for Q of type 0 hh, 1 ph, 2 pp: base + iq * mul
-  int64_t b0, b1, b2, m0, m1, m2;
-  int tp;
-};
-
-__device__ __forceinline__ RowAddr row_addr(const DevTab &t, int q, int nh, int nb, int np, int Ic) {
-  const int64_t *o6 = t.off + 6 * q;
-  RowAddr r;
-  r.tp = Ic < nh ? 0 : (Ic < nh + nb ? 1 : 2);
-  const int ip = Ic - (r.tp == 0 ? 0 : (r.tp == 1 ? nh : nh + nb));
-  const int rows_p = r.tp == 0 ? nh : (r.tp == 1 ? nb : np);
-  if (r.tp == 0) {
-    r.b0 = o6[HHHH] + ip; r.m0 = nh;
-    r.b1 = o6[PHHH] + (int64_t)ip * nb; r.m1 = 1;
-    r.b2 = o6[PPHH] + (int64_t)ip * np; r.m2 = 1;
-  } else if (r.tp == 1) {
-    r.b0 = o6[PHHH] + ip; r.m0 = rows_p;
-    r.b1 = o6[PHPH] + ip; r.m1 = rows_p;
-    r.b2 = o6[PPPH] + (int64_t)ip * np; r.m2 = 1;
-  } else {
-    r.b0 = o6[PPHH] + ip; r.m0 = rows_p;
-    r.b1 = o6[PPPH] + ip; r.m1 = rows_p;
-    r.b2 = o6[PPPP] + ip; r.m2 = rows_p;
-  }
-  return r;
-}
-
-__device__ __forceinline__ double c122_ket(const DevTab &t, const Ops &o, const double2 *__restrict__ sket,
-                                           const RowAddr &ra, int nh, int nb, int gJ) {
-  double r = 0.0;
-  const double h1 = o.h1, h2 = o.h2;
-#pragma unroll 4
-  for (int k = t.sub_start[gJ], ke = t.sub_start[gJ + 1]; k < ke; k++) {
-    const int Qp = (int)(t.sub_list[k] & 0xffffu);
-    const int tq = Qp < nh ? 0 : (Qp < nh + nb ? 1 : 2);
-    const int iq = Qp - (tq == 0 ? 0 : (tq == 1 ? nh : nh + nb));
-    const int64_t a = tq == 0 ? ra.b0 + iq * ra.m0 : (tq == 1 ? ra.b1 + iq * ra.m1 : ra.b2 + iq * ra.m2);
-    const bool tr = ra.tp < tq;
-    const double v2 = tr ? h2 * o.v2[a] : o.v2[a], v1 = tr ? h1 * o.v1[a] : o.v1[a];
-    const double2 c = sket[k];
-    r += c.x * v2 - c.y * v1;
-  }
-  return r;
-}
-
-// Pandya difference D = XCC - YCC of construct_cc_ints (src/ME_general.f90:
-// 948-949) at CC rows (a,b), (c,d); k_pandya_combine has already turned the
-// product P = ETACC HCC^T into D = hh P - eh P^T in place (YCC = eh P^T).
-__device__ __forceinline__ double pandya_d(const DevTab &t, const double *__restrict__ D, int a, int b, int c,
-                                           int d) {
-  const int2 m1 = t.cc_map[a * t.M + b], m2 = t.cc_map[c * t.M + d];
-  const int q = m1.x;
-  if (t.cc_nb[q] == 0) return 0.0;
-  return D[t.cc_poff[q] + m1.y + (int64_t)m2.y * t.cc_r[q]];
-}
-
 // In place D = hh * P - eh * P^T on every r x r CC product, one 32x32 tile
 // pair (ti <= tj) per work-group through LDS.
 __global__ void __launch_bounds__(256) k_pandya_combine(DevTab t, double *__restrict__ P,
@@ -195,192 +70,6 @@ void launch_pandya_combine(hipStream_t s, const DevTab &t, double *P, const Tile
                            double eh) {
   if (ntp <= 0) return;
   hipLaunchKernelGGL(k_pandya_combine, dim3(ntp), dim3(256), 0, s, t, P, tp, hh, eh);
-}
-
-// All stored two-body outputs of one task: rows of every pair type that is
-// stored against this ket type (hh kets: hh, ph, pp rows; ph kets: ph, pp
-// rows; pp kets: pp rows), i.e. the six sub-blocks of the flat layout.
-__global__ void __launch_bounds__(256) k_twobody(DevTab t, Ops o, double *__restrict__ out,
-                                                 const Task2 *__restrict__ tasks, int flags,
-                                                 const double *__restrict__ w1, const double *__restrict__ w2,
-                                                 const double *__restrict__ w3,
-                                                 const double2 *__restrict__ sbra, const double2 *__restrict__ sket) {
-  __shared__ double lds[TB_LDS_DOUBLES];
-  const Task2 tk = tasks[blockIdx.x];
-  if (tk.block < 0) return;
-  const int q = tk.block, tau = tk.tau, j0 = tk.j0, tj = tk.tj;
-  const int nh = t.nh[q], nb = t.nb[q], np = t.np[q], ntot = nh + nb + np;
-  const int base = t.hh_start[q];
-  const int tstart = tau == 0 ? 0 : (tau == 1 ? nh : nh + nb);
-  const bool do122 = (flags & TB_122) != 0;
-  if (do122) {
-    for (int e = threadIdx.x; e < ntot * tj; e += 256) {
-      const int r = e % ntot, j = e / ntot;
-      lds[e] = vfull(t, o.v1, o.h1, q, nh, nb, np, r, tstart + j0 + j);
-      lds[ntot * tj + e] = vfull(t, o.v2, o.h2, q, nh, nb, np, r, tstart + j0 + j);
-    }
-    __syncthreads();
-  }
-  const int nrows = ntot - tstart;  // row types stored against this ket type
-  const int64_t *off = t.off + 6 * q;
-  constexpr int JB = 8;  // kets per thread
-  const int njb = (tj + JB - 1) / JB;
-  const double *tile1 = lds, *tile2 = lds + ntot * tj;
-  for (int e = threadIdx.x; e < nrows * njb; e += 256) {
-    const int Ic = tstart + e % nrows, jb = (e / nrows) * JB;
-    const int rho = Ic < nh ? 0 : (Ic < nh + nb ? 1 : 2);
-    const int Il = Ic - (rho == 0 ? 0 : (rho == 1 ? nh : nh + nb));
-    const int rows = rho == 0 ? nh : (rho == 1 ? nb : np);
-    int sub;
-    if (tau == 0) sub = rho == 0 ? HHHH : (rho == 1 ? PHHH : PPHH);
-    else if (tau == 1) sub = rho == 1 ? PHPH : PPPH;
-    else sub = PPPP;
-    const int nj = tj - jb < JB ? tj - jb : JB;
-    double bsum[JB] = {};
-    RowAddr ra{};
-    if (do122) {
-      const int gI = base + Ic;
-#pragma unroll 4
-      for (int k = t.sub_start[gI], ke = t.sub_start[gI + 1]; k < ke; k++) {
-        const int Pp = (int)(t.sub_list[k] & 0xffffu);
-        const double2 c = sbra[k];
-#pragma unroll
-        for (int b = 0; b < JB; b++)
-          if (b < nj) bsum[b] += c.x * tile2[(jb + b) * ntot + Pp] - c.y * tile1[(jb + b) * ntot + Pp];
-      }
-      ra = row_addr(t, q, nh, nb, np, Ic);
-    }
-#pragma unroll
-    for (int b = 0; b < JB; b++) {
-      if (b >= nj) break;
-      const int Jl = j0 + jb + b, Jc = tstart + Jl;
-      const int64_t idx = off[sub] + Il + (int64_t)Jl * rows;
-      double acc = (flags & TB_ACCUM) ? out[idx] : 0.0;
-      if (do122) acc += bsum[b] + c122_ket(t, o, sket, ra, nh, nb, base + Jc);
-      if (flags & TB_LADDER) {
-        // transposed / signed combinations of the products of matrix_mult222
-        // (src/commutators.f90:999-1127) that cannot be accumulated by the GEMM
-        const double h1 = o.h1, h2 = o.h2;
-        const int64_t tr = off[sub] + Jl + (int64_t)Il * rows;  // (J,I) of a square sub-block
-        switch (sub) {
-          case HHHH:
-          case PPPP:
-            acc += h1 * w1[tr] - h2 * w1[idx];
-            acc += h1 * h2 * w3[tr] - w3[idx];
-            break;
-          case PHHH: acc += h1 * w2[idx] - h2 * w1[idx]; break;
-          case PPPH: acc += -h2 * w1[idx] + h1 * w2[idx]; break;
-          case PHPH:
-            acc += h1 * w2[tr] - h2 * w2[idx];
-            acc += h1 * w1[tr] - h2 * w1[idx];
-            break;
-          default: break;
-        }
-      }
-      out[idx] = acc;
-    }
-  }
-}
-
-void launch_twobody(hipStream_t s, const DevTab &t, const Ops &o, double *out, const Task2 *tasks, int ntasks,
-                    int flags, const double *w1, const double *w2, const double *w3, const double2 *sbra,
-                    const double2 *sket) {
-  if (ntasks <= 0) return;
-  hipLaunchKernelGGL(k_twobody, dim3(ntasks), dim3(256), 0, s, t, o, out, tasks, flags, w1, w2, w3, sbra, sket);
-}
-
-// The Pandya part of commutator_222 (src/commutators.f90:746-751 and the
-// analogous lines per sub-block): every stored output <x1 x2|V|x3 x4> adds
-// D(x3,x1|x2,x4) - D(x3,x2|x1,x4) + D(x4,x2|x1,x3) - D(x4,x1|x2,x3) of the
-// combined product D (storage-order indices carry the ph sign).  A task is
-// one ket group (x3, x4 in fixed single-particle channels) against a bundle
-// of row groups (x1, x2 in fixed channels).  The device rows of a CC block
-// are sorted by the channel of their first state, so for one (row group,
-// ket group) box each of the four terms reads a dense sub-block of D whose
-// offset is fixed per box: e.g. D(x3,x1|x2,x4) sits at
-// base + pos3 |ch x1| + pos1 + (pos2 |ch x4| + pos4) r, with pos = rank of
-// the state in its channel.  The per-box offsets are set up once in LDS.
-__global__ void __launch_bounds__(256) k_pandya_gather(DevTab t, double *__restrict__ out,
-                                                       const PTask *__restrict__ tasks,
-                                                       const int4 *__restrict__ groups,
-                                                       const double *__restrict__ P, int accumulate) {
-  __shared__ int4 gs[PG_MAX];          // row groups: first position, length, channels of x1, x2
-  __shared__ int pre[PG_MAX + 1];      // element offsets of the boxes
-  __shared__ int64_t bbase[PG_MAX][4]; // per box and term: D offset of the sub-block (-1: no D block)
-  __shared__ int bld[PG_MAX][4];       // per box and term: leading dimension r of the D block
-  __shared__ int64_t bout[PG_MAX];     // per box: offset of the output sub-block
-  __shared__ int4 binfo[PG_MAX];       // per box: output rows, first combined index of the rows, |ch x1|, |ch x2|
-  const PTask tk = tasks[blockIdx.x];
-  if (tk.block < 0) return;
-  const int q = tk.block, tau = tk.tau, ng = tk.ng;
-  const int nh = t.nh[q], nb = t.nb[q];
-  const int base = t.hh_start[q];
-  const int tstart = tau == 0 ? 0 : (tau == 1 ? nh : nh + nb);
-  const int4 col = groups[tk.gc];
-  const int A3 = col.z, A4 = col.w;
-  const int a3 = t.ch_start[A3 + 1] - t.ch_start[A3], a4 = t.ch_start[A4 + 1] - t.ch_start[A4];
-  if ((int)threadIdx.x < ng) {
-    const int g = threadIdx.x;
-    const int4 rg = groups[tk.g0 + g];
-    gs[g] = rg;
-    const int A1 = rg.z, A2 = rg.w;
-    // (row pair | column pair) channels of D(x3,x1|x2,x4), D(x3,x2|x1,x4),
-    // D(x4,x2|x1,x3), D(x4,x1|x2,x3)
-    const int rc0[4] = {A3, A3, A4, A4}, rc1[4] = {A1, A2, A2, A1}, cc0[4] = {A2, A1, A1, A2};
-    for (int k = 0; k < 4; k++) {
-      const int s0 = t.ch_states[t.ch_start[rc0[k]]], s1 = t.ch_states[t.ch_start[rc1[k]]];
-      const int K = t.cc_map[s0 * t.M + s1].x;
-      bld[g][k] = t.cc_r[K];
-      bbase[g][k] = t.cc_nb[K] == 0 ? -1
-                                     : t.cc_poff[K] + t.cc_rs[K * t.nch + rc0[k]] +
-                                           (int64_t)t.cc_rs[K * t.nch + cc0[k]] * t.cc_r[K];
-    }
-    const int rho = rg.x < nh ? 0 : (rg.x < nh + nb ? 1 : 2);
-    int sub;
-    if (tau == 0) sub = rho == 0 ? HHHH : (rho == 1 ? PHHH : PPHH);
-    else if (tau == 1) sub = rho == 1 ? PHPH : PPPH;
-    else sub = PPPP;
-    bout[g] = t.off[6 * q + sub];
-    binfo[g] = make_int4(rho == 0 ? nh : (rho == 1 ? nb : t.np[q]), rho == 0 ? 0 : (rho == 1 ? nh : nh + nb),
-                         t.ch_start[A1 + 1] - t.ch_start[A1], t.ch_start[A2 + 1] - t.ch_start[A2]);
-  }
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    int acc = 0;
-    for (int g = 0; g < ng; g++) { pre[g] = acc; acc += gs[g].y * col.y; }
-    pre[ng] = acc;
-  }
-  __syncthreads();
-  const int total = pre[ng];
-  // elements of one (row group, ket group) box are consecutive, rows fastest
-  for (int e = threadIdx.x; e < total; e += 256) {
-    int lo = 0, hi = ng - 1;
-    while (lo < hi) {
-      const int mid = (lo + hi + 1) >> 1;
-      if (pre[mid] <= e) lo = mid;
-      else hi = mid - 1;
-    }
-    const int le = e - pre[lo], len = gs[lo].y;
-    const uint32_t pr = t.pinfo[base + gs[lo].x + le % len], pc = t.pinfo[base + col.x + le / len];
-    const int Ic = (int)(pr & 0xffffu), p1 = (int)((pr >> 16) & 0xffu), p2 = (int)(pr >> 24);
-    const int Jc = (int)(pc & 0xffffu), p3 = (int)((pc >> 16) & 0xffu), p4 = (int)(pc >> 24);
-    const int4 bi = binfo[lo];
-    const int a1 = bi.z, a2 = bi.w;
-    const int64_t b0 = bbase[lo][0], b1 = bbase[lo][1], b2 = bbase[lo][2], b3 = bbase[lo][3];
-    const double d0 = b0 < 0 ? 0.0 : P[b0 + p3 * a1 + p1 + (int64_t)(p2 * a4 + p4) * bld[lo][0]];
-    const double d1 = b1 < 0 ? 0.0 : P[b1 + p3 * a2 + p2 + (int64_t)(p1 * a4 + p4) * bld[lo][1]];
-    const double d2 = b2 < 0 ? 0.0 : P[b2 + p4 * a2 + p2 + (int64_t)(p1 * a3 + p3) * bld[lo][2]];
-    const double d3 = b3 < 0 ? 0.0 : P[b3 + p4 * a1 + p1 + (int64_t)(p2 * a3 + p3) * bld[lo][3]];
-    const double d = d0 - d1 + d2 - d3;
-    const int64_t idx = bout[lo] + (Ic - bi.y) + (int64_t)(Jc - tstart) * bi.x;
-    out[idx] = accumulate ? out[idx] + d : d;
-  }
-}
-
-void launch_pandya_gather(hipStream_t s, const DevTab &t, double *out, const PTask *tasks, int ntasks,
-                          const int4 *groups, const double *P, bool accumulate) {
-  if (ntasks <= 0) return;
-  hipLaunchKernelGGL(k_pandya_gather, dim3(ntasks), dim3(256), 0, s, t, out, tasks, groups, P, accumulate ? 1 : 0);
 }
 
 // ---------------------------------------------------------------------------

@@ -59,14 +59,6 @@ struct Channels {
   // cc_rs[K * nch + ch(i)] + pos(i) * |ch(j)| + pos(j), pos = rank in the channel.
   std::vector<int> cc_rs;
   std::vector<int> sp_pos;                  // per state: rank within its channel
-  // one-index substitutions of every pair (for the 1b x 2b commutator):
-  // entry = P' (combined hh|ph|pp index in the block, 16 bits) | t << 16
-  // (11 bits) | pos << 27 (which index was replaced) | neg << 28 (sort sign)
-  std::vector<int> sub_start;               // per global pair index, size npairs + 1
-  std::vector<uint32_t> sub_list;
-  std::vector<int> rowperm;                 // per global pair index: row order of the Pandya gather
-  // per position of that row order: combined index | pos(i) << 16 | pos(j) << 24
-  std::vector<uint32_t> pinfo;
   // Class pairs of the two-body pass: per block, the unordered pairs {A, B}
   // (A <= B) of single-particle channels whose state pairs lie in the block.
   // Each has a grid of |A| x |B| ordered (x1, x2), x1 the pos-th state of A
@@ -108,10 +100,6 @@ struct DevTab {
   const uint32_t *cc_ab, *cc_hp;
   const int *cc_ab_start, *cc_hp_start;
   const int2 *cc_map;
-  const int *sub_start;
-  const uint32_t *sub_list;
-  const int *rowperm;
-  const uint32_t *pinfo;
   const int *cc_rs;
   const int *cp_a, *cp_b, *cp_grid_off;
   const uint32_t *cp_grid;

@@ -14,10 +14,6 @@ struct Seg2 {
 };
 struct Tile2 { int seg, start; };
 
-// ---- two-body tasks: one (block, ket type, ket tile) per work-group.  The
-// work-group stages both operands' full block columns of its kets in LDS.
-struct Task2 { int block, tau, j0, tj; };  // tau: 0 hh, 1 ph, 2 pp kets; block < 0: idle
-constexpr int TB_LDS_DOUBLES = 4096;        // 32 KB: two tiles of ntot * tj doubles
 constexpr int NXCD = 8;                     // MI355X: work-group i runs on XCD i % 8
 struct TilePair { int blk, ti, tj; };       // 32x32 tiles (ti, tj) and (tj, ti) of one CC block
 
@@ -90,23 +86,6 @@ void launch_gemm_small(hipStream_t s, const GemmProb *probs, const GemmSeg *segs
 // else a persistent grid striding the list.
 void launch_gemm(hipStream_t s, const GemmProb *probs, const GemmSeg *segs, const GemmTile *tiles,
                  int ntiles, const Slots &slots, int grid, int shape, int prec);
-// one-body coefficients of the substitution lists for operator pair o (bra
-// and ket use), consumed by the 122 part of launch_twobody
-void launch_subcoef(hipStream_t s, const DevTab &t, const Ops &o, double2 *bra, double2 *ket, int npairs);
-void launch_twobody(hipStream_t s, const DevTab &t, const Ops &o, double *out, const Task2 *tasks,
-                    int ntasks, int flags, const double *w1, const double *w2, const double *w3,
-                    const double2 *sbra, const double2 *sket);
-// Pandya part of commutator_222 from the combined product P (D = XCC - YCC),
-// added to out (or written, accumulate = false)
-// Work of the Pandya gather: one ket group gc (of pair class tau) against
-// row groups g0 .. g0+ng-1 of the group list; a group = a run of positions of
-// the block's rowperm order with one (pair class, channel of i, channel of
-// j), stored as (first position, length, channel of i, channel of j).
-struct PTask { int block, tau, gc, g0, ng; };
-constexpr int PG_MAX = 64;        // row groups per task
-constexpr int PG_TARGET = 1024;   // output elements per task (bundling target)
-void launch_pandya_gather(hipStream_t s, const DevTab &t, double *out, const PTask *tasks, int ntasks,
-                          const int4 *groups, const double *P, bool accumulate);
 
 // ---- class-pair boxes of the two-body pass.  A box = the stored outputs
 // whose bra lies in one class pair (rows) and whose ket in another (columns).
