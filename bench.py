@@ -26,11 +26,11 @@ sys.path.insert(0, ROOT)
 
 PEAK_FP64_MFMA = 78.6e12   # MI355X dense FP64 matrix, FLOP/s
 PEAK_HBM = 8.0e12          # MI355X HBM3E, B/s
-CLASS_NAMES = ["gemm_products", "gemm_accumulate", "twobody_gather", "onebody", "calc_cc", "other", "pandya_gather"]
+CLASS_NAMES = ["gemm_products", "gemm_accumulate", "twobody", "onebody", "calc_cc", "other", "reserved"]
 NCLS = len(CLASS_NAMES)
 # kernels behind each timing class (names as rocprofv3 reports them)
-CLASS_KERNELS = {0: ["k_gemm"], 1: ["k_gemm"], 2: ["k_subcoef", "k_twobody"], 3: ["k_cc_matvec", "k_onebody"],
-                 4: ["k_calc_cc"], 6: ["k_pandya_gather"]}
+CLASS_KERNELS = {0: ["k_gemm", "k_pandya_fused"], 1: ["k_gemm"], 2: ["k_fclass", "k_box2b"],
+                 3: ["k_cc_gemv", "k_onebody", "k_onebody221"], 4: ["k_calc_cc"]}
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "hbm_traffic_n20r14.json")
 
 
@@ -431,11 +431,11 @@ def main():
                     achieved=achieved / 1e12, peak=PEAK_FP64_MFMA / 1e12, unit="TFLOP/s")
     else:
         # algorithmic bytes per launch: two-body pass = per output element one
-        # write, two operand reads (the 1b x 2b term) and four reads of ladder
-        # intermediates; Pandya gather = one read-modify-write and four reads
-        # of the combined product; one-body = per one-body element one write
-        # and two operand reads; calc_cc = one gathered read and one write
-        byts = {2: 8.0 * n2b * (3 + 4), 6: 8.0 * n2b * (2 + 4), 3: 8.0 * 3 * (ch.neq - n2b),
+        # write, two operand reads (the 1b x 2b term), four reads of ladder
+        # intermediates and four reads of the Pandya product D; one-body = per
+        # one-body element one write and two operand reads; calc_cc = one
+        # gathered read and one write
+        byts = {2: 8.0 * n2b * (3 + 4 + 4), 3: 8.0 * 3 * (ch.neq - n2b),
                 4: 8.0 * 2 * ch.cc_size}.get(dom, 0.0)
         t = per_launch[CLASS_NAMES[dom]] * 1e-3
         roof = dict(kernel=f"{CLASS_NAMES[dom]} ({' + '.join(CLASS_KERNELS.get(dom, []))})", bound="hbm",

@@ -297,7 +297,7 @@ int qd_basis_expand(qd_basis *b, const double *coef, qd_op *out);
  *                   pairs (i, j) with i + j < S on BF16 MFMA
  * Split modes accumulate in FP32 over K stages of 32 and in FP64 across
  * stages and into the output.  Everything else stays FP64: the one-body
- * pieces, 122, the Pandya gather, the zero-body traces, White denominators,
+ * pieces, 122, the gather of the Pandya term, the zero-body traces, White denominators,
  * the BCH sum, norms and the Omega update.  No counterpart in the reference
  * (src/commutators.f90:999-1127 and src/ME_general.f90:948-949 call dgemm).
  * Drops captured BCH graphs. */
@@ -328,8 +328,9 @@ int qd_hf_hamiltonian(int nbody, int nshell, double hw, double *h0, int64_t neq,
 /* ---- timing of the dominant kernels (HIP events on the context stream) --
  * Accumulates per-kernel-class device time while enabled.  Classes: 0 the
  * product GEMM launch (ladder intermediates + Pandya products), 1 the
- * accumulating ladder GEMM launch, 2 two-body pass (122 + ladder combine),
- * 3 one-body, 4 calc_cc, 5 other, 6 Pandya gather. */
+ * accumulating ladder GEMM launch, 2 two-body pass (122, ladder combine and
+ * the gather of the Pandya term), 3 one-body, 4 calc_cc, 5 other, 6 reserved
+ * (always 0; the Pandya gather joined class 2). */
 int qd_timing_enable(qd_context *ctx, int on);
 int qd_timing_read(qd_context *ctx, double *ms_per_class, int64_t *launches_per_class, int nclass);
 /* Algorithmic FP64 flops of one full commutator's GEMM launches: the product

@@ -56,7 +56,7 @@ const double R4_0P1 = 0.10000000149011612;      // stp = 0.1, src/gs_IMSRG.f90:9
 const double R4_1EM7 = 1.0000000116860974e-07;  // conv_criteria, src/gs_IMSRG.f90:13
 
 constexpr int NCLASS = 7;
-enum { TC_PRODUCTS = 0, TC_ACCUM = 1, TC_TWOBODY = 2, TC_ONEBODY = 3, TC_CALCCC = 4, TC_OTHER = 5, TC_PANDYA = 6 };
+enum { TC_PRODUCTS = 0, TC_ACCUM = 1, TC_TWOBODY = 2, TC_ONEBODY = 3, TC_CALCCC = 4, TC_OTHER = 5 };  // 6 reserved
 
 // One grouped-GEMM work list.  Problems [0, npre) are the ladder products;
 // [npre, nprob) the Pandya products (empty for the accumulating list).

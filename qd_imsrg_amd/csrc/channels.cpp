@@ -160,7 +160,7 @@ std::string build_channels(int nbody, int nshell, Channels &c) {
   c.cc_rs.assign((size_t)c.ncc * c.nch, -1);
   for (int b = 0; b < c.ncc; b++)
     for (int d = c.cc_r[b] - 1; d >= 0; d--) c.cc_rs[(size_t)b * c.nch + c.spch[lo16(c.cc_ab_dev[c.cc_ab_start[b] + d])]] = d;
-  // the closed form of the device row, relied on by the Pandya gather
+  // the closed form of the device row, relied on by the two-body boxes
   for (int i = 0; i < M; i++)
     for (int j = 0; j < M; j++) {
       const int b = c.cc_map[((size_t)M * i + j) * 2], row = c.cc_map[((size_t)M * i + j) * 2 + 1];
