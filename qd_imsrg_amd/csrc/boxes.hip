@@ -61,7 +61,10 @@ __global__ void __launch_bounds__(256) k_box2b(DevTab t, Ops o, double *__restri
   __shared__ double fl[BX_FCAP];               // channel blocks: f1 then f2, A3, A4, then A1, A2 per box
   __shared__ int pre[BX_GMAX + 1];             // element offsets of the boxes
   __shared__ int4 bx[BX_GMAX];                 // class pair of the rows, rows in the task, grid offset, F offset
-  __shared__ int4 bs[BX_GMAX];                 // |A1|, |A2|, A1 == A2, unused
+  __shared__ int4 bs[BX_GMAX];                 // |A1|, |A2|, A1 == A2, first row entry in rent
+  __shared__ uint32_t rent[BX_RCAP];           // grid entries of the task's rows, box after box
+  __shared__ uint32_t cent[BX_CMAX];           // grid entries of the task's columns
+  __shared__ int64_t soff[3][3];               // stored sub-block of (row type, column type), row >= column
   __shared__ float2 brc[BX_GMAX];              // 1 / rows in the task, 1 / |A2|
   __shared__ int64_t pb[BX_GMAX][4];           // Pandya: D offset of each term's sub-block (-1: no D block)
   __shared__ int pl[BX_GMAX][4];               // Pandya: leading dimension of each term's D block
@@ -74,18 +77,25 @@ __global__ void __launch_bounds__(256) k_box2b(DevTab t, Ops o, double *__restri
   const int s3 = t.ch_start[A3 + 1] - t.ch_start[A3], s4 = t.ch_start[A4 + 1] - t.ch_start[A4];
   const int nc = mode == 1 ? tk.cnt : s3 * s4, ldc = nc | 1;
   const int c_off = mode == 1 ? tk.lo : 0, r_off = mode == 2 ? tk.lo : 0;
-  const uint32_t *cgrid = t.cp_grid + t.cp_grid_off[tk.gcol];
+  const uint32_t *cgrid = t.cp_grid + t.cp_grid_off[tk.gcol] + c_off;
   const bool do122 = (flags & TB_122) != 0, first = mode != 2;
   const bool dopd = first && (flags & TB_PANDYA) && D;
+  if (threadIdx.x < 9) {
+    const int tA = threadIdx.x / 3, tB = threadIdx.x % 3;
+    const int sub = tB == 0 ? (tA == 0 ? HHHH : (tA == 1 ? PHHH : PPHH)) : (tB == 1 ? (tA == 1 ? PHPH : PPPH) : PPPP);
+    soff[tA][tB] = tA >= tB ? t.off[6 * q + sub] : 0;
+  }
+  for (int k = threadIdx.x; k < nc; k += 256) cent[k] = cgrid[k];
   if (threadIdx.x == 0) {
-    int acc = 0, goff = 0, foff = 2 * (s3 * s3 + s4 * s4);
+    int acc = 0, goff = 0, foff = 2 * (s3 * s3 + s4 * s4), roff = 0;
     for (int b = 0; b < ng; b++) {
       const int g = blist[tk.first + b];
       const int A1 = t.cp_a[g], A2 = t.cp_b[g];
       const int s1 = t.ch_start[A1 + 1] - t.ch_start[A1], s2 = t.ch_start[A2 + 1] - t.ch_start[A2];
       const int nr = mode == 2 ? tk.cnt : s1 * s2;
       bx[b] = make_int4(g, nr, goff, foff);
-      bs[b] = make_int4(s1, s2, A1 == A2, 0);
+      bs[b] = make_int4(s1, s2, A1 == A2, roff);
+      roff += nr;
       brc[b] = make_float2(1.0f / (float)nr, 1.0f / (float)s2);
       pre[b] = acc;
       acc += nr * nc;
@@ -109,6 +119,12 @@ __global__ void __launch_bounds__(256) k_box2b(DevTab t, Ops o, double *__restri
                                            : t.cc_poff[K] + t.cc_rs[K * t.nch + rc0[k]] +
                                                  (int64_t)t.cc_rs[K * t.nch + cc0[k]] * t.cc_r[K];
     }
+  }
+  __syncthreads();
+  for (int b = 0; b < ng; b++) {
+    const int4 B = bx[b];
+    const uint32_t *rg = t.cp_grid + t.cp_grid_off[B.x] + r_off;
+    for (int k = threadIdx.x; k < B.y; k += 256) rent[bs[b].w + k] = rg[k];
   }
   __syncthreads();
   const int total = pre[ng];
@@ -142,11 +158,16 @@ __global__ void __launch_bounds__(256) k_box2b(DevTab t, Ops o, double *__restri
       if (b + 1 < ng && pre[b + 1] <= e) b = box_of(pre, b + 1, ng, e);
       const int4 B = bx[b];
       const int le = e - pre[b], c = qdiv(le, brc[b].x), r = le - c * B.y;
-      const uint32_t er = t.cp_grid[t.cp_grid_off[B.x] + r_off + r], ec = cgrid[c_off + c];
+      const uint32_t er = rent[bs[b].w + r], ec = cent[c];
       double v1 = 0.0, v2 = 0.0;
       if ((er >> 17) & (ec >> 17) & 1u) {
-        bool tr;
-        const int64_t a = vidx(t, q, nh, nb, np, (int)(er & 0xffffu), (int)(ec & 0xffffu), &tr);
+        // stored element of (I, J), or of (J, I) times herm when the row type
+        // is below the column type (branch-free over the pair types)
+        const int tyr = (int)((er >> 18) & 3u), tyc = (int)((ec >> 18) & 3u);
+        const int Il = (int)(er & 0xffffu) - tstart[tyr], Jl = (int)(ec & 0xffffu) - tstart[tyc];
+        const bool tr = tyr < tyc;
+        const int tA = tr ? tyc : tyr, tB = tr ? tyr : tyc;
+        const int64_t a = soff[tA][tB] + (tr ? Jl : Il) + (int64_t)(tr ? Il : Jl) * tcount[tA];
         const bool neg = ((er ^ ec) >> 16) & 1u;
         v1 = tr ? o.h1 * o.v1[a] : o.v1[a];
         v2 = tr ? o.h2 * o.v2[a] : o.v2[a];
@@ -157,7 +178,6 @@ __global__ void __launch_bounds__(256) k_box2b(DevTab t, Ops o, double *__restri
     }
     __syncthreads();
   }
-  const int64_t *off = t.off + 6 * q;
   const double h1 = o.h1, h2 = o.h2;
   const float rs4 = 1.0f / (float)s4;
   const bool same34 = A3 == A4;
@@ -166,7 +186,7 @@ __global__ void __launch_bounds__(256) k_box2b(DevTab t, Ops o, double *__restri
     const int4 B = bx[b];
     const int le = e - pre[b], c = qdiv(le, brc[b].x), r = le - c * B.y;
     const int rr = r_off + r, cc = c_off + c;
-    const uint32_t er = t.cp_grid[t.cp_grid_off[B.x] + rr], ec = cgrid[cc];
+    const uint32_t er = rent[bs[b].w + r], ec = cent[c];
     if (!((er >> 17) & (ec >> 17) & 1u)) continue;
     const int tyr = (int)((er >> 18) & 3u), tyc = (int)((ec >> 18) & 3u);
     const bool sr = (er >> 16) & 1u, sc = (ec >> 16) & 1u;
@@ -223,7 +243,7 @@ __global__ void __launch_bounds__(256) k_box2b(DevTab t, Ops o, double *__restri
     if (tyc == 0) sub = tyr == 0 ? HHHH : (tyr == 1 ? PHHH : PPHH);
     else if (tyc == 1) sub = tyr == 1 ? PHPH : PPPH;
     else sub = PPPP;
-    const int64_t idx = off[sub] + Il + (int64_t)Jl * rows;
+    const int64_t idx = soff[tyr][tyc] + Il + (int64_t)Jl * rows;
     if (!first) {
       out[idx] += val;
       continue;
@@ -233,7 +253,7 @@ __global__ void __launch_bounds__(256) k_box2b(DevTab t, Ops o, double *__restri
     if (flags & TB_LADDER) {
       // transposed / signed combinations of the products of matrix_mult222
       // (src/commutators.f90:999-1127) that the GEMMs cannot accumulate
-      const int64_t tr = off[sub] + Jl + (int64_t)Il * rows;  // (J,I) of a square sub-block
+      const int64_t tr = soff[tyr][tyc] + Jl + (int64_t)Il * rows;  // (J,I) of a square sub-block
       switch (sub) {
         case HHHH:
         case PPPP:

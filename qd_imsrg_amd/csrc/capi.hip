@@ -348,19 +348,19 @@ int build_worklists(qd_context *c) {
         const int s3 = csize(ch.cp_a[gc]), s4 = csize(ch.cp_b[gc]);
         const int nc = s3 * s4, ldc = nc | 1, fcol = 2 * (s3 * s3 + s4 * s4);
         std::vector<int> cur;
-        int goff = 0, foff = fcol;
+        int goff = 0, foff = fcol, roff = 0;
         auto flush = [&]() {
           if (!cur.empty()) {
             bt1[q].push_back(BTask{q, gc, (int)blist.size(), (int)cur.size(), 0, 0, 0});
             blist.insert(blist.end(), cur.begin(), cur.end());
           }
-          cur.clear(); goff = 0; foff = fcol;
+          cur.clear(); goff = 0; foff = fcol; roff = 0;
         };
         for (int gr = ch.cp_start[q]; gr < ch.cp_start[q + 1]; gr++) {
           if (ch.cp_tmax[gr] < ch.cp_tmin[gc]) continue;
           const int s1 = csize(ch.cp_a[gr]), s2 = csize(ch.cp_b[gr]);
           const int nr = s1 * s2, fneed = 2 * (s1 * s1 + s2 * s2);
-          if (fcol + fneed > BX_FCAP || 3 * nr > cap || ldc > cap)
+          if (fcol + fneed > BX_FCAP || 3 * nr > cap || ldc > cap || nc > BX_CMAX || nr > BX_RCAP)
             return fail(QD_E_LAYOUT, "single-particle channel of %d states too large for the two-body boxes",
                         std::max(std::max(s1, s2), std::max(s3, s4)));
           if (nr * ldc > cap) {
@@ -377,10 +377,12 @@ int build_worklists(qd_context *c) {
             }
             continue;
           }
-          if (goff + nr * ldc > cap || foff + fneed > BX_FCAP || (int)cur.size() == BX_GMAX) flush();
+          if (goff + nr * ldc > cap || foff + fneed > BX_FCAP || roff + nr > BX_RCAP || (int)cur.size() == BX_GMAX)
+            flush();
           cur.push_back(gr);
           goff += nr * ldc;
           foff += fneed;
+          roff += nr;
         }
         flush();
       }
@@ -1563,10 +1565,11 @@ int qd_get_precision(qd_context *c, int *prec) {
 namespace {
 int size_class(int x) { return x <= 16 ? 0 : x <= 32 ? 1 : x <= 64 ? 2 : x <= 128 ? 3 : x <= 256 ? 4 : 5; }
 
-// problems of a list grouped by size bucket (M, N, K classes), largest total work first
-std::vector<std::vector<int>> size_buckets(const GemmList &g) {
+// problems [0, np) of a list grouped by size bucket (M, N, K classes),
+// largest total work first
+std::vector<std::vector<int>> size_buckets(const GemmList &g, int np) {
   std::map<int, std::vector<int>> m;
-  for (int pi = 0; pi < (int)g.hprobs.size(); pi++) {
+  for (int pi = 0; pi < np; pi++) {
     const GemmProb &p = g.hprobs[pi];
     m[size_class(p.M) * 100 + size_class(p.N) * 10 + size_class(ktot(g, p))].push_back(pi);
   }
@@ -1585,9 +1588,12 @@ std::vector<std::vector<int>> size_buckets(const GemmList &g) {
 
 // Coordinate descent over size buckets: for each bucket (largest work first)
 // every tile shape is tried with all other buckets as they stand, and the one
-// giving the shortest device time of the whole GEMM work of a commutator (both
-// grouped lists, as launched) is kept.  Timing the whole launch rather than a
-// bucket alone prices a shape by what it adds to the grouped launch.
+// giving the shortest device time of the GEMM launches of a commutator with
+// the Pandya term (both grouped lists, as launched at the current precision
+// and launch form) is kept.  Timing the whole launch rather than a bucket
+// alone prices a shape by what it adds to the grouped launch.  Where the
+// Pandya products run in the fused kernel, their problems are neither timed
+// nor tuned.
 int qd_autotune(qd_context *c, int reps, double *ms_before, double *ms_after) {
   if (!c) return fail(QD_E_ARG, "null context");
   if (reps < 1) reps = 3;
@@ -1602,12 +1608,13 @@ int qd_autotune(qd_context *c, int reps, double *ms_before, double *ms_after) {
   hipEvent_t e0, e1;
   HIPC(hipEventCreate(&e0));
   HIPC(hipEventCreate(&e1));
+  const bool fused = c->launch_mode == 0 && c->gemm_prec == GEMM_FP64;
   auto total = [&](double *ms) -> int {
     int r2;
-    if ((r2 = run_gemm_list(c, c->g_all, true, s1)) || (r2 = run_gemm_list(c, c->g_acc, false, s1))) return r2;  // warm
+    if ((r2 = run_gemm_list(c, c->g_all, !fused, s1)) || (r2 = run_gemm_list(c, c->g_acc, false, s1))) return r2;
     HIPC(hipEventRecord(e0, c->stream));
     for (int r = 0; r < reps; r++) {
-      if ((r2 = run_gemm_list(c, c->g_all, true, s1))) return r2;
+      if ((r2 = run_gemm_list(c, c->g_all, !fused, s1))) return r2;
       if ((r2 = run_gemm_list(c, c->g_acc, false, s1))) return r2;
     }
     HIPC(hipEventRecord(e1, c->stream));
@@ -1621,7 +1628,8 @@ int qd_autotune(qd_context *c, int reps, double *ms_before, double *ms_after) {
   if ((rc = total(&before))) return rc;
   best_total = before;
   for (GemmList *g : {&c->g_all, &c->g_acc}) {
-    for (const auto &bk : size_buckets(*g)) {
+    const int np = (g == &c->g_all && fused) ? g->npre : (int)g->hprobs.size();
+    for (const auto &bk : size_buckets(*g, np)) {
       const int cur = g->hprobs[bk[0]].shape;
       int best = cur;
       for (int sh = 0; sh < NSHAPE; sh++) {

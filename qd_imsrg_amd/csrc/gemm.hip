@@ -24,6 +24,7 @@ typedef double d4 __attribute__((ext_vector_type(4)));
 // Walk of K in chunks of KC over all segments of a problem, with the next
 // chunk's global loads issued into registers (load) before the current
 // chunk's MFMAs (compute) and written to LDS (store) after the next barrier.
+// compute(kv) gets the count of valid k of the chunk (the rest is zero fill).
 template <int KC, class Load, class Store, class Compute>
 __device__ __forceinline__ void chunk_walk(const GemmProb &pr, const GemmSeg *__restrict__ segs, Load load,
                                            Store store, Compute compute) {
@@ -33,6 +34,7 @@ __device__ __forceinline__ void chunk_walk(const GemmProb &pr, const GemmSeg *__
   load(sg, 0);
   int cta = sg.ta, ctb = sg.tb;
   while (true) {
+    const int kv = sg.K - k0 < KC ? sg.K - k0 : KC;
     int nsi = si, nk0 = k0 + KC;
     GemmSeg nsg = sg;
     if (nk0 >= sg.K) {
@@ -49,7 +51,7 @@ __device__ __forceinline__ void chunk_walk(const GemmProb &pr, const GemmSeg *__
       cta = nsg.ta;
       ctb = nsg.tb;
     }
-    compute();
+    compute(kv);
     if (!more) break;
     si = nsi;
     k0 = nk0;
@@ -123,9 +125,10 @@ __device__ __forceinline__ void gemm_tile(const GemmProb &pr, const GemmSeg *__r
       }
     }
   };
-  auto compute = [&]() {
+  auto compute = [&](int kv) {
 #pragma unroll
     for (int kk = 0; kk < GEMM_TK; kk += 4) {
+      if (kk >= kv) break;  // zero fill past the segment's K adds nothing
       double a[FM], b[FN];
 #pragma unroll
       for (int x = 0; x < FM; x++) a[x] = As[(kk + lk) * (TM + 1) + m0 + x * 16 + lr];
@@ -278,7 +281,7 @@ __device__ __forceinline__ void gemm_tile_bf16(const GemmProb &pr, const GemmSeg
       }
     }
   };
-  auto compute = [&]() {
+  auto compute = [&](int) {
     bf8 a[S][FM], b[S][FN];
 #pragma unroll
     for (int s = 0; s < S; s++) {
@@ -358,7 +361,7 @@ __device__ __forceinline__ void gemm_tile_f32(const GemmProb &pr, const GemmSeg 
       }
     }
   };
-  auto compute = [&]() {
+  auto compute = [&](int) {
     f4 c[FM][FN];
 #pragma unroll
     for (int x = 0; x < FM; x++)
