@@ -80,6 +80,10 @@ struct qd_context {
   Channels ch;
   DevTab dt{};
   hipStream_t stream = nullptr;
+  // side stream (default priority) for the one-body and zero-body pieces of a
+  // commutator, forked from and joined back to `stream` with these events
+  hipStream_t aux = nullptr;
+  hipEvent_t ev_fork = nullptr, ev_join = nullptr;
   std::vector<void *> allocs;
   double *w1 = nullptr, *w2 = nullptr, *w3 = nullptr, *P = nullptr;
   double *u1 = nullptr, *u2 = nullptr;  // cross-coupled mat-vecs of commutator_121 (M*M each)
@@ -662,18 +666,33 @@ int full_commutator(qd_context *c, const qd_op *r1, const qd_op *r2, qd_op *r3, 
   int rc;
   const bool pd = pandya_on(hcc, ecc);
   if ((rc = run_products(c, r1, r2, r3, hcc, ecc))) return rc;
-  if ((rc = run_twobody(c, r1, r2, r3, TB_122 | TB_LADDER | (pd ? TB_PANDYA : 0), hcc, ecc))) return rc;
-  if ((rc = run_acc(c, r1, r2, r3))) return rc;
+  // The one-body part (it writes r3's one-body region and reads the products)
+  // and the zero-body traces (r3[0]) touch nothing that the two-body pass and
+  // the accumulating GEMMs write, so they run on the side stream meanwhile and
+  // are joined before return.  With timing on, everything stays on one stream
+  // so that each timing class is measured alone.
+  const bool side = !c->timing;
+  hipStream_t main = c->stream;
+  if (side) {
+    HIPC(hipEventRecord(c->ev_fork, main));
+    HIPC(hipStreamWaitEvent(c->aux, c->ev_fork, 0));
+    c->stream = c->aux;
+  }
   // with the Pandya inputs on hand, the 121 mat-vecs read them (hcc and ecc
   // are calc_cc of r2 and r1 by the entry point's contract)
-  if ((rc = run_onebody(c, r1, r2, r3, OB_111 | OB_121 | OB_221, pd ? ecc : nullptr, pd ? hcc : nullptr, true)))
-    return rc;
-  {
+  rc = run_onebody(c, r1, r2, r3, OB_111 | OB_121 | OB_221, pd ? ecc : nullptr, pd ? hcc : nullptr, true);
+  if (!rc) {
     Ops o{r1->d, r2->d, r1->herm, r2->herm};
     Timer tm(c, TC_OTHER);
     launch_zerobody(c->stream, c->dt, o, c->pphh_segs, c->npphh_segs, c->partials, r3->d, 3);
-    HIPC(hipGetLastError());
+    if (hipGetLastError() != hipSuccess) rc = fail(QD_E_HIP, "zero-body launch failed");
   }
+  c->stream = main;
+  if (side) HIPC(hipEventRecord(c->ev_join, c->aux));
+  if (rc) return rc;
+  if ((rc = run_twobody(c, r1, r2, r3, TB_122 | TB_LADDER | (pd ? TB_PANDYA : 0), hcc, ecc))) return rc;
+  if ((rc = run_acc(c, r1, r2, r3))) return rc;
+  if (side) HIPC(hipStreamWaitEvent(main, c->ev_join, 0));
   return QD_OK;
 }
 
@@ -772,6 +791,9 @@ int qd_context_create(int nbody, int nshell, qd_context **out) {
   int rc = QD_OK;
   auto bail = [&](int r) { qd_context_destroy(c); return r; };
   if (hipStreamCreate(&c->stream) != hipSuccess) { delete c; return fail(QD_E_HIP, "hipStreamCreate failed"); }
+  if (hipStreamCreate(&c->aux) != hipSuccess || hipEventCreateWithFlags(&c->ev_fork, hipEventDisableTiming) != hipSuccess ||
+      hipEventCreateWithFlags(&c->ev_join, hipEventDisableTiming) != hipSuccess)
+    return bail(fail(QD_E_HIP, "side stream or events could not be created"));
   {
     int dev = 0, ncu = 0;
     if (hipGetDevice(&dev) == hipSuccess &&
@@ -833,6 +855,9 @@ int qd_context_destroy(qd_context *c) {
   for (auto &kv : c->graphs) hipGraphExecDestroy(kv.second);
   for (void *p : c->allocs) hipFree(p);
   if (c->host_scal) hipHostFree(c->host_scal);
+  if (c->ev_fork) hipEventDestroy(c->ev_fork);
+  if (c->ev_join) hipEventDestroy(c->ev_join);
+  if (c->aux) hipStreamDestroy(c->aux);
   if (c->stream) hipStreamDestroy(c->stream);
   delete c;
   return QD_OK;

@@ -96,10 +96,10 @@ void launch_gemm(hipStream_t s, const GemmProb *probs, const GemmSeg *segs, cons
 struct BTask { int block, gcol, first, ng, mode, lo, cnt; };
 // LDS per work-group stays under 32 KB (five per CU); channels of up to 10
 // states (R = 20 shells) fit the channel-block area
-constexpr int BX_VCAP = 1400;  // grid elements per operand staged in LDS
+constexpr int BX_VCAP = 1320;  // grid elements per operand staged in LDS
 constexpr int BX_FCAP = 800;   // doubles of one-body channel blocks staged in LDS
 constexpr int BX_GMAX = 32;    // boxes per task
-constexpr int BX_RCAP = 512;   // row grid entries per task (staged in LDS)
+constexpr int BX_RCAP = 256;   // row grid entries per task (staged in LDS)
 constexpr int BX_CMAX = 128;   // column grid entries per task (a class pair of <= 11 x 11 states)
 // one-body matrices of both operands blocked by channel: F[0, fcls_size) of
 // r1, F[fcls_size, 2 fcls_size) of r2
