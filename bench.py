@@ -392,9 +392,9 @@ def main():
     HCC, ECC = ctx.cc(1.0), ctx.cc(1.0)
 
     def step():
-        qd.calc_cc(Y, HCC)
-        qd.calc_cc(X, ECC)
-        qd.commutator(X, Y, Z, HCC, ECC)
+        # calc_cc(Y, HCC), calc_cc(X, ECC), then [X, Y] with them (one call,
+        # asynchronous: E0 stays on the device)
+        qd.commutator_cc(X, Y, Z, HCC, ECC, e0=False)
 
     for _ in range(a.warmup):
         step()

@@ -148,6 +148,13 @@ int qd_mbpt2(qd_context *ctx, const qd_op *h, double *out);
  * means an all-zero CC (the reference's Magnus driver). */
 int qd_commutator(qd_context *ctx, const qd_op *r1, const qd_op *r2, qd_op *r3,
                   const qd_cc *hcc, const qd_cc *etacc, double *e0);
+/* calc_cc(r2, hcc) and calc_cc(r1, etacc), then the full [r1,r2] -> r3 with
+ * them: the sequence of dGam (src/gs_IMSRG.f90:256-268) in one call.  Both CC
+ * sets are overwritten (their herm fields are kept).  In the FP64 grouped form
+ * the two calc_cc launches overlap the ladder products, which do not read them.
+ * Equal to qd_calc_cc twice and qd_commutator. */
+int qd_commutator_cc(qd_context *ctx, const qd_op *r1, const qd_op *r2, qd_op *r3,
+                     qd_cc *hcc, qd_cc *etacc, double *e0);
 /* BCH_EXPAND(HS,G,H,...), src/magnus_IMSRG.f90:253-318: hs = sum_k cof(k)
  * ad_G^k H with the reference's literal coefficients and its 1e-5 (REAL(4))
  * relative truncation; *nterms = nested commutators taken (1..11). */

@@ -269,13 +269,25 @@ def MBPT2(r) -> float:
     return x.value
 
 
-def commutator(r1, r2, r3, HCC: CCMat | None = None, ETACC: CCMat | None = None) -> float:
-    """Full [r1, r2] -> r3 in one pass; CC sets None = zero (Magnus as shipped)."""
+def commutator(r1, r2, r3, HCC: CCMat | None = None, ETACC: CCMat | None = None, e0: bool = True):
+    """Full [r1, r2] -> r3 in one pass; CC sets None = zero (Magnus as shipped).
+    Returns the zero-body part (a device-to-host copy and wait), or None with
+    e0=False, which leaves the call asynchronous."""
     L, h = _L(r1)
-    e0 = C.c_double()
+    v = C.c_double()
     check(L.qd_commutator(h, r1.h, r2.h, r3.h, HCC.h if HCC else None, ETACC.h if ETACC else None,
-                          C.byref(e0)))
-    return e0.value
+                          C.byref(v) if e0 else None))
+    return v.value if e0 else None
+
+
+def commutator_cc(r1, r2, r3, HCC: CCMat, ETACC: CCMat, e0: bool = True):
+    """calc_cc(r2, HCC), calc_cc(r1, ETACC), then the full [r1, r2] -> r3 (the
+    dGam sequence in one call; the CC fills overlap the ladder products).
+    e0 as for commutator()."""
+    L, h = _L(r1)
+    v = C.c_double()
+    check(L.qd_commutator_cc(h, r1.h, r2.h, r3.h, HCC.h, ETACC.h, C.byref(v) if e0 else None))
+    return v.value if e0 else None
 
 
 def BCH_EXPAND(HS, G, H, HCC=None, OCC=None) -> int:

@@ -83,7 +83,7 @@ struct qd_context {
   // side stream (default priority) for the one-body and zero-body pieces of a
   // commutator, forked from and joined back to `stream` with these events
   hipStream_t aux = nullptr;
-  hipEvent_t ev_fork = nullptr, ev_join = nullptr;
+  hipEvent_t ev_fork = nullptr, ev_join = nullptr, ev_cc = nullptr;
   std::vector<void *> allocs;
   double *w1 = nullptr, *w2 = nullptr, *w3 = nullptr, *P = nullptr;
   double *u1 = nullptr, *u2 = nullptr;  // cross-coupled mat-vecs of commutator_121 (M*M each)
@@ -661,11 +661,11 @@ int read_scalars(qd_context *c, int n, double *out) {
   return QD_OK;
 }
 
-int full_commutator(qd_context *c, const qd_op *r1, const qd_op *r2, qd_op *r3, const qd_cc *hcc,
+// Everything of a commutator after the products phase.
+int commutator_rest(qd_context *c, const qd_op *r1, const qd_op *r2, qd_op *r3, const qd_cc *hcc,
                     const qd_cc *ecc) {
   int rc;
   const bool pd = pandya_on(hcc, ecc);
-  if ((rc = run_products(c, r1, r2, r3, hcc, ecc))) return rc;
   // The one-body part (it writes r3's one-body region and reads the products)
   // and the zero-body traces (r3[0]) touch nothing that the two-body pass and
   // the accumulating GEMMs write, so they run on the side stream meanwhile and
@@ -694,6 +694,51 @@ int full_commutator(qd_context *c, const qd_op *r1, const qd_op *r2, qd_op *r3, 
   if ((rc = run_acc(c, r1, r2, r3))) return rc;
   if (side) HIPC(hipStreamWaitEvent(main, c->ev_join, 0));
   return QD_OK;
+}
+
+// fill: FILL_H makes hcc = calc_cc(r2), FILL_E makes ecc = calc_cc(r1) first
+// (the dGam sequence, src/gs_IMSRG.f90:223-273).  In the grouped FP64 form
+// the fills run on the side stream while the ladder products, which do not
+// read the CC sets, run on the main one; the fused Pandya product waits for them.
+enum { FILL_H = 1, FILL_E = 2 };
+int full_commutator(qd_context *c, const qd_op *r1, const qd_op *r2, qd_op *r3, const qd_cc *hcc,
+                    const qd_cc *ecc, int fill = 0) {
+  int rc;
+  const bool side = !c->timing;
+  if (fill) {
+    qd_cc *fh = (fill & FILL_H) ? const_cast<qd_cc *>(hcc) : nullptr;
+    qd_cc *fe = (fill & FILL_E) ? const_cast<qd_cc *>(ecc) : nullptr;
+    const bool overlap = side && c->launch_mode == 0 && c->gemm_prec == GEMM_FP64 && hcc && ecc;
+    hipStream_t main = c->stream;
+    if (overlap) {
+      HIPC(hipEventRecord(c->ev_fork, main));
+      HIPC(hipStreamWaitEvent(c->aux, c->ev_fork, 0));
+      c->stream = c->aux;
+    }
+    {
+      Timer tm(c, TC_CALCCC);
+      if (fh) launch_calc_cc(c->stream, c->dt, r2->d, r2->herm, fh->d, c->cc_segs, c->cc_tiles, c->ncc_tiles);
+      if (fe) launch_calc_cc(c->stream, c->dt, r1->d, r1->herm, fe->d, c->cc_segs, c->cc_tiles, c->ncc_tiles);
+    }
+    c->stream = main;
+    if (hipGetLastError() != hipSuccess) return fail(QD_E_HIP, "calc_cc launch failed");
+    if (fh) fh->zero = false;
+    if (fe) fe->zero = false;
+    if (overlap) {
+      HIPC(hipEventRecord(c->ev_cc, c->aux));
+      Slots s = make_slots(c, r1, r2, r3, ecc, hcc);
+      {
+        Timer tm(c, TC_PRODUCTS);
+        if ((rc = run_gemm_list(c, c->g_all, false, s))) return rc;  // ladder products
+        HIPC(hipStreamWaitEvent(main, c->ev_cc, 0));
+        launch_pandya_fused(c->stream, c->dt, ecc->d, hcc->d, c->P, c->ftp, c->nftp, hcc->herm, ecc->herm);
+        HIPC(hipGetLastError());
+      }
+      return commutator_rest(c, r1, r2, r3, hcc, ecc);
+    }
+  }
+  if ((rc = run_products(c, r1, r2, r3, hcc, ecc))) return rc;
+  return commutator_rest(c, r1, r2, r3, hcc, ecc);
 }
 
 // Pulay (DIIS) weights: minimise |sum_i c_i r_i|^2 subject to sum_i c_i = 1,
@@ -792,7 +837,8 @@ int qd_context_create(int nbody, int nshell, qd_context **out) {
   auto bail = [&](int r) { qd_context_destroy(c); return r; };
   if (hipStreamCreate(&c->stream) != hipSuccess) { delete c; return fail(QD_E_HIP, "hipStreamCreate failed"); }
   if (hipStreamCreate(&c->aux) != hipSuccess || hipEventCreateWithFlags(&c->ev_fork, hipEventDisableTiming) != hipSuccess ||
-      hipEventCreateWithFlags(&c->ev_join, hipEventDisableTiming) != hipSuccess)
+      hipEventCreateWithFlags(&c->ev_join, hipEventDisableTiming) != hipSuccess ||
+      hipEventCreateWithFlags(&c->ev_cc, hipEventDisableTiming) != hipSuccess)
     return bail(fail(QD_E_HIP, "side stream or events could not be created"));
   {
     int dev = 0, ncu = 0;
@@ -857,6 +903,7 @@ int qd_context_destroy(qd_context *c) {
   if (c->host_scal) hipHostFree(c->host_scal);
   if (c->ev_fork) hipEventDestroy(c->ev_fork);
   if (c->ev_join) hipEventDestroy(c->ev_join);
+  if (c->ev_cc) hipEventDestroy(c->ev_cc);
   if (c->aux) hipStreamDestroy(c->aux);
   if (c->stream) hipStreamDestroy(c->stream);
   delete c;
@@ -1118,6 +1165,21 @@ int qd_commutator(qd_context *c, const qd_op *r1, const qd_op *r2, qd_op *r3, co
   return QD_OK;
 }
 
+int qd_commutator_cc(qd_context *c, const qd_op *r1, const qd_op *r2, qd_op *r3, qd_cc *hcc, qd_cc *etacc,
+                     double *e0) {
+  CHK3(r1, r2, r3);
+  if (!hcc || !etacc) return fail(QD_E_ARG, "null CC set");
+  if (r3 == r1 || r3 == r2) return fail(QD_E_ARG, "r3 must not alias r1 or r2");
+  int rc;
+  if ((rc = full_commutator(c, r1, r2, r3, hcc, etacc, FILL_H | FILL_E))) return rc;
+  if (e0) {
+    HIPC(hipMemcpyAsync(c->host_scal, r3->d, sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    HIPC(hipStreamSynchronize(c->stream));
+    *e0 = c->host_scal[0];
+  }
+  return QD_OK;
+}
+
 int qd_bch_expand(qd_context *c, qd_op *hs, const qd_op *g, const qd_op *h, const qd_cc *hcc, const qd_cc *occ,
                   int *nterms) {
   CHK3(g, h, hs);
@@ -1145,13 +1207,8 @@ int qd_bch_expand(qd_context *c, qd_op *hs, const qd_op *g, const qd_op *h, cons
     adh.herm = 1;
     // one term: [G, AD] into next, ||INT1|| (= HS before the update) and ||INT2||
     auto term = [&]() -> int {
-      if (pd) {
-        Timer tc(c, TC_CALCCC);
-        launch_calc_cc(c->stream, c->dt, adh.d, adh.herm, cca->d, c->cc_segs, c->cc_tiles, c->ncc_tiles);
-        HIPC(hipGetLastError());
-      }
-      int r;
-      if ((r = full_commutator(c, g, &adh, next, pd ? cca : nullptr, pd ? ccg : nullptr))) return r;
+      int r;  // with the Pandya term, cca = calc_cc(AD) is filled first
+      if ((r = full_commutator(c, g, &adh, next, pd ? cca : nullptr, pd ? ccg : nullptr, pd ? FILL_H : 0))) return r;
       if ((r = norm_into(c, hs, 0))) return r;
       return norm_into(c, next, 1);
     };
@@ -1209,14 +1266,13 @@ int qd_dgam(qd_context *c, const qd_op *h, qd_op *der, qd_cc *hcc, qd_cc *etacc)
   int rc;
   if ((rc = ensure_scratch(c))) return rc;
   qd_op *eta = c->bch_a;
-  if ((rc = qd_calc_cc(c, h, hcc))) return rc;
   hcc->herm = 1.0;
   if ((rc = qd_set_to_zero(c, eta))) return rc;
   eta->herm = -1;
   if ((rc = qd_build_white(c, eta, h))) return rc;
-  if ((rc = qd_calc_cc(c, eta, etacc))) return rc;
   etacc->herm = -1.0;
-  return full_commutator(c, eta, h, der, hcc, etacc);
+  // calc_cc(H, HCC) and calc_cc(eta, ETACC), then [eta, H]
+  return full_commutator(c, eta, h, der, hcc, etacc, FILL_H | FILL_E);
 }
 
 void qd_flow_params_default(qd_flow_params *p) {

@@ -138,6 +138,18 @@ module qdimsrg_c
        type(c_ptr), value :: ctx, r1, r2, r3, hcc, etacc
        integer(c_int) :: qd_commutator_222
      end function
+     function qd_commutator(ctx, r1, r2, r3, hcc, etacc, e0) bind(C, name="qd_commutator")
+       import :: c_int, c_ptr, c_double
+       type(c_ptr), value :: ctx, r1, r2, r3, hcc, etacc
+       real(c_double), intent(out) :: e0
+       integer(c_int) :: qd_commutator
+     end function
+     function qd_commutator_cc(ctx, r1, r2, r3, hcc, etacc, e0) bind(C, name="qd_commutator_cc")
+       import :: c_int, c_ptr, c_double
+       type(c_ptr), value :: ctx, r1, r2, r3, hcc, etacc
+       real(c_double), intent(out) :: e0
+       integer(c_int) :: qd_commutator_cc
+     end function
      function qd_calc_cc(ctx, h, hcc) bind(C, name="qd_calc_cc")
        import :: c_int, c_ptr
        type(c_ptr), value :: ctx, h, hcc

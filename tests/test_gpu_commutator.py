@@ -118,6 +118,23 @@ def test_full_commutator(case, h1, h2, pandya):
     assert e0 == got[0]
 
 
+@pytest.mark.parametrize("h1,h2", [(-1, 1), (1, 1)])
+def test_commutator_cc_equals_calc_cc_then_commutator(case, h1, h2):
+    # the fused entry overlaps its two calc_cc launches with the ladder
+    # products; the values must be those of the three separate calls
+    x, y, X, Y = case.ops(9, h1, h2)
+    ctx = case.ctx
+    Z1, Z2 = ctx.operator(-h1 * h2), ctx.operator(-h1 * h2)
+    H1, E1 = ctx.cc(float(h2)), ctx.cc(float(h1))
+    H2, E2 = ctx.cc(float(h2)), ctx.cc(float(h1))
+    qd.calc_cc(Y, H1); qd.calc_cc(X, E1)
+    e1 = qd.commutator(X, Y, Z1, H1, E1)
+    e2 = qd.commutator_cc(X, Y, Z2, H2, E2)
+    assert np.array_equal(H1.download(), H2.download()) and np.array_equal(E1.download(), E2.download())
+    assert np.array_equal(Z1.download(), Z2.download()) and e1 == e2
+    assert rel(Z2.download(), case.o.commutator(x, h1, y, h2, pandya=True)) <= TOL
+
+
 def test_dense_one_body(case):
     x, y, X, Y = case.ops(6, -1, 1, dense_f=True)
     Z = case.ctx.operator(1)
