@@ -252,7 +252,8 @@ __global__ void __launch_bounds__(256) k_box2b(DevTab t, Ops o, double *__restri
     acc += val;
     if (flags & TB_LADDER) {
       // transposed / signed combinations of the products of matrix_mult222
-      // (src/commutators.f90:999-1127) that the GEMMs cannot accumulate
+      // (src/commutators.f90:999-1127), and the summed products of w3 for
+      // Vpphh, Vphhh, Vppph
       const int64_t tr = soff[tyr][tyc] + Jl + (int64_t)Il * rows;  // (J,I) of a square sub-block
       switch (sub) {
         case HHHH:
@@ -260,8 +261,9 @@ __global__ void __launch_bounds__(256) k_box2b(DevTab t, Ops o, double *__restri
           acc += h1 * w1[tr] - h2 * w1[idx];
           acc += h1 * h2 * w3[tr] - w3[idx];
           break;
-        case PHHH: acc += h1 * w2[idx] - h2 * w1[idx]; break;
-        case PPPH: acc += -h2 * w1[idx] + h1 * w2[idx]; break;
+        case PPHH: acc += w3[idx]; break;
+        case PHHH: acc += h1 * w2[idx] - h2 * w1[idx] + w3[idx]; break;
+        case PPPH: acc += -h2 * w1[idx] + h1 * w2[idx] + w3[idx]; break;
         case PHPH:
           acc += h1 * w2[tr] - h2 * w2[idx];
           acc += h1 * w1[tr] - h2 * w1[idx];

@@ -83,7 +83,7 @@ struct qd_context {
   // side stream (default priority) for the one-body and zero-body pieces of a
   // commutator, forked from and joined back to `stream` with these events
   hipStream_t aux = nullptr;
-  hipEvent_t ev_fork = nullptr, ev_join = nullptr, ev_cc = nullptr;
+  hipEvent_t ev_fork = nullptr, ev_join = nullptr, ev_cc = nullptr, ev_acc = nullptr;
   std::vector<void *> allocs;
   double *w1 = nullptr, *w2 = nullptr, *w3 = nullptr, *P = nullptr;
   double *u1 = nullptr, *u2 = nullptr;  // cross-coupled mat-vecs of commutator_121 (M*M each)
@@ -460,20 +460,22 @@ int build_worklists(qd_context *c) {
     // w3.hhhh = r1.hhhh r2.hhhh ; w3.pppp = r2.pppp r1.pppp
     g1.begin(S_W3, o[HHHH], nh, nh, nh, 0); g1.seg(S_R1, o[HHHH], nh, 0, S_R2, o[HHHH], nh, 0, nh, 1); g1.end();
     g1.begin(S_W3, o[PPPP], np, np, np, 0); g1.seg(S_R2, o[PPPP], np, 0, S_R1, o[PPPP], np, 0, np, 1); g1.end();
-    // accumulate: Vpphh += r1.pppp r2.pphh - r2.pppp r1.pphh - r1.pphh r2.hhhh + r2.pphh r1.hhhh
-    g2.begin(S_OUT, o[PPHH], np, np, nh, 1);
+    // the products that add to Vpphh, Vphhh, Vppph go to the same sub-blocks of
+    // w3 (free there); the two-body pass adds them to r3
+    // w3.pphh = r1.pppp r2.pphh - r2.pppp r1.pphh - r1.pphh r2.hhhh + r2.pphh r1.hhhh
+    g2.begin(S_W3, o[PPHH], np, np, nh, 0);
     g2.seg(S_R1, o[PPPP], np, 0, S_R2, o[PPHH], np, 0, np, 1);
     g2.seg(S_R2, o[PPPP], np, 0, S_R1, o[PPHH], np, 0, np, -1);
     g2.seg(S_R1, o[PPHH], np, 0, S_R2, o[HHHH], nh, 0, nh, -1);
     g2.seg(S_R2, o[PPHH], np, 0, S_R1, o[HHHH], nh, 0, nh, 1);
     g2.end();
-    // Vphhh += -r1.phhh r2.hhhh + r2.phhh r1.hhhh
-    g2.begin(S_OUT, o[PHHH], nb, nb, nh, 1);
+    // w3.phhh = -r1.phhh r2.hhhh + r2.phhh r1.hhhh
+    g2.begin(S_W3, o[PHHH], nb, nb, nh, 0);
     g2.seg(S_R1, o[PHHH], nb, 0, S_R2, o[HHHH], nh, 0, nh, -1);
     g2.seg(S_R2, o[PHHH], nb, 0, S_R1, o[HHHH], nh, 0, nh, 1);
     g2.end();
-    // Vppph += r1.pppp r2.ppph - r2.pppp r1.ppph
-    g2.begin(S_OUT, o[PPPH], np, np, nb, 1);
+    // w3.ppph = r1.pppp r2.ppph - r2.pppp r1.ppph
+    g2.begin(S_W3, o[PPPH], np, np, nb, 0);
     g2.seg(S_R1, o[PPPP], np, 0, S_R2, o[PPPH], np, 0, np, 1);
     g2.seg(S_R2, o[PPPP], np, 0, S_R1, o[PPPH], np, 0, np, -1);
     g2.end();
@@ -661,22 +663,67 @@ int read_scalars(qd_context *c, int n, double *out) {
   return QD_OK;
 }
 
-// Everything of a commutator after the products phase.
-int commutator_rest(qd_context *c, const qd_op *r1, const qd_op *r2, qd_op *r3, const qd_cc *hcc,
-                    const qd_cc *ecc) {
-  int rc;
-  const bool pd = pandya_on(hcc, ecc);
-  // The one-body part (it writes r3's one-body region and reads the products)
-  // and the zero-body traces (r3[0]) touch nothing that the two-body pass and
-  // the accumulating GEMMs write, so they run on the side stream meanwhile and
-  // are joined before return.  With timing on, everything stays on one stream
-  // so that each timing class is measured alone.
+// fill: FILL_H makes hcc = calc_cc(r2), FILL_E makes ecc = calc_cc(r1) first
+// (the dGam sequence, src/gs_IMSRG.f90:223-273).
+//
+// Schedule.  Main stream: ladder products -> (CC ready) fused Pandya product
+// -> (accumulating products ready) two-body box pass.  Side stream (default
+// priority, forked and joined with events, which graph capture records as
+// edges): the CC fills and the accumulating products (they read only r1, r2
+// and write the free Vpphh/Vphhh/Vppph sub-blocks of w3), then, once the
+// products phase is done, the one-body part (r3's one-body region) and the
+// zero-body traces (r3[0]).  The two-body pass writes only r3's two-body
+// region.  With the timing events on, everything runs in that order on one
+// stream so that each class is timed alone.
+enum { FILL_H = 1, FILL_E = 2 };
+int full_commutator(qd_context *c, const qd_op *r1, const qd_op *r2, qd_op *r3, const qd_cc *hcc,
+                    const qd_cc *ecc, int fill = 0) {
+  int rc = QD_OK;
+  qd_cc *fh = (fill & FILL_H) ? const_cast<qd_cc *>(hcc) : nullptr;
+  qd_cc *fe = (fill & FILL_E) ? const_cast<qd_cc *>(ecc) : nullptr;
   const bool side = !c->timing;
   hipStream_t main = c->stream;
+  auto on = [&](hipStream_t st) { c->stream = st; };
   if (side) {
     HIPC(hipEventRecord(c->ev_fork, main));
     HIPC(hipStreamWaitEvent(c->aux, c->ev_fork, 0));
-    c->stream = c->aux;
+    on(c->aux);
+  }
+  {
+    Timer tm(c, TC_CALCCC);
+    if (fh) launch_calc_cc(c->stream, c->dt, r2->d, r2->herm, fh->d, c->cc_segs, c->cc_tiles, c->ncc_tiles);
+    if (fe) launch_calc_cc(c->stream, c->dt, r1->d, r1->herm, fe->d, c->cc_segs, c->cc_tiles, c->ncc_tiles);
+  }
+  if (hipGetLastError() != hipSuccess) rc = fail(QD_E_HIP, "calc_cc launch failed");
+  if (fh) fh->zero = false;
+  if (fe) fe->zero = false;
+  const bool pd = pandya_on(hcc, ecc);
+  const bool fused = pd && c->launch_mode == 0 && c->gemm_prec == GEMM_FP64;
+  if (side) {
+    HIPC(hipEventRecord(c->ev_cc, c->aux));
+    if (!rc) rc = run_acc(c, r1, r2, r3);
+    HIPC(hipEventRecord(c->ev_acc, c->aux));
+    on(main);
+  }
+  if (rc) return rc;
+  // products phase
+  if (fused) {
+    Slots s = make_slots(c, r1, r2, r3, ecc, hcc);
+    Timer tm(c, TC_PRODUCTS);
+    if ((rc = run_gemm_list(c, c->g_all, false, s))) return rc;  // ladder (and one-body) products
+    if (side) HIPC(hipStreamWaitEvent(main, c->ev_cc, 0));
+    launch_pandya_fused(c->stream, c->dt, ecc->d, hcc->d, c->P, c->ftp, c->nftp, hcc->herm, ecc->herm);
+    HIPC(hipGetLastError());
+  } else {
+    if (side) HIPC(hipStreamWaitEvent(main, c->ev_cc, 0));
+    if ((rc = run_products(c, r1, r2, r3, hcc, ecc))) return rc;
+  }
+  if (!side && (rc = run_acc(c, r1, r2, r3))) return rc;
+  // one-body and zero-body parts
+  if (side) {
+    HIPC(hipEventRecord(c->ev_fork, main));
+    HIPC(hipStreamWaitEvent(c->aux, c->ev_fork, 0));
+    on(c->aux);
   }
   // with the Pandya inputs on hand, the 121 mat-vecs read them (hcc and ecc
   // are calc_cc of r2 and r1 by the entry point's contract)
@@ -687,58 +734,15 @@ int commutator_rest(qd_context *c, const qd_op *r1, const qd_op *r2, qd_op *r3, 
     launch_zerobody(c->stream, c->dt, o, c->pphh_segs, c->npphh_segs, c->partials, r3->d, 3);
     if (hipGetLastError() != hipSuccess) rc = fail(QD_E_HIP, "zero-body launch failed");
   }
-  c->stream = main;
-  if (side) HIPC(hipEventRecord(c->ev_join, c->aux));
+  if (side) {
+    HIPC(hipEventRecord(c->ev_join, c->aux));
+    on(main);
+    HIPC(hipStreamWaitEvent(main, c->ev_acc, 0));
+  }
   if (rc) return rc;
   if ((rc = run_twobody(c, r1, r2, r3, TB_122 | TB_LADDER | (pd ? TB_PANDYA : 0), hcc, ecc))) return rc;
-  if ((rc = run_acc(c, r1, r2, r3))) return rc;
   if (side) HIPC(hipStreamWaitEvent(main, c->ev_join, 0));
   return QD_OK;
-}
-
-// fill: FILL_H makes hcc = calc_cc(r2), FILL_E makes ecc = calc_cc(r1) first
-// (the dGam sequence, src/gs_IMSRG.f90:223-273).  In the grouped FP64 form
-// the fills run on the side stream while the ladder products, which do not
-// read the CC sets, run on the main one; the fused Pandya product waits for them.
-enum { FILL_H = 1, FILL_E = 2 };
-int full_commutator(qd_context *c, const qd_op *r1, const qd_op *r2, qd_op *r3, const qd_cc *hcc,
-                    const qd_cc *ecc, int fill = 0) {
-  int rc;
-  const bool side = !c->timing;
-  if (fill) {
-    qd_cc *fh = (fill & FILL_H) ? const_cast<qd_cc *>(hcc) : nullptr;
-    qd_cc *fe = (fill & FILL_E) ? const_cast<qd_cc *>(ecc) : nullptr;
-    const bool overlap = side && c->launch_mode == 0 && c->gemm_prec == GEMM_FP64 && hcc && ecc;
-    hipStream_t main = c->stream;
-    if (overlap) {
-      HIPC(hipEventRecord(c->ev_fork, main));
-      HIPC(hipStreamWaitEvent(c->aux, c->ev_fork, 0));
-      c->stream = c->aux;
-    }
-    {
-      Timer tm(c, TC_CALCCC);
-      if (fh) launch_calc_cc(c->stream, c->dt, r2->d, r2->herm, fh->d, c->cc_segs, c->cc_tiles, c->ncc_tiles);
-      if (fe) launch_calc_cc(c->stream, c->dt, r1->d, r1->herm, fe->d, c->cc_segs, c->cc_tiles, c->ncc_tiles);
-    }
-    c->stream = main;
-    if (hipGetLastError() != hipSuccess) return fail(QD_E_HIP, "calc_cc launch failed");
-    if (fh) fh->zero = false;
-    if (fe) fe->zero = false;
-    if (overlap) {
-      HIPC(hipEventRecord(c->ev_cc, c->aux));
-      Slots s = make_slots(c, r1, r2, r3, ecc, hcc);
-      {
-        Timer tm(c, TC_PRODUCTS);
-        if ((rc = run_gemm_list(c, c->g_all, false, s))) return rc;  // ladder products
-        HIPC(hipStreamWaitEvent(main, c->ev_cc, 0));
-        launch_pandya_fused(c->stream, c->dt, ecc->d, hcc->d, c->P, c->ftp, c->nftp, hcc->herm, ecc->herm);
-        HIPC(hipGetLastError());
-      }
-      return commutator_rest(c, r1, r2, r3, hcc, ecc);
-    }
-  }
-  if ((rc = run_products(c, r1, r2, r3, hcc, ecc))) return rc;
-  return commutator_rest(c, r1, r2, r3, hcc, ecc);
 }
 
 // Pulay (DIIS) weights: minimise |sum_i c_i r_i|^2 subject to sum_i c_i = 1,
@@ -838,7 +842,8 @@ int qd_context_create(int nbody, int nshell, qd_context **out) {
   if (hipStreamCreate(&c->stream) != hipSuccess) { delete c; return fail(QD_E_HIP, "hipStreamCreate failed"); }
   if (hipStreamCreate(&c->aux) != hipSuccess || hipEventCreateWithFlags(&c->ev_fork, hipEventDisableTiming) != hipSuccess ||
       hipEventCreateWithFlags(&c->ev_join, hipEventDisableTiming) != hipSuccess ||
-      hipEventCreateWithFlags(&c->ev_cc, hipEventDisableTiming) != hipSuccess)
+      hipEventCreateWithFlags(&c->ev_cc, hipEventDisableTiming) != hipSuccess ||
+      hipEventCreateWithFlags(&c->ev_acc, hipEventDisableTiming) != hipSuccess)
     return bail(fail(QD_E_HIP, "side stream or events could not be created"));
   {
     int dev = 0, ncu = 0;
@@ -904,6 +909,7 @@ int qd_context_destroy(qd_context *c) {
   if (c->ev_fork) hipEventDestroy(c->ev_fork);
   if (c->ev_join) hipEventDestroy(c->ev_join);
   if (c->ev_cc) hipEventDestroy(c->ev_cc);
+  if (c->ev_acc) hipEventDestroy(c->ev_acc);
   if (c->aux) hipStreamDestroy(c->aux);
   if (c->stream) hipStreamDestroy(c->stream);
   delete c;
@@ -1091,8 +1097,8 @@ int qd_commutator_222(qd_context *c, const qd_op *r1, const qd_op *r2, qd_op *r3
   int rc;
   const bool pd = pandya_on(hcc, etacc);
   if ((rc = run_products(c, r1, r2, r3, hcc, etacc))) return rc;
-  if ((rc = run_twobody(c, r1, r2, r3, TB_LADDER | TB_ACCUM | (pd ? TB_PANDYA : 0), hcc, etacc))) return rc;
-  return run_acc(c, r1, r2, r3);
+  if ((rc = run_acc(c, r1, r2, r3))) return rc;
+  return run_twobody(c, r1, r2, r3, TB_LADDER | TB_ACCUM | (pd ? TB_PANDYA : 0), hcc, etacc);
 }
 int qd_calc_cc(qd_context *c, const qd_op *h, qd_cc *cc) {
   if (!c || !h || !cc) return fail(QD_E_ARG, "null argument");
