@@ -244,11 +244,12 @@ __global__ void __launch_bounds__(256) k_box2b(DevTab t, Ops o, double *__restri
     else if (tyc == 1) sub = tyr == 1 ? PHPH : PPPH;
     else sub = PPPP;
     const int64_t idx = soff[tyr][tyc] + Il + (int64_t)Jl * rows;
-    if (!first) {
-      out[idx] += val;
+    if (!first) {  // ket slabs run before the main pass: the first writer of their outputs
+      out[idx] = (flags & TB_ACCUM) ? out[idx] + val : val;
       continue;
     }
-    double acc = (flags & TB_ACCUM) ? out[idx] : 0.0;
+    // the bra slabs of a split box add to what the ket slabs wrote
+    double acc = ((flags & TB_ACCUM) || (mode == 1 && do122)) ? out[idx] : 0.0;
     acc += val;
     if (flags & TB_LADDER) {
       // transposed / signed combinations of the products of matrix_mult222

@@ -624,18 +624,23 @@ int run_acc(qd_context *c, const qd_op *r1, const qd_op *r2, qd_op *out) {
   return run_gemm_list(c, c->g_acc, false, s);
 }
 
+// The two-body pass in two parts: TB_PART_PRE = the channel blocks of f and
+// the ket-side slabs of the boxes too large for one task (they read only the
+// operands, so they may run early), TB_PART_MAIN = every stored output (1b x
+// 2b, ladder and Pandya terms), adding to what the slabs wrote.
+enum { TB_PART_PRE = 1, TB_PART_MAIN = 2 };
 int run_twobody(qd_context *c, const qd_op *r1, const qd_op *r2, qd_op *out, int flags, const qd_cc *hcc,
-                const qd_cc *ecc) {
+                const qd_cc *ecc, int part = TB_PART_PRE | TB_PART_MAIN) {
   Ops o{r1->d, r2->d, r1->herm, r2->herm};
   Timer tm(c, TC_TWOBODY);
-  if (flags & TB_122) launch_fclass(c->stream, c->dt, o, c->fcls);
-  // one pass writes every stored output (1b x 2b, ladder and Pandya terms);
-  // a second adds the ket-side 1b x 2b terms of the boxes split into slabs
-  launch_box2b(c->stream, c->dt, o, out->d, c->bt1, c->nbt1, c->blist, flags, c->w1, c->w2, c->w3, c->fcls,
-               (flags & TB_PANDYA) ? c->P : nullptr);
-  if (flags & TB_122)
+  if ((part & TB_PART_PRE) && (flags & TB_122)) {
+    launch_fclass(c->stream, c->dt, o, c->fcls);
     launch_box2b(c->stream, c->dt, o, out->d, c->bt2, c->nbt2, c->blist, flags, c->w1, c->w2, c->w3, c->fcls,
                  nullptr);
+  }
+  if (part & TB_PART_MAIN)
+    launch_box2b(c->stream, c->dt, o, out->d, c->bt1, c->nbt1, c->blist, flags, c->w1, c->w2, c->w3, c->fcls,
+                 (flags & TB_PANDYA) ? c->P : nullptr);
   HIPC(hipGetLastError());
   return QD_OK;
 }
@@ -667,12 +672,12 @@ int read_scalars(qd_context *c, int n, double *out) {
 // (the dGam sequence, src/gs_IMSRG.f90:223-273).
 //
 // Schedule.  Main stream: ladder products -> (CC ready) fused Pandya product
-// -> (accumulating products ready) two-body box pass.  Side stream (default
-// priority, forked and joined with events, which graph capture records as
-// edges): the CC fills and the accumulating products (they read only r1, r2
-// and write the free Vpphh/Vphhh/Vppph sub-blocks of w3), then, once the
-// products phase is done, the one-body part (r3's one-body region) and the
-// zero-body traces (r3[0]).  The two-body pass writes only r3's two-body
+// -> (side work ready) two-body box pass.  Side stream (default priority,
+// forked and joined with events, which graph capture records as edges): the
+// CC fills, the accumulating products (they read only r1, r2 and write the
+// free Vpphh/Vphhh/Vppph sub-blocks of w3) and the ket-side slabs of the
+// two-body pass (operands only), then, once the products phase is done, the
+// one-body part (r3's one-body region) and the zero-body traces (r3[0]).  The two-body pass writes only r3's two-body
 // region.  With the timing events on, everything runs in that order on one
 // stream so that each class is timed alone.
 enum { FILL_H = 1, FILL_E = 2 };
@@ -699,9 +704,11 @@ int full_commutator(qd_context *c, const qd_op *r1, const qd_op *r2, qd_op *r3, 
   if (fe) fe->zero = false;
   const bool pd = pandya_on(hcc, ecc);
   const bool fused = pd && c->launch_mode == 0 && c->gemm_prec == GEMM_FP64;
+  const int tbf = TB_122 | TB_LADDER | (pd ? TB_PANDYA : 0);
   if (side) {
     HIPC(hipEventRecord(c->ev_cc, c->aux));
     if (!rc) rc = run_acc(c, r1, r2, r3);
+    if (!rc) rc = run_twobody(c, r1, r2, r3, tbf, hcc, ecc, TB_PART_PRE);
     HIPC(hipEventRecord(c->ev_acc, c->aux));
     on(main);
   }
@@ -740,7 +747,7 @@ int full_commutator(qd_context *c, const qd_op *r1, const qd_op *r2, qd_op *r3, 
     HIPC(hipStreamWaitEvent(main, c->ev_acc, 0));
   }
   if (rc) return rc;
-  if ((rc = run_twobody(c, r1, r2, r3, TB_122 | TB_LADDER | (pd ? TB_PANDYA : 0), hcc, ecc))) return rc;
+  if ((rc = run_twobody(c, r1, r2, r3, tbf, hcc, ecc, side ? TB_PART_MAIN : TB_PART_PRE | TB_PART_MAIN))) return rc;
   if (side) HIPC(hipStreamWaitEvent(main, c->ev_join, 0));
   return QD_OK;
 }

@@ -91,8 +91,9 @@ void launch_gemm(hipStream_t s, const GemmProb *probs, const GemmSeg *segs, cons
 // whose bra lies in one class pair (rows) and whose ket in another (columns).
 // mode 0: whole boxes, the class pairs blist[first .. first+ng-1] against
 // column pair gcol; mode 1: one box, columns lo .. lo+cnt-1 of its grid, with
-// the bra-side 1b x 2b terms (plus ladder and Pandya terms); mode 2: one box,
-// rows lo .. lo+cnt-1, adding the ket-side 1b x 2b terms (second launch).
+// the bra-side 1b x 2b terms (plus ladder and Pandya terms), added to what the
+// mode-2 tasks wrote; mode 2: one box, rows lo .. lo+cnt-1, writing the
+// ket-side 1b x 2b terms (their own launch, before the others).
 struct BTask { int block, gcol, first, ng, mode, lo, cnt; };
 // LDS per work-group stays under 32 KB (five per CU); channels of up to 10
 // states (R = 20 shells) fit the channel-block area
