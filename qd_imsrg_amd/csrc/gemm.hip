@@ -158,6 +158,112 @@ __device__ __forceinline__ void gemm_tile(const GemmProb &pr, const GemmSeg *__r
     }
 }
 
+// 64 x 16 tile with split K: the chunk's 8 k-steps of 4 go to the four waves
+// two each (wave w takes k-steps w and w + 4), every wave accumulating the whole
+// 64 x 16 tile (four 16 x 16 MFMA blocks); the partial tiles are added through
+// LDS in wave order.  For outputs with N <= 16 this keeps four MFMAs per
+// k-step per wave without the padding of a wider tile.
+__device__ __forceinline__ void gemm_tile_splitk(const GemmProb &pr, const GemmSeg *__restrict__ segs,
+                                                 const Slots &slots, int tm, int tn, double *__restrict__ lds) {
+  constexpr int TM = 64, TN = 16;
+  double *As = lds;                       // [GEMM_TK][TM + 1]
+  double *Bs = lds + GEMM_TK * (TM + 1);  // [GEMM_TK][TN + 1]
+  const int row0 = tm * TM, col0 = tn * TN;
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int lr = lane & 15, lk = lane >> 4;
+  d4 acc[4];
+#pragma unroll
+  for (int x = 0; x < 4; x++) acc[x] = d4{0, 0, 0, 0};
+  constexpr int NA = (TM * GEMM_TK + 255) / 256, NB = (TN * GEMM_TK + 255) / 256;
+  double ra[NA], rb[NB];
+  auto load = [&](const GemmSeg &sg, int k0) {
+    const double *__restrict__ A = slots.p[sg.a_slot] + sg.a_off;
+    const double *__restrict__ B = slots.p[sg.b_slot] + sg.b_off;
+#pragma unroll
+    for (int r = 0; r < NA; r++) {
+      const int idx = tid + 256 * r;
+      int i, k;
+      if (!sg.ta) { i = idx % TM; k = idx / TM; }
+      else { k = idx % GEMM_TK; i = idx / GEMM_TK; }
+      const int gi = row0 + i, gk = k0 + k;
+      double v = 0.0;
+      if (idx < TM * GEMM_TK && gi < pr.M && gk < sg.K)
+        v = sg.ta ? A[gk + (int64_t)gi * sg.lda] : A[gi + (int64_t)gk * sg.lda];
+      ra[r] = sg.alpha * v;
+    }
+#pragma unroll
+    for (int r = 0; r < NB; r++) {
+      const int idx = tid + 256 * r;
+      int j, k;
+      if (!sg.tb) { k = idx % GEMM_TK; j = idx / GEMM_TK; }
+      else { j = idx % TN; k = idx / TN; }
+      const int gj = col0 + j, gk = k0 + k;
+      double v = 0.0;
+      if (idx < TN * GEMM_TK && gj < pr.N && gk < sg.K)
+        v = sg.tb ? B[gj + (int64_t)gk * sg.ldb] : B[gk + (int64_t)gj * sg.ldb];
+      rb[r] = v;
+    }
+  };
+  auto store = [&](int ta, int tb) {
+#pragma unroll
+    for (int r = 0; r < NA; r++) {
+      const int idx = tid + 256 * r;
+      if (idx < TM * GEMM_TK) {
+        const int i = ta ? idx / GEMM_TK : idx % TM, k = ta ? idx % GEMM_TK : idx / TM;
+        As[k * (TM + 1) + i] = ra[r];
+      }
+    }
+#pragma unroll
+    for (int r = 0; r < NB; r++) {
+      const int idx = tid + 256 * r;
+      if (idx < TN * GEMM_TK) {
+        const int j = tb ? idx % TN : idx / GEMM_TK, k = tb ? idx / TN : idx % GEMM_TK;
+        Bs[k * (TN + 1) + j] = rb[r];
+      }
+    }
+  };
+  auto compute = [&](int kv) {
+#pragma unroll
+    for (int h = 0; h < 2; h++) {
+      const int kk = 4 * (wave + 4 * h);
+      if (kk >= kv) break;  // zero fill past the segment's K adds nothing
+      const double b = Bs[(kk + lk) * (TN + 1) + lr];
+#pragma unroll
+      for (int x = 0; x < 4; x++) {
+        const double a = As[(kk + lk) * (TM + 1) + x * 16 + lr];
+        acc[x] = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, acc[x], 0, 0, 0);
+      }
+    }
+  };
+  chunk_walk<GEMM_TK>(pr, segs, load, store, compute);
+  // partial tiles of waves 1..3 through LDS (16 doubles per lane), added by wave 0
+  __syncthreads();
+  if (wave > 0) {
+#pragma unroll
+    for (int x = 0; x < 4; x++)
+#pragma unroll
+      for (int v = 0; v < 4; v++) lds[((wave - 1) * 16 + x * 4 + v) * 64 + lane] = acc[x][v];
+  }
+  __syncthreads();
+  if (wave == 0) {
+    double *__restrict__ C = slots.p[pr.c_slot] + pr.c_off;
+    const int c = col0 + lr;
+#pragma unroll
+    for (int x = 0; x < 4; x++)
+#pragma unroll
+      for (int v = 0; v < 4; v++) {
+        double sum = acc[x][v];
+#pragma unroll
+        for (int w = 0; w < 3; w++) sum += lds[(w * 16 + x * 4 + v) * 64 + lane];
+        const int r = row0 + x * 16 + 4 * v + lk;
+        if (r < pr.M && c < pr.N) {
+          double *p = C + r + (int64_t)c * pr.ldc;
+          *p = pr.beta1 ? *p + sum : sum;
+        }
+      }
+  }
+}
+
 // ---- split-precision tiles: FP32, or the mantissa cut into S BF16 slices
 //
 // x = x_0 + x_1 + ... + x_{S-1} + r, each x_s a BF16 number taken by
@@ -402,11 +508,15 @@ __global__ void __launch_bounds__(256) k_gemm(const GemmProb *__restrict__ probs
                                               const GemmTile *__restrict__ tiles, int ntiles, Slots slots) {
   constexpr int TM = SHAPE_TM[SH], TN = SHAPE_TN[SH];
   constexpr int WM = TN <= 16 ? 4 : (TM <= 16 ? 1 : 2), WN = 4 / WM;
-  __shared__ __align__(16) unsigned char lds[gemm_lds_bytes(TM, TN, P)];
+  constexpr int LB = (SH == SH_64x16K && P == GEMM_FP64)
+                         ? (gemm_lds_bytes(TM, TN, P) > 3 * 16 * 64 * 8 ? gemm_lds_bytes(TM, TN, P) : 3 * 16 * 64 * 8)
+                         : gemm_lds_bytes(TM, TN, P);
+  __shared__ __align__(16) unsigned char lds[LB];
   for (int w = blockIdx.x; w < ntiles; w += gridDim.x) {
     const GemmTile tl = tiles[w];
     const GemmProb pr = probs[tl.prob];
-    if constexpr (P == GEMM_FP64) gemm_tile<TM, TN, WM, WN>(pr, segs, slots, tl.tm, tl.tn, (double *)lds);
+    if constexpr (SH == SH_64x16K && P == GEMM_FP64) gemm_tile_splitk(pr, segs, slots, tl.tm, tl.tn, (double *)lds);
+    else if constexpr (P == GEMM_FP64) gemm_tile<TM, TN, WM, WN>(pr, segs, slots, tl.tm, tl.tn, (double *)lds);
     else if constexpr (P == GEMM_FP32) gemm_tile_f32<TM, TN, WM, WN>(pr, segs, slots, tl.tm, tl.tn, (float *)lds);
     else gemm_tile_bf16<TM, TN, WM, WN, P>(pr, segs, slots, tl.tm, tl.tn, (unsigned *)lds);
     __syncthreads();  // the next tile reuses the LDS staging area
@@ -424,6 +534,7 @@ static void launch_gemm_p(hipStream_t s, const GemmProb *probs, const GemmSeg *s
     case SH_64x16: hipLaunchKernelGGL((k_gemm<SH_64x16, P>), dim3(grid), dim3(256), 0, s, probs, segs, tiles, ntiles, slots); break;
     case SH_16x64: hipLaunchKernelGGL((k_gemm<SH_16x64, P>), dim3(grid), dim3(256), 0, s, probs, segs, tiles, ntiles, slots); break;
     case SH_16x128: hipLaunchKernelGGL((k_gemm<SH_16x128, P>), dim3(grid), dim3(256), 0, s, probs, segs, tiles, ntiles, slots); break;
+    case SH_64x16K: hipLaunchKernelGGL((k_gemm<SH_64x16K, P>), dim3(grid), dim3(256), 0, s, probs, segs, tiles, ntiles, slots); break;
     default: break;
   }
 }
