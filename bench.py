@@ -393,9 +393,11 @@ def main():
 
     def step():
         # calc_cc(Y, HCC), calc_cc(X, ECC), then [X, Y] with them (one call,
-        # asynchronous: E0 stays on the device)
+        # asynchronous: E0 stays on the device; with graphs on, a replay of the
+        # call captured in the first warm-up step)
         qd.commutator_cc(X, Y, Z, HCC, ECC, e0=False)
 
+    ctx.set_graphs(True)
     for _ in range(a.warmup):
         step()
     ctx.sync()

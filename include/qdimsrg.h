@@ -255,7 +255,8 @@ int qd_gs_flow(qd_context *ctx, const qd_op *h0, qd_op *hs, const qd_gs_params *
 int qd_set_launch_mode(qd_context *ctx, int mode);
 /* 1: qd_bch_expand captures each nested-commutator term (calc_cc of the
  * term, the fused commutator, both norms) in a hipGraph on first use and
- * replays it; graphs are keyed by the operand addresses.  0 (default): plain
+ * replays it, and qd_commutator_cc does the same for the whole call; graphs
+ * are keyed by the operand addresses and hermiticities.  0 (default): plain
  * stream launches.  No counterpart in the reference. */
 int qd_set_graphs(qd_context *ctx, int on);
 /* Project autotuner: coordinate descent over the (M, N, K) size buckets of

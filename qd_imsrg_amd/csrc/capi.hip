@@ -1184,7 +1184,29 @@ int qd_commutator_cc(qd_context *c, const qd_op *r1, const qd_op *r2, qd_op *r3,
   if (!hcc || !etacc) return fail(QD_E_ARG, "null CC set");
   if (r3 == r1 || r3 == r2) return fail(QD_E_ARG, "r3 must not alias r1 or r2");
   int rc;
-  if ((rc = full_commutator(c, r1, r2, r3, hcc, etacc, FILL_H | FILL_E))) return rc;
+  if (c->use_graphs && !c->timing) {
+    // the whole call (both streams) as one graph per operand set; hs = null
+    // and the herm code above 100 keep these keys apart from the BCH terms'
+    qd_context::GraphKey key{r1->d, r2->d, r3->d, nullptr, hcc->d, etacc->d,
+                             100 + 3 * (r1->herm + 1) + (r2->herm + 1), 2, hcc->herm, etacc->herm};
+    auto it = c->graphs.find(key);
+    if (it == c->graphs.end()) {
+      hipGraph_t graph;
+      HIPC(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+      rc = full_commutator(c, r1, r2, r3, hcc, etacc, FILL_H | FILL_E);
+      hipError_t ec = hipStreamEndCapture(c->stream, &graph);
+      if (rc) return rc;
+      if (ec != hipSuccess) return fail(QD_E_HIP, "hipStreamEndCapture: %s", hipGetErrorString(ec));
+      hipGraphExec_t exec;
+      HIPC(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0));
+      HIPC(hipGraphDestroy(graph));
+      it = c->graphs.emplace(key, exec).first;
+    }
+    HIPC(hipGraphLaunch(it->second, c->stream));
+    hcc->zero = etacc->zero = false;
+  } else if ((rc = full_commutator(c, r1, r2, r3, hcc, etacc, FILL_H | FILL_E))) {
+    return rc;
+  }
   if (e0) {
     HIPC(hipMemcpyAsync(c->host_scal, r3->d, sizeof(double), hipMemcpyDeviceToHost, c->stream));
     HIPC(hipStreamSynchronize(c->stream));

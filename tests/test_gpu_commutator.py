@@ -135,6 +135,28 @@ def test_commutator_cc_equals_calc_cc_then_commutator(case, h1, h2):
     assert rel(Z2.download(), case.o.commutator(x, h1, y, h2, pandya=True)) <= TOL
 
 
+def test_commutator_cc_graph_replays(case):
+    # with graphs on, the call is captured once and replayed; every replay,
+    # with eager work between them, equals the eager result bit for bit
+    x, y, X, Y = case.ops(10, -1, 1)
+    ctx = case.ctx
+    Z0, Z = ctx.operator(1), ctx.operator(1)
+    H0, E0 = ctx.cc(1.0), ctx.cc(-1.0)
+    H, E = ctx.cc(1.0), ctx.cc(-1.0)
+    ref = qd.commutator_cc(X, Y, Z0, H0, E0)
+    ctx.set_graphs(True)
+    try:
+        for rep in range(4):
+            W = ctx.operator(1)
+            qd.commutator(Y, X, W)  # eager work between the replays
+            e = qd.commutator_cc(X, Y, Z, H, E)
+            assert e == ref and np.array_equal(Z.download(), Z0.download()), rep
+            assert np.array_equal(H.download(), H0.download()) and np.array_equal(E.download(), E0.download())
+            Z.upload(np.zeros(case.ch.neq))
+    finally:
+        ctx.set_graphs(False)
+
+
 def test_dense_one_body(case):
     x, y, X, Y = case.ops(6, -1, 1, dense_f=True)
     Z = case.ctx.operator(1)
