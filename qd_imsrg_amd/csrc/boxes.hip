@@ -195,6 +195,42 @@ __global__ void __launch_bounds__(256) k_box2b(DevTab t, Ops o, double *__restri
     if (tyr < tyc || (S.z && sr) || (same34 && sc)) continue;
     const int s1 = S.x, s2 = S.y;
     const int i1 = qdiv(rr, brc[b].y), i2 = rr - i1 * s2, i3 = qdiv(cc, rs4), i4 = cc - i3 * s4;
+    // stored position (sub-block of the flat layout)
+    const int Il = (int)(er & 0xffffu) - tstart[tyr], Jl = (int)(ec & 0xffffu) - tstart[tyc];
+    const int rows = tcount[tyr];
+    int sub;
+    if (tyc == 0) sub = tyr == 0 ? HHHH : (tyr == 1 ? PHHH : PPHH);
+    else if (tyc == 1) sub = tyr == 1 ? PHPH : PPPH;
+    else sub = PPPP;
+    const int64_t idx = soff[tyr][tyc] + Il + (int64_t)Jl * rows;
+    // global loads first, so that their latency overlaps the LDS contraction
+    double d0 = 0.0, d1 = 0.0, d2 = 0.0, d3 = 0.0;
+    if (dopd) {
+      const int64_t b0 = pb[b][0], b1 = pb[b][1], b2 = pb[b][2], b3 = pb[b][3];
+      if (b0 >= 0) d0 = D[b0 + i3 * s1 + i1 + (int64_t)(i2 * s4 + i4) * pl[b][0]];
+      if (b1 >= 0) d1 = D[b1 + i3 * s2 + i2 + (int64_t)(i1 * s4 + i4) * pl[b][1]];
+      if (b2 >= 0) d2 = D[b2 + i4 * s2 + i2 + (int64_t)(i1 * s3 + i3) * pl[b][2]];
+      if (b3 >= 0) d3 = D[b3 + i4 * s1 + i1 + (int64_t)(i2 * s3 + i3) * pl[b][3]];
+    }
+    // ladder inputs (transposed / signed combinations of the products of
+    // matrix_mult222, src/commutators.f90:999-1127, and the summed products
+    // of w3 for Vpphh, Vphhh, Vppph): xa = w1(I,J), xb = w3 or w2 (PHPH) at
+    // (I,J), xc = w2(I,J) for PHHH / PPPH, xt1 = w1(J,I), xt2 = w3 or w2 (PHPH)
+    // at (J,I) of a square sub-block
+    const bool lad = first && (flags & TB_LADDER);
+    const bool sq = sub == HHHH || sub == PPPP || sub == PHPH;
+    const int64_t tr = soff[tyr][tyc] + Jl + (int64_t)Il * rows;
+    const double *wb = sub == PHPH ? w2 : w3;
+    double xa = 0.0, xb = 0.0, xc = 0.0, xt1 = 0.0, xt2 = 0.0;
+    if (lad) {
+      if (sub != PPHH) xa = w1[idx];
+      xb = wb[idx];
+      if (sub == PHHH || sub == PPPH) xc = w2[idx];
+      if (sq) { xt1 = w1[tr]; xt2 = wb[tr]; }
+    }
+    // the bra slabs of a split box add to what the ket slabs wrote
+    const bool rd = first ? ((flags & TB_ACCUM) || (mode == 1 && do122)) : (flags & TB_ACCUM) != 0;
+    const double o0 = rd ? out[idx] : 0.0;
     double val = 0.0;
     if (do122) {
       const double *G1 = g1 + B.z, *G2 = g2 + B.z;
@@ -227,47 +263,27 @@ __global__ void __launch_bounds__(256) k_box2b(DevTab t, Ops o, double *__restri
         val += h2 * k1 - h1 * k2;
       }
     }
-    if (dopd) {
-      const int64_t b0 = pb[b][0], b1 = pb[b][1], b2 = pb[b][2], b3 = pb[b][3];
-      const double d0 = b0 < 0 ? 0.0 : D[b0 + i3 * s1 + i1 + (int64_t)(i2 * s4 + i4) * pl[b][0]];
-      const double d1 = b1 < 0 ? 0.0 : D[b1 + i3 * s2 + i2 + (int64_t)(i1 * s4 + i4) * pl[b][1]];
-      const double d2 = b2 < 0 ? 0.0 : D[b2 + i4 * s2 + i2 + (int64_t)(i1 * s3 + i3) * pl[b][2]];
-      const double d3 = b3 < 0 ? 0.0 : D[b3 + i4 * s1 + i1 + (int64_t)(i2 * s3 + i3) * pl[b][3]];
-      val += d0 - d1 + d2 - d3;
-    }
+    if (dopd) val += d0 - d1 + d2 - d3;
     if (sr != sc) val = -val;
-    // stored position (sub-block of the flat layout)
-    const int Il = (int)(er & 0xffffu) - tstart[tyr], Jl = (int)(ec & 0xffffu) - tstart[tyc];
-    const int rows = tcount[tyr];
-    int sub;
-    if (tyc == 0) sub = tyr == 0 ? HHHH : (tyr == 1 ? PHHH : PPHH);
-    else if (tyc == 1) sub = tyr == 1 ? PHPH : PPPH;
-    else sub = PPPP;
-    const int64_t idx = soff[tyr][tyc] + Il + (int64_t)Jl * rows;
     if (!first) {  // ket slabs run before the main pass: the first writer of their outputs
-      out[idx] = (flags & TB_ACCUM) ? out[idx] + val : val;
+      out[idx] = o0 + val;
       continue;
     }
-    // the bra slabs of a split box add to what the ket slabs wrote
-    double acc = ((flags & TB_ACCUM) || (mode == 1 && do122)) ? out[idx] : 0.0;
+    double acc = o0;
     acc += val;
-    if (flags & TB_LADDER) {
-      // transposed / signed combinations of the products of matrix_mult222
-      // (src/commutators.f90:999-1127), and the summed products of w3 for
-      // Vpphh, Vphhh, Vppph
-      const int64_t tr = soff[tyr][tyc] + Jl + (int64_t)Il * rows;  // (J,I) of a square sub-block
+    if (lad) {
       switch (sub) {
         case HHHH:
         case PPPP:
-          acc += h1 * w1[tr] - h2 * w1[idx];
-          acc += h1 * h2 * w3[tr] - w3[idx];
+          acc += h1 * xt1 - h2 * xa;
+          acc += h1 * h2 * xt2 - xb;
           break;
-        case PPHH: acc += w3[idx]; break;
-        case PHHH: acc += h1 * w2[idx] - h2 * w1[idx] + w3[idx]; break;
-        case PPPH: acc += -h2 * w1[idx] + h1 * w2[idx] + w3[idx]; break;
+        case PPHH: acc += xb; break;
+        case PHHH: acc += h1 * xc - h2 * xa + xb; break;
+        case PPPH: acc += -h2 * xa + h1 * xc + xb; break;
         case PHPH:
-          acc += h1 * w2[tr] - h2 * w2[idx];
-          acc += h1 * w1[tr] - h2 * w1[idx];
+          acc += h1 * xt2 - h2 * xb;
+          acc += h1 * xt1 - h2 * xa;
           break;
         default: break;
       }
