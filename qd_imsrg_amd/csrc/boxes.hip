@@ -57,8 +57,8 @@ __global__ void __launch_bounds__(256) k_box2b(DevTab t, Ops o, double *__restri
                                                int flags, const double *__restrict__ w1,
                                                const double *__restrict__ w2, const double *__restrict__ w3,
                                                const double *__restrict__ F, const double *__restrict__ D) {
-  __shared__ double g1[BX_VCAP], g2[BX_VCAP];  // ordered grids of V1, V2: [row][ldc]
-  __shared__ double fl[BX_FCAP];               // channel blocks: f1 then f2, A3, A4, then A1, A2 per box
+  __shared__ double2 gg[BX_VCAP];              // ordered grids of (V1, V2): [row][ldc]
+  __shared__ double2 fl[BX_FCAP / 2];          // channel blocks of (f1, f2): A3, A4, then A1, A2 per box
   __shared__ int pre[BX_GMAX + 1];             // element offsets of the boxes
   __shared__ int4 bx[BX_GMAX];                 // class pair of the rows, rows in the task, grid offset, F offset
   __shared__ int4 bs[BX_GMAX];                 // |A1|, |A2|, A1 == A2, first row entry in rent
@@ -87,7 +87,7 @@ __global__ void __launch_bounds__(256) k_box2b(DevTab t, Ops o, double *__restri
   }
   for (int k = threadIdx.x; k < nc; k += 256) cent[k] = cgrid[k];
   if (threadIdx.x == 0) {
-    int acc = 0, goff = 0, foff = 2 * (s3 * s3 + s4 * s4), roff = 0;
+    int acc = 0, goff = 0, foff = s3 * s3 + s4 * s4, roff = 0;
     for (int b = 0; b < ng; b++) {
       const int g = blist[tk.first + b];
       const int A1 = t.cp_a[g], A2 = t.cp_b[g];
@@ -100,7 +100,7 @@ __global__ void __launch_bounds__(256) k_box2b(DevTab t, Ops o, double *__restri
       pre[b] = acc;
       acc += nr * nc;
       goff += nr * ldc;
-      foff += 2 * (s1 * s1 + s2 * s2);
+      foff += s1 * s1 + s2 * s2;
     }
     pre[ng] = acc;
   }
@@ -131,26 +131,18 @@ __global__ void __launch_bounds__(256) k_box2b(DevTab t, Ops o, double *__restri
   if (do122) {
     // channel blocks of f1 and f2
     const int fs = t.fcls_size;
-    for (int k = threadIdx.x; k < s3 * s3; k += 256) {
-      fl[k] = F[t.fcls_off[A3] + k];
-      fl[s3 * s3 + k] = F[fs + t.fcls_off[A3] + k];
-    }
-    for (int k = threadIdx.x; k < s4 * s4; k += 256) {
-      fl[2 * s3 * s3 + k] = F[t.fcls_off[A4] + k];
-      fl[2 * s3 * s3 + s4 * s4 + k] = F[fs + t.fcls_off[A4] + k];
-    }
+    for (int k = threadIdx.x; k < s3 * s3; k += 256)
+      fl[k] = make_double2(F[t.fcls_off[A3] + k], F[fs + t.fcls_off[A3] + k]);
+    for (int k = threadIdx.x; k < s4 * s4; k += 256)
+      fl[s3 * s3 + k] = make_double2(F[t.fcls_off[A4] + k], F[fs + t.fcls_off[A4] + k]);
     for (int b = 0; b < ng; b++) {
       const int4 B = bx[b];
       const int A1 = t.cp_a[B.x], A2 = t.cp_b[B.x];
       const int s1 = t.ch_start[A1 + 1] - t.ch_start[A1], s2 = t.ch_start[A2 + 1] - t.ch_start[A2];
-      for (int k = threadIdx.x; k < s1 * s1; k += 256) {
-        fl[B.w + k] = F[t.fcls_off[A1] + k];
-        fl[B.w + s1 * s1 + k] = F[fs + t.fcls_off[A1] + k];
-      }
-      for (int k = threadIdx.x; k < s2 * s2; k += 256) {
-        fl[B.w + 2 * s1 * s1 + k] = F[t.fcls_off[A2] + k];
-        fl[B.w + 2 * s1 * s1 + s2 * s2 + k] = F[fs + t.fcls_off[A2] + k];
-      }
+      for (int k = threadIdx.x; k < s1 * s1; k += 256)
+        fl[B.w + k] = make_double2(F[t.fcls_off[A1] + k], F[fs + t.fcls_off[A1] + k]);
+      for (int k = threadIdx.x; k < s2 * s2; k += 256)
+        fl[B.w + s1 * s1 + k] = make_double2(F[t.fcls_off[A2] + k], F[fs + t.fcls_off[A2] + k]);
     }
     // ordered grids: V(x1,x2,x3,x4) = sign * (herm if stored transposed) *
     // stored element of the canonical pairs, 0 for x1 == x2 or x3 == x4
@@ -173,8 +165,7 @@ __global__ void __launch_bounds__(256) k_box2b(DevTab t, Ops o, double *__restri
         v2 = tr ? o.h2 * o.v2[a] : o.v2[a];
         if (neg) { v1 = -v1; v2 = -v2; }
       }
-      g1[B.z + r * ldc + c] = v1;
-      g2[B.z + r * ldc + c] = v2;
+      gg[B.z + r * ldc + c] = make_double2(v1, v2);
     }
     __syncthreads();
   }
@@ -233,32 +224,32 @@ __global__ void __launch_bounds__(256) k_box2b(DevTab t, Ops o, double *__restri
     const double o0 = rd ? out[idx] : 0.0;
     double val = 0.0;
     if (do122) {
-      const double *G1 = g1 + B.z, *G2 = g2 + B.z;
+      const double2 *G = gg + B.z;
       if (first) {
-        const double *fa = fl + B.w, *fb = fl + B.w + 2 * s1 * s1;
+        const double2 *fa = fl + B.w, *fb = fl + B.w + s1 * s1;
         double acc = 0.0;
         for (int k = 0; k < s1; k++) {
-          const int gi = (k * s2 + i2) * ldc + c;
-          acc += fa[i1 * s1 + k] * G2[gi] - fa[s1 * s1 + i1 * s1 + k] * G1[gi];
+          const double2 f = fa[i1 * s1 + k], g = G[(k * s2 + i2) * ldc + c];
+          acc += f.x * g.y - f.y * g.x;
         }
         for (int k = 0; k < s2; k++) {
-          const int gi = (i1 * s2 + k) * ldc + c;
-          acc += fb[i2 * s2 + k] * G2[gi] - fb[s2 * s2 + i2 * s2 + k] * G1[gi];
+          const double2 f = fb[i2 * s2 + k], g = G[(i1 * s2 + k) * ldc + c];
+          acc += f.x * g.y - f.y * g.x;
         }
         val += acc;
       }
       if (mode != 1) {
-        const double *fc = fl, *fd = fl + 2 * s3 * s3;
+        const double2 *fc = fl, *fd = fl + s3 * s3;
         double k2 = 0.0, k1 = 0.0;
         for (int k = 0; k < s3; k++) {
-          const int gi = r * ldc + k * s4 + i4;
-          k2 += fc[i3 * s3 + k] * G2[gi];
-          k1 += fc[s3 * s3 + i3 * s3 + k] * G1[gi];
+          const double2 f = fc[i3 * s3 + k], g = G[r * ldc + k * s4 + i4];
+          k2 += f.x * g.y;
+          k1 += f.y * g.x;
         }
         for (int k = 0; k < s4; k++) {
-          const int gi = r * ldc + i3 * s4 + k;
-          k2 += fd[i4 * s4 + k] * G2[gi];
-          k1 += fd[s4 * s4 + i4 * s4 + k] * G1[gi];
+          const double2 f = fd[i4 * s4 + k], g = G[r * ldc + i3 * s4 + k];
+          k2 += f.x * g.y;
+          k1 += f.y * g.x;
         }
         val += h2 * k1 - h1 * k2;
       }
