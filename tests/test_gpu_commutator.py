@@ -1,8 +1,9 @@
 """GPU parity: every commutator entry point of the HIP library against the CPU
 oracle on the same seeded inputs.  Tolerance: max |gpu - oracle| <=
 1e-12 * max |oracle| (FP64; the MFMA accumulation order differs from the
-oracle's).  Sizes: N=6/R=6 (the CPU-runnable configuration) and N=20/R=14
-(the commutator microbenchmark's channel structure)."""
+oracle's).  Sizes: N=6/R=6 (the CPU-runnable configuration), N=12/R=10 (other
+channel sizes and box mixes) and N=20/R=14 (the commutator microbenchmark's
+channel structure, where the largest class-pair boxes are split into slabs)."""
 import sys
 
 import numpy as np
@@ -36,7 +37,7 @@ class Case:
         return x, y, self.ctx.operator(h1, x), self.ctx.operator(h2, y)
 
 
-@pytest.fixture(scope="module", params=[(6, 6), (20, 14)], ids=["N6R6", "N20R14"])
+@pytest.fixture(scope="module", params=[(6, 6), (12, 10), (20, 14)], ids=["N6R6", "N12R10", "N20R14"])
 def case(request):
     c = Case(*request.param)
     yield c
