@@ -60,12 +60,12 @@ __device__ __forceinline__ void chunk_walk(const GemmProb &pr, const GemmSeg *__
 }
 
 // shape table: TM x TN tile, WM x WN waves, each wave FM x FN MFMA tiles
-template <int TM, int TN, int WM, int WN>
+template <int TM, int TN, int WM, int WN, int KC = GEMM_TK>
 __device__ __forceinline__ void gemm_tile(const GemmProb &pr, const GemmSeg *__restrict__ segs, const Slots &slots,
                                           int tm, int tn, double *__restrict__ lds) {
   constexpr int FM = TM / WM / 16, FN = TN / WN / 16;
-  double *As = lds;                       // [GEMM_TK][TM + 1]
-  double *Bs = lds + GEMM_TK * (TM + 1);  // [GEMM_TK][TN + 1]
+  double *As = lds;                       // [KC][TM + 1]
+  double *Bs = lds + KC * (TM + 1);  // [KC][TN + 1]
   const int row0 = tm * TM, col0 = tn * TN;
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int wm = wave % WM, wn = wave / WM;
@@ -77,7 +77,7 @@ __device__ __forceinline__ void gemm_tile(const GemmProb &pr, const GemmSeg *__r
 #pragma unroll
     for (int b = 0; b < FN; b++) acc[a][b] = d4{0, 0, 0, 0};
 
-  constexpr int NA = (TM * GEMM_TK + 255) / 256, NB = (TN * GEMM_TK + 255) / 256;
+  constexpr int NA = (TM * KC + 255) / 256, NB = (TN * KC + 255) / 256;
   double ra[NA], rb[NB];
   auto load = [&](const GemmSeg &sg, int k0) {
     const double *__restrict__ A = slots.p[sg.a_slot] + sg.a_off;
@@ -87,10 +87,10 @@ __device__ __forceinline__ void gemm_tile(const GemmProb &pr, const GemmSeg *__r
       const int idx = tid + 256 * r;
       int i, k;
       if (!sg.ta) { i = idx % TM; k = idx / TM; }        // columns of A contiguous in i
-      else { k = idx % GEMM_TK; i = idx / GEMM_TK; }     // A stored transposed
+      else { k = idx % KC; i = idx / KC; }     // A stored transposed
       const int gi = row0 + i, gk = k0 + k;
       double v = 0.0;
-      if (idx < TM * GEMM_TK && gi < pr.M && gk < sg.K)
+      if (idx < TM * KC && gi < pr.M && gk < sg.K)
         v = sg.ta ? A[gk + (int64_t)gi * sg.lda] : A[gi + (int64_t)gk * sg.lda];
       ra[r] = sg.alpha * v;
     }
@@ -98,11 +98,11 @@ __device__ __forceinline__ void gemm_tile(const GemmProb &pr, const GemmSeg *__r
     for (int r = 0; r < NB; r++) {
       const int idx = tid + 256 * r;
       int j, k;
-      if (!sg.tb) { k = idx % GEMM_TK; j = idx / GEMM_TK; }  // columns of B contiguous in k
+      if (!sg.tb) { k = idx % KC; j = idx / KC; }  // columns of B contiguous in k
       else { j = idx % TN; k = idx / TN; }                   // B stored transposed
       const int gj = col0 + j, gk = k0 + k;
       double v = 0.0;
-      if (idx < TN * GEMM_TK && gj < pr.N && gk < sg.K)
+      if (idx < TN * KC && gj < pr.N && gk < sg.K)
         v = sg.tb ? B[gj + (int64_t)gk * sg.ldb] : B[gk + (int64_t)gj * sg.ldb];
       rb[r] = v;
     }
@@ -111,23 +111,23 @@ __device__ __forceinline__ void gemm_tile(const GemmProb &pr, const GemmSeg *__r
 #pragma unroll
     for (int r = 0; r < NA; r++) {
       const int idx = tid + 256 * r;
-      if (idx < TM * GEMM_TK) {
-        const int i = ta ? idx / GEMM_TK : idx % TM, k = ta ? idx % GEMM_TK : idx / TM;
+      if (idx < TM * KC) {
+        const int i = ta ? idx / KC : idx % TM, k = ta ? idx % KC : idx / TM;
         As[k * (TM + 1) + i] = ra[r];
       }
     }
 #pragma unroll
     for (int r = 0; r < NB; r++) {
       const int idx = tid + 256 * r;
-      if (idx < TN * GEMM_TK) {
-        const int j = tb ? idx % TN : idx / GEMM_TK, k = tb ? idx / TN : idx % GEMM_TK;
+      if (idx < TN * KC) {
+        const int j = tb ? idx % TN : idx / KC, k = tb ? idx / TN : idx % KC;
         Bs[k * (TN + 1) + j] = rb[r];
       }
     }
   };
   auto compute = [&](int kv) {
 #pragma unroll
-    for (int kk = 0; kk < GEMM_TK; kk += 4) {
+    for (int kk = 0; kk < KC; kk += 4) {
       if (kk >= kv) break;  // zero fill past the segment's K adds nothing
       double a[FM], b[FN];
 #pragma unroll
@@ -140,7 +140,7 @@ __device__ __forceinline__ void gemm_tile(const GemmProb &pr, const GemmSeg *__r
         for (int y = 0; y < FN; y++) acc[x][y] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[x], b[y], acc[x][y], 0, 0, 0);
     }
   };
-  chunk_walk<GEMM_TK>(pr, segs, load, store, compute);
+  chunk_walk<KC>(pr, segs, load, store, compute);
   double *__restrict__ C = slots.p[pr.c_slot] + pr.c_off;
 #pragma unroll
   for (int x = 0; x < FM; x++)
@@ -508,14 +508,26 @@ __global__ void __launch_bounds__(256) k_gemm(const GemmProb *__restrict__ probs
                                               const GemmTile *__restrict__ tiles, int ntiles, Slots slots) {
   constexpr int TM = SHAPE_TM[SH], TN = SHAPE_TN[SH];
   constexpr int WM = TN <= 16 ? 4 : (TM <= 16 ? 1 : 2), WN = 4 / WM;
+  constexpr bool BIG = SH == SH_128x128;  // FP64: K chunks of 16; split precisions: four 64 x 64 sub-tiles
   constexpr int LB = (SH == SH_64x16K && P == GEMM_FP64)
                          ? (gemm_lds_bytes(TM, TN, P) > 3 * 16 * 64 * 8 ? gemm_lds_bytes(TM, TN, P) : 3 * 16 * 64 * 8)
-                         : gemm_lds_bytes(TM, TN, P);
+                     : BIG ? (P == GEMM_FP64 ? 16 * (TM + 1 + TN + 1) * 8 : gemm_lds_bytes(64, 64, P))
+                           : gemm_lds_bytes(TM, TN, P);
   __shared__ __align__(16) unsigned char lds[LB];
   for (int w = blockIdx.x; w < ntiles; w += gridDim.x) {
     const GemmTile tl = tiles[w];
     const GemmProb pr = probs[tl.prob];
     if constexpr (SH == SH_64x16K && P == GEMM_FP64) gemm_tile_splitk(pr, segs, slots, tl.tm, tl.tn, (double *)lds);
+    else if constexpr (BIG && P == GEMM_FP64) gemm_tile<128, 128, 2, 2, 16>(pr, segs, slots, tl.tm, tl.tn, (double *)lds);
+    else if constexpr (BIG) {
+      for (int q = 0; q < 4; q++) {
+        if (q) __syncthreads();
+        if constexpr (P == GEMM_FP32)
+          gemm_tile_f32<64, 64, 2, 2>(pr, segs, slots, 2 * tl.tm + (q & 1), 2 * tl.tn + (q >> 1), (float *)lds);
+        else
+          gemm_tile_bf16<64, 64, 2, 2, P>(pr, segs, slots, 2 * tl.tm + (q & 1), 2 * tl.tn + (q >> 1), (unsigned *)lds);
+      }
+    }
     else if constexpr (P == GEMM_FP64) gemm_tile<TM, TN, WM, WN>(pr, segs, slots, tl.tm, tl.tn, (double *)lds);
     else if constexpr (P == GEMM_FP32) gemm_tile_f32<TM, TN, WM, WN>(pr, segs, slots, tl.tm, tl.tn, (float *)lds);
     else gemm_tile_bf16<TM, TN, WM, WN, P>(pr, segs, slots, tl.tm, tl.tn, (unsigned *)lds);
@@ -535,6 +547,7 @@ static void launch_gemm_p(hipStream_t s, const GemmProb *probs, const GemmSeg *s
     case SH_16x64: hipLaunchKernelGGL((k_gemm<SH_16x64, P>), dim3(grid), dim3(256), 0, s, probs, segs, tiles, ntiles, slots); break;
     case SH_16x128: hipLaunchKernelGGL((k_gemm<SH_16x128, P>), dim3(grid), dim3(256), 0, s, probs, segs, tiles, ntiles, slots); break;
     case SH_64x16K: hipLaunchKernelGGL((k_gemm<SH_64x16K, P>), dim3(grid), dim3(256), 0, s, probs, segs, tiles, ntiles, slots); break;
+    case SH_128x128: hipLaunchKernelGGL((k_gemm<SH_128x128, P>), dim3(grid), dim3(256), 0, s, probs, segs, tiles, ntiles, slots); break;
     default: break;
   }
 }

@@ -34,10 +34,13 @@ struct GemmTile { int prob, tm, tn; };
 constexpr int GEMM_TK = 32;
 // tile shapes of the grouped GEMM (rows x cols of C per work-group)
 // SH_64x16K: a 64 x 16 tile whose four waves split every K chunk and add
-// their partial tiles in a fixed order at the end (for N <= 16 problems)
-enum GemmShape { SH_64x64 = 0, SH_128x64, SH_32x32, SH_128x16, SH_64x16, SH_16x64, SH_16x128, SH_64x16K, NSHAPE };
-constexpr int SHAPE_TM[NSHAPE] = {64, 128, 32, 128, 64, 16, 16, 64};
-constexpr int SHAPE_TN[NSHAPE] = {64, 64, 32, 16, 16, 64, 128, 16};
+// their partial tiles in a fixed order at the end (for N <= 16 problems).
+// SH_128x128: four waves of 64 x 64 (K chunks of 16), half the operand
+// traffic per flop of 64 x 64 (for large problems)
+enum GemmShape { SH_64x64 = 0, SH_128x64, SH_32x32, SH_128x16, SH_64x16, SH_16x64, SH_16x128, SH_64x16K, SH_128x128,
+                 NSHAPE };
+constexpr int SHAPE_TM[NSHAPE] = {64, 128, 32, 128, 64, 16, 16, 64, 128};
+constexpr int SHAPE_TN[NSHAPE] = {64, 64, 32, 16, 16, 64, 128, 16, 128};
 // arithmetic of the grouped GEMM: FP64, FP32, or BF16 mantissa slices (the
 // value is the slice count)
 enum GemmPrec { GEMM_FP64 = 0, GEMM_FP32 = 1, GEMM_BF16X2 = 2, GEMM_BF16X3 = 3, GEMM_BF16X4 = 4 };
