@@ -66,12 +66,14 @@ struct GemmList {
   GemmProb *probs = nullptr;
   GemmSeg *segs = nullptr;
   int npre = 0;
-  GemmTile *tiles_pre = nullptr, *tiles_all = nullptr;  // sorted by shape, then decreasing work
+  GemmTile *tiles_pre = nullptr, *tiles_all = nullptr;  // by shape, then XCD-interleaved (prob -1: empty)
   int ntiles_pre = 0, ntiles_all = 0;
-  int shape_pre[NSHAPE + 1] = {}, shape_all[NSHAPE + 1] = {};  // per-shape ranges of the two lists
+  // per-launch ranges of the two lists: key sh < NSHAPE = that shape alone,
+  // key SH_MIX = the mixable shapes together
+  int shape_pre[NSHAPE + 2] = {}, shape_all[NSHAPE + 2] = {};
   GemmTile *ptiles = nullptr;                            // grouped per problem
   std::vector<int> ptile_off;
-  size_t cap = 0;                                        // capacity of each tile buffer
+  size_t cap = 0;                                        // capacity of tiles_pre / tiles_all
   double flops = 0, flops_pre = 0;                       // algorithmic 2*M*N*K
 };
 }  // namespace
@@ -247,31 +249,103 @@ int upload_tiles(qd_context *c, GemmList &g) {
   g.ptile_off[np] = (int)grouped.size();
   all = grouped;
   for (int pi = 0; pi < g.npre; pi++) tiles_of(g, pi, pre);
+  auto key = [&](const GemmTile &t) {
+    const int sh = g.hprobs[t.prob].shape;
+    return gemm_mixable(sh) ? SH_MIX : sh;
+  };
   auto bywork = [&](const GemmTile &a, const GemmTile &b) {
-    const int sa = g.hprobs[a.prob].shape, sb = g.hprobs[b.prob].shape;
+    const int sa = key(a), sb = key(b);
     if (sa != sb) return sa < sb;
     return work(a) > work(b);
   };
   std::stable_sort(pre.begin(), pre.end(), bywork);
   std::stable_sort(all.begin(), all.end(), bywork);
-  auto ranges = [&](const std::vector<GemmTile> &t, int *r) {
-    for (int sh = 0; sh <= NSHAPE; sh++) r[sh] = 0;
-    for (auto &x : t) r[g.hprobs[x.prob].shape + 1]++;
-    for (int sh = 0; sh < NSHAPE; sh++) r[sh + 1] += r[sh];
+  // XCD-aware order within each launch's range: a problem's operands (< 4 MB
+  // here) are read by all its tiles, so its tiles should run on one XCD and
+  // share that XCD's L2.  The persistent grid is a multiple of 8 work-groups
+  // striding the list and work-group i runs on XCD i % 8, so slot t of the
+  // list holds a tile of XCD t % 8.  Problems, cut into pieces of whole tile
+  // rows of at most 1/16 of the range's work, go to XCDs largest-first
+  // (LPT); each XCD keeps its tiles in decreasing work; short XCD queues are
+  // padded with empty tiles (prob -1).
+  auto xcd_order = [&](std::vector<GemmTile> &t, int *r) {
+    std::vector<GemmTile> out;
+    for (int sh = 0; sh <= SH_MIX; sh++) {
+      r[sh] = (int)out.size();
+      size_t b = 0;
+      while (b < t.size() && key(t[b]) < sh) b++;
+      size_t e = b;
+      while (e < t.size() && key(t[e]) == sh) e++;
+      if (e == b) continue;
+      double total = 0;
+      for (size_t k = b; k < e; k++) total += work(t[k]);
+      const double cut = total / 16;
+      // pieces: runs of one problem's tiles (same tm rows kept together)
+      std::vector<std::pair<size_t, size_t>> pieces;
+      std::vector<double> pw;
+      for (size_t k = b; k < e;) {
+        size_t m = k;
+        double w = 0;
+        while (m < e && t[m].prob == t[k].prob) {
+          size_t n = m;
+          double rw = 0;
+          while (n < e && t[n].prob == t[k].prob && t[n].tm == t[m].tm) rw += work(t[n++]);
+          if (m > k && w + rw > cut) break;
+          w += rw;
+          m = n;
+        }
+        pieces.push_back({k, m});
+        pw.push_back(w);
+        k = m;
+      }
+      std::vector<int> order(pieces.size());
+      for (size_t k = 0; k < order.size(); k++) order[k] = (int)k;
+      std::stable_sort(order.begin(), order.end(), [&](int x, int y) { return pw[x] > pw[y]; });
+      std::vector<double> load(NXCD, 0.0);
+      std::vector<std::vector<GemmTile>> q(NXCD);
+      for (int k : order) {
+        const int x = (int)(std::min_element(load.begin(), load.end()) - load.begin());
+        load[x] += pw[k];
+        for (size_t m = pieces[k].first; m < pieces[k].second; m++) q[x].push_back(t[m]);
+      }
+      size_t len = 0;
+      for (auto &v : q) {
+        std::stable_sort(v.begin(), v.end(), [&](const GemmTile &x, const GemmTile &y) { return work(x) > work(y); });
+        len = std::max(len, v.size());
+      }
+      for (size_t j = 0; j < len; j++)
+        for (int x = 0; x < NXCD; x++) out.push_back(j < q[x].size() ? q[x][j] : GemmTile{-1, 0, 0});
+    }
+    r[SH_MIX + 1] = (int)out.size();
+    t.swap(out);
   };
-  ranges(pre, g.shape_pre);
-  ranges(all, g.shape_all);
-  if (!g.tiles_all) {  // capacity: every shape covers at least 16 x 64 outputs
+  xcd_order(pre, g.shape_pre);
+  xcd_order(all, g.shape_all);
+  // capacity: grown here only (autotuning drops captured graphs first)
+  const size_t need = std::max(all.size(), pre.size());
+  if (need > g.cap) {
     size_t cap = 0;
     for (auto &p : g.hprobs) cap += (size_t)((p.M + 15) / 16) * ((p.N + 15) / 16);
-    g.cap = std::max<size_t>(cap, 1);
+    cap = std::max(cap, need + need / 4);
+    for (GemmTile *old : {g.tiles_all, g.tiles_pre}) {
+      if (!old) continue;
+      c->allocs.erase(std::find(c->allocs.begin(), c->allocs.end(), (void *)old));
+      HIPC(hipFree(old));
+    }
+    g.cap = cap;
     void *q;
     int rc;
     if ((rc = dev_alloc(c, sizeof(GemmTile) * g.cap, &q))) return rc; g.tiles_all = (GemmTile *)q;
     if ((rc = dev_alloc(c, sizeof(GemmTile) * g.cap, &q))) return rc; g.tiles_pre = (GemmTile *)q;
-    if ((rc = dev_alloc(c, sizeof(GemmTile) * g.cap, &q))) return rc; g.ptiles = (GemmTile *)q;
   }
-  if (all.size() > g.cap) return fail(QD_E_LAYOUT, "tile list exceeds its capacity");
+  if (!g.ptiles) {  // every shape covers at least 16 x 64 outputs
+    size_t units = 0;
+    for (auto &p : g.hprobs) units += (size_t)((p.M + 15) / 16) * ((p.N + 15) / 16);
+    void *q;
+    int rc;
+    if ((rc = dev_alloc(c, sizeof(GemmTile) * std::max<size_t>(units, 1), &q))) return rc;
+    g.ptiles = (GemmTile *)q;
+  }
   HIPC(hipMemcpy(g.tiles_all, all.data(), sizeof(GemmTile) * all.size(), hipMemcpyHostToDevice));
   HIPC(hipMemcpy(g.tiles_pre, pre.data(), sizeof(GemmTile) * pre.size(), hipMemcpyHostToDevice));
   HIPC(hipMemcpy(g.ptiles, grouped.data(), sizeof(GemmTile) * grouped.size(), hipMemcpyHostToDevice));
@@ -581,7 +655,7 @@ int run_gemm_list(qd_context *c, const GemmList &g, bool with_pandya, const Slot
   if (c->launch_mode == 0) {
     const int *r = with_pandya ? g.shape_all : g.shape_pre;
     GemmTile *t = with_pandya ? g.tiles_all : g.tiles_pre;
-    for (int sh = 0; sh < NSHAPE; sh++)
+    for (int sh = 0; sh <= SH_MIX; sh++)
       launch_gemm(c->stream, g.probs, g.segs, t + r[sh], r[sh + 1] - r[sh], s, c->gemm_grid, sh, prec);
   } else {
     const int np = with_pandya ? (int)g.hprobs.size() : g.npre;

@@ -504,35 +504,86 @@ constexpr int gemm_lds_bytes(int TM, int TN, int P) {
 }
 
 template <int SH, int P>
-__global__ void __launch_bounds__(256) k_gemm(const GemmProb *__restrict__ probs, const GemmSeg *__restrict__ segs,
-                                              const GemmTile *__restrict__ tiles, int ntiles, Slots slots) {
+constexpr int tile_lds_bytes() {
+  constexpr int TM = SHAPE_TM[SH], TN = SHAPE_TN[SH];
+  return (SH == SH_64x16K && P == GEMM_FP64)
+             ? (gemm_lds_bytes(TM, TN, P) > 3 * 16 * 64 * 8 ? gemm_lds_bytes(TM, TN, P) : 3 * 16 * 64 * 8)
+         : SH == SH_128x128 ? (P == GEMM_FP64 ? 16 * (TM + 1 + TN + 1) * 8 : gemm_lds_bytes(64, 64, P))
+                            : gemm_lds_bytes(TM, TN, P);
+}
+
+// one output tile of shape SH in precision P
+template <int SH, int P>
+__device__ __forceinline__ void run_tile(const GemmProb &pr, const GemmSeg *__restrict__ segs, const Slots &slots,
+                                         const GemmTile &tl, unsigned char *lds) {
   constexpr int TM = SHAPE_TM[SH], TN = SHAPE_TN[SH];
   constexpr int WM = TN <= 16 ? 4 : (TM <= 16 ? 1 : 2), WN = 4 / WM;
   constexpr bool BIG = SH == SH_128x128;  // FP64: K chunks of 16; split precisions: four 64 x 64 sub-tiles
-  constexpr int LB = (SH == SH_64x16K && P == GEMM_FP64)
-                         ? (gemm_lds_bytes(TM, TN, P) > 3 * 16 * 64 * 8 ? gemm_lds_bytes(TM, TN, P) : 3 * 16 * 64 * 8)
-                     : BIG ? (P == GEMM_FP64 ? 16 * (TM + 1 + TN + 1) * 8 : gemm_lds_bytes(64, 64, P))
-                           : gemm_lds_bytes(TM, TN, P);
-  __shared__ __align__(16) unsigned char lds[LB];
+  if constexpr (SH == SH_64x16K && P == GEMM_FP64) gemm_tile_splitk(pr, segs, slots, tl.tm, tl.tn, (double *)lds);
+  else if constexpr (BIG && P == GEMM_FP64) gemm_tile<128, 128, 2, 2, 16>(pr, segs, slots, tl.tm, tl.tn, (double *)lds);
+  else if constexpr (BIG) {
+    for (int q = 0; q < 4; q++) {
+      if (q) __syncthreads();
+      if constexpr (P == GEMM_FP32)
+        gemm_tile_f32<64, 64, 2, 2>(pr, segs, slots, 2 * tl.tm + (q & 1), 2 * tl.tn + (q >> 1), (float *)lds);
+      else
+        gemm_tile_bf16<64, 64, 2, 2, P>(pr, segs, slots, 2 * tl.tm + (q & 1), 2 * tl.tn + (q >> 1), (unsigned *)lds);
+    }
+  }
+  else if constexpr (P == GEMM_FP64) gemm_tile<TM, TN, WM, WN>(pr, segs, slots, tl.tm, tl.tn, (double *)lds);
+  else if constexpr (P == GEMM_FP32) gemm_tile_f32<TM, TN, WM, WN>(pr, segs, slots, tl.tm, tl.tn, (float *)lds);
+  else gemm_tile_bf16<TM, TN, WM, WN, P>(pr, segs, slots, tl.tm, tl.tn, (unsigned *)lds);
+}
+
+template <int SH, int P>
+__global__ void __launch_bounds__(256) k_gemm(const GemmProb *__restrict__ probs, const GemmSeg *__restrict__ segs,
+                                              const GemmTile *__restrict__ tiles, int ntiles, Slots slots) {
+  __shared__ __align__(16) unsigned char lds[tile_lds_bytes<SH, P>()];
   for (int w = blockIdx.x; w < ntiles; w += gridDim.x) {
     const GemmTile tl = tiles[w];
+    if (tl.prob < 0) continue;  // padding of a short XCD queue (uniform over the work-group)
     const GemmProb pr = probs[tl.prob];
-    if constexpr (SH == SH_64x16K && P == GEMM_FP64) gemm_tile_splitk(pr, segs, slots, tl.tm, tl.tn, (double *)lds);
-    else if constexpr (BIG && P == GEMM_FP64) gemm_tile<128, 128, 2, 2, 16>(pr, segs, slots, tl.tm, tl.tn, (double *)lds);
-    else if constexpr (BIG) {
-      for (int q = 0; q < 4; q++) {
-        if (q) __syncthreads();
-        if constexpr (P == GEMM_FP32)
-          gemm_tile_f32<64, 64, 2, 2>(pr, segs, slots, 2 * tl.tm + (q & 1), 2 * tl.tn + (q >> 1), (float *)lds);
-        else
-          gemm_tile_bf16<64, 64, 2, 2, P>(pr, segs, slots, 2 * tl.tm + (q & 1), 2 * tl.tn + (q >> 1), (unsigned *)lds);
-      }
-    }
-    else if constexpr (P == GEMM_FP64) gemm_tile<TM, TN, WM, WN>(pr, segs, slots, tl.tm, tl.tn, (double *)lds);
-    else if constexpr (P == GEMM_FP32) gemm_tile_f32<TM, TN, WM, WN>(pr, segs, slots, tl.tm, tl.tn, (float *)lds);
-    else gemm_tile_bf16<TM, TN, WM, WN, P>(pr, segs, slots, tl.tm, tl.tn, (unsigned *)lds);
+    run_tile<SH, P>(pr, segs, slots, tl, lds);
     __syncthreads();  // the next tile reuses the LDS staging area
   }
+}
+
+// The mixed launch: tiles of the shapes gemm_mixable() admits (64 x 64 and
+// 32 x 32), each taking its problem's shape, in one persistent grid, so that
+// a list whose problems want either shape still runs as one launch.  (Wider
+// mixes raise the register count past four waves per SIMD.)
+template <int P>
+constexpr int mix_lds_bytes() {
+  constexpr int a = tile_lds_bytes<SH_64x64, P>(), b = tile_lds_bytes<SH_32x32, P>();
+  return a > b ? a : b;
+}
+
+template <int P>
+__device__ __forceinline__ void gemm_mix_body(const GemmProb *__restrict__ probs, const GemmSeg *__restrict__ segs,
+                                              const GemmTile *__restrict__ tiles, int ntiles, const Slots &slots) {
+  __shared__ __align__(16) unsigned char lds[mix_lds_bytes<P>()];
+  for (int w = blockIdx.x; w < ntiles; w += gridDim.x) {
+    const GemmTile tl = tiles[w];
+    if (tl.prob < 0) continue;  // padding of a short XCD queue (uniform over the work-group)
+    const GemmProb pr = probs[tl.prob];
+    if (pr.shape == SH_32x32) run_tile<SH_32x32, P>(pr, segs, slots, tl, lds);
+    else run_tile<SH_64x64, P>(pr, segs, slots, tl, lds);
+    __syncthreads();  // the next tile reuses the LDS staging area
+  }
+}
+// four waves per SIMD (FP64, FP32, BF16x2 fit 128 registers without spills)
+template <int P>
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4)))
+k_gemm_mix(const GemmProb *__restrict__ probs, const GemmSeg *__restrict__ segs, const GemmTile *__restrict__ tiles,
+           int ntiles, Slots slots) {
+  gemm_mix_body<P>(probs, segs, tiles, ntiles, slots);
+}
+// BF16x3, BF16x4: more slices in flight, left to the register allocator
+template <int P>
+__global__ void __launch_bounds__(256) k_gemm_mix_wide(const GemmProb *__restrict__ probs,
+                                                       const GemmSeg *__restrict__ segs,
+                                                       const GemmTile *__restrict__ tiles, int ntiles, Slots slots) {
+  gemm_mix_body<P>(probs, segs, tiles, ntiles, slots);
 }
 
 template <int P>
@@ -556,6 +607,20 @@ void launch_gemm(hipStream_t s, const GemmProb *probs, const GemmSeg *segs, cons
                  const Slots &slots, int grid, int shape, int prec) {
   if (ntiles <= 0) return;
   if (grid <= 0 || grid > ntiles) grid = ntiles;
+  else if (grid >= NXCD) grid -= grid % NXCD;  // keeps tile slot t on XCD t % 8
+  if (shape == SH_MIX) {
+    switch (prec) {
+#define QD_MIX(K, P) hipLaunchKernelGGL((K<P>), dim3(grid), dim3(256), 0, s, probs, segs, tiles, ntiles, slots)
+      case GEMM_FP64: QD_MIX(k_gemm_mix, GEMM_FP64); break;
+      case GEMM_FP32: QD_MIX(k_gemm_mix, GEMM_FP32); break;
+      case GEMM_BF16X2: QD_MIX(k_gemm_mix, GEMM_BF16X2); break;
+      case GEMM_BF16X3: QD_MIX(k_gemm_mix_wide, GEMM_BF16X3); break;
+      case GEMM_BF16X4: QD_MIX(k_gemm_mix_wide, GEMM_BF16X4); break;
+#undef QD_MIX
+      default: break;
+    }
+    return;
+  }
   switch (prec) {
     case GEMM_FP64: launch_gemm_p<GEMM_FP64>(s, probs, segs, tiles, ntiles, slots, grid, shape); break;
     case GEMM_FP32: launch_gemm_p<GEMM_FP32>(s, probs, segs, tiles, ntiles, slots, grid, shape); break;

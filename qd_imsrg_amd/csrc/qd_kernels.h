@@ -87,8 +87,14 @@ enum { OB_111 = 1, OB_121 = 2, OB_221 = 4, OB_ACCUM = 8 };
 // Small products: 16 x 16 tiles (tm, tn in units of 16), FP64 FMA.
 void launch_gemm_small(hipStream_t s, const GemmProb *probs, const GemmSeg *segs, const GemmTile *tiles,
                        int ntiles, const Slots &slots);
-// All tiles must have the given shape.  grid <= 0: one work-group per tile;
-// else a persistent grid striding the list.
+// Shapes that can share one launch (similar LDS and register budgets).
+constexpr bool gemm_mixable(int sh) {
+  return sh == SH_64x64 || sh == SH_32x32;
+}
+constexpr int SH_MIX = NSHAPE;  // launch_gemm: tiles of any mixable shape, each its problem's
+// All tiles must have the given shape (or, for SH_MIX, a mixable one).
+// grid <= 0: one work-group per tile; else a persistent grid striding the
+// list; tiles with prob < 0 are skipped.
 void launch_gemm(hipStream_t s, const GemmProb *probs, const GemmSeg *segs, const GemmTile *tiles,
                  int ntiles, const Slots &slots, int grid, int shape, int prec);
 
