@@ -386,7 +386,7 @@ def main():
     ctx = qd.Context(a.nbody, a.nshell)
     tune = None
     if not a.no_autotune:
-        tune = ctx.autotune(reps=3)  # tile shapes per size bucket (setup, not timed)
+        tune = ctx.autotune(reps=10)  # tile shapes per size bucket (setup, not timed)
         progress(f"autotune: GEMM ms {tune[0]:.3f} -> {tune[1]:.3f}")
     X, Y, Z = ctx.operator(1, x), ctx.operator(1, y), ctx.operator(-1)
     HCC, ECC = ctx.cc(1.0), ctx.cc(1.0)

@@ -261,10 +261,12 @@ int qd_set_launch_mode(qd_context *ctx, int mode);
 int qd_set_graphs(qd_context *ctx, int on);
 /* Project autotuner: coordinate descent over the (M, N, K) size buckets of
  * the grouped GEMM problems, largest work first.  For each bucket every tile
- * shape (64x64, 128x64, 32x32, 128x16, 64x16, 16x64, 16x128) is tried, timing
- * one commutator's whole GEMM work (both grouped lists) on scratch operands at
- * the current precision, and the fastest is kept.  Reports that device time
- * before and after (ms).  Drops captured BCH graphs. */
+ * shape (64x64, 128x64, 32x32, 128x16, 64x16, 16x64, 16x128, 64x16 split-K,
+ * 128x128) is tried, timing one commutator's whole GEMM work (both grouped
+ * lists; the fastest of reps runs) on scratch operands at the current
+ * precision, and the fastest is kept.  64x64 and 32x32 tiles share one
+ * launch; every other shape present adds a launch.  Reports that device time
+ * before and after (ms).  Drops captured graphs. */
 int qd_autotune(qd_context *ctx, int reps, double *ms_before, double *ms_after);
 
 /* ---- learned warm start ----------------------------------------------
