@@ -641,7 +641,9 @@ void launch_gemm(hipStream_t s, const GemmProb *probs, const GemmSeg *segs, cons
 // same to the bit), then writes D_ij = h_H P_ij - h_eta P_ji^T and
 // D_ji = h_H P_ji - h_eta P_ij^T, exchanging the transposed tile through LDS.
 // The product matrix P is never written.
-__global__ void __launch_bounds__(256) k_pandya_fused(DevTab t, const double *__restrict__ E,
+// three waves per SIMD (160 registers, no spills; the uncapped allocation
+// takes 186 and leaves two)
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) k_pandya_fused(DevTab t, const double *__restrict__ E,
                                                       const double *__restrict__ H, double *__restrict__ D,
                                                       const TilePair *__restrict__ tp, double hh, double eh) {
   constexpr int T = 64, TK = 16, LD = T + 1;
