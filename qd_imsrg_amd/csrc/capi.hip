@@ -644,6 +644,55 @@ Slots make_slots(qd_context *c, const qd_op *r1, const qd_op *r2, qd_op *out, co
   return s;
 }
 
+// calc_cc (src/ME_general.f90:923-952) as a gather table: for the CC element
+// (row (i, j), column (k, l)) of every block in device order, the V element
+// <j k|V|i l> that v_elem (src/ME_general.f90:1557-1680) returns: the flat
+// index of the stored element after sorting both pairs (the sign of the
+// sorting in bit 31), bit 30 set where the element is stored transposed (the
+// herm factor applies), CC_ZERO for a repeated state or pairs in different
+// blocks.  Requires neq < CC_ZERO.
+std::vector<uint32_t> cc_gather_map(const Channels &ch) {
+  const int M = ch.M;
+  auto elem = [&](int a1, int a2, int a3, int a4) -> uint32_t {
+    if (a1 == a2 || a3 == a4) return CC_ZERO;
+    uint32_t sgn = 0;
+    if (a1 > a2) { std::swap(a1, a2); sgn ^= 1u; }
+    if (a3 > a4) { std::swap(a3, a4); sgn ^= 1u; }
+    const uint32_t I = ch.iarr[(size_t)a1 * M - (size_t)a1 * (a1 + 1) / 2 + (a2 - a1 - 1)];
+    const uint32_t J = ch.iarr[(size_t)a3 * M - (size_t)a3 * (a3 + 1) / 2 + (a4 - a3 - 1)];
+    const int q = ia_blk(I);
+    if (q != ia_blk(J)) return CC_ZERO;
+    const int64_t w1 = ia_idx(I), w2 = ia_idx(J), nh = ch.nh[q], np = ch.np[q], nb = ch.nb[q];
+    const int64_t *o = &ch.off[6 * (size_t)q];
+    int64_t k = 0;
+    uint32_t tr = 0;
+    switch (ia_type(I) * 4 + ia_type(J)) {
+      case T_HH * 4 + T_HH: k = o[HHHH] + w1 + w2 * nh; break;
+      case T_HH * 4 + T_PH: k = o[PHHH] + w2 + w1 * nb; tr = 1; break;
+      case T_HH * 4 + T_PP: k = o[PPHH] + w2 + w1 * np; tr = 1; break;
+      case T_PH * 4 + T_HH: k = o[PHHH] + w1 + w2 * nb; break;
+      case T_PH * 4 + T_PH: k = o[PHPH] + w1 + w2 * nb; break;
+      case T_PH * 4 + T_PP: k = o[PPPH] + w2 + w1 * np; tr = 1; break;
+      case T_PP * 4 + T_HH: k = o[PPHH] + w1 + w2 * np; break;
+      case T_PP * 4 + T_PH: k = o[PPPH] + w1 + w2 * np; break;
+      default: k = o[PPPP] + w1 + w2 * np; break;
+    }
+    return (uint32_t)k | (tr << 30) | (sgn << 31);
+  };
+  std::vector<uint32_t> m((size_t)ch.cc_size);
+  for (int q = 0; q < ch.ncc; q++) {
+    const int r = ch.cc_r[q], nb = ch.cc_nb[q];
+    for (int c = 0; c < nb; c++) {
+      const uint32_t hp = ch.cc_hp[ch.cc_hp_start[q] + c];
+      for (int a = 0; a < r; a++) {
+        const uint32_t ab = ch.cc_ab_dev[ch.cc_ab_start[q] + a];
+        m[(size_t)ch.cc_off[q] + a + (size_t)c * r] = elem(hi16(ab), lo16(hp), lo16(ab), hi16(hp));
+      }
+    }
+  }
+  return m;
+}
+
 bool pandya_on(const qd_cc *hcc, const qd_cc *ecc) {
   return hcc && ecc && !hcc->zero && !ecc->zero;
 }
@@ -953,6 +1002,12 @@ int qd_context_create(int nbody, int nshell, qd_context **out) {
   UP(ch.cp_a, cp_a); UP(ch.cp_b, cp_b); UP(ch.cp_grid_off, cp_grid_off); UP(ch.cp_grid, cp_grid);
   UP(ch.fcls_off, fcls_off);
   t.fcls_size = ch.fcls_size;
+  t.cc_size = ch.cc_size;
+  t.cc_gmap = nullptr;
+  if (ch.neq < (int64_t)CC_ZERO) {
+    const std::vector<uint32_t> gmap = cc_gather_map(ch);
+    UP(gmap, cc_gmap);
+  }
 #undef UP
   void *p;
   if ((rc = dev_alloc(c, sizeof(double) * ch.neq, &p))) return bail(rc); c->w1 = (double *)p;

@@ -306,9 +306,30 @@ __global__ void __launch_bounds__(256) k_calc_cc(DevTab t, const double *__restr
   cc[sg.off + e] = velem(t, v, herm, hi16(ab), lo16(hp), lo16(ab), hi16(hp));
 }
 
+// the same gather from the table: one coalesced index load, one V load and
+// one store per element (k_calc_cc resolves the pair tables per element)
+__global__ void __launch_bounds__(256) k_calc_cc_map(const uint32_t *__restrict__ map, const double *__restrict__ v,
+                                                     int herm, double *__restrict__ cc, int64_t n) {
+  const int64_t e = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  if (e >= n) return;
+  const uint32_t m = map[e], k = m & CC_ZERO;
+  double x = 0.0;
+  if (k != CC_ZERO) {
+    x = v[k];
+    if ((m >> 30) & 1u) x = herm * x;
+    if (m >> 31) x = -x;
+  }
+  cc[e] = x;
+}
+
 void launch_calc_cc(hipStream_t s, const DevTab &t, const double *v, int herm, double *cc, const Seg2 *cc_segs,
                     const Tile2 *cc_tiles, int ntiles) {
   if (ntiles <= 0) return;
+  if (t.cc_gmap) {
+    hipLaunchKernelGGL(k_calc_cc_map, dim3((unsigned)((t.cc_size + 255) / 256)), dim3(256), 0, s, t.cc_gmap, v, herm,
+                       cc, t.cc_size);
+    return;
+  }
   hipLaunchKernelGGL(k_calc_cc, dim3(ntiles), dim3(256), 0, s, t, v, herm, cc, cc_segs, cc_tiles);
 }
 

@@ -105,6 +105,13 @@ struct DevTab {
   const uint32_t *cp_grid;
   const int *fcls_off;
   int fcls_size;
+  // calc_cc as a table (null when a flat index does not fit 30 bits): per
+  // CC element in device order, the flat index of the V element it reads |
+  // transposed (herm factor) << 30 | pair-sorting sign << 31; CC_ZERO in the
+  // index bits for an element that is zero
+  const uint32_t *cc_gmap;
+  int64_t cc_size;
 };
+constexpr uint32_t CC_ZERO = 0x3fffffffu;
 
 }  // namespace qd
