@@ -56,7 +56,8 @@ __global__ void __launch_bounds__(256) k_box2b(DevTab t, Ops o, double *__restri
                                                const BTask *__restrict__ tasks, const int *__restrict__ blist,
                                                int flags, const double *__restrict__ w1,
                                                const double *__restrict__ w2, const double *__restrict__ w3,
-                                               const double *__restrict__ F, const double *__restrict__ D) {
+                                               const double *__restrict__ L, const double *__restrict__ F,
+                                               const double *__restrict__ D) {
   __shared__ double2 gg[BX_VCAP];              // ordered grids of (V1, V2): [row][ldc]
   __shared__ double2 fl[BX_FCAP / 2];          // channel blocks of (f1, f2): A3, A4, then A1, A2 per box
   __shared__ int pre[BX_GMAX + 1];             // element offsets of the boxes
@@ -203,21 +204,20 @@ __global__ void __launch_bounds__(256) k_box2b(DevTab t, Ops o, double *__restri
       if (b2 >= 0) d2 = D[b2 + i4 * s2 + i2 + (int64_t)(i1 * s3 + i3) * pl[b][2]];
       if (b3 >= 0) d3 = D[b3 + i4 * s1 + i1 + (int64_t)(i2 * s3 + i3) * pl[b][3]];
     }
-    // ladder inputs (transposed / signed combinations of the products of
-    // matrix_mult222, src/commutators.f90:999-1127, and the summed products
-    // of w3 for Vpphh, Vphhh, Vppph): xa = w1(I,J), xb = w3 or w2 (PHPH) at
-    // (I,J), xc = w2(I,J) for PHHH / PPPH, xt1 = w1(J,I), xt2 = w3 or w2 (PHPH)
-    // at (J,I) of a square sub-block
+    // ladder inputs (signed combinations of the products of matrix_mult222,
+    // src/commutators.f90:999-1127, and the summed products of w3 for Vpphh,
+    // Vphhh, Vppph): xa = w1(I,J), xb = w3(I,J), xc = w2(I,J) for PHHH /
+    // PPPH; the square sub-blocks read their whole ladder term from L
     const bool lad = first && (flags & TB_LADDER);
     const bool sq = sub == HHHH || sub == PPPP || sub == PHPH;
-    const int64_t tr = soff[tyr][tyc] + Jl + (int64_t)Il * rows;
-    const double *wb = sub == PHPH ? w2 : w3;
-    double xa = 0.0, xb = 0.0, xc = 0.0, xt1 = 0.0, xt2 = 0.0;
+    double xa = 0.0, xb = 0.0, xc = 0.0;
     if (lad) {
-      if (sub != PPHH) xa = w1[idx];
-      xb = wb[idx];
-      if (sub == PHHH || sub == PPPH) xc = w2[idx];
-      if (sq) { xt1 = w1[tr]; xt2 = wb[tr]; }
+      if (sq) xa = L[idx];
+      else {
+        if (sub != PPHH) xa = w1[idx];
+        xb = w3[idx];
+        if (sub == PHHH || sub == PPPH) xc = w2[idx];
+      }
     }
     // the bra slabs of a split box add to what the ket slabs wrote
     const bool rd = first ? ((flags & TB_ACCUM) || (mode == 1 && do122)) : (flags & TB_ACCUM) != 0;
@@ -266,16 +266,10 @@ __global__ void __launch_bounds__(256) k_box2b(DevTab t, Ops o, double *__restri
       switch (sub) {
         case HHHH:
         case PPPP:
-          acc += h1 * xt1 - h2 * xa;
-          acc += h1 * h2 * xt2 - xb;
-          break;
+        case PHPH: acc += xa; break;
         case PPHH: acc += xb; break;
         case PHHH: acc += h1 * xc - h2 * xa + xb; break;
         case PPPH: acc += -h2 * xa + h1 * xc + xb; break;
-        case PHPH:
-          acc += h1 * xt2 - h2 * xb;
-          acc += h1 * xt1 - h2 * xa;
-          break;
         default: break;
       }
     }
@@ -285,9 +279,54 @@ __global__ void __launch_bounds__(256) k_box2b(DevTab t, Ops o, double *__restri
 
 void launch_box2b(hipStream_t s, const DevTab &t, const Ops &o, double *out, const BTask *tasks, int ntasks,
                   const int *blist, int flags, const double *w1, const double *w2, const double *w3,
-                  const double *F, const double *D) {
+                  const double *L, const double *F, const double *D) {
   if (ntasks <= 0) return;
-  hipLaunchKernelGGL(k_box2b, dim3(ntasks), dim3(256), 0, s, t, o, out, tasks, blist, flags, w1, w2, w3, F, D);
+  hipLaunchKernelGGL(k_box2b, dim3(ntasks), dim3(256), 0, s, t, o, out, tasks, blist, flags, w1, w2, w3, L, F, D);
+}
+
+// The square sub-blocks' ladder terms need the products at (J, I): one
+// work-group per pair of 32x32 tiles (ij, ji) stages both tiles of both
+// products through LDS, so that the box pass reads one coalesced value per
+// element instead of two scattered transposed ones.
+__global__ void __launch_bounds__(256) k_ladder_sq(DevTab t, const double *__restrict__ w1,
+                                                   const double *__restrict__ w2, const double *__restrict__ w3,
+                                                   double *__restrict__ L, const TilePair *__restrict__ tp, double h1,
+                                                   double h2) {
+  __shared__ double Aij[32][33], Aji[32][33], Bij[32][33], Bji[32][33];
+  const TilePair p = tp[blockIdx.x];
+  const int q = p.blk / 3, kind = p.blk % 3;
+  const int n = kind == 0 ? t.nh[q] : (kind == 1 ? t.nb[q] : t.np[q]);
+  const int64_t off = t.off[6 * q + (kind == 0 ? HHHH : (kind == 1 ? PHPH : PPPP))];
+  const double *__restrict__ A = w1 + off;
+  const double *__restrict__ B = (kind == 1 ? w2 : w3) + off;
+  double *__restrict__ Lq = L + off;
+  const double c = kind == 1 ? h1 : h1 * h2, d = kind == 1 ? h2 : 1.0;
+  const int i0 = p.ti * 32, j0 = p.tj * 32;
+  for (int e = threadIdx.x; e < 1024; e += 256) {
+    const int li = e & 31, lj = e >> 5;
+    const bool in1 = i0 + li < n && j0 + lj < n, in2 = j0 + li < n && i0 + lj < n;
+    const int64_t a1 = (i0 + li) + (int64_t)(j0 + lj) * n, a2 = (j0 + li) + (int64_t)(i0 + lj) * n;
+    Aij[li][lj] = in1 ? A[a1] : 0.0;
+    Bij[li][lj] = in1 ? B[a1] : 0.0;
+    Aji[li][lj] = in2 ? A[a2] : 0.0;
+    Bji[li][lj] = in2 ? B[a2] : 0.0;
+  }
+  __syncthreads();
+  for (int e = threadIdx.x; e < 1024; e += 256) {
+    const int li = e & 31, lj = e >> 5;
+    if (i0 + li < n && j0 + lj < n)
+      Lq[(i0 + li) + (int64_t)(j0 + lj) * n] =
+          (h1 * Aji[lj][li] - h2 * Aij[li][lj]) + (c * Bji[lj][li] - d * Bij[li][lj]);
+    if (p.ti != p.tj && j0 + li < n && i0 + lj < n)
+      Lq[(j0 + li) + (int64_t)(i0 + lj) * n] =
+          (h1 * Aij[lj][li] - h2 * Aji[li][lj]) + (c * Bij[lj][li] - d * Bji[li][lj]);
+  }
+}
+
+void launch_ladder_sq(hipStream_t s, const DevTab &t, const double *w1, const double *w2, const double *w3,
+                      double *L, const TilePair *tp, int ntp, double h1, double h2) {
+  if (ntp <= 0) return;
+  hipLaunchKernelGGL(k_ladder_sq, dim3(ntp), dim3(256), 0, s, t, w1, w2, w3, L, tp, h1, h2);
 }
 
 }  // namespace qd

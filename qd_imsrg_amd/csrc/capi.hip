@@ -97,6 +97,8 @@ struct qd_context {
   double *fcls = nullptr;                                            // channel blocks of f (2 operands)
   TilePair *ptp = nullptr; int nptp = 0;  // 32x32 tile pairs of the Pandya combine
   TilePair *ftp = nullptr; int nftp = 0;  // 64x64 tile pairs of the fused Pandya product + combine
+  TilePair *lsq = nullptr; int nlsq = 0;  // 32x32 tile pairs of the square sub-blocks' ladder terms
+  double *wl = nullptr;                   // their ladder terms (square sub-blocks of the layout)
   Seg2 *pphh_segs = nullptr; Tile2 *pphh_tiles = nullptr; int npphh_segs = 0, npphh_tiles = 0;
   Seg2 *cc_segs = nullptr; Tile2 *cc_tiles = nullptr; int ncc_tiles = 0;
   GemmList g_all, g_acc;  // products (ladder + Pandya) and accumulating products
@@ -593,6 +595,24 @@ int build_worklists(qd_context *c) {
     if ((rc = dev_upload(c, ftp, &c->ftp))) return rc;
     c->nftp = (int)ftp.size();
   }
+  {  // square sub-blocks (HHHH, PHPH, PPPP) of every block, largest first
+    std::vector<std::pair<int64_t, int>> order;
+    for (int q = 0; q < ch.nblock; q++) {
+      const int n3[3] = {ch.nh[q], ch.nb[q], ch.np[q]};
+      for (int k = 0; k < 3; k++)
+        if (n3[k] > 0) order.push_back({(int64_t)n3[k] * n3[k], 3 * q + k});
+    }
+    std::stable_sort(order.begin(), order.end(), [](auto &a, auto &b) { return a.first > b.first; });
+    std::vector<TilePair> lsq;
+    for (auto &o : order) {
+      const int q = o.second / 3, k = o.second % 3;
+      const int n = k == 0 ? ch.nh[q] : (k == 1 ? ch.nb[q] : ch.np[q]), nt = (n + 31) / 32;
+      for (int ti = 0; ti < nt; ti++)
+        for (int tj = ti; tj < nt; tj++) lsq.push_back({o.second, ti, tj});
+    }
+    if ((rc = dev_upload(c, lsq, &c->lsq))) return rc;
+    c->nlsq = (int)lsq.size();
+  }
   if ((rc = upload_gemm(c, g1, npre, f1, c->g_all))) return rc;
   if ((rc = upload_gemm(c, g2, (int)g2.probs.size(), g2.flops, c->g_acc))) return rc;
   // ---- one-body products of commutator_111 on their own (16 x 16 FMA tiles,
@@ -752,18 +772,28 @@ int run_acc(qd_context *c, const qd_op *r1, const qd_op *r2, qd_op *out) {
 // operands, so they may run early), TB_PART_MAIN = every stored output (1b x
 // 2b, ladder and Pandya terms), adding to what the slabs wrote.
 enum { TB_PART_PRE = 1, TB_PART_MAIN = 2 };
+// host-only flag of run_twobody: the square sub-blocks' ladder terms are
+// already in wl (launched by the caller right after the products)
+constexpr int TB_LSQ_DONE = 16;
+void run_ladder_sq(qd_context *c, const qd_op *r1, const qd_op *r2) {
+  launch_ladder_sq(c->stream, c->dt, c->w1, c->w2, c->w3, c->wl, c->lsq, c->nlsq, r1->herm, r2->herm);
+}
 int run_twobody(qd_context *c, const qd_op *r1, const qd_op *r2, qd_op *out, int flags, const qd_cc *hcc,
                 const qd_cc *ecc, int part = TB_PART_PRE | TB_PART_MAIN) {
   Ops o{r1->d, r2->d, r1->herm, r2->herm};
   Timer tm(c, TC_TWOBODY);
+  const bool lsq_done = (flags & TB_LSQ_DONE) != 0;
+  flags &= ~TB_LSQ_DONE;
   if ((part & TB_PART_PRE) && (flags & TB_122)) {
     launch_fclass(c->stream, c->dt, o, c->fcls);
-    launch_box2b(c->stream, c->dt, o, out->d, c->bt2, c->nbt2, c->blist, flags, c->w1, c->w2, c->w3, c->fcls,
-                 nullptr);
+    launch_box2b(c->stream, c->dt, o, out->d, c->bt2, c->nbt2, c->blist, flags, c->w1, c->w2, c->w3, c->wl,
+                 c->fcls, nullptr);
   }
-  if (part & TB_PART_MAIN)
-    launch_box2b(c->stream, c->dt, o, out->d, c->bt1, c->nbt1, c->blist, flags, c->w1, c->w2, c->w3, c->fcls,
-                 (flags & TB_PANDYA) ? c->P : nullptr);
+  if (part & TB_PART_MAIN) {
+    if ((flags & TB_LADDER) && !lsq_done) run_ladder_sq(c, r1, r2);
+    launch_box2b(c->stream, c->dt, o, out->d, c->bt1, c->nbt1, c->blist, flags, c->w1, c->w2, c->w3, c->wl,
+                 c->fcls, (flags & TB_PANDYA) ? c->P : nullptr);
+  }
   HIPC(hipGetLastError());
   return QD_OK;
 }
@@ -841,12 +871,14 @@ int full_commutator(qd_context *c, const qd_op *r1, const qd_op *r2, qd_op *r3, 
     Slots s = make_slots(c, r1, r2, r3, ecc, hcc);
     Timer tm(c, TC_PRODUCTS);
     if ((rc = run_gemm_list(c, c->g_all, false, s))) return rc;  // ladder (and one-body) products
+    if (side) run_ladder_sq(c, r1, r2);  // while the CC fills finish on the side stream
     if (side) HIPC(hipStreamWaitEvent(main, c->ev_cc, 0));
     launch_pandya_fused(c->stream, c->dt, ecc->d, hcc->d, c->P, c->ftp, c->nftp, hcc->herm, ecc->herm);
     HIPC(hipGetLastError());
   } else {
     if (side) HIPC(hipStreamWaitEvent(main, c->ev_cc, 0));
     if ((rc = run_products(c, r1, r2, r3, hcc, ecc))) return rc;
+    if (side) run_ladder_sq(c, r1, r2);
   }
   if (!side && (rc = run_acc(c, r1, r2, r3))) return rc;
   // one-body and zero-body parts
@@ -870,7 +902,9 @@ int full_commutator(qd_context *c, const qd_op *r1, const qd_op *r2, qd_op *r3, 
     HIPC(hipStreamWaitEvent(main, c->ev_acc, 0));
   }
   if (rc) return rc;
-  if ((rc = run_twobody(c, r1, r2, r3, tbf, hcc, ecc, side ? TB_PART_MAIN : TB_PART_PRE | TB_PART_MAIN))) return rc;
+  if ((rc = run_twobody(c, r1, r2, r3, side ? tbf | TB_LSQ_DONE : tbf, hcc, ecc,
+                        side ? TB_PART_MAIN : TB_PART_PRE | TB_PART_MAIN)))
+    return rc;
   if (side) HIPC(hipStreamWaitEvent(main, c->ev_join, 0));
   return QD_OK;
 }
@@ -1013,6 +1047,7 @@ int qd_context_create(int nbody, int nshell, qd_context **out) {
   if ((rc = dev_alloc(c, sizeof(double) * ch.neq, &p))) return bail(rc); c->w1 = (double *)p;
   if ((rc = dev_alloc(c, sizeof(double) * ch.neq, &p))) return bail(rc); c->w2 = (double *)p;
   if ((rc = dev_alloc(c, sizeof(double) * ch.neq, &p))) return bail(rc); c->w3 = (double *)p;
+  if ((rc = dev_alloc(c, sizeof(double) * ch.neq, &p))) return bail(rc); c->wl = (double *)p;
   if ((rc = dev_alloc(c, sizeof(double) * ch.p_size, &p))) return bail(rc); c->P = (double *)p;
   if ((rc = dev_alloc(c, sizeof(double) * 2 * std::max(ch.fcls_size, 1), &p))) return bail(rc);
   c->fcls = (double *)p;

@@ -117,11 +117,19 @@ constexpr int BX_CMAX = 128;   // column grid entries per task (a class pair of 
 // r1, F[fcls_size, 2 fcls_size) of r2
 void launch_fclass(hipStream_t s, const DevTab &t, const Ops &o, double *F);
 // all stored two-body outputs of the tasks: 1b x 2b (TB_122), ladder
-// combinations of the products w1..w3 (TB_LADDER), Pandya term from D
+// combinations of the products w1..w3 and, for the square sub-blocks, of L
+// (TB_LADDER; launch_ladder_sq has filled L), Pandya term from D
 // (TB_PANDYA); TB_ACCUM adds to out (mode 2 tasks always add)
 void launch_box2b(hipStream_t s, const DevTab &t, const Ops &o, double *out, const BTask *tasks, int ntasks,
                   const int *blist, int flags, const double *w1, const double *w2, const double *w3,
-                  const double *F, const double *D);
+                  const double *L, const double *F, const double *D);
+// Ladder terms of the square sub-blocks (HHHH, PHPH, PPPP; they need the
+// products at (J, I) as well as (I, J)) into L, over 32x32 tile pairs (ti <=
+// tj; TilePair::blk = 3 * block + 0 HHHH / 1 PHPH / 2 PPPP):
+// HHHH, PPPP: L = (h1 w1^T - h2 w1) + (h1 h2 w3^T - w3);
+// PHPH:       L = (h1 w2^T - h2 w2) + (h1 w1^T - h2 w1)
+void launch_ladder_sq(hipStream_t s, const DevTab &t, const double *w1, const double *w2, const double *w3,
+                      double *L, const TilePair *tp, int ntp, double h1, double h2);
 // cc1 / cc2: calc_cc of r1 / r2 when the caller holds them (then the CC
 // mat-vecs of commutator_121 read them instead of gathering V), else null
 // OB_111 needs the one-body products of the one-body product list in the
