@@ -106,9 +106,10 @@ void launch_gemm(hipStream_t s, const GemmProb *probs, const GemmSeg *segs, cons
 // mode-2 tasks wrote; mode 2: one box, rows lo .. lo+cnt-1, writing the
 // ket-side 1b x 2b terms (their own launch, before the others).
 struct BTask { int block, gcol, first, ng, mode, lo, cnt; };
-// LDS per work-group stays under 32 KB (five per CU); channels of up to 10
-// states (R = 20 shells) fit the channel-block area
-constexpr int BX_VCAP = 1320;  // grid elements per operand staged in LDS
+// LDS per work-group stays under 40 KB (four per CU, as many as the kernel's
+// registers allow); channels of up to 10 states (R = 20 shells) fit the
+// channel-block area
+constexpr int BX_VCAP = 1800;  // grid elements per operand staged in LDS
 constexpr int BX_FCAP = 800;   // doubles of one-body channel blocks staged in LDS
 constexpr int BX_GMAX = 32;    // boxes per task
 constexpr int BX_RCAP = 256;   // row grid entries per task (staged in LDS)
