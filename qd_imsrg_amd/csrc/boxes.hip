@@ -195,29 +195,32 @@ __global__ void __launch_bounds__(256) k_box2b(DevTab t, Ops o, double *__restri
     else if (tyc == 1) sub = tyr == 1 ? PHPH : PPPH;
     else sub = PPPP;
     const int64_t idx = soff[tyr][tyc] + Il + (int64_t)Jl * rows;
-    // global loads first, so that their latency overlaps the LDS contraction
+    // global loads first, so that their latency overlaps the LDS contraction.
+    // Every load is issued unconditionally and nothing reads its value before
+    // the combination: loads under divergent branches share registers across
+    // the paths, and the compiler then drains all outstanding loads between
+    // them.  A term without a D block reads the zero at D[p_size].
     double d0 = 0.0, d1 = 0.0, d2 = 0.0, d3 = 0.0;
     if (dopd) {
-      const int64_t b0 = pb[b][0], b1 = pb[b][1], b2 = pb[b][2], b3 = pb[b][3];
-      if (b0 >= 0) d0 = D[b0 + i3 * s1 + i1 + (int64_t)(i2 * s4 + i4) * pl[b][0]];
-      if (b1 >= 0) d1 = D[b1 + i3 * s2 + i2 + (int64_t)(i1 * s4 + i4) * pl[b][1]];
-      if (b2 >= 0) d2 = D[b2 + i4 * s2 + i2 + (int64_t)(i1 * s3 + i3) * pl[b][2]];
-      if (b3 >= 0) d3 = D[b3 + i4 * s1 + i1 + (int64_t)(i2 * s3 + i3) * pl[b][3]];
+      const int64_t b0 = pb[b][0], b1 = pb[b][1], b2 = pb[b][2], b3 = pb[b][3], z = t.p_size;
+      d0 = D[b0 >= 0 ? b0 + i3 * s1 + i1 + (int64_t)(i2 * s4 + i4) * pl[b][0] : z];
+      d1 = D[b1 >= 0 ? b1 + i3 * s2 + i2 + (int64_t)(i1 * s4 + i4) * pl[b][1] : z];
+      d2 = D[b2 >= 0 ? b2 + i4 * s2 + i2 + (int64_t)(i1 * s3 + i3) * pl[b][2] : z];
+      d3 = D[b3 >= 0 ? b3 + i4 * s1 + i1 + (int64_t)(i2 * s3 + i3) * pl[b][3] : z];
     }
     // ladder inputs (signed combinations of the products of matrix_mult222,
     // src/commutators.f90:999-1127, and the summed products of w3 for Vpphh,
     // Vphhh, Vppph): xa = w1(I,J), xb = w3(I,J), xc = w2(I,J) for PHHH /
-    // PPPH; the square sub-blocks read their whole ladder term from L
+    // PPPH; the square sub-blocks read their whole ladder term from L.  A
+    // value the sub-block does not use is read at element 0 and ignored.
     const bool lad = first && (flags & TB_LADDER);
     const bool sq = sub == HHHH || sub == PPPP || sub == PHPH;
+    const bool usec = sub == PHHH || sub == PPPH;
     double xa = 0.0, xb = 0.0, xc = 0.0;
     if (lad) {
-      if (sq) xa = L[idx];
-      else {
-        if (sub != PPHH) xa = w1[idx];
-        xb = w3[idx];
-        if (sub == PHHH || sub == PPPH) xc = w2[idx];
-      }
+      xa = (sq ? L : w1)[sub == PPHH ? 0 : idx];
+      xb = w3[sq ? 0 : idx];
+      xc = w2[usec ? idx : 0];
     }
     // the bra slabs of a split box add to what the ket slabs wrote
     const bool rd = first ? ((flags & TB_ACCUM) || (mode == 1 && do122)) : (flags & TB_ACCUM) != 0;
@@ -302,14 +305,30 @@ __global__ void __launch_bounds__(256) k_ladder_sq(DevTab t, const double *__res
   double *__restrict__ Lq = L + off;
   const double c = kind == 1 ? h1 : h1 * h2, d = kind == 1 ? h2 : 1.0;
   const int i0 = p.ti * 32, j0 = p.tj * 32;
-  for (int e = threadIdx.x; e < 1024; e += 256) {
-    const int li = e & 31, lj = e >> 5;
+  // all sixteen loads of a thread in flight together, then the LDS stores
+  double ra[4][4];
+#pragma unroll
+  for (int u = 0; u < 4; u++) {
+    const int e = threadIdx.x + 256 * u, li = e & 31, lj = e >> 5;
     const bool in1 = i0 + li < n && j0 + lj < n, in2 = j0 + li < n && i0 + lj < n;
-    const int64_t a1 = (i0 + li) + (int64_t)(j0 + lj) * n, a2 = (j0 + li) + (int64_t)(i0 + lj) * n;
-    Aij[li][lj] = in1 ? A[a1] : 0.0;
-    Bij[li][lj] = in1 ? B[a1] : 0.0;
-    Aji[li][lj] = in2 ? A[a2] : 0.0;
-    Bji[li][lj] = in2 ? B[a2] : 0.0;
+    // unconditional loads at an in-bounds address (element 0 outside the
+    // block), so that no branch splits them
+    const int64_t a1 = in1 ? (i0 + li) + (int64_t)(j0 + lj) * n : 0;
+    const int64_t a2 = in2 ? (j0 + li) + (int64_t)(i0 + lj) * n : 0;
+    ra[u][0] = A[a1];
+    ra[u][1] = B[a1];
+    ra[u][2] = A[a2];
+    ra[u][3] = B[a2];
+    if (!in1) ra[u][0] = ra[u][1] = 0.0;
+    if (!in2) ra[u][2] = ra[u][3] = 0.0;
+  }
+#pragma unroll
+  for (int u = 0; u < 4; u++) {
+    const int e = threadIdx.x + 256 * u, li = e & 31, lj = e >> 5;
+    Aij[li][lj] = ra[u][0];
+    Bij[li][lj] = ra[u][1];
+    Aji[li][lj] = ra[u][2];
+    Bji[li][lj] = ra[u][3];
   }
   __syncthreads();
   for (int e = threadIdx.x; e < 1024; e += 256) {

@@ -1037,6 +1037,7 @@ int qd_context_create(int nbody, int nshell, qd_context **out) {
   UP(ch.fcls_off, fcls_off);
   t.fcls_size = ch.fcls_size;
   t.cc_size = ch.cc_size;
+  t.p_size = ch.p_size;
   t.cc_gmap = nullptr;
   if (ch.neq < (int64_t)CC_ZERO) {
     const std::vector<uint32_t> gmap = cc_gather_map(ch);
@@ -1048,7 +1049,9 @@ int qd_context_create(int nbody, int nshell, qd_context **out) {
   if ((rc = dev_alloc(c, sizeof(double) * ch.neq, &p))) return bail(rc); c->w2 = (double *)p;
   if ((rc = dev_alloc(c, sizeof(double) * ch.neq, &p))) return bail(rc); c->w3 = (double *)p;
   if ((rc = dev_alloc(c, sizeof(double) * ch.neq, &p))) return bail(rc); c->wl = (double *)p;
-  if ((rc = dev_alloc(c, sizeof(double) * ch.p_size, &p))) return bail(rc); c->P = (double *)p;
+  // one element more than the Pandya products: D[p_size] stays zero (the box
+  // pass reads it for a term without a D block)
+  if ((rc = dev_alloc(c, sizeof(double) * (ch.p_size + 1), &p))) return bail(rc); c->P = (double *)p;
   if ((rc = dev_alloc(c, sizeof(double) * 2 * std::max(ch.fcls_size, 1), &p))) return bail(rc);
   c->fcls = (double *)p;
   if (const char *env = getenv("QD_BOX_CAP")) {

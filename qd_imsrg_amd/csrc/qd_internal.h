@@ -111,6 +111,7 @@ struct DevTab {
   // index bits for an element that is zero
   const uint32_t *cc_gmap;
   int64_t cc_size;
+  int64_t p_size;  // Pandya product entries; element p_size of D is a zero
 };
 constexpr uint32_t CC_ZERO = 0x3fffffffu;
 
