@@ -146,27 +146,41 @@ __global__ void __launch_bounds__(256) k_box2b(DevTab t, Ops o, double *__restri
         fl[B.w + s1 * s1 + k] = make_double2(F[t.fcls_off[A2] + k], F[fs + t.fcls_off[A2] + k]);
     }
     // ordered grids: V(x1,x2,x3,x4) = sign * (herm if stored transposed) *
-    // stored element of the canonical pairs, 0 for x1 == x2 or x3 == x4
-    for (int e = threadIdx.x, b = 0; e < total; e += 256) {
-      if (b + 1 < ng && pre[b + 1] <= e) b = box_of(pre, b + 1, ng, e);
-      const int4 B = bx[b];
-      const int le = e - pre[b], c = qdiv(le, brc[b].x), r = le - c * B.y;
-      const uint32_t er = rent[bs[b].w + r], ec = cent[c];
-      double v1 = 0.0, v2 = 0.0;
-      if ((er >> 17) & (ec >> 17) & 1u) {
+    // stored element of the canonical pairs, 0 for x1 == x2 or x3 == x4.
+    // Four elements per thread and pass: all their loads are issued
+    // branch-free (past the end the last element is read again; an invalid
+    // entry reads element 0) before the first LDS store, so the round trips
+    // overlap; validity and sign are applied at the store.
+    for (int e0 = threadIdx.x, b = 0; e0 < total; e0 += 1024) {
+      double x1[4], x2[4], g1[4], g2[4];
+      int dst[4];
+      bool ok[4];
+#pragma unroll
+      for (int u = 0; u < 4; u++) {
+        const int e = min(e0 + 256 * u, total - 1);
+        if (b + 1 < ng && pre[b + 1] <= e) b = box_of(pre, b + 1, ng, e);
+        const int4 B = bx[b];
+        const int le = e - pre[b], c = qdiv(le, brc[b].x), r = le - c * B.y;
+        const uint32_t er = rent[bs[b].w + r], ec = cent[c];
         // stored element of (I, J), or of (J, I) times herm when the row type
         // is below the column type (branch-free over the pair types)
         const int tyr = (int)((er >> 18) & 3u), tyc = (int)((ec >> 18) & 3u);
         const int Il = (int)(er & 0xffffu) - tstart[tyr], Jl = (int)(ec & 0xffffu) - tstart[tyc];
         const bool tr = tyr < tyc;
         const int tA = tr ? tyc : tyr, tB = tr ? tyr : tyc;
-        const int64_t a = soff[tA][tB] + (tr ? Jl : Il) + (int64_t)(tr ? Il : Jl) * tcount[tA];
+        ok[u] = (er >> 17) & (ec >> 17) & 1u;
+        const int64_t a = ok[u] ? soff[tA][tB] + (tr ? Jl : Il) + (int64_t)(tr ? Il : Jl) * tcount[tA] : 0;
         const bool neg = ((er ^ ec) >> 16) & 1u;
-        v1 = tr ? o.h1 * o.v1[a] : o.v1[a];
-        v2 = tr ? o.h2 * o.v2[a] : o.v2[a];
-        if (neg) { v1 = -v1; v2 = -v2; }
+        x1[u] = o.v1[a];
+        x2[u] = o.v2[a];
+        g1[u] = (neg ? -1.0 : 1.0) * (tr ? (double)o.h1 : 1.0);
+        g2[u] = (neg ? -1.0 : 1.0) * (tr ? (double)o.h2 : 1.0);
+        dst[u] = B.z + r * ldc + c;
       }
-      gg[B.z + r * ldc + c] = make_double2(v1, v2);
+#pragma unroll
+      for (int u = 0; u < 4; u++)
+        if (e0 + 256 * u < total)
+          gg[dst[u]] = ok[u] ? make_double2(g1[u] * x1[u], g2[u] * x2[u]) : make_double2(0.0, 0.0);
     }
     __syncthreads();
   }
