@@ -69,6 +69,8 @@ __global__ void __launch_bounds__(256) k_box2b(DevTab t, Ops o, double *__restri
   __shared__ float2 brc[BX_GMAX];              // 1 / rows in the task, 1 / |A2|
   __shared__ int64_t pb[BX_GMAX][4];           // Pandya: D offset of each term's sub-block (-1: no D block)
   __shared__ int pl[BX_GMAX][4];               // Pandya: leading dimension of each term's D block
+  __shared__ int2 bfo[BX_GMAX];                // offsets of the box's A1 and A2 blocks in F
+  __shared__ int fsz_s;                        // f entries staged (A3, A4 and every box's A1, A2)
   const BTask tk = tasks[blockIdx.x];
   if (tk.block < 0) return;
   const int q = tk.block, mode = tk.mode, ng = tk.ng;
@@ -102,8 +104,10 @@ __global__ void __launch_bounds__(256) k_box2b(DevTab t, Ops o, double *__restri
       acc += nr * nc;
       goff += nr * ldc;
       foff += s1 * s1 + s2 * s2;
+      bfo[b] = make_int2(t.fcls_off[A1], t.fcls_off[A2]);
     }
     pre[ng] = acc;
+    fsz_s = foff;
   }
   if (dopd && (int)threadIdx.x < ng) {
     // D(x3,x1|x2,x4), D(x3,x2|x1,x4), D(x4,x2|x1,x3), D(x4,x1|x2,x3): device
@@ -128,22 +132,38 @@ __global__ void __launch_bounds__(256) k_box2b(DevTab t, Ops o, double *__restri
     for (int k = threadIdx.x; k < B.y; k += 256) rent[bs[b].w + k] = rg[k];
   }
   __syncthreads();
-  const int total = pre[ng];
+  const int total = pre[ng], fsz = fsz_s;
   if (do122) {
-    // channel blocks of f1 and f2
-    const int fs = t.fcls_size;
-    for (int k = threadIdx.x; k < s3 * s3; k += 256)
-      fl[k] = make_double2(F[t.fcls_off[A3] + k], F[fs + t.fcls_off[A3] + k]);
-    for (int k = threadIdx.x; k < s4 * s4; k += 256)
-      fl[s3 * s3 + k] = make_double2(F[t.fcls_off[A4] + k], F[fs + t.fcls_off[A4] + k]);
-    for (int b = 0; b < ng; b++) {
-      const int4 B = bx[b];
-      const int A1 = t.cp_a[B.x], A2 = t.cp_b[B.x];
-      const int s1 = t.ch_start[A1 + 1] - t.ch_start[A1], s2 = t.ch_start[A2 + 1] - t.ch_start[A2];
-      for (int k = threadIdx.x; k < s1 * s1; k += 256)
-        fl[B.w + k] = make_double2(F[t.fcls_off[A1] + k], F[fs + t.fcls_off[A1] + k]);
-      for (int k = threadIdx.x; k < s2 * s2; k += 256)
-        fl[B.w + s1 * s1 + k] = make_double2(F[t.fcls_off[A2] + k], F[fs + t.fcls_off[A2] + k]);
+    // channel blocks of f1 and f2 (A3, A4, then A1, A2 of every box) in one
+    // flat pass, two entries per thread at a time with both loads issued
+    // before the LDS stores
+    const int fs = t.fcls_size, q3 = s3 * s3, q34 = q3 + s4 * s4, ftot = fsz;
+    const int f3 = t.fcls_off[A3], f4 = t.fcls_off[A4];
+    for (int k0 = threadIdx.x; k0 < ftot; k0 += 512) {
+      double y1[2], y2[2];
+      int kk[2];
+#pragma unroll
+      for (int u = 0; u < 2; u++) {
+        const int k = min(k0 + 256 * u, ftot - 1);
+        int src;
+        if (k < q34) src = k < q3 ? f3 + k : f4 + (k - q3);
+        else {
+          int lo = 0, hi = ng - 1;  // last box whose blocks start at or before k
+          while (lo < hi) {
+            const int mid = (lo + hi + 1) >> 1;
+            if (bx[mid].w <= k) lo = mid;
+            else hi = mid - 1;
+          }
+          const int j = k - bx[lo].w, q1 = bs[lo].x * bs[lo].x;
+          src = j < q1 ? bfo[lo].x + j : bfo[lo].y + (j - q1);
+        }
+        y1[u] = F[src];
+        y2[u] = F[fs + src];
+        kk[u] = k;
+      }
+#pragma unroll
+      for (int u = 0; u < 2; u++)
+        if (k0 + 256 * u < ftot) fl[kk[u]] = make_double2(y1[u], y2[u]);
     }
     // ordered grids: V(x1,x2,x3,x4) = sign * (herm if stored transposed) *
     // stored element of the canonical pairs, 0 for x1 == x2 or x3 == x4.
