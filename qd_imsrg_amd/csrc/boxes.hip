@@ -71,6 +71,8 @@ __global__ void __launch_bounds__(256) k_box2b(DevTab t, Ops o, double *__restri
   __shared__ int pl[BX_GMAX][4];               // Pandya: leading dimension of each term's D block
   __shared__ int2 bfo[BX_GMAX];                // offsets of the box's A1 and A2 blocks in F
   __shared__ int fsz_s;                        // f entries staged (A3, A4 and every box's A1, A2)
+  __shared__ int bgo[BX_GMAX];                 // first row grid entry of the box in cp_grid
+  __shared__ int rtot_s;                       // row grid entries of the task
   const BTask tk = tasks[blockIdx.x];
   if (tk.block < 0) return;
   const int q = tk.block, mode = tk.mode, ng = tk.ng;
@@ -89,47 +91,80 @@ __global__ void __launch_bounds__(256) k_box2b(DevTab t, Ops o, double *__restri
     soff[tA][tB] = tA >= tB ? t.off[6 * q + sub] : 0;
   }
   for (int k = threadIdx.x; k < nc; k += 256) cent[k] = cgrid[k];
-  if (threadIdx.x == 0) {
-    int acc = 0, goff = 0, foff = s3 * s3 + s4 * s4, roff = 0;
-    for (int b = 0; b < ng; b++) {
-      const int g = blist[tk.first + b];
-      const int A1 = t.cp_a[g], A2 = t.cp_b[g];
-      const int s1 = t.ch_start[A1 + 1] - t.ch_start[A1], s2 = t.ch_start[A2 + 1] - t.ch_start[A2];
-      const int nr = mode == 2 ? tk.cnt : s1 * s2;
-      bx[b] = make_int4(g, nr, goff, foff);
-      bs[b] = make_int4(s1, s2, A1 == A2, roff);
-      roff += nr;
-      brc[b] = make_float2(1.0f / (float)nr, 1.0f / (float)s2);
-      pre[b] = acc;
-      acc += nr * nc;
-      goff += nr * ldc;
-      foff += s1 * s1 + s2 * s2;
-      bfo[b] = make_int2(t.fcls_off[A1], t.fcls_off[A2]);
+  if (threadIdx.x < 64) {
+    // wave 0, one lane per box: the box's tables (three dependent loads for
+    // all boxes at once), then its offsets by exclusive prefix sums over the
+    // lanes (ng <= 32)
+    const int b = threadIdx.x;
+    int g = 0, A1 = 0, A2 = 0, s1 = 1, s2 = 1, f1 = 0, f2 = 0, go = 0;
+    if (b < ng) {
+      g = blist[tk.first + b];
+      A1 = t.cp_a[g];
+      A2 = t.cp_b[g];
+      go = t.cp_grid_off[g];
+      s1 = t.ch_start[A1 + 1] - t.ch_start[A1];
+      s2 = t.ch_start[A2 + 1] - t.ch_start[A2];
+      f1 = t.fcls_off[A1];
+      f2 = t.fcls_off[A2];
     }
-    pre[ng] = acc;
-    fsz_s = foff;
-  }
-  if (dopd && (int)threadIdx.x < ng) {
-    // D(x3,x1|x2,x4), D(x3,x2|x1,x4), D(x4,x2|x1,x3), D(x4,x1|x2,x3): device
-    // rows of a CC block are ordered by the channel of the first state, so
-    // each term's box is a dense sub-block at a fixed offset
-    const int g = blist[tk.first + threadIdx.x];
+    const int nr = b < ng ? (mode == 2 ? tk.cnt : s1 * s2) : 0;
+    const int fn = b < ng ? s1 * s1 + s2 * s2 : 0;
+    int e = nr * nc, gz = nr * ldc, fz = fn, rz = nr;  // inclusive sums
+    for (int d = 1; d < 64; d <<= 1) {
+      const int e2 = __shfl_up(e, d, 64), g2 = __shfl_up(gz, d, 64), f2s = __shfl_up(fz, d, 64),
+                r2 = __shfl_up(rz, d, 64);
+      if (b >= d) {
+        e += e2;
+        gz += g2;
+        fz += f2s;
+        rz += r2;
+      }
+    }
+    const int q34 = s3 * s3 + s4 * s4;
+    if (b < ng) {
+      bx[b] = make_int4(g, nr, gz - nr * ldc, q34 + fz - fn);
+      bs[b] = make_int4(s1, s2, A1 == A2, rz - nr);
+      brc[b] = make_float2(1.0f / (float)nr, 1.0f / (float)s2);
+      pre[b] = e - nr * nc;
+      bfo[b] = make_int2(f1, f2);
+      bgo[b] = go + r_off;
+    }
+    if (b == ng - 1) {
+      pre[ng] = e;
+      fsz_s = q34 + fz;
+      rtot_s = rz;
+    }
+  } else if (dopd && (int)threadIdx.x - 64 < ng) {
+    // wave 1, one lane per box: D(x3,x1|x2,x4), D(x3,x2|x1,x4),
+    // D(x4,x2|x1,x3), D(x4,x1|x2,x3): device rows of a CC block are ordered
+    // by the channel of the first state, so each term's box is a dense
+    // sub-block at a fixed offset
+    const int b = threadIdx.x - 64;
+    const int g = blist[tk.first + b];
     const int A1 = t.cp_a[g], A2 = t.cp_b[g];
     const int rc0[4] = {A3, A3, A4, A4}, rc1[4] = {A1, A2, A2, A1}, cc0[4] = {A2, A1, A1, A2};
     for (int k = 0; k < 4; k++) {
       const int x = t.ch_states[t.ch_start[rc0[k]]], y = t.ch_states[t.ch_start[rc1[k]]];
       const int K = t.cc_map[x * t.M + y].x;
-      pl[threadIdx.x][k] = t.cc_r[K];
-      pb[threadIdx.x][k] = t.cc_nb[K] == 0 ? -1
-                                           : t.cc_poff[K] + t.cc_rs[K * t.nch + rc0[k]] +
-                                                 (int64_t)t.cc_rs[K * t.nch + cc0[k]] * t.cc_r[K];
+      pl[b][k] = t.cc_r[K];
+      pb[b][k] = t.cc_nb[K] == 0 ? -1
+                                 : t.cc_poff[K] + t.cc_rs[K * t.nch + rc0[k]] +
+                                       (int64_t)t.cc_rs[K * t.nch + cc0[k]] * t.cc_r[K];
     }
   }
   __syncthreads();
-  for (int b = 0; b < ng; b++) {
-    const int4 B = bx[b];
-    const uint32_t *rg = t.cp_grid + t.cp_grid_off[B.x] + r_off;
-    for (int k = threadIdx.x; k < B.y; k += 256) rent[bs[b].w + k] = rg[k];
+  {
+    // row grid entries of all boxes in one flat pass
+    const int rtot = rtot_s;
+    for (int k = threadIdx.x; k < rtot; k += 256) {
+      int lo = 0, hi = ng - 1;  // last box whose rows start at or before k
+      while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (bs[mid].w <= k) lo = mid;
+        else hi = mid - 1;
+      }
+      rent[k] = t.cp_grid[bgo[lo] + (k - bs[lo].w)];
+    }
   }
   __syncthreads();
   const int total = pre[ng], fsz = fsz_s;
