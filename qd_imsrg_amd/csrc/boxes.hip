@@ -357,14 +357,15 @@ void launch_box2b(hipStream_t s, const DevTab &t, const Ops &o, double *out, con
 }
 
 // The square sub-blocks' ladder terms need the products at (J, I): one
-// work-group per pair of 32x32 tiles (ij, ji) stages both tiles of both
-// products through LDS, so that the box pass reads one coalesced value per
-// element instead of two scattered transposed ones.
+// work-group per pair of 32x32 tiles (ij, ji).  With U = h1 A + c B and
+// W = -h2 A - d B, L = U^T + W: only U goes through LDS (both tiles), W stays
+// in the registers of the thread that loaded it, so the box pass reads one
+// coalesced value per element instead of two scattered transposed ones.
 __global__ void __launch_bounds__(256) k_ladder_sq(DevTab t, const double *__restrict__ w1,
                                                    const double *__restrict__ w2, const double *__restrict__ w3,
                                                    double *__restrict__ L, const TilePair *__restrict__ tp, double h1,
                                                    double h2) {
-  __shared__ double Aij[32][33], Aji[32][33], Bij[32][33], Bji[32][33];
+  __shared__ double Uij[32][33], Uji[32][33];
   const TilePair p = tp[blockIdx.x];
   const int q = p.blk / 3, kind = p.blk % 3;
   const int n = kind == 0 ? t.nh[q] : (kind == 1 ? t.nb[q] : t.np[q]);
@@ -374,40 +375,35 @@ __global__ void __launch_bounds__(256) k_ladder_sq(DevTab t, const double *__res
   double *__restrict__ Lq = L + off;
   const double c = kind == 1 ? h1 : h1 * h2, d = kind == 1 ? h2 : 1.0;
   const int i0 = p.ti * 32, j0 = p.tj * 32;
-  // all sixteen loads of a thread in flight together, then the LDS stores
+  // all sixteen loads of a thread in flight together (unconditional, at an
+  // in-bounds address; element 0 outside the block), then the LDS stores
   double ra[4][4];
 #pragma unroll
   for (int u = 0; u < 4; u++) {
     const int e = threadIdx.x + 256 * u, li = e & 31, lj = e >> 5;
     const bool in1 = i0 + li < n && j0 + lj < n, in2 = j0 + li < n && i0 + lj < n;
-    // unconditional loads at an in-bounds address (element 0 outside the
-    // block), so that no branch splits them
     const int64_t a1 = in1 ? (i0 + li) + (int64_t)(j0 + lj) * n : 0;
     const int64_t a2 = in2 ? (j0 + li) + (int64_t)(i0 + lj) * n : 0;
     ra[u][0] = A[a1];
     ra[u][1] = B[a1];
     ra[u][2] = A[a2];
     ra[u][3] = B[a2];
-    if (!in1) ra[u][0] = ra[u][1] = 0.0;
-    if (!in2) ra[u][2] = ra[u][3] = 0.0;
   }
+  double wij[4], wji[4];
 #pragma unroll
   for (int u = 0; u < 4; u++) {
     const int e = threadIdx.x + 256 * u, li = e & 31, lj = e >> 5;
-    Aij[li][lj] = ra[u][0];
-    Bij[li][lj] = ra[u][1];
-    Aji[li][lj] = ra[u][2];
-    Bji[li][lj] = ra[u][3];
+    Uij[li][lj] = h1 * ra[u][0] + c * ra[u][1];
+    Uji[li][lj] = h1 * ra[u][2] + c * ra[u][3];
+    wij[u] = -h2 * ra[u][0] - d * ra[u][1];
+    wji[u] = -h2 * ra[u][2] - d * ra[u][3];
   }
   __syncthreads();
-  for (int e = threadIdx.x; e < 1024; e += 256) {
-    const int li = e & 31, lj = e >> 5;
-    if (i0 + li < n && j0 + lj < n)
-      Lq[(i0 + li) + (int64_t)(j0 + lj) * n] =
-          (h1 * Aji[lj][li] - h2 * Aij[li][lj]) + (c * Bji[lj][li] - d * Bij[li][lj]);
-    if (p.ti != p.tj && j0 + li < n && i0 + lj < n)
-      Lq[(j0 + li) + (int64_t)(i0 + lj) * n] =
-          (h1 * Aij[lj][li] - h2 * Aji[li][lj]) + (c * Bij[lj][li] - d * Bji[li][lj]);
+#pragma unroll
+  for (int u = 0; u < 4; u++) {
+    const int e = threadIdx.x + 256 * u, li = e & 31, lj = e >> 5;
+    if (i0 + li < n && j0 + lj < n) Lq[(i0 + li) + (int64_t)(j0 + lj) * n] = Uji[lj][li] + wij[u];
+    if (p.ti != p.tj && j0 + li < n && i0 + lj < n) Lq[(j0 + li) + (int64_t)(i0 + lj) * n] = Uij[lj][li] + wji[u];
   }
 }
 
