@@ -278,17 +278,19 @@ __global__ void __launch_bounds__(256) k_box2b(DevTab t, Ops o, double *__restri
       d3 = D[b3 >= 0 ? b3 + i4 * s1 + i1 + (int64_t)(i2 * s3 + i3) * pl[b][3] : z];
     }
     // ladder inputs (signed combinations of the products of matrix_mult222,
-    // src/commutators.f90:999-1127, and the summed products of w3 for Vpphh,
-    // Vphhh, Vppph): xa = w1(I,J), xb = w3(I,J), xc = w2(I,J) for PHHH /
-    // PPPH; the square sub-blocks read their whole ladder term from L.  A
-    // value the sub-block does not use is read at element 0 and ignored.
+    // src/commutators.f90:999-1127, and the summed products for Vpphh,
+    // Vphhh, Vppph): xa = w1(I,J), xb = the summed product at (I,J) (L for
+    // PPHH and PPPH, whose sums come in two halves; w3 for PHHH), xc =
+    // w2(I,J) for PHHH / PPPH; the square sub-blocks read their whole ladder
+    // term from L.  A value the sub-block does not use is read at element 0
+    // and ignored.
     const bool lad = first && (flags & TB_LADDER);
     const bool sq = sub == HHHH || sub == PPPP || sub == PHPH;
     const bool usec = sub == PHHH || sub == PPPH;
     double xa = 0.0, xb = 0.0, xc = 0.0;
     if (lad) {
       xa = (sq ? L : w1)[sub == PPHH ? 0 : idx];
-      xb = w3[sq ? 0 : idx];
+      xb = (sub == PHHH ? w3 : L)[sq ? 0 : idx];
       xc = w2[usec ? idx : 0];
     }
     // the bra slabs of a split box add to what the ket slabs wrote
@@ -354,6 +356,23 @@ void launch_box2b(hipStream_t s, const DevTab &t, const Ops &o, double *out, con
                   const double *L, const double *F, const double *D) {
   if (ntasks <= 0) return;
   hipLaunchKernelGGL(k_box2b, dim3(ntasks), dim3(256), 0, s, t, o, out, tasks, blist, flags, w1, w2, w3, L, F, D);
+}
+
+__global__ void __launch_bounds__(256) k_sum_halves(const double *__restrict__ w3, const double *__restrict__ w4,
+                                                    double *__restrict__ L, const Seg2 *__restrict__ segs,
+                                                    const Tile2 *__restrict__ tiles) {
+  const Tile2 tl = tiles[blockIdx.x];
+  const Seg2 sg = segs[tl.seg];
+  const int e = tl.start + threadIdx.x;
+  if (e >= sg.size) return;
+  const int64_t k = sg.off + e;
+  L[k] = w3[k] + w4[k];
+}
+
+void launch_sum_halves(hipStream_t s, const double *w3, const double *w4, double *L, const Seg2 *segs,
+                       const Tile2 *tiles, int ntiles) {
+  if (ntiles <= 0) return;
+  hipLaunchKernelGGL(k_sum_halves, dim3(ntiles), dim3(256), 0, s, w3, w4, L, segs, tiles);
 }
 
 // The square sub-blocks' ladder terms need the products at (J, I): one

@@ -87,7 +87,7 @@ struct qd_context {
   hipStream_t aux = nullptr;
   hipEvent_t ev_fork = nullptr, ev_join = nullptr, ev_cc = nullptr, ev_acc = nullptr;
   std::vector<void *> allocs;
-  double *w1 = nullptr, *w2 = nullptr, *w3 = nullptr, *P = nullptr;
+  double *w1 = nullptr, *w2 = nullptr, *w3 = nullptr, *w4 = nullptr, *P = nullptr;
   double *u1 = nullptr, *u2 = nullptr;  // cross-coupled mat-vecs of commutator_121 (M*M each)
   double *partials = nullptr, *scal = nullptr;  // reduction partials, 8 device scalars
   double *host_scal = nullptr;                  // pinned mirror of scal
@@ -100,6 +100,7 @@ struct qd_context {
   TilePair *lsq = nullptr; int nlsq = 0;  // 32x32 tile pairs of the square sub-blocks' ladder terms
   double *wl = nullptr;                   // their ladder terms (square sub-blocks of the layout)
   Seg2 *pphh_segs = nullptr; Tile2 *pphh_tiles = nullptr; int npphh_segs = 0, npphh_tiles = 0;
+  Seg2 *half_segs = nullptr; Tile2 *half_tiles = nullptr; int nhalf_tiles = 0;  // Vpphh, Vppph sub-blocks
   Seg2 *cc_segs = nullptr; Tile2 *cc_tiles = nullptr; int ncc_tiles = 0;
   GemmList g_all, g_acc;  // products (ladder + Pandya) and accumulating products
   GemmList g_1b;          // one-body products of commutator_111 (always FP64)
@@ -496,6 +497,21 @@ int build_worklists(qd_context *c) {
   }
   if ((rc = dev_upload(c, pseg, &c->pphh_segs))) return rc;
   if ((rc = dev_upload(c, ptiles, &c->pphh_tiles))) return rc;
+  {  // the sub-blocks whose summed products come in two halves (w3 + w4 -> wl)
+    std::vector<Seg2> hseg;
+    std::vector<Tile2> htiles;
+    for (int q = 0; q < ch.nblock; q++)
+      for (int sub : {PPHH, PPPH}) {
+        const int rows = ch.np[q], cols = sub == PPHH ? ch.nh[q] : ch.nb[q];
+        if (rows * cols == 0) continue;
+        Seg2 sg{ch.off[(size_t)q * 6 + sub], rows, rows * cols, q, sub, 0, 0};
+        add_tiles(htiles, (int)hseg.size(), sg.size);
+        hseg.push_back(sg);
+      }
+    if ((rc = dev_upload(c, hseg, &c->half_segs))) return rc;
+    if ((rc = dev_upload(c, htiles, &c->half_tiles))) return rc;
+    c->nhalf_tiles = (int)htiles.size();
+  }
   c->npphh_segs = (int)pseg.size();
   c->npphh_tiles = (int)ptiles.size();
   // ---- CC segments
@@ -539,10 +555,14 @@ int build_worklists(qd_context *c) {
     // the products that add to Vpphh, Vphhh, Vppph go to the same sub-blocks of
     // w3 (free there); the two-body pass adds them to r3
     // w3.pphh = r1.pppp r2.pphh - r2.pppp r1.pphh - r1.pphh r2.hhhh + r2.pphh r1.hhhh
+    // in two halves (w3, w4; their sum goes to wl, which the two-body pass
+    // reads), so that the longest tiles walk half the K
     g2.begin(S_W3, o[PPHH], np, np, nh, 0);
     g2.seg(S_R1, o[PPPP], np, 0, S_R2, o[PPHH], np, 0, np, 1);
-    g2.seg(S_R2, o[PPPP], np, 0, S_R1, o[PPHH], np, 0, np, -1);
     g2.seg(S_R1, o[PPHH], np, 0, S_R2, o[HHHH], nh, 0, nh, -1);
+    g2.end();
+    g2.begin(S_W4, o[PPHH], np, np, nh, 0);
+    g2.seg(S_R2, o[PPPP], np, 0, S_R1, o[PPHH], np, 0, np, -1);
     g2.seg(S_R2, o[PPHH], np, 0, S_R1, o[HHHH], nh, 0, nh, 1);
     g2.end();
     // w3.phhh = -r1.phhh r2.hhhh + r2.phhh r1.hhhh
@@ -553,6 +573,8 @@ int build_worklists(qd_context *c) {
     // w3.ppph = r1.pppp r2.ppph - r2.pppp r1.ppph
     g2.begin(S_W3, o[PPPH], np, np, nb, 0);
     g2.seg(S_R1, o[PPPP], np, 0, S_R2, o[PPPH], np, 0, np, 1);
+    g2.end();
+    g2.begin(S_W4, o[PPPH], np, np, nb, 0);
     g2.seg(S_R2, o[PPPP], np, 0, S_R1, o[PPPH], np, 0, np, -1);
     g2.end();
   }
@@ -657,7 +679,7 @@ Slots make_slots(qd_context *c, const qd_op *r1, const qd_op *r2, qd_op *out, co
   s.p[S_R1] = r1 ? r1->d : nullptr;
   s.p[S_R2] = r2 ? r2->d : nullptr;
   s.p[S_OUT] = out ? out->d : nullptr;
-  s.p[S_W1] = c->w1; s.p[S_W2] = c->w2; s.p[S_W3] = c->w3;
+  s.p[S_W1] = c->w1; s.p[S_W2] = c->w2; s.p[S_W3] = c->w3; s.p[S_W4] = c->w4;
   s.p[S_CC1] = ecc ? ecc->d : nullptr;
   s.p[S_CC2] = hcc ? hcc->d : nullptr;
   s.p[S_P] = c->P;
@@ -764,7 +786,13 @@ int run_products(qd_context *c, const qd_op *r1, const qd_op *r2, qd_op *out, co
 int run_acc(qd_context *c, const qd_op *r1, const qd_op *r2, qd_op *out) {
   Slots s = make_slots(c, r1, r2, out, nullptr, nullptr);
   Timer tm(c, TC_ACCUM);
-  return run_gemm_list(c, c->g_acc, false, s);
+  int rc = run_gemm_list(c, c->g_acc, false, s);
+  if (rc) return rc;
+  // the Vpphh and Vppph sums come in two halves (shorter K walks); their sum
+  // goes to the same sub-blocks of wl, which the two-body pass reads
+  launch_sum_halves(c->stream, c->w3, c->w4, c->wl, c->half_segs, c->half_tiles, c->nhalf_tiles);
+  HIPC(hipGetLastError());
+  return QD_OK;
 }
 
 // The two-body pass in two parts: TB_PART_PRE = the channel blocks of f and
@@ -1049,6 +1077,8 @@ int qd_context_create(int nbody, int nshell, qd_context **out) {
   if ((rc = dev_alloc(c, sizeof(double) * ch.neq, &p))) return bail(rc); c->w2 = (double *)p;
   if ((rc = dev_alloc(c, sizeof(double) * ch.neq, &p))) return bail(rc); c->w3 = (double *)p;
   if ((rc = dev_alloc(c, sizeof(double) * ch.neq, &p))) return bail(rc); c->wl = (double *)p;
+  // second halves of the summed products of Vpphh and Vppph
+  if ((rc = dev_alloc(c, sizeof(double) * ch.neq, &p))) return bail(rc); c->w4 = (double *)p;
   // one element more than the Pandya products: D[p_size] stays zero (the box
   // pass reads it for a term without a D block)
   if ((rc = dev_alloc(c, sizeof(double) * (ch.p_size + 1), &p))) return bail(rc); c->P = (double *)p;

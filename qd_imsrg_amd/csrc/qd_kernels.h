@@ -19,7 +19,7 @@ struct TilePair { int blk, ti, tj; };       // 32x32 tiles (ti, tj) and (tj, ti)
 
 // ---- grouped FP64 GEMM work lists
 // Operand slots resolved at launch time (the same lists serve every call).
-enum Slot { S_R1 = 0, S_R2, S_OUT, S_W1, S_W2, S_W3, S_CC1, S_CC2, S_P, NSLOT };
+enum Slot { S_R1 = 0, S_R2, S_OUT, S_W1, S_W2, S_W3, S_W4, S_CC1, S_CC2, S_P, NSLOT };
 struct Slots { double *p[NSLOT]; };
 struct GemmSeg {   // one product op(A)*op(B) of a problem, alpha = +-1
   int64_t a_off, b_off;
@@ -118,12 +118,17 @@ constexpr int BX_CMAX = 128;   // column grid entries per task (a class pair of 
 // r1, F[fcls_size, 2 fcls_size) of r2
 void launch_fclass(hipStream_t s, const DevTab &t, const Ops &o, double *F);
 // all stored two-body outputs of the tasks: 1b x 2b (TB_122), ladder
-// combinations of the products w1..w3 and, for the square sub-blocks, of L
+// combinations of the products w1..w3 and, for the square sub-blocks and
+// the summed products of Vpphh and Vppph, of L
 // (TB_LADDER; launch_ladder_sq has filled L), Pandya term from D
 // (TB_PANDYA); TB_ACCUM adds to out (mode 2 tasks always add)
 void launch_box2b(hipStream_t s, const DevTab &t, const Ops &o, double *out, const BTask *tasks, int ntasks,
                   const int *blist, int flags, const double *w1, const double *w2, const double *w3,
                   const double *L, const double *F, const double *D);
+// L = w3 + w4 over the given sub-blocks (the two halves of the summed
+// products of Vpphh and Vppph)
+void launch_sum_halves(hipStream_t s, const double *w3, const double *w4, double *L, const Seg2 *segs,
+                       const Tile2 *tiles, int ntiles);
 // Ladder terms of the square sub-blocks (HHHH, PHPH, PPPP; they need the
 // products at (J, I) as well as (I, J)) into L, over 32x32 tile pairs (ti <=
 // tj; TilePair::blk = 3 * block + 0 HHHH / 1 PHPH / 2 PPPP):
