@@ -106,6 +106,7 @@ struct qd_context {
   GemmList g_1b;          // one-body products of commutator_111 (always FP64)
   GemmTile *ob_tiles = nullptr; int nob_tiles = 0;  // their 16 x 16 tiles
   int *ob221 = nullptr; int nob221 = 0;  // one-body outputs whose states share a channel
+  uint32_t *g221 = nullptr;               // their trace elements as codes (velem_code), or null
   int launch_mode = 0;    // 0 grouped persistent launch, 1 one launch per channel problem
   int gemm_prec = GEMM_FP64;  // arithmetic of the 2b-2b GEMMs (GemmPrec)
   bool use_graphs = false;  // capture each BCH term (commutator + norms) in a hipGraph
@@ -188,6 +189,8 @@ struct Timer {  // brackets one launch with events when timing is on
     }
   }
 };
+
+uint32_t velem_code(const Channels &ch, int a1, int a2, int a3, int a4);
 
 void add_tiles(std::vector<Tile2> &tiles, int seg, int size) {
   for (int s = 0; s < size; s += 256) tiles.push_back({seg, s});
@@ -661,6 +664,23 @@ int build_worklists(qd_context *c) {
       if (ch.spch[x] == ch.spch[y]) l221.push_back((int)e);
     }
     c->nob221 = (int)l221.size();
+    if (ch.neq < (int64_t)CC_ZERO && !l221.empty()) {
+      // per listed output (x, y) and state a: the codes of <a x|w|a y> and
+      // <a y|w|a x> (w1, w2 share the layout; herm = 1)
+      std::vector<uint32_t> g((size_t)l221.size() * ch.M * 2);
+      for (size_t k = 0; k < l221.size(); k++) {
+        const int64_t e = l221[k];
+        int x, y;
+        if (e < (int64_t)n * n) { x = (int)(e % n); y = (int)(e / n); }
+        else if (e < (int64_t)n * n + (int64_t)mp * mp) { const int64_t f = e - (int64_t)n * n; x = n + (int)(f % mp); y = n + (int)(f / mp); }
+        else { const int64_t f = e - (int64_t)n * n - (int64_t)mp * mp; x = n + (int)(f % mp); y = (int)(f / mp); }
+        for (int a = 0; a < ch.M; a++) {
+          g[(k * ch.M + a) * 2] = velem_code(ch, a, x, a, y);
+          g[(k * ch.M + a) * 2 + 1] = velem_code(ch, a, y, a, x);
+        }
+      }
+      if ((rc = dev_upload(c, g, &c->g221))) return rc;
+    }
     if (l221.empty()) l221.push_back(0);
     if ((rc = dev_upload(c, l221, &c->ob221))) return rc;
   }
@@ -686,41 +706,43 @@ Slots make_slots(qd_context *c, const qd_op *r1, const qd_op *r2, qd_op *out, co
   return s;
 }
 
-// calc_cc (src/ME_general.f90:923-952) as a gather table: for the CC element
-// (row (i, j), column (k, l)) of every block in device order, the V element
-// <j k|V|i l> that v_elem (src/ME_general.f90:1557-1680) returns: the flat
-// index of the stored element after sorting both pairs (the sign of the
-// sorting in bit 31), bit 30 set where the element is stored transposed (the
-// herm factor applies), CC_ZERO for a repeated state or pairs in different
-// blocks.  Requires neq < CC_ZERO.
-std::vector<uint32_t> cc_gather_map(const Channels &ch) {
+// v_elem (src/ME_general.f90:1557-1680) as a code: the flat index of the
+// stored element of <a1 a2|V|a3 a4> after sorting both pairs | stored
+// transposed (the herm factor applies) << 30 | the sorting sign << 31, or
+// CC_ZERO for a repeated state or pairs in different blocks.
+uint32_t velem_code(const Channels &ch, int a1, int a2, int a3, int a4) {
   const int M = ch.M;
-  auto elem = [&](int a1, int a2, int a3, int a4) -> uint32_t {
-    if (a1 == a2 || a3 == a4) return CC_ZERO;
-    uint32_t sgn = 0;
-    if (a1 > a2) { std::swap(a1, a2); sgn ^= 1u; }
-    if (a3 > a4) { std::swap(a3, a4); sgn ^= 1u; }
-    const uint32_t I = ch.iarr[(size_t)a1 * M - (size_t)a1 * (a1 + 1) / 2 + (a2 - a1 - 1)];
-    const uint32_t J = ch.iarr[(size_t)a3 * M - (size_t)a3 * (a3 + 1) / 2 + (a4 - a3 - 1)];
-    const int q = ia_blk(I);
-    if (q != ia_blk(J)) return CC_ZERO;
-    const int64_t w1 = ia_idx(I), w2 = ia_idx(J), nh = ch.nh[q], np = ch.np[q], nb = ch.nb[q];
-    const int64_t *o = &ch.off[6 * (size_t)q];
-    int64_t k = 0;
-    uint32_t tr = 0;
-    switch (ia_type(I) * 4 + ia_type(J)) {
-      case T_HH * 4 + T_HH: k = o[HHHH] + w1 + w2 * nh; break;
-      case T_HH * 4 + T_PH: k = o[PHHH] + w2 + w1 * nb; tr = 1; break;
-      case T_HH * 4 + T_PP: k = o[PPHH] + w2 + w1 * np; tr = 1; break;
-      case T_PH * 4 + T_HH: k = o[PHHH] + w1 + w2 * nb; break;
-      case T_PH * 4 + T_PH: k = o[PHPH] + w1 + w2 * nb; break;
-      case T_PH * 4 + T_PP: k = o[PPPH] + w2 + w1 * np; tr = 1; break;
-      case T_PP * 4 + T_HH: k = o[PPHH] + w1 + w2 * np; break;
-      case T_PP * 4 + T_PH: k = o[PPPH] + w1 + w2 * np; break;
-      default: k = o[PPPP] + w1 + w2 * np; break;
-    }
-    return (uint32_t)k | (tr << 30) | (sgn << 31);
-  };
+  if (a1 == a2 || a3 == a4) return CC_ZERO;
+  uint32_t sgn = 0;
+  if (a1 > a2) { std::swap(a1, a2); sgn ^= 1u; }
+  if (a3 > a4) { std::swap(a3, a4); sgn ^= 1u; }
+  const uint32_t I = ch.iarr[(size_t)a1 * M - (size_t)a1 * (a1 + 1) / 2 + (a2 - a1 - 1)];
+  const uint32_t J = ch.iarr[(size_t)a3 * M - (size_t)a3 * (a3 + 1) / 2 + (a4 - a3 - 1)];
+  const int q = ia_blk(I);
+  if (q != ia_blk(J)) return CC_ZERO;
+  const int64_t w1 = ia_idx(I), w2 = ia_idx(J), nh = ch.nh[q], np = ch.np[q], nb = ch.nb[q];
+  const int64_t *o = &ch.off[6 * (size_t)q];
+  int64_t k = 0;
+  uint32_t tr = 0;
+  switch (ia_type(I) * 4 + ia_type(J)) {
+    case T_HH * 4 + T_HH: k = o[HHHH] + w1 + w2 * nh; break;
+    case T_HH * 4 + T_PH: k = o[PHHH] + w2 + w1 * nb; tr = 1; break;
+    case T_HH * 4 + T_PP: k = o[PPHH] + w2 + w1 * np; tr = 1; break;
+    case T_PH * 4 + T_HH: k = o[PHHH] + w1 + w2 * nb; break;
+    case T_PH * 4 + T_PH: k = o[PHPH] + w1 + w2 * nb; break;
+    case T_PH * 4 + T_PP: k = o[PPPH] + w2 + w1 * np; tr = 1; break;
+    case T_PP * 4 + T_HH: k = o[PPHH] + w1 + w2 * np; break;
+    case T_PP * 4 + T_PH: k = o[PPPH] + w1 + w2 * np; break;
+    default: k = o[PPPP] + w1 + w2 * np; break;
+  }
+  return (uint32_t)k | (tr << 30) | (sgn << 31);
+}
+
+// calc_cc (src/ME_general.f90:923-952) as a gather table: for the CC element
+// (row (i, j), column (k, l)) of every block in device order, the code of
+// the V element <j k|V|i l>.  Requires neq < CC_ZERO.
+std::vector<uint32_t> cc_gather_map(const Channels &ch) {
+  auto elem = [&](int a1, int a2, int a3, int a4) { return velem_code(ch, a1, a2, a3, a4); };
   std::vector<uint32_t> m((size_t)ch.cc_size);
   for (int q = 0; q < ch.ncc; q++) {
     const int r = ch.cc_r[q], nb = ch.cc_nb[q];
@@ -837,7 +859,7 @@ int run_onebody(qd_context *c, const qd_op *r1, const qd_op *r2, qd_op *out, int
     launch_gemm_small(c->stream, c->g_1b.probs, c->g_1b.segs, c->ob_tiles, c->nob_tiles,
                       make_slots(c, r1, r2, out, nullptr, nullptr));
   launch_onebody(c->stream, c->dt, o, out->d, flags, c->w1, c->w2, c->w3, c->u1, c->u2, ecc ? ecc->d : nullptr,
-                 hcc ? hcc->d : nullptr, c->ob221, c->nob221);
+                 hcc ? hcc->d : nullptr, c->ob221, c->nob221, c->g221);
   HIPC(hipGetLastError());
   return QD_OK;
 }

@@ -225,9 +225,47 @@ __global__ void __launch_bounds__(256) k_onebody221(DevTab t, Ops o, double *__r
   if (lane == 0) out[1 + e] += r;
 }
 
+// The same traces from the code table: per lane and state a, one coalesced
+// table load and two gathers, with the arrays and coefficients chosen by
+// selects (no branch splits the loads).
+__global__ void __launch_bounds__(256) k_onebody221_map(DevTab t, Ops o, double *__restrict__ out,
+                                                        const int *__restrict__ list, int nlist,
+                                                        const uint2 *__restrict__ g, const double *__restrict__ w1,
+                                                        const double *__restrict__ w2) {
+  const int lane = threadIdx.x & 63;
+  const int k = blockIdx.x * 4 + (threadIdx.x >> 6);
+  if (k >= nlist) return;  // whole wave exits together
+  const int64_t e = list[k];
+  int kind, i, j, x, y;
+  ob_decode(t, e, kind, i, j, x, y);
+  const int n = t.n, M = t.M;
+  const double h1 = o.h1, h2 = o.h2;
+  const uint2 *gk = g + (int64_t)k * M;
+  double r = 0.0;
+#pragma unroll 4
+  for (int a = lane; a < M; a += 64) {
+    const uint2 m = gk[a];  // codes of <a x|w|a y>, <a y|w|a x>
+    const uint32_t ka = m.x & CC_ZERO, kb = m.y & CC_ZERO;
+    const bool hole = a < n;
+    // hole:     kind 2: h1 w2(axay) - h2 w1(axay); else h1 w1(ayax) - h2 w1(axay)
+    // particle: kind 1: -h1 w1(ayax) + h2 w1(axay); kind 0: -h1 w2(ayax) + h2 w2(axay);
+    //           kind 2: -h1 w2(axay) + h2 w1(axay)
+    const bool first_ax = kind == 2;
+    const double *P1 = (kind == 2 || (!hole && kind == 0)) ? w2 : w1;
+    const double *P2 = (!hole && kind == 0) ? w2 : w1;
+    const uint32_t c1 = first_ax ? m.x : m.y, k1 = first_ax ? ka : kb;
+    const double v1 = P1[k1 == CC_ZERO ? 0 : k1], v2 = P2[ka == CC_ZERO ? 0 : ka];
+    const double x1 = k1 == CC_ZERO ? 0.0 : ((c1 >> 31) ? -v1 : v1);
+    const double x2 = ka == CC_ZERO ? 0.0 : ((m.x >> 31) ? -v2 : v2);
+    r += hole ? h1 * x1 - h2 * x2 : -h1 * x1 + h2 * x2;
+  }
+  r = wave_sum(r);
+  if (lane == 0) out[1 + e] += r;
+}
+
 void launch_onebody(hipStream_t s, const DevTab &t, const Ops &o, double *out, int flags, const double *w1,
                     const double *w2, const double *w3, double *u1, double *u2, const double *cc1,
-                    const double *cc2, const int *list221, int n221) {
+                    const double *cc2, const int *list221, int n221, const uint32_t *g221) {
   if ((flags & OB_121) && cc1 && cc2) {
     const int m = t.M * t.M;
     hipLaunchKernelGGL(k_cc_gemv, dim3((m + 31) / 32), dim3(256), 0, s, t, cc2, o.v1, u2);
@@ -240,8 +278,13 @@ void launch_onebody(hipStream_t s, const DevTab &t, const Ops &o, double *out, i
   if ((flags & (OB_111 | OB_121)) || !(flags & OB_ACCUM))
     hipLaunchKernelGGL(k_onebody, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, s, t, o, out, flags, w1, w2, w3,
                        u1, u2);
-  if ((flags & OB_221) && n221 > 0)
-    hipLaunchKernelGGL(k_onebody221, dim3((n221 + 3) / 4), dim3(256), 0, s, t, o, out, list221, n221, w1, w2);
+  if ((flags & OB_221) && n221 > 0) {
+    if (g221)
+      hipLaunchKernelGGL(k_onebody221_map, dim3((n221 + 3) / 4), dim3(256), 0, s, t, o, out, list221, n221,
+                         (const uint2 *)g221, w1, w2);
+    else
+      hipLaunchKernelGGL(k_onebody221, dim3((n221 + 3) / 4), dim3(256), 0, s, t, o, out, list221, n221, w1, w2);
+  }
 }
 
 // ---------------------------------------------------------------------------

@@ -141,9 +141,12 @@ void launch_ladder_sq(hipStream_t s, const DevTab &t, const double *w1, const do
 // OB_111 needs the one-body products of the one-body product list in the
 // one-body region of w1, w2, w3; list221: the n221 outputs whose states share
 // a channel (221 traces)
+// g221: per listed output and state a, the codes (flat index | sign << 31,
+// CC_ZERO) of <a x|w|a y> and <a y|w|a x>, or null (the traces then resolve
+// the pair tables per element)
 void launch_onebody(hipStream_t s, const DevTab &t, const Ops &o, double *out, int flags,
                     const double *w1, const double *w2, const double *w3, double *u1, double *u2,
-                    const double *cc1, const double *cc2, const int *list221, int n221);
+                    const double *cc1, const double *cc2, const int *list221, int n221, const uint32_t *g221);
 // zero-body part (commutator_110 + commutator_220) into *dst (device scalar)
 void launch_zerobody(hipStream_t s, const DevTab &t, const Ops &o, const Seg2 *pphh_segs, int npphh,
                      double *part, double *dst, int which /*1: 110, 2: 220, 3: both*/);
