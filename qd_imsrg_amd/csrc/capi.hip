@@ -1088,6 +1088,12 @@ int qd_context_create(int nbody, int nshell, qd_context **out) {
   t.fcls_size = ch.fcls_size;
   t.cc_size = ch.cc_size;
   t.p_size = ch.p_size;
+  {
+    std::vector<int> rowblk((size_t)ch.M * ch.M, 0);
+    for (int K = 0; K < ch.ncc; K++)
+      for (int d = 0; d < ch.cc_r[K]; d++) rowblk[(size_t)ch.cc_ab_start[K] + d] = K;
+    UP(rowblk, cc_rowblk);
+  }
   t.cc_gmap = nullptr;
   if (ch.neq < (int64_t)CC_ZERO) {
     const std::vector<uint32_t> gmap = cc_gather_map(ch);

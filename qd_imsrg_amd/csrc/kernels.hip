@@ -117,15 +117,11 @@ __global__ void __launch_bounds__(256) k_cc_gemv(DevTab t, const double *__restr
   const int g = blockIdx.x * 32 + rl;
   double s = 0.0;
   if (g < t.M * t.M) {
-    int lo = 0, hi = t.ncc - 1;  // block of device row g: last with cc_ab_start <= g
-    while (lo < hi) {
-      const int mid = (lo + hi + 1) >> 1;
-      if (t.cc_ab_start[mid] <= g) lo = mid;
-      else hi = mid - 1;
-    }
-    const int K = lo, d = g - t.cc_ab_start[K], r = t.cc_r[K], nb = t.cc_nb[K];
+    const int K = t.cc_rowblk[g];  // CC block of device row g
+    const int d = g - t.cc_ab_start[K], r = t.cc_r[K], nb = t.cc_nb[K];
     const double *col = cc + t.cc_off[K] + d;
     const uint32_t *hp = t.cc_hp + t.cc_hp_start[K];
+#pragma unroll 4
     for (int c = cg; c < nb; c += 8) {
       const int i = lo16(hp[c]), a = hi16(hp[c]);
       s += col[(int64_t)c * r] * f[t.off_fph + (a - t.n) + (int64_t)i * t.mp];

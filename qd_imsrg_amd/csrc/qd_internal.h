@@ -112,6 +112,7 @@ struct DevTab {
   const uint32_t *cc_gmap;
   int64_t cc_size;
   int64_t p_size;  // Pandya product entries; element p_size of D is a zero
+  const int *cc_rowblk;  // per CC row (M * M, rows of the blocks in order): its block
 };
 constexpr uint32_t CC_ZERO = 0x3fffffffu;
 
