@@ -431,15 +431,11 @@ int build_worklists(qd_context *c) {
       for (int gc = ch.cp_start[q]; gc < ch.cp_start[q + 1]; gc++) {
         const int s3 = csize(ch.cp_a[gc]), s4 = csize(ch.cp_b[gc]);
         const int nc = s3 * s4, ldc = nc | 1, fcol = 2 * (s3 * s3 + s4 * s4);
-        std::vector<int> cur;
-        int goff = 0, foff = fcol, roff = 0;
-        auto flush = [&]() {
-          if (!cur.empty()) {
-            bt1[q].push_back(BTask{q, gc, (int)blist.size(), (int)cur.size(), 0, 0, 0});
-            blist.insert(blist.end(), cur.begin(), cur.end());
-          }
-          cur.clear(); goff = 0; foff = fcol; roff = 0;
-        };
+        // whole boxes go to tasks first-fit decreasing (by grid size) under
+        // the LDS limits: fuller tasks than bundling in class-pair order
+        struct Bin { std::vector<int> g; int goff, foff, roff; };
+        std::vector<Bin> bins;
+        std::vector<std::pair<int, int>> whole;  // (grid size, row class pair)
         for (int gr = ch.cp_start[q]; gr < ch.cp_start[q + 1]; gr++) {
           if (ch.cp_tmax[gr] < ch.cp_tmin[gc]) continue;
           const int s1 = csize(ch.cp_a[gr]), s2 = csize(ch.cp_b[gr]);
@@ -461,14 +457,33 @@ int build_worklists(qd_context *c) {
             }
             continue;
           }
-          if (goff + nr * ldc > cap || foff + fneed > BX_FCAP || roff + nr > BX_RCAP || (int)cur.size() == BX_GMAX)
-            flush();
-          cur.push_back(gr);
-          goff += nr * ldc;
-          foff += fneed;
-          roff += nr;
+          whole.push_back({nr * ldc, gr});
         }
-        flush();
+        std::stable_sort(whole.begin(), whole.end(), [](auto &x, auto &y) { return x.first > y.first; });
+        for (auto &w : whole) {
+          const int gr = w.second;
+          const int s1 = csize(ch.cp_a[gr]), s2 = csize(ch.cp_b[gr]);
+          const int nr = s1 * s2, fneed = 2 * (s1 * s1 + s2 * s2);
+          Bin *fit = nullptr;
+          for (auto &bn : bins)
+            if (bn.goff + nr * ldc <= cap && bn.foff + fneed <= BX_FCAP && bn.roff + nr <= BX_RCAP &&
+                (int)bn.g.size() < BX_GMAX) {
+              fit = &bn;
+              break;
+            }
+          if (!fit) {
+            bins.push_back(Bin{{}, 0, fcol, 0});
+            fit = &bins.back();
+          }
+          fit->g.push_back(gr);
+          fit->goff += nr * ldc;
+          fit->foff += fneed;
+          fit->roff += nr;
+        }
+        for (auto &bn : bins) {
+          bt1[q].push_back(BTask{q, gc, (int)blist.size(), (int)bn.g.size(), 0, 0, 0});
+          blist.insert(blist.end(), bn.g.begin(), bn.g.end());
+        }
       }
   }
   {
