@@ -69,8 +69,11 @@ int qd_context_info(const qd_context *ctx, qd_info *out);
  * 1..3, block, index; 0-based), 3 CC map (M*M x 2: block,row), 4 CC dims
  * (ncc x 2: r, nb), 5 pair lists (hh, ph, pp of every block, concatenated,
  * x 2: (i, j) with i < j), 6 sizes (neq, cc_size, nblock, ncc), 7 single-
- * particle quantum numbers (M x 2: m_l, m_s as 2*s_z, states order).  len is
- * the capacity of out in int64 elements; returns the count. */
+ * particle quantum numbers (M x 2: m_l, m_s as 2*s_z, states order), 8 the
+ * calc_cc gather (cc_size x 2: per CC element in device order its code,
+ * flat V index | stored transposed << 30 | pair-sorting sign << 31 or
+ * 0x3fffffff for zero, and its flat index in the reference's CC order).
+ * len is the capacity of out in int64 elements; returns the count. */
 int64_t qd_context_table(const qd_context *ctx, int which, int64_t *out, int64_t len);
 /* The same tables without a context or a GPU (host-only channel builder);
  * which = 6 gives (neq, cc_size, nblock, ncc). */

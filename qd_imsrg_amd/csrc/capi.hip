@@ -1044,6 +1044,19 @@ int64_t layout_table(const Channels &ch, int which, int64_t *out, int64_t len) {
     case 5: for (uint32_t pr : ch.pairs) { v.push_back(lo16(pr)); v.push_back(hi16(pr)); } break;
     case 6: v.push_back(ch.neq); v.push_back(ch.cc_size); v.push_back(ch.nblock); v.push_back(ch.ncc); break;
     case 7: for (int i = 0; i < ch.M; i++) { v.push_back(ch.ml[i]); v.push_back(ch.ms[i]); } break;
+    case 8: {  // calc_cc gather codes in device order, with each element's flat index in reference order
+      if (ch.neq >= (int64_t)CC_ZERO) return fail(QD_E_LAYOUT, "no gather table: neq too large");
+      const std::vector<uint32_t> m = cc_gather_map(ch);
+      std::vector<int64_t> ref(m.size(), 0);
+      for (int q = 0; q < ch.ncc; q++) {
+        const int r = ch.cc_r[q], s0 = ch.cc_ab_start[q];
+        for (int row = 0; row < r; row++)
+          for (int c = 0; c < ch.cc_nb[q]; c++)
+            ref[(size_t)ch.cc_off[q] + ch.cc_dev_row[s0 + row] + (size_t)c * r] = ch.cc_off[q] + row + (int64_t)c * r;
+      }
+      for (size_t e = 0; e < m.size(); e++) { v.push_back(m[e]); v.push_back(ref[e]); }
+      break;
+    }
     default: return fail(QD_E_ARG, "unknown table %d", which);
   }
   if (out) {

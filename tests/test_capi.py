@@ -73,3 +73,30 @@ def test_layout_view_matches_oracle(N, R):
     a = random_operator(lay, np.random.default_rng(5), -1)
     b = random_flat(ch, np.random.default_rng(5), -1)
     assert np.array_equal(a, b)
+
+
+@pytest.mark.parametrize("N,R", [(6, 6), (12, 8)])
+def test_cc_gather_table_reproduces_calc_cc(N, R):
+    """The host-built calc_cc gather (layout table 8: per CC element in device
+    order its code and its reference-order index) applied to a random
+    operator on the CPU equals the oracle's calc_cc bit for bit, for both
+    herm factors; the reference indices are a permutation."""
+    from oracle.oracle import Oracle
+    from tests.pins.layout import random_flat
+    ch = Channels(N, R)
+    o = Oracle(ch)
+    t = _table(N, R, 8).reshape(-1, 2)
+    code, ref = t[:, 0], t[:, 1]
+    assert len(code) == ch.cc_size
+    assert np.array_equal(np.sort(ref), np.arange(ch.cc_size))
+    v = random_flat(ch, np.random.default_rng(11), 1)
+    k = code & 0x3FFFFFFF
+    zero = k == 0x3FFFFFFF
+    assert zero.any() and (~zero).any()
+    for h in (1, -1):
+        val = np.where(zero, 0.0, v[np.where(zero, 0, k)])
+        val = np.where((code >> 30) & 1, h * val, val)
+        val = np.where((code >> 31) & 1, -val, val)
+        got = np.zeros(ch.cc_size)
+        got[ref] = val
+        assert np.array_equal(got, o.calc_cc(v, h))
