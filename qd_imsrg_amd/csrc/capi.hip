@@ -107,6 +107,7 @@ struct qd_context {
   GemmTile *ob_tiles = nullptr; int nob_tiles = 0;  // their 16 x 16 tiles
   int *ob221 = nullptr; int nob221 = 0;  // one-body outputs whose states share a channel
   uint32_t *g221 = nullptr;               // their trace elements as codes (velem_code), or null
+  bool gather_tables = true;              // build the calc_cc and 221 gather tables
   int launch_mode = 0;    // 0 grouped persistent launch, 1 one launch per channel problem
   int gemm_prec = GEMM_FP64;  // arithmetic of the 2b-2b GEMMs (GemmPrec)
   bool use_graphs = false;  // capture each BCH term (commutator + norms) in a hipGraph
@@ -679,7 +680,7 @@ int build_worklists(qd_context *c) {
       if (ch.spch[x] == ch.spch[y]) l221.push_back((int)e);
     }
     c->nob221 = (int)l221.size();
-    if (ch.neq < (int64_t)CC_ZERO && !l221.empty()) {
+    if (ch.neq < (int64_t)CC_ZERO && c->gather_tables && !l221.empty()) {
       // per listed output (x, y) and state a: the codes of <a x|w|a y> and
       // <a y|w|a x> (w1, w2 share the layout; herm = 1)
       std::vector<uint32_t> g((size_t)l221.size() * ch.M * 2);
@@ -1123,7 +1124,12 @@ int qd_context_create(int nbody, int nshell, qd_context **out) {
     UP(rowblk, cc_rowblk);
   }
   t.cc_gmap = nullptr;
-  if (ch.neq < (int64_t)CC_ZERO) {
+  // QD_GATHER_TABLES=0: no gather tables (the calc_cc and 221 kernels then
+  // resolve the pair tables per element, as they must for neq >= CC_ZERO;
+  // tests use it to check that path)
+  const char *gt = getenv("QD_GATHER_TABLES");
+  c->gather_tables = !(gt && gt[0] == '0');
+  if (ch.neq < (int64_t)CC_ZERO && c->gather_tables) {
     const std::vector<uint32_t> gmap = cc_gather_map(ch);
     UP(gmap, cc_gmap);
   }

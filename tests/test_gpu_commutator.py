@@ -241,3 +241,22 @@ def test_box_slabs(monkeypatch, cap):
             assert rel(Z.download()[two], ref[two]) <= TOL
     finally:
         c.ctx.close()
+
+
+def test_without_gather_tables(monkeypatch):
+    # QD_GATHER_TABLES=0: calc_cc and the 221 traces resolve the pair tables
+    # per element, the path of operators whose flat index needs more than
+    # 30 bits; calc_cc bit for bit and the full commutator against the oracle
+    monkeypatch.setenv("QD_GATHER_TABLES", "0")
+    c = Case(6, 6)
+    try:
+        for h1, h2 in [(-1, 1), (1, 1)]:
+            x, y, X, Y = c.ops(23, h1, h2)
+            Z = c.ctx.operator(-h1 * h2)
+            HCC, ECC = c.ctx.cc(float(h2)), c.ctx.cc(float(h1))
+            qd.calc_cc(Y, HCC); qd.calc_cc(X, ECC)
+            assert np.array_equal(HCC.download(), c.o.calc_cc(y, h2)[:c.ch.cc_size])
+            qd.commutator(X, Y, Z, HCC, ECC)
+            assert rel(Z.download(), c.o.commutator(x, h1, y, h2, pandya=True)) <= TOL
+    finally:
+        c.ctx.close()
