@@ -58,27 +58,6 @@ __device__ __forceinline__ double felem(const DevTab &t, const double *__restric
   return (b < n) ? v[t.off_fph + (a - n) + (int64_t)b * t.mp] : v[t.off_fpp + (a - n) + (int64_t)(b - n) * t.mp];
 }
 
-// Offset of block element (P, Q) (combined indices) in the flat layout; *tr is
-// set when the element is stored transposed (the herm factor of v_elem applies).
-__device__ __forceinline__ int64_t vidx(const DevTab &t, int q, int nh, int nb, int np, int P, int Q, bool *tr) {
-  const int64_t *o = t.off + 6 * q;
-  const int tp = P < nh ? 0 : (P < nh + nb ? 1 : 2), tq = Q < nh ? 0 : (Q < nh + nb ? 1 : 2);
-  const int ip = P - (tp == 0 ? 0 : (tp == 1 ? nh : nh + nb));
-  const int iq = Q - (tq == 0 ? 0 : (tq == 1 ? nh : nh + nb));
-  *tr = tp < tq;
-  switch (tp * 3 + tq) {
-    case 0: return o[HHHH] + ip + (int64_t)iq * nh;
-    case 1: return o[PHHH] + iq + (int64_t)ip * nb;
-    case 2: return o[PPHH] + iq + (int64_t)ip * np;
-    case 3: return o[PHHH] + ip + (int64_t)iq * nb;
-    case 4: return o[PHPH] + ip + (int64_t)iq * nb;
-    case 5: return o[PPPH] + iq + (int64_t)ip * np;
-    case 6: return o[PPHH] + ip + (int64_t)iq * np;
-    case 7: return o[PPPH] + ip + (int64_t)iq * np;
-    default: return o[PPPP] + ip + (int64_t)iq * np;
-  }
-}
-
 // flags of the fused two-body kernel
 enum { TB_122 = 1, TB_LADDER = 2, TB_PANDYA = 4, TB_ACCUM = 8 };
 // flags of the one-body kernel
