@@ -29,8 +29,9 @@ PEAK_HBM = 8.0e12          # MI355X HBM3E, B/s
 CLASS_NAMES = ["gemm_products", "gemm_accumulate", "twobody", "onebody", "calc_cc", "other", "reserved"]
 NCLS = len(CLASS_NAMES)
 # kernels behind each timing class (names as rocprofv3 reports them)
-CLASS_KERNELS = {0: ["k_gemm", "k_pandya_fused"], 1: ["k_gemm"], 2: ["k_fclass", "k_box2b"],
-                 3: ["k_cc_gemv", "k_onebody", "k_onebody221"], 4: ["k_calc_cc"]}
+CLASS_KERNELS = {0: ["k_gemm_mix", "k_pandya_fused"], 1: ["k_gemm_mix", "k_sum_halves"],
+                 2: ["k_ladder_sq", "k_fclass", "k_box2b"], 3: ["k_cc_gemv", "k_onebody", "k_onebody221_map"],
+                 4: ["k_calc_cc_map"]}
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "hbm_traffic_n20r14.json")
 
 
