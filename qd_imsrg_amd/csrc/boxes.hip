@@ -52,7 +52,7 @@ __device__ __forceinline__ int box_of(const int *pre, int lo, int ng, int e) {
 // from the nearest integer
 __device__ __forceinline__ int qdiv(int a, float rd) { return (int)(((float)a + 0.5f) * rd); }
 
-__global__ void __launch_bounds__(256) k_box2b(DevTab t, Ops o, double *__restrict__ out,
+__global__ void __launch_bounds__(BX_THREADS) k_box2b(DevTab t, Ops o, double *__restrict__ out,
                                                const BTask *__restrict__ tasks, const int *__restrict__ blist,
                                                int flags, const double *__restrict__ w1,
                                                const double *__restrict__ w2, const double *__restrict__ w3,
@@ -90,7 +90,7 @@ __global__ void __launch_bounds__(256) k_box2b(DevTab t, Ops o, double *__restri
     const int sub = tB == 0 ? (tA == 0 ? HHHH : (tA == 1 ? PHHH : PPHH)) : (tB == 1 ? (tA == 1 ? PHPH : PPPH) : PPPP);
     soff[tA][tB] = tA >= tB ? t.off[6 * q + sub] : 0;
   }
-  for (int k = threadIdx.x; k < nc; k += 256) cent[k] = cgrid[k];
+  for (int k = threadIdx.x; k < nc; k += BX_THREADS) cent[k] = cgrid[k];
   if (threadIdx.x < 64) {
     // wave 0, one lane per box: the box's tables (three dependent loads for
     // all boxes at once), then its offsets by exclusive prefix sums over the
@@ -156,7 +156,7 @@ __global__ void __launch_bounds__(256) k_box2b(DevTab t, Ops o, double *__restri
   {
     // row grid entries of all boxes in one flat pass
     const int rtot = rtot_s;
-    for (int k = threadIdx.x; k < rtot; k += 256) {
+    for (int k = threadIdx.x; k < rtot; k += BX_THREADS) {
       int lo = 0, hi = ng - 1;  // last box whose rows start at or before k
       while (lo < hi) {
         const int mid = (lo + hi + 1) >> 1;
@@ -174,12 +174,12 @@ __global__ void __launch_bounds__(256) k_box2b(DevTab t, Ops o, double *__restri
     // before the LDS stores
     const int fs = t.fcls_size, q3 = s3 * s3, q34 = q3 + s4 * s4, ftot = fsz;
     const int f3 = t.fcls_off[A3], f4 = t.fcls_off[A4];
-    for (int k0 = threadIdx.x; k0 < ftot; k0 += 512) {
+    for (int k0 = threadIdx.x; k0 < ftot; k0 += 2 * BX_THREADS) {
       double y1[2], y2[2];
       int kk[2];
 #pragma unroll
       for (int u = 0; u < 2; u++) {
-        const int k = min(k0 + 256 * u, ftot - 1);
+        const int k = min(k0 + BX_THREADS * u, ftot - 1);
         int src;
         if (k < q34) src = k < q3 ? f3 + k : f4 + (k - q3);
         else {
@@ -198,7 +198,7 @@ __global__ void __launch_bounds__(256) k_box2b(DevTab t, Ops o, double *__restri
       }
 #pragma unroll
       for (int u = 0; u < 2; u++)
-        if (k0 + 256 * u < ftot) fl[kk[u]] = make_double2(y1[u], y2[u]);
+        if (k0 + BX_THREADS * u < ftot) fl[kk[u]] = make_double2(y1[u], y2[u]);
     }
     // ordered grids: V(x1,x2,x3,x4) = sign * (herm if stored transposed) *
     // stored element of the canonical pairs, 0 for x1 == x2 or x3 == x4.
@@ -206,13 +206,13 @@ __global__ void __launch_bounds__(256) k_box2b(DevTab t, Ops o, double *__restri
     // branch-free (past the end the last element is read again; an invalid
     // entry reads element 0) before the first LDS store, so the round trips
     // overlap; validity and sign are applied at the store.
-    for (int e0 = threadIdx.x, b = 0; e0 < total; e0 += 1024) {
+    for (int e0 = threadIdx.x, b = 0; e0 < total; e0 += 4 * BX_THREADS) {
       double x1[4], x2[4], g1[4], g2[4];
       int dst[4];
       bool ok[4];
 #pragma unroll
       for (int u = 0; u < 4; u++) {
-        const int e = min(e0 + 256 * u, total - 1);
+        const int e = min(e0 + BX_THREADS * u, total - 1);
         if (b + 1 < ng && pre[b + 1] <= e) b = box_of(pre, b + 1, ng, e);
         const int4 B = bx[b];
         const int le = e - pre[b], c = qdiv(le, brc[b].x), r = le - c * B.y;
@@ -234,7 +234,7 @@ __global__ void __launch_bounds__(256) k_box2b(DevTab t, Ops o, double *__restri
       }
 #pragma unroll
       for (int u = 0; u < 4; u++)
-        if (e0 + 256 * u < total)
+        if (e0 + BX_THREADS * u < total)
           gg[dst[u]] = ok[u] ? make_double2(g1[u] * x1[u], g2[u] * x2[u]) : make_double2(0.0, 0.0);
     }
     __syncthreads();
@@ -242,7 +242,7 @@ __global__ void __launch_bounds__(256) k_box2b(DevTab t, Ops o, double *__restri
   const double h1 = o.h1, h2 = o.h2;
   const float rs4 = 1.0f / (float)s4;
   const bool same34 = A3 == A4;
-  for (int e = threadIdx.x, b = 0; e < total; e += 256) {
+  for (int e = threadIdx.x, b = 0; e < total; e += BX_THREADS) {
     if (b + 1 < ng && pre[b + 1] <= e) b = box_of(pre, b + 1, ng, e);
     const int4 B = bx[b];
     const int le = e - pre[b], c = qdiv(le, brc[b].x), r = le - c * B.y;
@@ -355,7 +355,7 @@ void launch_box2b(hipStream_t s, const DevTab &t, const Ops &o, double *out, con
                   const int *blist, int flags, const double *w1, const double *w2, const double *w3,
                   const double *L, const double *F, const double *D) {
   if (ntasks <= 0) return;
-  hipLaunchKernelGGL(k_box2b, dim3(ntasks), dim3(256), 0, s, t, o, out, tasks, blist, flags, w1, w2, w3, L, F, D);
+  hipLaunchKernelGGL(k_box2b, dim3(ntasks), dim3(BX_THREADS), 0, s, t, o, out, tasks, blist, flags, w1, w2, w3, L, F, D);
 }
 
 __global__ void __launch_bounds__(256) k_sum_halves(const double *__restrict__ w3, const double *__restrict__ w4,

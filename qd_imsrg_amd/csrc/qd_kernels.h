@@ -85,13 +85,15 @@ void launch_gemm(hipStream_t s, const GemmProb *probs, const GemmSeg *segs, cons
 // mode-2 tasks wrote; mode 2: one box, rows lo .. lo+cnt-1, writing the
 // ket-side 1b x 2b terms (their own launch, before the others).
 struct BTask { int block, gcol, first, ng, mode, lo, cnt; };
-// LDS per work-group stays under 40 KB (four per CU, as many as the kernel's
-// registers allow); channels of up to 10 states (R = 20 shells) fit the
-// channel-block area
-constexpr int BX_VCAP = 1800;  // grid elements per operand staged in LDS
-constexpr int BX_FCAP = 800;   // doubles of one-body channel blocks staged in LDS
-constexpr int BX_GMAX = 32;    // boxes per task
-constexpr int BX_RCAP = 256;   // row grid entries per task (staged in LDS)
+// 512 threads per task: LDS per work-group stays under 80 KB (two per CU,
+// 16 waves, as many as the kernel's registers allow), so tasks hold up to
+// 3600 grid elements (no box of R = 14 is split); channels of up to 10
+// states (R = 20 shells) fit the channel-block area
+constexpr int BX_THREADS = 512;
+constexpr int BX_VCAP = 3600;  // grid elements per operand staged in LDS
+constexpr int BX_FCAP = 1200;  // doubles of one-body channel blocks staged in LDS
+constexpr int BX_GMAX = 48;    // boxes per task (at most 64: one wave's lanes at set-up)
+constexpr int BX_RCAP = 384;   // row grid entries per task (staged in LDS)
 constexpr int BX_CMAX = 128;   // column grid entries per task (a class pair of <= 11 x 11 states)
 // one-body matrices of both operands blocked by channel: F[0, fcls_size) of
 // r1, F[fcls_size, 2 fcls_size) of r2
