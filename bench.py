@@ -431,11 +431,12 @@ def main():
     n2b = ch.n2b()
     if dom == 0:
         achieved = (gf["products"] + gf["pandya"]) / (per_launch["gemm_products"] * 1e-3)
-        roof = dict(kernel="k_gemm (ladder intermediates + Pandya products)", bound="mfma",
+        roof = dict(kernel="gemm_products (k_gemm_mix ladder list + k_pandya_fused: ladder intermediates and "
+                           "Pandya products)", bound="mfma",
                     achieved=achieved / 1e12, peak=PEAK_FP64_MFMA / 1e12, unit="TFLOP/s")
     elif dom == 1:
         achieved = gf["accumulate"] / (per_launch["gemm_accumulate"] * 1e-3)
-        roof = dict(kernel="k_gemm (accumulating ladder products)", bound="mfma",
+        roof = dict(kernel="gemm_accumulate (k_gemm_mix accumulating list + k_sum_halves)", bound="mfma",
                     achieved=achieved / 1e12, peak=PEAK_FP64_MFMA / 1e12, unit="TFLOP/s")
     else:
         # algorithmic bytes per launch: two-body pass = per output element one
