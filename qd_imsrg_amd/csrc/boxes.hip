@@ -94,7 +94,8 @@ __global__ void __launch_bounds__(BX_THREADS) k_box2b(DevTab t, Ops o, double *_
   if (threadIdx.x < 64) {
     // wave 0, one lane per box: the box's tables (three dependent loads for
     // all boxes at once), then its offsets by exclusive prefix sums over the
-    // lanes (ng <= 32)
+    // lanes (ng <= BX_GMAX <= 64)
+    static_assert(BX_GMAX <= 64, "one lane of wave 0 per box");
     const int b = threadIdx.x;
     int g = 0, A1 = 0, A2 = 0, s1 = 1, s2 = 1, f1 = 0, f2 = 0, go = 0;
     if (b < ng) {
