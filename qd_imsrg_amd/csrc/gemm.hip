@@ -649,8 +649,12 @@ void launch_gemm(hipStream_t s, const GemmProb *probs, const GemmSeg *segs, cons
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) k_pandya_fused(DevTab t, const double *__restrict__ E,
                                                       const double *__restrict__ H, double *__restrict__ D,
                                                       const TilePair *__restrict__ tp, double hh, double eh) {
-  constexpr int T = 64, TK = 16, LD = T + 1;
-  __shared__ double lds[4 * TK * LD];  // E_i, E_j, H_i, H_j as [TK][LD]; later one [T][LD] tile
+  // panel rows 80 doubles apart: a ds_read_b64 of 16 rows x 2 k-steps per
+  // half-wave hits 32 distinct bank pairs; the transposed tile of the
+  // epilogue keeps rows of 65
+  constexpr int T = 64, TK = 16, LD = T + 16, LDE = T + 1;
+  static_assert(T * LDE <= 4 * TK * LD, "epilogue tile fits the panels' space");
+  __shared__ double lds[4 * TK * LD];  // E_i, E_j, H_i, H_j as [TK][LD]; later one [T][LDE] tile
   const TilePair pr = tp[blockIdx.x];
   const int q = pr.blk, r = t.cc_r[q], K = t.cc_nb[q];
   const double *__restrict__ Eq = E + t.cc_off[q];
@@ -719,14 +723,14 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) k
   // epilogue; the MFMAs take H first, so they hold the tiles transposed:
   // lane l, component v is row l % 16, column 4 v + l / 16 of the P tile,
   // and each store covers four columns of 16 consecutive rows of D
-  double *Tt = lds;  // [T][LD]
+  double *Tt = lds;  // [T][LDE]
   auto put = [&](const d4 (&a)[2][2]) {
 #pragma unroll
     for (int x = 0; x < 2; x++)
 #pragma unroll
       for (int y = 0; y < 2; y++)
 #pragma unroll
-        for (int v = 0; v < 4; v++) Tt[(m0 + x * 16 + lr) * LD + n0 + y * 16 + 4 * v + lk] = a[x][y][v];
+        for (int v = 0; v < 4; v++) Tt[(m0 + x * 16 + lr) * LDE + n0 + y * 16 + 4 * v + lk] = a[x][y][v];
   };
   auto out = [&](const d4 (&a)[2][2], int r0, int c0) {  // D[r0 + x][c0 + y] = hh a[x][y] - eh Tt[y][x]
 #pragma unroll
@@ -737,7 +741,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) k
         for (int v = 0; v < 4; v++) {
           const int lx = m0 + x * 16 + lr, ly = n0 + y * 16 + 4 * v + lk;
           if (r0 + lx < r && c0 + ly < r)
-            Dq[(r0 + lx) + (int64_t)(c0 + ly) * r] = hh * a[x][y][v] - eh * Tt[ly * LD + lx];
+            Dq[(r0 + lx) + (int64_t)(c0 + ly) * r] = hh * a[x][y][v] - eh * Tt[ly * LDE + lx];
         }
   };
   __syncthreads();
