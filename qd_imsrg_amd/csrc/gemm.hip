@@ -639,22 +639,22 @@ void launch_gemm(hipStream_t s, const GemmProb *probs, const GemmSeg *segs, cons
 // construct_cc_ints (src/ME_general.f90:923-952) needs D = XCC - YCC =
 // h_H P - h_eta P^T with P = ETACC HCC^T per CC block.  One work-group takes
 // a pair of 64x64 tiles (ti <= tj) of a block: it computes P_ij = E_i H_j^T
-// and P_ji = E_j H_i^T (K staged through LDS in chunks of 16, the next chunk
+// and P_ji = E_j H_i^T (K staged through LDS in chunks of 8, the next chunk
 // prefetched into registers, k ascending as in the grouped GEMM, so P is the
 // same to the bit), then writes D_ij = h_H P_ij - h_eta P_ji^T and
 // D_ji = h_H P_ji - h_eta P_ij^T, exchanging the transposed tile through LDS.
 // The product matrix P is never written.
-// three waves per SIMD (160 registers, no spills; the uncapped allocation
-// takes 186 and leaves two)
-__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) k_pandya_fused(DevTab t, const double *__restrict__ E,
+// Four waves per SIMD (125 registers, 33 KB of LDS for the epilogue tile);
+// chunks of 16 needed 160 registers and ran three.
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) k_pandya_fused(DevTab t, const double *__restrict__ E,
                                                       const double *__restrict__ H, double *__restrict__ D,
                                                       const TilePair *__restrict__ tp, double hh, double eh) {
   // panel rows 80 doubles apart: a ds_read_b64 of 16 rows x 2 k-steps per
   // half-wave hits 32 distinct bank pairs; the transposed tile of the
-  // epilogue keeps rows of 65
-  constexpr int T = 64, TK = 16, LD = T + 16, LDE = T + 1;
-  static_assert(T * LDE <= 4 * TK * LD, "epilogue tile fits the panels' space");
-  __shared__ double lds[4 * TK * LD];  // E_i, E_j, H_i, H_j as [TK][LD]; later one [T][LDE] tile
+  // epilogue keeps rows of 65 and sets the LDS size
+  constexpr int T = 64, TK = 8, LD = T + 16, LDE = T + 1, NU = T * TK / 256;
+  constexpr int NLDS = 4 * TK * LD > T * LDE ? 4 * TK * LD : T * LDE;
+  __shared__ double lds[NLDS];  // E_i, E_j, H_i, H_j as [TK][LD]; later one [T][LDE] tile
   const TilePair pr = tp[blockIdx.x];
   const int q = pr.blk, r = t.cc_r[q], K = t.cc_nb[q];
   const double *__restrict__ Eq = E + t.cc_off[q];
@@ -673,10 +673,10 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) k
 #pragma unroll
     for (int y = 0; y < 2; y++) a1[x][y] = a2[x][y] = d4{0, 0, 0, 0};
   // staging: element (row x, k) of a panel is M[x + k r] (column-major r x K)
-  double rg[4][4];
+  double rg[4][NU];
   auto load = [&](int k0) {
 #pragma unroll
-    for (int u = 0; u < 4; u++) {
+    for (int u = 0; u < NU; u++) {
       const int idx = tid + 256 * u, x = idx % T, k = idx / T, gk = k0 + k;
       const bool kin = gk < K;
       rg[0][u] = (kin && i0 + x < r) ? Eq[(i0 + x) + (int64_t)gk * r] : 0.0;
@@ -689,7 +689,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) k
   for (int k0 = 0; k0 < K; k0 += TK) {
     __syncthreads();
 #pragma unroll
-    for (int u = 0; u < 4; u++) {
+    for (int u = 0; u < NU; u++) {
       const int idx = tid + 256 * u, x = idx % T, k = idx / T;
       Ei[k * LD + x] = rg[0][u];
       Hi[k * LD + x] = rg[2][u];
