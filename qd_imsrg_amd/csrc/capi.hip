@@ -1091,8 +1091,14 @@ int qd_context_create(int nbody, int nshell, qd_context **out) {
   {
     int dev = 0, ncu = 0;
     if (hipGetDevice(&dev) == hipSuccess &&
-        hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev) == hipSuccess && ncu > 0)
-      c->gemm_grid = 4 * ncu;
+        hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev) == hipSuccess && ncu > 0) {
+      // work-groups per CU of the persistent GEMM grid (QD_GEMM_WG_PER_CU, 1..16;
+      // default 6, the FP64 mixed launch's occupancy: at N=20 R=14 steps of
+      // 0.560 / 0.532 / 0.541 ms at 4 / 6 / 8)
+      const char *e = getenv("QD_GEMM_WG_PER_CU");
+      const int per = e && atoi(e) >= 1 && atoi(e) <= 16 ? atoi(e) : 6;
+      c->gemm_grid = per * ncu;
+    }
   }
   const Channels &ch = c->ch;
   DevTab &t = c->dt;

@@ -516,7 +516,7 @@ constexpr int tile_lds_bytes() {
 }
 
 // one output tile of shape SH in precision P
-template <int SH, int P>
+template <int SH, int P, int KC = GEMM_TK>
 __device__ __forceinline__ void run_tile(const GemmProb &pr, const GemmSeg *__restrict__ segs, const Slots &slots,
                                          const GemmTile &tl, unsigned char *lds) {
   constexpr int TM = SHAPE_TM[SH], TN = SHAPE_TN[SH];
@@ -533,7 +533,7 @@ __device__ __forceinline__ void run_tile(const GemmProb &pr, const GemmSeg *__re
         gemm_tile_bf16<64, 64, 2, 2, P>(pr, segs, slots, 2 * tl.tm + (q & 1), 2 * tl.tn + (q >> 1), (unsigned *)lds);
     }
   }
-  else if constexpr (P == GEMM_FP64) gemm_tile<TM, TN, WM, WN>(pr, segs, slots, tl.tm, tl.tn, (double *)lds);
+  else if constexpr (P == GEMM_FP64) gemm_tile<TM, TN, WM, WN, KC>(pr, segs, slots, tl.tm, tl.tn, (double *)lds);
   else if constexpr (P == GEMM_FP32) gemm_tile_f32<TM, TN, WM, WN>(pr, segs, slots, tl.tm, tl.tn, (float *)lds);
   else gemm_tile_bf16<TM, TN, WM, WN, P>(pr, segs, slots, tl.tm, tl.tn, (unsigned *)lds);
 }
@@ -561,16 +561,16 @@ constexpr int mix_lds_bytes() {
   return a > b ? a : b;
 }
 
-template <int P>
+template <int P, int KC = GEMM_TK>
 __device__ __forceinline__ void gemm_mix_body(const GemmProb *__restrict__ probs, const GemmSeg *__restrict__ segs,
                                               const GemmTile *__restrict__ tiles, int ntiles, const Slots &slots) {
-  __shared__ __align__(16) unsigned char lds[mix_lds_bytes<P>()];
+  __shared__ __align__(16) unsigned char lds[KC == GEMM_TK ? mix_lds_bytes<P>() : KC * (64 + 1 + 64 + 1) * 8];
   for (int w = blockIdx.x; w < ntiles; w += gridDim.x) {
     const GemmTile tl = tiles[w];
     if (tl.prob < 0) continue;  // padding of a short XCD queue (uniform over the work-group)
     const GemmProb pr = probs[tl.prob];
-    if (pr.shape == SH_32x32) run_tile<SH_32x32, P>(pr, segs, slots, tl, lds);
-    else run_tile<SH_64x64, P>(pr, segs, slots, tl, lds);
+    if (pr.shape == SH_32x32) run_tile<SH_32x32, P, KC>(pr, segs, slots, tl, lds);
+    else run_tile<SH_64x64, P, KC>(pr, segs, slots, tl, lds);
     __syncthreads();  // the next tile reuses the LDS staging area
   }
 }
@@ -580,6 +580,12 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4)))
 k_gemm_mix(const GemmProb *__restrict__ probs, const GemmSeg *__restrict__ segs, const GemmTile *__restrict__ tiles,
            int ntiles, Slots slots) {
   gemm_mix_body<P>(probs, segs, tiles, ntiles, slots);
+}
+// FP64 in K chunks of 16 (16.6 KB of LDS, 78 registers): six waves per SIMD
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6)))
+k_gemm_mix_f64(const GemmProb *__restrict__ probs, const GemmSeg *__restrict__ segs, const GemmTile *__restrict__ tiles,
+               int ntiles, Slots slots) {
+  gemm_mix_body<GEMM_FP64, 16>(probs, segs, tiles, ntiles, slots);
 }
 // BF16x3, BF16x4: more slices in flight, left to the register allocator
 template <int P>
@@ -614,7 +620,7 @@ void launch_gemm(hipStream_t s, const GemmProb *probs, const GemmSeg *segs, cons
   if (shape == SH_MIX) {
     switch (prec) {
 #define QD_MIX(K, P) hipLaunchKernelGGL((K<P>), dim3(grid), dim3(256), 0, s, probs, segs, tiles, ntiles, slots)
-      case GEMM_FP64: QD_MIX(k_gemm_mix, GEMM_FP64); break;
+      case GEMM_FP64: hipLaunchKernelGGL(k_gemm_mix_f64, dim3(grid), dim3(256), 0, s, probs, segs, tiles, ntiles, slots); break;
       case GEMM_FP32: QD_MIX(k_gemm_mix, GEMM_FP32); break;
       case GEMM_BF16X2: QD_MIX(k_gemm_mix, GEMM_BF16X2); break;
       case GEMM_BF16X3: QD_MIX(k_gemm_mix_wide, GEMM_BF16X3); break;
