@@ -29,7 +29,7 @@ PEAK_HBM = 8.0e12          # MI355X HBM3E, B/s
 CLASS_NAMES = ["gemm_products", "gemm_accumulate", "twobody", "onebody", "calc_cc", "other", "reserved"]
 NCLS = len(CLASS_NAMES)
 # kernels behind each timing class (names as rocprofv3 reports them)
-CLASS_KERNELS = {0: ["k_gemm_mix", "k_pandya_fused"], 1: ["k_gemm_mix", "k_sum_halves"],
+CLASS_KERNELS = {0: ["k_gemm_mix_f64", "k_pandya_fused"], 1: ["k_gemm_mix_f64", "k_sum_halves"],
                  2: ["k_ladder_sq", "k_fclass", "k_box2b"], 3: ["k_cc_gemv", "k_onebody", "k_onebody221_map"],
                  4: ["k_calc_cc_map"]}
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "hbm_traffic_n20r14.json")
@@ -431,12 +431,12 @@ def main():
     n2b = ch.n2b()
     if dom == 0:
         achieved = (gf["products"] + gf["pandya"]) / (per_launch["gemm_products"] * 1e-3)
-        roof = dict(kernel="gemm_products (k_gemm_mix ladder list + k_pandya_fused: ladder intermediates and "
+        roof = dict(kernel="gemm_products (k_gemm_mix_f64 ladder list + k_pandya_fused: ladder intermediates and "
                            "Pandya products)", bound="mfma",
                     achieved=achieved / 1e12, peak=PEAK_FP64_MFMA / 1e12, unit="TFLOP/s")
     elif dom == 1:
         achieved = gf["accumulate"] / (per_launch["gemm_accumulate"] * 1e-3)
-        roof = dict(kernel="gemm_accumulate (k_gemm_mix accumulating list + k_sum_halves)", bound="mfma",
+        roof = dict(kernel="gemm_accumulate (k_gemm_mix_f64 accumulating list + k_sum_halves)", bound="mfma",
                     achieved=achieved / 1e12, peak=PEAK_FP64_MFMA / 1e12, unit="TFLOP/s")
     else:
         # algorithmic bytes per launch: two-body pass = per output element one

@@ -12,7 +12,8 @@
 // register budget), launched once over that shape's part of the tile list.
 // The launches are persistent: a fixed grid strides over the tiles, which are
 // sorted by decreasing work.  Work-group = 256 threads = 4 waves; K is staged through
-// LDS in chunks of 16 (4 MFMA k-steps).  MFMA operand layout: A(16x4) lane l
+// LDS in chunks of 32 (8 MFMA k-steps; 16 for the FP64 mixed launch and the
+// 128 x 128 tiles).  MFMA operand layout: A(16x4) lane l
 // holds A[l%16][l/16], B(4x16) lane l holds B[l/16][l%16], D(16x16) lane l
 // holds D[4*v + l/16][l%16] in component v (FP64: one row per group).
 #include "qd_kernels.h"
