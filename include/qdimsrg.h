@@ -60,7 +60,12 @@ const char *qd_last_error(void);
  * Replaces arrange_states / reindex / build_cross_coupled
  * (src/ME_general.f90:688-744, 956-1116, 363-448).  Holes are the first
  * nbody states in states order (the reference's energy->states map is the
- * identity, convertF src/ME_general.f90:1527-1536); nbody must fill shells. */
+ * identity, convertF src/ME_general.f90:1527-1536); nbody must fill shells.
+ * Environment read at creation (tuning only; results do not change):
+ * QD_BOX_CAP grid elements per two-body box task (16..3600, default 3600),
+ * QD_GATHER_TABLES=0 per-element calc_cc and 221 paths instead of the gather
+ * tables, QD_GEMM_WG_PER_CU work-groups per CU of the persistent GEMM grid
+ * (1..16, default 6). */
 int qd_context_create(int nbody, int nshell, qd_context **out);
 int qd_context_destroy(qd_context *ctx);
 int qd_context_info(const qd_context *ctx, qd_info *out);
