@@ -180,7 +180,7 @@ int qd_dgam(qd_context *ctx, const qd_op *h, qd_op *der, qd_cc *hcc, qd_cc *etac
 /* s_evolve, src/magnus_IMSRG.f90:194-248: the Magnus flow with the White
  * generator from H(0) = h0, Euler steps on Omega, BCH for H(s), reject-and-
  * halve when ||eta|| grows past 1.5x its initial value, stop when
- * |E_MBPT2(H(s))| <= crit.  Literal constants are the reference's REAL(4)
+ * |E_MBPT2(H(s))| <= conv (or, with stop_rule 1, |dE/ds| <= conv_deds).  Literal constants are the reference's REAL(4)
  * values widened to REAL(8).  pandya = 0 reproduces the reference as shipped
  * (zero CC inputs); 1 fills the CC matrices with calc_cc every commutator. */
 typedef struct {
@@ -202,6 +202,12 @@ typedef struct {
                        stopping rule.  No counterpart in the reference; it
                        removes the slow tail of the plain map near decoupling,
                        which is what a warm start lands in.                    */
+  int stop_rule;     /* 0: stop when |MBPT2(H(s))| <= conv (the reference).  1: stop
+                       when |dE/ds| <= conv_deds, with dE/ds the zero-body part of
+                       [eta(s), H(s)] (commutator_110 + commutator_220), the
+                       energy derivative of the IM-SRG flow equation.  No
+                       counterpart in the reference.                           */
+  double conv_deds;  /* threshold of stop_rule 1 (default 1e-8 Ha)                  */
 } qd_flow_params;
 typedef struct {
   double s, e0, crit;          /* final flow parameter, E0(s), |MBPT2(H(s))|          */
@@ -209,6 +215,7 @@ typedef struct {
   int64_t bch_terms;           /* nested commutators evaluated in all BCH expansions  */
   int converged;
   int trace_rows;              /* rows written to trace                              */
+  double mbpt2, deds;          /* MBPT2(H(s)) and dE/ds = [eta(s), H(s)]_0 at the end */
 } qd_flow_result;
 /* Fills *p with the reference's constants. */
 void qd_flow_params_default(qd_flow_params *p);
@@ -238,12 +245,16 @@ int qd_ode_stats(const qd_ode *o, int *nsteps, int64_t *nfe, int *order, double 
 /* gs_IMSRG's decoupling loop (src/gs_IMSRG.f90:100-120): ode over output
  * intervals of stp until |(E_old - E_new)/E_old| over one interval is below
  * conv.  Defaults are the reference's REAL(4) literals widened: relerr 1e-8,
- * abserr 1e-6, stp 0.1, conv 1e-7.  hs receives H(s); trace (optional,
+ * abserr 1e-6, stp 0.1, conv 1e-7.  As in the reference, a call of ode that
+ * returns iflag 3 (tolerances raised), 4 (500 steps) or 5 (stiff) is followed
+ * by the next call with that flag, which continues the integration; iflag 6
+ * (bad input) ends the loop unconverged.  hs receives H(s); trace (optional,
  * trace_len rows of 4: output, s, E0, crit) the reference's per-output line. */
 typedef struct {
   double relerr, abserr, stp, conv;
   int max_outputs;
   int verbose;       /* 1: the reference's per-output line to stdout, 2: stderr */
+  int maxnum;        /* integrator steps per ode call before iflag 4 (reference: 500) */
 } qd_gs_params;
 typedef struct {
   double s, e0, crit;
@@ -267,6 +278,9 @@ int qd_set_launch_mode(qd_context *ctx, int mode);
  * are keyed by the operand addresses and hermiticities.  0 (default): plain
  * stream launches.  No counterpart in the reference. */
 int qd_set_graphs(qd_context *ctx, int on);
+/* Number of captured graphs the context holds.  A graph is dropped when an
+ * operator or CC set it was captured on is destroyed. */
+int64_t qd_graph_count(const qd_context *ctx);
 /* Project autotuner: coordinate descent over the (M, N, K) size buckets of
  * the grouped GEMM problems, largest work first.  For each bucket every tile
  * shape (64x64, 128x64, 32x32, 128x16, 64x16, 16x64, 16x128, 64x16 split-K,
@@ -342,6 +356,35 @@ double qd_v_int(const int *qi, const int *qj, const int *qk, const int *ql, doub
  * order, iters the SCF iteration count.  Eigenvector phases: the
  * largest-magnitude component of each HF orbital is positive. */
 int qd_hf_hamiltonian(int nbody, int nshell, double hw, double *h0, int64_t neq, double *eig, int *iters);
+/* The same H(0) with the interaction read from an ME.bin file, as
+ * construct_two_particle_HF_BASIS reads ../TBMEfiles/ME.bin (src/HFQDmod.f90:
+ * 97-129): a stream of records n1,m1,n2,m2,n3,m3,n4,m4 (int32) and Vdir
+ * (real64) = <ab|V|cd> at hw = 1, scaled by sqrt(hw) on load; records with an
+ * orbital beyond nshell shells are skipped; each record sets <ab|V|cd> and
+ * <cd|V|ab>; the exchange element falls back to <ba|V|cd> where <ab|V|dc> is
+ * below 1e-10 (src/HFQDmod.f90:163-166).  A truncated record is an error (the
+ * reference's 'failfile' stop). */
+int qd_hf_hamiltonian_me(int nbody, int nshell, double hw, const char *me_path, double *h0, int64_t neq,
+                         double *eig, int *iters);
+/* Writes ME.bin for nshell shells from v_int at hw = 1: one record per
+ * (a, b, c, d) with m_a + m_b = m_c + m_d and (a, b) <= (c, d) in orbital
+ * order; *nrec (optional) receives the record count.  The reference ships no
+ * writer (its ME.bin is an external file). */
+int qd_me_bin_write(int nshell, const char *me_path, int64_t *nrec);
+
+/* ---- on-disk format of an operator --------------------------------------
+ * export_hamiltonian / import_hamiltonian (src/ME_general.f90:207-361): the
+ * file the reference's gs_IMSRG writes to ../hamiltonians/ (src/gs_IMSRG.f90:
+ * 141-142), Fortran unformatted sequential with one record per REAL(8)
+ * (4-byte length 8, value, length): E0, fhh, fph, fpp, then per block Vhhhh,
+ * Vpppp, Vpphh, Vphph, Vphhh, Vppph, every matrix row by row.  path is the
+ * whole file name (the caller adds the reference's ../hamiltonians/).
+ * Import checks the size and every record length.  The _host forms take a
+ * vectorize-order host vector and need no GPU. */
+int qd_export_hamiltonian(const qd_op *op, const char *path);
+int qd_import_hamiltonian(qd_op *op, const char *path);
+int qd_export_hamiltonian_host(int nbody, int nshell, const double *vec, int64_t neq, const char *path);
+int qd_import_hamiltonian_host(int nbody, int nshell, double *vec, int64_t neq, const char *path);
 
 /* ---- timing of the dominant kernels (HIP events on the context stream) --
  * Accumulates per-kernel-class device time while enabled.  Classes: 0 the

@@ -329,22 +329,27 @@ GS_STP = float(np.float32(0.1))
 GS_CONV = float(np.float32(1e-7))
 
 
-def gs_flow(o, h0, max_out: int = 100000):
+def gs_flow(o, h0, max_out: int = 100000, relerr: float = GS_REL, abserr: float = GS_ABS):
     """gs_IMSRG's decoupling loop (src/gs_IMSRG.f90:100-120): integrate
     dH/ds = dGam(H) in output intervals of stp until |dE0/E0| over one
     interval is below 1e-7.  RHS = dGam (src/gs_IMSRG.f90:223-273): White
-    generator, calc_cc of H and eta, [eta, H] with the Pandya term."""
+    generator, calc_cc of H and eta, [eta, H] with the Pandya term.  As in
+    the reference, each output interval calls ode with the flag the last call
+    returned (3, 4 and 5 continue the integration; 6 leaves y unchanged, so
+    the next test sees no energy change and the loop ends)."""
     def dgam(_t, y):
         eta = o.build_white(y, 1)
         return o.commutator(eta, -1, y, 1, pandya=True)
     a = Adams(dgam, h0.size)
     y = h0.copy()
     s, flag, crit, out = 0.0, 1, 1.0, []
-    rel, ab = GS_REL, GS_ABS
+    rel, ab = relerr, abserr
+    flags = []
     while crit > GS_CONV and len(out) < max_out:
         e_old = y[0]
         y, s, rel, ab, flag = a.ode(y, s, s + GS_STP, rel, ab, flag)
         crit = abs((e_old - y[0]) / e_old)
         out.append((len(out) + 1, s, y[0], crit))
+        flags.append(flag)
     return dict(hs=y, e0=y[0], s=s, outputs=len(out), nfe=a.nfe, nsteps=a.nsteps, flag=flag,
-                trace=np.array(out), log=a.log)
+                flags=flags, relerr=rel, abserr=ab, trace=np.array(out), log=a.log)

@@ -57,27 +57,36 @@ def diis_weights(B: np.ndarray) -> np.ndarray | None:
 
 def magnus_flow(o: Oracle, h0: np.ndarray, stp: float = R4_0P7, conv: float = R4_1EM9,
                 max_iter: int = 100000, omega0: np.ndarray | None = None, pandya: bool = False,
-                diis: int = 0):
+                diis: int = 0, stop_rule: str = "mbpt2", conv_deds: float = 1e-8):
     """Returns dict(hs, omega, s, e0, iterations, rejected, bch_terms, trace).
     diis > 0: once |MBPT2| <= DIIS_START, Pulay acceleration over the last
     ``diis`` (<= 6) iterates of the map Omega -> Omega + stp eta(Omega)
     (history dropped when ||eta|| grows past twice its smallest value since);
     the reference's step with its rejection rule otherwise (the library's
-    qd_magnus_flow with qd_flow_params.diis; not in the reference)."""
+    qd_magnus_flow with qd_flow_params.diis; not in the reference).
+    stop_rule "deds": stop when |dE/ds| <= conv_deds instead, dE/ds being the
+    zero-body part of [eta(s), H(s)] (commutator_110 + commutator_220 of the
+    White generator and H(s)); the library's qd_flow_params.stop_rule = 1."""
+    def deds(eta_, hs_):
+        return o.commutator_110(eta_, -1, hs_, 1) + o.commutator_220(eta_, -1, hs_, 1)
+
+    def crit_of(eta_, hs_):
+        return abs(deds(eta_, hs_)) if stop_rule == "deds" else abs(o.mbpt2(hs_))
     bch = (lambda g, h: bch_expand_pandya(o, g, h)) if pandya else (lambda g, h: o.bch_expand(g, -1, h))
+    if stop_rule == "deds":
+        conv = conv_deds
     if omega0 is None:
         g = o.zeros()
         hs = h0.copy()
         terms = 0
-        crit = abs(o.mbpt2(h0))
     else:
         g = omega0.copy()
         hs, terms = bch(g, h0)
-        crit = abs(o.mbpt2(hs))
     # step-rejection scale: from H(0) also for a warm start (the library's
     # qd_magnus_flow does the same; the reference always starts cold)
     nrm1 = o.norm(o.build_white(h0, 1))
     eta = o.build_white(hs, 1)
+    crit = crit_of(eta, hs)
     s = 0.0
     it = rej = steps = 0
     trace = []
@@ -107,7 +116,7 @@ def magnus_flow(o: Oracle, h0: np.ndarray, stp: float = R4_0P7, conv: float = R4
                 best = nrm2
             if nrm2 > 2.0 * best:
                 count = 0
-            crit = abs(o.mbpt2(hs))
+            crit = crit_of(eta, hs)
             it += 1
             continue
         count = 0
@@ -128,7 +137,7 @@ def magnus_flow(o: Oracle, h0: np.ndarray, stp: float = R4_0P7, conv: float = R4
             stp = stp / 2.0 + R4_1EM3
             rej += 1
             continue
-        crit = abs(o.mbpt2(hs))
+        crit = crit_of(eta, hs)
         it += 1
     return dict(hs=hs, omega=g, s=s, e0=hs[0], iterations=it, rejected=rej, bch_terms=terms,
-                crit=crit, trace=np.array(trace).reshape(-1, 6))
+                crit=crit, mbpt2=o.mbpt2(hs), deds=deds(eta, hs), trace=np.array(trace).reshape(-1, 6))

@@ -85,6 +85,12 @@ class Context:
         """Capture each BCH term in a hipGraph and replay it."""
         check(self.L.qd_set_graphs(self.h, int(bool(on))))
 
+    def graph_count(self) -> int:
+        n = int(self.L.qd_graph_count(self.h))
+        if n < 0:
+            check(n)
+        return n
+
     def set_precision(self, prec):
         """Arithmetic of the 2b-2b GEMMs: "fp64" (default), "fp32", or
         "bf16x2" / "bf16x3" / "bf16x4" (BF16 mantissa slices); or the
@@ -314,14 +320,19 @@ def dGam(H, DER, HCC, ETACC):
 
 def magnus_flow(ctx: Context, H0: Operator, HS: Operator, OMEGA: Operator, pandya: bool = False,
                 warm_start: bool = False, verbose: int = 0, max_iter: int = 100000,
-                conv: float | None = None, trace_len: int = 4096, diis: int = 0):
+                conv: float | None = None, trace_len: int = 4096, diis: int = 0,
+                stop_rule: str = "mbpt2", conv_deds: float = 1e-8):
     """s_evolve (src/magnus_IMSRG.f90:194-248) on the GPU; returns (result, trace).
     verbose: 1 prints the reference's per-iteration line to stdout, 2 to stderr.
-    diis: depth of the Pulay acceleration (0 = the reference's iteration)."""
+    diis: depth of the Pulay acceleration (0 = the reference's iteration).
+    stop_rule: "mbpt2" (the reference: |MBPT2(H(s))| <= conv) or "deds"
+    (|dE/ds| <= conv_deds, dE/ds = zero-body part of [eta(s), H(s)])."""
     p = FlowParams()
     ctx.L.qd_flow_params_default(C.byref(p))
     p.pandya, p.warm_start, p.verbose, p.max_iter = int(pandya), int(warm_start), int(verbose), int(max_iter)
     p.diis = int(diis)
+    p.stop_rule = {"mbpt2": 0, "deds": 1}[stop_rule]
+    p.conv_deds = float(conv_deds)
     if conv is not None:
         p.conv = float(conv)
     res = FlowResult()
@@ -386,16 +397,59 @@ def v_int(qi, qj, qk, ql, hw: float = 1.0) -> float:
     return L.qd_v_int(*[C.cast(a, _lib.ip) for a in arr], float(hw))
 
 
-def hf_hamiltonian(nbody: int, nshell: int, hw: float):
-    """H(0) in the HF basis (vectorize order) and the HF energies (host)."""
+def _neq(nbody: int, nshell: int) -> int:
     L = _lib.load()
     n = L.qd_layout_table(int(nbody), int(nshell), 6, None, 0)
     if n < 0:
         check(int(n))
     tab = np.zeros(n, dtype=np.int64)
     L.qd_layout_table(int(nbody), int(nshell), 6, tab.ctypes.data_as(_lib.i64p), n)
-    neq = int(tab[0])
+    return int(tab[0])
+
+
+def hf_hamiltonian(nbody: int, nshell: int, hw: float, me_bin: str | None = None):
+    """H(0) in the HF basis (vectorize order) and the HF energies (host).
+    me_bin: read the interaction from an ME.bin file (the reference's
+    ../TBMEfiles/ME.bin) instead of evaluating v_int."""
+    L = _lib.load()
+    neq = _neq(nbody, nshell)
     M = nshell * (nshell + 1)
     h0, eig, it = np.zeros(neq), np.zeros(M), C.c_int()
-    check(L.qd_hf_hamiltonian(int(nbody), int(nshell), float(hw), _dp(h0), neq, _dp(eig), C.byref(it)))
+    if me_bin is None:
+        check(L.qd_hf_hamiltonian(int(nbody), int(nshell), float(hw), _dp(h0), neq, _dp(eig), C.byref(it)))
+    else:
+        check(L.qd_hf_hamiltonian_me(int(nbody), int(nshell), float(hw), str(me_bin).encode(), _dp(h0), neq,
+                                     _dp(eig), C.byref(it)))
     return h0, eig, it.value
+
+
+def me_bin_write(nshell: int, path: str) -> int:
+    """Write ME.bin (<ab|V|cd> at hw = 1 from v_int) for nshell shells;
+    returns the record count."""
+    L = _lib.load()
+    n = C.c_int64()
+    check(L.qd_me_bin_write(int(nshell), str(path).encode(), C.byref(n)))
+    return n.value
+
+
+# ---- on-disk formats (src/ME_general.f90:207-361) ---------------------------
+def export_hamiltonian(op: Operator, path: str):
+    """export_hamiltonian: the reference's unformatted per-scalar dump."""
+    check(op.ctx.L.qd_export_hamiltonian(op.h, str(path).encode()))
+
+
+def import_hamiltonian(op: Operator, path: str):
+    check(op.ctx.L.qd_import_hamiltonian(op.h, str(path).encode()))
+
+
+def export_hamiltonian_host(nbody: int, nshell: int, vec: np.ndarray, path: str):
+    L = _lib.load()
+    v = np.ascontiguousarray(vec, dtype=np.float64)
+    check(L.qd_export_hamiltonian_host(int(nbody), int(nshell), _dp(v), v.size, str(path).encode()))
+
+
+def import_hamiltonian_host(nbody: int, nshell: int, path: str) -> np.ndarray:
+    L = _lib.load()
+    v = np.zeros(_neq(nbody, nshell))
+    check(L.qd_import_hamiltonian_host(int(nbody), int(nshell), _dp(v), v.size, str(path).encode()))
+    return v

@@ -24,18 +24,19 @@ class Info(C.Structure):
 
 class FlowParams(C.Structure):
     _fields_ = [("stp", C.c_double), ("conv", C.c_double), ("max_iter", C.c_int), ("pandya", C.c_int),
-                ("verbose", C.c_int), ("warm_start", C.c_int), ("diis", C.c_int)]
+                ("verbose", C.c_int), ("warm_start", C.c_int), ("diis", C.c_int), ("stop_rule", C.c_int),
+                ("conv_deds", C.c_double)]
 
 
 class FlowResult(C.Structure):
     _fields_ = [("s", C.c_double), ("e0", C.c_double), ("crit", C.c_double), ("iterations", C.c_int),
                 ("rejected", C.c_int), ("bch_terms", C.c_int64), ("converged", C.c_int),
-                ("trace_rows", C.c_int)]
+                ("trace_rows", C.c_int), ("mbpt2", C.c_double), ("deds", C.c_double)]
 
 
 class GsParams(C.Structure):
     _fields_ = [("relerr", C.c_double), ("abserr", C.c_double), ("stp", C.c_double), ("conv", C.c_double),
-                ("max_outputs", C.c_int), ("verbose", C.c_int)]
+                ("max_outputs", C.c_int), ("verbose", C.c_int), ("maxnum", C.c_int)]
 
 
 class GsResult(C.Structure):
@@ -92,6 +93,7 @@ SIGNATURES = {
     "qd_gemm_flops": (C.c_int, [vp, dp, dp, dp]),
     "qd_set_launch_mode": (C.c_int, [vp, C.c_int]),
     "qd_set_graphs": (C.c_int, [vp, C.c_int]),
+    "qd_graph_count": (C.c_int64, [vp]),
     "qd_autotune": (C.c_int, [vp, C.c_int, dp, dp]),
     "qd_set_precision": (C.c_int, [vp, C.c_int]),
     "qd_ode_create": (C.c_int, [vp, C.POINTER(vp)]),
@@ -115,6 +117,12 @@ SIGNATURES = {
     "qd_gemm_shapes": (C.c_int, [vp, ip, C.c_int64]),
     "qd_v_int": (C.c_double, [ip, ip, ip, ip, C.c_double]),
     "qd_hf_hamiltonian": (C.c_int, [C.c_int, C.c_int, C.c_double, dp, C.c_int64, dp, ip]),
+    "qd_hf_hamiltonian_me": (C.c_int, [C.c_int, C.c_int, C.c_double, C.c_char_p, dp, C.c_int64, dp, ip]),
+    "qd_me_bin_write": (C.c_int, [C.c_int, C.c_char_p, i64p]),
+    "qd_export_hamiltonian": (C.c_int, [vp, C.c_char_p]),
+    "qd_import_hamiltonian": (C.c_int, [vp, C.c_char_p]),
+    "qd_export_hamiltonian_host": (C.c_int, [C.c_int, C.c_int, dp, C.c_int64, C.c_char_p]),
+    "qd_import_hamiltonian_host": (C.c_int, [C.c_int, C.c_int, dp, C.c_int64, C.c_char_p]),
 }
 
 _LIB = None

@@ -14,7 +14,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 OBJ = os.path.join(HERE, "_obj")
 LIB = os.path.join(HERE, "libqdimsrg.so")
-SOURCES = ["channels.cpp", "hf.cpp", "kernels.hip", "boxes.hip", "gemm.hip", "capi.hip", "mlp.hip", "ode.hip"]
+SOURCES = ["channels.cpp", "hf.cpp", "io.cpp", "kernels.hip", "boxes.hip", "gemm.hip", "capi.hip", "mlp.hip", "ode.hip"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = "gfx950"
 FLAGS = ["-O3", "-fPIC", "-std=c++17", f"--offload-arch={ARCH}", "-Wall", "-Wno-unused-function",
@@ -40,14 +40,14 @@ BIN = os.path.join(HERE, "bin")
 
 
 def build_fortran(verbose: bool = False) -> str:
-    """The iso_c_binding module and the run_magnus / run_gs drivers (amdflang)."""
+    """The iso_c_binding module and the run_magnus / gs_decouple drivers (amdflang)."""
     os.makedirs(BIN, exist_ok=True)
     mod = [FLANG, "-O2", "-c", os.path.join(FORTRAN, "qdimsrg_c.f90"), "-module-dir", OBJ,
            "-o", os.path.join(OBJ, "qdimsrg_c.o")]
     r = subprocess.run(mod, capture_output=True, text=True)
     if r.returncode != 0:
         raise RuntimeError(f"amdflang failed on qdimsrg_c.f90:\n{r.stderr}")
-    for prog in ("run_magnus", "run_gs"):
+    for prog in ("run_magnus", "gs_decouple"):
         exe = os.path.join(BIN, prog)
         cmd = [FLANG, "-O2", os.path.join(FORTRAN, prog + ".f90"), "-I", OBJ, os.path.join(OBJ, "qdimsrg_c.o"),
                "-o", exe, "-L", HERE, "-lqdimsrg", "-Wl,-rpath,$ORIGIN/..", "-Wl,-rpath,/opt/rocm/lib"]

@@ -149,6 +149,7 @@ hipStream_t qd::context_stream(qd_context *c) { return c->stream; }
 int64_t qd::context_neq(const qd_context *c) { return c->ch.neq; }
 qd_context *qd::op_context(qd_op *o) { return o->ctx; }
 double *qd::op_data(qd_op *o) { return o->d; }
+const Channels &qd::context_channels(const qd_context *c) { return c->ch; }
 
 namespace {
 
@@ -1225,9 +1226,25 @@ int qd_op_create(qd_context *c, int herm, qd_op **out) {
   *out = new qd_op{c, (double *)p, herm};
   return QD_OK;
 }
+// Captured graphs are keyed by the device addresses of their operands: drop
+// every one that names memory about to be freed, so the cache does not grow
+// with short-lived operators (nor replay into a reused address).
+static void evict_graphs(qd_context *c, const void *p) {
+  for (auto it = c->graphs.begin(); it != c->graphs.end();) {
+    const qd_context::GraphKey &k = it->first;
+    if (k.g == p || k.prev == p || k.next == p || k.hs == p || k.cca == p || k.ccg == p) {
+      hipGraphExecDestroy(it->second);
+      it = c->graphs.erase(it);
+    } else {
+      ++it;
+    }
+  }
+}
+
 int qd_op_destroy(qd_op *o) {
   if (!o) return QD_OK;
   hipStreamSynchronize(o->ctx->stream);
+  evict_graphs(o->ctx, o->d);
   hipFree(o->d);
   delete o;
   return QD_OK;
@@ -1271,6 +1288,7 @@ int qd_cc_create(qd_context *c, double herm, qd_cc **out) {
 int qd_cc_destroy(qd_cc *cc) {
   if (!cc) return QD_OK;
   hipStreamSynchronize(cc->ctx->stream);
+  evict_graphs(cc->ctx, cc->d);
   hipFree(cc->d);
   delete cc;
   return QD_OK;
@@ -1584,6 +1602,8 @@ void qd_flow_params_default(qd_flow_params *p) {
   p->verbose = 0;
   p->warm_start = 0;
   p->diis = 0;
+  p->stop_rule = 0;
+  p->conv_deds = 1e-8;
 }
 
 int qd_magnus_flow(qd_context *c, const qd_op *h0, qd_op *hs, qd_op *omega, const qd_flow_params *p,
@@ -1625,9 +1645,25 @@ int qd_magnus_flow(qd_context *c, const qd_op *h0, qd_op *hs, qd_op *omega, cons
   if ((rc = qd_build_white(c, eta, p->warm_start ? h0 : hs))) return done(rc);
   if ((rc = qd_mat_2_norm(c, eta, &nrm1))) return done(rc);
   if (p->warm_start && (rc = qd_build_white(c, eta, hs))) return done(rc);
-  double m2;
-  if ((rc = qd_mbpt2(c, p->warm_start ? hs : h0, &m2))) return done(rc);
-  crit = std::fabs(m2);
+  // stopping quantity: |MBPT2(H(s))| (the reference) or |dE/ds|, the
+  // zero-body part of [eta(s), H(s)] (commutator_110 + commutator_220), the
+  // energy derivative of the IM-SRG flow equation at s; both are evaluated
+  // every accepted step, crit is the one the rule names
+  double m2, deds = 0.0;
+  const bool rule_deds = p->stop_rule == 1;
+  const double conv = rule_deds ? p->conv_deds : p->conv;
+  auto eval_crit = [&](const qd_op *h) -> int {
+    int r;
+    if ((r = qd_mbpt2(c, h, &m2))) return r;
+    Ops zo{eta->d, h->d, eta->herm, h->herm};
+    launch_zerobody(c->stream, c->dt, zo, c->pphh_segs, c->npphh_segs, c->partials, c->scal, 3);
+    HIPC(hipGetLastError());
+    if ((r = read_scalars(c, 1, &deds))) return r;
+    crit = rule_deds ? std::fabs(deds) : std::fabs(m2);
+    return QD_OK;
+  };
+  if (p->stop_rule != 0 && p->stop_rule != 1) return done(fail(QD_E_ARG, "stop_rule must be 0 or 1"));
+  if ((rc = eval_crit(hs))) return done(rc);
   int steps = 0;
   // Pulay (DIIS) acceleration (no counterpart in the reference): once the
   // flow is close to decoupling (crit <= DIIS_START), keep the last m
@@ -1669,7 +1705,7 @@ int qd_magnus_flow(qd_context *c, const qd_op *h0, qd_op *hs, qd_op *omega, cons
       return fail(QD_E_NUMERIC, "non-finite E0 or ||eta|| at s=%g", s);
     return QD_OK;
   };
-  while (crit > p->conv && steps < p->max_iter) {
+  while (crit > conv && steps < p->max_iter) {
     steps++;
     if (m > 0 && crit <= DIIS_START) {
       const int slot = count % m;
@@ -1704,8 +1740,7 @@ int qd_magnus_flow(qd_context *c, const qd_op *h0, qd_op *hs, qd_op *omega, cons
       log_step();
       if (count == 1 || nrm2 < best) best = nrm2;
       if (nrm2 > 2.0 * best) count = 0;  // drop the history
-      if ((rc = qd_mbpt2(c, hs, &m2))) return done2(rc);
-      crit = std::fabs(m2);
+      if ((rc = eval_crit(hs))) return done2(rc);
       iteration++;
       continue;
     }
@@ -1724,15 +1759,14 @@ int qd_magnus_flow(qd_context *c, const qd_op *h0, qd_op *hs, qd_op *omega, cons
     if (nrm1 * 1.5 < nrm2) {
       if (p->verbose) fprintf(p->verbose == 2 ? stderr : stdout, " faillll\n");
       s -= stp;
-      qd_copy_arrays(c, g0, omega);
-      qd_copy_arrays(c, hs0, hs);
-      qd_copy_arrays(c, eta0, eta);
+      if ((rc = qd_copy_arrays(c, g0, omega)) || (rc = qd_copy_arrays(c, hs0, hs)) ||
+          (rc = qd_copy_arrays(c, eta0, eta)))
+        return done2(rc);
       stp = stp / 2.0 + R4_1EM3;
       rejected++;
       continue;
     }
-    if ((rc = qd_mbpt2(c, hs, &m2))) return done2(rc);
-    crit = std::fabs(m2);
+    if ((rc = eval_crit(hs))) return done2(rc);
     iteration++;
   }
   free_hist();
@@ -1744,8 +1778,10 @@ int qd_magnus_flow(qd_context *c, const qd_op *h0, qd_op *hs, qd_op *omega, cons
   res->iterations = iteration;
   res->rejected = rejected;
   res->bch_terms = terms;
-  res->converged = crit <= p->conv;
+  res->converged = crit <= conv;
   res->trace_rows = rows;
+  res->mbpt2 = m2;
+  res->deds = deds;
   if (p->verbose) fflush(stdout);
   return done(QD_OK);
 }
@@ -1828,6 +1864,7 @@ void qd_gs_params_default(qd_gs_params *p) {
   p->conv = R4_1EM7;
   p->max_outputs = 100000;
   p->verbose = 0;
+  p->maxnum = 500;
 }
 
 int qd_gs_flow(qd_context *c, const qd_op *h0, qd_op *hs, const qd_gs_params *p, qd_gs_result *res, double *trace,
@@ -1835,7 +1872,9 @@ int qd_gs_flow(qd_context *c, const qd_op *h0, qd_op *hs, const qd_gs_params *p,
   if (!c || !h0 || !hs || !p || !res) return fail(QD_E_ARG, "null argument");
   int rc;
   qd_ode *o;
+  if (p->maxnum < 1) return fail(QD_E_ARG, "maxnum must be positive");
   if ((rc = qd_ode_create(c, &o))) return rc;
+  o->a.maxnum = p->maxnum;
   memset(res, 0, sizeof *res);
   if ((rc = qd_copy_arrays(c, h0, hs))) { qd_ode_destroy(o); return rc; }
   hs->herm = 1;
@@ -1863,7 +1902,10 @@ int qd_gs_flow(qd_context *c, const qd_op *h0, qd_op *hs, const qd_gs_params *p,
       double *r = trace + 4 * (size_t)(pr - 1);
       r[0] = pr; r[1] = s; r[2] = e_new; r[3] = crit;
     }
-    if (flag != 2) break;  // tolerance raised (3), too many steps (4, 5) or bad input (6)
+    // As the reference's loop, the next call continues from a flag of 3
+    // (tolerances raised), 4 (500 steps) or 5 (stiff) towards the next output
+    // point; only 6 (bad input) ends the flow, unconverged.
+    if (flag == 6) break;
   }
   res->s = s;
   res->e0 = e_new;
@@ -1920,6 +1962,11 @@ int qd_set_graphs(qd_context *c, int on) {
   int rc = drop_graphs(c);
   c->use_graphs = on != 0;
   return rc;
+}
+
+int64_t qd_graph_count(const qd_context *c) {
+  if (!c) return fail(QD_E_ARG, "null context");
+  return (int64_t)c->graphs.size();
 }
 
 int qd_set_launch_mode(qd_context *c, int mode) {

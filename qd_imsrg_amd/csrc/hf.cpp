@@ -170,6 +170,8 @@ double v_int_impl(const int *qi, const int *qj, const int *qk, const int *ql, do
   return (double)(std::sqrt((ld)hw) * sm * std::sqrt(nrm));
 }
 
+}  // namespace
+
 // cyclic Jacobi eigensolver for a small symmetric matrix (row-major a[n*n]);
 // eigenvalues ascending, eigenvectors in the columns of v
 void jacobi(int n, std::vector<double> a, std::vector<double> &w, std::vector<double> &v) {
@@ -221,6 +223,8 @@ void jacobi(int n, std::vector<double> a, std::vector<double> &w, std::vector<do
   v.swap(vs);
 }
 
+namespace {
+
 // <ab|V|cd> at hw = 1 for m_a + m_b = m_c + m_d: for each (a, b, c) the d run
 // over the orbitals with m_d = m_a + m_b - m_c (compressed, N^3 x ~R/2)
 struct Spatial {
@@ -230,14 +234,25 @@ struct Spatial {
   std::vector<int> pos_in_m;            // position of an orbital in its m list
   std::vector<int64_t> start;           // N^3
   std::vector<double> tab;
+  std::vector<uint8_t> present;         // table read from ME.bin: element given in the file
   double get(int a, int b, int c, int d) const {
     const int md = m[a] + m[b] - m[c];
     if (md != m[d]) return 0.0;
     return tab[start[((size_t)a * N + b) * N + c] + pos_in_m[d]];
   }
+  // the exchange element <ab|V|dc> as construct_two_particle_HF_BASIS reads
+  // it from ME.bin (src/HFQDmod.f90:163-166): if it is (near) zero there,
+  // <ba|V|cd> is taken instead; for a complete table the two are equal
+  double get_exch(int a, int b, int c, int d) const {
+    const double x = get(a, b, d, c);
+    if (present.empty() || std::fabs(x) >= 1e-10) return x;
+    return get(b, a, c, d);
+  }
 };
 
-void build_spatial(int nshell, Spatial &S) {
+// orbitals (shell, then m ascending), per-m lists and the compressed offsets
+// of the (a, b, c) runs; the table is zero-filled
+void spatial_layout(int nshell, Spatial &S) {
   S.R = nshell;
   for (int R = 0; R < nshell; R++)
     for (int m = -R; m <= R; m += 2) { S.n.push_back((R - std::abs(m)) / 2); S.m.push_back(m); }
@@ -259,6 +274,11 @@ void build_spatial(int nshell, Spatial &S) {
         if (md >= -S.mmax && md <= S.mmax) k += (int64_t)S.by_m[md + S.mmax].size();
       }
   S.tab.assign(k, 0.0);
+}
+
+void build_spatial(int nshell, Spatial &S) {
+  spatial_layout(nshell, S);
+  const int N = S.N;
   // <ab|V|cd> is invariant under (a,b,c,d) -> (b,a,d,c) (particle exchange),
   // -> (c,d,a,b) (real and symmetric) and under m -> -m on all four orbitals
   // (time reversal); each orbit of this group of 8 is evaluated once, at its
@@ -364,7 +384,14 @@ double qd_v_int(const int *qi, const int *qj, const int *qk, const int *ql, doub
   return v_int_impl(qi, qj, qk, ql, hw);
 }
 
-int qd_hf_hamiltonian(int nbody, int nshell, double hw, double *h0, int64_t neq, double *eig_out, int *iters) {
+}  // extern "C"
+
+namespace qd {
+namespace {
+// construct_two_particle_HF_BASIS + HF_E2 on a Coulomb table S (<ab|V|cd> at
+// hw = 1 over the spatial orbitals of nshell shells)
+int hf_build(int nbody, int nshell, double hw, const Spatial &S, double *h0, int64_t neq, double *eig_out,
+             int *iters) {
   Channels ch;
   std::string why = build_channels(nbody, nshell, ch);
   if (!why.empty()) return set_error(QD_E_LAYOUT, "%s", why.c_str());
@@ -375,9 +402,6 @@ int qd_hf_hamiltonian(int nbody, int nshell, double hw, double *h0, int64_t neq,
   const bool trace = std::getenv("QD_HF_TRACE") != nullptr;
   auto now = [] { return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count(); };
   double t_mark = now();
-  const Spatial &S = spatial_table(nshell);
-  if (trace) fprintf(stderr, "hf: Coulomb table (R=%d) %.2f s\n", nshell, now() - t_mark);
-  t_mark = now();
   const double shw = std::sqrt(hw);
   // spin-orbital -> spatial orbital (nm_index order), spin
   std::vector<int> sp(M), spin(M);
@@ -392,7 +416,7 @@ int qd_hf_hamiltonian(int nbody, int nshell, double hw, double *h0, int64_t neq,
   auto asym = [&](int p, int q, int r, int s) -> double {  // <pq|V|rs>_A
     double v = 0;
     if (spin[p] == spin[r] && spin[q] == spin[s]) v += S.get(sp[p], sp[q], sp[r], sp[s]);
-    if (spin[p] == spin[s] && spin[q] == spin[r]) v -= S.get(sp[p], sp[q], sp[s], sp[r]);
+    if (spin[p] == spin[s] && spin[q] == spin[r]) v -= S.get_exch(sp[p], sp[q], sp[r], sp[s]);
     return shw * v;
   };
   // single-particle channels
@@ -564,6 +588,121 @@ int qd_hf_hamiltonian(int nbody, int nshell, double hw, double *h0, int64_t neq,
   if (eig_out) std::memcpy(eig_out, eig.data(), sizeof(double) * M);
   if (iters) *iters = total;
   if (crit > 1e-6) return set_error(QD_E_NUMERIC, "Hartree-Fock did not converge (crit %g after %d iterations)", crit, total);
+  return QD_OK;
+}
+
+// nm_index (src/ME_general.f90:2140-2149) less one: shell R = 2n + |m|
+int nm_orbital(int n, int m) {
+  const int R = 2 * n + std::abs(m);
+  return R * (R + 1) / 2 + (R + m) / 2;
+}
+
+// ME.bin as construct_two_particle_HF_BASIS reads it (src/HFQDmod.f90:97-129):
+// a stream of records n1,m1,n2,m2,n3,m3,n4,m4 (int32) and Vdir (real64),
+// Vdir = <ab|V|cd> at hw = 1 with a = (n1,m1) .. d = (n4,m4); records with an
+// orbital beyond nshell shells are skipped; each record sets <ab|V|cd> and
+// <cd|ab> (the file's later records win).  Records that break m_a + m_b =
+// m_c + m_d are never read by the HF and are dropped.
+int load_me_bin(const char *path, int nshell, Spatial &S, int64_t *nrec) {
+  FILE *f = std::fopen(path, "rb");
+  if (!f) return set_error(QD_E_ARG, "cannot open %s", path);
+  spatial_layout(nshell, S);
+  S.present.assign(S.tab.size(), 0);
+  const int N = S.N;
+  auto slot = [&](int a, int b, int c, int d) -> int64_t {
+    if (S.m[a] + S.m[b] != S.m[c] + S.m[d]) return -1;
+    return S.start[((size_t)a * N + b) * N + c] + S.pos_in_m[d];
+  };
+  int64_t n = 0;
+  for (;;) {
+    unsigned char rec[40];
+    const size_t got = std::fread(rec, 1, sizeof rec, f);
+    if (got == 0) break;
+    if (got != sizeof rec) { std::fclose(f); return set_error(QD_E_ARG, "%s: truncated record %lld ('failfile')", path, (long long)n + 1); }
+    int32_t q[8];
+    double v;
+    std::memcpy(q, rec, 32);
+    std::memcpy(&v, rec + 32, 8);
+    n++;
+    bool skip = false;
+    for (int k = 0; k < 4; k++)
+      if (q[2 * k] < 0 || 2 * q[2 * k] + std::abs(q[2 * k + 1]) > nshell - 1) skip = true;
+    if (skip) continue;
+    const int a = nm_orbital(q[0], q[1]), b = nm_orbital(q[2], q[3]), c = nm_orbital(q[4], q[5]),
+              d = nm_orbital(q[6], q[7]);
+    const int64_t s1 = slot(a, b, c, d), s2 = slot(c, d, a, b);
+    if (s1 < 0) continue;
+    S.tab[s1] = v; S.present[s1] = 1;
+    S.tab[s2] = v; S.present[s2] = 1;
+  }
+  std::fclose(f);
+  if (nrec) *nrec = n;
+  return QD_OK;
+}
+}  // namespace
+}  // namespace qd
+
+extern "C" {
+
+int qd_hf_hamiltonian(int nbody, int nshell, double hw, double *h0, int64_t neq, double *eig_out, int *iters) {
+  if (nshell < 1 || nshell > 40) return set_error(QD_E_LAYOUT, "nshell out of range");
+  const bool trace = std::getenv("QD_HF_TRACE") != nullptr;
+  auto t0 = std::chrono::steady_clock::now();
+  const Spatial &S = spatial_table(nshell);
+  if (trace)
+    fprintf(stderr, "hf: Coulomb table (R=%d) %.2f s\n", nshell,
+            std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count());
+  return hf_build(nbody, nshell, hw, S, h0, neq, eig_out, iters);
+}
+
+int qd_hf_hamiltonian_me(int nbody, int nshell, double hw, const char *me_path, double *h0, int64_t neq,
+                         double *eig_out, int *iters) {
+  if (!me_path) return set_error(QD_E_ARG, "null path");
+  if (nshell < 1 || nshell > 40) return set_error(QD_E_LAYOUT, "nshell out of range");
+  Spatial S;
+  int rc = load_me_bin(me_path, nshell, S, nullptr);
+  if (rc) return rc;
+  return hf_build(nbody, nshell, hw, S, h0, neq, eig_out, iters);
+}
+
+int qd_me_bin_write(int nshell, const char *me_path, int64_t *nrec) {
+  if (!me_path) return set_error(QD_E_ARG, "null path");
+  if (nshell < 1 || nshell > 40) return set_error(QD_E_LAYOUT, "nshell out of range");
+  const Spatial &S = spatial_table(nshell);
+  FILE *f = std::fopen(me_path, "wb");
+  if (!f) return set_error(QD_E_ARG, "cannot create %s", me_path);
+  // one record per (a, b, c, d) with m_a + m_b = m_c + m_d and (a, b) <= (c, d)
+  // (the loader stores each record at <ab|cd> and <cd|ab>)
+  int64_t n = 0;
+  const int N = S.N;
+  std::vector<unsigned char> buf;
+  buf.reserve(40 * 4096);
+  for (int a = 0; a < N; a++)
+    for (int b = 0; b < N; b++)
+      for (int c = 0; c < N; c++) {
+        const int md = S.m[a] + S.m[b] - S.m[c];
+        if (md < -S.mmax || md > S.mmax) continue;
+        for (int d : S.by_m[md + S.mmax]) {
+          if (std::make_pair(a, b) > std::make_pair(c, d)) continue;
+          const int32_t q[8] = {S.n[a], S.m[a], S.n[b], S.m[b], S.n[c], S.m[c], S.n[d], S.m[d]};
+          const double v = S.get(a, b, c, d);
+          unsigned char rec[40];
+          std::memcpy(rec, q, 32);
+          std::memcpy(rec + 32, &v, 8);
+          buf.insert(buf.end(), rec, rec + 40);
+          n++;
+          if (buf.size() >= 40 * 4096) {
+            if (std::fwrite(buf.data(), 1, buf.size(), f) != buf.size()) { std::fclose(f); return set_error(QD_E_ARG, "write failed: %s", me_path); }
+            buf.clear();
+          }
+        }
+      }
+  if (!buf.empty() && std::fwrite(buf.data(), 1, buf.size(), f) != buf.size()) {
+    std::fclose(f);
+    return set_error(QD_E_ARG, "write failed: %s", me_path);
+  }
+  if (std::fclose(f) != 0) return set_error(QD_E_ARG, "write failed: %s", me_path);
+  if (nrec) *nrec = n;
   return QD_OK;
 }
 

@@ -21,6 +21,7 @@ struct AdamsDevice {
   bool start = false, phase1 = false, nornd = false;
   int64_t nfe = 0;
   int nsteps = 0;
+  int maxnum = 500;  // steps per call before iflag 4 (src/adams_ode.f90:376)
 
   int init(hipStream_t stream, int64_t neq);
   void release();

@@ -457,7 +457,6 @@ int AdamsDevice::intrp(double xout, double *yout) {
 int AdamsDevice::ode(const OdeRhs &f, double *y, double *t, double tout, double *relerr, double *abserr,
                      int *iflag) {
   const double fouru = 4.0 * 2.220446049250313e-16;
-  const int maxnum = 500;
   if (n < 1 || *t == tout || *relerr < 0.0 || *abserr < 0.0) { *iflag = 6; return QD_OK; }
   double eps = std::max(*relerr, *abserr);
   if (eps <= 0.0 || *iflag == 0) { *iflag = 6; return QD_OK; }

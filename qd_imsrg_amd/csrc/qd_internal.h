@@ -83,6 +83,10 @@ hipStream_t context_stream(qd_context *c);
 int64_t context_neq(const qd_context *c);
 qd_context *op_context(qd_op *o);
 double *op_data(qd_op *o);
+const Channels &context_channels(const qd_context *c);
+// cyclic Jacobi eigensolver for a small symmetric matrix (row-major a[n*n]);
+// eigenvalues ascending in w, eigenvectors in the columns of v (hf.cpp)
+void jacobi(int n, std::vector<double> a, std::vector<double> &w, std::vector<double> &v);
 
 // Device copy of the tables, passed by value to kernels.
 struct DevTab {

@@ -16,12 +16,13 @@ module qdimsrg_c
 
   type, bind(C) :: qd_flow_params
      real(c_double) :: stp, conv
-     integer(c_int) :: max_iter, pandya, verbose, warm_start, diis
+     integer(c_int) :: max_iter, pandya, verbose, warm_start, diis, stop_rule
+     real(c_double) :: conv_deds
   end type qd_flow_params
 
   type, bind(C) :: qd_gs_params
      real(c_double) :: relerr, abserr, stp, conv
-     integer(c_int) :: max_outputs, verbose
+     integer(c_int) :: max_outputs, verbose, maxnum
   end type qd_gs_params
 
   type, bind(C) :: qd_gs_result
@@ -36,6 +37,7 @@ module qdimsrg_c
      integer(c_int) :: iterations, rejected
      integer(c_int64_t) :: bch_terms
      integer(c_int) :: converged, trace_rows
+     real(c_double) :: mbpt2, deds
   end type qd_flow_result
 
   interface
@@ -277,6 +279,19 @@ module qdimsrg_c
        integer(c_int64_t), value :: neq
        integer(c_int) :: iters
        integer(c_int) :: qd_hf_hamiltonian
+     end function
+     ! ---- on-disk format (src/ME_general.f90:207-361) ----
+     function qd_export_hamiltonian(op, path) bind(C, name="qd_export_hamiltonian")
+       import :: c_int, c_ptr, c_char
+       type(c_ptr), value :: op
+       character(kind=c_char), dimension(*) :: path
+       integer(c_int) :: qd_export_hamiltonian
+     end function
+     function qd_import_hamiltonian(op, path) bind(C, name="qd_import_hamiltonian")
+       import :: c_int, c_ptr, c_char
+       type(c_ptr), value :: op
+       character(kind=c_char), dimension(*) :: path
+       integer(c_int) :: qd_import_hamiltonian
      end function
      function qd_last_error() bind(C, name="qd_last_error")
        import :: c_ptr

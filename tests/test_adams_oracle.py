@@ -64,3 +64,39 @@ def test_bad_input_flag():
     a = Adams(lambda t, y: -y, 1)
     assert a.ode(np.array([1.0]), 1.0, 1.0, 1e-6, 1e-6, 1)[4] == 6      # t == tout
     assert a.ode(np.array([1.0]), 0.0, 1.0, -1e-6, 1e-6, 1)[4] == 6     # negative tolerance
+
+
+def test_gs_loop_flag3_without_progress_ends_it():
+    """gs_IMSRG's loop (src/gs_IMSRG.f90:100-120) passes the flag back into
+    the next ode call.  Tolerances below what the arithmetic supports make
+    the first call return 3 with raised tolerances before any step: y and t
+    are unchanged, so the loop's test |dE/E| sees 0 and the loop ends there
+    (a quirk of the reference kept as is)."""
+    from oracle.adams import gs_flow
+    from oracle.channels import Channels
+    from oracle.oracle import Oracle
+    from tests.pins.hamiltonians import model_hamiltonian
+    ch = Channels(2, 3)
+    o = Oracle(ch)
+    h0 = model_hamiltonian(ch, 4)
+    tight = gs_flow(o, h0, relerr=1e-18, abserr=1e-18)
+    assert tight["flags"] == [3] and tight["outputs"] == 1 and tight["s"] == 0.0
+    assert tight["relerr"] > 1e-18 and np.array_equal(tight["hs"], h0)
+
+
+def test_gs_loop_forced_too_many_steps_continues(monkeypatch):
+    """With the step limit per call cut to 3 (the reference's is 500), ode
+    returns 4 before the output point; the loop's next call continues and the
+    flow ends where the unlimited one does."""
+    import oracle.adams as ad
+    from oracle.channels import Channels
+    from oracle.oracle import Oracle
+    from tests.pins.hamiltonians import model_hamiltonian
+    ch = Channels(2, 3)
+    o = Oracle(ch)
+    h0 = model_hamiltonian(ch, 4)
+    ref = ad.gs_flow(o, h0)
+    monkeypatch.setattr(ad, "MAXNUM", 3)
+    cut = ad.gs_flow(o, h0)
+    assert 4 in cut["flags"] and cut["flags"][-1] == 2
+    assert abs(cut["e0"] - ref["e0"]) <= 1e-6 * abs(ref["e0"])

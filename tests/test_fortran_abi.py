@@ -1,0 +1,80 @@
+"""The Fortran side of the boundary, no GPU: every bind(C) derived type of
+qd_imsrg_amd/fortran/qdimsrg_c.f90 has the size and member offsets of its C
+struct in include/qdimsrg.h (an iso_c_binding layout slip would otherwise pass
+garbage through qd_magnus_flow / qd_gs_flow silently), and the two driver
+programs of the reference's job scripts are built under their names."""
+import os
+import shutil
+import subprocess
+
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+FLANG = os.environ.get("FLANG", "/opt/rocm/bin/amdflang")
+OBJ = os.path.join(ROOT, "qd_imsrg_amd", "_obj")
+
+# type -> members in declaration order (the same names on both sides)
+TYPES = {
+    "qd_info": ["nbody", "nshell", "msp", "nblock", "ncc", "neq", "cc_size", "n2b"],
+    "qd_flow_params": ["stp", "conv", "max_iter", "pandya", "verbose", "warm_start", "diis", "stop_rule",
+                       "conv_deds"],
+    "qd_flow_result": ["s", "e0", "crit", "iterations", "rejected", "bch_terms", "converged", "trace_rows",
+                       "mbpt2", "deds"],
+    "qd_gs_params": ["relerr", "abserr", "stp", "conv", "max_outputs", "verbose", "maxnum"],
+    "qd_gs_result": ["s", "e0", "crit", "outputs", "nsteps", "nfe", "iflag", "converged"],
+}
+
+
+def c_layout(tmp):
+    src = os.path.join(tmp, "lay.c")
+    lines = ['#include <stdio.h>', '#include <stddef.h>', f'#include "{ROOT}/include/qdimsrg.h"', "int main(void) {"]
+    for t, mem in TYPES.items():
+        lines.append(f'  printf("{t} %zu", sizeof({t}));')
+        for m in mem:
+            lines.append(f'  printf(" %zu", offsetof({t}, {m}));')
+        lines.append('  printf("\\n");')
+    lines.append("  return 0; }")
+    open(src, "w").write("\n".join(lines))
+    exe = os.path.join(tmp, "lay_c")
+    subprocess.run(["gcc", "-std=c11", src, "-o", exe], check=True)
+    return subprocess.run([exe], check=True, capture_output=True, text=True).stdout.split("\n")
+
+
+def f_layout(tmp):
+    src = os.path.join(tmp, "lay.f90")
+    lines = ["program lay", "  use iso_c_binding", "  use qdimsrg_c", "  implicit none"]
+    for t in TYPES:
+        lines.append(f"  type({t}), target :: v_{t}")
+    lines.append("  integer(c_intptr_t) :: b")
+    for t, mem in TYPES.items():
+        lines.append(f"  b = transfer(c_loc(v_{t}), b)")
+        lines.append(f"  write(*, '(A,1X,I0)', advance='no') '{t}', c_sizeof(v_{t})")
+        for m in mem:
+            lines.append(f"  write(*, '(1X,I0)', advance='no') transfer(c_loc(v_{t}%{m}), b) - b")
+        lines.append("  write(*, *)")
+    lines.append("end program lay")
+    open(src, "w").write("\n".join(lines))
+    exe = os.path.join(tmp, "lay_f")
+    subprocess.run([FLANG, src, "-I", OBJ, "-o", exe], check=True, capture_output=True)
+    return subprocess.run([exe], check=True, capture_output=True, text=True).stdout.split("\n")
+
+
+@pytest.mark.skipif(not os.path.exists(FLANG) or shutil.which("gcc") is None, reason="amdflang or gcc absent")
+def test_bind_c_types_match_header(tmp_path):
+    if not os.path.exists(os.path.join(OBJ, "qdimsrg_c.mod")):
+        pytest.skip("the Fortran module is not built (run __graft_entry__.build())")
+    c = {l.split()[0]: [int(x) for x in l.split()[1:]] for l in c_layout(str(tmp_path)) if l.strip()}
+    f = {l.split()[0]: [int(x) for x in l.split()[1:]] for l in f_layout(str(tmp_path)) if l.strip()}
+    assert set(c) == set(f) == set(TYPES)
+    for t in TYPES:
+        assert c[t] == f[t], (t, c[t], f[t])
+
+
+def test_drivers_built_under_reference_names():
+    """build_job_script.py:112 runs ./run_magnus N hwd0 R and :223
+    ./gs_decouple N hwd0 R 0 1 20 (src/makefile:3-6 names the programs)."""
+    bindir = os.path.join(ROOT, "qd_imsrg_amd", "bin")
+    if not os.path.isdir(bindir):
+        pytest.skip("drivers not built (run __graft_entry__.build())")
+    for prog in ("run_magnus", "gs_decouple"):
+        assert os.access(os.path.join(bindir, prog), os.X_OK), prog

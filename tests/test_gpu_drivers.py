@@ -1,0 +1,79 @@
+"""The Fortran driver programs of the reference's job scripts, run on the GPU
+(build_job_script.py:112 "./run_magnus N hwd0 R", :223 "./gs_decouple N hwd0
+R 0 1 20"): their stdout lines in the reference's edit descriptors, the TRAD
+summary and the exported H(s), checked against the library's own flows."""
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+import qd_imsrg_amd as qd
+
+pytestmark = pytest.mark.gpu
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+BIN = os.path.join(ROOT, "qd_imsrg_amd", "bin")
+
+
+def run(prog, args, cwd):
+    print(f"[case] {prog} {' '.join(args)}", flush=True)
+    r = subprocess.run([os.path.join(BIN, prog), *args], cwd=cwd, capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stdout[-2000:] + r.stderr[-2000:]
+    return r.stdout.splitlines()
+
+
+def fixed(line, widths):
+    out, k = [], 0
+    for w in widths:
+        out.append(line[k:k + w])
+        k += w
+    assert k == len(line.rstrip("\n")), (line, widths)
+    return out
+
+
+def test_run_magnus_lines_match_library(tmp_path):
+    """'(I5,5(f18.9))' per step: iteration, s, E0, ||eta_0||, ||eta||, crit;
+    no summary line after the flow (the reference stops at
+    src/magnus_IMSRG.f90:144)."""
+    lines = run("run_magnus", ["6", "1.0d0", "6"], str(tmp_path))
+    steps = [l for l in lines if len(l) == 95]
+    assert steps and all(len(l) == 95 or l.strip() == "faillll" for l in lines), lines[:3]
+    rows = np.array([[float(x) for x in fixed(l, [5] + [18] * 5)] for l in steps])
+    h0, _, _ = qd.hf_hamiltonian(6, 6, 1.0)
+    ctx = qd.Context(6, 6)
+    H0, HS, OM = ctx.operator(1, h0), ctx.operator(1), ctx.operator(-1)
+    res, trace = qd.magnus_flow(ctx, H0, HS, OM)
+    assert rows.shape[0] == trace.shape[0] == res.iterations + res.rejected
+    assert np.abs(rows - trace).max() <= 1e-9 + 1e-9 * np.abs(trace).max()
+    assert abs(rows[-1, 2] - res.e0) <= 1e-9
+    ctx.close()
+
+
+def test_gs_decouple_outputs_match_library(tmp_path):
+    """'(I5,3(e14.6))' per output point; '(I4,I5,3(f15.9),f15.4)' appended to
+    ../output/TRAD_6_1.00.dat; H(s) in ../hamiltonians/ham_6_1.00_6."""
+    work = tmp_path / "run"
+    for d in ("run", "output", "hamiltonians"):
+        (tmp_path / d).mkdir()
+    lines = run("gs_decouple", ["6", "1.0d0", "6", "0", "1", "20"], str(work))
+    assert lines[0].strip().startswith("Hartree-Fock Energy:")
+    outs = [l for l in lines[1:] if len(l) == 47]
+    rows = np.array([[float(x) for x in fixed(l, [5, 14, 14, 14])] for l in outs])
+    h0, _, _ = qd.hf_hamiltonian(6, 6, 1.0)
+    ctx = qd.Context(6, 6)
+    H0, HS = ctx.operator(1, h0), ctx.operator(1)
+    res, trace = qd.gs_flow(ctx, H0, HS)
+    assert rows.shape[0] == res.outputs
+    assert np.array_equal(rows[:, 0], trace[:, 0])
+    # E14.6 keeps six significant digits
+    err = np.abs(rows[:, 1:3] - trace[:, 1:3]) / np.abs(trace[:, 1:3])
+    assert float(err.max()) <= 5.01e-7
+    trad = (tmp_path / "output" / "TRAD_6_1.00.dat").read_text().splitlines()
+    assert len(trad) == 1 and len(trad[0]) == 4 + 5 + 45 + 15
+    f = fixed(trad[0], [4, 5, 15, 15, 15, 15])
+    assert (int(f[0]), int(f[1])) == (6, 42)
+    assert abs(float(f[2]) - h0[0]) <= 1e-9 and abs(float(f[4]) - res.e0) <= 1e-9
+    ham = qd.import_hamiltonian_host(6, 6, str(tmp_path / "hamiltonians" / "ham_6_1.00_6"))
+    hs = HS.download()
+    assert np.abs(ham - hs).max() <= 1e-10 * np.abs(hs).max()
+    ctx.close()

@@ -129,3 +129,87 @@ def test_ode_integrate_entry_point():
     assert (st["nsteps"], st["nfe"]) == (a.nsteps, a.nfe)
     ode.close()
     ctx.close()
+
+
+@pytest.mark.parametrize("pandya", [False, True])
+def test_deds_stop_rule_matches_oracle(pandya):
+    """The stopping rule |dE/ds| < 1e-8 Ha (dE/ds = zero-body part of
+    [eta(s), H(s)]) at N=6 R=6 hw=1 (HF basis): same accepted and rejected
+    steps as the oracle's flow under the same rule, E_gs to 1e-10 relative,
+    and the final |dE/ds| below the threshold on both sides."""
+    print(f"[case] deds rule N=6 R=6 pandya={pandya}", flush=True)
+    ch = Channels(6, 6)
+    o = Oracle(ch)
+    h0, _, _ = qd.hf_hamiltonian(6, 6, 1.0)
+    ref = magnus_flow(o, h0, pandya=pandya, stop_rule="deds", conv_deds=1e-8)
+    ctx = qd.Context(6, 6)
+    H0, HS, OM = ctx.operator(1, h0), ctx.operator(1), ctx.operator(-1)
+    res, _ = qd.magnus_flow(ctx, H0, HS, OM, pandya=pandya, stop_rule="deds", conv_deds=1e-8)
+    print(f"iterations {res.iterations} ({ref['iterations']}), dE/ds {res.deds:.3e} ({ref['deds']:.3e})")
+    assert res.converged and abs(res.deds) <= 1e-8 and abs(ref["deds"]) <= 1e-8
+    assert (res.iterations, res.rejected) == (ref["iterations"], ref["rejected"])
+    assert abs(res.e0 - ref["e0"]) <= 1e-10 * abs(ref["e0"])
+    assert abs(res.mbpt2 - ref["mbpt2"]) <= 1e-10 * max(abs(ref["mbpt2"]), 1e-12) + 1e-18
+    ctx.close()
+
+
+def test_gs_flow_continues_past_step_limit(monkeypatch):
+    """qd_gs_flow with the integrator's step limit per call cut to 3 (the
+    reference's is 500): ode returns 4 before output points, the loop calls
+    it again with that flag, as gs_IMSRG does, and the flow matches the
+    oracle's loop under the same limit (outputs, steps, RHS evaluations,
+    E_gs to 1e-10)."""
+    import oracle.adams as ad
+    print("[case] gs flow maxnum=3 N=2 R=3", flush=True)
+    ch = Channels(2, 3)
+    o = Oracle(ch)
+    h0 = model_hamiltonian(ch, 4)
+    monkeypatch.setattr(ad, "MAXNUM", 3)
+    ref = ad.gs_flow(o, h0)
+    assert 4 in ref["flags"]
+    ctx = qd.Context(2, 3)
+    H0, HS = ctx.operator(1, h0), ctx.operator(1)
+    res, trace = qd.gs_flow(ctx, H0, HS, maxnum=3)
+    assert res.converged and res.iflag == 2
+    assert (res.outputs, res.nsteps, res.nfe) == (ref["outputs"], ref["nsteps"], ref["nfe"])
+    assert abs(res.e0 - ref["e0"]) <= 1e-10 * abs(ref["e0"])
+    ctx.close()
+
+
+def test_graph_cache_drops_destroyed_operands():
+    """A graph captured on an operand set is dropped when one of its
+    operators or CC sets is destroyed."""
+    ch = Channels(6, 6)
+    ctx = qd.Context(6, 6)
+    ctx.set_graphs(True)
+    rng = np.random.default_rng(9)
+    from tests.pins.layout import random_flat
+    X, Y = ctx.operator(-1, random_flat(ch, rng, -1)), ctx.operator(1, random_flat(ch, rng, 1))
+    HCC, ECC = ctx.cc(1.0), ctx.cc(-1.0)
+    for _ in range(3):
+        Z = ctx.operator(1)
+        qd.commutator_cc(X, Y, Z, HCC, ECC)
+        assert ctx.graph_count() == 1
+        Z.close()
+        assert ctx.graph_count() == 0
+    Z = ctx.operator(1)
+    qd.commutator_cc(X, Y, Z, HCC, ECC)
+    HCC.close()
+    assert ctx.graph_count() == 0
+    ctx.close()
+
+
+def test_export_import_device_operator(tmp_path):
+    """export_hamiltonian of a device operator, read back into another one:
+    bit for bit, and equal to the host-vector export of the same values."""
+    ch = Channels(6, 6)
+    h0, _, _ = qd.hf_hamiltonian(6, 6, 1.0)
+    ctx = qd.Context(6, 6)
+    H, G = ctx.operator(1, h0), ctx.operator(1)
+    p1, p2 = str(tmp_path / "ham_dev"), str(tmp_path / "ham_host")
+    qd.export_hamiltonian(H, p1)
+    qd.export_hamiltonian_host(6, 6, h0, p2)
+    assert open(p1, "rb").read() == open(p2, "rb").read()
+    qd.import_hamiltonian(G, p1)
+    assert np.array_equal(G.download(), h0)
+    ctx.close()
