@@ -65,9 +65,9 @@ def test_gs_decouple_outputs_match_library(tmp_path):
     res, trace = qd.gs_flow(ctx, H0, HS)
     assert rows.shape[0] == res.outputs
     assert np.array_equal(rows[:, 0], trace[:, 0])
-    # E14.6 keeps six significant digits
+    # E14.6 prints 0.dddddd: half a unit of the sixth digit is up to 5e-6 relative
     err = np.abs(rows[:, 1:3] - trace[:, 1:3]) / np.abs(trace[:, 1:3])
-    assert float(err.max()) <= 5.01e-7
+    assert float(err.max()) <= 5.01e-6
     trad = (tmp_path / "output" / "TRAD_6_1.00.dat").read_text().splitlines()
     assert len(trad) == 1 and len(trad[0]) == 4 + 5 + 45 + 15
     f = fixed(trad[0], [4, 5, 15, 15, 15, 15])
