@@ -148,6 +148,15 @@ int qd_set_to_zero(qd_context *ctx, qd_op *r);
 int qd_mat_2_norm(qd_context *ctx, const qd_op *r, double *out);
 /* MBPT2(rec), src/HFQDmod.f90:325-362 */
 int qd_mbpt2(qd_context *ctx, const qd_op *h, double *out);
+/* qd_mbpt2 for callers that cannot allocate (code under graph capture, tight
+ * loops, harnesses that forbid hipMalloc); no counterpart in the reference.
+ * Same kernels and result as qd_mbpt2, but the reduction partials and the
+ * result live in a file-scope device array of 4096 + 8 doubles (32 KB per
+ * GPU, held until the process ends), so the call allocates nothing.  A
+ * channel structure needing more than 4096 partials (one per 256 pphh
+ * elements; N=42 R=20 needs 2710) is rejected with QD_E_ARG.  All calls on
+ * one GPU share that array: calls that overlap on one GPU race. */
+int qd_mbpt2_noalloc(qd_context *ctx, const qd_op *h, double *out);
 
 /* ---- whole commutator and flow-level entry points ----------------------
  * Full [r1,r2] -> r3 (0b in r3's E0, also returned in *e0 if non-null), in

@@ -80,6 +80,7 @@ SIGNATURES = {
     "qd_set_to_zero": (C.c_int, [vp, vp]),
     "qd_mat_2_norm": (C.c_int, [vp, vp, dp]),
     "qd_mbpt2": (C.c_int, [vp, vp, dp]),
+    "qd_mbpt2_noalloc": (C.c_int, [vp, vp, dp]),
     "qd_commutator": (C.c_int, [vp, vp, vp, vp, vp, vp, dp]),
     "qd_commutator_cc": (C.c_int, [vp, vp, vp, vp, vp, vp, dp]),
     "qd_bch_expand": (C.c_int, [vp, vp, vp, vp, vp, vp, ip]),

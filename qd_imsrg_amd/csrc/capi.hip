@@ -6,6 +6,7 @@
 #include <cstdio>
 #include <cstring>
 #include <map>
+#include <mutex>
 #include <tuple>
 #include <string>
 #include <vector>
@@ -122,6 +123,11 @@ struct qd_context {
   };
   std::map<GraphKey, hipGraphExec_t> graphs;
   int gemm_grid = 1024;   // persistent grid of the grouped launch
+  // allocation-free MBPT2: this device's address of the file-scope scratch
+  // array and the partial count it may use (NOALLOC_PARTS, lowered by
+  // QD_NOALLOC_CAP at creation)
+  double *noalloc_scratch = nullptr;
+  int noalloc_cap = 0;
   // scratch operators and CC sets of the flow drivers (created on first use)
   qd_op *bch_a = nullptr, *bch_b = nullptr;
   qd_cc *cc_t1 = nullptr, *cc_t2 = nullptr;
@@ -144,6 +150,35 @@ struct qd_cc {
   double herm;
   bool zero;  // known to be all zeros (skips the Pandya work, exactly)
 };
+
+// Scratch of qd_mbpt2_noalloc, the entry point that allocates nothing: the
+// reduction partials (one per pphh tile) and the result.  One copy per GPU,
+// held until the process ends; calls that overlap on one GPU share it and
+// race.  Host code never uses the array's name as an address: the device
+// address comes from hipGetSymbolAddress once per device (noalloc_setup).
+constexpr int NOALLOC_PARTS = 4096;
+static __device__ __align__(256) double scratch[NOALLOC_PARTS + 8];
+
+namespace {
+std::mutex g_scratch_mu;
+std::map<int, double *> g_scratch_ptr;  // device -> address of its scratch
+
+// the calling thread's current device is the context's device
+int noalloc_setup(double **out) {
+  int dev = 0;
+  HIPC(hipGetDevice(&dev));
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  auto it = g_scratch_ptr.find(dev);
+  if (it == g_scratch_ptr.end()) {
+    double *ptr = nullptr;
+    hipError_t e = hipGetSymbolAddress((void **)&ptr, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !ptr) return fail(QD_E_HIP, "hipGetSymbolAddress(scratch): %s", hipGetErrorString(e));
+    it = g_scratch_ptr.emplace(dev, ptr).first;
+  }
+  *out = it->second;
+  return QD_OK;
+}
+}  // namespace
 
 hipStream_t qd::context_stream(qd_context *c) { return c->stream; }
 int64_t qd::context_neq(const qd_context *c) { return c->ch.neq; }
@@ -1167,6 +1202,13 @@ int qd_context_create(int nbody, int nshell, qd_context **out) {
   if ((rc = build_worklists(c))) return bail(rc);
   if (c->npphh_tiles > 1024 * 64 || c->npphh_segs + 1 > 1024 * 64)
     return bail(fail(QD_E_LAYOUT, "too many pphh tiles for the reduction buffer"));
+  if ((rc = noalloc_setup(&c->noalloc_scratch))) return bail(rc);
+  c->noalloc_cap = NOALLOC_PARTS;
+  if (const char *env = getenv("QD_NOALLOC_CAP")) {
+    // a lower cap exercises the size check without a huge channel structure
+    const int v = atoi(env);
+    if (v >= 1 && v <= NOALLOC_PARTS) c->noalloc_cap = v;
+  }
   *out = c;
   return QD_OK;
 }
@@ -1444,6 +1486,25 @@ int qd_mbpt2(qd_context *c, const qd_op *h, double *out) {
     HIPC(hipGetLastError());
   }
   return read_scalars(c, 1, out);
+}
+
+// Allocation-free MBPT2: the kernels of qd_mbpt2 with the partials and the
+// result in the file-scope scratch array instead of the context's buffers; no
+// allocation, no timing events.  Every partial is written before it is read,
+// so nothing needs clearing between calls.
+int qd_mbpt2_noalloc(qd_context *c, const qd_op *h, double *out) {
+  if (!c || !h || !out) return fail(QD_E_ARG, "null argument");
+  if (h->ctx != c) return fail(QD_E_ARG, "operator belongs to another context");
+  if (!c->noalloc_scratch) return fail(QD_E_HIP, "allocation-free scratch not set up for this context");
+  if (c->npphh_tiles > c->noalloc_cap)
+    return fail(QD_E_ARG, "MBPT2 needs %d reduction partials; the allocation-free scratch holds %d", c->npphh_tiles,
+                c->noalloc_cap);
+  double *part = c->noalloc_scratch, *dst = c->noalloc_scratch + NOALLOC_PARTS;
+  launch_mbpt2(c->stream, c->dt, h->d, c->pphh_segs, c->pphh_tiles, c->npphh_tiles, part, dst);
+  HIPC(hipGetLastError());
+  HIPC(hipMemcpyAsync(out, dst, sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  HIPC(hipStreamSynchronize(c->stream));
+  return QD_OK;
 }
 
 // ---- whole commutator, BCH, Magnus ----------------------------------------------
