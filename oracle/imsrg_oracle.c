@@ -85,24 +85,61 @@ static inline int same_ch(const tab_t *t, i64 a, i64 b) {
   return t->ml[a] == t->ml[b] && t->ms[a] == t->ms[b];
 }
 
-/* column-major C(m,n) = alpha*op(A)*op(B) + beta*C, the dgemm contract */
+/* column-major C(m,n) = alpha*op(A)*op(B) + beta*C, the dgemm contract.
+ * Every element is the plain sum over l = 0..k-1 in order (no fused
+ * multiply-add: ISO C, contraction off), computed for 8 x 4 elements at a
+ * time so that the compiler vectorises across elements. */
+#define GEMM_MI 8
+#define GEMM_NJ 4
+typedef double v4d __attribute__((vector_size(32)));
 static void gemm(char ta, char tb, i64 m, i64 n, i64 k, double alpha, const double *A, i64 lda,
                  const double *B, i64 ldb, double beta, double *C, i64 ldc) {
   if (m <= 0 || n <= 0) return;
-  double *a = (double *)malloc(sizeof(double) * (size_t)(m * (k > 0 ? k : 1)));
-  double *b = (double *)malloc(sizeof(double) * (size_t)(n * (k > 0 ? k : 1)));
-  for (i64 i = 0; i < m; i++)
-    for (i64 l = 0; l < k; l++) a[i * k + l] = (ta == 'N') ? A[i + l * lda] : A[l + i * lda];
+  const i64 kk = k > 0 ? k : 1;
+  const i64 mp = (m + GEMM_MI - 1) / GEMM_MI * GEMM_MI;  /* rows padded with zeros */
+  /* a: panels of GEMM_MI rows, a[(i/MI)*MI*k + l*MI + i%MI] = op(A)(i,l) */
+  double *a = (double *)calloc((size_t)(mp * kk), sizeof(double));
+  double *b = (double *)malloc(sizeof(double) * (size_t)(n * kk));  /* b[j*k + l] = op(B)(l,j) */
+  for (i64 l = 0; l < k; l++)
+    for (i64 i = 0; i < m; i++)
+      a[(i / GEMM_MI) * GEMM_MI * kk + l * GEMM_MI + i % GEMM_MI] = (ta == 'N') ? A[i + l * lda] : A[l + i * lda];
   for (i64 j = 0; j < n; j++)
     for (i64 l = 0; l < k; l++) b[j * k + l] = (tb == 'N') ? B[l + j * ldb] : B[j + l * ldb];
-  for (i64 j = 0; j < n; j++)
-    for (i64 i = 0; i < m; i++) {
-      double s = 0.0;
-      const double *ai = a + i * k, *bj = b + j * k;
-      for (i64 l = 0; l < k; l++) s += ai[l] * bj[l];
-      double c = (beta == 0.0) ? 0.0 : beta * C[i + j * ldc];
-      C[i + j * ldc] = c + alpha * s;
+  for (i64 j0 = 0; j0 < n; j0 += GEMM_NJ) {
+    const i64 nj = n - j0 < GEMM_NJ ? n - j0 : GEMM_NJ;
+    const double *b0 = b + j0 * k, *b1 = b0 + (nj > 1 ? k : 0), *b2 = b0 + (nj > 2 ? 2 * k : 0),
+                 *b3 = b0 + (nj > 3 ? 3 * k : 0);
+    for (i64 i0 = 0; i0 < m; i0 += GEMM_MI) {
+      /* eight rows as two 4-wide vectors per column: accumulators stay in
+       * registers; separate multiply and add, as the scalar sum does */
+      v4d s0a = {0}, s0b = {0}, s1a = {0}, s1b = {0}, s2a = {0}, s2b = {0}, s3a = {0}, s3b = {0};
+      const double *ap = a + i0 * kk;
+      for (i64 l = 0; l < k; l++) {
+        v4d lo, hi;
+        memcpy(&lo, ap + l * GEMM_MI, sizeof lo);
+        memcpy(&hi, ap + l * GEMM_MI + 4, sizeof hi);
+        const v4d x0 = {b0[l], b0[l], b0[l], b0[l]}, x1 = {b1[l], b1[l], b1[l], b1[l]};
+        const v4d x2 = {b2[l], b2[l], b2[l], b2[l]}, x3 = {b3[l], b3[l], b3[l], b3[l]};
+        v4d p;
+        p = lo * x0; s0a = s0a + p; p = hi * x0; s0b = s0b + p;
+        p = lo * x1; s1a = s1a + p; p = hi * x1; s1b = s1b + p;
+        p = lo * x2; s2a = s2a + p; p = hi * x2; s2b = s2b + p;
+        p = lo * x3; s3a = s3a + p; p = hi * x3; s3b = s3b + p;
+      }
+      double s0[GEMM_MI], s1[GEMM_MI], s2[GEMM_MI], s3[GEMM_MI];
+      memcpy(s0, &s0a, 32); memcpy(s0 + 4, &s0b, 32);
+      memcpy(s1, &s1a, 32); memcpy(s1 + 4, &s1b, 32);
+      memcpy(s2, &s2a, 32); memcpy(s2 + 4, &s2b, 32);
+      memcpy(s3, &s3a, 32); memcpy(s3 + 4, &s3b, 32);
+      const double *sv[GEMM_NJ] = {s0, s1, s2, s3};
+      for (i64 jj = 0; jj < nj; jj++)
+        for (i64 ii = 0; ii < GEMM_MI && i0 + ii < m; ii++) {
+          double *cij = &C[(i0 + ii) + (j0 + jj) * ldc];
+          const double c = (beta == 0.0) ? 0.0 : beta * *cij;
+          *cij = c + alpha * sv[jj][ii];
+        }
     }
+  }
   free(a);
   free(b);
 }
@@ -280,12 +317,65 @@ static inline double frow(const tab_t *t, const double *v, i64 x, i64 tt) {
   return (x < n) ? FHH(v, x, tt) : FPH(v, x - n, tt);
 }
 
-static double c122_half(const tab_t *t, const double *fa, int ha, const double *vb, int hb,
+/* The states t that can contribute to an element: those in the (m_l, m_s)
+ * channel of one of x1..x4 (f conserves the channel; every other t has all
+ * four terms zero).  chan[k] lists the states of a channel in ascending
+ * order, cstart/clen give each state's run.  The candidates are visited in
+ * ascending order, as a loop over all t would visit them. */
+typedef struct {
+  i64 *chan, *cstart, *clen;
+} chan_t;
+
+static chan_t make_chan(const tab_t *t) {
+  chan_t c;
+  i64 M = t->M;
+  c.chan = malloc(sizeof(i64) * (size_t)M);
+  c.cstart = malloc(sizeof(i64) * (size_t)M);
+  c.clen = malloc(sizeof(i64) * (size_t)M);
+  i64 k = 0;
+  char *done = calloc((size_t)M, 1);
+  for (i64 a = 0; a < M; a++) {
+    if (done[a]) continue;
+    i64 s0 = k;
+    for (i64 b = a; b < M; b++)
+      if (t->ml[b] == t->ml[a] && t->ms[b] == t->ms[a]) { c.chan[k++] = b; done[b] = 1; }
+    for (i64 j = s0; j < k; j++) { c.cstart[c.chan[j]] = s0; c.clen[c.chan[j]] = k - s0; }
+  }
+  free(done);
+  return c;
+}
+
+static void free_chan(chan_t c) { free(c.chan); free(c.cstart); free(c.clen); }
+
+/* sorted, distinct candidates t for the element (x1, x2, x3, x4); returns the count */
+static int candidates(const chan_t *c, i64 x1, i64 x2, i64 x3, i64 x4, i64 *out) {
+  const i64 xs[4] = {x1, x2, x3, x4};
+  int n = 0;
+  for (int q = 0; q < 4; q++) {
+    int seen = 0;
+    for (int r = 0; r < q; r++)
+      if (c->cstart[xs[r]] == c->cstart[xs[q]]) seen = 1;
+    if (seen) continue;
+    for (i64 j = 0; j < c->clen[xs[q]]; j++) out[n++] = c->chan[c->cstart[xs[q]] + j];
+  }
+  for (int i = 1; i < n; i++) {  /* insertion sort: a few dozen entries */
+    i64 v = out[i];
+    int j = i - 1;
+    while (j >= 0 && out[j] > v) { out[j + 1] = out[j]; j--; }
+    out[j + 1] = v;
+  }
+  return n;
+}
+
+static double c122_half(const tab_t *t, const chan_t *c, const double *fa, int ha, const double *vb, int hb,
                         i64 x1, i64 x2, i64 x3, i64 x4) {
-  i64 n = t->n, M = t->M;
+  i64 n = t->n;
+  i64 cand[4 * 64];
+  int nc = candidates(c, x1, x2, x3, x4, cand);
   double r = 0.0;
   /* hole sums: t in the (m_l, m_s) channel of the replaced index */
-  for (i64 tt = 0; tt < n; tt++) {
+  for (int k = 0; k < nc && cand[k] < n; k++) {
+    i64 tt = cand[k];
     double s = 0.0;
     if (same_ch(t, tt, x1)) s += frow(t, fa, x1, tt) * o_velem(t, vb, hb, tt, x2, x3, x4);
     if (same_ch(t, tt, x2)) s += frow(t, fa, x2, tt) * o_velem(t, vb, hb, x1, tt, x3, x4);
@@ -294,7 +384,9 @@ static double c122_half(const tab_t *t, const double *fa, int ha, const double *
     if (same_ch(t, tt, x4)) u += frow(t, fa, x4, tt) * o_velem(t, vb, hb, x1, x2, x3, tt);
     r += s - ha * u;
   }
-  for (i64 tt = n; tt < M; tt++) {
+  for (int k = 0; k < nc; k++) {
+    i64 tt = cand[k];
+    if (tt < n) continue;
     double s = 0.0;
     if (same_ch(t, tt, x1)) s += fcol(t, fa, tt, x1) * o_velem(t, vb, hb, tt, x2, x3, x4);
     if (same_ch(t, tt, x2)) s += fcol(t, fa, tt, x2) * o_velem(t, vb, hb, x1, tt, x3, x4);
@@ -306,38 +398,48 @@ static double c122_half(const tab_t *t, const double *fa, int ha, const double *
   return r;
 }
 
-static inline double c122_elem(const tab_t *t, const double *v1, int h1, const double *v2, int h2,
-                               i64 x1, i64 x2, i64 x3, i64 x4) {
-  return c122_half(t, v1, h1, v2, h2, x1, x2, x3, x4) - c122_half(t, v2, h2, v1, h1, x1, x2, x3, x4);
+static inline double c122_elem(const tab_t *t, const chan_t *c, const double *v1, int h1, const double *v2,
+                               int h2, i64 x1, i64 x2, i64 x3, i64 x4) {
+  return c122_half(t, c, v1, h1, v2, h2, x1, x2, x3, x4) - c122_half(t, c, v2, h2, v1, h1, x1, x2, x3, x4);
 }
 
 void o_comm122(const tab_t *t, const double *v1, int h1, const double *v2, int h2, double *v3) {
-#pragma omp parallel for schedule(dynamic)
+  /* blocks in turn, the columns of each sub-block in parallel (the largest
+   * blocks hold most of the elements) */
+  chan_t cs = make_chan(t);
+  const chan_t *c = &cs;
   for (i64 q = 0; q < t->nblock; q++) {
     i64 nh = t->nh[q], np = t->np[q], nb = t->nb[q];
     const i64 *o = t->off + 6 * q;
     const i64 *HH = t->qnhh + 2 * t->hh_start[q], *PP = t->qnpp + 2 * t->pp_start[q],
               *PH = t->qnph + 2 * t->ph_start[q];
+#pragma omp parallel for schedule(dynamic)
     for (i64 J = 0; J < nh; J++)
       for (i64 I = 0; I < nh; I++)
-        v3[o[HHHH] + I + J * nh] = c122_elem(t, v1, h1, v2, h2, HH[2 * I], HH[2 * I + 1], HH[2 * J], HH[2 * J + 1]);
+        v3[o[HHHH] + I + J * nh] = c122_elem(t, c, v1, h1, v2, h2, HH[2 * I], HH[2 * I + 1], HH[2 * J], HH[2 * J + 1]);
+#pragma omp parallel for schedule(dynamic)
     for (i64 J = 0; J < np; J++)
       for (i64 I = 0; I < np; I++)
-        v3[o[PPPP] + I + J * np] = c122_elem(t, v1, h1, v2, h2, PP[2 * I], PP[2 * I + 1], PP[2 * J], PP[2 * J + 1]);
+        v3[o[PPPP] + I + J * np] = c122_elem(t, c, v1, h1, v2, h2, PP[2 * I], PP[2 * I + 1], PP[2 * J], PP[2 * J + 1]);
+#pragma omp parallel for schedule(dynamic)
     for (i64 J = 0; J < nh; J++)
       for (i64 I = 0; I < np; I++)
-        v3[o[PPHH] + I + J * np] = c122_elem(t, v1, h1, v2, h2, PP[2 * I], PP[2 * I + 1], HH[2 * J], HH[2 * J + 1]);
+        v3[o[PPHH] + I + J * np] = c122_elem(t, c, v1, h1, v2, h2, PP[2 * I], PP[2 * I + 1], HH[2 * J], HH[2 * J + 1]);
     /* ph pairs: particle first, pre = -1 (src/commutators.f90:466-471) */
+#pragma omp parallel for schedule(dynamic)
     for (i64 J = 0; J < nh; J++)
       for (i64 I = 0; I < nb; I++)
-        v3[o[PHHH] + I + J * nb] = -c122_elem(t, v1, h1, v2, h2, PH[2 * I + 1], PH[2 * I], HH[2 * J], HH[2 * J + 1]);
+        v3[o[PHHH] + I + J * nb] = -c122_elem(t, c, v1, h1, v2, h2, PH[2 * I + 1], PH[2 * I], HH[2 * J], HH[2 * J + 1]);
+#pragma omp parallel for schedule(dynamic)
     for (i64 J = 0; J < nb; J++)
       for (i64 I = 0; I < np; I++)
-        v3[o[PPPH] + I + J * np] = -c122_elem(t, v1, h1, v2, h2, PP[2 * I], PP[2 * I + 1], PH[2 * J + 1], PH[2 * J]);
+        v3[o[PPPH] + I + J * np] = -c122_elem(t, c, v1, h1, v2, h2, PP[2 * I], PP[2 * I + 1], PH[2 * J + 1], PH[2 * J]);
+#pragma omp parallel for schedule(dynamic)
     for (i64 J = 0; J < nb; J++)
       for (i64 I = 0; I < nb; I++)
-        v3[o[PHPH] + I + J * nb] = c122_elem(t, v1, h1, v2, h2, PH[2 * I + 1], PH[2 * I], PH[2 * J + 1], PH[2 * J]);
+        v3[o[PHPH] + I + J * nb] = c122_elem(t, c, v1, h1, v2, h2, PH[2 * I + 1], PH[2 * I], PH[2 * J + 1], PH[2 * J]);
   }
+  free_chan(cs);
 }
 
 /* construct_cc_ints, src/ME_general.f90:923-952:

@@ -1,10 +1,11 @@
 """Precision form: the 2b-2b GEMMs in FP32 or in 2..4 BF16 mantissa slices
 (FP32 MFMA accumulation per K stage of 32, FP64 across stages), everything
-else FP64.  Errors are measured against the plain FP64 GPU form on the same
-inputs and must stay inside the stated bounds:
+else FP64.  Errors are measured against the FP64 oracle (oracle/, the CPU
+restatement of the reference) and, as a second line, against the plain FP64
+GPU form on the same inputs, and must stay inside the stated bounds:
 
-  commutator at N=20 R=14 (random X, Y), max |err| / max |[X,Y]_fp64|:
-      fp32, bf16x3, bf16x4 <= 1e-6;  bf16x2 <= 1e-3
+  commutator at N=20 R=14 (random X, Y), max |err| / max |[X,Y]_oracle|:
+      fp64 <= 1e-12; fp32, bf16x3, bf16x4 <= 1e-6;  bf16x2 <= 1e-3
   Magnus flow at N=6 R=6 hw=1 (HF basis): |E_gs - E_gs(fp64)| <= 1e-6 Ha for
       fp32, bf16x3, bf16x4; relative error of ||Omega|| <= 1e-4 and
       absolute error of ||Gamma_od|| (small at convergence) <= 1e-5.
@@ -16,6 +17,7 @@ import pytest
 
 import qd_imsrg_amd as qd
 from oracle.channels import SUB_PPHH, Channels
+from oracle.oracle import Oracle
 from tests.pins.layout import random_flat
 
 pytestmark = pytest.mark.gpu
@@ -52,21 +54,25 @@ def test_commutator_errors_per_precision():
     ctx = qd.Context(20, 14)
     rng = np.random.default_rng(41)
     x, y = random_flat(ch, rng, -1), random_flat(ch, rng, 1)
+    oracle = Oracle(ch).commutator(x, -1, y, 1)
     e_ref, ref = full(ctx, x, y)
-    errs = {}
+    errs, errs_gpu = {"fp64": rel(ref, oracle)}, {}
     for p in ("fp32", "bf16x2", "bf16x3", "bf16x4"):
         ctx.set_precision(p)
         assert ctx.precision() == p
         e0, z = full(ctx, x, y)
-        errs[p] = rel(z, ref)
+        errs[p] = rel(z, oracle)
+        errs_gpu[p] = rel(z, ref)
         assert e0 == e_ref  # zero-body traces stay FP64 and do not use the GEMMs
     ctx.set_precision("fp64")
     e_again, again = full(ctx, x, y)
     ctx.close()
-    print("commutator max-rel error vs FP64:", {k: f"{v:.2e}" for k, v in errs.items()})
+    print("commutator max-rel error vs the FP64 oracle:", {k: f"{v:.2e}" for k, v in errs.items()})
+    print("                    vs the FP64 GPU form:", {k: f"{v:.2e}" for k, v in errs_gpu.items()})
     assert np.array_equal(again, ref) and e_again == e_ref
-    for p, e in errs.items():
-        assert e <= BOUND_COMM[p], (p, e)
+    assert errs["fp64"] <= 1e-12
+    for p in errs_gpu:
+        assert errs[p] <= BOUND_COMM[p] and errs_gpu[p] <= BOUND_COMM[p], (p, errs[p], errs_gpu[p])
     assert errs["bf16x3"] < errs["bf16x2"]
 
 
