@@ -263,6 +263,10 @@ def learned_form(nbody=20, nshell=14, n_omega=64, lo=0.1, hi=1.0, steps=20000, l
     out = []
     for k in held:
         H0 = ctx.operator(1, runs[k]["h0"])
+        # the judge: the cold flow (as-shipped map) carried to |MBPT2| <= 1e-12;
+        # at the shipped 1e-9 it stops up to ~1e-6 Ha short of this fixed point
+        HS, OM = ctx.operator(1), ctx.operator(-1)
+        rj, _ = qd.magnus_flow(ctx, H0, HS, OM, conv=1e-12, max_iter=1000)
         HS, OM = ctx.operator(1), ctx.operator(-1)
         t0 = time.perf_counter()
         rd, _ = qd.magnus_flow(ctx, H0, HS, OM, diis=6, max_iter=500)
@@ -275,9 +279,11 @@ def learned_form(nbody=20, nshell=14, n_omega=64, lo=0.1, hi=1.0, steps=20000, l
         e_w = rw.e0
         rec = dict(omega=float(omegas[k]), cold_steps=runs[k]["steps"], cold_s=runs[k]["s"],
                    cold_diis_steps=rd.iterations, cold_diis_s=t_cd, warm_steps=steps_w, warm_s=t_w,
-                   abs_dE_warm_vs_cold=abs(e_w - runs[k]["e0"]), fallback=fallback)
+                   abs_dE_warm_vs_converged_cold=abs(e_w - rj.e0), abs_dE_warm_vs_cold=abs(e_w - runs[k]["e0"]),
+                   abs_dE_cold_vs_converged_cold=abs(runs[k]["e0"] - rj.e0), fallback=fallback)
         progress(f"held-out omega {omegas[k]:.4f}: cold {rec['cold_steps']}, cold+DIIS {rec['cold_diis_steps']}, "
-                 f"warm {steps_w}, |dE| {rec['abs_dE_warm_vs_cold']:.2e}")
+                 f"warm {steps_w}, |dE| vs converged cold {rec['abs_dE_warm_vs_converged_cold']:.2e} "
+                 f"(vs cold as shipped {rec['abs_dE_warm_vs_cold']:.2e})")
         out.append(rec)
     ws.close()
     ctx.close()
@@ -290,7 +296,9 @@ def learned_form(nbody=20, nshell=14, n_omega=64, lo=0.1, hi=1.0, steps=20000, l
             "median_steps": {"cold": median("cold_steps"), "cold_diis": median("cold_diis_steps"),
                              "warm": median("warm_steps")},
             "mean_flow_s": {"cold": mean("cold_s"), "cold_diis": mean("cold_diis_s"), "warm": mean("warm_s")},
-            "max_abs_dE": max((r["abs_dE_warm_vs_cold"] for r in out), default=None),
+            "max_abs_dE_vs_converged_cold": max((r["abs_dE_warm_vs_converged_cold"] for r in out), default=None),
+            "max_abs_dE_vs_cold_as_shipped": max((r["abs_dE_warm_vs_cold"] for r in out), default=None),
+            "steps_saved_vs_cold_diis": (mean("cold_diis_steps") - mean("warm_steps")) if out else None,
             "fallbacks": sum(r["fallback"] for r in out)}
 
 
@@ -357,18 +365,18 @@ def main():
     ap.add_argument("--breakdown", action="store_true",
                     help="also time each reference entry point (110..222, calc_cc) on its own")
     ap.add_argument("--no-flow", action="store_true", help="skip the N=6 R=6 time-to-E_gs flows")
-    ap.add_argument("--batched", action="store_true",
-                    help="also run the N=20 R=16 batched-form flow against the per-channel path")
+    ap.add_argument("--no-batched", action="store_true",
+                    help="skip the N=20 R=16 batched-form flows (grouped vs per-channel path)")
     ap.add_argument("--no-autotune", action="store_true", help="keep the default 64x64 GEMM tiles")
     ap.add_argument("--calibrate", action="store_true",
                     help="after the timed steps, run add_arrays (k_axpby: reads 16, writes 8 bytes per "
                          "element of the operator) 3 times, the reference copy for HBM counter scaling")
     ap.add_argument("--no-precision-micro", action="store_true",
                     help="skip the per-precision repeat of the microbenchmark step")
-    ap.add_argument("--learned", action="store_true",
-                    help="also run the N=20 R=14 learned warm-start sweep (64 frequencies)")
-    ap.add_argument("--precision", action="store_true",
-                    help="also run the N=42 R=20 precision-form flows (FP64, FP32, BF16 slices)")
+    ap.add_argument("--no-learned", action="store_true",
+                    help="skip the N=20 R=14 learned warm-start sweep (64 frequencies)")
+    ap.add_argument("--no-precision", action="store_true",
+                    help="skip the N=42 R=20 precision-form flows (FP64, FP32, BF16 slices)")
     a = ap.parse_args()
     if a.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {a.gpus} is not supported "
@@ -478,11 +486,14 @@ def main():
     if not a.no_flow:
         flows = time_to_egs()
         out["time_to_egs"] = flows["runs"]
-    if a.batched:
-        out["batched_form"] = batched_flow()
-    if a.precision:
+    if not a.no_batched:
+        out["batched_form"] = bf = batched_flow()
+        out["time_to_egs_c3"] = {k: {"s": v["time_to_egs_s"], "abs_dE_vs_plain_fp64": v["abs_dE_vs_plain_fp64"],
+                                     "speedup_vs_plain_fp64": v["speedup_vs_plain_fp64"]}
+                                 for k, v in bf.items() if isinstance(v, dict)}
+    if not a.no_precision:
         out["precision_form"] = precision_form()
-    if a.learned:
+    if not a.no_learned:
         out["learned_form"] = learned_form()
     if not a.no_cpu:
         out["cpu_baseline"] = cpu_baseline(x, y, a.nbody, a.nshell, fl["total"], flows)
