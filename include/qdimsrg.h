@@ -381,6 +381,33 @@ int qd_hf_hamiltonian_me(int nbody, int nshell, double hw, const char *me_path, 
  * writer (its ME.bin is an external file). */
 int qd_me_bin_write(int nshell, const char *me_path, int64_t *nrec);
 
+/* ---- consumers of the decoupled H(s) --------------------------------------
+ * TDA spectrum: build_TDAmat + calc_TDA + diagonalize_blocks (src/operators.f90:
+ * 7-124, src/ME_general.f90:576-590) for the (M_L, M_S) = (ml, ms) transfer
+ * (m_s in units of 1/2, as the reference's states) with particles below the
+ * 1-based state cut (a = N+1 .. cut): the matrix d_ij f_ab - d_ab f_ji +
+ * <aj|V|ib> over (hole i, particle a) pairs, hole outer, gathered on the
+ * device; its eigenvalues (ascending, relative to E0) on the host from the
+ * upper triangle, as dsyev('V','U') reads it.  *dim receives the block
+ * dimension (0 for an empty block); eig needs that many entries. */
+int qd_tda_spectrum(qd_context *ctx, const qd_op *h, int ml, int ms, int cut, double *eig, int cap, int *dim);
+/* EOM-IM-SRG excited states of H(s) for the (ml, ms) transfer:
+ * calculate_excited_states / LANCZOS_DIAGONALIZE / matvec_prod
+ * (src/EOM_IMSRG.f90:10-47, 132-280, 551-660) with HQ_comm_1b / HQ_comm_2b
+ * (src/commutators.f90:1162-1260).  The ladder operator has ph components
+ * (particle outer) then pphh components (a < b, i < j, a outer), the order of
+ * unwrap/rewrap (src/EOM_IMSRG.f90:815-884, 1011-1080).
+ * qd_eom_dim: the counts of ph and pphh components.
+ * qd_eom_matvec: y = [H, Q(x)] on those components (host vectors of length n).
+ * qd_eom_excited_states: the nev smallest eigenvalues (excitation energies
+ * dE, ascending) by Lanczos with full reorthogonalisation on the device (in
+ * place of ARPACK's dsaupd 'SA'; relative residual 1e-10), the ph weight
+ * sum(fph**2) of each eigenvector (optional) and the Lanczos steps taken. */
+int qd_eom_dim(const qd_context *ctx, int ml, int ms, int64_t *n_ph, int64_t *n_pphh);
+int qd_eom_matvec(qd_context *ctx, const qd_op *h, int ml, int ms, const double *x, double *y, int64_t n);
+int qd_eom_excited_states(qd_context *ctx, const qd_op *h, int ml, int ms, int nev, double *eig, double *w1p1h,
+                          int *iters);
+
 /* ---- on-disk format of an operator --------------------------------------
  * export_hamiltonian / import_hamiltonian (src/ME_general.f90:207-361): the
  * file the reference's gs_IMSRG writes to ../hamiltonians/ (src/gs_IMSRG.f90:

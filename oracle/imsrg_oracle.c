@@ -700,3 +700,123 @@ int o_bch_expand(const tab_t *t, double *hs, const double *g, int hg, const doub
   }
   return k > 12 ? 11 : k - 1;
 }
+
+/* ---- EOM-IM-SRG matrix-vector product (excited states) --------------------
+ * matvec_prod (src/EOM_IMSRG.f90:551-660) with HQ_comm_1b / HQ_comm_2b
+ * (src/commutators.f90:1162-1260) restated loop for loop: the ladder operator
+ * Q of transfer (ml, ms) has ph part Qph (mp x n, column-major like fph) and
+ * pphh part Qpp (fermionic index of a < b, times that of i < j); x and y are
+ * the reference's wrapped vectors (unwrap / rewrap, src/EOM_IMSRG.f90:815-
+ * 884, 1011-1080): ph components with the particle outer, then pphh
+ * components (a < b, i < j), a outer.  Returns the vector length. */
+static i64 fidx0(i64 x, i64 y, i64 cnt) { return x * cnt - x * (x + 1) / 2 + (y - x - 1); }
+
+typedef struct {
+  i64 n, mp, np2;
+  const double *ph, *pp;
+} eomq_t;
+
+static double o_qf(const eomq_t *q, i64 a, i64 b) {
+  if (a < q->n || b >= q->n) return 0.0;
+  return q->ph[(a - q->n) + b * q->mp];
+}
+static double o_qe(const eomq_t *q, i64 a, i64 b, i64 c, i64 d) {
+  i64 n = q->n;
+  if (a == b || c == d || a < n || b < n || c >= n || d >= n) return 0.0;
+  double pre = 1.0;
+  if (a > b) { i64 x = a; a = b; b = x; pre = -pre; }
+  if (c > d) { i64 x = c; c = d; d = x; pre = -pre; }
+  return pre * q->pp[fidx0(a - n, b - n, q->mp) + fidx0(c, d, n) * q->np2];
+}
+
+static double hq_comm_1b(const tab_t *t, const double *h, int hh, const eomq_t *q, i64 a, i64 i) {
+  i64 N = t->n, M = t->M;
+  double sm = 0.0;
+  for (i64 b = N; b < M; b++) sm += o_felem(t, h, hh, a, b) * o_qf(q, b, i);
+  for (i64 j = 0; j < N; j++) sm -= o_qf(q, a, j) * o_felem(t, h, hh, j, i);
+  for (i64 b = N; b < M; b++)
+    for (i64 j = 0; j < N; j++) {
+      sm += o_felem(t, h, hh, j, b) * o_qe(q, a, b, i, j);
+      sm += o_qf(q, b, j) * o_velem(t, h, hh, a, j, i, b);
+    }
+  for (i64 b = N; b < M; b++)
+    for (i64 c = b + 1; c < M; c++)
+      for (i64 j = 0; j < N; j++) sm += o_qe(q, c, b, i, j) * o_velem(t, h, hh, a, j, c, b);
+  for (i64 b = N; b < M; b++)
+    for (i64 k = 0; k < N; k++)
+      for (i64 j = k + 1; j < N; j++) sm -= o_qe(q, a, b, k, j) * o_velem(t, h, hh, k, j, i, b);
+  return sm;
+}
+
+static double hq_comm_2b(const tab_t *t, const double *h, int hh, const eomq_t *q, i64 a, i64 b, i64 i, i64 j) {
+  i64 N = t->n, M = t->M;
+  double sm = 0.0;
+  for (i64 c = N; c < M; c++) {
+    sm += o_felem(t, h, hh, a, c) * o_qe(q, c, b, i, j) - o_felem(t, h, hh, b, c) * o_qe(q, c, a, i, j);
+    sm += o_qf(q, c, i) * o_velem(t, h, hh, a, b, c, j) - o_qf(q, c, j) * o_velem(t, h, hh, a, b, c, i);
+  }
+  for (i64 k = 0; k < N; k++) {
+    sm += -o_qe(q, a, b, k, j) * o_felem(t, h, hh, k, i) + o_qe(q, a, b, k, i) * o_felem(t, h, hh, k, j);
+    sm += -o_velem(t, h, hh, b, k, j, i) * o_qf(q, a, k) + o_velem(t, h, hh, a, k, j, i) * o_qf(q, b, k);
+  }
+  for (i64 c = N; c < M; c++)
+    for (i64 d = c + 1; d < M; d++) sm += o_velem(t, h, hh, a, b, c, d) * o_qe(q, c, d, i, j);
+  for (i64 k = 0; k < N; k++)
+    for (i64 l = k + 1; l < N; l++) sm += o_velem(t, h, hh, k, l, i, j) * o_qe(q, a, b, k, l);
+  for (i64 k = 0; k < N; k++)
+    for (i64 c = N; c < M; c++) {
+      sm -= o_velem(t, h, hh, b, k, c, i) * o_qe(q, a, c, k, j);
+      sm += o_velem(t, h, hh, a, k, c, i) * o_qe(q, b, c, k, j);
+      sm += o_velem(t, h, hh, b, k, c, j) * o_qe(q, a, c, k, i);
+      sm -= o_velem(t, h, hh, a, k, c, j) * o_qe(q, b, c, k, i);
+    }
+  return sm;
+}
+
+i64 o_eom_matvec(const tab_t *t, const double *h, int hh, int ml, int ms, const double *x, double *y) {
+  i64 N = t->n, M = t->M, mp = t->mp;
+  i64 np2 = mp * (mp - 1) / 2, nh2 = N * (N - 1) / 2;
+  double *ph = calloc((size_t)(mp * N + 1), sizeof(double));
+  double *pp = calloc((size_t)(np2 * (nh2 > 0 ? nh2 : 1) + 1), sizeof(double));
+  eomq_t q = {N, mp, np2, ph, pp};
+  i64 g = 0;
+  /* unwrap */
+  for (i64 a = N; a < M; a++)
+    for (i64 i = 0; i < N; i++)
+      if (t->ml[a] - t->ml[i] == ml && t->ms[a] - t->ms[i] == ms) {
+        if (x) ph[(a - N) + i * mp] = x[g];
+        g++;
+      }
+  i64 sps = g;
+  for (i64 a = N; a < M; a++)
+    for (i64 b = a + 1; b < M; b++)
+      for (i64 i = 0; i < N; i++)
+        for (i64 j = i + 1; j < N; j++)
+          if (t->ml[a] + t->ml[b] - t->ml[i] - t->ml[j] == ml && t->ms[a] + t->ms[b] - t->ms[i] - t->ms[j] == ms) {
+            if (x) pp[fidx0(a - N, b - N, mp) + fidx0(i, j, N) * np2] = x[g];
+            g++;
+          }
+  if (x && y) {
+    /* the products, then rewrap in the same order */
+    i64 e = 0;
+    for (i64 a = N; a < M; a++)
+      for (i64 i = 0; i < N; i++)
+        if (t->ml[a] - t->ml[i] == ml && t->ms[a] - t->ms[i] == ms) y[e++] = hq_comm_1b(t, h, hh, &q, a, i);
+    i64 *lst = malloc(sizeof(i64) * 4 * (size_t)(g - sps + 1));
+    i64 k = 0;
+    for (i64 a = N; a < M; a++)
+      for (i64 b = a + 1; b < M; b++)
+        for (i64 i = 0; i < N; i++)
+          for (i64 j = i + 1; j < N; j++)
+            if (t->ml[a] + t->ml[b] - t->ml[i] - t->ml[j] == ml && t->ms[a] + t->ms[b] - t->ms[i] - t->ms[j] == ms) {
+              lst[4 * k] = a; lst[4 * k + 1] = b; lst[4 * k + 2] = i; lst[4 * k + 3] = j;
+              k++;
+            }
+#pragma omp parallel for schedule(dynamic, 16)
+    for (i64 r = 0; r < k; r++) y[sps + r] = hq_comm_2b(t, h, hh, &q, lst[4 * r], lst[4 * r + 1], lst[4 * r + 2], lst[4 * r + 3]);
+    free(lst);
+  }
+  free(ph);
+  free(pp);
+  return g;
+}

@@ -58,6 +58,7 @@ def lib():
             "o_add": (None, [T, dp, C.c_double, dp, C.c_double, dp]),
             "o_commutator": (C.c_double, [T, dp, C.c_int, dp, C.c_int, dp, dp, C.c_double, dp, C.c_double]),
             "o_bch_expand": (C.c_int, [T, dp, dp, C.c_int, dp, dp, C.c_double, dp, C.c_double, dp, dp, dp]),
+            "o_eom_matvec": (C.c_int64, [T, dp, C.c_int, C.c_int, C.c_int, dp, dp]),
         }
         for name, (res, args) in sig.items():
             f = getattr(L, name)
@@ -158,6 +159,19 @@ class Oracle:
         self.L.o_commutator(self.t, _dp(v1), h1, _dp(v2), h2, _dp(out),
                             _dp(hcc), float(h2), _dp(ecc), float(h1))
         return out
+
+    def eom_dim(self, ml, ms):
+        """Length of the EOM vector of transfer (ml, ms): ph + pphh components."""
+        return int(self.L.o_eom_matvec(self.t, None, 1, int(ml), int(ms), None, None))
+
+    def eom_matvec(self, h, hh, ml, ms, x):
+        """matvec_prod (src/EOM_IMSRG.f90:551-660): [H, Q] on the ph and pphh
+        components of the ladder operator Q wrapped in x."""
+        x = np.ascontiguousarray(x, dtype=np.float64)
+        y = np.zeros_like(x)
+        n = self.L.o_eom_matvec(self.t, _dp(h), int(hh), int(ml), int(ms), _dp(x), _dp(y))
+        assert n == x.size, (n, x.size)
+        return y
 
     def bch_expand(self, g, hg, h, hcc=None, hh=1.0, occ=None, oh=-1.0):
         hcc = self.cc_zeros() if hcc is None else hcc

@@ -453,3 +453,35 @@ def import_hamiltonian_host(nbody: int, nshell: int, path: str) -> np.ndarray:
     v = np.zeros(_neq(nbody, nshell))
     check(L.qd_import_hamiltonian_host(int(nbody), int(nshell), _dp(v), v.size, str(path).encode()))
     return v
+
+
+# ---- consumers of H(s): TDA spectrum and EOM-IM-SRG ---------------------------
+def tda_spectrum(h: Operator, ml: int, ms: int, cut: int) -> np.ndarray:
+    """Eigenvalues of the TDA block (src/operators.f90:7-124), relative to E0."""
+    L, ctx = h.ctx.L, h.ctx
+    cap = max(ctx.nbody * ctx.msp, 1)
+    eig, d = np.zeros(cap), C.c_int()
+    check(L.qd_tda_spectrum(ctx.h, h.h, int(ml), int(ms), int(cut), _dp(eig), cap, C.byref(d)))
+    return eig[:d.value].copy()
+
+
+def eom_dim(ctx: Context, ml: int, ms: int):
+    a, b = C.c_int64(), C.c_int64()
+    check(ctx.L.qd_eom_dim(ctx.h, int(ml), int(ms), C.byref(a), C.byref(b)))
+    return a.value, b.value
+
+
+def eom_matvec(h: Operator, ml: int, ms: int, x: np.ndarray) -> np.ndarray:
+    x = np.ascontiguousarray(x, dtype=np.float64)
+    y = np.zeros_like(x)
+    check(h.ctx.L.qd_eom_matvec(h.ctx.h, h.h, int(ml), int(ms), _dp(x), _dp(y), x.size))
+    return y
+
+
+def calculate_excited_states(h: Operator, ml: int, ms: int, nev: int):
+    """EOM-IM-SRG excitation energies dE (ascending), the ph weight of each
+    state and the Lanczos steps (calculate_excited_states,
+    src/EOM_IMSRG.f90:10-47)."""
+    eig, w, it = np.zeros(nev), np.zeros(nev), C.c_int()
+    check(h.ctx.L.qd_eom_excited_states(h.ctx.h, h.h, int(ml), int(ms), int(nev), _dp(eig), _dp(w), C.byref(it)))
+    return eig, w, it.value

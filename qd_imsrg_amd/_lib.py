@@ -120,6 +120,10 @@ SIGNATURES = {
     "qd_hf_hamiltonian": (C.c_int, [C.c_int, C.c_int, C.c_double, dp, C.c_int64, dp, ip]),
     "qd_hf_hamiltonian_me": (C.c_int, [C.c_int, C.c_int, C.c_double, C.c_char_p, dp, C.c_int64, dp, ip]),
     "qd_me_bin_write": (C.c_int, [C.c_int, C.c_char_p, i64p]),
+    "qd_tda_spectrum": (C.c_int, [vp, vp, C.c_int, C.c_int, C.c_int, dp, C.c_int, ip]),
+    "qd_eom_dim": (C.c_int, [vp, C.c_int, C.c_int, i64p, i64p]),
+    "qd_eom_matvec": (C.c_int, [vp, vp, C.c_int, C.c_int, dp, dp, C.c_int64]),
+    "qd_eom_excited_states": (C.c_int, [vp, vp, C.c_int, C.c_int, C.c_int, dp, dp, ip]),
     "qd_export_hamiltonian": (C.c_int, [vp, C.c_char_p]),
     "qd_import_hamiltonian": (C.c_int, [vp, C.c_char_p]),
     "qd_export_hamiltonian_host": (C.c_int, [C.c_int, C.c_int, dp, C.c_int64, C.c_char_p]),

@@ -185,6 +185,9 @@ int64_t qd::context_neq(const qd_context *c) { return c->ch.neq; }
 qd_context *qd::op_context(qd_op *o) { return o->ctx; }
 double *qd::op_data(qd_op *o) { return o->d; }
 const Channels &qd::context_channels(const qd_context *c) { return c->ch; }
+const DevTab &qd::context_devtab(const qd_context *c) { return c->dt; }
+const double *qd::op_cdata(const qd_op *o) { return o->d; }
+int qd::op_herm(const qd_op *o) { return o->herm; }
 
 namespace {
 

@@ -84,6 +84,10 @@ int64_t context_neq(const qd_context *c);
 qd_context *op_context(qd_op *o);
 double *op_data(qd_op *o);
 const Channels &context_channels(const qd_context *c);
+struct DevTab;
+const DevTab &context_devtab(const qd_context *c);
+const double *op_cdata(const qd_op *o);
+int op_herm(const qd_op *o);
 // cyclic Jacobi eigensolver for a small symmetric matrix (row-major a[n*n]);
 // eigenvalues ascending in w, eigenvectors in the columns of v (hf.cpp)
 void jacobi(int n, std::vector<double> a, std::vector<double> &w, std::vector<double> &v);

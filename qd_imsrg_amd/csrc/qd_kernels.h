@@ -51,6 +51,38 @@ struct Ops {  // two operators and their hermiticity
 };
 
 // ---- device helpers shared by the kernel sources
+// v_elem (src/ME_general.f90:1557-1680): pair sorting with sign, the i_array
+// lookup, zero across blocks, and the herm factor for the three sub-blocks
+// that are stored transposed
+__device__ __forceinline__ int64_t pidx(int i, int j, int M) {
+  return (int64_t)i * M - (int64_t)i * (i + 1) / 2 + (j - i - 1);
+}
+
+__device__ __forceinline__ double velem(const DevTab &t, const double *__restrict__ v, int herm, int a1,
+                                        int a2, int a3, int a4) {
+  if (a1 == a2 || a3 == a4) return 0.0;
+  double pre = 1.0;
+  if (a1 > a2) { int x = a1; a1 = a2; a2 = x; pre = -pre; }
+  if (a3 > a4) { int x = a3; a3 = a4; a4 = x; pre = -pre; }
+  const uint32_t I = t.iarr[pidx(a1, a2, t.M)], J = t.iarr[pidx(a3, a4, t.M)];
+  const int q = ia_blk(I);
+  if (q != ia_blk(J)) return 0.0;
+  const int w1 = ia_idx(I), w2 = ia_idx(J);
+  const int64_t *o = t.off + 6 * q;
+  const int nh = t.nh[q], np = t.np[q], nb = t.nb[q];
+  switch (ia_type(I) * 4 + ia_type(J)) {
+    case T_HH * 4 + T_HH: return pre * v[o[HHHH] + w1 + (int64_t)w2 * nh];
+    case T_HH * 4 + T_PH: return pre * herm * v[o[PHHH] + w2 + (int64_t)w1 * nb];
+    case T_HH * 4 + T_PP: return pre * herm * v[o[PPHH] + w2 + (int64_t)w1 * np];
+    case T_PH * 4 + T_HH: return pre * v[o[PHHH] + w1 + (int64_t)w2 * nb];
+    case T_PH * 4 + T_PH: return pre * v[o[PHPH] + w1 + (int64_t)w2 * nb];
+    case T_PH * 4 + T_PP: return pre * herm * v[o[PPPH] + w2 + (int64_t)w1 * np];
+    case T_PP * 4 + T_HH: return pre * v[o[PPHH] + w1 + (int64_t)w2 * np];
+    case T_PP * 4 + T_PH: return pre * v[o[PPPH] + w1 + (int64_t)w2 * np];
+    default: return pre * v[o[PPPP] + w1 + (int64_t)w2 * np];
+  }
+}
+
 // f_elem (src/ME_general.f90:1780-1813)
 __device__ __forceinline__ double felem(const DevTab &t, const double *__restrict__ v, int herm, int a, int b) {
   const int n = t.n;
