@@ -217,6 +217,14 @@ typedef struct {
                        energy derivative of the IM-SRG flow equation.  No
                        counterpart in the reference.                           */
   double conv_deds;  /* threshold of stop_rule 1 (default 1e-8 Ha)                  */
+  int tda_ml, tda_ms, tda_cut;  /* TDA block of the per-step spectrum (qd_tda_spectrum;
+                                   tda_cut 0: an empty block)                      */
+  const char *spectrum_path;    /* non-null: the reference's spectrum_magnus.dat
+                       (src/magnus_IMSRG.f90:130-136, 165-189, 232-243): the file is
+                       rewritten from the start; one line for H(0), then two per
+                       accepted step (write_spec is called twice there), each
+                       n(M-n)+2 fields '(f12.7)': s, E0, then E0 + the TDA
+                       eigenvalues, the rest 0 (left undefined by the reference) */
 } qd_flow_params;
 typedef struct {
   double s, e0, crit;          /* final flow parameter, E0(s), |MBPT2(H(s))|          */

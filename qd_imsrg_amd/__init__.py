@@ -321,18 +321,23 @@ def dGam(H, DER, HCC, ETACC):
 def magnus_flow(ctx: Context, H0: Operator, HS: Operator, OMEGA: Operator, pandya: bool = False,
                 warm_start: bool = False, verbose: int = 0, max_iter: int = 100000,
                 conv: float | None = None, trace_len: int = 4096, diis: int = 0,
-                stop_rule: str = "mbpt2", conv_deds: float = 1e-8):
+                stop_rule: str = "mbpt2", conv_deds: float = 1e-8, tda=(0, 0, 0), spectrum_path: str | None = None):
     """s_evolve (src/magnus_IMSRG.f90:194-248) on the GPU; returns (result, trace).
     verbose: 1 prints the reference's per-iteration line to stdout, 2 to stderr.
     diis: depth of the Pulay acceleration (0 = the reference's iteration).
     stop_rule: "mbpt2" (the reference: |MBPT2(H(s))| <= conv) or "deds"
-    (|dE/ds| <= conv_deds, dE/ds = zero-body part of [eta(s), H(s)])."""
+    (|dE/ds| <= conv_deds, dE/ds = zero-body part of [eta(s), H(s)]).
+    spectrum_path: write the reference's spectrum_magnus.dat there, with the
+    TDA block tda = (M_L, M_S, cut)."""
     p = FlowParams()
     ctx.L.qd_flow_params_default(C.byref(p))
     p.pandya, p.warm_start, p.verbose, p.max_iter = int(pandya), int(warm_start), int(verbose), int(max_iter)
     p.diis = int(diis)
     p.stop_rule = {"mbpt2": 0, "deds": 1}[stop_rule]
     p.conv_deds = float(conv_deds)
+    p.tda_ml, p.tda_ms, p.tda_cut = (int(x) for x in tda)
+    path = None if spectrum_path is None else str(spectrum_path).encode()
+    p.spectrum_path = path
     if conv is not None:
         p.conv = float(conv)
     res = FlowResult()

@@ -25,7 +25,8 @@ class Info(C.Structure):
 class FlowParams(C.Structure):
     _fields_ = [("stp", C.c_double), ("conv", C.c_double), ("max_iter", C.c_int), ("pandya", C.c_int),
                 ("verbose", C.c_int), ("warm_start", C.c_int), ("diis", C.c_int), ("stop_rule", C.c_int),
-                ("conv_deds", C.c_double)]
+                ("conv_deds", C.c_double), ("tda_ml", C.c_int), ("tda_ms", C.c_int), ("tda_cut", C.c_int),
+                ("spectrum_path", C.c_char_p)]
 
 
 class FlowResult(C.Structure):

@@ -1668,6 +1668,8 @@ void qd_flow_params_default(qd_flow_params *p) {
   p->diis = 0;
   p->stop_rule = 0;
   p->conv_deds = 1e-8;
+  p->tda_ml = p->tda_ms = p->tda_cut = 0;
+  p->spectrum_path = nullptr;
 }
 
 int qd_magnus_flow(qd_context *c, const qd_op *h0, qd_op *hs, qd_op *omega, const qd_flow_params *p,
@@ -1728,6 +1730,35 @@ int qd_magnus_flow(qd_context *c, const qd_op *h0, qd_op *hs, qd_op *omega, cons
   };
   if (p->stop_rule != 0 && p->stop_rule != 1) return done(fail(QD_E_ARG, "stop_rule must be 0 or 1"));
   if ((rc = eval_crit(hs))) return done(rc);
+  // write_spec (src/magnus_IMSRG.f90:165-189) into spectrum_path
+  FILE *spec = nullptr;
+  std::vector<double> tda_eig;
+  const int nlev = c->ch.n * c->ch.mp + 2;
+  if (p->spectrum_path) {
+    spec = fopen(p->spectrum_path, "w");
+    if (!spec) return done(fail(QD_E_ARG, "cannot create %s", p->spectrum_path));
+    tda_eig.resize((size_t)std::max(nlev, 1));
+  }
+  auto write_spec = [&](double e0now) -> int {
+    if (!spec) return QD_OK;
+    int d = 0, r;
+    if ((r = qd_tda_spectrum(c, hs, p->tda_ml, p->tda_ms, p->tda_cut, tda_eig.data(), (int)tda_eig.size(), &d)))
+      return r;
+    fprintf(spec, "%12.7f%12.7f", s, e0now);
+    for (int k = 2; k < nlev; k++) fprintf(spec, "%12.7f", k - 2 < d ? tda_eig[k - 2] + e0now : 0.0);
+    fprintf(spec, "\n");
+    return QD_OK;
+  };
+  auto done_spec = [&](int r) {
+    if (spec) fclose(spec);
+    spec = nullptr;
+    return r;
+  };
+  {
+    HIPC(hipMemcpyAsync(c->host_scal, hs->d, sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    HIPC(hipStreamSynchronize(c->stream));
+    if ((rc = write_spec(c->host_scal[0]))) return done(done_spec(rc));
+  }
   int steps = 0;
   // Pulay (DIIS) acceleration (no counterpart in the reference): once the
   // flow is close to decoupling (crit <= DIIS_START), keep the last m
@@ -1745,8 +1776,11 @@ int qd_magnus_flow(qd_context *c, const qd_op *h0, qd_op *hs, qd_op *omega, cons
     for (auto *o : HR) qd_op_destroy(o);
   };
   for (int i = 0; i < m; i++)
-    if ((rc = qd_op_create(c, -1, &HO[i])) || (rc = qd_op_create(c, -1, &HR[i]))) { free_hist(); return done(rc); }
-  auto done2 = [&](int r) { free_hist(); return done(r); };
+    if ((rc = qd_op_create(c, -1, &HO[i])) || (rc = qd_op_create(c, -1, &HR[i]))) {
+      free_hist();
+      return done(done_spec(rc));
+    }
+  auto done2 = [&](int r) { free_hist(); return done(done_spec(r)); };
   std::vector<double> B((size_t)std::max(m, 1) * std::max(m, 1), 0.0);
   int count = 0;
   double best = 0.0;
@@ -1804,6 +1838,7 @@ int qd_magnus_flow(qd_context *c, const qd_op *h0, qd_op *hs, qd_op *omega, cons
       log_step();
       if (count == 1 || nrm2 < best) best = nrm2;
       if (nrm2 > 2.0 * best) count = 0;  // drop the history
+      if ((rc = write_spec(e0)) || (rc = write_spec(e0))) return done2(rc);
       if ((rc = eval_crit(hs))) return done2(rc);
       iteration++;
       continue;
@@ -1830,10 +1865,12 @@ int qd_magnus_flow(qd_context *c, const qd_op *h0, qd_op *hs, qd_op *omega, cons
       rejected++;
       continue;
     }
+    if ((rc = write_spec(e0)) || (rc = write_spec(e0))) return done2(rc);
     if ((rc = eval_crit(hs))) return done2(rc);
     iteration++;
   }
   free_hist();
+  done_spec(QD_OK);
   HIPC(hipMemcpyAsync(c->host_scal, hs->d, sizeof(double), hipMemcpyDeviceToHost, c->stream));
   HIPC(hipStreamSynchronize(c->stream));
   res->s = s;

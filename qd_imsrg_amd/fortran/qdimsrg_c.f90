@@ -18,6 +18,8 @@ module qdimsrg_c
      real(c_double) :: stp, conv
      integer(c_int) :: max_iter, pandya, verbose, warm_start, diis, stop_rule
      real(c_double) :: conv_deds
+     integer(c_int) :: tda_ml, tda_ms, tda_cut
+     type(c_ptr) :: spectrum_path
   end type qd_flow_params
 
   type, bind(C) :: qd_gs_params
@@ -279,6 +281,23 @@ module qdimsrg_c
        integer(c_int64_t), value :: neq
        integer(c_int) :: iters
        integer(c_int) :: qd_hf_hamiltonian
+     end function
+     ! ---- consumers of H(s) (src/operators.f90:7-124, src/EOM_IMSRG.f90:10-47) ----
+     function qd_eom_excited_states(ctx, h, ml, ms, nev, eig, w1p1h, iters) bind(C, name="qd_eom_excited_states")
+       import :: c_int, c_ptr, c_double
+       type(c_ptr), value :: ctx, h
+       integer(c_int), value :: ml, ms, nev
+       real(c_double) :: eig(*), w1p1h(*)
+       integer(c_int) :: iters
+       integer(c_int) :: qd_eom_excited_states
+     end function
+     function qd_tda_spectrum(ctx, h, ml, ms, cut, eig, cap, dim) bind(C, name="qd_tda_spectrum")
+       import :: c_int, c_ptr, c_double
+       type(c_ptr), value :: ctx, h
+       integer(c_int), value :: ml, ms, cut, cap
+       real(c_double) :: eig(*)
+       integer(c_int) :: dim
+       integer(c_int) :: qd_tda_spectrum
      end function
      ! ---- on-disk format (src/ME_general.f90:207-361) ----
      function qd_export_hamiltonian(op, path) bind(C, name="qd_export_hamiltonian")

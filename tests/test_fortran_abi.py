@@ -17,7 +17,7 @@ OBJ = os.path.join(ROOT, "qd_imsrg_amd", "_obj")
 TYPES = {
     "qd_info": ["nbody", "nshell", "msp", "nblock", "ncc", "neq", "cc_size", "n2b"],
     "qd_flow_params": ["stp", "conv", "max_iter", "pandya", "verbose", "warm_start", "diis", "stop_rule",
-                       "conv_deds"],
+                       "conv_deds", "tda_ml", "tda_ms", "tda_cut", "spectrum_path"],
     "qd_flow_result": ["s", "e0", "crit", "iterations", "rejected", "bch_terms", "converged", "trace_rows",
                        "mbpt2", "deds"],
     "qd_gs_params": ["relerr", "abserr", "stp", "conv", "max_outputs", "verbose", "maxnum"],

@@ -33,11 +33,14 @@ def fixed(line, widths):
 
 def test_run_magnus_lines_match_library(tmp_path):
     """'(I5,5(f18.9))' per step: iteration, s, E0, ||eta_0||, ||eta||, crit;
-    no summary line after the flow (the reference stops at
-    src/magnus_IMSRG.f90:144)."""
-    lines = run("run_magnus", ["6", "1.0d0", "6"], str(tmp_path))
-    steps = [l for l in lines if len(l) == 95]
-    assert steps and all(len(l) == 95 or l.strip() == "faillll" for l in lines), lines[:3]
+    spectrum_magnus.dat with the TDA block of (ML, MS) = (1, 0) below state
+    20 (one line at the start, two per accepted step); then the EOM block of
+    calculate_excited_states (10 states) and nothing after it (the reference
+    stops at src/magnus_IMSRG.f90:144)."""
+    lines = run("run_magnus", ["6", "1.0d0", "6", "1", "0", "20"], str(tmp_path))
+    k = next(i for i, l in enumerate(lines) if not l.strip())
+    steps = [l for l in lines[:k] if len(l) == 95]
+    assert steps and all(len(l) == 95 or l.strip() == "faillll" for l in lines[:k]), lines[:3]
     rows = np.array([[float(x) for x in fixed(l, [5] + [18] * 5)] for l in steps])
     h0, _, _ = qd.hf_hamiltonian(6, 6, 1.0)
     ctx = qd.Context(6, 6)
@@ -46,6 +49,26 @@ def test_run_magnus_lines_match_library(tmp_path):
     assert rows.shape[0] == trace.shape[0] == res.iterations + res.rejected
     assert np.abs(rows - trace).max() <= 1e-9 + 1e-9 * np.abs(trace).max()
     assert abs(rows[-1, 2] - res.e0) <= 1e-9
+    # spectrum_magnus.dat
+    spec = (tmp_path / "spectrum_magnus.dat").read_text().splitlines()
+    nlev = 6 * 36 + 2
+    assert len(spec) == 1 + 2 * res.iterations and all(len(l) == 12 * nlev for l in spec)
+    last = np.array([float(x) for x in fixed(spec[-1], [12] * nlev)])
+    tda = qd.tda_spectrum(HS, 1, 0, 20)
+    assert 0 < tda.size < nlev - 2
+    assert abs(last[1] - res.e0) <= 1e-7 and np.abs(last[2:2 + tda.size] - (tda + res.e0)).max() <= 1e-6
+    assert np.all(last[2 + tda.size:] == 0.0)
+    first = np.array([float(x) for x in fixed(spec[0], [12] * nlev)])
+    assert first[0] == 0.0 and abs(first[1] - h0[0]) <= 1e-7
+    # the EOM block
+    tail = lines[k:]
+    assert any("EXECUTING EOM CALCULATION FOR EXCITED STATES: Ml= 1 Ms= 0" in l for l in tail)
+    assert any(l.startswith("Ground State Energy: ") and abs(float(l[21:]) - res.e0) <= 1e-8 for l in tail)
+    eom = [l for l in tail if len(l) == 48]
+    assert len(eom) == 10
+    de, _, _ = qd.calculate_excited_states(HS, 1, 0, 10)
+    got = np.array([[float(x) for x in fixed(l, [16] * 3)] for l in eom])
+    assert np.abs(got[:, 0] - de).max() <= 1e-8 and np.abs(got[:, 1] - (de + res.e0)).max() <= 1e-8
     ctx.close()
 
 
