@@ -415,6 +415,22 @@ int qd_eom_dim(const qd_context *ctx, int ml, int ms, int64_t *n_ph, int64_t *n_
 int qd_eom_matvec(qd_context *ctx, const qd_op *h, int ml, int ms, const double *x, double *y, int64_t n);
 int qd_eom_excited_states(qd_context *ctx, const qd_op *h, int ml, int ms, int nev, double *eig, double *w1p1h,
                           int *iters);
+/* One particle attached (mode 1) or removed (mode 2): calculate_1p_attached /
+ * calculate_1h_removed with LANCZOS_1p_ATTACHED / LANCZOS_1h_REMOVED and
+ * matvec_1p_attached / matvec_1h_removed (src/EOM_IMSRG.f90:50-128, 283-548,
+ * 662-812; HQ_comm_1p/2p1h/1h/2h1p, src/commutators.f90:1263-1425).  The
+ * ladder operator has one-index components (mode 1: particles a with
+ * (m_l, m_s) = (ml, ms); mode 2: holes i with -(m_l, m_s) = (ml, ms)), then
+ * three-index ones (mode 1: a < b particles, hole i; mode 2: i < j holes,
+ * particle a), the order of unwrap_1p / unwrap_1h (src/EOM_IMSRG.f90:886-1009).
+ * qd_eom13_dim: the two counts; qd_eom13_matvec: y = A x (host vectors);
+ * qd_eom_attached_removed: the nev smallest eigenvalues dE, the weight of the
+ * one-index components of each eigenvector (norm_1p, src/EOM_IMSRG.f90:409-
+ * 411) and the Lanczos steps. */
+int qd_eom13_dim(const qd_context *ctx, int mode, int ml, int ms, int64_t *n1, int64_t *n3);
+int qd_eom13_matvec(qd_context *ctx, const qd_op *h, int mode, int ml, int ms, const double *x, double *y, int64_t n);
+int qd_eom_attached_removed(qd_context *ctx, const qd_op *h, int mode, int ml, int ms, int nev, double *eig,
+                            double *w1, int *iters);
 
 /* ---- on-disk format of an operator --------------------------------------
  * export_hamiltonian / import_hamiltonian (src/ME_general.f90:207-361): the

@@ -820,3 +820,138 @@ i64 o_eom_matvec(const tab_t *t, const double *h, int hh, int ml, int ms, const 
   free(pp);
   return g;
 }
+
+/* ---- EOM-IM-SRG for one particle attached (mode 1) or removed (mode 2) -----
+ * matvec_1p_attached / matvec_1h_removed (src/EOM_IMSRG.f90:662-812) with
+ * HQ_comm_1p, HQ_comm_2p1h, HQ_comm_1h, HQ_comm_2h1p (src/commutators.f90:
+ * 1263-1425) on the reference's vec layout (get_X1 / get_X3, src/ME_general.
+ * f90:1229-1280): mode 1: X1 of particle a at a - n, X3(a<b particles, hole
+ * i) at mp + fidx(a-n, b-n, mp) n + i; mode 2: X1 of hole i at i, X3(a<b
+ * holes, particle i) at fidx(a, b, n) mp + i.  x and y are the wrapped target
+ * components (unwrap_1p / unwrap_1h order).  Returns their count. */
+typedef struct { i64 n, mp, mode; double *v; } vec13_t;
+
+static double gx1(const vec13_t *q, i64 a) { return a >= q->n ? q->v[a - q->n] : q->v[a]; }
+static i64 x3pos(const vec13_t *q, i64 ax, i64 bx, i64 i, double *ph) {
+  i64 a = ax, b = bx;
+  *ph = 1.0;
+  if (ax > bx) { a = bx; b = ax; *ph = -1.0; }
+  if (i >= q->n) return fidx0(a, b, q->n) * q->mp + i;            /* third index a particle */
+  return q->mp + fidx0(a - q->n, b - q->n, q->mp) * q->n + i;     /* third index a hole */
+}
+static double gx3(const vec13_t *q, i64 ax, i64 bx, i64 i) {
+  if (ax == bx) return 0.0;
+  double ph;
+  i64 k = x3pos(q, ax, bx, i, &ph);
+  return q->v[k] * ph;
+}
+
+static double hq_1p(const tab_t *t, const double *h, int hh, const vec13_t *q, i64 a) {
+  i64 N = t->n, M = t->M;
+  double sm = 0.0;
+  for (i64 b = N; b < M; b++) sm += o_felem(t, h, hh, a, b) * gx1(q, b);
+  for (i64 b = N; b < M; b++)
+    for (i64 i = 0; i < N; i++) sm += o_felem(t, h, hh, i, b) * gx3(q, a, b, i);
+  for (i64 b = N; b < M; b++)
+    for (i64 c = b + 1; c < M; c++)
+      for (i64 i = 0; i < N; i++) sm += o_velem(t, h, hh, a, i, b, c) * gx3(q, b, c, i);
+  return sm;
+}
+static double hq_1h(const tab_t *t, const double *h, int hh, const vec13_t *q, i64 i) {
+  i64 N = t->n, M = t->M;
+  double sm = 0.0;
+  for (i64 j = 0; j < N; j++) sm -= o_felem(t, h, hh, j, i) * gx1(q, j);
+  for (i64 b = N; b < M; b++)
+    for (i64 j = 0; j < N; j++) sm += o_felem(t, h, hh, j, b) * gx3(q, i, j, b);
+  for (i64 j = 0; j < N; j++)
+    for (i64 k = j + 1; k < N; k++)
+      for (i64 b = N; b < M; b++) sm += o_velem(t, h, hh, j, k, b, i) * gx3(q, j, k, b);
+  return sm;
+}
+static double hq_2p1h(const tab_t *t, const double *h, int hh, const vec13_t *q, i64 a, i64 b, i64 i) {
+  i64 N = t->n, M = t->M;
+  double sm = 0.0;
+  for (i64 c = N; c < M; c++)
+    sm += o_velem(t, h, hh, a, b, c, i) * gx1(q, c) + o_felem(t, h, hh, a, c) * gx3(q, c, b, i) -
+          o_felem(t, h, hh, b, c) * gx3(q, c, a, i);
+  for (i64 j = 0; j < N; j++) sm -= o_felem(t, h, hh, j, i) * gx3(q, a, b, j);
+  for (i64 c = N; c < M; c++)
+    for (i64 d = c + 1; d < M; d++) sm += o_velem(t, h, hh, a, b, c, d) * gx3(q, c, d, i);
+  for (i64 c = N; c < M; c++)
+    for (i64 j = 0; j < N; j++)
+      sm = sm - o_velem(t, h, hh, a, j, c, i) * gx3(q, c, b, j) + o_velem(t, h, hh, b, j, c, i) * gx3(q, c, a, j);
+  return sm;
+}
+static double hq_2h1p(const tab_t *t, const double *h, int hh, const vec13_t *q, i64 i, i64 j, i64 a) {
+  i64 N = t->n, M = t->M;
+  double sm = 0.0;
+  for (i64 k = 0; k < N; k++)
+    sm += o_velem(t, h, hh, a, k, i, j) * gx1(q, k) - o_felem(t, h, hh, k, j) * gx3(q, i, k, a) +
+          o_felem(t, h, hh, k, i) * gx3(q, j, k, a);
+  for (i64 b = N; b < M; b++) sm += o_felem(t, h, hh, a, b) * gx3(q, i, j, b);
+  for (i64 k = 0; k < N; k++)
+    for (i64 l = k + 1; l < N; l++) sm += o_velem(t, h, hh, k, l, i, j) * gx3(q, k, l, a);
+  for (i64 b = N; b < M; b++)
+    for (i64 k = 0; k < N; k++)
+      sm = sm - o_velem(t, h, hh, a, k, b, i) * gx3(q, k, j, b) + o_velem(t, h, hh, a, k, b, j) * gx3(q, k, i, b);
+  return sm;
+}
+
+i64 o_eom13_matvec(const tab_t *t, const double *h, int hh, int mode, int ml, int ms, const double *x, double *y) {
+  i64 N = t->n, M = t->M, mp = t->mp;
+  i64 dm = mode == 1 ? mp + N * mp * (mp - 1) / 2 : N + mp * N * (N - 1) / 2;
+  double *vin = calloc((size_t)dm + 1, sizeof(double)), *vout = calloc((size_t)dm + 1, sizeof(double));
+  vec13_t qi = {N, mp, mode, vin};
+  /* unwrap: one-index components, then (x < y, z) */
+  i64 g = 0;
+  i64 (*lst)[3] = malloc(sizeof(i64[3]) * (size_t)(dm + 1));
+  i64 nl = 0;
+  if (mode == 1) {
+    for (i64 a = N; a < M; a++)
+      if (t->ml[a] == ml && t->ms[a] == ms) { lst[nl][0] = a; lst[nl][1] = -1; nl++; }
+  } else {
+    for (i64 i = 0; i < N; i++)
+      if (-t->ml[i] == ml && -t->ms[i] == ms) { lst[nl][0] = i; lst[nl][1] = -1; nl++; }
+  }
+  i64 n1 = nl;
+  if (mode == 1) {
+    for (i64 a = N; a < M; a++)
+      for (i64 b = a + 1; b < M; b++)
+        for (i64 i = 0; i < N; i++)
+          if (t->ml[a] + t->ml[b] - t->ml[i] == ml && t->ms[a] + t->ms[b] - t->ms[i] == ms) {
+            lst[nl][0] = a; lst[nl][1] = b; lst[nl][2] = i; nl++;
+          }
+  } else {
+    for (i64 a = 0; a < N; a++)
+      for (i64 b = a + 1; b < N; b++)
+        for (i64 i = N; i < M; i++)
+          if (-t->ml[a] - t->ml[b] + t->ml[i] == ml && -t->ms[a] - t->ms[b] + t->ms[i] == ms) {
+            lst[nl][0] = a; lst[nl][1] = b; lst[nl][2] = i; nl++;
+          }
+  }
+  g = nl;
+  if (x && y) {
+    for (i64 k = 0; k < nl; k++) {
+      if (k < n1) {
+        i64 a = lst[k][0];
+        vin[a >= N ? a - N : a] = x[k];
+      } else {
+        double ph;
+        i64 pos = x3pos(&qi, lst[k][0], lst[k][1], lst[k][2], &ph);
+        vin[pos] = x[k] * ph;
+      }
+    }
+#pragma omp parallel for schedule(dynamic, 4)
+    for (i64 k = 0; k < nl; k++) {
+      if (k < n1)
+        y[k] = mode == 1 ? hq_1p(t, h, hh, &qi, lst[k][0]) : hq_1h(t, h, hh, &qi, lst[k][0]);
+      else
+        y[k] = mode == 1 ? hq_2p1h(t, h, hh, &qi, lst[k][0], lst[k][1], lst[k][2])
+                         : hq_2h1p(t, h, hh, &qi, lst[k][0], lst[k][1], lst[k][2]);
+    }
+  }
+  free(lst);
+  free(vin);
+  free(vout);
+  return g;
+}

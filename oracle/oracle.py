@@ -59,6 +59,7 @@ def lib():
             "o_commutator": (C.c_double, [T, dp, C.c_int, dp, C.c_int, dp, dp, C.c_double, dp, C.c_double]),
             "o_bch_expand": (C.c_int, [T, dp, dp, C.c_int, dp, dp, C.c_double, dp, C.c_double, dp, dp, dp]),
             "o_eom_matvec": (C.c_int64, [T, dp, C.c_int, C.c_int, C.c_int, dp, dp]),
+            "o_eom13_matvec": (C.c_int64, [T, dp, C.c_int, C.c_int, C.c_int, C.c_int, dp, dp]),
         }
         for name, (res, args) in sig.items():
             f = getattr(L, name)
@@ -170,6 +171,20 @@ class Oracle:
         x = np.ascontiguousarray(x, dtype=np.float64)
         y = np.zeros_like(x)
         n = self.L.o_eom_matvec(self.t, _dp(h), int(hh), int(ml), int(ms), _dp(x), _dp(y))
+        assert n == x.size, (n, x.size)
+        return y
+
+    def eom13_dim(self, mode, ml, ms):
+        """Length of the particle-attached (mode 1) or hole-removed (mode 2)
+        EOM vector of (ml, ms)."""
+        return int(self.L.o_eom13_matvec(self.t, None, 1, int(mode), int(ml), int(ms), None, None))
+
+    def eom13_matvec(self, h, hh, mode, ml, ms, x):
+        """matvec_1p_attached (mode 1) / matvec_1h_removed (mode 2),
+        src/EOM_IMSRG.f90:662-812."""
+        x = np.ascontiguousarray(x, dtype=np.float64)
+        y = np.zeros_like(x)
+        n = self.L.o_eom13_matvec(self.t, _dp(h), int(hh), int(mode), int(ml), int(ms), _dp(x), _dp(y))
         assert n == x.size, (n, x.size)
         return y
 

@@ -490,3 +490,34 @@ def calculate_excited_states(h: Operator, ml: int, ms: int, nev: int):
     eig, w, it = np.zeros(nev), np.zeros(nev), C.c_int()
     check(h.ctx.L.qd_eom_excited_states(h.ctx.h, h.h, int(ml), int(ms), int(nev), _dp(eig), _dp(w), C.byref(it)))
     return eig, w, it.value
+
+
+def eom13_dim(ctx: Context, mode: int, ml: int, ms: int):
+    a, b = C.c_int64(), C.c_int64()
+    check(ctx.L.qd_eom13_dim(ctx.h, int(mode), int(ml), int(ms), C.byref(a), C.byref(b)))
+    return a.value, b.value
+
+
+def eom13_matvec(h: Operator, mode: int, ml: int, ms: int, x: np.ndarray) -> np.ndarray:
+    x = np.ascontiguousarray(x, dtype=np.float64)
+    y = np.zeros_like(x)
+    check(h.ctx.L.qd_eom13_matvec(h.ctx.h, h.h, int(mode), int(ml), int(ms), _dp(x), _dp(y), x.size))
+    return y
+
+
+def _attached_removed(h, mode, ml, ms, nev):
+    eig, w, it = np.zeros(nev), np.zeros(nev), C.c_int()
+    check(h.ctx.L.qd_eom_attached_removed(h.ctx.h, h.h, int(mode), int(ml), int(ms), int(nev), _dp(eig), _dp(w),
+                                          C.byref(it)))
+    return eig, w, it.value
+
+
+def calculate_1p_attached(h: Operator, ml: int, ms: int, nev: int):
+    """Particle-attached EOM energies dE, the 1p weight of each state and the
+    Lanczos steps (calculate_1p_attached, src/EOM_IMSRG.f90:50-87)."""
+    return _attached_removed(h, 1, ml, ms, nev)
+
+
+def calculate_1h_removed(h: Operator, ml: int, ms: int, nev: int):
+    """Hole-removed EOM energies (calculate_1h_removed, src/EOM_IMSRG.f90:92-128)."""
+    return _attached_removed(h, 2, ml, ms, nev)

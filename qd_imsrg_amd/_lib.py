@@ -125,6 +125,9 @@ SIGNATURES = {
     "qd_eom_dim": (C.c_int, [vp, C.c_int, C.c_int, i64p, i64p]),
     "qd_eom_matvec": (C.c_int, [vp, vp, C.c_int, C.c_int, dp, dp, C.c_int64]),
     "qd_eom_excited_states": (C.c_int, [vp, vp, C.c_int, C.c_int, C.c_int, dp, dp, ip]),
+    "qd_eom13_dim": (C.c_int, [vp, C.c_int, C.c_int, C.c_int, i64p, i64p]),
+    "qd_eom13_matvec": (C.c_int, [vp, vp, C.c_int, C.c_int, C.c_int, dp, dp, C.c_int64]),
+    "qd_eom_attached_removed": (C.c_int, [vp, vp, C.c_int, C.c_int, C.c_int, C.c_int, dp, dp, ip]),
     "qd_export_hamiltonian": (C.c_int, [vp, C.c_char_p]),
     "qd_import_hamiltonian": (C.c_int, [vp, C.c_char_p]),
     "qd_export_hamiltonian_host": (C.c_int, [C.c_int, C.c_int, dp, C.c_int64, C.c_char_p]),

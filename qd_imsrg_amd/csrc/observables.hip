@@ -264,15 +264,136 @@ EomSpace eom_space(const Channels &ch, int ml, int ms) {
   return s;
 }
 
+// ---- particle-attached (mode 1) and hole-removed (mode 2) EOM ---------------
+// LANCZOS_1p_ATTACHED / LANCZOS_1h_REMOVED with matvec_1p_attached /
+// matvec_1h_removed (src/EOM_IMSRG.f90:283-548, 662-812) and HQ_comm_1p,
+// HQ_comm_2p1h, HQ_comm_1h, HQ_comm_2h1p (src/commutators.f90:1263-1425).
+// The ladder operator has one-index components X1 (mode 1: particles a with
+// (m_l, m_s) = (ML, MS); mode 2: holes i with -(m_l, m_s) = (ML, MS)) and
+// three-index components X3 (mode 1: a < b particles, hole i, m_a + m_b - m_i
+// = M; mode 2: i < j holes, particle a, -m_i - m_j + m_a = M), first index
+// outer.  On the device X1 is a dense vector over the states and X3 a dense
+// array over (pair of the two like indices, third index); get_X3 is
+// antisymmetric in its first two indices (src/ME_general.f90:1243-1280).
+struct Eom13Dev {
+  int mode, n, mp;
+  int s1, s3;                    // X1 and X3 components
+  const int *l1;                 // state of each X1 component
+  const int4 *l3;                // (x, y, z, 0) of each X3 component
+  double *X1, *X3;               // dense: X1[state], X3[pair * third + k]
+};
+__device__ __forceinline__ double gx1(const Eom13Dev &q, int a) { return q.X1[a]; }
+__device__ __forceinline__ double gx3(const Eom13Dev &q, int x, int y, int z) {
+  if (x == y) return 0.0;
+  double pre = 1.0;
+  if (x > y) { int w = x; x = y; y = w; pre = -pre; }
+  if (q.mode == 1) {  // x, y particles, z hole
+    if (x < q.n || z >= q.n) return 0.0;
+    return pre * q.X3[(int64_t)fidx(x - q.n, y - q.n, q.mp) * q.n + z];
+  }
+  if (y >= q.n || z < q.n) return 0.0;  // x, y holes, z particle
+  return pre * q.X3[(int64_t)fidx(x, y, q.n) * q.mp + (z - q.n)];
+}
+__global__ void __launch_bounds__(256) k_eom13_scatter(Eom13Dev q, const double *__restrict__ x) {
+  const int64_t e = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  if (e < q.s1) {
+    q.X1[q.l1[e]] = x[e];
+  } else if (e < (int64_t)q.s1 + q.s3) {
+    const int4 p = q.l3[e - q.s1];
+    const int64_t k = q.mode == 1 ? (int64_t)fidx(p.x - q.n, p.y - q.n, q.mp) * q.n + p.z
+                                  : (int64_t)fidx(p.x, p.y, q.n) * q.mp + (p.z - q.n);
+    q.X3[k] = x[e];
+  }
+}
+// one thread per output component
+__global__ void __launch_bounds__(256) k_eom13(DevTab t, const double *__restrict__ hv, int hh, Eom13Dev q,
+                                               double *__restrict__ y) {
+  const int64_t e = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  if (e >= (int64_t)q.s1 + q.s3) return;
+  const int n = q.n, M = t.M;
+  double s = 0.0;
+  if (q.mode == 1 && e < q.s1) {  // HQ_comm_1p(a)
+    const int a = q.l1[e];
+    for (int b = n; b < M; b++) s += felem(t, hv, hh, a, b) * gx1(q, b);
+    for (int b = n; b < M; b++)
+      for (int i = 0; i < n; i++) s += felem(t, hv, hh, i, b) * gx3(q, a, b, i);
+    for (int b = n; b < M; b++)
+      for (int c = b + 1; c < M; c++)
+        for (int i = 0; i < n; i++) s += velem(t, hv, hh, a, i, b, c) * gx3(q, b, c, i);
+  } else if (q.mode == 1) {  // HQ_comm_2p1h(a, b, i)
+    const int4 p = q.l3[e - q.s1];
+    const int a = p.x, b = p.y, i = p.z;
+    for (int c = n; c < M; c++)
+      s += velem(t, hv, hh, a, b, c, i) * gx1(q, c) + felem(t, hv, hh, a, c) * gx3(q, c, b, i) -
+           felem(t, hv, hh, b, c) * gx3(q, c, a, i);
+    for (int j = 0; j < n; j++) s -= felem(t, hv, hh, j, i) * gx3(q, a, b, j);
+    for (int c = n; c < M; c++)
+      for (int d = c + 1; d < M; d++) s += velem(t, hv, hh, a, b, c, d) * gx3(q, c, d, i);
+    for (int c = n; c < M; c++)
+      for (int j = 0; j < n; j++)
+        s += -velem(t, hv, hh, a, j, c, i) * gx3(q, c, b, j) + velem(t, hv, hh, b, j, c, i) * gx3(q, c, a, j);
+  } else if (e < q.s1) {  // HQ_comm_1h(i)
+    const int i = q.l1[e];
+    for (int j = 0; j < n; j++) s -= felem(t, hv, hh, j, i) * gx1(q, j);
+    for (int b = n; b < M; b++)
+      for (int j = 0; j < n; j++) s += felem(t, hv, hh, j, b) * gx3(q, i, j, b);
+    for (int j = 0; j < n; j++)
+      for (int k = j + 1; k < n; k++)
+        for (int b = n; b < M; b++) s += velem(t, hv, hh, j, k, b, i) * gx3(q, j, k, b);
+  } else {  // HQ_comm_2h1p(i, j, a)
+    const int4 p = q.l3[e - q.s1];
+    const int i = p.x, j = p.y, a = p.z;
+    for (int k = 0; k < n; k++)
+      s += velem(t, hv, hh, a, k, i, j) * gx1(q, k) - felem(t, hv, hh, k, j) * gx3(q, i, k, a) +
+           felem(t, hv, hh, k, i) * gx3(q, j, k, a);
+    for (int b = n; b < M; b++) s += felem(t, hv, hh, a, b) * gx3(q, i, j, b);
+    for (int k = 0; k < n; k++)
+      for (int l = k + 1; l < n; l++) s += velem(t, hv, hh, k, l, i, j) * gx3(q, k, l, a);
+    for (int b = n; b < M; b++)
+      for (int k = 0; k < n; k++)
+        s += -velem(t, hv, hh, a, k, b, i) * gx3(q, k, j, b) + velem(t, hv, hh, a, k, b, j) * gx3(q, k, i, b);
+  }
+  y[e] = s;
+}
+
+struct Eom13Space {
+  std::vector<int> l1;
+  std::vector<int4> l3;
+};
+Eom13Space eom13_space(const Channels &ch, int mode, int ml, int ms) {
+  Eom13Space s;
+  const int n = ch.n, M = ch.M;
+  if (mode == 1) {
+    for (int a = n; a < M; a++)
+      if (ch.ml[a] == ml && ch.ms[a] == ms) s.l1.push_back(a);
+    for (int a = n; a < M; a++)
+      for (int b = a + 1; b < M; b++)
+        for (int i = 0; i < n; i++)
+          if (ch.ml[a] + ch.ml[b] - ch.ml[i] == ml && ch.ms[a] + ch.ms[b] - ch.ms[i] == ms)
+            s.l3.push_back(make_int4(a, b, i, 0));
+  } else {
+    for (int i = 0; i < n; i++)
+      if (-ch.ml[i] == ml && -ch.ms[i] == ms) s.l1.push_back(i);
+    for (int a = 0; a < n; a++)
+      for (int b = a + 1; b < n; b++)
+        for (int i = n; i < M; i++)
+          if (-ch.ml[a] - ch.ml[b] + ch.ml[i] == ml && -ch.ms[a] - ch.ms[b] + ch.ms[i] == ms)
+            s.l3.push_back(make_int4(a, b, i, 0));
+  }
+  return s;
+}
+
 // device buffers of one EOM solve (freed by the destructor)
 struct EomRun {
+  int mode = 0;  // 0 excited states, 1 particle attached, 2 hole removed
+  Eom13Dev q13{};
   EomDev q{};
   int2 *ph = nullptr;
   int4 *pp = nullptr;
   double *Qph = nullptr, *Qpp = nullptr, *x = nullptr, *y = nullptr;
   std::vector<void *> mem;
   ~EomRun() {
-    for (void *p : mem) hipFree(p);
+    for (void *p : mem) (void)hipFree(p);
   }
   int alloc(void **p, size_t bytes) {
     HIPX(hipMalloc(p, std::max<size_t>(bytes, 16)));
@@ -301,8 +422,48 @@ struct EomRun {
     q.Qpp = Qpp;
     return QD_OK;
   }
+  int init13(const Channels &ch, int md, const Eom13Space &s) {
+    mode = md;
+    q13.mode = md;
+    q13.n = ch.n;
+    q13.mp = ch.mp;
+    q13.s1 = (int)s.l1.size();
+    q13.s3 = (int)s.l3.size();
+    const int64_t x3 = md == 1 ? (int64_t)ch.mp * (ch.mp - 1) / 2 * ch.n : (int64_t)ch.n * (ch.n - 1) / 2 * ch.mp;
+    int *l1 = nullptr;
+    int4 *l3 = nullptr;
+    int rc;
+    if ((rc = alloc((void **)&l1, sizeof(int) * s.l1.size())) || (rc = alloc((void **)&l3, sizeof(int4) * s.l3.size())) ||
+        (rc = alloc((void **)&Qph, sizeof(double) * (size_t)ch.M)) ||
+        (rc = alloc((void **)&Qpp, sizeof(double) * (size_t)std::max<int64_t>(x3, 1))) ||
+        (rc = alloc((void **)&x, sizeof(double) * (s.l1.size() + s.l3.size()))) ||
+        (rc = alloc((void **)&y, sizeof(double) * (s.l1.size() + s.l3.size()))))
+      return rc;
+    if (!s.l1.empty()) HIPX(hipMemcpy(l1, s.l1.data(), sizeof(int) * s.l1.size(), hipMemcpyHostToDevice));
+    if (!s.l3.empty()) HIPX(hipMemcpy(l3, s.l3.data(), sizeof(int4) * s.l3.size(), hipMemcpyHostToDevice));
+    q13.l1 = l1;
+    q13.l3 = l3;
+    q13.X1 = Qph;
+    q13.X3 = Qpp;
+    x3size = std::max<int64_t>(x3, 1);
+    return QD_OK;
+  }
+  int64_t x3size = 0;
+  int64_t size() const { return mode == 0 ? (int64_t)q.sps + q.tps : (int64_t)q13.s1 + q13.s3; }
+  int64_t first() const { return mode == 0 ? q.sps : q13.s1; }  // one-index (ph / X1) components
   // y = A x for device vectors (x may be this->x)
   int matvec(hipStream_t st, const DevTab &t, const double *hv, int hherm, const double *xin, double *yout) {
+    if (mode != 0) {
+      const int64_t N = size();
+      hipLaunchKernelGGL(k_fill0, dim3(64), dim3(256), 0, st, q13.X1, (int64_t)t.M);
+      hipLaunchKernelGGL(k_fill0, dim3(1024), dim3(256), 0, st, q13.X3, x3size);
+      if (N > 0) {
+        hipLaunchKernelGGL(k_eom13_scatter, dim3((unsigned)((N + 255) / 256)), dim3(256), 0, st, q13, xin);
+        hipLaunchKernelGGL(k_eom13, dim3((unsigned)((N + 255) / 256)), dim3(256), 0, st, t, hv, hherm, q13, yout);
+      }
+      HIPX(hipGetLastError());
+      return QD_OK;
+    }
     const int64_t N = (int64_t)q.sps + q.tps;
     hipLaunchKernelGGL(k_fill0, dim3(256), dim3(256), 0, st, q.Qph, (int64_t)q.mp * q.n);
     hipLaunchKernelGGL(k_fill0, dim3(1024), dim3(256), 0, st, q.Qpp, (int64_t)q.np2 * q.nh2);
@@ -334,7 +495,7 @@ int qd_tda_spectrum(qd_context *ctx, const qd_op *h, int ml, int ms, int cut, do
   double *dm = nullptr;
   HIPX(hipMalloc(&dl, sizeof(int2) * d));
   if (hipMalloc(&dm, sizeof(double) * (size_t)d * d) != hipSuccess) {
-    hipFree(dl);
+    (void)hipFree(dl);
     return set_error(QD_E_HIP, "hipMalloc of the TDA matrix (%d x %d) failed", d, d);
   }
   std::vector<double> m((size_t)d * d);
@@ -346,8 +507,8 @@ int qd_tda_spectrum(qd_context *ctx, const qd_op *h, int ml, int ms, int cut, do
   }
   if (e == hipSuccess) e = hipMemcpyAsync(m.data(), dm, sizeof(double) * m.size(), hipMemcpyDeviceToHost, st);
   if (e == hipSuccess) e = hipStreamSynchronize(st);
-  hipFree(dl);
-  hipFree(dm);
+  (void)hipFree(dl);
+  (void)hipFree(dm);
   if (e != hipSuccess) return set_error(QD_E_HIP, "TDA matrix: %s", hipGetErrorString(e));
   // dsyev('V','U', ...) reads the upper triangle (src/ME_general.f90:576-590)
   std::vector<double> a((size_t)d * d), w, v;
@@ -386,16 +547,21 @@ int qd_eom_matvec(qd_context *ctx, const qd_op *h, int ml, int ms, const double 
   return QD_OK;
 }
 
-int qd_eom_excited_states(qd_context *ctx, const qd_op *h, int ml, int ms, int nev, double *eig, double *w1p1h,
-                          int *iters) {
+}  // extern "C"
+
+namespace {
+// Lanczos for the nev smallest eigenvalues of the EOM matrix of mode 0/1/2
+int eom_solve(qd_context *ctx, const qd_op *h, int mode, int ml, int ms, int nev, double *eig, double *w1p1h,
+              int *iters) {
   if (!ctx || !h || !eig || nev < 1) return set_error(QD_E_ARG, "null argument or nev < 1");
   const Channels &ch = context_channels(ctx);
-  const EomSpace s = eom_space(ch, ml, ms);
-  const int64_t N = (int64_t)s.ph.size() + (int64_t)s.pp.size();
-  if (N < nev) return set_error(QD_E_ARG, "EOM space of dimension %lld below nev = %d", (long long)N, nev);
   EomRun r;
   int rc;
-  if ((rc = r.init(ch, s))) return rc;
+  if (mode == 0) rc = r.init(ch, eom_space(ch, ml, ms));
+  else rc = r.init13(ch, mode, eom13_space(ch, mode, ml, ms));
+  if (rc) return rc;
+  const int64_t N = r.size();
+  if (N < nev) return set_error(QD_E_ARG, "EOM space of dimension %lld below nev = %d", (long long)N, nev);
   hipStream_t st = context_stream(ctx);
   const DevTab &t = context_devtab(ctx);
   const int kmax = (int)std::min<int64_t>(N, std::max(20 * nev, 200));
@@ -465,7 +631,7 @@ int qd_eom_excited_states(qd_context *ctx, const qd_op *h, int ml, int ms, int n
   if (!done) return set_error(QD_E_NUMERIC, "EOM Lanczos did not converge in %d iterations", kmax);
   for (int i = 0; i < nev; i++) eig[i] = ritz[i];
   if (w1p1h) {  // ph weight of each Ritz vector: sum over its ph components squared
-    const int sps = (int)s.ph.size();
+    const int sps = (int)r.first();
     std::vector<double> Vph((size_t)sps * k);
     for (int j = 0; j < k && sps > 0; j++)
       HIPX(hipMemcpyAsync(Vph.data() + (size_t)j * sps, V + (size_t)j * N, sizeof(double) * sps, hipMemcpyDeviceToHost, st));
@@ -481,6 +647,44 @@ int qd_eom_excited_states(qd_context *ctx, const qd_op *h, int ml, int ms, int n
     }
   }
   if (iters) *iters = k;
+  return QD_OK;
+}
+}  // namespace
+
+extern "C" {
+
+int qd_eom_excited_states(qd_context *ctx, const qd_op *h, int ml, int ms, int nev, double *eig, double *w1p1h,
+                          int *iters) {
+  return eom_solve(ctx, h, 0, ml, ms, nev, eig, w1p1h, iters);
+}
+
+int qd_eom_attached_removed(qd_context *ctx, const qd_op *h, int mode, int ml, int ms, int nev, double *eig,
+                            double *w1, int *iters) {
+  if (mode != 1 && mode != 2) return set_error(QD_E_ARG, "mode must be 1 (particle attached) or 2 (hole removed)");
+  return eom_solve(ctx, h, mode, ml, ms, nev, eig, w1, iters);
+}
+
+int qd_eom13_dim(const qd_context *ctx, int mode, int ml, int ms, int64_t *n1, int64_t *n3) {
+  if (!ctx || (mode != 1 && mode != 2)) return set_error(QD_E_ARG, "null context or bad mode");
+  const Eom13Space s = eom13_space(context_channels(ctx), mode, ml, ms);
+  if (n1) *n1 = (int64_t)s.l1.size();
+  if (n3) *n3 = (int64_t)s.l3.size();
+  return QD_OK;
+}
+
+int qd_eom13_matvec(qd_context *ctx, const qd_op *h, int mode, int ml, int ms, const double *x, double *y, int64_t n) {
+  if (!ctx || !h || !x || !y || (mode != 1 && mode != 2)) return set_error(QD_E_ARG, "null argument or bad mode");
+  const Channels &ch = context_channels(ctx);
+  EomRun r;
+  int rc;
+  if ((rc = r.init13(ch, mode, eom13_space(ch, mode, ml, ms)))) return rc;
+  const int64_t N = r.size();
+  if (n != N) return set_error(QD_E_ARG, "EOM space has %lld components, got %lld", (long long)N, (long long)n);
+  hipStream_t st = context_stream(ctx);
+  HIPX(hipMemcpyAsync(r.x, x, sizeof(double) * N, hipMemcpyHostToDevice, st));
+  if ((rc = r.matvec(st, context_devtab(ctx), op_cdata(h), op_herm(h), r.x, r.y))) return rc;
+  HIPX(hipMemcpyAsync(y, r.y, sizeof(double) * N, hipMemcpyDeviceToHost, st));
+  HIPX(hipStreamSynchronize(st));
   return QD_OK;
 }
 

@@ -8,8 +8,11 @@
 !     (src/gs_IMSRG.f90:35-56);
 !   * 'Hartree-Fock Energy: ' line, then the per-output line '(I5,3(e14.6))'
 !     (src/gs_IMSRG.f90:66,117);
-!   * ML and MS are read (the reference's EOM targets, src/gs_IMSRG.f90:
-!     122-129);
+!   * EOM-IM-SRG on the decoupled H(s) with 5 states (src/gs_IMSRG.f90:122-
+!     129): for MS = 0 the excited states of (ML, MS), otherwise one particle
+!     attached and one removed, each printing the reference's block and
+!     appending '(4(I5),f8.2,4(f17.9))' to ../output/EOM_particle_attached.dat
+!     / EOM_particle_removed.dat (src/EOM_IMSRG.f90:10-128);
 !   * '(I4,I5,3(f15.9),f15.4)' emax, M, E_HF, E_HF + MBPT2, E_gs, log(wall
 !     seconds) appended to ../output/TRAD_<N>_<hw>.dat (src/gs_IMSRG.f90:
 !     134-139);
@@ -71,6 +74,12 @@ program gs_decouple
   call qd_gs_params_default(p)
   p%verbose = 1
   call qd_check(qd_gs_flow(ctx, h0, hs, p, res, trace, 0_c_int), 'gs flow')
+  if (ms == 0) then
+     call excited_states(ctx, hs, ml, ms, res%e0)
+  else
+     call attached_removed(ctx, hs, 1, ml, ms, res%e0, emax, n, hw)
+     call attached_removed(ctx, hs, 2, ml, ms, res%e0, emax, n, hw)
+  end if
 
   call system_clock(c1)
   wall = real(c1 - c0, 8) / real(rate, 8)
@@ -91,4 +100,83 @@ program gs_decouple
   call qd_check(qd_op_destroy(h0), 'destroy')
   call qd_check(qd_op_destroy(hs), 'destroy')
   call qd_check(qd_context_destroy(ctx), 'destroy')
+contains
+
+  ! calculate_excited_states(ML, MS, 5, HS), src/EOM_IMSRG.f90:10-47
+  subroutine excited_states(ctx, hs, ml, ms, e0)
+    type(c_ptr), intent(in) :: ctx, hs
+    integer, intent(in) :: ml, ms
+    real(8), intent(in) :: e0
+    real(c_double) :: de(5), w(5)
+    integer(c_int) :: it
+    integer :: k
+    print *
+    write(*, '((A55),(I2),(A4),(I2))') 'EXECUTING EOM CALCULATION'//' FOR EXCITED STATES: Ml=', ml, ' Ms=', ms
+    print *
+    call qd_check(qd_eom_excited_states(ctx, hs, int(ml, c_int), int(ms, c_int), 5_c_int, de, w, it), 'EOM')
+    write(6, *)
+    print *
+    write(*, '((A21),(f16.9))') 'Ground State Energy: ', e0
+    print *
+    print *, 'EXCITED STATE ENERGIES:'
+    print *, '=================================='
+    print *, '      dE             E_0 + dE'
+    print *, '=================================='
+    do k = 1, 5
+       write(*, '(3(f16.9))') de(k), de(k) + e0, w(k)
+    end do
+  end subroutine excited_states
+
+  ! calculate_1p_attached (mode 1) / calculate_1h_removed (mode 2) with 5
+  ! states, src/EOM_IMSRG.f90:50-128
+  subroutine attached_removed(ctx, hs, mode, ml, ms, e0, emax, n, hw)
+    type(c_ptr), intent(in) :: ctx, hs
+    integer, intent(in) :: mode, ml, ms, emax, n
+    real(8), intent(in) :: e0, hw
+    real(c_double) :: de(5), w(5)
+    integer(c_int) :: it
+    integer :: k, u, ios, nshell, nfill
+    character(40) :: fname
+    nshell = emax
+    nfill = nint((sqrt(4.d0 * n + 1.d0) - 1) / 2.d0)
+    print *
+    if (mode == 1) then
+       write(*, '((A65),(I2),(A4),(I2))') 'EXECUTING EOM CALCULATION'//' FOR PARTICLE ATTACHED STATES: Ml=', &
+            ml, ' Ms=', ms
+    else
+       write(*, '((A64),(I2),(A4),(I2))') 'EXECUTING EOM CALCULATION'//' FOR PARTICLE REMOVED STATES: Ml=', &
+            ml, ' Ms=', ms
+    end if
+    print *
+    call qd_check(qd_eom_attached_removed(ctx, hs, int(mode, c_int), int(ml, c_int), int(ms, c_int), 5_c_int, &
+         de, w, it), 'EOM')
+    write(6, *)
+    print *
+    write(*, '((A21),(f16.9))') 'Ground State Energy: ', e0
+    print *
+    if (mode == 1) then
+       print *, '1p ATTACHED ENERGIES:'
+    else
+       print *, '1p REMOVED ENERGIES:'
+    end if
+    print *, '=================================='
+    print *, '      dE             E_0 + dE'
+    print *, '=================================='
+    do k = 1, 5
+       write(*, '(3(f16.9))') de(k), de(k) + e0, w(k)
+    end do
+    if (mode == 1) then
+       fname = '../output/EOM_particle_attached.dat'
+    else
+       fname = '../output/EOM_particle_removed.dat'
+    end if
+    open(newunit=u, file=trim(fname), position='append', action='write', iostat=ios)
+    if (ios == 0) then
+       write(u, '(4(I5),f8.2,4(f17.9))') nshell, nfill, ml, ms, hw, e0, de(1), de(1) + e0, w(1)
+       close(u)
+    else
+       write(0, *) 'gs_decouple: cannot open ', trim(fname)
+    end if
+  end subroutine attached_removed
+
 end program gs_decouple

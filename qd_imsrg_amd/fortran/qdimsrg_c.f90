@@ -291,6 +291,15 @@ module qdimsrg_c
        integer(c_int) :: iters
        integer(c_int) :: qd_eom_excited_states
      end function
+     function qd_eom_attached_removed(ctx, h, mode, ml, ms, nev, eig, w1, iters) &
+          bind(C, name="qd_eom_attached_removed")
+       import :: c_int, c_ptr, c_double
+       type(c_ptr), value :: ctx, h
+       integer(c_int), value :: mode, ml, ms, nev
+       real(c_double) :: eig(*), w1(*)
+       integer(c_int) :: iters
+       integer(c_int) :: qd_eom_attached_removed
+     end function
      function qd_tda_spectrum(ctx, h, ml, ms, cut, eig, cap, dim) bind(C, name="qd_tda_spectrum")
        import :: c_int, c_ptr, c_double
        type(c_ptr), value :: ctx, h

@@ -138,3 +138,65 @@ def test_tda_matrix_matches_fock_space(ml, ms):
     # the cut keeps only particles below it (1-based a <= cutshell)
     cut = ch.n + 2
     assert all(a < cut for _, a in tda_labels(ch, ml, ms, cut))
+
+
+def test_sector_matrix_matches_fock_space():
+    """The K-particle sector builder (tests/pins/sector.py) against the full
+    Fock-space matrix of fock.py (M = 6, every K)."""
+    from tests.pins.sector import sector_matrix
+    ch = Channels(2, 2)
+    op = flat_to_dense(ch, random_flat(ch, np.random.default_rng(21), 1), 1)
+    Hf = Fock(ch.M, ch.n).build(op)
+    for K in range(1, ch.M):
+        dets, H = sector_matrix(op, ch.n, K)
+        assert np.abs(H - Hf[np.ix_(dets, dets)]).max() <= 1e-12 * np.abs(Hf).max()
+
+
+def decoupled_dense(ch, seed):
+    h = random_flat(ch, np.random.default_rng(seed), 1)
+    h[ch.off_fph:ch.off_fph + ch.mp * ch.n] = 0.0
+    for q in range(ch.nblock):
+        k = int(ch.off[q, 2])
+        h[k:k + int(ch.np_[q]) * int(ch.nh[q])] = 0.0
+    return h
+
+
+@pytest.mark.parametrize("N,mode,ml,ms", [(2, 1, 0, 1), (2, 1, 1, -1), (2, 1, -2, 1), (6, 2, 0, 1), (6, 2, 1, -1),
+                                          (6, 2, -1, 1)])
+def test_eom_attached_removed_spectrum_matches_sector(N, mode, ml, ms):
+    """For a decoupled H (f_ph = 0, Gamma_pphh = 0) H|Phi> = E0|Phi>, so the
+    EOM matrix of matvec_1p_attached / matvec_1h_removed is H - E0 on the
+    span of the determinants a+|Phi>, a+ b+ i|Phi> (attached) or i|Phi>,
+    a+ j i|Phi> (removed): its eigenvalues equal those of that block of the
+    N +- 1 particle sector Hamiltonian (basis signs drop out)."""
+    from tests.pins.sector import sector_matrix
+    ch = Channels(N, 3)
+    o = Oracle(ch)
+    h = decoupled_dense(ch, 30 + mode)
+    nx = o.eom13_dim(mode, ml, ms)
+    assert nx > 1
+    A = np.zeros((nx, nx))
+    for k in range(nx):
+        e = np.zeros(nx)
+        e[k] = 1.0
+        A[:, k] = o.eom13_matvec(h, 1, mode, ml, ms, e)
+    n, M = ch.n, ch.M
+    phi = (1 << n) - 1
+    if mode == 1:
+        basis = [phi | (1 << a) for a in range(n, M) if ch.ml[a] == ml and ch.ms[a] == ms]
+        basis += [(phi & ~(1 << i)) | (1 << a) | (1 << b) for a in range(n, M) for b in range(a + 1, M)
+                  for i in range(n) if ch.ml[a] + ch.ml[b] - ch.ml[i] == ml and ch.ms[a] + ch.ms[b] - ch.ms[i] == ms]
+        K = n + 1
+    else:
+        basis = [phi & ~(1 << i) for i in range(n) if -ch.ml[i] == ml and -ch.ms[i] == ms]
+        basis += [(phi & ~(1 << a) & ~(1 << b)) | (1 << i) for a in range(n) for b in range(a + 1, n)
+                  for i in range(n, M) if -ch.ml[a] - ch.ml[b] + ch.ml[i] == ml and -ch.ms[a] - ch.ms[b] + ch.ms[i] == ms]
+        K = n - 1
+    assert len(basis) == nx
+    dets, H = sector_matrix(flat_to_dense(ch, h, 1), n, K)
+    idx = {d: i for i, d in enumerate(dets)}
+    sel = [idx[b] for b in basis]
+    P = H[np.ix_(sel, sel)] - h[0] * np.eye(nx)
+    ev_a = np.sort(np.linalg.eigvals(A).real)
+    ev_p = np.sort(np.linalg.eigvalsh(P))
+    assert np.abs(ev_a - ev_p).max() <= 1e-11 * max(1.0, np.abs(ev_p).max())

@@ -99,4 +99,14 @@ def test_gs_decouple_outputs_match_library(tmp_path):
     ham = qd.import_hamiltonian_host(6, 6, str(tmp_path / "hamiltonians" / "ham_6_1.00_6"))
     hs = HS.download()
     assert np.abs(ham - hs).max() <= 1e-10 * np.abs(hs).max()
+    # MS = 1: one particle attached and one removed (src/gs_IMSRG.f90:124-128)
+    for mode, key, fname in ((1, "PARTICLE ATTACHED", "EOM_particle_attached.dat"),
+                             (2, "PARTICLE REMOVED", "EOM_particle_removed.dat")):
+        assert any(f"EXECUTING EOM CALCULATION FOR {key} STATES: Ml= 0 Ms= 1" in l for l in lines)
+        de, w, _ = (qd.calculate_1p_attached if mode == 1 else qd.calculate_1h_removed)(HS, 0, 1, 5)
+        rec = (tmp_path / "output" / fname).read_text().splitlines()
+        assert len(rec) == 1 and len(rec[0]) == 4 * 5 + 8 + 4 * 17
+        f = fixed(rec[0], [5] * 4 + [8] + [17] * 4)
+        assert [int(x) for x in f[:4]] == [6, 2, 0, 1] and float(f[4]) == 1.0
+        assert abs(float(f[6]) - de[0]) <= 1e-8 and abs(float(f[8]) - w[0]) <= 1e-8
     ctx.close()

@@ -84,3 +84,35 @@ def test_eom_excited_states(c1, ml, ms):
     Ah = eom_matrix(o, hs, 1, ml, ms)
     ref2 = np.linalg.eigvalsh(0.5 * (Ah + Ah.T))[:10]
     assert np.abs(eig2 - ref2).max() <= 1e-6
+
+
+@pytest.mark.parametrize("mode,ml,ms", [(1, 0, 1), (1, 2, -1), (2, 0, 1), (2, -1, -1)])
+def test_eom_attached_removed(c1, mode, ml, ms):
+    """matvec_1p_attached / matvec_1h_removed against the oracle (1e-12) and
+    the Lanczos eigenvalues of a decoupled H against a dense eigensolver of
+    the oracle's matrix (1e-9 Ha), one-index weights 1e-6."""
+    ch, o, ctx, H0, HS = c1
+    h = random_flat(ch, np.random.default_rng(90 + mode), 1)
+    H = ctx.operator(1, h)
+    n1, n3 = qd.eom13_dim(ctx, mode, ml, ms)
+    assert n1 + n3 == o.eom13_dim(mode, ml, ms) > 10
+    x = np.random.default_rng(91).standard_normal(n1 + n3)
+    got = qd.eom13_matvec(H, mode, ml, ms, x)
+    ref = o.eom13_matvec(h, 1, mode, ml, ms, x)
+    assert np.abs(got - ref).max() <= 1e-12 * np.abs(ref).max()
+    hd = decoupled(ch, HS.download())
+    D = ctx.operator(1, hd)
+    f = qd.calculate_1p_attached if mode == 1 else qd.calculate_1h_removed
+    nev = 5
+    eig, w, it = f(D, ml, ms, nev)
+    nx = n1 + n3
+    A = np.zeros((nx, nx))
+    for k in range(nx):
+        e = np.zeros(nx)
+        e[k] = 1.0
+        A[:, k] = o.eom13_matvec(hd, 1, mode, ml, ms, e)
+    assert np.abs(A - A.T).max() <= 1e-12 * np.abs(A).max()
+    lam, U = np.linalg.eigh(A)
+    print(f"mode {mode} (ML, MS) = ({ml}, {ms}): {it} Lanczos steps, dE = {np.round(eig, 6)}")
+    assert np.abs(eig - lam[:nev]).max() <= 1e-9
+    assert np.abs(w - (U[:n1, :nev] ** 2).sum(axis=0)).max() <= 1e-6
