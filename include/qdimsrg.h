@@ -383,6 +383,15 @@ int qd_hf_hamiltonian(int nbody, int nshell, double hw, double *h0, int64_t neq,
  * reference's 'failfile' stop). */
 int qd_hf_hamiltonian_me(int nbody, int nshell, double hw, const char *me_path, double *h0, int64_t neq,
                          double *eig, int *iters);
+/* qd_hf_hamiltonian with the heavy parts on the device and H(0) written into
+ * the hermitian operator h0 of ctx (its layout gives nbody and nshell): the
+ * Fock build of every SCF iteration (construct_V, src/HFQDmod.f90:196-221,
+ * 260-290) and the two-body basis transform (new_VMAT, src/ME_general.f90:
+ * 1362-1492) run on the context's stream from the Coulomb table uploaded once;
+ * the per-channel diagonalisations and the convergence test stay on the host.
+ * Same SCF, phases and E0 rule as qd_hf_hamiltonian; results agree with it to
+ * FP64 rounding. */
+int qd_hf_hamiltonian_device(qd_context *ctx, double hw, qd_op *h0, double *eig, int *iters);
 /* Writes ME.bin for nshell shells from v_int at hw = 1: one record per
  * (a, b, c, d) with m_a + m_b = m_c + m_d and (a, b) <= (c, d) in orbital
  * order; *nrec (optional) receives the record count.  The reference ships no

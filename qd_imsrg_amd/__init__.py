@@ -428,6 +428,18 @@ def hf_hamiltonian(nbody: int, nshell: int, hw: float, me_bin: str | None = None
     return h0, eig, it.value
 
 
+def hf_hamiltonian_device(ctx: "Context", hw: float, out: "Operator | None" = None):
+    """H(0) in the HF basis built on the device (Fock builds and the two-body
+    transform) into the hermitian operator out (a new one when None); returns
+    (operator, HF energies, SCF iteration count)."""
+    L = _lib.load()
+    H = ctx.operator(1) if out is None else out
+    M = ctx.nshell * (ctx.nshell + 1)
+    eig, it = np.zeros(M), C.c_int()
+    check(L.qd_hf_hamiltonian_device(ctx.h, float(hw), H.h, _dp(eig), C.byref(it)))
+    return H, eig, it.value
+
+
 def me_bin_write(nshell: int, path: str) -> int:
     """Write ME.bin (<ab|V|cd> at hw = 1 from v_int) for nshell shells;
     returns the record count."""

@@ -120,6 +120,7 @@ SIGNATURES = {
     "qd_v_int": (C.c_double, [ip, ip, ip, ip, C.c_double]),
     "qd_hf_hamiltonian": (C.c_int, [C.c_int, C.c_int, C.c_double, dp, C.c_int64, dp, ip]),
     "qd_hf_hamiltonian_me": (C.c_int, [C.c_int, C.c_int, C.c_double, C.c_char_p, dp, C.c_int64, dp, ip]),
+    "qd_hf_hamiltonian_device": (C.c_int, [vp, C.c_double, vp, dp, ip]),
     "qd_me_bin_write": (C.c_int, [C.c_int, C.c_char_p, i64p]),
     "qd_tda_spectrum": (C.c_int, [vp, vp, C.c_int, C.c_int, C.c_int, dp, C.c_int, ip]),
     "qd_eom_dim": (C.c_int, [vp, C.c_int, C.c_int, i64p, i64p]),

@@ -14,7 +14,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 OBJ = os.path.join(HERE, "_obj")
 LIB = os.path.join(HERE, "libqdimsrg.so")
-SOURCES = ["channels.cpp", "hf.cpp", "io.cpp", "kernels.hip", "boxes.hip", "gemm.hip", "capi.hip", "mlp.hip", "ode.hip", "observables.hip"]
+SOURCES = ["channels.cpp", "hf.cpp", "io.cpp", "kernels.hip", "boxes.hip", "gemm.hip", "capi.hip", "mlp.hip", "ode.hip", "observables.hip", "hf_device.hip"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = "gfx950"
 FLAGS = ["-O3", "-fPIC", "-std=c++17", f"--offload-arch={ARCH}", "-Wall", "-Wno-unused-function",

@@ -21,6 +21,7 @@
 
 #include <omp.h>
 
+#include "hf_device.h"
 #include "qd_internal.h"
 
 namespace qd {
@@ -390,12 +391,14 @@ namespace qd {
 namespace {
 // construct_two_particle_HF_BASIS + HF_E2 on a Coulomb table S (<ab|V|cd> at
 // hw = 1 over the spatial orbitals of nshell shells)
+// dctx/dout: the Fock builds and the two-body transform run on the device and
+// H(0) is written into the device operator dout (h0 unused)
 int hf_build(int nbody, int nshell, double hw, const Spatial &S, double *h0, int64_t neq, double *eig_out,
-             int *iters) {
+             int *iters, qd_context *dctx = nullptr, qd_op *dout = nullptr) {
   Channels ch;
   std::string why = build_channels(nbody, nshell, ch);
   if (!why.empty()) return set_error(QD_E_LAYOUT, "%s", why.c_str());
-  if (!h0) return set_error(QD_E_ARG, "h0 is null");
+  if (!h0 && !dout) return set_error(QD_E_ARG, "h0 is null");
   if (neq != ch.neq) return set_error(QD_E_ARG, "h0 holds %lld elements, the layout needs %lld", (long long)neq,
                                       (long long)ch.neq);
   const int M = ch.M, n = ch.n;
@@ -427,6 +430,17 @@ int hf_build(int nbody, int nshell, double hw, const Spatial &S, double *h0, int
     const int R = 2 * S.n[sp[i]] + std::abs(S.m[sp[i]]);
     Tm[i] = hw * (R + 1);
     C[(size_t)i * M + i] = 1.0;  // C[ho*M + hf]
+  }
+  HfDevice *dev = nullptr;
+  struct DevGuard {
+    HfDevice *&d;
+    ~DevGuard() { hf_device_destroy(d); }
+  } guard{dev};
+  if (dctx) {
+    const CoulombView cv{S.N, S.tab.data(), (int64_t)S.tab.size(), S.start.data(), S.pos_in_m.data(), S.m.data(),
+                         !S.present.empty()};
+    int rc = hf_device_create(dctx, cv, sp, spin, shw, &dev);
+    if (rc) return rc;
   }
   // Roothaan iteration as in the reference; if it has not converged after
   // 100 iterations (it can settle into a 2-cycle, e.g. N=20, hw=0.5, R>=10,
@@ -464,19 +478,24 @@ int hf_build(int nbody, int nshell, double hw, const Spatial &S, double *h0, int
       rho_old = rho;
     }
     std::fill(F.begin(), F.end(), 0.0);
+    if (dev) {
+      int rc = hf_device_fock(dev, rho, Tm, F);
+      if (rc) return rc;
+    } else {
 #pragma omp parallel for schedule(dynamic, 1)
-    for (int cc = 0; cc < (int)chan.size(); cc++)
-      for (int i : chan[cc])
-        for (int k : chan[cc]) {
-          double v = 0;
-          for (auto &cj : chan)
-            for (int j : cj)
-              for (int l : cj) {
-                const double r = rho[(size_t)l * M + j];
-                if (r != 0.0) v += r * asym(j, i, l, k);
-              }
-          F[(size_t)i * M + k] = (i == k ? Tm[i] : 0.0) + v;
-        }
+      for (int cc = 0; cc < (int)chan.size(); cc++)
+        for (int i : chan[cc])
+          for (int k : chan[cc]) {
+            double v = 0;
+            for (auto &cj : chan)
+              for (int j : cj)
+                for (int l : cj) {
+                  const double r = rho[(size_t)l * M + j];
+                  if (r != 0.0) v += r * asym(j, i, l, k);
+                }
+            F[(size_t)i * M + k] = (i == k ? Tm[i] : 0.0) + v;
+          }
+    }
     std::vector<double> eold = eig;
     std::fill(C.begin(), C.end(), 0.0);
     for (auto &cs : chan) {
@@ -515,32 +534,30 @@ int hf_build(int nbody, int nshell, double hw, const Spatial &S, double *h0, int
       FHF[(size_t)a * M + b] = s;
     }
   // pack: E0 set below, one-body parts column-major
-  std::memset(h0, 0, sizeof(double) * neq);
+  std::vector<double> hbuf;
+  if (dev) hbuf.assign((size_t)ch.off_2b, 0.0);  // one-body region, uploaded below
+  double *hv = dev ? hbuf.data() : h0;
+  if (!dev) std::memset(h0, 0, sizeof(double) * neq);
   const int mp = ch.mp;
   for (int j = 0; j < n; j++)
-    for (int i = 0; i < n; i++) h0[ch.off_fhh + i + (int64_t)j * n] = FHF[(size_t)i * M + j];
+    for (int i = 0; i < n; i++) hv[ch.off_fhh + i + (int64_t)j * n] = FHF[(size_t)i * M + j];
   for (int j = 0; j < mp; j++)
-    for (int i = 0; i < mp; i++) h0[ch.off_fpp + i + (int64_t)j * mp] = FHF[(size_t)(n + i) * M + n + j];
+    for (int i = 0; i < mp; i++) hv[ch.off_fpp + i + (int64_t)j * mp] = FHF[(size_t)(n + i) * M + n + j];
   for (int j = 0; j < n; j++)
-    for (int i = 0; i < mp; i++) h0[ch.off_fph + i + (int64_t)j * mp] = FHF[(size_t)(n + i) * M + j];
-  // two-body: per block HFV = T V_HO T^T over the block's pairs (hh | ph | pp)
-#pragma omp parallel for schedule(dynamic, 1)
-  for (int q = 0; q < ch.nblock; q++) {
+    for (int i = 0; i < mp; i++) hv[ch.off_fph + i + (int64_t)j * mp] = FHF[(size_t)(n + i) * M + j];
+  // two-body: per block HFV = T V_HO T^T over the block's pairs (hh | ph | pp).
+  // T[x][z] = C[b1,a1] C[b2,a2] - C[b2,a1] C[b1,a2] for HF pair x = (a1,a2) and
+  // oscillator pair z = (b1,b2): non-zero only for b1, b2 in the channels of
+  // a1, a2 (either order), so each row is built from those channels alone
+  auto t_rows = [&](int q) {
     const int nh = ch.nh[q], nb = ch.nb[q], np = ch.np[q], nt = nh + nb + np;
-    if (nt == 0) continue;
     const uint32_t *P = &ch.pairs[ch.hh_start[q]];  // hh, ph, pp are contiguous
-    // T[x][z] = C[b1,a1] C[b2,a2] - C[b2,a1] C[b1,a2] for HF pair x = (a1,a2) and
-    // oscillator pair z = (b1,b2): non-zero only for b1, b2 in the channels of
-    // a1, a2 (either order), so each row is built from those channels alone
-    std::vector<double> V((size_t)nt * nt), TV((size_t)nt * nt, 0.0), H((size_t)nt * nt, 0.0);
     std::vector<std::vector<std::pair<int, double>>> T(nt);
     auto comb = [&](int i, int j) {  // combined (hh | ph | pp) index of pair i < j in block q
       const uint32_t ia = ch.iarr[(size_t)i * M - (size_t)i * (i + 1) / 2 + (j - i - 1)];
       const int t = ia_type(ia), k = ia_idx(ia);
       return t == T_HH ? k : (t == T_PH ? nh + k : nh + nb + k);
     };
-    for (int x = 0; x < nt; x++)
-      for (int y = 0; y < nt; y++) V[(size_t)x * nt + y] = asym(lo16(P[x]), hi16(P[x]), lo16(P[y]), hi16(P[y]));
     for (int x = 0; x < nt; x++) {
       const int a1 = lo16(P[x]), a2 = hi16(P[x]);
       const auto &c1 = chan[ch.spch[a1]], &c2 = chan[ch.spch[a2]];
@@ -553,38 +570,77 @@ int hf_build(int nbody, int nshell, double hw, const Spatial &S, double *h0, int
           if (t != 0.0) T[x].push_back({comb(lo, hi), t});
         }
     }
-    for (int x = 0; x < nt; x++)
-      for (const auto &zt : T[x]) {
-        const double *v = &V[(size_t)zt.first * nt];
-        double *tv = &TV[(size_t)x * nt];
-        for (int y = 0; y < nt; y++) tv[y] += zt.second * v[y];
+    return T;
+  };
+  if (dev) {
+    hipStream_t st = context_stream(dctx);
+    if (hipMemcpyAsync(op_data(dout), hbuf.data(), sizeof(double) * hbuf.size(), hipMemcpyHostToDevice, st) !=
+            hipSuccess ||
+        hipMemsetAsync(op_data(dout) + ch.off_2b, 0, sizeof(double) * (size_t)(neq - ch.off_2b), st) != hipSuccess)
+      return set_error(QD_E_HIP, "upload of the one-body part failed");
+    std::vector<std::vector<std::vector<std::pair<int, double>>>> rows(ch.nblock);
+#pragma omp parallel for schedule(dynamic, 1)
+    for (int q = 0; q < ch.nblock; q++) rows[q] = t_rows(q);
+    SparseRows Tr;
+    Tr.row_start.push_back(0);
+    for (int q = 0; q < ch.nblock; q++)
+      for (const auto &r : rows[q]) {
+        for (const auto &zt : r) {
+          Tr.col.push_back(zt.first);
+          Tr.val.push_back(zt.second);
+        }
+        Tr.row_start.push_back((int)Tr.col.size());
       }
-    for (int x = 0; x < nt; x++)
-      for (int y = 0; y < nt; y++) {
-        double s = 0;
-        for (const auto &zt : T[y]) s += TV[(size_t)x * nt + zt.first] * zt.second;
-        H[(size_t)x * nt + y] = s;
-      }
-    const int64_t *o = &ch.off[(size_t)q * 6];
-    auto put = [&](int sub, int r0, int rows, int c0, int cols) {
-      for (int c = 0; c < cols; c++)
-        for (int r = 0; r < rows; r++) h0[o[sub] + r + (int64_t)c * rows] = H[(size_t)(r0 + r) * nt + c0 + c];
-    };
-    const int h = 0, x = nh, p = nh + nb;
-    put(HHHH, h, nh, h, nh);
-    put(PPPP, p, np, p, np);
-    put(PPHH, p, np, h, nh);
-    put(PHPH, x, nb, x, nb);
-    put(PPPH, p, np, x, nb);
-    put(PHHH, x, nb, h, nh);
+    int rc;
+    if ((rc = hf_device_transform(dev, Tr, dout))) return rc;
+  } else {
+#pragma omp parallel for schedule(dynamic, 1)
+    for (int q = 0; q < ch.nblock; q++) {
+      const int nh = ch.nh[q], nb = ch.nb[q], np = ch.np[q], nt = nh + nb + np;
+      if (nt == 0) continue;
+      const uint32_t *P = &ch.pairs[ch.hh_start[q]];
+      std::vector<double> V((size_t)nt * nt), TV((size_t)nt * nt, 0.0), H((size_t)nt * nt, 0.0);
+      const auto T = t_rows(q);
+      for (int x = 0; x < nt; x++)
+        for (int y = 0; y < nt; y++) V[(size_t)x * nt + y] = asym(lo16(P[x]), hi16(P[x]), lo16(P[y]), hi16(P[y]));
+      for (int x = 0; x < nt; x++)
+        for (const auto &zt : T[x]) {
+          const double *v = &V[(size_t)zt.first * nt];
+          double *tv = &TV[(size_t)x * nt];
+          for (int y = 0; y < nt; y++) tv[y] += zt.second * v[y];
+        }
+      for (int x = 0; x < nt; x++)
+        for (int y = 0; y < nt; y++) {
+          double sum = 0;
+          for (const auto &zt : T[y]) sum += TV[(size_t)x * nt + zt.first] * zt.second;
+          H[(size_t)x * nt + y] = sum;
+        }
+      const int64_t *o = &ch.off[(size_t)q * 6];
+      auto put = [&](int sub, int r0, int rows, int c0, int cols) {
+        for (int c = 0; c < cols; c++)
+          for (int r = 0; r < rows; r++) h0[o[sub] + r + (int64_t)c * rows] = H[(size_t)(r0 + r) * nt + c0 + c];
+      };
+      const int h = 0, x = nh, p = nh + nb;
+      put(HHHH, h, nh, h, nh);
+      put(PPPP, p, np, p, np);
+      put(PPHH, p, np, h, nh);
+      put(PHPH, x, nb, x, nb);
+      put(PPPH, p, np, x, nb);
+      put(PHHH, x, nb, h, nh);
+    }
   }
   if (trace) fprintf(stderr, "hf: basis transform %.2f s\n", now() - t_mark);
-  double vsum = 0;  // sum over blocks of the Vhhhh diagonal, in block order
-  for (int q = 0; q < ch.nblock; q++)
-    for (int j = 0; j < ch.nh[q]; j++) vsum += h0[ch.off[(size_t)q * 6 + HHHH] + j + (int64_t)j * ch.nh[q]];
   double eh = 0;
   for (int i = 0; i < n; i++) eh += eig[i];
-  h0[0] = eh - vsum;  // HF_E2 (src/HFQDmod.f90:292-323)
+  if (dev) {
+    int rc = hf_device_e0(dctx, eh, dout);
+    if (rc) return rc;
+  } else {
+    double vsum = 0;  // sum over blocks of the Vhhhh diagonal, in block order
+    for (int q = 0; q < ch.nblock; q++)
+      for (int j = 0; j < ch.nh[q]; j++) vsum += h0[ch.off[(size_t)q * 6 + HHHH] + j + (int64_t)j * ch.nh[q]];
+    h0[0] = eh - vsum;  // HF_E2 (src/HFQDmod.f90:292-323)
+  }
   if (eig_out) std::memcpy(eig_out, eig.data(), sizeof(double) * M);
   if (iters) *iters = total;
   if (crit > 1e-6) return set_error(QD_E_NUMERIC, "Hartree-Fock did not converge (crit %g after %d iterations)", crit, total);
@@ -653,6 +709,20 @@ int qd_hf_hamiltonian(int nbody, int nshell, double hw, double *h0, int64_t neq,
     fprintf(stderr, "hf: Coulomb table (R=%d) %.2f s\n", nshell,
             std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count());
   return hf_build(nbody, nshell, hw, S, h0, neq, eig_out, iters);
+}
+
+int qd_hf_hamiltonian_device(qd_context *ctx, double hw, qd_op *h0, double *eig_out, int *iters) {
+  if (!ctx || !h0) return set_error(QD_E_ARG, "null argument");
+  if (op_context(h0) != ctx) return set_error(QD_E_ARG, "operator belongs to another context");
+  if (op_herm(h0) != 1) return set_error(QD_E_ARG, "H(0) needs a hermitian operator");
+  const Channels &ch = context_channels(ctx);
+  const bool trace = std::getenv("QD_HF_TRACE") != nullptr;
+  auto t0 = std::chrono::steady_clock::now();
+  const Spatial &S = spatial_table(ch.R);
+  if (trace)
+    fprintf(stderr, "hf: Coulomb table (R=%d) %.2f s\n", ch.R,
+            std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count());
+  return hf_build(ch.n, ch.R, hw, S, nullptr, ch.neq, eig_out, iters, ctx, h0);
 }
 
 int qd_hf_hamiltonian_me(int nbody, int nshell, double hw, const char *me_path, double *h0, int64_t neq,
