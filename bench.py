@@ -80,11 +80,11 @@ def cpu_baseline(x, y, nbody, nshell, flops_ref, flows, budget_s=20.0):
         recs = []
         for rec in flows["runs"]:
             t0 = time.perf_counter()
-            ref = magnus_flow(o6, flows["h0"], pandya=rec["pandya"])
+            ref = magnus_flow(o6, flows["h0"], pandya=rec["pandya"], stop_rule=rec["stop_rule"], conv_deds=1e-8)
             t = time.perf_counter() - t0
             recs.append(dict(workload=rec["workload"], cpu_oracle_s=t, oracle_e_gs=ref["e0"],
                              gpu_s=rec["gpu_s"], abs_dE_gpu_vs_oracle=abs(rec["e_gs"] - ref["e0"])))
-            progress(f"CPU oracle flow pandya={rec['pandya']}: {t:.2f} s")
+            progress(f"CPU oracle flow pandya={rec['pandya']} stop {rec['stop_rule']}: {t:.2f} s")
         out["time_to_egs"] = recs
     return out
 
@@ -92,11 +92,13 @@ def cpu_baseline(x, y, nbody, nshell, flops_ref, flows, budget_s=20.0):
 def time_to_egs():
     """IMSRG(2) time-to-E_gs at N=6, R=6, hw=1 (HF basis, Magnus, White):
     wall time from H(0) resident on the GPU to convergence, as shipped (zero
-    CC inputs) and with calc_cc-filled CC inputs (the Pandya term)."""
+    CC inputs) and with calc_cc-filled CC inputs (the Pandya term), each
+    under the reference's stop rule (|MBPT2| <= 1e-9) and under
+    |dE/ds| < 1e-8 Ha."""
     import qd_imsrg_amd as qd
     h0, _, _ = qd.hf_hamiltonian(6, 6, 1.0)
     out = []
-    for pandya in (False, True):
+    for pandya, rule in ((False, "mbpt2"), (True, "mbpt2"), (False, "deds"), (True, "deds")):
         ctx = qd.Context(6, 6)
         ctx.set_graphs(True)
         H0 = ctx.operator(1, h0)
@@ -105,14 +107,15 @@ def time_to_egs():
             HS, OM = ctx.operator(1), ctx.operator(-1)
             ctx.sync()
             t0 = time.perf_counter()
-            res, _ = qd.magnus_flow(ctx, H0, HS, OM, pandya=pandya)
+            res, _ = qd.magnus_flow(ctx, H0, HS, OM, pandya=pandya, stop_rule=rule, conv_deds=1e-8)
             t = time.perf_counter() - t0
             best = t if best is None else min(best, t)
-            progress(f"N=6 R=6 flow pandya={pandya} rep {rep}: {t:.3f} s, E={res.e0:.9f}")
+            progress(f"N=6 R=6 flow pandya={pandya} stop {rule} rep {rep}: {t:.3f} s, E={res.e0:.9f}")
         out.append({"workload": f"N=6 R=6 hw=1.0 HF basis, Magnus IM-SRG(2) White, "
-                                f"{'calc_cc CC inputs (Pandya term)' if pandya else 'as shipped (zero CC inputs)'}",
-                    "pandya": pandya, "gpu_s": best, "e_gs": res.e0, "iterations": res.iterations,
-                    "bch_terms": int(res.bch_terms)})
+                                f"{'calc_cc CC inputs (Pandya term)' if pandya else 'as shipped (zero CC inputs)'}, "
+                                f"stop {'|MBPT2| <= 1e-9' if rule == 'mbpt2' else '|dE/ds| < 1e-8'}",
+                    "pandya": pandya, "stop_rule": rule, "gpu_s": best, "e_gs": res.e0,
+                    "iterations": res.iterations, "bch_terms": int(res.bch_terms), "deds": res.deds})
         ctx.close()
     return {"runs": out, "h0": h0}
 
@@ -126,17 +129,17 @@ def precision_form(nbody=42, nshell=20, hw=0.28, modes=("fp64", "bf16x3", "fp32"
     flops / time)."""
     import qd_imsrg_amd as qd
     from qd_imsrg_amd.layout import Layout
-    t0 = time.perf_counter()
-    h0, _, it = qd.hf_hamiltonian(nbody, nshell, hw)
-    t_hf = time.perf_counter() - t0
-    progress(f"N={nbody} R={nshell} HF done in {t_hf:.1f} s")
     lay = Layout(nbody, nshell)
     flops = lay.reference_gemm_flops()["total"]
     ctx = qd.Context(nbody, nshell)
+    t0 = time.perf_counter()
+    H0, _, it = qd.hf_hamiltonian_device(ctx, hw)
+    h0 = H0.download()
+    t_hf = time.perf_counter() - t0
+    progress(f"N={nbody} R={nshell} HF (device) done in {t_hf:.1f} s")
     tune = ctx.autotune(reps=1)
     progress(f"autotune GEMM ms {tune[0]:.2f} -> {tune[1]:.2f}")
     ctx.set_graphs(True)
-    H0 = ctx.operator(1, h0)
     HCC, ECC, Z = ctx.cc(1.0), ctx.cc(-1.0), ctx.operator(1)
     res = {}
     for p in modes:
@@ -173,7 +176,8 @@ def precision_form(nbody=42, nshell=20, hw=0.28, modes=("fp64", "bf16x3", "fp32"
             v["rel_err_gamma_od_norm"] = abs(v["gamma_od_norm"] - ref["gamma_od_norm"]) / ref["gamma_od_norm"]
             v["speedup_vs_fp64"] = ref["time_to_egs_s"] / v["time_to_egs_s"]
     res["workload"] = (f"N={nbody} R={nshell} hw={hw} HF basis (E_HF={h0[0]:.9f}, {it} SCF iterations, "
-                       f"{t_hf:.1f} s host), Magnus IM-SRG(2) White, as shipped; F_2b2b={flops / 1e9:.1f} GF")
+                       f"{t_hf:.1f} s on the device), Magnus IM-SRG(2) White, as shipped; "
+                       f"F_2b2b={flops / 1e9:.1f} GF")
     return res
 
 
@@ -184,10 +188,13 @@ def batched_flow(nbody=20, nshell=16, hw=0.5):
     the same H(0).  Two further rows add the Pulay-accelerated tail and then
     BF16x3 2b-2b GEMMs, each with |E_gs - E_gs(plain FP64)|."""
     import qd_imsrg_amd as qd
+    ctx = qd.Context(nbody, nshell)
     t0 = time.perf_counter()
-    h0, _, it = qd.hf_hamiltonian(nbody, nshell, hw)
+    H0, _, it = qd.hf_hamiltonian_device(ctx, hw)
+    h0 = H0.download()
     t_hf = time.perf_counter() - t0
-    progress(f"N={nbody} R={nshell} HF done in {t_hf:.1f} s")
+    ctx.close()
+    progress(f"N={nbody} R={nshell} HF (device) done in {t_hf:.1f} s")
     res = {}
     modes = [("per_channel", 1, False, 0, "fp64"), ("grouped_autotuned_graphs", 0, True, 0, "fp64"),
              ("grouped_graphs_diis", 0, True, 6, "fp64"), ("grouped_graphs_diis_bf16x3", 0, True, 6, "bf16x3")]
@@ -219,7 +226,8 @@ def batched_flow(nbody=20, nshell=16, hw=0.5):
         v["speedup_vs_plain_fp64"] = ref["time_to_egs_s"] / v["time_to_egs_s"]
     res["abs_dE_between_paths"] = abs(ref["e_gs"] - res["grouped_autotuned_graphs"]["e_gs"])
     res["speedup"] = ref["time_to_egs_s"] / res["grouped_autotuned_graphs"]["time_to_egs_s"]
-    res["workload"] = f"N={nbody} R={nshell} hw={hw} HF basis (E_HF={h0[0]:.9f}, {it} SCF iterations, {t_hf:.1f} s host)"
+    res["workload"] = (f"N={nbody} R={nshell} hw={hw} HF basis (E_HF={h0[0]:.9f}, {it} SCF iterations, "
+                       f"{t_hf:.1f} s on the device)")
     return res
 
 
@@ -243,8 +251,9 @@ def learned_form(nbody=20, nshell=14, n_omega=64, lo=0.1, hi=1.0, steps=20000, l
     t_sweep = time.perf_counter()
     for k, w in enumerate(omegas):
         try:
-            h0, _, _ = qd.hf_hamiltonian(nbody, nshell, float(w))
-            H0, HS, OM = ctx.operator(1, h0), ctx.operator(1), ctx.operator(-1)
+            H0, _, _ = qd.hf_hamiltonian_device(ctx, float(w))
+            h0 = H0.download()
+            HS, OM = ctx.operator(1), ctx.operator(-1)
             t0 = time.perf_counter()
             r, _ = qd.magnus_flow(ctx, H0, HS, OM)
             t = time.perf_counter() - t0

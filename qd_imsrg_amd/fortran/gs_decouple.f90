@@ -61,13 +61,12 @@ program gs_decouple
   call qd_check(qd_context_create(int(n, c_int), int(emax, c_int), ctx), 'context')
   call qd_check(qd_context_info(ctx, info), 'info')
   allocate(vec(info%neq), eig(m))
-  call qd_check(qd_hf_hamiltonian(int(n, c_int), int(emax, c_int), real(hw, c_double), vec, info%neq, eig, it), &
-       'Hartree-Fock')
-  print *, 'Hartree-Fock Energy: ', vec(1)
-  flush(6)
   call qd_check(qd_op_create(ctx, 1_c_int, h0), 'op')
   call qd_check(qd_op_create(ctx, 1_c_int, hs), 'op')
-  call qd_check(qd_op_upload(h0, vec, info%neq), 'upload')
+  call qd_check(qd_hf_hamiltonian_device(ctx, real(hw, c_double), h0, eig, it), 'Hartree-Fock')
+  call qd_check(qd_op_download(h0, vec, info%neq), 'download')
+  print *, 'Hartree-Fock Energy: ', vec(1)
+  flush(6)
   call qd_check(qd_mbpt2(ctx, h0, e2), 'MBPT2')
   e2nd = vec(1) + e2
 

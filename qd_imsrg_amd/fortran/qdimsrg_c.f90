@@ -282,6 +282,14 @@ module qdimsrg_c
        integer(c_int) :: iters
        integer(c_int) :: qd_hf_hamiltonian
      end function
+     function qd_hf_hamiltonian_device(ctx, hw, h0, eig, iters) bind(C, name="qd_hf_hamiltonian_device")
+       import :: c_int, c_double, c_ptr
+       type(c_ptr), value :: ctx, h0
+       real(c_double), value :: hw
+       real(c_double) :: eig(*)
+       integer(c_int) :: iters
+       integer(c_int) :: qd_hf_hamiltonian_device
+     end function
      ! ---- consumers of H(s) (src/operators.f90:7-124, src/EOM_IMSRG.f90:10-47) ----
      function qd_eom_excited_states(ctx, h, ml, ms, nev, eig, w1p1h, iters) bind(C, name="qd_eom_excited_states")
        import :: c_int, c_ptr, c_double

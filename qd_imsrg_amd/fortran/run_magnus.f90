@@ -30,7 +30,7 @@ program run_magnus
   type(qd_info) :: info
   type(qd_flow_params) :: p
   type(qd_flow_result) :: res
-  real(c_double), allocatable :: vec(:), eig(:)
+  real(c_double), allocatable :: eig(:)
   real(c_double) :: trace(6), de(10), w(10)
   integer(c_int) :: it
 
@@ -65,13 +65,11 @@ program run_magnus
 
   call qd_check(qd_context_create(int(n, c_int), int(emax, c_int), ctx), 'context')
   call qd_check(qd_context_info(ctx, info), 'info')
-  allocate(vec(info%neq), eig(m))
-  call qd_check(qd_hf_hamiltonian(int(n, c_int), int(emax, c_int), real(hw, c_double), vec, info%neq, eig, it), &
-       'Hartree-Fock')
+  allocate(eig(m))
   call qd_check(qd_op_create(ctx, 1_c_int, h0), 'op')
   call qd_check(qd_op_create(ctx, 1_c_int, hs), 'op')
   call qd_check(qd_op_create(ctx, -1_c_int, om), 'op')
-  call qd_check(qd_op_upload(h0, vec, info%neq), 'upload')
+  call qd_check(qd_hf_hamiltonian_device(ctx, real(hw, c_double), h0, eig, it), 'Hartree-Fock')
 
   call qd_flow_params_default(p)
   p%verbose = 1
