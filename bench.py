@@ -377,6 +377,10 @@ def main():
     ap.add_argument("--no-batched", action="store_true",
                     help="skip the N=20 R=16 batched-form flows (grouped vs per-channel path)")
     ap.add_argument("--no-autotune", action="store_true", help="keep the default 64x64 GEMM tiles")
+    ap.add_argument("--save-shapes", default=None, help="write the autotuned GEMM tile shapes to this JSON file")
+    ap.add_argument("--load-shapes", default=None,
+                    help="set the GEMM tile shapes from a file --save-shapes wrote instead of autotuning "
+                         "(a profiled run then launches the tiles of the run that saved them)")
     ap.add_argument("--calibrate", action="store_true",
                     help="after the timed steps, run add_arrays (k_axpby: reads 16, writes 8 bytes per "
                          "element of the operator) 3 times, the reference copy for HBM counter scaling")
@@ -403,9 +407,16 @@ def main():
 
     ctx = qd.Context(a.nbody, a.nshell)
     tune = None
-    if not a.no_autotune:
+    if a.load_shapes:
+        with open(a.load_shapes) as f:
+            ctx.set_gemm_shapes(json.load(f)["shapes"])
+        progress(f"GEMM tile shapes from {a.load_shapes}")
+    elif not a.no_autotune:
         tune = ctx.autotune(reps=10)  # tile shapes per size bucket (setup, not timed)
         progress(f"autotune: GEMM ms {tune[0]:.3f} -> {tune[1]:.3f}")
+    if a.save_shapes:
+        with open(a.save_shapes, "w") as f:
+            json.dump({"nbody": a.nbody, "nshell": a.nshell, "shapes": ctx.gemm_shapes().tolist()}, f)
     X, Y, Z = ctx.operator(1, x), ctx.operator(1, y), ctx.operator(-1)
     HCC, ECC = ctx.cc(1.0), ctx.cc(1.0)
 
@@ -443,6 +454,18 @@ def main():
 
     gf = ctx.gemm_flops()
     per_launch = {CLASS_NAMES[i]: (ms_cls[i] / n_cls[i] if n_cls[i] else 0.0) for i in range(NCLS)}
+    # matrix-core work: executed MFMA flops (tile padding included) against
+    # the algorithmic flops per list, and the class's MFMA rate (the launches
+    # timed alone, with the timing events on)
+    ex = ctx.gemm_mfma_flops()
+    mfma = {k: {"algorithmic_gflop": gf[k] / 1e9, "mfma_gflop": ex[k] / 1e9, "padding": ex[k] / gf[k]}
+            for k in ("products", "pandya", "accumulate") if gf[k] > 0}
+    for cls, keys in (("gemm_products", ("products", "pandya")), ("gemm_accumulate", ("accumulate",))):
+        t = per_launch[cls] * 1e-3
+        if t > 0:
+            e = sum(ex[k] for k in keys)
+            mfma[cls] = {"ms": t * 1e3, "mfma_tflops": e / t / 1e12, "mfma_frac_of_peak": e / t / PEAK_FP64_MFMA,
+                         "useful_tflops": sum(gf[k] for k in keys) / t / 1e12}
     # dominant kernel class by device time
     dom = int(np.argmax(ms_cls))
     n2b = ch.n2b()
@@ -484,6 +507,7 @@ def main():
         "kernel_ms_per_step": {CLASS_NAMES[i]: ms_cls[i] / a.steps for i in range(NCLS)},
         "gemm_flops_per_step": gf,
         "roofline": roof,
+        "mfma": mfma,
     }
     out["autotune_gemm_ms_before_after"] = tune
     if a.breakdown:

@@ -360,6 +360,10 @@ int qd_get_precision(qd_context *ctx, int *prec);
 /* Current tile shape of every GEMM problem (products, then accumulating);
  * returns the count. */
 int qd_gemm_shapes(const qd_context *ctx, int *shapes, int64_t len);
+/* Sets every problem's tile shape (len = the count qd_gemm_shapes returns,
+ * values 0..8 as it reports them), e.g. to replay an autotuned set in another
+ * process; drops captured BCH graphs.  No counterpart in the reference. */
+int qd_set_gemm_shapes(qd_context *ctx, const int *shapes, int64_t len);
 
 /* ---- input side (host FP64, no GPU) --------------------------------------
  * v_int(qi,qj,qk,ql,hw), src/ME_general.f90:2205-2276: the analytic 2D
@@ -466,6 +470,14 @@ int qd_timing_read(qd_context *ctx, double *ms_per_class, int64_t *launches_per_
 /* Algorithmic FP64 flops of one full commutator's GEMM launches: the product
  * launch without / with its Pandya part, and the accumulating launch. */
 int qd_gemm_flops(const qd_context *ctx, double *ladder, double *pandya, double *acc);
+/* Matrix-core flops the same launches execute at the current tile shapes and
+ * precision (zero-filled tile edges and K steps included; BF16 modes count
+ * every slice product, FP32 and BF16 full K stages of 32): the product
+ * launch's ladder and one-body problems, its Pandya part (the fused kernel's
+ * tile pairs in the grouped FP64 form), and the accumulating launch.  Against
+ * qd_gemm_flops this is the padding of each list; against the launch time,
+ * its matrix-core rate. */
+int qd_gemm_mfma_flops(const qd_context *ctx, double *products, double *pandya, double *acc);
 
 #ifdef __cplusplus
 }

@@ -113,6 +113,18 @@ class Context:
         self.L.qd_gemm_shapes(self.h, out.ctypes.data_as(_lib.ip), n)
         return out[:n]
 
+    def set_gemm_shapes(self, shapes):
+        """Tile shape of every GEMM problem (as gemm_shapes reports them)."""
+        a = np.ascontiguousarray(shapes, dtype=np.int32)
+        check(self.L.qd_set_gemm_shapes(self.h, a.ctypes.data_as(_lib.ip), int(a.size)))
+
+    def gemm_mfma_flops(self):
+        """Matrix-core flops the GEMM launches of one commutator execute at the
+        current shapes and precision (tile padding included)."""
+        a, b, c = C.c_double(), C.c_double(), C.c_double()
+        check(self.L.qd_gemm_mfma_flops(self.h, C.byref(a), C.byref(b), C.byref(c)))
+        return dict(products=a.value, pandya=b.value, accumulate=c.value)
+
     def close(self):
         if getattr(self, "h", None):
             self.L.qd_context_destroy(self.h)

@@ -117,6 +117,8 @@ SIGNATURES = {
     "qd_basis_expand": (C.c_int, [vp, dp, vp]),
     "qd_get_precision": (C.c_int, [vp, ip]),
     "qd_gemm_shapes": (C.c_int, [vp, ip, C.c_int64]),
+    "qd_set_gemm_shapes": (C.c_int, [vp, ip, C.c_int64]),
+    "qd_gemm_mfma_flops": (C.c_int, [vp, dp, dp, dp]),
     "qd_v_int": (C.c_double, [ip, ip, ip, ip, C.c_double]),
     "qd_hf_hamiltonian": (C.c_int, [C.c_int, C.c_int, C.c_double, dp, C.c_int64, dp, ip]),
     "qd_hf_hamiltonian_me": (C.c_int, [C.c_int, C.c_int, C.c_double, C.c_char_p, dp, C.c_int64, dp, ip]),

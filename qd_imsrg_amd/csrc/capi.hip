@@ -2199,6 +2199,67 @@ int qd_gemm_shapes(const qd_context *c, int *shapes, int64_t len) {
   return (int)n;
 }
 
+int qd_set_gemm_shapes(qd_context *c, const int *shapes, int64_t len) {
+  if (!c || !shapes) return fail(QD_E_ARG, "null argument");
+  const int64_t n = (int64_t)(c->g_all.hprobs.size() + c->g_acc.hprobs.size());
+  if (len != n) return fail(QD_E_ARG, "%lld shapes given, the lists hold %lld problems", (long long)len, (long long)n);
+  for (int64_t k = 0; k < n; k++)
+    if (shapes[k] < 0 || shapes[k] >= NSHAPE) return fail(QD_E_ARG, "shape %d of problem %lld", shapes[k], (long long)k);
+  int rc;
+  if ((rc = drop_graphs(c))) return rc;
+  int64_t k = 0;
+  for (auto &p : c->g_all.hprobs) p.shape = shapes[k++];
+  for (auto &p : c->g_acc.hprobs) p.shape = shapes[k++];
+  if ((rc = upload_tiles(c, c->g_all)) || (rc = upload_tiles(c, c->g_acc))) return rc;
+  return QD_OK;
+}
+
+// Matrix-core flops one commutator's GEMM launches execute at the current
+// shapes and precision: every tile covers TM x TN outputs whatever the
+// problem's M, N (edge tiles zero-filled); FP64 tiles walk each segment's K
+// in MFMA k-steps of 4 (the zero fill past K skipped), the fused Pandya
+// kernel likewise over 64 x 64 tile pairs (one tile on the diagonal); FP32
+// tiles run K stages of 32 in full, BF16 tiles S(S+1)/2 slice products per
+// stage of 32.
+int qd_gemm_mfma_flops(const qd_context *c, double *products, double *pandya, double *acc) {
+  if (!c) return fail(QD_E_ARG, "null context");
+  const int prec = c->gemm_prec;
+  const double slice = prec >= GEMM_BF16X2 ? prec * (prec + 1) / 2.0 : 1.0;
+  auto list = [&](const GemmList &g, int p0, int p1) {
+    double f = 0;
+    for (int pi = p0; pi < p1; pi++) {
+      const GemmProb &p = g.hprobs[pi];
+      const int TM = SHAPE_TM[p.shape], TN = SHAPE_TN[p.shape];
+      const double tiles = (double)((p.M + TM - 1) / TM) * ((p.N + TN - 1) / TN);
+      for (int s = 0; s < p.nseg; s++) {
+        const int K = g.hsegs[p.seg0 + s].K;
+        const double kk = prec == GEMM_FP64 ? 4.0 * ((K + 3) / 4) : 32.0 * ((K + 31) / 32);
+        f += tiles * 2.0 * TM * TN * kk * slice;
+      }
+    }
+    return f;
+  };
+  const bool fused = c->launch_mode == 0 && prec == GEMM_FP64;
+  const int npre = c->g_all.npre, nall = (int)c->g_all.hprobs.size();
+  if (products) *products = list(c->g_all, 0, npre);
+  if (acc) *acc = list(c->g_acc, 0, (int)c->g_acc.hprobs.size());
+  if (pandya) {
+    if (fused) {
+      double f = 0;
+      for (int q = 0; q < c->ch.ncc; q++) {
+        const int r = c->ch.cc_r[q], K = c->ch.cc_nb[q];
+        if (r * K == 0) continue;
+        const double nt = (r + 63) / 64, pairs = nt * (nt + 1) / 2;
+        f += (2 * pairs - nt) * 2.0 * 64 * 64 * 4.0 * ((K + 3) / 4);  // two tiles per pair, one on the diagonal
+      }
+      *pandya = f;
+    } else {
+      *pandya = list(c->g_all, npre, nall);
+    }
+  }
+  return QD_OK;
+}
+
 int qd_gemm_flops(const qd_context *c, double *ladder, double *pandya, double *acc) {
   if (!c) return fail(QD_E_ARG, "null context");
   if (ladder) *ladder = c->g_all.flops_pre;

@@ -86,3 +86,44 @@ def test_magnus_flow_with_graphs_matches_plain():
     assert (r1.iterations, r1.bch_terms) == (r2.iterations, r2.bch_terms)
     assert np.array_equal(t1, t2) and np.array_equal(HS2.download(), hs1)
     ctx.close()
+
+
+def test_gemm_shapes_replay_and_mfma_flops(case):
+    """Autotuned shapes set on a fresh context give the tuned context's result
+    bit for bit; the matrix-core flop count of the accumulating list at 64 x 64
+    tiles equals the count from the block dimensions (every tile 64 x 64, K in
+    steps of 4, the problems of matrix_mult222's Vpphh / Vphhh / Vppph sums);
+    executed >= algorithmic in every list; bad shapes are rejected."""
+    ch, o, ctx = case
+    rng = np.random.default_rng(33)
+    x, y = random_flat(ch, rng, -1), random_flat(ch, rng, 1)
+    shapes = ctx.gemm_shapes()
+    tuned = full(ctx, x, y)
+    fresh = qd.Context(20, 14)
+    try:
+        fresh.set_gemm_shapes(shapes)
+        assert np.array_equal(fresh.gemm_shapes(), shapes)
+        assert np.array_equal(full(fresh, x, y), tuned)
+        fresh.set_gemm_shapes(np.zeros_like(shapes))
+        ex, alg = fresh.gemm_mfma_flops(), fresh.gemm_flops()
+        for k in ("products", "pandya", "accumulate"):
+            assert ex[k] >= alg[k] > 0
+        cd = lambda a, b: -(-a // b)
+        k4 = lambda k: 4 * cd(k, 4)
+        want = 0.0
+        for q in range(ch.nblock):
+            h, b, p = int(ch.nh[q]), int(ch.nb[q]), int(ch.np_[q])
+            for M, N, ks in ((p, h, (p, h)), (p, h, (p, h)), (b, h, (h, h)), (p, b, (p,)), (p, b, (p,))):
+                if M * N:
+                    want += cd(M, 64) * cd(N, 64) * 2.0 * 64 * 64 * sum(k4(k) for k in ks)
+        assert ex["accumulate"] == want
+        print(f"64x64 tiles: executed / algorithmic = " +
+              ", ".join(f"{k} {ex[k] / alg[k]:.2f}" for k in ex))
+        with pytest.raises(qd.QDError):
+            fresh.set_gemm_shapes(shapes[:-1])
+        bad = shapes.copy()
+        bad[0] = 99
+        with pytest.raises(qd.QDError):
+            fresh.set_gemm_shapes(bad)
+    finally:
+        fresh.close()
