@@ -57,7 +57,7 @@ __global__ void __launch_bounds__(BX_THREADS) k_box2b(DevTab t, Ops o, double *_
                                                int flags, const double *__restrict__ w1,
                                                const double *__restrict__ w2, const double *__restrict__ w3,
                                                const double *__restrict__ L, const double *__restrict__ F,
-                                               const double *__restrict__ D) {
+                                               const double *__restrict__ D, double dsign) {
   __shared__ double2 gg[BX_VCAP];              // ordered grids of (V1, V2): [row][ldc]
   __shared__ double2 fl[BX_FCAP / 2];          // channel blocks of (f1, f2): A3, A4, then A1, A2 per box
   __shared__ int pre[BX_GMAX + 1];             // element offsets of the boxes
@@ -67,8 +67,9 @@ __global__ void __launch_bounds__(BX_THREADS) k_box2b(DevTab t, Ops o, double *_
   __shared__ uint32_t cent[BX_CMAX];           // grid entries of the task's columns
   __shared__ int64_t soff[3][3];               // stored sub-block of (row type, column type), row >= column
   __shared__ float2 brc[BX_GMAX];              // 1 / rows in the task, 1 / |A2|
-  __shared__ int64_t pb[BX_GMAX][4];           // Pandya: D offset of each term's sub-block (-1: no D block)
-  __shared__ int pl[BX_GMAX][4];               // Pandya: leading dimension of each term's D block
+  __shared__ int64_t pb[BX_GMAX][4];           // Pandya: offset of each term's D block (-1: no D block)
+  __shared__ int pl[BX_GMAX][4];               // Pandya: leading dimension (rows) of each term's D block
+  __shared__ int2 prc[BX_GMAX][4];             // Pandya: first row and column of each term's sub-block
   __shared__ int2 bfo[BX_GMAX];                // offsets of the box's A1 and A2 blocks in F
   __shared__ int fsz_s;                        // f entries staged (A3, A4 and every box's A1, A2)
   __shared__ int bgo[BX_GMAX];                 // first row grid entry of the box in cp_grid
@@ -148,9 +149,8 @@ __global__ void __launch_bounds__(BX_THREADS) k_box2b(DevTab t, Ops o, double *_
       const int x = t.ch_states[t.ch_start[rc0[k]]], y = t.ch_states[t.ch_start[rc1[k]]];
       const int K = t.cc_map[x * t.M + y].x;
       pl[b][k] = t.cc_r[K];
-      pb[b][k] = t.cc_nb[K] == 0 ? -1
-                                 : t.cc_poff[K] + t.cc_rs[K * t.nch + rc0[k]] +
-                                       (int64_t)t.cc_rs[K * t.nch + cc0[k]] * t.cc_r[K];
+      pb[b][k] = t.cc_nb[K] == 0 ? -1 : t.cc_poff[K];
+      prc[b][k] = make_int2(t.cc_rs[K * t.nch + rc0[k]], t.cc_rs[K * t.nch + cc0[k]]);
     }
   }
   __syncthreads();
@@ -272,11 +272,25 @@ __global__ void __launch_bounds__(BX_THREADS) k_box2b(DevTab t, Ops o, double *_
     // them.  A term without a D block reads the zero at D[p_size].
     double d0 = 0.0, d1 = 0.0, d2 = 0.0, d3 = 0.0;
     if (dopd) {
-      const int64_t b0 = pb[b][0], b1 = pb[b][1], b2 = pb[b][2], b3 = pb[b][3], z = t.p_size;
-      d0 = D[b0 >= 0 ? b0 + i3 * s1 + i1 + (int64_t)(i2 * s4 + i4) * pl[b][0] : z];
-      d1 = D[b1 >= 0 ? b1 + i3 * s2 + i2 + (int64_t)(i1 * s4 + i4) * pl[b][1] : z];
-      d2 = D[b2 >= 0 ? b2 + i4 * s2 + i2 + (int64_t)(i1 * s3 + i3) * pl[b][2] : z];
-      d3 = D[b3 >= 0 ? b3 + i4 * s1 + i1 + (int64_t)(i2 * s3 + i3) * pl[b][3] : z];
+      // term k reads D(row, col) of its block: row and column within the
+      // block from the term's sub-block corner; with D kept as its upper
+      // 64 x 64 tiles a lower element comes from (col, row) times dsign
+      const int64_t z = t.p_size;
+      const bool tri = dsign != 0.0;
+      auto dget = [&](int k, int row, int col, double &sg) -> int64_t {
+        const int64_t base = pb[b][k];
+        const int r = row + prc[b][k].x, c = col + prc[b][k].y;
+        const bool lo = tri && (r >> 6) > (c >> 6);
+        sg = lo ? dsign : 1.0;
+        return base < 0 ? z : base + (lo ? c + (int64_t)r * pl[b][k] : r + (int64_t)c * pl[b][k]);
+      };
+      double g0, g1, g2, g3;
+      const int64_t a0 = dget(0, i3 * s1 + i1, i2 * s4 + i4, g0), a1 = dget(1, i3 * s2 + i2, i1 * s4 + i4, g1),
+                    a2 = dget(2, i4 * s2 + i2, i1 * s3 + i3, g2), a3 = dget(3, i4 * s1 + i1, i2 * s3 + i3, g3);
+      d0 = g0 * D[a0];
+      d1 = g1 * D[a1];
+      d2 = g2 * D[a2];
+      d3 = g3 * D[a3];
     }
     // ladder inputs (signed combinations of the products of matrix_mult222,
     // src/commutators.f90:999-1127, and the summed products for Vpphh,
@@ -354,9 +368,10 @@ __global__ void __launch_bounds__(BX_THREADS) k_box2b(DevTab t, Ops o, double *_
 
 void launch_box2b(hipStream_t s, const DevTab &t, const Ops &o, double *out, const BTask *tasks, int ntasks,
                   const int *blist, int flags, const double *w1, const double *w2, const double *w3,
-                  const double *L, const double *F, const double *D) {
+                  const double *L, const double *F, const double *D, double dsign) {
   if (ntasks <= 0) return;
-  hipLaunchKernelGGL(k_box2b, dim3(ntasks), dim3(BX_THREADS), 0, s, t, o, out, tasks, blist, flags, w1, w2, w3, L, F, D);
+  hipLaunchKernelGGL(k_box2b, dim3(ntasks), dim3(BX_THREADS), 0, s, t, o, out, tasks, blist, flags, w1, w2, w3, L, F, D,
+                     dsign);
 }
 
 __global__ void __launch_bounds__(256) k_sum_halves(const double *__restrict__ w3, const double *__restrict__ w4,

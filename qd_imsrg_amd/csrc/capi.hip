@@ -85,6 +85,10 @@ struct qd_context {
   hipStream_t stream = nullptr;
   // side stream (default priority) for the one-body and zero-body pieces of a
   // commutator, forked from and joined back to `stream` with these events
+  // layout of the Pandya product in P: 0 = whole D (combine pass), else D
+  // holds its upper 64 x 64 tiles and a lower element is this sign times the
+  // transposed one (fused kernel: -h_H h_eta)
+  double d_sign = 0.0;
   hipStream_t aux = nullptr;
   hipEvent_t ev_fork = nullptr, ev_join = nullptr, ev_cc = nullptr, ev_acc = nullptr;
   std::vector<void *> allocs;
@@ -850,10 +854,12 @@ int run_products(qd_context *c, const qd_op *r1, const qd_op *r2, qd_op *out, co
     if (fused) {
       launch_pandya_fused(c->stream, c->dt, ecc->d, hcc->d, c->P, c->ftp, c->nftp, hcc->herm, ecc->herm);
       HIPC(hipGetLastError());
+      c->d_sign = -hcc->herm * ecc->herm;
     }
   }
   if (pd && !fused) {
     Timer tc(c, TC_OTHER);
+    c->d_sign = 0.0;
     launch_pandya_combine(c->stream, c->dt, c->P, c->ptp, c->nptp, hcc->herm, ecc->herm);
     HIPC(hipGetLastError());
   }
@@ -892,12 +898,12 @@ int run_twobody(qd_context *c, const qd_op *r1, const qd_op *r2, qd_op *out, int
   if ((part & TB_PART_PRE) && (flags & TB_122)) {
     launch_fclass(c->stream, c->dt, o, c->fcls);
     launch_box2b(c->stream, c->dt, o, out->d, c->bt2, c->nbt2, c->blist, flags, c->w1, c->w2, c->w3, c->wl,
-                 c->fcls, nullptr);
+                 c->fcls, nullptr, 0.0);
   }
   if (part & TB_PART_MAIN) {
     if ((flags & TB_LADDER) && !lsq_done) run_ladder_sq(c, r1, r2);
     launch_box2b(c->stream, c->dt, o, out->d, c->bt1, c->nbt1, c->blist, flags, c->w1, c->w2, c->w3, c->wl,
-                 c->fcls, (flags & TB_PANDYA) ? c->P : nullptr);
+                 c->fcls, (flags & TB_PANDYA) ? c->P : nullptr, c->d_sign);
   }
   HIPC(hipGetLastError());
   return QD_OK;
@@ -980,6 +986,7 @@ int full_commutator(qd_context *c, const qd_op *r1, const qd_op *r2, qd_op *r3, 
     if (side) HIPC(hipStreamWaitEvent(main, c->ev_cc, 0));
     launch_pandya_fused(c->stream, c->dt, ecc->d, hcc->d, c->P, c->ftp, c->nftp, hcc->herm, ecc->herm);
     HIPC(hipGetLastError());
+    c->d_sign = -hcc->herm * ecc->herm;
   } else {
     if (side) HIPC(hipStreamWaitEvent(main, c->ev_cc, 0));
     if ((rc = run_products(c, r1, r2, r3, hcc, ecc))) return rc;

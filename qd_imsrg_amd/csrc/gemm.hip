@@ -648,9 +648,10 @@ void launch_gemm(hipStream_t s, const GemmProb *probs, const GemmSeg *segs, cons
 // a pair of 64x64 tiles (ti <= tj) of a block: it computes P_ij = E_i H_j^T
 // and P_ji = E_j H_i^T (K staged through LDS in chunks of 8, the next chunk
 // prefetched into registers, k ascending as in the grouped GEMM, so P is the
-// same to the bit), then writes D_ij = h_H P_ij - h_eta P_ji^T and
-// D_ji = h_H P_ji - h_eta P_ij^T, exchanging the transposed tile through LDS.
-// The product matrix P is never written.
+// same to the bit), then writes D_ij = h_H P_ij - h_eta P_ji^T, the
+// transposed tile exchanged through LDS.  D_ji = -h_H h_eta D_ij^T is not
+// written: D is kept as its upper 64 x 64 tiles (row tile <= column tile),
+// which halves its write, and the product matrix P is never written.
 // Four waves per SIMD (125 registers, 33 KB of LDS for the epilogue tile);
 // chunks of 16 needed 160 registers and ran three.
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) k_pandya_fused(DevTab t, const double *__restrict__ E,
@@ -751,16 +752,13 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) k
             Dq[(r0 + lx) + (int64_t)(c0 + ly) * r] = hh * a[x][y][v] - eh * Tt[ly * LDE + lx];
         }
   };
+  // only the tile (ti, tj), ti <= tj, is written: D^T = -h_H h_eta D, so the
+  // lower tiles are the transposed upper ones times that sign (the two-body
+  // pass reads them there), and half of D's write is saved
   __syncthreads();
   put(diag ? a1 : a2);
   __syncthreads();
   out(a1, i0, j0);
-  if (!diag) {
-    __syncthreads();
-    put(a1);
-    __syncthreads();
-    out(a2, j0, i0);
-  }
 }
 
 void launch_pandya_fused(hipStream_t s, const DevTab &t, const double *etacc, const double *hcc, double *D,

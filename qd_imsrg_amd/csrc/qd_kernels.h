@@ -134,10 +134,12 @@ void launch_fclass(hipStream_t s, const DevTab &t, const Ops &o, double *F);
 // combinations of the products w1..w3 and, for the square sub-blocks and
 // the summed products of Vpphh and Vppph, of L
 // (TB_LADDER; launch_ladder_sq has filled L), Pandya term from D
-// (TB_PANDYA); TB_ACCUM adds to out (mode 2 tasks always add)
+// (TB_PANDYA); TB_ACCUM adds to out (mode 2 tasks always add).  dsign != 0:
+// D holds only its upper 64 x 64 tiles (launch_pandya_fused), a lower element
+// (x, y) is read as dsign * D(y, x); dsign = 0: D is whole
 void launch_box2b(hipStream_t s, const DevTab &t, const Ops &o, double *out, const BTask *tasks, int ntasks,
                   const int *blist, int flags, const double *w1, const double *w2, const double *w3,
-                  const double *L, const double *F, const double *D);
+                  const double *L, const double *F, const double *D, double dsign);
 // L = w3 + w4 over the given sub-blocks (the two halves of the summed
 // products of Vpphh and Vppph)
 void launch_sum_halves(hipStream_t s, const double *w3, const double *w4, double *L, const Seg2 *segs,
@@ -170,7 +172,8 @@ void launch_white(hipStream_t s, const DevTab &t, const double *rec, int hrec, d
 void launch_pandya_combine(hipStream_t s, const DevTab &t, double *P, const TilePair *tp, int ntp, double hh,
                            double eh);
 // D = hh * E H^T - eh * (E H^T)^T per CC block straight from the CC sets
-// (E = ETACC, H = HCC), over 64x64 tile pairs (ti <= tj)
+// (E = ETACC, H = HCC), over 64x64 tile pairs (ti <= tj); only the upper
+// tiles of D are written (the lower ones are -hh * eh times their transpose)
 void launch_pandya_fused(hipStream_t s, const DevTab &t, const double *etacc, const double *hcc, double *D,
                          const TilePair *tp, int ntp, double hh, double eh);
 void launch_axpby(hipStream_t s, int64_t n, double s1, const double *x, double s2, const double *y, double *z);
