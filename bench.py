@@ -122,8 +122,8 @@ def time_to_egs():
 
 def precision_form(nbody=42, nshell=20, hw=0.28, modes=("fp64", "bf16x3", "fp32", "bf16x2", "bf16x4")):
     """Precision form at N=42, R=20, hw=0.28 (HF basis, Magnus, White): the
-    same flow with the 2b-2b GEMMs in each arithmetic, grouped + autotuned +
-    graphs throughout.  Per mode: time to E_gs, |dE_gs|, relative error of
+    same flow with the 2b-2b GEMMs in each arithmetic, grouped + graphs
+    throughout, tile shapes autotuned for each arithmetic.  Per mode: time to E_gs, |dE_gs|, relative error of
     ||Omega|| and of ||Gamma_od|| against the FP64 GPU flow, and the device
     rate of one full commutator [Omega, H(s)] (FP64-equivalent reference
     flops / time)."""
@@ -144,6 +144,9 @@ def precision_form(nbody=42, nshell=20, hw=0.28, modes=("fp64", "bf16x3", "fp32"
     res = {}
     for p in modes:
         ctx.set_precision(p)
+        if p != "fp64":  # tile shapes tuned for this arithmetic
+            t2 = ctx.autotune(reps=1)
+            progress(f"autotune {p}: GEMM ms {t2[0]:.2f} -> {t2[1]:.2f}")
         HS, OM = ctx.operator(1), ctx.operator(-1)
         ctx.sync()
         t0 = time.perf_counter()
@@ -211,6 +214,8 @@ def batched_flow(nbody=20, nshell=16, hw=0.5):
                 tune = ctx.autotune(reps=2)
         ctx.set_graphs(graphs)
         ctx.set_precision(prec)
+        if prec != "fp64":
+            tune = ctx.autotune(reps=2)
         H0, HS, OM = ctx.operator(1, h0), ctx.operator(1), ctx.operator(-1)
         ctx.sync()
         t0 = time.perf_counter()
