@@ -169,7 +169,14 @@ int qd_commutator(qd_context *ctx, const qd_op *r1, const qd_op *r2, qd_op *r3,
  * them: the sequence of dGam (src/gs_IMSRG.f90:256-268) in one call.  Both CC
  * sets are overwritten (their herm fields are kept).  In the FP64 grouped form
  * the two calc_cc launches overlap the ladder products, which do not read them.
- * Equal to qd_calc_cc twice and qd_commutator. */
+ * Equal to qd_calc_cc twice and qd_commutator to rounding (1e-14): when the CC
+ * herm fields equal the operators' flags, the two-body pass takes <J|Z|I> as
+ * h_Z <I|Z|J> (h_Z = -h1 h2) for its 1b x 2b and Pandya parts, which assumes
+ * operators that obey their herm flags (square sub-blocks and fhh, fpp
+ * (anti)symmetric), as every full_ham of the reference does.  qd_dgam,
+ * qd_bch_expand and every call without the Pandya term do the same;
+ * qd_commutator and qd_commutator_222 with caller-supplied CC sets evaluate
+ * each stored element on its own. */
 int qd_commutator_cc(qd_context *ctx, const qd_op *r1, const qd_op *r2, qd_op *r3,
                      qd_cc *hcc, qd_cc *etacc, double *e0);
 /* BCH_EXPAND(HS,G,H,...), src/magnus_IMSRG.f90:253-318: hs = sum_k cof(k)

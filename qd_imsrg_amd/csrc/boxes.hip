@@ -73,6 +73,7 @@ __global__ void __launch_bounds__(BX_THREADS) k_box2b(DevTab t, Ops o, double *_
   __shared__ int2 bfo[BX_GMAX];                // offsets of the box's A1 and A2 blocks in F
   __shared__ int fsz_s;                        // f entries staged (A3, A4 and every box's A1, A2)
   __shared__ int bgo[BX_GMAX];                 // first row grid entry of the box in cp_grid
+  __shared__ int bpr[BX_GMAX];                 // the box also writes the transposed box's outputs
   __shared__ int rtot_s;                       // row grid entries of the task
   const BTask tk = tasks[blockIdx.x];
   if (tk.block < 0) return;
@@ -99,8 +100,11 @@ __global__ void __launch_bounds__(BX_THREADS) k_box2b(DevTab t, Ops o, double *_
     static_assert(BX_GMAX <= 64, "one lane of wave 0 per box");
     const int b = threadIdx.x;
     int g = 0, A1 = 0, A2 = 0, s1 = 1, s2 = 1, f1 = 0, f2 = 0, go = 0;
+    int pr = 0;
     if (b < ng) {
       g = blist[tk.first + b];
+      pr = (g & BOX_PAIRED) != 0;
+      g &= ~BOX_PAIRED;
       A1 = t.cp_a[g];
       A2 = t.cp_b[g];
       go = t.cp_grid_off[g];
@@ -130,6 +134,7 @@ __global__ void __launch_bounds__(BX_THREADS) k_box2b(DevTab t, Ops o, double *_
       pre[b] = e - nr * nc;
       bfo[b] = make_int2(f1, f2);
       bgo[b] = go + r_off;
+      bpr[b] = pr;
     }
     if (b == ng - 1) {
       pre[ng] = e;
@@ -142,7 +147,7 @@ __global__ void __launch_bounds__(BX_THREADS) k_box2b(DevTab t, Ops o, double *_
     // by the channel of the first state, so each term's box is a dense
     // sub-block at a fixed offset
     const int b = threadIdx.x - 64;
-    const int g = blist[tk.first + b];
+    const int g = blist[tk.first + b] & ~BOX_PAIRED;
     const int A1 = t.cp_a[g], A2 = t.cp_b[g];
     const int rc0[4] = {A3, A3, A4, A4}, rc1[4] = {A1, A2, A2, A1}, cc0[4] = {A2, A1, A1, A2};
     for (int k = 0; k < 4; k++) {
@@ -240,9 +245,13 @@ __global__ void __launch_bounds__(BX_THREADS) k_box2b(DevTab t, Ops o, double *_
     }
     __syncthreads();
   }
-  const double h1 = o.h1, h2 = o.h2;
+  const double h1 = o.h1, h2 = o.h2, hz = -h1 * h2;  // hz: herm of [r1, r2]
   const float rs4 = 1.0f / (float)s4;
   const bool same34 = A3 == A4;
+  // stored sub-block of (row type, column type), row type >= column type
+  auto sub_of = [](int ta, int tb) {
+    return tb == 0 ? (ta == 0 ? HHHH : (ta == 1 ? PHHH : PPHH)) : (tb == 1 ? (ta == 1 ? PHPH : PPPH) : PPPP);
+  };
   for (int e = threadIdx.x, b = 0; e < total; e += BX_THREADS) {
     if (b + 1 < ng && pre[b + 1] <= e) b = box_of(pre, b + 1, ng, e);
     const int4 B = bx[b];
@@ -253,18 +262,19 @@ __global__ void __launch_bounds__(BX_THREADS) k_box2b(DevTab t, Ops o, double *_
     const int tyr = (int)((er >> 18) & 3u), tyc = (int)((ec >> 18) & 3u);
     const bool sr = (er >> 16) & 1u, sc = (ec >> 16) & 1u;
     const int4 S = bs[b];
-    // stored: row type >= column type; one orientation of a same-channel pair
-    if (tyr < tyc || (S.z && sr) || (same34 && sc)) continue;
+    // one orientation of a same-channel pair
+    if ((S.z && sr) || (same34 && sc)) continue;
+    // stored: <I|Z|J> with row type >= column type; in a paired box also
+    // <J|Z|I> (column type >= row type)
+    const bool st1 = tyr >= tyc, st2 = bpr[b] && tyc >= tyr;
+    if (!st1 && !st2) continue;
     const int s1 = S.x, s2 = S.y;
     const int i1 = qdiv(rr, brc[b].y), i2 = rr - i1 * s2, i3 = qdiv(cc, rs4), i4 = cc - i3 * s4;
-    // stored position (sub-block of the flat layout)
+    // stored positions (sub-blocks of the flat layout); 0 where not stored
     const int Il = (int)(er & 0xffffu) - tstart[tyr], Jl = (int)(ec & 0xffffu) - tstart[tyc];
-    const int rows = tcount[tyr];
-    int sub;
-    if (tyc == 0) sub = tyr == 0 ? HHHH : (tyr == 1 ? PHHH : PPHH);
-    else if (tyc == 1) sub = tyr == 1 ? PHPH : PPPH;
-    else sub = PPPP;
-    const int64_t idx = soff[tyr][tyc] + Il + (int64_t)Jl * rows;
+    const int sub1 = st1 ? sub_of(tyr, tyc) : HHHH, sub2 = st2 ? sub_of(tyc, tyr) : HHHH;
+    const int64_t idx1 = st1 ? soff[tyr][tyc] + Il + (int64_t)Jl * tcount[tyr] : 0;
+    const int64_t idx2 = st2 ? soff[tyc][tyr] + Jl + (int64_t)Il * tcount[tyc] : 0;
     // global loads first, so that their latency overlaps the LDS contraction.
     // Every load is issued unconditionally and nothing reads its value before
     // the combination: loads under divergent branches share registers across
@@ -298,19 +308,24 @@ __global__ void __launch_bounds__(BX_THREADS) k_box2b(DevTab t, Ops o, double *_
     // PPHH and PPPH, whose sums come in two halves; w3 for PHHH), xc =
     // w2(I,J) for PHHH / PPPH; the square sub-blocks read their whole ladder
     // term from L.  A value the sub-block does not use is read at element 0
-    // and ignored.
+    // and ignored.  One set per stored output.
     const bool lad = first && (flags & TB_LADDER);
-    const bool sq = sub == HHHH || sub == PPPP || sub == PHPH;
-    const bool usec = sub == PHHH || sub == PPPH;
-    double xa = 0.0, xb = 0.0, xc = 0.0;
+    double xa1 = 0.0, xb1 = 0.0, xc1 = 0.0, xa2 = 0.0, xb2 = 0.0, xc2 = 0.0;
     if (lad) {
-      xa = (sq ? L : w1)[sub == PPHH ? 0 : idx];
-      xb = (sub == PHHH ? w3 : L)[sq ? 0 : idx];
-      xc = w2[usec ? idx : 0];
+      auto ld = [&](int sub, int64_t idx, double &xa, double &xb, double &xc) {
+        const bool sq = sub == HHHH || sub == PPPP || sub == PHPH;
+        const bool usec = sub == PHHH || sub == PPPH;
+        xa = (sq ? L : w1)[sub == PPHH ? 0 : idx];
+        xb = (sub == PHHH ? w3 : L)[sq ? 0 : idx];
+        xc = w2[usec ? idx : 0];
+      };
+      ld(sub1, idx1, xa1, xb1, xc1);
+      ld(sub2, idx2, xa2, xb2, xc2);
     }
     // the bra slabs of a split box add to what the ket slabs wrote
     const bool rd = first ? ((flags & TB_ACCUM) || (mode == 1 && do122)) : (flags & TB_ACCUM) != 0;
-    const double o0 = rd ? out[idx] : 0.0;
+    const double o1 = rd ? out[idx1] : 0.0;
+    const double o2 = rd ? out[idx2] : 0.0;
     double val = 0.0;
     if (do122) {
       const double2 *G = gg + B.z;
@@ -346,23 +361,25 @@ __global__ void __launch_bounds__(BX_THREADS) k_box2b(DevTab t, Ops o, double *_
     if (dopd) val += d0 - d1 + d2 - d3;
     if (sr != sc) val = -val;
     if (!first) {  // ket slabs run before the main pass: the first writer of their outputs
-      out[idx] = o0 + val;
+      out[idx1] = o1 + val;
       continue;
     }
-    double acc = o0;
-    acc += val;
-    if (lad) {
-      switch (sub) {
-        case HHHH:
-        case PPPP:
-        case PHPH: acc += xa; break;
-        case PPHH: acc += xb; break;
-        case PHHH: acc += h1 * xc - h2 * xa + xb; break;
-        case PPPH: acc += -h2 * xa + h1 * xc + xb; break;
-        default: break;
+    auto finish = [&](int sub, double acc, double xa, double xb, double xc) {
+      if (lad) {
+        switch (sub) {
+          case HHHH:
+          case PPPP:
+          case PHPH: acc += xa; break;
+          case PPHH: acc += xb; break;
+          case PHHH: acc += h1 * xc - h2 * xa + xb; break;
+          case PPPH: acc += -h2 * xa + h1 * xc + xb; break;
+          default: break;
+        }
       }
-    }
-    out[idx] = acc;
+      return acc;
+    };
+    if (st1) out[idx1] = finish(sub1, o1 + val, xa1, xb1, xc1);
+    if (st2) out[idx2] = finish(sub2, o2 + hz * val, xa2, xb2, xc2);
   }
 }
 

@@ -96,7 +96,8 @@ struct qd_context {
   double *u1 = nullptr, *u2 = nullptr;  // cross-coupled mat-vecs of commutator_121 (M*M each)
   double *partials = nullptr, *scal = nullptr;  // reduction partials, 8 device scalars
   double *host_scal = nullptr;                  // pinned mirror of scal
-  BTask *bt1 = nullptr, *bt2 = nullptr; int nbt1 = 0, nbt2 = 0;      // class-pair box passes
+  BTask *bt1 = nullptr, *bt2 = nullptr, *bt1p = nullptr;              // class-pair box passes
+  int nbt1 = 0, nbt2 = 0, nbt1p = 0;
   int *blist = nullptr;                                              // class pairs of the box tasks
   int box_cap = BX_VCAP;                                             // grid elements per box task
   double *fcls = nullptr;                                            // channel blocks of f (2 operands)
@@ -466,11 +467,25 @@ int build_worklists(qd_context *c) {
   // row type >= column type somewhere); a box too large for the LDS grids is
   // split into column slabs (bra terms, first launch) and row slabs (ket
   // terms, second launch).  Blocks go to XCDs as for the two-body tasks.
-  std::vector<std::vector<BTask>> bt1(ch.nblock), bt2(ch.nblock);
+  //
+  // Paired form (bt1p): for class pairs a < b of a block whose boxes fit the
+  // LDS grid both ways, the box (rows a, columns b) also writes the outputs of
+  // the box (rows b, columns a): <J|Z|I> = h_Z <I|Z|J> for the 1b x 2b and
+  // Pandya parts, so one staged grid and one contraction serve both.  Its
+  // blist entry carries BOX_PAIRED; the box (b, a) is left out.
+  std::vector<std::vector<BTask>> bt1(ch.nblock), bt2(ch.nblock), bt1p(ch.nblock);
   std::vector<int> blist;
   {
     const int cap = c->box_cap;
     auto csize = [&](int h) { return ch.ch_start[h + 1] - ch.ch_start[h]; };
+    auto whole_box = [&](int gr, int gc) {  // rows gr x columns gc fits one task's grid
+      const int nr = csize(ch.cp_a[gr]) * csize(ch.cp_b[gr]), nc = csize(ch.cp_a[gc]) * csize(ch.cp_b[gc]);
+      return nr * (nc | 1) <= cap;
+    };
+    auto pairable = [&](int x, int y) {  // x < y
+      return whole_box(x, y) && whole_box(y, x) &&
+             (ch.cp_tmax[x] >= ch.cp_tmin[y] || ch.cp_tmax[y] >= ch.cp_tmin[x]);
+    };
     for (int q = 0; q < ch.nblock; q++)
       for (int gc = ch.cp_start[q]; gc < ch.cp_start[q + 1]; gc++) {
         const int s3 = csize(ch.cp_a[gc]), s4 = csize(ch.cp_b[gc]);
@@ -478,20 +493,23 @@ int build_worklists(qd_context *c) {
         // whole boxes go to tasks first-fit decreasing (by grid size) under
         // the LDS limits: fuller tasks than bundling in class-pair order
         struct Bin { std::vector<int> g; int goff, foff, roff; };
-        std::vector<Bin> bins;
-        std::vector<std::pair<int, int>> whole;  // (grid size, row class pair)
+        std::vector<std::pair<int, int>> whole, wholep;  // (grid size, row class pair [| BOX_PAIRED])
         for (int gr = ch.cp_start[q]; gr < ch.cp_start[q + 1]; gr++) {
-          if (ch.cp_tmax[gr] < ch.cp_tmin[gc]) continue;
           const int s1 = csize(ch.cp_a[gr]), s2 = csize(ch.cp_b[gr]);
           const int nr = s1 * s2, fneed = 2 * (s1 * s1 + s2 * s2);
           if (fcol + fneed > BX_FCAP || 3 * nr > cap || ldc > cap || nc > BX_CMAX || nr > BX_RCAP)
             return fail(QD_E_LAYOUT, "single-particle channel of %d states too large for the two-body boxes",
                         std::max(std::max(s1, s2), std::max(s3, s4)));
+          const bool pair_here = gr < gc && pairable(gr, gc), pair_there = gr > gc && pairable(gc, gr);
+          if (pair_here) wholep.push_back({nr * ldc, gr | BOX_PAIRED});
+          if (ch.cp_tmax[gr] < ch.cp_tmin[gc]) continue;
           if (nr * ldc > cap) {
             int w = cap / nr;                   // column slab: nr * (w | 1) <= cap
             if ((w | 1) > cap / nr) w--;
             for (int lo = 0; lo < nc; lo += w) {
-              bt1[q].push_back(BTask{q, gc, (int)blist.size(), 1, 1, lo, std::min(w, nc - lo)});
+              const BTask t{q, gc, (int)blist.size(), 1, 1, lo, std::min(w, nc - lo)};
+              bt1[q].push_back(t);
+              bt1p[q].push_back(t);
               blist.push_back(gr);
             }
             const int h = cap / ldc;            // row slab: h * ldc <= cap
@@ -502,31 +520,37 @@ int build_worklists(qd_context *c) {
             continue;
           }
           whole.push_back({nr * ldc, gr});
+          if (!pair_here && !pair_there) wholep.push_back({nr * ldc, gr});
         }
-        std::stable_sort(whole.begin(), whole.end(), [](auto &x, auto &y) { return x.first > y.first; });
-        for (auto &w : whole) {
-          const int gr = w.second;
-          const int s1 = csize(ch.cp_a[gr]), s2 = csize(ch.cp_b[gr]);
-          const int nr = s1 * s2, fneed = 2 * (s1 * s1 + s2 * s2);
-          Bin *fit = nullptr;
-          for (auto &bn : bins)
-            if (bn.goff + nr * ldc <= cap && bn.foff + fneed <= BX_FCAP && bn.roff + nr <= BX_RCAP &&
-                (int)bn.g.size() < BX_GMAX) {
-              fit = &bn;
-              break;
+        for (int form = 0; form < 2; form++) {
+          auto &wl = form == 0 ? whole : wholep;
+          auto &dst = form == 0 ? bt1[q] : bt1p[q];
+          std::vector<Bin> bins;
+          std::stable_sort(wl.begin(), wl.end(), [](auto &x, auto &y) { return x.first > y.first; });
+          for (auto &w : wl) {
+            const int gr = w.second & ~BOX_PAIRED;
+            const int s1 = csize(ch.cp_a[gr]), s2 = csize(ch.cp_b[gr]);
+            const int nr = s1 * s2, fneed = 2 * (s1 * s1 + s2 * s2);
+            Bin *fit = nullptr;
+            for (auto &bn : bins)
+              if (bn.goff + nr * ldc <= cap && bn.foff + fneed <= BX_FCAP && bn.roff + nr <= BX_RCAP &&
+                  (int)bn.g.size() < BX_GMAX) {
+                fit = &bn;
+                break;
+              }
+            if (!fit) {
+              bins.push_back(Bin{{}, 0, fcol, 0});
+              fit = &bins.back();
             }
-          if (!fit) {
-            bins.push_back(Bin{{}, 0, fcol, 0});
-            fit = &bins.back();
+            fit->g.push_back(w.second);
+            fit->goff += nr * ldc;
+            fit->foff += fneed;
+            fit->roff += nr;
           }
-          fit->g.push_back(gr);
-          fit->goff += nr * ldc;
-          fit->foff += fneed;
-          fit->roff += nr;
-        }
-        for (auto &bn : bins) {
-          bt1[q].push_back(BTask{q, gc, (int)blist.size(), (int)bn.g.size(), 0, 0, 0});
-          blist.insert(blist.end(), bn.g.begin(), bn.g.end());
+          for (auto &bn : bins) {
+            dst.push_back(BTask{q, gc, (int)blist.size(), (int)bn.g.size(), 0, 0, 0});
+            blist.insert(blist.end(), bn.g.begin(), bn.g.end());
+          }
         }
       }
   }
@@ -535,8 +559,8 @@ int build_worklists(qd_context *c) {
     for (int q = 0; q < ch.nblock; q++) order[q] = q;
     std::sort(order.begin(), order.end(), [&](int a, int b) { return work[a] > work[b]; });
     std::vector<double> load(NXCD, 0.0);
-    for (int pass = 0; pass < 2; pass++) {
-      auto &src = pass == 0 ? bt1 : bt2;
+    for (int pass = 0; pass < 3; pass++) {
+      auto &src = pass == 0 ? bt1 : pass == 1 ? bt2 : bt1p;
       std::vector<std::vector<BTask>> bx(NXCD);
       std::fill(load.begin(), load.end(), 0.0);
       for (int q : order) {
@@ -551,8 +575,8 @@ int build_worklists(qd_context *c) {
       for (int x = 0; x < NXCD; x++)
         for (size_t k = 0; k < bx[x].size(); k++) bt[k * NXCD + x] = bx[x][k];
       if (bt.empty()) continue;
-      if ((rc = dev_upload(c, bt, pass == 0 ? &c->bt1 : &c->bt2))) return rc;
-      (pass == 0 ? c->nbt1 : c->nbt2) = (int)bt.size();
+      if ((rc = dev_upload(c, bt, pass == 0 ? &c->bt1 : pass == 1 ? &c->bt2 : &c->bt1p))) return rc;
+      (pass == 0 ? c->nbt1 : pass == 1 ? c->nbt2 : c->nbt1p) = (int)bt.size();
     }
     if (blist.empty()) blist.push_back(0);
     if ((rc = dev_upload(c, blist, &c->blist))) return rc;
@@ -889,8 +913,11 @@ constexpr int TB_LSQ_DONE = 16;
 void run_ladder_sq(qd_context *c, const qd_op *r1, const qd_op *r2) {
   launch_ladder_sq(c->stream, c->dt, c->w1, c->w2, c->w3, c->wl, c->lsq, c->nlsq, r1->herm, r2->herm);
 }
+// paired: the main pass runs the paired box tasks (one staged grid writes the
+// outputs of both orientations of a class-pair box); valid when the operators
+// obey their herm flags and any Pandya input was filled by calc_cc of r1, r2
 int run_twobody(qd_context *c, const qd_op *r1, const qd_op *r2, qd_op *out, int flags, const qd_cc *hcc,
-                const qd_cc *ecc, int part = TB_PART_PRE | TB_PART_MAIN) {
+                const qd_cc *ecc, int part = TB_PART_PRE | TB_PART_MAIN, bool paired = false) {
   Ops o{r1->d, r2->d, r1->herm, r2->herm};
   Timer tm(c, TC_TWOBODY);
   const bool lsq_done = (flags & TB_LSQ_DONE) != 0;
@@ -902,8 +929,8 @@ int run_twobody(qd_context *c, const qd_op *r1, const qd_op *r2, qd_op *out, int
   }
   if (part & TB_PART_MAIN) {
     if ((flags & TB_LADDER) && !lsq_done) run_ladder_sq(c, r1, r2);
-    launch_box2b(c->stream, c->dt, o, out->d, c->bt1, c->nbt1, c->blist, flags, c->w1, c->w2, c->w3, c->wl,
-                 c->fcls, (flags & TB_PANDYA) ? c->P : nullptr, c->d_sign);
+    launch_box2b(c->stream, c->dt, o, out->d, paired ? c->bt1p : c->bt1, paired ? c->nbt1p : c->nbt1, c->blist, flags,
+                 c->w1, c->w2, c->w3, c->wl, c->fcls, (flags & TB_PANDYA) ? c->P : nullptr, c->d_sign);
   }
   HIPC(hipGetLastError());
   return QD_OK;
@@ -945,8 +972,9 @@ int read_scalars(qd_context *c, int n, double *out) {
 // region.  With the timing events on, everything runs in that order on one
 // stream so that each class is timed alone.
 enum { FILL_H = 1, FILL_E = 2 };
+// e_from_r1: the caller filled ecc by calc_cc of r1 for this commutator
 int full_commutator(qd_context *c, const qd_op *r1, const qd_op *r2, qd_op *r3, const qd_cc *hcc,
-                    const qd_cc *ecc, int fill = 0) {
+                    const qd_cc *ecc, int fill = 0, bool e_from_r1 = false) {
   int rc = QD_OK;
   qd_cc *fh = (fill & FILL_H) ? const_cast<qd_cc *>(hcc) : nullptr;
   qd_cc *fe = (fill & FILL_E) ? const_cast<qd_cc *>(ecc) : nullptr;
@@ -969,6 +997,10 @@ int full_commutator(qd_context *c, const qd_op *r1, const qd_op *r2, qd_op *r3, 
   const bool pd = pandya_on(hcc, ecc);
   const bool fused = pd && c->launch_mode == 0 && c->gemm_prec == GEMM_FP64;
   const int tbf = TB_122 | TB_LADDER | (pd ? TB_PANDYA : 0);
+  // paired boxes: without the Pandya term, or with CC sets that are calc_cc
+  // of r2 and r1 under the operators' own herm flags
+  const bool paired = !pd || ((fill & FILL_H) && ((fill & FILL_E) || e_from_r1) && hcc->herm == r2->herm &&
+                              ecc->herm == r1->herm);
   if (side) {
     HIPC(hipEventRecord(c->ev_cc, c->aux));
     if (!rc) rc = run_acc(c, r1, r2, r3);
@@ -1015,7 +1047,7 @@ int full_commutator(qd_context *c, const qd_op *r1, const qd_op *r2, qd_op *r3, 
   }
   if (rc) return rc;
   if ((rc = run_twobody(c, r1, r2, r3, side ? tbf | TB_LSQ_DONE : tbf, hcc, ecc,
-                        side ? TB_PART_MAIN : TB_PART_PRE | TB_PART_MAIN)))
+                        side ? TB_PART_MAIN : TB_PART_PRE | TB_PART_MAIN, paired)))
     return rc;
   if (side) HIPC(hipStreamWaitEvent(main, c->ev_join, 0));
   return QD_OK;
@@ -1431,7 +1463,7 @@ int qd_commutator_221(qd_context *c, const qd_op *r1, const qd_op *r2, qd_op *r3
 }
 int qd_commutator_122(qd_context *c, const qd_op *r1, const qd_op *r2, qd_op *r3) {
   CHK3(r1, r2, r3);
-  return run_twobody(c, r1, r2, r3, TB_122, nullptr, nullptr);
+  return run_twobody(c, r1, r2, r3, TB_122, nullptr, nullptr, TB_PART_PRE | TB_PART_MAIN, true);
 }
 int qd_commutator_222(qd_context *c, const qd_op *r1, const qd_op *r2, qd_op *r3, const qd_cc *hcc,
                       const qd_cc *etacc) {
@@ -1440,7 +1472,8 @@ int qd_commutator_222(qd_context *c, const qd_op *r1, const qd_op *r2, qd_op *r3
   const bool pd = pandya_on(hcc, etacc);
   if ((rc = run_products(c, r1, r2, r3, hcc, etacc))) return rc;
   if ((rc = run_acc(c, r1, r2, r3))) return rc;
-  return run_twobody(c, r1, r2, r3, TB_LADDER | TB_ACCUM | (pd ? TB_PANDYA : 0), hcc, etacc);
+  return run_twobody(c, r1, r2, r3, TB_LADDER | TB_ACCUM | (pd ? TB_PANDYA : 0), hcc, etacc,
+                     TB_PART_PRE | TB_PART_MAIN, !pd);
 }
 int qd_calc_cc(qd_context *c, const qd_op *h, qd_cc *cc) {
   if (!c || !h || !cc) return fail(QD_E_ARG, "null argument");
@@ -1597,7 +1630,8 @@ int qd_bch_expand(qd_context *c, qd_op *hs, const qd_op *g, const qd_op *h, cons
     // one term: [G, AD] into next, ||INT1|| (= HS before the update) and ||INT2||
     auto term = [&]() -> int {
       int r;  // with the Pandya term, cca = calc_cc(AD) is filled first
-      if ((r = full_commutator(c, g, &adh, next, pd ? cca : nullptr, pd ? ccg : nullptr, pd ? FILL_H : 0))) return r;
+      if ((r = full_commutator(c, g, &adh, next, pd ? cca : nullptr, pd ? ccg : nullptr, pd ? FILL_H : 0, pd)))
+        return r;
       if ((r = norm_into(c, hs, 0))) return r;
       return norm_into(c, next, 1);
     };

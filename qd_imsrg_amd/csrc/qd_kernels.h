@@ -117,6 +117,9 @@ void launch_gemm(hipStream_t s, const GemmProb *probs, const GemmSeg *segs, cons
 // mode-2 tasks wrote; mode 2: one box, rows lo .. lo+cnt-1, writing the
 // ket-side 1b x 2b terms (their own launch, before the others).
 struct BTask { int block, gcol, first, ng, mode, lo, cnt; };
+// blist entry flag: the box (rows a, columns b) also writes the outputs of the
+// box (rows b, columns a) as h_Z times its own 1b x 2b and Pandya parts
+constexpr int BOX_PAIRED = 1 << 30;
 // 512 threads per task: LDS per work-group stays under 80 KB (two per CU,
 // 16 waves, as many as the kernel's registers allow), so tasks hold up to
 // 3600 grid elements (no box of R = 14 is split); channels of up to 10
@@ -137,6 +140,9 @@ void launch_fclass(hipStream_t s, const DevTab &t, const Ops &o, double *F);
 // (TB_PANDYA); TB_ACCUM adds to out (mode 2 tasks always add).  dsign != 0:
 // D holds only its upper 64 x 64 tiles (launch_pandya_fused), a lower element
 // (x, y) is read as dsign * D(y, x); dsign = 0: D is whole
+// Paired boxes (BOX_PAIRED in blist, the bt1p task list) need operators that
+// obey their herm flags (square sub-blocks and fhh, fpp (anti)symmetric) and,
+// with the Pandya term, D built from calc_cc of those operators.
 void launch_box2b(hipStream_t s, const DevTab &t, const Ops &o, double *out, const BTask *tasks, int ntasks,
                   const int *blist, int flags, const double *w1, const double *w2, const double *w3,
                   const double *L, const double *F, const double *D, double dsign);
