@@ -124,7 +124,11 @@ int qd_commutator_121(qd_context *ctx, const qd_op *r1, const qd_op *r2, qd_op *
 /* commutator_221(r1,r2,r3,w1,w2), src/commutators.f90:198-352; w1/w2 are
  * context workspace */
 int qd_commutator_221(qd_context *ctx, const qd_op *r1, const qd_op *r2, qd_op *r3);
-/* commutator_122(r1,r2,r3), src/commutators.f90:355-676 */
+/* commutator_122(r1,r2,r3), src/commutators.f90:355-676.  The two-body pass
+ * takes <J|Z|I> as h_Z <I|Z|J> (h_Z = -h1 h2) where one staged class-pair box
+ * holds both, which assumes operators that obey their herm flags (square
+ * sub-blocks and fhh, fpp (anti)symmetric); equal to the element-by-element
+ * evaluation to rounding. */
 int qd_commutator_122(qd_context *ctx, const qd_op *r1, const qd_op *r2, qd_op *r3);
 /* commutator_222(r1,r2,r3,w1,HCC,ETACC), src/commutators.f90:679-1132 with
  * construct_cc_ints (src/ME_general.f90:923-952): hcc = CC of r2, etacc = CC
