@@ -100,6 +100,7 @@ struct qd_context {
   int nbt1 = 0, nbt2 = 0, nbt1p = 0;
   int *blist = nullptr;                                              // class pairs of the box tasks
   int box_cap = BX_VCAP;                                             // grid elements per box task
+  bool box_pairs = true;                                             // paired box tasks where valid
   double *fcls = nullptr;                                            // channel blocks of f (2 operands)
   TilePair *ptp = nullptr; int nptp = 0;  // 32x32 tile pairs of the Pandya combine
   TilePair *ftp = nullptr; int nftp = 0;  // 64x64 tile pairs of the fused Pandya product + combine
@@ -929,6 +930,7 @@ int run_twobody(qd_context *c, const qd_op *r1, const qd_op *r2, qd_op *out, int
   }
   if (part & TB_PART_MAIN) {
     if ((flags & TB_LADDER) && !lsq_done) run_ladder_sq(c, r1, r2);
+    paired = paired && c->box_pairs;
     launch_box2b(c->stream, c->dt, o, out->d, paired ? c->bt1p : c->bt1, paired ? c->nbt1p : c->nbt1, c->blist, flags,
                  c->w1, c->w2, c->w3, c->wl, c->fcls, (flags & TB_PANDYA) ? c->P : nullptr, c->d_sign);
   }
@@ -1235,6 +1237,7 @@ int qd_context_create(int nbody, int nshell, qd_context **out) {
     const int v = atoi(env);
     if (v >= 16 && v <= BX_VCAP) c->box_cap = v;
   }
+  if (const char *env = getenv("QD_BOX_PAIRS")) c->box_pairs = env[0] != '0';
   if ((rc = dev_alloc(c, sizeof(double) * ch.M * ch.M, &p))) return bail(rc); c->u1 = (double *)p;
   if ((rc = dev_alloc(c, sizeof(double) * ch.M * ch.M, &p))) return bail(rc); c->u2 = (double *)p;
   if ((rc = dev_alloc(c, sizeof(double) * 1024 * 64, &p))) return bail(rc); c->partials = (double *)p;

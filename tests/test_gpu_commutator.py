@@ -251,8 +251,33 @@ def test_box_slabs(monkeypatch, cap):
             ref = c.o.zeros()
             qd.commutator_122(X, Y, Z); c.o.commutator_122(x, h1, y, h2, ref)
             assert rel(Z.download()[two], ref[two]) <= TOL
+            # paired boxes (the ones that fit the cap both ways) beside slabs
+            Z = c.ctx.operator(-h1 * h2)
+            qd.commutator_cc(X, Y, Z, HCC, ECC)
+            assert rel(Z.download(), c.o.commutator(x, h1, y, h2, pandya=True)) <= TOL
     finally:
         c.ctx.close()
+
+
+def test_box_pairs_switch(monkeypatch):
+    # QD_BOX_PAIRS=0: every class-pair box on its own; the paired form (the
+    # transposed box's outputs as h_Z times the direct ones) agrees to 1e-14
+    outs = {}
+    for pairs in ("1", "0"):
+        monkeypatch.setenv("QD_BOX_PAIRS", pairs)
+        c = Case(6, 6)
+        try:
+            x, y, X, Y = c.ops(23, -1, 1)
+            Z, Z2 = c.ctx.operator(-1), c.ctx.operator(1)
+            HCC, ECC = c.ctx.cc(1.0), c.ctx.cc(-1.0)
+            qd.commutator_cc(X, Y, Z, HCC, ECC)
+            qd.commutator_122(X, Y, Z2)
+            outs[pairs] = (Z.download(), Z2.download())
+            assert rel(outs[pairs][0], c.o.commutator(x, -1, y, 1, pandya=True)) <= TOL
+        finally:
+            c.ctx.close()
+    for k in range(2):
+        assert rel(outs["1"][k], outs["0"][k]) <= 1e-14
 
 
 def test_without_gather_tables(monkeypatch):
