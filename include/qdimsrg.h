@@ -67,7 +67,10 @@ const char *qd_last_error(void);
  * tables, QD_GEMM_WG_PER_CU work-groups per CU of the persistent GEMM grid
  * (1..16, default 6), QD_BOX_PAIRS=0 every class-pair box of the two-body
  * pass evaluated on its own instead of paired with its transpose (results
- * then differ at rounding level only). */
+ * then differ at rounding level only), QD_BOX_WIDE=0/1 the narrow (512
+ * threads, 3600-element grids) or wide (1024 threads, 8000) box tasks instead
+ * of the automatic choice (wide where the narrow form would split more than a
+ * tenth of the grid elements into slabs; QD_BOX_CAP then ranges up to 8000). */
 int qd_context_create(int nbody, int nshell, qd_context **out);
 int qd_context_destroy(qd_context *ctx);
 int qd_context_info(const qd_context *ctx, qd_info *out);

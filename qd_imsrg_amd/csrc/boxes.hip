@@ -52,18 +52,20 @@ __device__ __forceinline__ int box_of(const int *pre, int lo, int ng, int e) {
 // from the nearest integer
 __device__ __forceinline__ int qdiv(int a, float rd) { return (int)(((float)a + 0.5f) * rd); }
 
-__global__ void __launch_bounds__(BX_THREADS) k_box2b(DevTab t, Ops o, double *__restrict__ out,
+// NT threads per task, grids of up to VC elements, RC row grid entries
+template <int NT, int VC, int RC>
+__global__ void __launch_bounds__(NT) k_box2b(DevTab t, Ops o, double *__restrict__ out,
                                                const BTask *__restrict__ tasks, const int *__restrict__ blist,
                                                int flags, const double *__restrict__ w1,
                                                const double *__restrict__ w2, const double *__restrict__ w3,
                                                const double *__restrict__ L, const double *__restrict__ F,
                                                const double *__restrict__ D, double dsign) {
-  __shared__ double2 gg[BX_VCAP];              // ordered grids of (V1, V2): [row][ldc]
+  __shared__ double2 gg[VC];              // ordered grids of (V1, V2): [row][ldc]
   __shared__ double2 fl[BX_FCAP / 2];          // channel blocks of (f1, f2): A3, A4, then A1, A2 per box
   __shared__ int pre[BX_GMAX + 1];             // element offsets of the boxes
   __shared__ int4 bx[BX_GMAX];                 // class pair of the rows, rows in the task, grid offset, F offset
   __shared__ int4 bs[BX_GMAX];                 // |A1|, |A2|, A1 == A2, first row entry in rent
-  __shared__ uint32_t rent[BX_RCAP];           // grid entries of the task's rows, box after box
+  __shared__ uint32_t rent[RC];           // grid entries of the task's rows, box after box
   __shared__ uint32_t cent[BX_CMAX];           // grid entries of the task's columns
   __shared__ int64_t soff[3][3];               // stored sub-block of (row type, column type), row >= column
   __shared__ float2 brc[BX_GMAX];              // 1 / rows in the task, 1 / |A2|
@@ -92,7 +94,7 @@ __global__ void __launch_bounds__(BX_THREADS) k_box2b(DevTab t, Ops o, double *_
     const int sub = tB == 0 ? (tA == 0 ? HHHH : (tA == 1 ? PHHH : PPHH)) : (tB == 1 ? (tA == 1 ? PHPH : PPPH) : PPPP);
     soff[tA][tB] = tA >= tB ? t.off[6 * q + sub] : 0;
   }
-  for (int k = threadIdx.x; k < nc; k += BX_THREADS) cent[k] = cgrid[k];
+  for (int k = threadIdx.x; k < nc; k += NT) cent[k] = cgrid[k];
   if (threadIdx.x < 64) {
     // wave 0, one lane per box: the box's tables (three dependent loads for
     // all boxes at once), then its offsets by exclusive prefix sums over the
@@ -162,7 +164,7 @@ __global__ void __launch_bounds__(BX_THREADS) k_box2b(DevTab t, Ops o, double *_
   {
     // row grid entries of all boxes in one flat pass
     const int rtot = rtot_s;
-    for (int k = threadIdx.x; k < rtot; k += BX_THREADS) {
+    for (int k = threadIdx.x; k < rtot; k += NT) {
       int lo = 0, hi = ng - 1;  // last box whose rows start at or before k
       while (lo < hi) {
         const int mid = (lo + hi + 1) >> 1;
@@ -180,12 +182,12 @@ __global__ void __launch_bounds__(BX_THREADS) k_box2b(DevTab t, Ops o, double *_
     // before the LDS stores
     const int fs = t.fcls_size, q3 = s3 * s3, q34 = q3 + s4 * s4, ftot = fsz;
     const int f3 = t.fcls_off[A3], f4 = t.fcls_off[A4];
-    for (int k0 = threadIdx.x; k0 < ftot; k0 += 2 * BX_THREADS) {
+    for (int k0 = threadIdx.x; k0 < ftot; k0 += 2 * NT) {
       double y1[2], y2[2];
       int kk[2];
 #pragma unroll
       for (int u = 0; u < 2; u++) {
-        const int k = min(k0 + BX_THREADS * u, ftot - 1);
+        const int k = min(k0 + NT * u, ftot - 1);
         int src;
         if (k < q34) src = k < q3 ? f3 + k : f4 + (k - q3);
         else {
@@ -204,7 +206,7 @@ __global__ void __launch_bounds__(BX_THREADS) k_box2b(DevTab t, Ops o, double *_
       }
 #pragma unroll
       for (int u = 0; u < 2; u++)
-        if (k0 + BX_THREADS * u < ftot) fl[kk[u]] = make_double2(y1[u], y2[u]);
+        if (k0 + NT * u < ftot) fl[kk[u]] = make_double2(y1[u], y2[u]);
     }
     // ordered grids: V(x1,x2,x3,x4) = sign * (herm if stored transposed) *
     // stored element of the canonical pairs, 0 for x1 == x2 or x3 == x4.
@@ -212,13 +214,13 @@ __global__ void __launch_bounds__(BX_THREADS) k_box2b(DevTab t, Ops o, double *_
     // branch-free (past the end the last element is read again; an invalid
     // entry reads element 0) before the first LDS store, so the round trips
     // overlap; validity and sign are applied at the store.
-    for (int e0 = threadIdx.x, b = 0; e0 < total; e0 += 4 * BX_THREADS) {
+    for (int e0 = threadIdx.x, b = 0; e0 < total; e0 += 4 * NT) {
       double x1[4], x2[4], g1[4], g2[4];
       int dst[4];
       bool ok[4];
 #pragma unroll
       for (int u = 0; u < 4; u++) {
-        const int e = min(e0 + BX_THREADS * u, total - 1);
+        const int e = min(e0 + NT * u, total - 1);
         if (b + 1 < ng && pre[b + 1] <= e) b = box_of(pre, b + 1, ng, e);
         const int4 B = bx[b];
         const int le = e - pre[b], c = qdiv(le, brc[b].x), r = le - c * B.y;
@@ -240,7 +242,7 @@ __global__ void __launch_bounds__(BX_THREADS) k_box2b(DevTab t, Ops o, double *_
       }
 #pragma unroll
       for (int u = 0; u < 4; u++)
-        if (e0 + BX_THREADS * u < total)
+        if (e0 + NT * u < total)
           gg[dst[u]] = ok[u] ? make_double2(g1[u] * x1[u], g2[u] * x2[u]) : make_double2(0.0, 0.0);
     }
     __syncthreads();
@@ -252,7 +254,7 @@ __global__ void __launch_bounds__(BX_THREADS) k_box2b(DevTab t, Ops o, double *_
   auto sub_of = [](int ta, int tb) {
     return tb == 0 ? (ta == 0 ? HHHH : (ta == 1 ? PHHH : PPHH)) : (tb == 1 ? (ta == 1 ? PHPH : PPPH) : PPPP);
   };
-  for (int e = threadIdx.x, b = 0; e < total; e += BX_THREADS) {
+  for (int e = threadIdx.x, b = 0; e < total; e += NT) {
     if (b + 1 < ng && pre[b + 1] <= e) b = box_of(pre, b + 1, ng, e);
     const int4 B = bx[b];
     const int le = e - pre[b], c = qdiv(le, brc[b].x), r = le - c * B.y;
@@ -385,10 +387,14 @@ __global__ void __launch_bounds__(BX_THREADS) k_box2b(DevTab t, Ops o, double *_
 
 void launch_box2b(hipStream_t s, const DevTab &t, const Ops &o, double *out, const BTask *tasks, int ntasks,
                   const int *blist, int flags, const double *w1, const double *w2, const double *w3,
-                  const double *L, const double *F, const double *D, double dsign) {
+                  const double *L, const double *F, const double *D, double dsign, bool wide) {
   if (ntasks <= 0) return;
-  hipLaunchKernelGGL(k_box2b, dim3(ntasks), dim3(BX_THREADS), 0, s, t, o, out, tasks, blist, flags, w1, w2, w3, L, F, D,
-                     dsign);
+  if (wide)
+    hipLaunchKernelGGL((k_box2b<BX_WIDE_THREADS, BX_WIDE_VCAP, BX_WIDE_RCAP>), dim3(ntasks), dim3(BX_WIDE_THREADS), 0, s,
+                       t, o, out, tasks, blist, flags, w1, w2, w3, L, F, D, dsign);
+  else
+    hipLaunchKernelGGL((k_box2b<BX_THREADS, BX_VCAP, BX_RCAP>), dim3(ntasks), dim3(BX_THREADS), 0, s, t, o, out, tasks,
+                       blist, flags, w1, w2, w3, L, F, D, dsign);
 }
 
 __global__ void __launch_bounds__(256) k_sum_halves(const double *__restrict__ w3, const double *__restrict__ w4,

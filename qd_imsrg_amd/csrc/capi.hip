@@ -100,6 +100,8 @@ struct qd_context {
   int nbt1 = 0, nbt2 = 0, nbt1p = 0;
   int *blist = nullptr;                                              // class pairs of the box tasks
   int box_cap = BX_VCAP;                                             // grid elements per box task
+  int box_rcap = BX_RCAP;                                            // row grid entries per box task
+  bool box_wide = false;                                             // tasks for the wide box form
   bool box_pairs = true;                                             // paired box tasks where valid
   double *fcls = nullptr;                                            // channel blocks of f (2 operands)
   TilePair *ptp = nullptr; int nptp = 0;  // 32x32 tile pairs of the Pandya combine
@@ -498,7 +500,7 @@ int build_worklists(qd_context *c) {
         for (int gr = ch.cp_start[q]; gr < ch.cp_start[q + 1]; gr++) {
           const int s1 = csize(ch.cp_a[gr]), s2 = csize(ch.cp_b[gr]);
           const int nr = s1 * s2, fneed = 2 * (s1 * s1 + s2 * s2);
-          if (fcol + fneed > BX_FCAP || 3 * nr > cap || ldc > cap || nc > BX_CMAX || nr > BX_RCAP)
+          if (fcol + fneed > BX_FCAP || 3 * nr > cap || ldc > cap || nc > BX_CMAX || nr > c->box_rcap)
             return fail(QD_E_LAYOUT, "single-particle channel of %d states too large for the two-body boxes",
                         std::max(std::max(s1, s2), std::max(s3, s4)));
           const bool pair_here = gr < gc && pairable(gr, gc), pair_there = gr > gc && pairable(gc, gr);
@@ -534,7 +536,7 @@ int build_worklists(qd_context *c) {
             const int nr = s1 * s2, fneed = 2 * (s1 * s1 + s2 * s2);
             Bin *fit = nullptr;
             for (auto &bn : bins)
-              if (bn.goff + nr * ldc <= cap && bn.foff + fneed <= BX_FCAP && bn.roff + nr <= BX_RCAP &&
+              if (bn.goff + nr * ldc <= cap && bn.foff + fneed <= BX_FCAP && bn.roff + nr <= c->box_rcap &&
                   (int)bn.g.size() < BX_GMAX) {
                 fit = &bn;
                 break;
@@ -926,13 +928,13 @@ int run_twobody(qd_context *c, const qd_op *r1, const qd_op *r2, qd_op *out, int
   if ((part & TB_PART_PRE) && (flags & TB_122)) {
     launch_fclass(c->stream, c->dt, o, c->fcls);
     launch_box2b(c->stream, c->dt, o, out->d, c->bt2, c->nbt2, c->blist, flags, c->w1, c->w2, c->w3, c->wl,
-                 c->fcls, nullptr, 0.0);
+                 c->fcls, nullptr, 0.0, c->box_wide);
   }
   if (part & TB_PART_MAIN) {
     if ((flags & TB_LADDER) && !lsq_done) run_ladder_sq(c, r1, r2);
     paired = paired && c->box_pairs;
     launch_box2b(c->stream, c->dt, o, out->d, paired ? c->bt1p : c->bt1, paired ? c->nbt1p : c->nbt1, c->blist, flags,
-                 c->w1, c->w2, c->w3, c->wl, c->fcls, (flags & TB_PANDYA) ? c->P : nullptr, c->d_sign);
+                 c->w1, c->w2, c->w3, c->wl, c->fcls, (flags & TB_PANDYA) ? c->P : nullptr, c->d_sign, c->box_wide);
   }
   HIPC(hipGetLastError());
   return QD_OK;
@@ -1232,10 +1234,28 @@ int qd_context_create(int nbody, int nshell, qd_context **out) {
   if ((rc = dev_alloc(c, sizeof(double) * (ch.p_size + 1), &p))) return bail(rc); c->P = (double *)p;
   if ((rc = dev_alloc(c, sizeof(double) * 2 * std::max(ch.fcls_size, 1), &p))) return bail(rc);
   c->fcls = (double *)p;
+  {
+    // the wide box form where the narrow one would split more than a tenth
+    // of the grid elements into slabs (QD_BOX_WIDE=0/1 decides instead)
+    double all = 0, split = 0;
+    auto csz = [&](int h) { return ch.ch_start[h + 1] - ch.ch_start[h]; };
+    for (int q = 0; q < ch.nblock; q++)
+      for (int gr = ch.cp_start[q]; gr < ch.cp_start[q + 1]; gr++)
+        for (int gc = ch.cp_start[q]; gc < ch.cp_start[q + 1]; gc++) {
+          if (ch.cp_tmax[gr] < ch.cp_tmin[gc]) continue;
+          const int nr = csz(ch.cp_a[gr]) * csz(ch.cp_b[gr]), nc = csz(ch.cp_a[gc]) * csz(ch.cp_b[gc]);
+          all += (double)nr * nc;
+          if (nr * (nc | 1) > BX_VCAP) split += (double)nr * nc;
+        }
+    c->box_wide = split > 0.1 * all;
+    if (const char *env = getenv("QD_BOX_WIDE")) c->box_wide = env[0] == '1';
+    c->box_cap = c->box_wide ? BX_WIDE_VCAP : BX_VCAP;
+    c->box_rcap = c->box_wide ? BX_WIDE_RCAP : BX_RCAP;
+  }
   if (const char *env = getenv("QD_BOX_CAP")) {
     // grid elements per box task (tests use a small cap to exercise the slabs)
     const int v = atoi(env);
-    if (v >= 16 && v <= BX_VCAP) c->box_cap = v;
+    if (v >= 16 && v <= c->box_cap) c->box_cap = v;
   }
   if (const char *env = getenv("QD_BOX_PAIRS")) c->box_pairs = env[0] != '0';
   if ((rc = dev_alloc(c, sizeof(double) * ch.M * ch.M, &p))) return bail(rc); c->u1 = (double *)p;

@@ -130,6 +130,13 @@ constexpr int BX_FCAP = 1200;  // doubles of one-body channel blocks staged in L
 constexpr int BX_GMAX = 48;    // boxes per task (at most 64: one wave's lanes at set-up)
 constexpr int BX_RCAP = 384;   // row grid entries per task (staged in LDS)
 constexpr int BX_CMAX = 128;   // column grid entries per task (a class pair of <= 11 x 11 states)
+// The wide form: 1024 threads and 148 KB of LDS per task (one per CU, 16
+// waves), grids of up to 8000 elements, for channel structures where many
+// boxes exceed 3600 elements (R = 20: a third of the grid elements), which
+// the narrow form splits into slabs
+constexpr int BX_WIDE_THREADS = 1024;
+constexpr int BX_WIDE_VCAP = 8000;
+constexpr int BX_WIDE_RCAP = 768;
 // one-body matrices of both operands blocked by channel: F[0, fcls_size) of
 // r1, F[fcls_size, 2 fcls_size) of r2
 void launch_fclass(hipStream_t s, const DevTab &t, const Ops &o, double *F);
@@ -137,7 +144,8 @@ void launch_fclass(hipStream_t s, const DevTab &t, const Ops &o, double *F);
 // combinations of the products w1..w3 and, for the square sub-blocks and
 // the summed products of Vpphh and Vppph, of L
 // (TB_LADDER; launch_ladder_sq has filled L), Pandya term from D
-// (TB_PANDYA); TB_ACCUM adds to out (mode 2 tasks always add).  dsign != 0:
+// (TB_PANDYA); TB_ACCUM adds to out (mode 2 tasks always add).  wide: the
+// tasks were built for the wide form (BX_WIDE_*).  dsign != 0:
 // D holds only its upper 64 x 64 tiles (launch_pandya_fused), a lower element
 // (x, y) is read as dsign * D(y, x); dsign = 0: D is whole
 // Paired boxes (BOX_PAIRED in blist, the bt1p task list) need operators that
@@ -145,7 +153,7 @@ void launch_fclass(hipStream_t s, const DevTab &t, const Ops &o, double *F);
 // with the Pandya term, D built from calc_cc of those operators.
 void launch_box2b(hipStream_t s, const DevTab &t, const Ops &o, double *out, const BTask *tasks, int ntasks,
                   const int *blist, int flags, const double *w1, const double *w2, const double *w3,
-                  const double *L, const double *F, const double *D, double dsign);
+                  const double *L, const double *F, const double *D, double dsign, bool wide);
 // L = w3 + w4 over the given sub-blocks (the two halves of the summed
 // products of Vpphh and Vppph)
 void launch_sum_halves(hipStream_t s, const double *w3, const double *w4, double *L, const Seg2 *segs,

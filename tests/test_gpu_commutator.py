@@ -297,3 +297,29 @@ def test_without_gather_tables(monkeypatch):
             assert rel(Z.download(), c.o.commutator(x, h1, y, h2, pandya=True)) <= TOL
     finally:
         c.ctx.close()
+
+
+@pytest.mark.parametrize("cap", ["8000", "100"])
+def test_wide_box_form(monkeypatch, cap):
+    # QD_BOX_WIDE=1: 1024-thread tasks with grids of up to 8000 elements (the
+    # form chosen by itself at R = 20); with a cap of 100 most boxes take the
+    # slab path of the wide form
+    monkeypatch.setenv("QD_BOX_WIDE", "1")
+    monkeypatch.setenv("QD_BOX_CAP", cap)
+    c = Case(6, 6)
+    try:
+        for h1, h2 in [(-1, 1), (1, 1)]:
+            x, y, X, Y = c.ops(25, h1, h2)
+            Z = c.ctx.operator(-h1 * h2)
+            HCC, ECC = c.ctx.cc(float(h2)), c.ctx.cc(float(h1))
+            qd.commutator_cc(X, Y, Z, HCC, ECC)
+            assert rel(Z.download(), c.o.commutator(x, h1, y, h2, pandya=True)) <= TOL
+            qd.commutator(X, Y, Z, HCC, ECC)
+            assert rel(Z.download(), c.o.commutator(x, h1, y, h2, pandya=True)) <= TOL
+            two = slice(c.ch.off[0, 0], None)
+            Z = c.ctx.operator(1)
+            ref = c.o.zeros()
+            qd.commutator_122(X, Y, Z); c.o.commutator_122(x, h1, y, h2, ref)
+            assert rel(Z.download()[two], ref[two]) <= TOL
+    finally:
+        c.ctx.close()
