@@ -26,6 +26,10 @@ typedef double d4 __attribute__((ext_vector_type(4)));
 // chunk's global loads issued into registers (load) before the current
 // chunk's MFMAs (compute) and written to LDS (store) after the next barrier.
 // compute(kv) gets the count of valid k of the chunk (the rest is zero fill).
+// load only issues the loads: anything computed from the loaded values (the
+// segment's alpha, a conversion, the slicing) belongs in store, which gets
+// the staged segment's ta, tb and alpha.  Arithmetic on them inside load
+// would make the loads complete before the current chunk's MFMAs start.
 template <int KC, class Load, class Store, class Compute>
 __device__ __forceinline__ void chunk_walk(const GemmProb &pr, const GemmSeg *__restrict__ segs, Load load,
                                            Store store, Compute compute) {
@@ -34,6 +38,7 @@ __device__ __forceinline__ void chunk_walk(const GemmProb &pr, const GemmSeg *__
   GemmSeg sg = segs[pr.seg0];
   load(sg, 0);
   int cta = sg.ta, ctb = sg.tb;
+  double calpha = sg.alpha;
   while (true) {
     const int kv = sg.K - k0 < KC ? sg.K - k0 : KC;
     int nsi = si, nk0 = k0 + KC;
@@ -45,12 +50,13 @@ __device__ __forceinline__ void chunk_walk(const GemmProb &pr, const GemmSeg *__
     }
     const bool more = nsi < pr.nseg;
     __syncthreads();
-    store(cta, ctb);
+    store(cta, ctb, calpha);
     __syncthreads();
     if (more) {
       load(nsg, nk0);
       cta = nsg.ta;
       ctb = nsg.tb;
+      calpha = nsg.alpha;
     }
     compute(kv);
     if (!more) break;
@@ -93,7 +99,7 @@ __device__ __forceinline__ void gemm_tile(const GemmProb &pr, const GemmSeg *__r
       double v = 0.0;
       if (idx < TM * KC && gi < pr.M && gk < sg.K)
         v = sg.ta ? A[gk + (int64_t)gi * sg.lda] : A[gi + (int64_t)gk * sg.lda];
-      ra[r] = sg.alpha * v;
+      ra[r] = v;
     }
 #pragma unroll
     for (int r = 0; r < NB; r++) {
@@ -108,13 +114,13 @@ __device__ __forceinline__ void gemm_tile(const GemmProb &pr, const GemmSeg *__r
       rb[r] = v;
     }
   };
-  auto store = [&](int ta, int tb) {
+  auto store = [&](int ta, int tb, double alpha) {
 #pragma unroll
     for (int r = 0; r < NA; r++) {
       const int idx = tid + 256 * r;
       if (idx < TM * KC) {
         const int i = ta ? idx / KC : idx % TM, k = ta ? idx % KC : idx / TM;
-        As[k * (TM + 1) + i] = ra[r];
+        As[k * (TM + 1) + i] = alpha * ra[r];
       }
     }
 #pragma unroll
@@ -144,20 +150,26 @@ __device__ __forceinline__ void gemm_tile(const GemmProb &pr, const GemmSeg *__r
   chunk_walk<KC>(pr, segs, load, store, compute);
   // acc holds the tile transposed (B^T A^T): lane l, component v is row
   // l % 16, column 4 v + l / 16, so each store covers four columns of 16
-  // consecutive rows of the column-major C (whole 128-byte lines)
+  // consecutive rows of the column-major C (whole 128-byte lines).  With
+  // beta = 1 the four old values of a 16 x 16 block are loaded before its
+  // first store, one global round trip per block instead of one per element.
   double *__restrict__ C = slots.p[pr.c_slot] + pr.c_off;
 #pragma unroll
   for (int x = 0; x < FM; x++)
 #pragma unroll
     for (int y = 0; y < FN; y++) {
       const int r = row0 + m0 + x * 16 + lr;
+      double old[4];
 #pragma unroll
       for (int v = 0; v < 4; v++) {
         const int c = col0 + n0 + y * 16 + 4 * v + lk;
-        if (r < pr.M && c < pr.N) {
-          double *p = C + r + (int64_t)c * pr.ldc;
-          *p = pr.beta1 ? *p + acc[x][y][v] : acc[x][y][v];
-        }
+        old[v] = 0.0;
+        if (pr.beta1 && r < pr.M && c < pr.N) old[v] = C[r + (int64_t)c * pr.ldc];
+      }
+#pragma unroll
+      for (int v = 0; v < 4; v++) {
+        const int c = col0 + n0 + y * 16 + 4 * v + lk;
+        if (r < pr.M && c < pr.N) C[r + (int64_t)c * pr.ldc] = pr.beta1 ? old[v] + acc[x][y][v] : acc[x][y][v];
       }
     }
 }
@@ -193,7 +205,7 @@ __device__ __forceinline__ void gemm_tile_splitk(const GemmProb &pr, const GemmS
       double v = 0.0;
       if (idx < TM * GEMM_TK && gi < pr.M && gk < sg.K)
         v = sg.ta ? A[gk + (int64_t)gi * sg.lda] : A[gi + (int64_t)gk * sg.lda];
-      ra[r] = sg.alpha * v;
+      ra[r] = v;
     }
 #pragma unroll
     for (int r = 0; r < NB; r++) {
@@ -208,13 +220,13 @@ __device__ __forceinline__ void gemm_tile_splitk(const GemmProb &pr, const GemmS
       rb[r] = v;
     }
   };
-  auto store = [&](int ta, int tb) {
+  auto store = [&](int ta, int tb, double alpha) {
 #pragma unroll
     for (int r = 0; r < NA; r++) {
       const int idx = tid + 256 * r;
       if (idx < TM * GEMM_TK) {
         const int i = ta ? idx / GEMM_TK : idx % TM, k = ta ? idx % GEMM_TK : idx / TM;
-        As[k * (TM + 1) + i] = ra[r];
+        As[k * (TM + 1) + i] = alpha * ra[r];
       }
     }
 #pragma unroll
@@ -252,6 +264,15 @@ __device__ __forceinline__ void gemm_tile_splitk(const GemmProb &pr, const GemmS
   if (wave == 0) {
     double *__restrict__ C = slots.p[pr.c_slot] + pr.c_off;
     const int c = col0 + lr;
+    double old[4][4];  // with beta = 1, every old value is loaded before the first store
+#pragma unroll
+    for (int x = 0; x < 4; x++)
+#pragma unroll
+      for (int v = 0; v < 4; v++) {
+        const int r = row0 + x * 16 + 4 * v + lk;
+        old[x][v] = 0.0;
+        if (pr.beta1 && r < pr.M && c < pr.N) old[x][v] = C[r + (int64_t)c * pr.ldc];
+      }
 #pragma unroll
     for (int x = 0; x < 4; x++)
 #pragma unroll
@@ -260,10 +281,7 @@ __device__ __forceinline__ void gemm_tile_splitk(const GemmProb &pr, const GemmS
 #pragma unroll
         for (int w = 0; w < 3; w++) sum += lds[(w * 16 + x * 4 + v) * 64 + lane];
         const int r = row0 + x * 16 + 4 * v + lk;
-        if (r < pr.M && c < pr.N) {
-          double *p = C + r + (int64_t)c * pr.ldc;
-          *p = pr.beta1 ? *p + sum : sum;
-        }
+        if (r < pr.M && c < pr.N) C[r + (int64_t)c * pr.ldc] = pr.beta1 ? old[x][v] + sum : sum;
       }
   }
 }
@@ -312,19 +330,24 @@ __device__ __forceinline__ void store_f32_layout(const GemmProb &pr, const Slots
                                                  const d4 (&acc)[FM][FN]) {
   double *__restrict__ C = slots.p[pr.c_slot] + pr.c_off;
 #pragma unroll
-  for (int x = 0; x < FM; x++)
+  for (int x = 0; x < FM; x++) {
+    double old[FN][4];  // with beta = 1, the row block's old values are loaded before its first store
 #pragma unroll
-    for (int y = 0; y < FN; y++) {
-      const int c = c0 + y * 16 + lr;
+    for (int y = 0; y < FN; y++)
 #pragma unroll
       for (int v = 0; v < 4; v++) {
-        const int r = r0 + x * 16 + 4 * lk + v;
-        if (r < pr.M && c < pr.N) {
-          double *p = C + r + (int64_t)c * pr.ldc;
-          *p = pr.beta1 ? *p + acc[x][y][v] : acc[x][y][v];
-        }
+        const int r = r0 + x * 16 + 4 * lk + v, c = c0 + y * 16 + lr;
+        old[y][v] = 0.0;
+        if (pr.beta1 && r < pr.M && c < pr.N) old[y][v] = C[r + (int64_t)c * pr.ldc];
       }
-    }
+#pragma unroll
+    for (int y = 0; y < FN; y++)
+#pragma unroll
+      for (int v = 0; v < 4; v++) {
+        const int r = r0 + x * 16 + 4 * lk + v, c = c0 + y * 16 + lr;
+        if (r < pr.M && c < pr.N) C[r + (int64_t)c * pr.ldc] = pr.beta1 ? old[y][v] + acc[x][y][v] : acc[x][y][v];
+      }
+  }
 }
 
 template <int TM, int TN, int WM, int WN, int S>
@@ -353,8 +376,8 @@ __device__ __forceinline__ void gemm_tile_bf16(const GemmProb &pr, const GemmSeg
       const int idx = tid + 256 * r;
       const int i = sg.ta ? idx / KP : idx % TM, kp = sg.ta ? idx % KP : idx / TM;
       const bool in = idx < TM * KP;
-      ra[r][0] = in ? sg.alpha * load_a(sg, A, pr, row0 + i, k0 + 2 * kp) : 0.0;
-      ra[r][1] = in ? sg.alpha * load_a(sg, A, pr, row0 + i, k0 + 2 * kp + 1) : 0.0;
+      ra[r][0] = in ? load_a(sg, A, pr, row0 + i, k0 + 2 * kp) : 0.0;
+      ra[r][1] = in ? load_a(sg, A, pr, row0 + i, k0 + 2 * kp + 1) : 0.0;
     }
 #pragma unroll
     for (int r = 0; r < NB; r++) {
@@ -365,15 +388,15 @@ __device__ __forceinline__ void gemm_tile_bf16(const GemmProb &pr, const GemmSeg
       rb[r][1] = in ? load_b(sg, B, pr, col0 + j, k0 + 2 * kp + 1) : 0.0;
     }
   };
-  auto store = [&](int ta, int tb) {
+  auto store = [&](int ta, int tb, double alpha) {
 #pragma unroll
     for (int r = 0; r < NA; r++) {
       const int idx = tid + 256 * r;
       if (idx < TM * KP) {
         const int i = ta ? idx / KP : idx % TM, kp = ta ? idx % KP : idx / TM;
         unsigned h0[S], h1[S];
-        bf16_slices<S>(ra[r][0], h0);
-        bf16_slices<S>(ra[r][1], h1);
+        bf16_slices<S>(alpha * ra[r][0], h0);
+        bf16_slices<S>(alpha * ra[r][1], h1);
 #pragma unroll
         for (int s = 0; s < S; s++) As[(s * TM + i) * SPLIT_LDW + kp] = h0[s] | (h1[s] << 16);
       }
@@ -436,7 +459,7 @@ __device__ __forceinline__ void gemm_tile_f32(const GemmProb &pr, const GemmSeg 
   for (int a = 0; a < FM; a++)
 #pragma unroll
     for (int b = 0; b < FN; b++) acc[a][b] = d4{0, 0, 0, 0};
-  float ra[NA], rb[NB];
+  double ra[NA], rb[NB];
   auto load = [&](const GemmSeg &sg, int k0) {
     const double *__restrict__ A = slots.p[sg.a_slot] + sg.a_off;
     const double *__restrict__ B = slots.p[sg.b_slot] + sg.b_off;
@@ -444,22 +467,22 @@ __device__ __forceinline__ void gemm_tile_f32(const GemmProb &pr, const GemmSeg 
     for (int r = 0; r < NA; r++) {
       const int idx = tid + 256 * r;
       const int i = sg.ta ? idx / SPLIT_TK : idx % TM, k = sg.ta ? idx % SPLIT_TK : idx / TM;
-      ra[r] = idx < TM * SPLIT_TK ? (float)(sg.alpha * load_a(sg, A, pr, row0 + i, k0 + k)) : 0.0f;
+      ra[r] = idx < TM * SPLIT_TK ? load_a(sg, A, pr, row0 + i, k0 + k) : 0.0;
     }
 #pragma unroll
     for (int r = 0; r < NB; r++) {
       const int idx = tid + 256 * r;
       const int j = sg.tb ? idx % TN : idx / SPLIT_TK, k = sg.tb ? idx / TN : idx % SPLIT_TK;
-      rb[r] = idx < TN * SPLIT_TK ? (float)load_b(sg, B, pr, col0 + j, k0 + k) : 0.0f;
+      rb[r] = idx < TN * SPLIT_TK ? load_b(sg, B, pr, col0 + j, k0 + k) : 0.0;
     }
   };
-  auto store = [&](int ta, int tb) {
+  auto store = [&](int ta, int tb, double alpha) {
 #pragma unroll
     for (int r = 0; r < NA; r++) {
       const int idx = tid + 256 * r;
       if (idx < TM * SPLIT_TK) {
         const int i = ta ? idx / SPLIT_TK : idx % TM, k = ta ? idx % SPLIT_TK : idx / TM;
-        As[k * (TM + 1) + i] = ra[r];
+        As[k * (TM + 1) + i] = (float)(alpha * ra[r]);
       }
     }
 #pragma unroll
@@ -467,7 +490,7 @@ __device__ __forceinline__ void gemm_tile_f32(const GemmProb &pr, const GemmSeg 
       const int idx = tid + 256 * r;
       if (idx < TN * SPLIT_TK) {
         const int j = tb ? idx % TN : idx / SPLIT_TK, k = tb ? idx / TN : idx % SPLIT_TK;
-        Bs[k * (TN + 1) + j] = rb[r];
+        Bs[k * (TN + 1) + j] = (float)rb[r];
       }
     }
   };
