@@ -490,7 +490,8 @@ int qd_gemm_flops(const qd_context *ctx, double *ladder, double *pandya, double 
  * precision (zero-filled tile edges and K steps included; BF16 modes count
  * every slice product, FP32 and BF16 full K stages of 32): the product
  * launch's ladder and one-body problems, its Pandya part (the fused kernel's
- * tile pairs in the grouped FP64 form), and the accumulating launch.  Against
+ * tile pairs in the grouped FP64 and BF16 forms), and the accumulating
+ * launch.  Against
  * qd_gemm_flops this is the padding of each list; against the launch time,
  * its matrix-core rate. */
 int qd_gemm_mfma_flops(const qd_context *ctx, double *products, double *pandya, double *acc);

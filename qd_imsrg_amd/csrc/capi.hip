@@ -871,15 +871,15 @@ int run_gemm_list(qd_context *c, const GemmList &g, bool with_pandya, const Slot
 int run_products(qd_context *c, const qd_op *r1, const qd_op *r2, qd_op *out, const qd_cc *hcc, const qd_cc *ecc) {
   const bool pd = pandya_on(hcc, ecc);
   Slots s = make_slots(c, r1, r2, out, ecc, hcc);
-  // grouped FP64: the Pandya product and its combine run fused; per-channel
-  // launches and the split precisions keep product GEMM + combine pass
-  const bool fused = pd && c->launch_mode == 0 && c->gemm_prec == GEMM_FP64;
+  // grouped FP64 and BF16 slices: the Pandya product and its combine run
+  // fused; per-channel launches and FP32 keep product GEMM + combine pass
+  const bool fused = pd && c->launch_mode == 0 && pandya_fused_prec(c->gemm_prec);
   {
     Timer tm(c, TC_PRODUCTS);
     int rc = run_gemm_list(c, c->g_all, pd && !fused, s);
     if (rc) return rc;
     if (fused) {
-      launch_pandya_fused(c->stream, c->dt, ecc->d, hcc->d, c->P, c->ftp, c->nftp, hcc->herm, ecc->herm);
+      launch_pandya_fused(c->stream, c->dt, ecc->d, hcc->d, c->P, c->ftp, c->nftp, hcc->herm, ecc->herm, c->gemm_prec);
       HIPC(hipGetLastError());
       c->d_sign = -hcc->herm * ecc->herm;
     }
@@ -999,7 +999,7 @@ int full_commutator(qd_context *c, const qd_op *r1, const qd_op *r2, qd_op *r3, 
   if (fh) fh->zero = false;
   if (fe) fe->zero = false;
   const bool pd = pandya_on(hcc, ecc);
-  const bool fused = pd && c->launch_mode == 0 && c->gemm_prec == GEMM_FP64;
+  const bool fused = pd && c->launch_mode == 0 && pandya_fused_prec(c->gemm_prec);
   const int tbf = TB_122 | TB_LADDER | (pd ? TB_PANDYA : 0);
   // paired boxes: without the Pandya term, or with CC sets that are calc_cc
   // of r2 and r1 under the operators' own herm flags
@@ -1020,7 +1020,7 @@ int full_commutator(qd_context *c, const qd_op *r1, const qd_op *r2, qd_op *r3, 
     if ((rc = run_gemm_list(c, c->g_all, false, s))) return rc;  // ladder (and one-body) products
     if (side) run_ladder_sq(c, r1, r2);  // while the CC fills finish on the side stream
     if (side) HIPC(hipStreamWaitEvent(main, c->ev_cc, 0));
-    launch_pandya_fused(c->stream, c->dt, ecc->d, hcc->d, c->P, c->ftp, c->nftp, hcc->herm, ecc->herm);
+    launch_pandya_fused(c->stream, c->dt, ecc->d, hcc->d, c->P, c->ftp, c->nftp, hcc->herm, ecc->herm, c->gemm_prec);
     HIPC(hipGetLastError());
     c->d_sign = -hcc->herm * ecc->herm;
   } else {
@@ -2201,7 +2201,7 @@ int qd_autotune(qd_context *c, int reps, double *ms_before, double *ms_after) {
   hipEvent_t e0, e1;
   HIPC(hipEventCreate(&e0));
   HIPC(hipEventCreate(&e1));
-  const bool fused = c->launch_mode == 0 && c->gemm_prec == GEMM_FP64;
+  const bool fused = c->launch_mode == 0 && pandya_fused_prec(c->gemm_prec);
   // the fastest of reps timed runs of both lists (after one untimed run):
   // the lists are deterministic work, so the minimum is the least noisy
   auto total = [&](double *ms) -> int {
@@ -2281,10 +2281,10 @@ int qd_set_gemm_shapes(qd_context *c, const int *shapes, int64_t len) {
 // Matrix-core flops one commutator's GEMM launches execute at the current
 // shapes and precision: every tile covers TM x TN outputs whatever the
 // problem's M, N (edge tiles zero-filled); FP64 tiles walk each segment's K
-// in MFMA k-steps of 4 (the zero fill past K skipped), the fused Pandya
-// kernel likewise over 64 x 64 tile pairs (one tile on the diagonal); FP32
-// tiles run K stages of 32 in full, BF16 tiles S(S+1)/2 slice products per
-// stage of 32.
+// in MFMA k-steps of 4 (the zero fill past K skipped); FP32 tiles run K
+// stages of 32 in full, BF16 tiles S(S+1)/2 slice products per stage of 32.
+// The fused Pandya kernels count likewise over 64 x 64 tile pairs (one tile
+// on the diagonal).
 int qd_gemm_mfma_flops(const qd_context *c, double *products, double *pandya, double *acc) {
   if (!c) return fail(QD_E_ARG, "null context");
   const int prec = c->gemm_prec;
@@ -2303,7 +2303,7 @@ int qd_gemm_mfma_flops(const qd_context *c, double *products, double *pandya, do
     }
     return f;
   };
-  const bool fused = c->launch_mode == 0 && prec == GEMM_FP64;
+  const bool fused = c->launch_mode == 0 && pandya_fused_prec(prec);
   const int npre = c->g_all.npre, nall = (int)c->g_all.hprobs.size();
   if (products) *products = list(c->g_all, 0, npre);
   if (acc) *acc = list(c->g_acc, 0, (int)c->g_acc.hprobs.size());
@@ -2314,7 +2314,8 @@ int qd_gemm_mfma_flops(const qd_context *c, double *products, double *pandya, do
         const int r = c->ch.cc_r[q], K = c->ch.cc_nb[q];
         if (r * K == 0) continue;
         const double nt = (r + 63) / 64, pairs = nt * (nt + 1) / 2;
-        f += (2 * pairs - nt) * 2.0 * 64 * 64 * 4.0 * ((K + 3) / 4);  // two tiles per pair, one on the diagonal
+        const double kk = prec == GEMM_FP64 ? 4.0 * ((K + 3) / 4) : 32.0 * ((K + 31) / 32) * slice;
+        f += (2 * pairs - nt) * 2.0 * 64 * 64 * kk;  // two tiles per pair, one on the diagonal
       }
       *pandya = f;
     } else {

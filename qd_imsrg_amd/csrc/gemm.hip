@@ -784,11 +784,143 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) k
   out(a1, i0, j0);
 }
 
-void launch_pandya_fused(hipStream_t s, const DevTab &t, const double *etacc, const double *hcc, double *D,
-                         const TilePair *tp, int ntp, double hh, double eh) {
-  if (ntp <= 0) return;
-  hipLaunchKernelGGL(k_pandya_fused, dim3(ntp), dim3(256), 0, s, t, etacc, hcc, D, tp, hh, eh);
+// The same fused pass with P = E H^T in BF16 slices (split precision S,
+// numerics of gemm_tile_bf16: K stages of 32 from k = 0, each stage's slice
+// products summed in FP32 smallest first, stage sums added into FP64).  The
+// two products of a tile pair run one after the other, each with its own K
+// walk over two staged panels (E rows, H rows), so only one product's
+// operands and slices are live at a time; P_ji stays in registers until the
+// epilogue.  The MFMAs take the H slices first, so they hold each 16 x 16
+// block transposed: lane l, component v is row l % 16, column 4 (l / 16) + v.
+template <int S>
+__device__ __forceinline__ void pandya_split_product(const double *__restrict__ E, const double *__restrict__ H, int r,
+                                                     int K, int e0, int h0, d4 (&acc)[2][2], unsigned *__restrict__ lds) {
+  constexpr int T = 64, KP = SPLIT_TK / 2, NU = T * KP / 256;
+  unsigned *Es = lds;                       // [S][T][SPLIT_LDW]: slice s of E(e0 + x, k), k pairs packed
+  unsigned *Hs = lds + S * T * SPLIT_LDW;   // [S][T][SPLIT_LDW]: slice s of H(h0 + x, k)
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int lr = lane & 15, lk = lane >> 4;
+  const int m0 = (wave & 1) * 32, n0 = (wave >> 1) * 32;
+  double re[NU][2], rh[NU][2];  // pairs of consecutive k
+  auto load = [&](int k0) {
+#pragma unroll
+    for (int u = 0; u < NU; u++) {
+      const int idx = tid + 256 * u, x = idx % T, gk = k0 + 2 * (idx / T);
+#pragma unroll
+      for (int h = 0; h < 2; h++) {
+        const bool kin = gk + h < K;
+        re[u][h] = (kin && e0 + x < r) ? E[(e0 + x) + (int64_t)(gk + h) * r] : 0.0;
+        rh[u][h] = (kin && h0 + x < r) ? H[(h0 + x) + (int64_t)(gk + h) * r] : 0.0;
+      }
+    }
+  };
+  load(0);
+  for (int k0 = 0; k0 < K; k0 += SPLIT_TK) {
+    __syncthreads();
+#pragma unroll
+    for (int u = 0; u < NU; u++) {
+      const int idx = tid + 256 * u, x = idx % T, kp = idx / T;
+      unsigned a0[S], a1[S], b0[S], b1[S];
+      bf16_slices<S>(re[u][0], a0);
+      bf16_slices<S>(re[u][1], a1);
+      bf16_slices<S>(rh[u][0], b0);
+      bf16_slices<S>(rh[u][1], b1);
+#pragma unroll
+      for (int q = 0; q < S; q++) {
+        Es[(q * T + x) * SPLIT_LDW + kp] = a0[q] | (a1[q] << 16);
+        Hs[(q * T + x) * SPLIT_LDW + kp] = b0[q] | (b1[q] << 16);
+      }
+    }
+    __syncthreads();
+    if (k0 + SPLIT_TK < K) load(k0 + SPLIT_TK);
+    bf8 a[S][2], b[S][2];
+#pragma unroll
+    for (int q = 0; q < S; q++)
+#pragma unroll
+      for (int x = 0; x < 2; x++) {
+        a[q][x] = __builtin_bit_cast(bf8, *(const u4 *)(Es + (q * T + m0 + x * 16 + lr) * SPLIT_LDW + 4 * lk));
+        b[q][x] = __builtin_bit_cast(bf8, *(const u4 *)(Hs + (q * T + n0 + x * 16 + lr) * SPLIT_LDW + 4 * lk));
+      }
+#pragma unroll
+    for (int x = 0; x < 2; x++)
+#pragma unroll
+      for (int y = 0; y < 2; y++) {
+        f4 c = f4{0, 0, 0, 0};
+#pragma unroll
+        for (int d = S - 1; d >= 0; d--)
+#pragma unroll
+          for (int i = 0; i <= d; i++) c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(b[d - i][y], a[i][x], c, 0, 0, 0);
+#pragma unroll
+        for (int v = 0; v < 4; v++) acc[x][y][v] += (double)c[v];
+      }
+  }
 }
+
+template <int S>
+__global__ void __launch_bounds__(256) k_pandya_fused_bf16(DevTab t, const double *__restrict__ E,
+                                                           const double *__restrict__ H, double *__restrict__ D,
+                                                           const TilePair *__restrict__ tp, double hh, double eh) {
+  constexpr int T = 64, LDE = T + 1;
+  constexpr int NSTAGE = 2 * S * T * SPLIT_LDW * 4, NEPI = T * LDE * 8;
+  __shared__ __align__(16) unsigned char lds[NSTAGE > NEPI ? NSTAGE : NEPI];
+  const TilePair pr = tp[blockIdx.x];
+  const int q = pr.blk, r = t.cc_r[q], K = t.cc_nb[q];
+  const double *__restrict__ Eq = E + t.cc_off[q];
+  const double *__restrict__ Hq = H + t.cc_off[q];
+  double *__restrict__ Dq = D + t.cc_poff[q];
+  const bool diag = pr.ti == pr.tj;
+  const int i0 = pr.ti * T, j0 = pr.tj * T;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int lr = lane & 15, lk = lane >> 4;
+  const int m0 = (wave & 1) * 32, n0 = (wave >> 1) * 32;
+  d4 a1[2][2], a2[2][2];
+#pragma unroll
+  for (int x = 0; x < 2; x++)
+#pragma unroll
+    for (int y = 0; y < 2; y++) a1[x][y] = a2[x][y] = d4{0, 0, 0, 0};
+  if (!diag) pandya_split_product<S>(Eq, Hq, r, K, j0, i0, a2, (unsigned *)lds);  // P_ji = E_j H_i^T
+  pandya_split_product<S>(Eq, Hq, r, K, i0, j0, a1, (unsigned *)lds);             // P_ij = E_i H_j^T
+  // epilogue of k_pandya_fused: D_ij = hh P_ij - eh P_ji^T, P_ji through LDS
+  double *Tt = (double *)lds;  // [T][LDE]
+  __syncthreads();
+#pragma unroll
+  for (int x = 0; x < 2; x++)
+#pragma unroll
+    for (int y = 0; y < 2; y++)
+#pragma unroll
+      for (int v = 0; v < 4; v++)
+        Tt[(m0 + x * 16 + lr) * LDE + n0 + y * 16 + 4 * lk + v] = diag ? a1[x][y][v] : a2[x][y][v];
+  __syncthreads();
+#pragma unroll
+  for (int x = 0; x < 2; x++)
+#pragma unroll
+    for (int y = 0; y < 2; y++)
+#pragma unroll
+      for (int v = 0; v < 4; v++) {
+        const int lx = m0 + x * 16 + lr, ly = n0 + y * 16 + 4 * lk + v;
+        if (i0 + lx < r && j0 + ly < r) Dq[(i0 + lx) + (int64_t)(j0 + ly) * r] = hh * a1[x][y][v] - eh * Tt[ly * LDE + lx];
+      }
+}
+
+void launch_pandya_fused(hipStream_t s, const DevTab &t, const double *etacc, const double *hcc, double *D,
+                         const TilePair *tp, int ntp, double hh, double eh, int prec) {
+  if (ntp <= 0) return;
+  switch (prec) {
+    case GEMM_FP64: hipLaunchKernelGGL(k_pandya_fused, dim3(ntp), dim3(256), 0, s, t, etacc, hcc, D, tp, hh, eh); break;
+    case GEMM_BF16X2:
+      hipLaunchKernelGGL(k_pandya_fused_bf16<2>, dim3(ntp), dim3(256), 0, s, t, etacc, hcc, D, tp, hh, eh);
+      break;
+    case GEMM_BF16X3:
+      hipLaunchKernelGGL(k_pandya_fused_bf16<3>, dim3(ntp), dim3(256), 0, s, t, etacc, hcc, D, tp, hh, eh);
+      break;
+    case GEMM_BF16X4:
+      hipLaunchKernelGGL(k_pandya_fused_bf16<4>, dim3(ntp), dim3(256), 0, s, t, etacc, hcc, D, tp, hh, eh);
+      break;
+    default: break;
+  }
+}
+
+bool pandya_fused_prec(int prec) { return prec == GEMM_FP64 || prec >= GEMM_BF16X2; }
 
 // Small products (the one-body blocks of commutator_111, K <= a few hundred):
 // one 16 x 16 output tile per work-group and one output per thread, K staged

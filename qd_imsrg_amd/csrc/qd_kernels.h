@@ -187,9 +187,12 @@ void launch_pandya_combine(hipStream_t s, const DevTab &t, double *P, const Tile
                            double eh);
 // D = hh * E H^T - eh * (E H^T)^T per CC block straight from the CC sets
 // (E = ETACC, H = HCC), over 64x64 tile pairs (ti <= tj); only the upper
-// tiles of D are written (the lower ones are -hh * eh times their transpose)
+// tiles of D are written (the lower ones are -hh * eh times their transpose).
+// E H^T in FP64 or in BF16 slices (prec); pandya_fused_prec tells which
+// precisions have the fused pass (FP32 keeps product GEMM + combine).
 void launch_pandya_fused(hipStream_t s, const DevTab &t, const double *etacc, const double *hcc, double *D,
-                         const TilePair *tp, int ntp, double hh, double eh);
+                         const TilePair *tp, int ntp, double hh, double eh, int prec);
+bool pandya_fused_prec(int prec);
 void launch_axpby(hipStream_t s, int64_t n, double s1, const double *x, double s2, const double *y, double *z);
 // sum of squares of v[1..neq) -> *dst, via partials (deterministic)
 void launch_norm2(hipStream_t s, const double *v, int64_t neq, double *partials, double *dst);

@@ -101,3 +101,44 @@ def test_magnus_flow_errors_per_precision():
     for p in ("fp32", "bf16x3", "bf16x4"):
         de, dom, dod, _ = out[p]
         assert de <= BOUND_E and dom <= 1e-4 and dod <= 1e-5, (p, out[p])
+
+
+def test_fused_bf16_pandya_matches_product_and_combine():
+    """In the grouped form the BF16-slice Pandya product runs fused with its
+    combine (upper tiles of D only); the per-channel form keeps the product
+    GEMM + combine pass.  Both cut the same slices and sum the same slice
+    products in the same order (K stages of 32 from k = 0, FP32 within a
+    stage, FP64 across stages), so the results are bitwise equal, and both
+    stay inside the oracle bound, with the Pandya term on (commutator_cc:
+    calc_cc-consistent sets) and with caller-supplied CC sets."""
+    print("[case] fused BF16 Pandya vs product + combine, N=20 R=14", flush=True)
+    ch = Channels(20, 14)
+    ctx = qd.Context(20, 14)
+    rng = np.random.default_rng(43)
+    x, y = random_flat(ch, rng, -1), random_flat(ch, rng, 1)
+    oracle = Oracle(ch).commutator(x, -1, y, 1)
+    X, Y = ctx.operator(-1, x), ctx.operator(1, y)
+
+    def both(mode, p):
+        ctx.set_launch_mode(mode)
+        ctx.set_precision(p)
+        Z1, Z2 = ctx.operator(1), ctx.operator(1)
+        HCC, ECC = ctx.cc(1.0), ctx.cc(-1.0)
+        qd.calc_cc(Y, HCC)
+        qd.calc_cc(X, ECC)
+        qd.commutator(X, Y, Z1, HCC, ECC)
+        qd.commutator_cc(X, Y, Z2, ctx.cc(1.0), ctx.cc(-1.0))
+        return Z1.download(), Z2.download()
+
+    for p in ("bf16x2", "bf16x3", "bf16x4"):
+        f1, f2 = both(0, p)
+        u1, u2 = both(1, p)
+        d1, d2 = rel(f1, u1), rel(f2, u2)
+        print(f"  {p}: fused vs unfused {d1:.2e} / {d2:.2e} (bitwise {np.array_equal(f1, u1)}), "
+              f"vs oracle {rel(f1, oracle):.2e}", flush=True)
+        assert np.array_equal(f1, u1) and np.array_equal(f2, u2), (p, d1, d2)
+        for z in (f1, f2, u1, u2):
+            assert rel(z, oracle) <= BOUND_COMM[p], p
+    ctx.set_launch_mode(0)
+    ctx.set_precision("fp64")
+    ctx.close()
