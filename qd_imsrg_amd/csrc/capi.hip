@@ -106,6 +106,12 @@ struct qd_context {
   double *fcls = nullptr;                                            // channel blocks of f (2 operands)
   TilePair *ptp = nullptr; int nptp = 0;  // 32x32 tile pairs of the Pandya combine
   TilePair *ftp = nullptr; int nftp = 0;  // 64x64 tile pairs of the fused Pandya product + combine
+  // BF16 slice planes of the two CC sets for the fused split-precision Pandya
+  // pass: per CC block, units of 8 consecutive k of one row (16 bytes per
+  // slice), unit (k / 8, x) at cs_off[q] / 8 + (k / 8) r + x
+  int64_t *cs_off = nullptr; int64_t cs_len = 0;  // per-block offsets (bf16 elements), plane length
+  int2 *cs_units = nullptr; int ncs_units = 0;    // (block, first unit) of each slicing work-group
+  unsigned short *cs = nullptr;                   // E planes then H planes, 4 each (on first BF16 use)
   TilePair *lsq = nullptr; int nlsq = 0;  // 32x32 tile pairs of the square sub-blocks' ladder terms
   double *wl = nullptr;                   // their ladder terms (square sub-blocks of the layout)
   Seg2 *pphh_segs = nullptr; Tile2 *pphh_tiles = nullptr; int npphh_segs = 0, npphh_tiles = 0;
@@ -705,6 +711,20 @@ int build_worklists(qd_context *c) {
     }
     if ((rc = dev_upload(c, ftp, &c->ftp))) return rc;
     c->nftp = (int)ftp.size();
+    std::vector<int64_t> soff(ch.ncc, 0);
+    std::vector<int2> units;
+    int64_t len = 0;
+    for (int q = 0; q < ch.ncc; q++) {
+      soff[q] = len;
+      const int r = ch.cc_r[q], nb = ch.cc_nb[q];
+      if (r * nb == 0) continue;
+      const int64_t nu = (int64_t)((nb + 7) / 8) * r;
+      for (int64_t u0 = 0; u0 < nu; u0 += CS_UNITS_WG) units.push_back(make_int2(q, (int)u0));
+      len += 8 * nu;
+    }
+    if ((rc = dev_upload(c, soff, &c->cs_off)) || (rc = dev_upload(c, units, &c->cs_units))) return rc;
+    c->cs_len = len;
+    c->ncs_units = (int)units.size();
   }
   {  // square sub-blocks (HHHH, PHPH, PPPP) of every block, largest first
     std::vector<std::pair<int64_t, int>> order;
@@ -879,7 +899,8 @@ int run_products(qd_context *c, const qd_op *r1, const qd_op *r2, qd_op *out, co
     int rc = run_gemm_list(c, c->g_all, pd && !fused, s);
     if (rc) return rc;
     if (fused) {
-      launch_pandya_fused(c->stream, c->dt, ecc->d, hcc->d, c->P, c->ftp, c->nftp, hcc->herm, ecc->herm, c->gemm_prec);
+      launch_pandya_fused(c->stream, c->dt, ecc->d, hcc->d, c->P, c->ftp, c->nftp, hcc->herm, ecc->herm, c->gemm_prec,
+                        CcSlices{c->cs, c->cs_len, c->cs_off, c->cs_units, c->ncs_units});
       HIPC(hipGetLastError());
       c->d_sign = -hcc->herm * ecc->herm;
     }
@@ -1020,7 +1041,8 @@ int full_commutator(qd_context *c, const qd_op *r1, const qd_op *r2, qd_op *r3, 
     if ((rc = run_gemm_list(c, c->g_all, false, s))) return rc;  // ladder (and one-body) products
     if (side) run_ladder_sq(c, r1, r2);  // while the CC fills finish on the side stream
     if (side) HIPC(hipStreamWaitEvent(main, c->ev_cc, 0));
-    launch_pandya_fused(c->stream, c->dt, ecc->d, hcc->d, c->P, c->ftp, c->nftp, hcc->herm, ecc->herm, c->gemm_prec);
+    launch_pandya_fused(c->stream, c->dt, ecc->d, hcc->d, c->P, c->ftp, c->nftp, hcc->herm, ecc->herm, c->gemm_prec,
+                        CcSlices{c->cs, c->cs_len, c->cs_off, c->cs_units, c->ncs_units});
     HIPC(hipGetLastError());
     c->d_sign = -hcc->herm * ecc->herm;
   } else {
@@ -2144,8 +2166,14 @@ int qd_set_launch_mode(qd_context *c, int mode) {
 int qd_set_precision(qd_context *c, int prec) {
   if (!c || prec < QD_PREC_FP64 || prec > QD_PREC_BF16X4)
     return fail(QD_E_ARG, "precision must be QD_PREC_FP64, QD_PREC_FP32 or QD_PREC_BF16X2..4");
+  int rc;
+  if ((rc = drop_graphs(c))) return rc;
+  if (prec >= GEMM_BF16X2 && !c->cs) {  // slice planes of the fused BF16 Pandya pass
+    void *p = nullptr;
+    if ((rc = dev_alloc(c, sizeof(unsigned short) * 2 * 4 * std::max<int64_t>(c->cs_len, 8), &p))) return rc;
+    c->cs = (unsigned short *)p;
+  }
   c->gemm_prec = prec;  // QD_PREC_* equal the GemmPrec values
-  drop_graphs(c);
   return QD_OK;
 }
 

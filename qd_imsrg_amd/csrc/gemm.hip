@@ -786,87 +786,108 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) k
 
 // The same fused pass with P = E H^T in BF16 slices (split precision S,
 // numerics of gemm_tile_bf16: K stages of 32 from k = 0, each stage's slice
-// products summed in FP32 smallest first, stage sums added into FP64).  The
-// two products of a tile pair run one after the other, each with its own K
-// walk over two staged panels (E rows, H rows), so only one product's
-// operands and slices are live at a time; P_ji stays in registers until the
-// epilogue.  The MFMAs take the H slices first, so they hold each 16 x 16
-// block transposed: lane l, component v is row l % 16, column 4 (l / 16) + v.
+// products summed in FP32 smallest first, stage sums added into FP64).
+// Both CC sets are first cut into slice planes by k_cc_slices (one pass over
+// each set), so the tiles, which read every E and H row panel once per tile
+// pair it belongs to, stage ready slices: one 16-byte load per slice, row and
+// 8 k, no slicing arithmetic.  The two products of a tile pair run one after
+// the other, each with its own K walk over two staged panels (E rows, H
+// rows); P_ji stays in registers until the epilogue.  The MFMAs take the H
+// slices first, so they hold each 16 x 16 block transposed: lane l,
+// component v is row l % 16, column 4 (l / 16) + v.
 template <int S>
-__device__ __forceinline__ void pandya_split_product(const double *__restrict__ E, const double *__restrict__ H, int r,
-                                                     int K, int e0, int h0, d4 (&acc)[2][2], unsigned *__restrict__ lds) {
-  constexpr int T = 64, KP = SPLIT_TK / 2, NU = T * KP / 256;
+__global__ void __launch_bounds__(256) k_cc_slices(DevTab t, const double *__restrict__ cc,
+                                                   unsigned short *__restrict__ planes, int64_t len,
+                                                   const int64_t *__restrict__ off, const int2 *__restrict__ units) {
+  const int2 w = units[blockIdx.x];
+  const int q = w.x, r = t.cc_r[q], K = t.cc_nb[q];
+  const int64_t u = (int64_t)w.y + threadIdx.x;
+  if (u >= (int64_t)((K + 7) / 8) * r) return;
+  const int kb = (int)(u / r), x = (int)(u % r);
+  const double *__restrict__ src = cc + t.cc_off[q] + x;
+  double v[8];
+#pragma unroll
+  for (int e = 0; e < 8; e++) v[e] = 8 * kb + e < K ? src[(int64_t)(8 * kb + e) * r] : 0.0;
+  unsigned wv[S][4];
+#pragma unroll
+  for (int e = 0; e < 4; e++) {
+    unsigned h0[S], h1[S];
+    bf16_slices<S>(v[2 * e], h0);
+    bf16_slices<S>(v[2 * e + 1], h1);
+#pragma unroll
+    for (int sl = 0; sl < S; sl++) wv[sl][e] = h0[sl] | (h1[sl] << 16);
+  }
+#pragma unroll
+  for (int sl = 0; sl < S; sl++)
+    *(u4 *)(planes + sl * len + off[q] + 8 * u) = u4{wv[sl][0], wv[sl][1], wv[sl][2], wv[sl][3]};
+}
+
+template <int S>
+__device__ __forceinline__ void pandya_split_product(const unsigned short *__restrict__ Ep,
+                                                     const unsigned short *__restrict__ Hp, int64_t len, int r, int K,
+                                                     int e0, int h0, d4 (&acc)[2][2], unsigned *__restrict__ lds) {
+  constexpr int T = 64;
   unsigned *Es = lds;                       // [S][T][SPLIT_LDW]: slice s of E(e0 + x, k), k pairs packed
   unsigned *Hs = lds + S * T * SPLIT_LDW;   // [S][T][SPLIT_LDW]: slice s of H(h0 + x, k)
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int lr = lane & 15, lk = lane >> 4;
   const int m0 = (wave & 1) * 32, n0 = (wave >> 1) * 32;
-  double re[NU][2], rh[NU][2];  // pairs of consecutive k
+  const int x = tid & 63, j = tid >> 6, nkb = (K + 7) / 8;  // staging: row x, k unit j of the stage
+  u4 re[S], rh[S];
   auto load = [&](int k0) {
+    const int kb = k0 / 8 + j;
+    const bool ein = kb < nkb && e0 + x < r, hin = kb < nkb && h0 + x < r;
 #pragma unroll
-    for (int u = 0; u < NU; u++) {
-      const int idx = tid + 256 * u, x = idx % T, gk = k0 + 2 * (idx / T);
-#pragma unroll
-      for (int h = 0; h < 2; h++) {
-        const bool kin = gk + h < K;
-        re[u][h] = (kin && e0 + x < r) ? E[(e0 + x) + (int64_t)(gk + h) * r] : 0.0;
-        rh[u][h] = (kin && h0 + x < r) ? H[(h0 + x) + (int64_t)(gk + h) * r] : 0.0;
-      }
+    for (int sl = 0; sl < S; sl++) {
+      re[sl] = ein ? *(const u4 *)(Ep + sl * len + 8 * ((int64_t)kb * r + e0 + x)) : u4{0, 0, 0, 0};
+      rh[sl] = hin ? *(const u4 *)(Hp + sl * len + 8 * ((int64_t)kb * r + h0 + x)) : u4{0, 0, 0, 0};
     }
   };
   load(0);
   for (int k0 = 0; k0 < K; k0 += SPLIT_TK) {
     __syncthreads();
 #pragma unroll
-    for (int u = 0; u < NU; u++) {
-      const int idx = tid + 256 * u, x = idx % T, kp = idx / T;
-      unsigned a0[S], a1[S], b0[S], b1[S];
-      bf16_slices<S>(re[u][0], a0);
-      bf16_slices<S>(re[u][1], a1);
-      bf16_slices<S>(rh[u][0], b0);
-      bf16_slices<S>(rh[u][1], b1);
-#pragma unroll
-      for (int q = 0; q < S; q++) {
-        Es[(q * T + x) * SPLIT_LDW + kp] = a0[q] | (a1[q] << 16);
-        Hs[(q * T + x) * SPLIT_LDW + kp] = b0[q] | (b1[q] << 16);
-      }
+    for (int sl = 0; sl < S; sl++) {
+      *(u4 *)(Es + (sl * T + x) * SPLIT_LDW + 4 * j) = re[sl];
+      *(u4 *)(Hs + (sl * T + x) * SPLIT_LDW + 4 * j) = rh[sl];
     }
     __syncthreads();
     if (k0 + SPLIT_TK < K) load(k0 + SPLIT_TK);
     bf8 a[S][2], b[S][2];
 #pragma unroll
-    for (int q = 0; q < S; q++)
+    for (int sl = 0; sl < S; sl++)
 #pragma unroll
-      for (int x = 0; x < 2; x++) {
-        a[q][x] = __builtin_bit_cast(bf8, *(const u4 *)(Es + (q * T + m0 + x * 16 + lr) * SPLIT_LDW + 4 * lk));
-        b[q][x] = __builtin_bit_cast(bf8, *(const u4 *)(Hs + (q * T + n0 + x * 16 + lr) * SPLIT_LDW + 4 * lk));
+      for (int y = 0; y < 2; y++) {
+        a[sl][y] = __builtin_bit_cast(bf8, *(const u4 *)(Es + (sl * T + m0 + y * 16 + lr) * SPLIT_LDW + 4 * lk));
+        b[sl][y] = __builtin_bit_cast(bf8, *(const u4 *)(Hs + (sl * T + n0 + y * 16 + lr) * SPLIT_LDW + 4 * lk));
       }
 #pragma unroll
-    for (int x = 0; x < 2; x++)
+    for (int xx = 0; xx < 2; xx++)
 #pragma unroll
       for (int y = 0; y < 2; y++) {
         f4 c = f4{0, 0, 0, 0};
 #pragma unroll
         for (int d = S - 1; d >= 0; d--)
 #pragma unroll
-          for (int i = 0; i <= d; i++) c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(b[d - i][y], a[i][x], c, 0, 0, 0);
+          for (int i = 0; i <= d; i++) c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(b[d - i][y], a[i][xx], c, 0, 0, 0);
 #pragma unroll
-        for (int v = 0; v < 4; v++) acc[x][y][v] += (double)c[v];
+        for (int v = 0; v < 4; v++) acc[xx][y][v] += (double)c[v];
       }
   }
 }
 
 template <int S>
-__global__ void __launch_bounds__(256) k_pandya_fused_bf16(DevTab t, const double *__restrict__ E,
-                                                           const double *__restrict__ H, double *__restrict__ D,
-                                                           const TilePair *__restrict__ tp, double hh, double eh) {
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(S <= 3 ? 3 : 2)))
+k_pandya_fused_bf16(DevTab t, const unsigned short *__restrict__ planes,
+                                                           int64_t len, const int64_t *__restrict__ off,
+                                                           double *__restrict__ D, const TilePair *__restrict__ tp,
+                                                           double hh, double eh) {
   constexpr int T = 64, LDE = T + 1;
   constexpr int NSTAGE = 2 * S * T * SPLIT_LDW * 4, NEPI = T * LDE * 8;
   __shared__ __align__(16) unsigned char lds[NSTAGE > NEPI ? NSTAGE : NEPI];
   const TilePair pr = tp[blockIdx.x];
   const int q = pr.blk, r = t.cc_r[q], K = t.cc_nb[q];
-  const double *__restrict__ Eq = E + t.cc_off[q];
-  const double *__restrict__ Hq = H + t.cc_off[q];
+  const unsigned short *Ep = planes + off[q], *Hp = planes + 4 * len + off[q];
   double *__restrict__ Dq = D + t.cc_poff[q];
   const bool diag = pr.ti == pr.tj;
   const int i0 = pr.ti * T, j0 = pr.tj * T;
@@ -878,8 +899,8 @@ __global__ void __launch_bounds__(256) k_pandya_fused_bf16(DevTab t, const doubl
   for (int x = 0; x < 2; x++)
 #pragma unroll
     for (int y = 0; y < 2; y++) a1[x][y] = a2[x][y] = d4{0, 0, 0, 0};
-  if (!diag) pandya_split_product<S>(Eq, Hq, r, K, j0, i0, a2, (unsigned *)lds);  // P_ji = E_j H_i^T
-  pandya_split_product<S>(Eq, Hq, r, K, i0, j0, a1, (unsigned *)lds);             // P_ij = E_i H_j^T
+  if (!diag) pandya_split_product<S>(Ep, Hp, len, r, K, j0, i0, a2, (unsigned *)lds);  // P_ji = E_j H_i^T
+  pandya_split_product<S>(Ep, Hp, len, r, K, i0, j0, a1, (unsigned *)lds);             // P_ij = E_i H_j^T
   // epilogue of k_pandya_fused: D_ij = hh P_ij - eh P_ji^T, P_ji through LDS
   double *Tt = (double *)lds;  // [T][LDE]
   __syncthreads();
@@ -902,20 +923,26 @@ __global__ void __launch_bounds__(256) k_pandya_fused_bf16(DevTab t, const doubl
       }
 }
 
+template <int S>
+static void launch_pandya_bf16(hipStream_t s, const DevTab &t, const double *etacc, const double *hcc, double *D,
+                               const TilePair *tp, int ntp, double hh, double eh, const CcSlices &cs) {
+  if (cs.nunits > 0) {
+    hipLaunchKernelGGL(k_cc_slices<S>, dim3(cs.nunits), dim3(CS_UNITS_WG), 0, s, t, etacc, cs.planes, cs.len, cs.off,
+                       cs.units);
+    hipLaunchKernelGGL(k_cc_slices<S>, dim3(cs.nunits), dim3(CS_UNITS_WG), 0, s, t, hcc, cs.planes + 4 * cs.len,
+                       cs.len, cs.off, cs.units);
+  }
+  hipLaunchKernelGGL(k_pandya_fused_bf16<S>, dim3(ntp), dim3(256), 0, s, t, cs.planes, cs.len, cs.off, D, tp, hh, eh);
+}
+
 void launch_pandya_fused(hipStream_t s, const DevTab &t, const double *etacc, const double *hcc, double *D,
-                         const TilePair *tp, int ntp, double hh, double eh, int prec) {
+                         const TilePair *tp, int ntp, double hh, double eh, int prec, const CcSlices &cs) {
   if (ntp <= 0) return;
   switch (prec) {
     case GEMM_FP64: hipLaunchKernelGGL(k_pandya_fused, dim3(ntp), dim3(256), 0, s, t, etacc, hcc, D, tp, hh, eh); break;
-    case GEMM_BF16X2:
-      hipLaunchKernelGGL(k_pandya_fused_bf16<2>, dim3(ntp), dim3(256), 0, s, t, etacc, hcc, D, tp, hh, eh);
-      break;
-    case GEMM_BF16X3:
-      hipLaunchKernelGGL(k_pandya_fused_bf16<3>, dim3(ntp), dim3(256), 0, s, t, etacc, hcc, D, tp, hh, eh);
-      break;
-    case GEMM_BF16X4:
-      hipLaunchKernelGGL(k_pandya_fused_bf16<4>, dim3(ntp), dim3(256), 0, s, t, etacc, hcc, D, tp, hh, eh);
-      break;
+    case GEMM_BF16X2: launch_pandya_bf16<2>(s, t, etacc, hcc, D, tp, ntp, hh, eh, cs); break;
+    case GEMM_BF16X3: launch_pandya_bf16<3>(s, t, etacc, hcc, D, tp, ntp, hh, eh, cs); break;
+    case GEMM_BF16X4: launch_pandya_bf16<4>(s, t, etacc, hcc, D, tp, ntp, hh, eh, cs); break;
     default: break;
   }
 }

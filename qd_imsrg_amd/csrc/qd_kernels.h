@@ -189,9 +189,20 @@ void launch_pandya_combine(hipStream_t s, const DevTab &t, double *P, const Tile
 // (E = ETACC, H = HCC), over 64x64 tile pairs (ti <= tj); only the upper
 // tiles of D are written (the lower ones are -hh * eh times their transpose).
 // E H^T in FP64 or in BF16 slices (prec); pandya_fused_prec tells which
-// precisions have the fused pass (FP32 keeps product GEMM + combine).
+// precisions have the fused pass (FP32 keeps product GEMM + combine).  The
+// BF16 pass first cuts both CC sets into slice planes (CcSlices): per CC
+// block, units of 8 consecutive k of one row, zero past K, unit (k / 8, x)
+// of slice s at planes + s * len + off[q] + 8 ((k / 8) r + x).
+constexpr int CS_UNITS_WG = 256;  // slicing work-group: 256 units of one block
+struct CcSlices {
+  unsigned short *planes;  // E slices 0..3, then H slices 0..3, len elements each
+  int64_t len;
+  const int64_t *off;      // per CC block
+  const int2 *units;       // (block, first unit) per slicing work-group
+  int nunits;
+};
 void launch_pandya_fused(hipStream_t s, const DevTab &t, const double *etacc, const double *hcc, double *D,
-                         const TilePair *tp, int ntp, double hh, double eh, int prec);
+                         const TilePair *tp, int ntp, double hh, double eh, int prec, const CcSlices &cs);
 bool pandya_fused_prec(int prec);
 void launch_axpby(hipStream_t s, int64_t n, double s1, const double *x, double s2, const double *y, double *z);
 // sum of squares of v[1..neq) -> *dst, via partials (deterministic)
