@@ -47,9 +47,9 @@ __device__ __forceinline__ int box_of(const int *pre, int lo, int ng, int e) {
   return lo;
 }
 
-// floor(a / d) for 0 <= a < 2^12 and 1 <= d < 2^12 from rd = 1.0f / d: the
-// product's error (< 2^-11) stays below the distance 0.5 / d of (a + 0.5) / d
-// from the nearest integer
+// floor(a / d) for 0 <= a < 2^22 and 1 <= d < 2^12 from rd = 1.0f / d: the
+// product's relative error (< 2^-23) times (a + 0.5) / d stays below the
+// distance 0.5 / d of (a + 0.5) / d from the nearest integer
 __device__ __forceinline__ int qdiv(int a, float rd) { return (int)(((float)a + 0.5f) * rd); }
 
 // NT threads per task, grids of up to VC elements, RC row grid entries
@@ -69,9 +69,8 @@ __global__ void __launch_bounds__(NT) k_box2b(DevTab t, Ops o, double *__restric
   __shared__ uint32_t cent[BX_CMAX];           // grid entries of the task's columns
   __shared__ int64_t soff[3][3];               // stored sub-block of (row type, column type), row >= column
   __shared__ float2 brc[BX_GMAX];              // 1 / rows in the task, 1 / |A2|
-  __shared__ int64_t pb[BX_GMAX][4];           // Pandya: offset of each term's D block (-1: no D block)
-  __shared__ int pl[BX_GMAX][4];               // Pandya: leading dimension (rows) of each term's D block
-  __shared__ int2 prc[BX_GMAX][4];             // Pandya: first row and column of each term's sub-block
+  __shared__ int64_t pb[BX_GMAX][4];           // Pandya: offset in D of each term's stored sub-block (-1: none)
+  __shared__ int pl[BX_GMAX][4];               // Pandya: its leading dimension | stored transposed << 30
   __shared__ int2 bfo[BX_GMAX];                // offsets of the box's A1 and A2 blocks in F
   __shared__ int fsz_s;                        // f entries staged (A3, A4 and every box's A1, A2)
   __shared__ int bgo[BX_GMAX];                 // first row grid entry of the box in cp_grid
@@ -145,19 +144,38 @@ __global__ void __launch_bounds__(NT) k_box2b(DevTab t, Ops o, double *__restric
     }
   } else if (dopd && (int)threadIdx.x - 64 < ng) {
     // wave 1, one lane per box: D(x3,x1|x2,x4), D(x3,x2|x1,x4),
-    // D(x4,x2|x1,x3), D(x4,x1|x2,x3): device rows of a CC block are ordered
-    // by the channel of the first state, so each term's box is a dense
-    // sub-block at a fixed offset
+    // D(x4,x2|x1,x3), D(x4,x1|x2,x3): device rows of a CC block come in
+    // runs by the channel of the first state, so each term's box is the
+    // dense sub-block of two runs (rows: channel rc0 with partner rc1,
+    // columns: cc0 with partner cc1).  In run panels (dsign != 0) the
+    // sub-block is contiguous, the stored one being the transposed sub-block
+    // when the row run starts after the column run; whole column-major D
+    // (dsign == 0) has leading dimension r
     const int b = threadIdx.x - 64;
     const int g = blist[tk.first + b] & ~BOX_PAIRED;
     const int A1 = t.cp_a[g], A2 = t.cp_b[g];
-    const int rc0[4] = {A3, A3, A4, A4}, rc1[4] = {A1, A2, A2, A1}, cc0[4] = {A2, A1, A1, A2};
+    const int rc0[4] = {A3, A3, A4, A4}, rc1[4] = {A1, A2, A2, A1}, cc0[4] = {A2, A1, A1, A2},
+              cc1[4] = {A4, A4, A3, A3};
+    auto csz = [&](int h) { return t.ch_start[h + 1] - t.ch_start[h]; };
     for (int k = 0; k < 4; k++) {
       const int x = t.ch_states[t.ch_start[rc0[k]]], y = t.ch_states[t.ch_start[rc1[k]]];
-      const int K = t.cc_map[x * t.M + y].x;
-      pl[b][k] = t.cc_r[K];
-      pb[b][k] = t.cc_nb[K] == 0 ? -1 : t.cc_poff[K];
-      prc[b][k] = make_int2(t.cc_rs[K * t.nch + rc0[k]], t.cc_rs[K * t.nch + cc0[k]]);
+      const int K = t.cc_map[x * t.M + y].x, r = t.cc_r[K];
+      const int Lr = t.cc_rs[K * t.nch + rc0[k]], Lc = t.cc_rs[K * t.nch + cc0[k]];
+      const int lr = csz(rc0[k]) * csz(rc1[k]), lc = csz(cc0[k]) * csz(cc1[k]);
+      const int64_t po = t.cc_poff[K];
+      if (t.cc_nb[K] == 0) {
+        pb[b][k] = -1;
+        pl[b][k] = 0;
+      } else if (dsign == 0.0) {
+        pb[b][k] = po + Lr + (int64_t)Lc * r;
+        pl[b][k] = r;
+      } else if (Lr <= Lc) {
+        pb[b][k] = po + (int64_t)Lr * r + (int64_t)Lc * lr;
+        pl[b][k] = lr;
+      } else {
+        pb[b][k] = po + (int64_t)Lc * r + (int64_t)Lr * lc;
+        pl[b][k] = lc | (1 << 30);
+      }
     }
   }
   __syncthreads();
@@ -176,6 +194,63 @@ __global__ void __launch_bounds__(NT) k_box2b(DevTab t, Ops o, double *__restric
   }
   __syncthreads();
   const int total = pre[ng], fsz = fsz_s;
+  const float rs4 = 1.0f / (float)s4;
+  // value of Pandya term k at (row a, column c) of its sub-block, the
+  // sub-block at src (global D or its staged copy in LDS)
+  auto dterm = [&](const double *__restrict__ src, int b, int k, int a, int c) -> double {
+    const int w = pl[b][k], ld = w & 0x3fffffff;
+    const bool tr = (w >> 30) != 0;
+    return (tr ? dsign : 1.0) * src[tr ? c + (int64_t)a * ld : a + (int64_t)c * ld];
+  };
+  // Pandya term of whole boxes with D in run panels: every term's sub-block
+  // is contiguous, so it is staged in LDS with coalesced loads (two terms at
+  // a time, in the grid's LDS before the grids), and each thread keeps the
+  // sum for its grid elements e = tid + k NT, in order, in registers
+  constexpr int KQ = (VC + NT - 1) / NT;
+  const bool stage_d = dopd && mode == 0 && dsign != 0.0 && (flags & TB_DSTAGE);
+  double pq[KQ];
+#pragma unroll
+  for (int k = 0; k < KQ; k++) pq[k] = 0.0;
+  if (stage_d) {
+    double *__restrict__ gd = (double *)gg;  // [2][total]
+    for (int h = 0; h < 2; h++) {
+#pragma unroll
+      for (int w = 0; w < 2; w++) {
+        const int k = 2 * h + w;
+        for (int j0 = threadIdx.x, b = 0; j0 < total; j0 += 4 * NT) {
+          double v[4];
+#pragma unroll
+          for (int u = 0; u < 4; u++) {
+            const int j = min(j0 + NT * u, total - 1);
+            if (b + 1 < ng && pre[b + 1] <= j) b = box_of(pre, b + 1, ng, j);
+            const int64_t base = pb[b][k];  // no D block: the zero at D[p_size]
+            v[u] = D[base < 0 ? t.p_size : base + (j - pre[b])];
+          }
+#pragma unroll
+          for (int u = 0; u < 4; u++)
+            if (j0 + NT * u < total) gd[w * total + j0 + NT * u] = v[u];
+        }
+      }
+      __syncthreads();
+      int b = 0;
+#pragma unroll
+      for (int k = 0; k < KQ; k++) {
+        const int e = threadIdx.x + k * NT;
+        if (e < total) {
+          if (b + 1 < ng && pre[b + 1] <= e) b = box_of(pre, b + 1, ng, e);
+          const int le = e - pre[b], c = qdiv(le, brc[b].x), r = le - c * bx[b].y;
+          const int s1 = bs[b].x, s2 = bs[b].y;
+          const int i1 = qdiv(r, brc[b].y), i2 = r - i1 * s2, i3 = qdiv(c, rs4), i4 = c - i3 * s4;
+          const double *__restrict__ g0 = gd + pre[b], *__restrict__ g1 = gd + total + pre[b];
+          if (h == 0)
+            pq[k] = dterm(g0, b, 0, i3 * s1 + i1, i2 * s4 + i4) - dterm(g1, b, 1, i3 * s2 + i2, i1 * s4 + i4);
+          else
+            pq[k] = (pq[k] + dterm(g0, b, 2, i4 * s2 + i2, i1 * s3 + i3)) - dterm(g1, b, 3, i4 * s1 + i1, i2 * s3 + i3);
+        }
+      }
+      __syncthreads();
+    }
+  }
   if (do122) {
     // channel blocks of f1 and f2 (A3, A4, then A1, A2 of every box) in one
     // flat pass, two entries per thread at a time with both loads issued
@@ -248,13 +323,16 @@ __global__ void __launch_bounds__(NT) k_box2b(DevTab t, Ops o, double *__restric
     __syncthreads();
   }
   const double h1 = o.h1, h2 = o.h2, hz = -h1 * h2;  // hz: herm of [r1, r2]
-  const float rs4 = 1.0f / (float)s4;
   const bool same34 = A3 == A4;
   // stored sub-block of (row type, column type), row type >= column type
   auto sub_of = [](int ta, int tb) {
     return tb == 0 ? (ta == 0 ? HHHH : (ta == 1 ? PHHH : PPHH)) : (tb == 1 ? (ta == 1 ? PHPH : PPPH) : PPPP);
   };
   for (int e = threadIdx.x, b = 0; e < total; e += NT) {
+    // this element's staged Pandya sum; the queue moves up by one
+    const double pdv = pq[0];
+#pragma unroll
+    for (int k = 0; k + 1 < KQ; k++) pq[k] = pq[k + 1];
     if (b + 1 < ng && pre[b + 1] <= e) b = box_of(pre, b + 1, ng, e);
     const int4 B = bx[b];
     const int le = e - pre[b], c = qdiv(le, brc[b].x), r = le - c * B.y;
@@ -283,26 +361,20 @@ __global__ void __launch_bounds__(NT) k_box2b(DevTab t, Ops o, double *__restric
     // the paths, and the compiler then drains all outstanding loads between
     // them.  A term without a D block reads the zero at D[p_size].
     double d0 = 0.0, d1 = 0.0, d2 = 0.0, d3 = 0.0;
-    if (dopd) {
-      // term k reads D(row, col) of its block: row and column within the
-      // block from the term's sub-block corner; with D kept as its upper
-      // 64 x 64 tiles a lower element comes from (col, row) times dsign
-      const int64_t z = t.p_size;
-      const bool tri = dsign != 0.0;
-      auto dget = [&](int k, int row, int col, double &sg) -> int64_t {
+    if (dopd && !stage_d) {
+      // column slabs and whole column-major D: each term read from global D
+      // (a term without a D block reads the zero at D[p_size])
+      auto dg = [&](int k, int a, int c) -> double {
         const int64_t base = pb[b][k];
-        const int r = row + prc[b][k].x, c = col + prc[b][k].y;
-        const bool lo = tri && (r >> 6) > (c >> 6);
-        sg = lo ? dsign : 1.0;
-        return base < 0 ? z : base + (lo ? c + (int64_t)r * pl[b][k] : r + (int64_t)c * pl[b][k]);
+        const int w = pl[b][k], ld = w & 0x3fffffff;
+        const bool tr = (w >> 30) != 0;
+        const int64_t at = base < 0 ? t.p_size : base + (tr ? c + (int64_t)a * ld : a + (int64_t)c * ld);
+        return (tr ? dsign : 1.0) * D[at];
       };
-      double g0, g1, g2, g3;
-      const int64_t a0 = dget(0, i3 * s1 + i1, i2 * s4 + i4, g0), a1 = dget(1, i3 * s2 + i2, i1 * s4 + i4, g1),
-                    a2 = dget(2, i4 * s2 + i2, i1 * s3 + i3, g2), a3 = dget(3, i4 * s1 + i1, i2 * s3 + i3, g3);
-      d0 = g0 * D[a0];
-      d1 = g1 * D[a1];
-      d2 = g2 * D[a2];
-      d3 = g3 * D[a3];
+      d0 = dg(0, i3 * s1 + i1, i2 * s4 + i4);
+      d1 = dg(1, i3 * s2 + i2, i1 * s4 + i4);
+      d2 = dg(2, i4 * s2 + i2, i1 * s3 + i3);
+      d3 = dg(3, i4 * s1 + i1, i2 * s3 + i3);
     }
     // ladder inputs (signed combinations of the products of matrix_mult222,
     // src/commutators.f90:999-1127, and the summed products for Vpphh,
@@ -360,7 +432,8 @@ __global__ void __launch_bounds__(NT) k_box2b(DevTab t, Ops o, double *__restric
         val += h2 * k1 - h1 * k2;
       }
     }
-    if (dopd) val += d0 - d1 + d2 - d3;
+    if (stage_d) val += pdv;
+    else if (dopd) val += d0 - d1 + d2 - d3;
     if (sr != sc) val = -val;
     if (!first) {  // ket slabs run before the main pass: the first writer of their outputs
       out[idx1] = o1 + val;

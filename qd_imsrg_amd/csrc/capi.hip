@@ -85,9 +85,10 @@ struct qd_context {
   hipStream_t stream = nullptr;
   // side stream (default priority) for the one-body and zero-body pieces of a
   // commutator, forked from and joined back to `stream` with these events
-  // layout of the Pandya product in P: 0 = whole D (combine pass), else D
-  // holds its upper 64 x 64 tiles and a lower element is this sign times the
-  // transposed one (fused kernel: -h_H h_eta)
+  // layout of the Pandya product in P: 0 = whole column-major D (combine
+  // pass), else D in run panels with the upper sub-blocks of runs only, a
+  // lower element being this sign times the transposed one (fused kernel:
+  // -h_H h_eta)
   double d_sign = 0.0;
   hipStream_t aux = nullptr;
   hipEvent_t ev_fork = nullptr, ev_join = nullptr, ev_cc = nullptr, ev_acc = nullptr;
@@ -103,6 +104,11 @@ struct qd_context {
   int box_rcap = BX_RCAP;                                            // row grid entries per box task
   bool box_wide = false;                                             // tasks for the wide box form
   bool box_pairs = true;                                             // paired box tasks where valid
+  // whole boxes stage their Pandya sub-blocks of D (contiguous in run panels)
+  // in LDS instead of reading each term from HBM: chosen with the wide box
+  // form (N=42 R=20 two-body class 2.39 -> 2.28 ms); in the narrow form the
+  // extra staging rounds cost more than the reads (N=20 R=14 0.202 -> 0.220)
+  bool d_stage = false;
   double *fcls = nullptr;                                            // channel blocks of f (2 operands)
   TilePair *ptp = nullptr; int nptp = 0;  // 32x32 tile pairs of the Pandya combine
   TilePair *ftp = nullptr; int nftp = 0;  // 64x64 tile pairs of the fused Pandya product + combine
@@ -954,8 +960,9 @@ int run_twobody(qd_context *c, const qd_op *r1, const qd_op *r2, qd_op *out, int
   if (part & TB_PART_MAIN) {
     if ((flags & TB_LADDER) && !lsq_done) run_ladder_sq(c, r1, r2);
     paired = paired && c->box_pairs;
-    launch_box2b(c->stream, c->dt, o, out->d, paired ? c->bt1p : c->bt1, paired ? c->nbt1p : c->nbt1, c->blist, flags,
-                 c->w1, c->w2, c->w3, c->wl, c->fcls, (flags & TB_PANDYA) ? c->P : nullptr, c->d_sign, c->box_wide);
+    launch_box2b(c->stream, c->dt, o, out->d, paired ? c->bt1p : c->bt1, paired ? c->nbt1p : c->nbt1, c->blist,
+                 flags | (c->d_stage ? TB_DSTAGE : 0), c->w1, c->w2, c->w3, c->wl, c->fcls,
+                 (flags & TB_PANDYA) ? c->P : nullptr, c->d_sign, c->box_wide);
   }
   HIPC(hipGetLastError());
   return QD_OK;
@@ -1221,7 +1228,7 @@ int qd_context_create(int nbody, int nshell, qd_context **out) {
   UP(ch.ml, ml); UP(ch.ms, ms); UP(ch.cc_r, cc_r); UP(ch.cc_nb, cc_nb); UP(ch.cc_off, cc_off);
   UP(ch.cc_poff, cc_poff); UP(ch.cc_ab_dev, cc_ab); UP(ch.cc_hp, cc_hp); UP(ch.cc_ab_start, cc_ab_start);
   UP(ch.cc_hp_start, cc_hp_start); UP(ccmap, cc_map);
-  UP(ch.cc_rs, cc_rs);
+  UP(ch.cc_rs, cc_rs); UP(ch.cc_run, cc_run);
   UP(ch.cp_a, cp_a); UP(ch.cp_b, cp_b); UP(ch.cp_grid_off, cp_grid_off); UP(ch.cp_grid, cp_grid);
   UP(ch.fcls_off, fcls_off);
   t.fcls_size = ch.fcls_size;
@@ -1271,6 +1278,8 @@ int qd_context_create(int nbody, int nshell, qd_context **out) {
         }
     c->box_wide = split > 0.1 * all;
     if (const char *env = getenv("QD_BOX_WIDE")) c->box_wide = env[0] == '1';
+    c->d_stage = c->box_wide;
+    if (const char *env = getenv("QD_DSTAGE")) c->d_stage = env[0] != '0';
     c->box_cap = c->box_wide ? BX_WIDE_VCAP : BX_VCAP;
     c->box_rcap = c->box_wide ? BX_WIDE_RCAP : BX_RCAP;
   }

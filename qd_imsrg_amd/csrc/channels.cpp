@@ -160,6 +160,18 @@ std::string build_channels(int nbody, int nshell, Channels &c) {
   c.cc_rs.assign((size_t)c.ncc * c.nch, -1);
   for (int b = 0; b < c.ncc; b++)
     for (int d = c.cc_r[b] - 1; d >= 0; d--) c.cc_rs[(size_t)b * c.nch + c.spch[lo16(c.cc_ab_dev[c.cc_ab_start[b] + d])]] = d;
+  // runs of the device row order: per row, the first row of its run (the rows
+  // whose first states share a channel) and the run's length
+  c.cc_run.assign(c.cc_ab_dev.size(), 0u);
+  for (int b = 0; b < c.ncc; b++) {
+    if (c.cc_r[b] > 65535) return "cross-coupled block too large for the run table";
+    for (int d = 0; d < c.cc_r[b]; d++) {
+      const uint32_t ab = c.cc_ab_dev[c.cc_ab_start[b] + d];
+      const int hi = c.spch[lo16(ab)], hj = c.spch[hi16(ab)];
+      const int len = (c.ch_start[hi + 1] - c.ch_start[hi]) * (c.ch_start[hj + 1] - c.ch_start[hj]);
+      c.cc_run[c.cc_ab_start[b] + d] = (uint32_t)c.cc_rs[(size_t)b * c.nch + hi] | ((uint32_t)len << 16);
+    }
+  }
   // the closed form of the device row, relied on by the two-body boxes
   for (int i = 0; i < M; i++)
     for (int j = 0; j < M; j++) {

@@ -672,11 +672,95 @@ void launch_gemm(hipStream_t s, const GemmProb *probs, const GemmSeg *segs, cons
 // and P_ji = E_j H_i^T (K staged through LDS in chunks of 8, the next chunk
 // prefetched into registers, k ascending as in the grouped GEMM, so P is the
 // same to the bit), then writes D_ij = h_H P_ij - h_eta P_ji^T, the
-// transposed tile exchanged through LDS.  D_ji = -h_H h_eta D_ij^T is not
-// written: D is kept as its upper 64 x 64 tiles (row tile <= column tile),
-// which halves its write, and the product matrix P is never written.
+// transposed tile exchanged through LDS.  The product matrix P is never
+// written.
+//
+// Layout of D ("run panels").  The device rows of a CC block come in runs,
+// the rows whose first states share a channel; the Pandya term of one output
+// box reads a dense (run x run) sub-block of D per term.  Each run of rows is
+// stored as its own column-major panel (run length) x r at (run start) * r,
+// so the sub-block of runs (a, b) is contiguous: element (x, y) at
+// a_start * r + b_start * a_len + (x - a_start) + (y - b_start) * a_len.
+// D^T = -h_H h_eta D, so only the sub-blocks with a_start <= b_start are
+// written (the two-body pass reads a lower one as that sign times the
+// transposed upper one), which halves D's write: the tile pair (ti < tj)
+// writes all of D_ij and, from D_ji = -h_H h_eta D_ij^T, the few elements of
+// the lower tile that lie in a diagonal sub-block (a run crossing the tile
+// boundary); a diagonal tile writes its elements with a_start <= b_start.
 // Four waves per SIMD (125 registers, 33 KB of LDS for the epilogue tile);
 // chunks of 16 needed 160 registers and ran three.
+//
+// VLK: the 16 x 16 blocks of a1, a2 are transposed P blocks with lane l,
+// component v at row l % 16 and column 4 (l / 16) + v (BF16 MFMA layout),
+// else column 4 v + l / 16 (FP64 MFMA layout).  lds: at least 64 x 65 doubles.
+template <bool VLK>
+__device__ __forceinline__ void store_d_tile(const DevTab &t, int q, double *__restrict__ Dq, int r, int i0, int j0,
+                                             bool diag, const d4 (&a1)[2][2], const d4 (&a2)[2][2],
+                                             double *__restrict__ Tt, double hh, double eh) {
+  constexpr int LDE = 65;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int lr = lane & 15, lk = lane >> 4;
+  const int m0 = (wave & 1) * 32, n0 = (wave >> 1) * 32;
+  auto col_of = [&](int y, int v) { return n0 + y * 16 + (VLK ? 4 * lk + v : 4 * v + lk); };
+  const uint32_t *__restrict__ runs = t.cc_run + t.cc_ab_start[q];
+  // run start | length << 16 of the tile's rows and columns held by this lane
+  uint32_t rrow[2], rcol[2][4];
+#pragma unroll
+  for (int x = 0; x < 2; x++) rrow[x] = i0 + m0 + x * 16 + lr < r ? runs[i0 + m0 + x * 16 + lr] : 0u;
+#pragma unroll
+  for (int y = 0; y < 2; y++)
+#pragma unroll
+    for (int v = 0; v < 4; v++) rcol[y][v] = j0 + col_of(y, v) < r ? runs[j0 + col_of(y, v)] : 0u;
+  __syncthreads();
+#pragma unroll
+  for (int x = 0; x < 2; x++)
+#pragma unroll
+    for (int y = 0; y < 2; y++)
+#pragma unroll
+      for (int v = 0; v < 4; v++) Tt[(m0 + x * 16 + lr) * LDE + col_of(y, v)] = diag ? a1[x][y][v] : a2[x][y][v];
+  __syncthreads();
+  double dv[2][2][4];
+#pragma unroll
+  for (int x = 0; x < 2; x++)
+#pragma unroll
+    for (int y = 0; y < 2; y++)
+#pragma unroll
+      for (int v = 0; v < 4; v++) {
+        const int lx = m0 + x * 16 + lr, ly = col_of(y, v);
+        dv[x][y][v] = hh * a1[x][y][v] - eh * Tt[ly * LDE + lx];
+        const int row = i0 + lx, col = j0 + ly;
+        const int L = (int)(rrow[x] & 0xffffu), l = (int)(rrow[x] >> 16);
+        if (row < r && col < r && L <= (int)(rcol[y][v] & 0xffffu))
+          Dq[(int64_t)L * r + (int64_t)col * l + (row - L)] = dv[x][y][v];
+      }
+  // rows of tile tj in a run that starts inside tile ti: the lower tile's
+  // elements of diagonal sub-blocks, D(j0 + a, i0 + b) = -hh eh D_ij(b, a)
+  if (diag || j0 >= r || (int)(runs[j0] & 0xffffu) > i0 + 63) return;
+  __syncthreads();
+#pragma unroll
+  for (int x = 0; x < 2; x++)
+#pragma unroll
+    for (int y = 0; y < 2; y++)
+#pragma unroll
+      for (int v = 0; v < 4; v++) Tt[(m0 + x * 16 + lr) * LDE + col_of(y, v)] = dv[x][y][v];
+  __syncthreads();
+  const double ds = -hh * eh;
+#pragma unroll
+  for (int x = 0; x < 2; x++)
+#pragma unroll
+    for (int y = 0; y < 2; y++)
+#pragma unroll
+      for (int v = 0; v < 4; v++) {
+        const int a = m0 + x * 16 + lr, b = col_of(y, v), row = j0 + a, col = i0 + b;
+        if (row < r && col < r) {
+          const uint32_t ru = runs[row];
+          const int L = (int)(ru & 0xffffu), l = (int)(ru >> 16);
+          if (L == (int)(runs[col] & 0xffffu)) Dq[(int64_t)L * r + (int64_t)col * l + (row - L)] = ds * Tt[b * LDE + a];
+        }
+      }
+}
+
+// D_ij of one tile pair from the CC sets in FP64 (layout and epilogue above)
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) k_pandya_fused(DevTab t, const double *__restrict__ E,
                                                       const double *__restrict__ H, double *__restrict__ D,
                                                       const TilePair *__restrict__ tp, double hh, double eh) {
@@ -754,34 +838,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) k
   // epilogue; the MFMAs take H first, so they hold the tiles transposed:
   // lane l, component v is row l % 16, column 4 v + l / 16 of the P tile,
   // and each store covers four columns of 16 consecutive rows of D
-  double *Tt = lds;  // [T][LDE]
-  auto put = [&](const d4 (&a)[2][2]) {
-#pragma unroll
-    for (int x = 0; x < 2; x++)
-#pragma unroll
-      for (int y = 0; y < 2; y++)
-#pragma unroll
-        for (int v = 0; v < 4; v++) Tt[(m0 + x * 16 + lr) * LDE + n0 + y * 16 + 4 * v + lk] = a[x][y][v];
-  };
-  auto out = [&](const d4 (&a)[2][2], int r0, int c0) {  // D[r0 + x][c0 + y] = hh a[x][y] - eh Tt[y][x]
-#pragma unroll
-    for (int x = 0; x < 2; x++)
-#pragma unroll
-      for (int y = 0; y < 2; y++)
-#pragma unroll
-        for (int v = 0; v < 4; v++) {
-          const int lx = m0 + x * 16 + lr, ly = n0 + y * 16 + 4 * v + lk;
-          if (r0 + lx < r && c0 + ly < r)
-            Dq[(r0 + lx) + (int64_t)(c0 + ly) * r] = hh * a[x][y][v] - eh * Tt[ly * LDE + lx];
-        }
-  };
-  // only the tile (ti, tj), ti <= tj, is written: D^T = -h_H h_eta D, so the
-  // lower tiles are the transposed upper ones times that sign (the two-body
-  // pass reads them there), and half of D's write is saved
-  __syncthreads();
-  put(diag ? a1 : a2);
-  __syncthreads();
-  out(a1, i0, j0);
+  store_d_tile<false>(t, q, Dq, r, i0, j0, diag, a1, a2, lds, hh, eh);
 }
 
 // The same fused pass with P = E H^T in BF16 slices (split precision S,
@@ -902,25 +959,7 @@ k_pandya_fused_bf16(DevTab t, const unsigned short *__restrict__ planes,
   if (!diag) pandya_split_product<S>(Ep, Hp, len, r, K, j0, i0, a2, (unsigned *)lds);  // P_ji = E_j H_i^T
   pandya_split_product<S>(Ep, Hp, len, r, K, i0, j0, a1, (unsigned *)lds);             // P_ij = E_i H_j^T
   // epilogue of k_pandya_fused: D_ij = hh P_ij - eh P_ji^T, P_ji through LDS
-  double *Tt = (double *)lds;  // [T][LDE]
-  __syncthreads();
-#pragma unroll
-  for (int x = 0; x < 2; x++)
-#pragma unroll
-    for (int y = 0; y < 2; y++)
-#pragma unroll
-      for (int v = 0; v < 4; v++)
-        Tt[(m0 + x * 16 + lr) * LDE + n0 + y * 16 + 4 * lk + v] = diag ? a1[x][y][v] : a2[x][y][v];
-  __syncthreads();
-#pragma unroll
-  for (int x = 0; x < 2; x++)
-#pragma unroll
-    for (int y = 0; y < 2; y++)
-#pragma unroll
-      for (int v = 0; v < 4; v++) {
-        const int lx = m0 + x * 16 + lr, ly = n0 + y * 16 + 4 * lk + v;
-        if (i0 + lx < r && j0 + ly < r) Dq[(i0 + lx) + (int64_t)(j0 + ly) * r] = hh * a1[x][y][v] - eh * Tt[ly * LDE + lx];
-      }
+  store_d_tile<true>(t, q, Dq, r, i0, j0, diag, a1, a2, (double *)lds, hh, eh);
 }
 
 template <int S>

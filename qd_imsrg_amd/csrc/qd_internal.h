@@ -58,6 +58,9 @@ struct Channels {
   // channel c (-1 if none).  The device row of (i, j) is then
   // cc_rs[K * nch + ch(i)] + pos(i) * |ch(j)| + pos(j), pos = rank in the channel.
   std::vector<int> cc_rs;
+  // per device row (cc_ab_start + row): first row of its run | run length << 16
+  // (a run = the rows whose first states share a channel)
+  std::vector<uint32_t> cc_run;
   std::vector<int> sp_pos;                  // per state: rank within its channel
   // Class pairs of the two-body pass: per block, the unordered pairs {A, B}
   // (A <= B) of single-particle channels whose state pairs lie in the block.
@@ -121,6 +124,7 @@ struct DevTab {
   int64_t cc_size;
   int64_t p_size;  // Pandya product entries; element p_size of D is a zero
   const int *cc_rowblk;  // per CC row (M * M, rows of the blocks in order): its block
+  const uint32_t *cc_run;  // per CC row: first row of its run | run length << 16
 };
 constexpr uint32_t CC_ZERO = 0x3fffffffu;
 

@@ -91,7 +91,8 @@ __device__ __forceinline__ double felem(const DevTab &t, const double *__restric
 }
 
 // flags of the fused two-body kernel
-enum { TB_122 = 1, TB_LADDER = 2, TB_PANDYA = 4, TB_ACCUM = 8 };
+// TB_DSTAGE: whole boxes stage the Pandya sub-blocks of D (run panels) in LDS
+enum { TB_122 = 1, TB_LADDER = 2, TB_PANDYA = 4, TB_ACCUM = 8, TB_DSTAGE = 32 };
 // flags of the one-body kernel
 enum { OB_111 = 1, OB_121 = 2, OB_221 = 4, OB_ACCUM = 8 };
 
@@ -145,9 +146,10 @@ void launch_fclass(hipStream_t s, const DevTab &t, const Ops &o, double *F);
 // the summed products of Vpphh and Vppph, of L
 // (TB_LADDER; launch_ladder_sq has filled L), Pandya term from D
 // (TB_PANDYA); TB_ACCUM adds to out (mode 2 tasks always add).  wide: the
-// tasks were built for the wide form (BX_WIDE_*).  dsign != 0:
-// D holds only its upper 64 x 64 tiles (launch_pandya_fused), a lower element
-// (x, y) is read as dsign * D(y, x); dsign = 0: D is whole
+// tasks were built for the wide form (BX_WIDE_*).  dsign != 0: D is in run
+// panels with only the sub-blocks of runs (a, b), a_start <= b_start, stored
+// (launch_pandya_fused), a lower element (x, y) is read as dsign * D(y, x);
+// dsign = 0: D is whole and column-major
 // Paired boxes (BOX_PAIRED in blist, the bt1p task list) need operators that
 // obey their herm flags (square sub-blocks and fhh, fpp (anti)symmetric) and,
 // with the Pandya term, D built from calc_cc of those operators.
@@ -186,8 +188,11 @@ void launch_white(hipStream_t s, const DevTab &t, const double *rec, int hrec, d
 void launch_pandya_combine(hipStream_t s, const DevTab &t, double *P, const TilePair *tp, int ntp, double hh,
                            double eh);
 // D = hh * E H^T - eh * (E H^T)^T per CC block straight from the CC sets
-// (E = ETACC, H = HCC), over 64x64 tile pairs (ti <= tj); only the upper
-// tiles of D are written (the lower ones are -hh * eh times their transpose).
+// (E = ETACC, H = HCC), over 64x64 tile pairs (ti <= tj).  D is written in
+// run panels (each run of device rows whose first states share a channel, of
+// length l starting at row L, is a column-major l x r panel at L r), and only
+// the sub-blocks of runs (a, b) with a_start <= b_start (the others are
+// -hh * eh times their transpose).
 // E H^T in FP64 or in BF16 slices (prec); pandya_fused_prec tells which
 // precisions have the fused pass (FP32 keeps product GEMM + combine).  The
 // BF16 pass first cuts both CC sets into slice planes (CcSlices): per CC

@@ -280,6 +280,33 @@ def test_box_pairs_switch(monkeypatch):
         assert rel(outs["1"][k], outs["0"][k]) <= 1e-14
 
 
+@pytest.mark.parametrize("n,r", [(6, 6), (20, 14)])
+def test_staged_pandya_reads(monkeypatch, n, r):
+    # QD_DSTAGE=1: whole boxes copy their Pandya sub-blocks of D (contiguous
+    # in run panels) into LDS and sum the four terms from there; QD_DSTAGE=0
+    # reads each term from D.  Same terms, same order: bitwise equal, both
+    # against the oracle (paired boxes with calc_cc inputs, and every box
+    # direct with caller-supplied CC sets)
+    outs = {}
+    for st in ("1", "0"):
+        monkeypatch.setenv("QD_DSTAGE", st)
+        c = Case(n, r)
+        try:
+            x, y, X, Y = c.ops(29, -1, 1)
+            ref = c.o.commutator(x, -1, y, 1, pandya=True)
+            Z1, Z2 = c.ctx.operator(1), c.ctx.operator(1)
+            HCC, ECC = c.ctx.cc(1.0), c.ctx.cc(-1.0)
+            qd.commutator_cc(X, Y, Z1, HCC, ECC)
+            qd.commutator(X, Y, Z2, HCC, ECC)
+            outs[st] = (Z1.download(), Z2.download())
+            for z in outs[st]:
+                assert rel(z, ref) <= TOL
+        finally:
+            c.ctx.close()
+    for k in range(2):
+        assert np.array_equal(outs["1"][k], outs["0"][k])
+
+
 def test_without_gather_tables(monkeypatch):
     # QD_GATHER_TABLES=0: calc_cc and the 221 traces resolve the pair tables
     # per element, the path of operators whose flat index needs more than
