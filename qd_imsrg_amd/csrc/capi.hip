@@ -998,11 +998,13 @@ int read_scalars(qd_context *c, int n, double *out) {
 // -> (side work ready) two-body box pass.  Side stream (default priority,
 // forked and joined with events, which graph capture records as edges): the
 // CC fills, the accumulating products (they read only r1, r2 and write the
-// free Vpphh/Vphhh/Vppph sub-blocks of w3) and the ket-side slabs of the
-// two-body pass (operands only), then, once the products phase is done, the
-// one-body part (r3's one-body region) and the zero-body traces (r3[0]).  The two-body pass writes only r3's two-body
-// region.  With the timing events on, everything runs in that order on one
-// stream so that each class is timed alone.
+// free Vpphh/Vphhh/Vppph sub-blocks of w3), the ket-side slabs of the
+// two-body pass (operands only), the 121 mat-vecs (u1, u2) and the zero-body
+// traces (r3[0]), then, once the products phase is done, the rest of the
+// one-body part (r3's one-body region).  The two-body pass writes only r3's
+// two-body region.  With the timing events on, everything runs on one stream
+// (mat-vecs and zero-body traces after the products) so that each class is
+// timed alone.
 enum { FILL_H = 1, FILL_E = 2 };
 // e_from_r1: the caller filled ecc by calc_cc of r1 for this commutator
 int full_commutator(qd_context *c, const qd_op *r1, const qd_op *r2, qd_op *r3, const qd_cc *hcc,
@@ -1033,11 +1035,26 @@ int full_commutator(qd_context *c, const qd_op *r1, const qd_op *r2, qd_op *r3, 
   // of r2 and r1 under the operators' own herm flags
   const bool paired = !pd || ((fill & FILL_H) && ((fill & FILL_E) || e_from_r1) && hcc->herm == r2->herm &&
                               ecc->herm == r1->herm);
+  // the 121 mat-vecs and the zero-body traces read only the operands and CC
+  // sets: on the side stream right after the work the two-body pass waits
+  // for, beside the Pandya product (N=20 R=14 step 0.484 -> 0.479 ms, N=20
+  // R=16 without the Pandya term 0.985 -> 0.938 ms; ahead of the accumulating
+  // products they delayed the two-body pass: 0.493 ms)
+  auto early_1b = [&]() {
+    rc = run_onebody(c, r1, r2, r3, OB_121 | OB_MV, pd ? ecc : nullptr, pd ? hcc : nullptr, true);
+    if (!rc) {
+      Ops o{r1->d, r2->d, r1->herm, r2->herm};
+      Timer tm(c, TC_OTHER);
+      launch_zerobody(c->stream, c->dt, o, c->pphh_segs, c->npphh_segs, c->partials, r3->d, 3);
+      if (hipGetLastError() != hipSuccess) rc = fail(QD_E_HIP, "zero-body launch failed");
+    }
+  };
   if (side) {
     HIPC(hipEventRecord(c->ev_cc, c->aux));
     if (!rc) rc = run_acc(c, r1, r2, r3);
     if (!rc) rc = run_twobody(c, r1, r2, r3, tbf, hcc, ecc, TB_PART_PRE);
     HIPC(hipEventRecord(c->ev_acc, c->aux));
+    if (!rc) early_1b();
     on(main);
   }
   if (rc) return rc;
@@ -1066,8 +1083,9 @@ int full_commutator(qd_context *c, const qd_op *r1, const qd_op *r2, qd_op *r3, 
   }
   // with the Pandya inputs on hand, the 121 mat-vecs read them (hcc and ecc
   // are calc_cc of r2 and r1 by the entry point's contract)
-  rc = run_onebody(c, r1, r2, r3, OB_111 | OB_121 | OB_221, pd ? ecc : nullptr, pd ? hcc : nullptr, true);
-  if (!rc) {
+  rc = run_onebody(c, r1, r2, r3, OB_111 | OB_121 | OB_221 | (side ? OB_NOMV : 0), pd ? ecc : nullptr,
+                   pd ? hcc : nullptr, true);
+  if (!rc && !side) {
     Ops o{r1->d, r2->d, r1->herm, r2->herm};
     Timer tm(c, TC_OTHER);
     launch_zerobody(c->stream, c->dt, o, c->pphh_segs, c->npphh_segs, c->partials, r3->d, 3);

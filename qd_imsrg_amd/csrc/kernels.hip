@@ -233,7 +233,8 @@ __global__ void __launch_bounds__(256) k_onebody221_map(DevTab t, Ops o, double 
 void launch_onebody(hipStream_t s, const DevTab &t, const Ops &o, double *out, int flags, const double *w1,
                     const double *w2, const double *w3, double *u1, double *u2, const double *cc1,
                     const double *cc2, const int *list221, int n221, const uint32_t *g221) {
-  if ((flags & OB_121) && cc1 && cc2) {
+  if (flags & OB_NOMV) {  // u1, u2 filled by an earlier OB_MV call
+  } else if ((flags & OB_121) && cc1 && cc2) {
     const int m = t.M * t.M;
     hipLaunchKernelGGL(k_cc_gemv, dim3((m + 31) / 32), dim3(256), 0, s, t, cc2, o.v1, u2);
     hipLaunchKernelGGL(k_cc_gemv, dim3((m + 31) / 32), dim3(256), 0, s, t, cc1, o.v2, u1);
@@ -241,6 +242,7 @@ void launch_onebody(hipStream_t s, const DevTab &t, const Ops &o, double *out, i
     launch_cc_matvec(s, t, o.v2, o.h2, o.v1, u2);
     launch_cc_matvec(s, t, o.v1, o.h1, o.v2, u1);
   }
+  if (flags & OB_MV) return;
   const int64_t tot = (int64_t)t.n * t.n + (int64_t)t.mp * t.mp + (int64_t)t.mp * t.n;
   if ((flags & (OB_111 | OB_121)) || !(flags & OB_ACCUM))
     hipLaunchKernelGGL(k_onebody, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, s, t, o, out, flags, w1, w2, w3,

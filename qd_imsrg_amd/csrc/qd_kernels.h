@@ -93,8 +93,9 @@ __device__ __forceinline__ double felem(const DevTab &t, const double *__restric
 // flags of the fused two-body kernel
 // TB_DSTAGE: whole boxes stage the Pandya sub-blocks of D (run panels) in LDS
 enum { TB_122 = 1, TB_LADDER = 2, TB_PANDYA = 4, TB_ACCUM = 8, TB_DSTAGE = 32 };
-// flags of the one-body kernel
-enum { OB_111 = 1, OB_121 = 2, OB_221 = 4, OB_ACCUM = 8 };
+// flags of the one-body kernel; OB_MV: only the 121 mat-vecs into u1, u2
+// (they read the operands and CC sets only), OB_NOMV: u1, u2 are filled
+enum { OB_111 = 1, OB_121 = 2, OB_221 = 4, OB_ACCUM = 8, OB_MV = 16, OB_NOMV = 32 };
 
 // Small products: 16 x 16 tiles (tm, tn in units of 16), FP64 FMA.
 void launch_gemm_small(hipStream_t s, const GemmProb *probs, const GemmSeg *segs, const GemmTile *tiles,
