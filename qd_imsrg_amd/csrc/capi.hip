@@ -136,9 +136,10 @@ struct qd_context {
     const void *g, *prev, *next, *hs, *cca, *ccg;
     int hg, pd;
     double hh, eh;
+    int term;  // BCH term k (its coefficient is a kernel argument); 0 otherwise
     bool operator<(const GraphKey &o) const {
-      return std::tie(g, prev, next, hs, cca, ccg, hg, pd, hh, eh) <
-             std::tie(o.g, o.prev, o.next, o.hs, o.cca, o.ccg, o.hg, o.pd, o.hh, o.eh);
+      return std::tie(g, prev, next, hs, cca, ccg, hg, pd, hh, eh, term) <
+             std::tie(o.g, o.prev, o.next, o.hs, o.cca, o.ccg, o.hg, o.pd, o.hh, o.eh, o.term);
     }
   };
   std::map<GraphKey, hipGraphExec_t> graphs;
@@ -1647,7 +1648,7 @@ int qd_commutator_cc(qd_context *c, const qd_op *r1, const qd_op *r2, qd_op *r3,
     // the whole call (both streams) as one graph per operand set; hs = null
     // and the herm code above 100 keep these keys apart from the BCH terms'
     qd_context::GraphKey key{r1->d, r2->d, r3->d, nullptr, hcc->d, etacc->d,
-                             100 + 3 * (r1->herm + 1) + (r2->herm + 1), 2, hcc->herm, etacc->herm};
+                             100 + 3 * (r1->herm + 1) + (r2->herm + 1), 2, hcc->herm, etacc->herm, 0};
     auto it = c->graphs.find(key);
     if (it == c->graphs.end()) {
       hipGraph_t graph;
@@ -1699,17 +1700,20 @@ int qd_bch_expand(qd_context *c, qd_op *hs, const qd_op *g, const qd_op *h, cons
     next->herm = 1;
     qd_op adh = *prev;  // AD carries herm = +1 (src/magnus_IMSRG.f90:273-275)
     adh.herm = 1;
-    // one term: [G, AD] into next, ||INT1|| (= HS before the update) and ||INT2||
+    // one term: [G, AD] into next, then in one pass ||INT1|| (= HS before
+    // the update), ||INT2|| and HS += cof(k) AD
     auto term = [&]() -> int {
       int r;  // with the Pandya term, cca = calc_cc(AD) is filled first
       if ((r = full_commutator(c, g, &adh, next, pd ? cca : nullptr, pd ? ccg : nullptr, pd ? FILL_H : 0, pd)))
         return r;
-      if ((r = norm_into(c, hs, 0))) return r;
-      return norm_into(c, next, 1);
+      Timer tm(c, TC_OTHER);
+      launch_bch_update(c->stream, hs->d, next->d, BCH_COF[k - 2], c->ch.neq, c->partials, c->scal);
+      HIPC(hipGetLastError());
+      return QD_OK;
     };
     if (c->use_graphs && !c->timing) {
       qd_context::GraphKey key{g->d, prev->d, next->d, hs->d, pd ? cca->d : nullptr, pd ? ccg->d : nullptr,
-                               g->herm, pd ? 1 : 0, pd ? cca->herm : 0.0, pd ? ccg->herm : 0.0};
+                               g->herm, pd ? 1 : 0, pd ? cca->herm : 0.0, pd ? ccg->herm : 0.0, k};
       auto it = c->graphs.find(key);
       if (it == c->graphs.end()) {
         hipGraph_t graph;
@@ -1728,11 +1732,6 @@ int qd_bch_expand(qd_context *c, qd_op *hs, const qd_op *g, const qd_op *h, cons
       return rc;
     }
     taken++;
-    {
-      Timer tm(c, TC_OTHER);
-      launch_axpby(c->stream, c->ch.neq, 1.0, hs->d, BCH_COF[k - 2], next->d, hs->d);
-      HIPC(hipGetLastError());
-    }
     double nn[2];
     if ((rc = read_scalars(c, 2, nn))) return rc;
     if (std::sqrt(nn[1]) * BCH_COF[k - 2] / std::sqrt(nn[0]) < R4_1EM5) break;

@@ -423,6 +423,37 @@ void launch_norm2(hipStream_t s, const double *v, int64_t neq, double *partials,
   hipLaunchKernelGGL(k_sum_final, dim3(1), dim3(256), 0, s, partials, NPART, dst);
 }
 
+// One BCH term's bookkeeping in one pass (src/magnus_IMSRG.f90:300-316):
+// ||HS||^2 before the update and ||AD||^2 over [1, n) with the partition and
+// summation order of k_sumsq_part (the same sums to the bit), and HS += cof AD
+// over [0, n)
+__global__ void __launch_bounds__(256) k_bch_update(double *__restrict__ hs, const double *__restrict__ ad,
+                                                    double cof, int64_t n, double *part) {
+  const int64_t chunk = (n - 1 + NPART - 1) / NPART;
+  const int64_t b = 1 + (int64_t)blockIdx.x * chunk, e = b + chunk < n ? b + chunk : n;
+  double s1 = 0.0, s2 = 0.0;
+  for (int64_t k = b + threadIdx.x; k < e; k += 256) {
+    const double x = hs[k], y = ad[k];
+    s1 += x * x;
+    s2 += y * y;
+    hs[k] = x + cof * y;
+  }
+  if (blockIdx.x == 0 && threadIdx.x == 0) hs[0] = hs[0] + cof * ad[0];
+  s1 = block_sum256(s1);
+  s2 = block_sum256(s2);
+  if (threadIdx.x == 0) {
+    part[blockIdx.x] = s1;
+    part[NPART + blockIdx.x] = s2;
+  }
+}
+
+void launch_bch_update(hipStream_t s, double *hs, const double *ad, double cof, int64_t neq, double *partials,
+                       double *dst) {
+  hipLaunchKernelGGL(k_bch_update, dim3(NPART), dim3(256), 0, s, hs, ad, cof, neq, partials);
+  hipLaunchKernelGGL(k_sum_final, dim3(1), dim3(256), 0, s, partials, NPART, dst);
+  hipLaunchKernelGGL(k_sum_final, dim3(1), dim3(256), 0, s, partials + NPART, NPART, dst + 1);
+}
+
 // dot product over [1, n) (E0 excluded, as in mat_2_norm), same partition
 __global__ void __launch_bounds__(256) k_dot_part(const double *__restrict__ a, const double *__restrict__ b,
                                                   int64_t n, double *part) {

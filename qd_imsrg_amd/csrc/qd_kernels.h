@@ -213,6 +213,10 @@ bool pandya_fused_prec(int prec);
 void launch_axpby(hipStream_t s, int64_t n, double s1, const double *x, double s2, const double *y, double *z);
 // sum of squares of v[1..neq) -> *dst, via partials (deterministic)
 void launch_norm2(hipStream_t s, const double *v, int64_t neq, double *partials, double *dst);
+// BCH term: dst[0] = ||hs||^2 before and dst[1] = ||ad||^2 over [1, neq)
+// (as launch_norm2), then hs += cof * ad over [0, neq); partials: 2048 doubles
+void launch_bch_update(hipStream_t s, double *hs, const double *ad, double cof, int64_t neq, double *partials,
+                       double *dst);
 // sum of a[k] b[k] over [1, neq) -> *dst (deterministic)
 void launch_dot(hipStream_t s, const double *a, const double *b, int64_t neq, double *partials, double *dst);
 constexpr int LINCOMB_MAX = 32;
