@@ -76,6 +76,14 @@ struct GemmList {
   std::vector<int> ptile_off;
   size_t cap = 0;                                        // capacity of tiles_pre / tiles_all
   double flops = 0, flops_pre = 0;                       // algorithmic 2*M*N*K
+  // per segment, for an operand that is a sub-block of r1 or r2: slot << 8 |
+  // its type (ZB bit), else -1
+  std::vector<int2> hzb;
+  // the list without the segments that read sub-block types of r1 absent from
+  // a mask (those products are exact zeros), per mask: device problems (same
+  // tiles and shapes; a problem without segments writes a zero tile) and
+  // segments
+  std::map<uint32_t, std::pair<GemmProb *, GemmSeg *>> var;
 };
 }  // namespace
 
@@ -104,6 +112,15 @@ struct qd_context {
   int box_rcap = BX_RCAP;                                            // row grid entries per box task
   bool box_wide = false;                                             // tasks for the wide box form
   bool box_pairs = true;                                             // paired box tasks where valid
+  // while nz_on (the BCH terms), nz_mask holds the sub-block types of the
+  // first operand (Omega, which holds only fph and Vpphh) with a nonzero
+  // element (ZB bits, from the scan into nz); the GEMM lists then run without
+  // the products of its other sub-block types
+  uint32_t *nz = nullptr;
+  uint32_t nz_mask = ZB_ALL;
+  ZTile *ztiles = nullptr;
+  int nztiles = 0;
+  bool nz_on = false, zskip = true;  // QD_ZSKIP=0: never skip
   // whole boxes stage their Pandya sub-blocks of D (contiguous in run panels)
   // in LDS instead of reading each term from HBM: chosen with the wide box
   // form (N=42 R=20 two-body class 2.39 -> 2.28 ms); in the narrow form the
@@ -136,10 +153,11 @@ struct qd_context {
     const void *g, *prev, *next, *hs, *cca, *ccg;
     int hg, pd;
     double hh, eh;
-    int term;  // BCH term k (its coefficient is a kernel argument); 0 otherwise
+    int term;       // BCH term k (its coefficient is a kernel argument); 0 otherwise
+    uint32_t zm;    // the GEMM lists' mask of r1 (nz_mask)
     bool operator<(const GraphKey &o) const {
-      return std::tie(g, prev, next, hs, cca, ccg, hg, pd, hh, eh, term) <
-             std::tie(o.g, o.prev, o.next, o.hs, o.cca, o.ccg, o.hg, o.pd, o.hh, o.eh, o.term);
+      return std::tie(g, prev, next, hs, cca, ccg, hg, pd, hh, eh, term, zm) <
+             std::tie(o.g, o.prev, o.next, o.hs, o.cca, o.ccg, o.hg, o.pd, o.hh, o.eh, o.term, o.zm);
     }
   };
   std::map<GraphKey, hipGraphExec_t> graphs;
@@ -261,6 +279,13 @@ struct GemmBuilder {
   std::vector<GemmProb> probs;
   std::vector<GemmSeg> segs;
   double flops = 0;
+  const std::map<int64_t, int> *zb = nullptr;  // operator offset -> sub-block type (ZB bit)
+  std::vector<int2> zinfo;  // per segment: the operands' slot << 8 | ZB bit, or -1
+  int zbit(int slot, int64_t off) const {
+    if (!zb || (slot != S_R1 && slot != S_R2)) return -1;
+    auto it = zb->find(off);
+    return it == zb->end() ? -1 : (slot << 8 | it->second);
+  }
   void begin(int c_slot, int64_t c_off, int ldc, int M, int N, int beta1) {
     GemmProb p{};
     p.c_off = c_off; p.c_slot = c_slot; p.ldc = std::max(ldc, 1); p.M = M; p.N = N; p.beta1 = beta1;
@@ -272,13 +297,18 @@ struct GemmBuilder {
     GemmSeg s{};
     s.a_off = a_off; s.b_off = b_off; s.a_slot = a_slot; s.b_slot = b_slot;
     s.lda = std::max(lda, 1); s.ldb = std::max(ldb, 1); s.ta = ta; s.tb = tb; s.K = K; s.alpha = alpha;
+    zinfo.push_back(make_int2(zbit(a_slot, a_off), zbit(b_slot, b_off)));
     segs.push_back(s);
     probs.back().nseg++;
     flops += 2.0 * probs.back().M * probs.back().N * K;
   }
   void end() {
     const GemmProb &p = probs.back();
-    if (p.M <= 0 || p.N <= 0) { segs.resize(p.seg0); probs.pop_back(); }
+    if (p.M <= 0 || p.N <= 0) {
+      segs.resize(p.seg0);
+      zinfo.resize(p.seg0);
+      probs.pop_back();
+    }
   }
 };
 
@@ -298,6 +328,26 @@ void tiles_of(const GemmList &g, int pi, std::vector<GemmTile> &out) {
   const int TM = SHAPE_TM[p.shape], TN = SHAPE_TN[p.shape];
   for (int tm = 0; tm * TM < p.M; tm++)
     for (int tn = 0; tn * TN < p.N; tn++) out.push_back({pi, tm, tn});
+}
+
+// The problems and segments of g without the segments that read a sub-block
+// type of r1 that is not in nz1 (zero in every block): their products are
+// exact zeros.  Shapes and tile lists are g's.
+void variant_of(const GemmList &g, uint32_t nz1, std::vector<GemmProb> &vp, std::vector<GemmSeg> &vs) {
+  auto dead = [&](int info) { return info >= 0 && (info >> 8) == S_R1 && !((nz1 >> (info & 0xff)) & 1u); };
+  vp = g.hprobs;
+  vs.clear();
+  for (auto &p : vp) {
+    const int s0 = p.seg0, n = p.nseg;
+    p.seg0 = (int)vs.size();
+    p.nseg = 0;
+    for (int i = s0; i < s0 + n; i++)
+      if (!dead(g.hzb[i].x) && !dead(g.hzb[i].y)) {
+        vs.push_back(g.hsegs[i]);
+        p.nseg++;
+      }
+  }
+  if (vs.empty()) vs.push_back(GemmSeg{});
 }
 
 // (re)builds the three tile lists from the problems' current shapes
@@ -417,6 +467,12 @@ int upload_tiles(qd_context *c, GemmList &g) {
   HIPC(hipMemcpy(g.tiles_pre, pre.data(), sizeof(GemmTile) * pre.size(), hipMemcpyHostToDevice));
   HIPC(hipMemcpy(g.ptiles, grouped.data(), sizeof(GemmTile) * grouped.size(), hipMemcpyHostToDevice));
   HIPC(hipMemcpy(g.probs, g.hprobs.data(), sizeof(GemmProb) * np, hipMemcpyHostToDevice));
+  for (auto &v : g.var) {  // the variants take the new shapes
+    std::vector<GemmProb> vp;
+    std::vector<GemmSeg> vs;
+    variant_of(g, v.first, vp, vs);
+    HIPC(hipMemcpy(v.second.first, vp.data(), sizeof(GemmProb) * np, hipMemcpyHostToDevice));
+  }
   g.ntiles_all = (int)all.size();
   g.ntiles_pre = (int)pre.size();
   return QD_OK;
@@ -426,6 +482,8 @@ int upload_gemm(qd_context *c, GemmBuilder &b, int npre, double flops_pre, GemmL
   int rc;
   g.hprobs = b.probs;
   g.hsegs = b.segs;
+  g.hzb = b.zinfo;
+  if (g.hzb.size() != g.hsegs.size()) return fail(QD_E_LAYOUT, "GEMM list: segment tables out of step");
   g.npre = npre;
   g.flops = b.flops;
   g.flops_pre = flops_pre;
@@ -630,8 +688,30 @@ int build_worklists(qd_context *c) {
   if ((rc = dev_upload(c, ctiles, &c->cc_tiles))) return rc;
   c->ncc_tiles = (int)ctiles.size();
 
+  // ---- operator sub-blocks: the ZB bit of each non-empty sub-block by its
+  // offset (for skipping products of all-zero sub-block types) and the tiles
+  // of the scan that finds them
+  std::map<int64_t, int> zb;
+  std::vector<ZTile> zt;
+  auto zregion = [&](int64_t off, int64_t size, int bit) {
+    if (size <= 0) return;
+    zb.emplace(off, bit);
+    for (int64_t k = 0; k < size; k += 16384) zt.push_back({off + k, (int)std::min<int64_t>(16384, size - k), bit});
+  };
+  zregion(ch.off_fhh, (int64_t)ch.n * ch.n, ZB_FHH);
+  zregion(ch.off_fpp, (int64_t)ch.mp * ch.mp, ZB_FPP);
+  zregion(ch.off_fph, (int64_t)ch.mp * ch.n, ZB_FPH);
+  for (int q = 0; q < ch.nblock; q++) {
+    const int64_t nh = ch.nh[q], np = ch.np[q], nb = ch.nb[q];
+    const int64_t sz[6] = {nh * nh, np * np, np * nh, nb * nb, np * nb, nb * nh};  // enum Sub order
+    for (int u = 0; u < 6; u++) zregion(ch.off[(size_t)q * 6 + u], sz[u], u);
+  }
+  if ((rc = dev_upload(c, zt, &c->ztiles))) return rc;
+  c->nztiles = (int)zt.size();
+
   // ---- GEMM products (matrix_mult222 / commutator_221 intermediates)
   GemmBuilder g1, g2;
+  g1.zb = g2.zb = &zb;
   for (int q = 0; q < ch.nblock; q++) {
     const int nh = ch.nh[q], np = ch.np[q], nb = ch.nb[q];
     const int64_t *o = &ch.off[(size_t)q * 6];
@@ -874,20 +954,44 @@ bool pandya_on(const qd_cc *hcc, const qd_cc *ecc) {
   return hcc && ecc && !hcc->zero && !ecc->zero;
 }
 
+// device problems and segments of g for the context's current mask of r1
+int gemm_ptrs(qd_context *c, GemmList &g, GemmProb **pp, GemmSeg **ps) {
+  *pp = g.probs;
+  *ps = g.segs;
+  if (!c->nz_on || c->nz_mask == (uint32_t)ZB_ALL) return QD_OK;
+  auto it = g.var.find(c->nz_mask);
+  if (it == g.var.end()) {
+    std::vector<GemmProb> vp;
+    std::vector<GemmSeg> vs;
+    variant_of(g, c->nz_mask, vp, vs);
+    std::pair<GemmProb *, GemmSeg *> d{nullptr, nullptr};
+    int rc;
+    if ((rc = dev_upload(c, vp, &d.first)) || (rc = dev_upload(c, vs, &d.second))) return rc;
+    it = g.var.emplace(c->nz_mask, d).first;
+  }
+  *pp = it->second.first;
+  *ps = it->second.second;
+  return QD_OK;
+}
+
 // One grouped persistent launch over the sorted tile list, or (per-channel
 // mode) one launch per problem, the way the reference calls dgemm per block.
-int run_gemm_list(qd_context *c, const GemmList &g, bool with_pandya, const Slots &s, int prec = -1) {
+int run_gemm_list(qd_context *c, GemmList &g, bool with_pandya, const Slots &s, int prec = -1) {
   if (prec < 0) prec = c->gemm_prec;
+  GemmProb *probs;
+  GemmSeg *segs;
+  int rc = gemm_ptrs(c, g, &probs, &segs);
+  if (rc) return rc;
   if (c->launch_mode == 0) {
     const int *r = with_pandya ? g.shape_all : g.shape_pre;
     GemmTile *t = with_pandya ? g.tiles_all : g.tiles_pre;
     for (int sh = 0; sh <= SH_MIX; sh++)
-      launch_gemm(c->stream, g.probs, g.segs, t + r[sh], r[sh + 1] - r[sh], s, c->gemm_grid, sh, prec);
+      launch_gemm(c->stream, probs, segs, t + r[sh], r[sh + 1] - r[sh], s, c->gemm_grid, sh, prec);
   } else {
     const int np = with_pandya ? (int)g.hprobs.size() : g.npre;
     for (int pi = 0; pi < np; pi++) {
       const int n = g.ptile_off[pi + 1] - g.ptile_off[pi];
-      launch_gemm(c->stream, g.probs, g.segs, g.ptiles + g.ptile_off[pi], n, s, 0, g.hprobs[pi].shape, prec);
+      launch_gemm(c->stream, probs, segs, g.ptiles + g.ptile_off[pi], n, s, 0, g.hprobs[pi].shape, prec);
     }
   }
   HIPC(hipGetLastError());
@@ -1311,6 +1415,8 @@ int qd_context_create(int nbody, int nshell, qd_context **out) {
   if ((rc = dev_alloc(c, sizeof(double) * ch.M * ch.M, &p))) return bail(rc); c->u1 = (double *)p;
   if ((rc = dev_alloc(c, sizeof(double) * ch.M * ch.M, &p))) return bail(rc); c->u2 = (double *)p;
   if ((rc = dev_alloc(c, sizeof(double) * 1024 * 64, &p))) return bail(rc); c->partials = (double *)p;
+  if ((rc = dev_alloc(c, sizeof(uint32_t) * 2, &p))) return bail(rc); c->nz = (uint32_t *)p;
+  if (const char *env = getenv("QD_ZSKIP")) c->zskip = env[0] != '0';
   if ((rc = dev_alloc(c, sizeof(double) * 8, &p))) return bail(rc); c->scal = (double *)p;
   if (hipHostMalloc((void **)&c->host_scal, sizeof(double) * 8, hipHostMallocDefault) != hipSuccess)
     return bail(fail(QD_E_HIP, "hipHostMalloc failed"));
@@ -1648,7 +1754,7 @@ int qd_commutator_cc(qd_context *c, const qd_op *r1, const qd_op *r2, qd_op *r3,
     // the whole call (both streams) as one graph per operand set; hs = null
     // and the herm code above 100 keep these keys apart from the BCH terms'
     qd_context::GraphKey key{r1->d, r2->d, r3->d, nullptr, hcc->d, etacc->d,
-                             100 + 3 * (r1->herm + 1) + (r2->herm + 1), 2, hcc->herm, etacc->herm, 0};
+                             100 + 3 * (r1->herm + 1) + (r2->herm + 1), 2, hcc->herm, etacc->herm, 0, ZB_ALL};
     auto it = c->graphs.find(key);
     if (it == c->graphs.end()) {
       hipGraph_t graph;
@@ -1690,6 +1796,29 @@ int qd_bch_expand(qd_context *c, qd_op *hs, const qd_op *g, const qd_op *h, cons
     cca->herm = hcc->herm;
     ccg->zero = cca->zero = false;  // cca is refilled per term by the launch below
   }
+  // the sub-block types of G that are zero in every block (a Magnus Omega
+  // holds only fph and Vpphh): the products reading them are skipped in every
+  // term, their contributions being exact zeros
+  struct NzScope {
+    qd_context *c;
+    ~NzScope() {
+      c->nz_on = false;
+      c->nz_mask = ZB_ALL;
+    }
+  } nz_scope{c};
+  if (c->zskip) {
+    launch_zero_scan(c->stream, g->d, c->ztiles, c->nztiles, c->nz);
+    HIPC(hipGetLastError());
+    HIPC(hipMemcpyAsync(c->host_scal, c->nz, sizeof(uint32_t), hipMemcpyDeviceToHost, c->stream));
+    HIPC(hipStreamSynchronize(c->stream));
+    memcpy(&c->nz_mask, c->host_scal, sizeof(uint32_t));
+    c->nz_on = true;
+    // the lists for this mask exist before any term is captured in a graph
+    // (no allocation during capture)
+    GemmProb *pp;
+    GemmSeg *ps;
+    if ((rc = gemm_ptrs(c, c->g_all, &pp, &ps)) || (rc = gemm_ptrs(c, c->g_acc, &pp, &ps))) return rc;
+  }
   // HS = H ; AD_0 = H ; then AD_k = [G, AD_{k-1}], HS += cof(k) AD_k
   if ((rc = qd_copy_arrays(c, h, hs))) return rc;
   const qd_op *prev = h;
@@ -1713,7 +1842,8 @@ int qd_bch_expand(qd_context *c, qd_op *hs, const qd_op *g, const qd_op *h, cons
     };
     if (c->use_graphs && !c->timing) {
       qd_context::GraphKey key{g->d, prev->d, next->d, hs->d, pd ? cca->d : nullptr, pd ? ccg->d : nullptr,
-                               g->herm, pd ? 1 : 0, pd ? cca->herm : 0.0, pd ? ccg->herm : 0.0, k};
+                               g->herm, pd ? 1 : 0, pd ? cca->herm : 0.0, pd ? ccg->herm : 0.0, k,
+                               c->nz_on ? c->nz_mask : (uint32_t)ZB_ALL};
       auto it = c->graphs.find(key);
       if (it == c->graphs.end()) {
         hipGraph_t graph;

@@ -394,6 +394,21 @@ __global__ void __launch_bounds__(256) k_axpby(int64_t n, double s1, const doubl
     z[e] = s1 * x[e] + s2 * y[e];
 }
 
+__global__ void k_zero_scan_init(uint32_t *nz) { *nz = 0u; }
+
+__global__ void __launch_bounds__(256) k_zero_scan(const double *__restrict__ v, const ZTile *__restrict__ tiles,
+                                                   uint32_t *nz) {
+  const ZTile t = tiles[blockIdx.x];
+  bool any = false;
+  for (int k = threadIdx.x; k < t.len; k += 256) any = any || v[t.off + k] != 0.0;
+  if (__any(any) && (threadIdx.x & 63) == 0) atomicOr(nz, 1u << t.bit);
+}
+
+void launch_zero_scan(hipStream_t s, const double *v, const ZTile *tiles, int ntiles, uint32_t *nz) {
+  hipLaunchKernelGGL(k_zero_scan_init, dim3(1), dim3(1), 0, s, nz);
+  if (ntiles > 0) hipLaunchKernelGGL(k_zero_scan, dim3(ntiles), dim3(256), 0, s, v, tiles, nz);
+}
+
 void launch_axpby(hipStream_t s, int64_t n, double s1, const double *x, double s2, const double *y, double *z) {
   int64_t blocks = (n + 255) / 256;
   if (blocks > 8192) blocks = 8192;

@@ -21,6 +21,9 @@ struct TilePair { int blk, ti, tj; };       // 32x32 tiles (ti, tj) and (tj, ti)
 // Operand slots resolved at launch time (the same lists serve every call).
 enum Slot { S_R1 = 0, S_R2, S_OUT, S_W1, S_W2, S_W3, S_W4, S_CC1, S_CC2, S_P, NSLOT };
 struct Slots { double *p[NSLOT]; };
+// bits of the sub-block types of an operator: the six two-body sub-blocks
+// (enum Sub), then the one-body fhh, fpp, fph
+enum { ZB_FHH = 6, ZB_FPP = 7, ZB_FPH = 8, ZB_ALL = (1 << 9) - 1 };
 struct GemmSeg {   // one product op(A)*op(B) of a problem, alpha = +-1
   int64_t a_off, b_off;
   int a_slot, b_slot, lda, ldb, ta, tb, K;
@@ -211,6 +214,10 @@ void launch_pandya_fused(hipStream_t s, const DevTab &t, const double *etacc, co
                          const TilePair *tp, int ntp, double hh, double eh, int prec, const CcSlices &cs);
 bool pandya_fused_prec(int prec);
 void launch_axpby(hipStream_t s, int64_t n, double s1, const double *x, double s2, const double *y, double *z);
+// nonzero sub-block types of an operator: *nz = OR over the tiles of (bit of
+// the tile's type if any of its elements is nonzero)
+struct ZTile { int64_t off; int len, bit; };
+void launch_zero_scan(hipStream_t s, const double *v, const ZTile *tiles, int ntiles, uint32_t *nz);
 // sum of squares of v[1..neq) -> *dst, via partials (deterministic)
 void launch_norm2(hipStream_t s, const double *v, int64_t neq, double *partials, double *dst);
 // BCH term: dst[0] = ||hs||^2 before and dst[1] = ||ad||^2 over [1, neq)

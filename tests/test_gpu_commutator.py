@@ -232,6 +232,55 @@ def test_bch_expand(case):
     assert rel(HS.download(), ref) <= 1e-11
 
 
+def generator_shaped(ch, v):
+    """v with only fph and the Vpphh sub-blocks kept (the shape of a White
+    generator and of the Magnus Omega built from them)."""
+    w = np.zeros_like(v)
+    w[ch.off_fph:ch.off_fph + ch.mp * ch.n] = v[ch.off_fph:ch.off_fph + ch.mp * ch.n]
+    for q in range(ch.nblock):
+        o, nh, np_ = int(ch.off[q][2]), int(ch.nh[q]), int(ch.np_[q])
+        w[o:o + np_ * nh] = v[o:o + np_ * nh]
+    return w
+
+
+@pytest.mark.parametrize("n,r", [(6, 6), (20, 14)])
+def test_bch_skips_zero_subblocks(monkeypatch, n, r):
+    # BCH_EXPAND scans G for sub-block types that are zero in every block and
+    # skips the GEMM products that read them (QD_ZSKIP=0 never skips): with a
+    # generator-shaped G (fph and Vpphh only) both give the same result and
+    # the oracle's, with the same term count; with the Pandya term too
+    outs = {}
+    for zs in ("1", "0"):
+        monkeypatch.setenv("QD_ZSKIP", zs)
+        c = Case(n, r)
+        try:
+            rng = np.random.default_rng(31)
+            g = generator_shaped(c.ch, random_flat(c.ch, rng, -1, scale=0.02))
+            h = random_flat(c.ch, rng, 1)
+            G, H = c.ctx.operator(-1, g), c.ctx.operator(1, h)
+            from oracle.flow import bch_expand_pandya
+            res = []
+            for pandya in (False, True):
+                HS = c.ctx.operator(1)
+                if pandya:
+                    HCC, OCC = c.ctx.cc(1.0), c.ctx.cc(-1.0)
+                    qd.calc_cc(H, HCC)
+                    qd.calc_cc(G, OCC)
+                    nt = qd.BCH_EXPAND(HS, G, H, HCC, OCC)
+                    ref, nref = bch_expand_pandya(c.o, g, h)
+                else:
+                    nt = qd.BCH_EXPAND(HS, G, H)
+                    ref, nref = c.o.bch_expand(g, -1, h)
+                assert nt == nref
+                assert rel(HS.download(), ref) <= 1e-11
+                res.append((nt, HS.download()))
+            outs[zs] = res
+        finally:
+            c.ctx.close()
+    for (n1, a), (n0, b) in zip(outs["1"], outs["0"]):
+        assert n1 == n0 and np.array_equal(a, b)
+
+
 @pytest.mark.parametrize("cap", [40, 100])
 def test_box_slabs(monkeypatch, cap):
     # a small cap on the LDS grid of the two-body boxes splits most boxes of
