@@ -70,7 +70,11 @@ const char *qd_last_error(void);
  * then differ at rounding level only), QD_BOX_WIDE=0/1 the narrow (512
  * threads, 3600-element grids) or wide (1024 threads, 8000) box tasks instead
  * of the automatic choice (wide where the narrow form would split more than a
- * tenth of the grid elements into slabs; QD_BOX_CAP then ranges up to 8000). */
+ * tenth of the grid elements into slabs; QD_BOX_CAP then ranges up to 8000),
+ * QD_DSTAGE=0/1 whole boxes read their Pandya sub-blocks of D from HBM or
+ * stage them in LDS (default: staged in the wide form), QD_ZSKIP=0 the BCH
+ * terms run every GEMM product instead of skipping those that read a
+ * sub-block type of G that is zero in every block (the same values). */
 int qd_context_create(int nbody, int nshell, qd_context **out);
 int qd_context_destroy(qd_context *ctx);
 int qd_context_info(const qd_context *ctx, qd_info *out);
@@ -190,7 +194,10 @@ int qd_commutator_cc(qd_context *ctx, const qd_op *r1, const qd_op *r2, qd_op *r
                      qd_cc *hcc, qd_cc *etacc, double *e0);
 /* BCH_EXPAND(HS,G,H,...), src/magnus_IMSRG.f90:253-318: hs = sum_k cof(k)
  * ad_G^k H with the reference's literal coefficients and its 1e-5 (REAL(4))
- * relative truncation; *nterms = nested commutators taken (1..11). */
+ * relative truncation; *nterms = nested commutators taken (1..11).  G is
+ * scanned once for sub-block types that are zero in every block (a Magnus
+ * Omega holds only fph and Vpphh); the GEMM products reading them are exact
+ * zeros and are not launched (QD_ZSKIP=0 launches them). */
 int qd_bch_expand(qd_context *ctx, qd_op *hs, const qd_op *g, const qd_op *h,
                   const qd_cc *hcc, const qd_cc *occ, int *nterms);
 /* MAGNUS_EXPAND(DG,G,ETA,...), src/magnus_IMSRG.f90:321-369, as shipped:
@@ -306,10 +313,11 @@ int qd_gs_flow(qd_context *ctx, const qd_op *h0, qd_op *hs, const qd_gs_params *
  * this is the batching control of the MI355X engine. */
 int qd_set_launch_mode(qd_context *ctx, int mode);
 /* 1: qd_bch_expand captures each nested-commutator term (calc_cc of the
- * term, the fused commutator, both norms) in a hipGraph on first use and
- * replays it, and qd_commutator_cc does the same for the whole call; graphs
- * are keyed by the operand addresses and hermiticities.  0 (default): plain
- * stream launches.  No counterpart in the reference. */
+ * term, the fused commutator, both norms and the update of hs) in a hipGraph
+ * on first use and replays it, and qd_commutator_cc does the same for the
+ * whole call; graphs are keyed by the operand addresses, hermiticities, the
+ * term index and the zero sub-block mask of G.  0 (default): plain stream
+ * launches.  No counterpart in the reference. */
 int qd_set_graphs(qd_context *ctx, int on);
 /* Number of captured graphs the context holds.  A graph is dropped when an
  * operator or CC set it was captured on is destroyed. */
