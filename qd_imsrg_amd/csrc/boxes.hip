@@ -492,10 +492,13 @@ void launch_sum_halves(hipStream_t s, const double *w3, const double *w4, double
 // W = -h2 A - d B, L = U^T + W: only U goes through LDS (both tiles), W stays
 // in the registers of the thread that loaded it, so the box pass reads one
 // coalesced value per element instead of two scattered transposed ones.
+// nz1: the nonzero sub-block types of r1 (ZB bits); a product of a zero type
+// is known to be zero (the BCH terms) and is not read: w3 of HHHH and PPPP
+// (r1 hhhh, pppp), w1 and w2 of PHPH (r1 ppph, phhh)
 __global__ void __launch_bounds__(256) k_ladder_sq(DevTab t, const double *__restrict__ w1,
                                                    const double *__restrict__ w2, const double *__restrict__ w3,
                                                    double *__restrict__ L, const TilePair *__restrict__ tp, double h1,
-                                                   double h2) {
+                                                   double h2, uint32_t nz1) {
   __shared__ double Uij[32][33], Uji[32][33];
   const TilePair p = tp[blockIdx.x];
   const int q = p.blk / 3, kind = p.blk % 3;
@@ -505,6 +508,8 @@ __global__ void __launch_bounds__(256) k_ladder_sq(DevTab t, const double *__res
   const double *__restrict__ B = (kind == 1 ? w2 : w3) + off;
   double *__restrict__ Lq = L + off;
   const double c = kind == 1 ? h1 : h1 * h2, d = kind == 1 ? h2 : 1.0;
+  const bool ua = kind != 1 || ((nz1 >> PPPH) & 1u);
+  const bool ub = (nz1 >> (kind == 0 ? HHHH : (kind == 1 ? PHHH : PPPP))) & 1u;
   const int i0 = p.ti * 32, j0 = p.tj * 32;
   // all sixteen loads of a thread in flight together (unconditional, at an
   // in-bounds address; element 0 outside the block), then the LDS stores
@@ -515,10 +520,10 @@ __global__ void __launch_bounds__(256) k_ladder_sq(DevTab t, const double *__res
     const bool in1 = i0 + li < n && j0 + lj < n, in2 = j0 + li < n && i0 + lj < n;
     const int64_t a1 = in1 ? (i0 + li) + (int64_t)(j0 + lj) * n : 0;
     const int64_t a2 = in2 ? (j0 + li) + (int64_t)(i0 + lj) * n : 0;
-    ra[u][0] = A[a1];
-    ra[u][1] = B[a1];
-    ra[u][2] = A[a2];
-    ra[u][3] = B[a2];
+    ra[u][0] = ua ? A[a1] : 0.0;
+    ra[u][1] = ub ? B[a1] : 0.0;
+    ra[u][2] = ua ? A[a2] : 0.0;
+    ra[u][3] = ub ? B[a2] : 0.0;
   }
   double wij[4], wji[4];
 #pragma unroll
@@ -539,9 +544,9 @@ __global__ void __launch_bounds__(256) k_ladder_sq(DevTab t, const double *__res
 }
 
 void launch_ladder_sq(hipStream_t s, const DevTab &t, const double *w1, const double *w2, const double *w3,
-                      double *L, const TilePair *tp, int ntp, double h1, double h2) {
+                      double *L, const TilePair *tp, int ntp, double h1, double h2, uint32_t nz1) {
   if (ntp <= 0) return;
-  hipLaunchKernelGGL(k_ladder_sq, dim3(ntp), dim3(256), 0, s, t, w1, w2, w3, L, tp, h1, h2);
+  hipLaunchKernelGGL(k_ladder_sq, dim3(ntp), dim3(256), 0, s, t, w1, w2, w3, L, tp, h1, h2, nz1);
 }
 
 }  // namespace qd

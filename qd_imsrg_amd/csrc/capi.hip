@@ -81,9 +81,16 @@ struct GemmList {
   std::vector<int2> hzb;
   // the list without the segments that read sub-block types of r1 absent from
   // a mask (those products are exact zeros), per mask: device problems (same
-  // tiles and shapes; a problem without segments writes a zero tile) and
-  // segments
-  std::map<uint32_t, std::pair<GemmProb *, GemmSeg *>> var;
+  // tiles and shapes; a problem left without segments has nseg = -1 and its
+  // tiles are skipped, its output region being zeroed once per BCH expansion
+  // by the chunks zc) and segments
+  struct Variant {
+    GemmProb *probs = nullptr;
+    GemmSeg *segs = nullptr;
+    ZChunk *zc = nullptr;
+    int nzc = 0;
+  };
+  std::map<uint32_t, Variant> var;
 };
 }  // namespace
 
@@ -333,10 +340,12 @@ void tiles_of(const GemmList &g, int pi, std::vector<GemmTile> &out) {
 // The problems and segments of g without the segments that read a sub-block
 // type of r1 that is not in nz1 (zero in every block): their products are
 // exact zeros.  Shapes and tile lists are g's.
-void variant_of(const GemmList &g, uint32_t nz1, std::vector<GemmProb> &vp, std::vector<GemmSeg> &vs) {
+void variant_of(const GemmList &g, uint32_t nz1, std::vector<GemmProb> &vp, std::vector<GemmSeg> &vs,
+                std::vector<ZChunk> *zc = nullptr) {
   auto dead = [&](int info) { return info >= 0 && (info >> 8) == S_R1 && !((nz1 >> (info & 0xff)) & 1u); };
   vp = g.hprobs;
   vs.clear();
+  if (zc) zc->clear();
   for (auto &p : vp) {
     const int s0 = p.seg0, n = p.nseg;
     p.seg0 = (int)vs.size();
@@ -346,6 +355,15 @@ void variant_of(const GemmList &g, uint32_t nz1, std::vector<GemmProb> &vp, std:
         vs.push_back(g.hsegs[i]);
         p.nseg++;
       }
+    // no product left: the output is zero; a contiguous output is zeroed
+    // once (chunks) and the problem's tiles are skipped, any other keeps its
+    // tiles, which write zeros
+    if (p.nseg == 0 && (p.beta1 || p.ldc == p.M)) {
+      if (!p.beta1 && zc)
+        for (int64_t k = 0; k < (int64_t)p.M * p.N; k += 65536)
+          zc->push_back({p.c_slot, (int)std::min<int64_t>(65536, (int64_t)p.M * p.N - k), p.c_off + k});
+      p.nseg = -1;
+    }
   }
   if (vs.empty()) vs.push_back(GemmSeg{});
 }
@@ -471,7 +489,7 @@ int upload_tiles(qd_context *c, GemmList &g) {
     std::vector<GemmProb> vp;
     std::vector<GemmSeg> vs;
     variant_of(g, v.first, vp, vs);
-    HIPC(hipMemcpy(v.second.first, vp.data(), sizeof(GemmProb) * np, hipMemcpyHostToDevice));
+    HIPC(hipMemcpy(v.second.probs, vp.data(), sizeof(GemmProb) * np, hipMemcpyHostToDevice));
   }
   g.ntiles_all = (int)all.size();
   g.ntiles_pre = (int)pre.size();
@@ -954,23 +972,35 @@ bool pandya_on(const qd_cc *hcc, const qd_cc *ecc) {
   return hcc && ecc && !hcc->zero && !ecc->zero;
 }
 
-// device problems and segments of g for the context's current mask of r1
-int gemm_ptrs(qd_context *c, GemmList &g, GemmProb **pp, GemmSeg **ps) {
-  *pp = g.probs;
-  *ps = g.segs;
+// the variant of g for the context's current mask of r1 (built on first use),
+// or null when nothing is skipped
+int gemm_variant(qd_context *c, GemmList &g, GemmList::Variant **out) {
+  *out = nullptr;
   if (!c->nz_on || c->nz_mask == (uint32_t)ZB_ALL) return QD_OK;
   auto it = g.var.find(c->nz_mask);
   if (it == g.var.end()) {
     std::vector<GemmProb> vp;
     std::vector<GemmSeg> vs;
-    variant_of(g, c->nz_mask, vp, vs);
-    std::pair<GemmProb *, GemmSeg *> d{nullptr, nullptr};
+    std::vector<ZChunk> zc;
+    variant_of(g, c->nz_mask, vp, vs, &zc);
+    GemmList::Variant v;
+    v.nzc = (int)zc.size();
+    if (zc.empty()) zc.push_back(ZChunk{});
     int rc;
-    if ((rc = dev_upload(c, vp, &d.first)) || (rc = dev_upload(c, vs, &d.second))) return rc;
-    it = g.var.emplace(c->nz_mask, d).first;
+    if ((rc = dev_upload(c, vp, &v.probs)) || (rc = dev_upload(c, vs, &v.segs)) || (rc = dev_upload(c, zc, &v.zc)))
+      return rc;
+    it = g.var.emplace(c->nz_mask, v).first;
   }
-  *pp = it->second.first;
-  *ps = it->second.second;
+  *out = &it->second;
+  return QD_OK;
+}
+
+int gemm_ptrs(qd_context *c, GemmList &g, GemmProb **pp, GemmSeg **ps) {
+  GemmList::Variant *v;
+  int rc = gemm_variant(c, g, &v);
+  if (rc) return rc;
+  *pp = v ? v->probs : g.probs;
+  *ps = v ? v->segs : g.segs;
   return QD_OK;
 }
 
@@ -1046,7 +1076,8 @@ enum { TB_PART_PRE = 1, TB_PART_MAIN = 2 };
 // already in wl (launched by the caller right after the products)
 constexpr int TB_LSQ_DONE = 16;
 void run_ladder_sq(qd_context *c, const qd_op *r1, const qd_op *r2) {
-  launch_ladder_sq(c->stream, c->dt, c->w1, c->w2, c->w3, c->wl, c->lsq, c->nlsq, r1->herm, r2->herm);
+  launch_ladder_sq(c->stream, c->dt, c->w1, c->w2, c->w3, c->wl, c->lsq, c->nlsq, r1->herm, r2->herm,
+                   c->nz_on ? c->nz_mask : (uint32_t)ZB_ALL);
 }
 // paired: the main pass runs the paired box tasks (one staged grid writes the
 // outputs of both orientations of a class-pair box); valid when the operators
@@ -1814,10 +1845,14 @@ int qd_bch_expand(qd_context *c, qd_op *hs, const qd_op *g, const qd_op *h, cons
     memcpy(&c->nz_mask, c->host_scal, sizeof(uint32_t));
     c->nz_on = true;
     // the lists for this mask exist before any term is captured in a graph
-    // (no allocation during capture)
-    GemmProb *pp;
-    GemmSeg *ps;
-    if ((rc = gemm_ptrs(c, c->g_all, &pp, &ps)) || (rc = gemm_ptrs(c, c->g_acc, &pp, &ps))) return rc;
+    // (no allocation during capture); the outputs of the problems they leave
+    // out are zeroed here, once for all terms (nothing else writes them)
+    for (GemmList *gl : {&c->g_all, &c->g_acc}) {
+      GemmList::Variant *v;
+      if ((rc = gemm_variant(c, *gl, &v))) return rc;
+      if (v) launch_zero_chunks(c->stream, make_slots(c, nullptr, nullptr, nullptr, nullptr, nullptr), v->zc, v->nzc);
+    }
+    HIPC(hipGetLastError());
   }
   // HS = H ; AD_0 = H ; then AD_k = [G, AD_{k-1}], HS += cof(k) AD_k
   if ((rc = qd_copy_arrays(c, h, hs))) return rc;

@@ -570,6 +570,7 @@ __global__ void __launch_bounds__(256) k_gemm(const GemmProb *__restrict__ probs
     const GemmTile tl = tiles[w];
     if (tl.prob < 0) continue;  // padding of a short XCD queue (uniform over the work-group)
     const GemmProb pr = probs[tl.prob];
+    if (pr.nseg < 0) continue;  // a problem left out of this list variant (uniform)
     run_tile<SH, P>(pr, segs, slots, tl, lds);
     __syncthreads();  // the next tile reuses the LDS staging area
   }
@@ -593,6 +594,7 @@ __device__ __forceinline__ void gemm_mix_body(const GemmProb *__restrict__ probs
     const GemmTile tl = tiles[w];
     if (tl.prob < 0) continue;  // padding of a short XCD queue (uniform over the work-group)
     const GemmProb pr = probs[tl.prob];
+    if (pr.nseg < 0) continue;  // a problem left out of this list variant (uniform)
     if (pr.shape == SH_32x32) run_tile<SH_32x32, P, KC>(pr, segs, slots, tl, lds);
     else run_tile<SH_64x64, P, KC>(pr, segs, slots, tl, lds);
     __syncthreads();  // the next tile reuses the LDS staging area

@@ -404,6 +404,16 @@ __global__ void __launch_bounds__(256) k_zero_scan(const double *__restrict__ v,
   if (__any(any) && (threadIdx.x & 63) == 0) atomicOr(nz, 1u << t.bit);
 }
 
+__global__ void __launch_bounds__(256) k_zero_chunks(Slots sl, const ZChunk *__restrict__ chunks) {
+  const ZChunk c = chunks[blockIdx.x];
+  double *__restrict__ p = sl.p[c.slot] + c.off;
+  for (int k = threadIdx.x; k < c.len; k += 256) p[k] = 0.0;
+}
+
+void launch_zero_chunks(hipStream_t s, const Slots &slots, const ZChunk *chunks, int n) {
+  if (n > 0) hipLaunchKernelGGL(k_zero_chunks, dim3(n), dim3(256), 0, s, slots, chunks);
+}
+
 void launch_zero_scan(hipStream_t s, const double *v, const ZTile *tiles, int ntiles, uint32_t *nz) {
   hipLaunchKernelGGL(k_zero_scan_init, dim3(1), dim3(1), 0, s, nz);
   if (ntiles > 0) hipLaunchKernelGGL(k_zero_scan, dim3(ntiles), dim3(256), 0, s, v, tiles, nz);

@@ -34,6 +34,9 @@ struct GemmProb {  // C(M,N) (+)= sum over segments
   int c_slot, ldc, M, N, beta1, seg0, nseg, shape;
 };
 struct GemmTile { int prob, tm, tn; };
+// a range of a workspace slot to zero: len doubles from off
+struct ZChunk { int slot, len; int64_t off; };
+void launch_zero_chunks(hipStream_t s, const Slots &slots, const ZChunk *chunks, int n);
 constexpr int GEMM_TK = 32;
 // tile shapes of the grouped GEMM (rows x cols of C per work-group)
 // SH_64x16K: a 64 x 16 tile whose four waves split every K chunk and add
@@ -169,8 +172,9 @@ void launch_sum_halves(hipStream_t s, const double *w3, const double *w4, double
 // tj; TilePair::blk = 3 * block + 0 HHHH / 1 PHPH / 2 PPPP):
 // HHHH, PPPP: L = (h1 w1^T - h2 w1) + (h1 h2 w3^T - w3);
 // PHPH:       L = (h1 w2^T - h2 w2) + (h1 w1^T - h2 w1)
+// nz1: nonzero sub-block types of r1 (ZB bits; ZB_ALL outside the BCH terms)
 void launch_ladder_sq(hipStream_t s, const DevTab &t, const double *w1, const double *w2, const double *w3,
-                      double *L, const TilePair *tp, int ntp, double h1, double h2);
+                      double *L, const TilePair *tp, int ntp, double h1, double h2, uint32_t nz1);
 // cc1 / cc2: calc_cc of r1 / r2 when the caller holds them (then the CC
 // mat-vecs of commutator_121 read them instead of gathering V), else null
 // OB_111 needs the one-body products of the one-body product list in the
