@@ -1181,7 +1181,7 @@ int full_commutator(qd_context *c, const qd_op *r1, const qd_op *r2, qd_op *r3, 
     if (!rc) {
       Ops o{r1->d, r2->d, r1->herm, r2->herm};
       Timer tm(c, TC_OTHER);
-      launch_zerobody(c->stream, c->dt, o, c->pphh_segs, c->npphh_segs, c->partials, r3->d, 3);
+      launch_zerobody(c->stream, c->dt, o, c->pphh_segs, c->pphh_tiles, c->npphh_tiles, c->partials, r3->d, 3);
       if (hipGetLastError() != hipSuccess) rc = fail(QD_E_HIP, "zero-body launch failed");
     }
   };
@@ -1224,7 +1224,7 @@ int full_commutator(qd_context *c, const qd_op *r1, const qd_op *r2, qd_op *r3, 
   if (!rc && !side) {
     Ops o{r1->d, r2->d, r1->herm, r2->herm};
     Timer tm(c, TC_OTHER);
-    launch_zerobody(c->stream, c->dt, o, c->pphh_segs, c->npphh_segs, c->partials, r3->d, 3);
+    launch_zerobody(c->stream, c->dt, o, c->pphh_segs, c->pphh_tiles, c->npphh_tiles, c->partials, r3->d, 3);
     if (hipGetLastError() != hipSuccess) rc = fail(QD_E_HIP, "zero-body launch failed");
   }
   if (side) {
@@ -1452,7 +1452,7 @@ int qd_context_create(int nbody, int nshell, qd_context **out) {
   if (hipHostMalloc((void **)&c->host_scal, sizeof(double) * 8, hipHostMallocDefault) != hipSuccess)
     return bail(fail(QD_E_HIP, "hipHostMalloc failed"));
   if ((rc = build_worklists(c))) return bail(rc);
-  if (c->npphh_tiles > 1024 * 64 || c->npphh_segs + 1 > 1024 * 64)
+  if (c->npphh_tiles + 1 > 1024 * 64)
     return bail(fail(QD_E_LAYOUT, "too many pphh tiles for the reduction buffer"));
   if ((rc = noalloc_setup(&c->noalloc_scratch))) return bail(rc);
   c->noalloc_cap = NOALLOC_PARTS;
@@ -1646,14 +1646,14 @@ int qd_cc_zero(qd_context *c, qd_cc *cc) {
 int qd_commutator_110(qd_context *c, const qd_op *r1, const qd_op *r2, double *out) {
   CHK3(r1, r2, (const qd_op *)nullptr);
   Ops o{r1->d, r2->d, r1->herm, r2->herm};
-  launch_zerobody(c->stream, c->dt, o, c->pphh_segs, c->npphh_segs, c->partials, c->scal, 1);
+  launch_zerobody(c->stream, c->dt, o, c->pphh_segs, c->pphh_tiles, c->npphh_tiles, c->partials, c->scal, 1);
   HIPC(hipGetLastError());
   return read_scalars(c, 1, out);
 }
 int qd_commutator_220(qd_context *c, const qd_op *r1, const qd_op *r2, double *out) {
   CHK3(r1, r2, (const qd_op *)nullptr);
   Ops o{r1->d, r2->d, r1->herm, r2->herm};
-  launch_zerobody(c->stream, c->dt, o, c->pphh_segs, c->npphh_segs, c->partials, c->scal, 2);
+  launch_zerobody(c->stream, c->dt, o, c->pphh_segs, c->pphh_tiles, c->npphh_tiles, c->partials, c->scal, 2);
   HIPC(hipGetLastError());
   return read_scalars(c, 1, out);
 }
@@ -1999,7 +1999,7 @@ int qd_magnus_flow(qd_context *c, const qd_op *h0, qd_op *hs, qd_op *omega, cons
     int r;
     if ((r = qd_mbpt2(c, h, &m2))) return r;
     Ops zo{eta->d, h->d, eta->herm, h->herm};
-    launch_zerobody(c->stream, c->dt, zo, c->pphh_segs, c->npphh_segs, c->partials, c->scal, 3);
+    launch_zerobody(c->stream, c->dt, zo, c->pphh_segs, c->pphh_tiles, c->npphh_tiles, c->partials, c->scal, 3);
     HIPC(hipGetLastError());
     if ((r = read_scalars(c, 1, &deds))) return r;
     crit = rule_deds ? std::fabs(deds) : std::fabs(m2);

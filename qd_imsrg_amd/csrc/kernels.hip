@@ -108,6 +108,45 @@ __global__ void __launch_bounds__(256) k_cc_gemv(DevTab t, const double *__restr
   }
 }
 
+// The same mat-vec without a CC set: each CC element is gathered from V
+// through the calc_cc table (the element k_calc_cc_map would write), in the
+// work split and summation order of k_cc_gemv.  Saves the CC set's write and
+// re-read, and replaces k_cc_matvec's per-element pair resolution (several
+// dependent loads) by one table load.
+__global__ void __launch_bounds__(256) k_cc_gemv_map(DevTab t, const double *__restrict__ v, int herm,
+                                                     const double *__restrict__ f, double *__restrict__ u) {
+  __shared__ double part[8][33];
+  const int rl = threadIdx.x & 31, cg = threadIdx.x >> 5;
+  const int g = blockIdx.x * 32 + rl;
+  double s = 0.0;
+  if (g < t.M * t.M) {
+    const int K = t.cc_rowblk[g];  // CC block of device row g
+    const int d = g - t.cc_ab_start[K], r = t.cc_r[K], nb = t.cc_nb[K];
+    const uint32_t *col = t.cc_gmap + t.cc_off[K] + d;
+    const uint32_t *hp = t.cc_hp + t.cc_hp_start[K];
+#pragma unroll 4
+    for (int c = cg; c < nb; c += 8) {
+      const uint32_t m = col[(int64_t)c * r], k = m & CC_ZERO;
+      double x = 0.0;
+      if (k != CC_ZERO) {
+        x = v[k];
+        if ((m >> 30) & 1u) x = herm * x;
+        if (m >> 31) x = -x;
+      }
+      const int i = lo16(hp[c]), a = hi16(hp[c]);
+      s += x * f[t.off_fph + (a - t.n) + (int64_t)i * t.mp];
+    }
+  }
+  part[cg][rl] = s;
+  __syncthreads();
+  if (cg == 0 && g < t.M * t.M) {
+    double w = part[0][rl];
+    for (int k = 1; k < 8; k++) w += part[k][rl];
+    const uint32_t pq = t.cc_ab[g];
+    u[(int64_t)lo16(pq) * t.M + hi16(pq)] = w;
+  }
+}
+
 // Output element e of fhh, fpp, fph (flat offset 1 + e): kind 0 hh, 1 pp, 2 ph;
 // (i, j) within its block, (x, y) as states
 __device__ __forceinline__ void ob_decode(const DevTab &t, int64_t e, int &kind, int &i, int &j, int &x, int &y) {
@@ -238,6 +277,10 @@ void launch_onebody(hipStream_t s, const DevTab &t, const Ops &o, double *out, i
     const int m = t.M * t.M;
     hipLaunchKernelGGL(k_cc_gemv, dim3((m + 31) / 32), dim3(256), 0, s, t, cc2, o.v1, u2);
     hipLaunchKernelGGL(k_cc_gemv, dim3((m + 31) / 32), dim3(256), 0, s, t, cc1, o.v2, u1);
+  } else if ((flags & OB_121) && t.cc_gmap) {
+    const int m = t.M * t.M;
+    hipLaunchKernelGGL(k_cc_gemv_map, dim3((m + 31) / 32), dim3(256), 0, s, t, o.v2, o.h2, o.v1, u2);
+    hipLaunchKernelGGL(k_cc_gemv_map, dim3((m + 31) / 32), dim3(256), 0, s, t, o.v1, o.h1, o.v2, u1);
   } else if (flags & OB_121) {
     launch_cc_matvec(s, t, o.v2, o.h2, o.v1, u2);
     launch_cc_matvec(s, t, o.v1, o.h1, o.v2, u1);
@@ -275,17 +318,19 @@ __device__ __forceinline__ double block_sum256(double v) {
 // (sum fph1 .* fph2 + sum over blocks Vpphh1 .* Vpphh2).  Work-group 0 takes
 // the fph product, work-group g >= 1 the pphh segment g-1; k_zero_final adds
 // the partials in a fixed order.
-__global__ void __launch_bounds__(256) k_zerobody_part(DevTab t, Ops o, const Seg2 *__restrict__ segs, int which,
-                                                       double *part) {
+__global__ void __launch_bounds__(256) k_zerobody_part(DevTab t, Ops o, const Seg2 *__restrict__ segs,
+                                                       const Tile2 *__restrict__ tiles, int which, double *part) {
   double a = 0.0;
   if (blockIdx.x == 0) {
     if (which & 1) {
       const int64_t m = (int64_t)t.mp * t.n;
       for (int64_t e = threadIdx.x; e < m; e += 256) a += o.v1[t.off_fph + e] * o.v2[t.off_fph + e];
     }
-  } else if (which & 2) {
-    const Seg2 sg = segs[blockIdx.x - 1];
-    for (int e = threadIdx.x; e < sg.size; e += 256) a += o.v1[sg.off + e] * o.v2[sg.off + e];
+  } else if (which & 2) {  // one tile of 256 elements of a Vpphh sub-block
+    const Tile2 tl = tiles[blockIdx.x - 1];
+    const Seg2 sg = segs[tl.seg];
+    const int e = tl.start + threadIdx.x;
+    if (e < sg.size) a = o.v1[sg.off + e] * o.v2[sg.off + e];
   }
   a = block_sum256(a);
   if (threadIdx.x == 0) part[blockIdx.x] = a;
@@ -297,10 +342,10 @@ __global__ void __launch_bounds__(256) k_zero_final(Ops o, const double *__restr
   if (threadIdx.x == 0) *dst = (double)(o.h1 - o.h2) * part[0] + (double)(o.h1 - o.h2) * b;
 }
 
-void launch_zerobody(hipStream_t s, const DevTab &t, const Ops &o, const Seg2 *pphh_segs, int npphh,
-                     double *part, double *dst, int which) {
-  hipLaunchKernelGGL(k_zerobody_part, dim3(1 + npphh), dim3(256), 0, s, t, o, pphh_segs, which, part);
-  hipLaunchKernelGGL(k_zero_final, dim3(1), dim3(256), 0, s, o, part, 1 + npphh, dst);
+void launch_zerobody(hipStream_t s, const DevTab &t, const Ops &o, const Seg2 *pphh_segs, const Tile2 *pphh_tiles,
+                     int ntiles, double *part, double *dst, int which) {
+  hipLaunchKernelGGL(k_zerobody_part, dim3(1 + ntiles), dim3(256), 0, s, t, o, pphh_segs, pphh_tiles, which, part);
+  hipLaunchKernelGGL(k_zero_final, dim3(1), dim3(256), 0, s, o, part, 1 + ntiles, dst);
 }
 
 // ---------------------------------------------------------------------------

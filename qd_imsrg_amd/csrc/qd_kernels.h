@@ -187,8 +187,8 @@ void launch_onebody(hipStream_t s, const DevTab &t, const Ops &o, double *out, i
                     const double *w1, const double *w2, const double *w3, double *u1, double *u2,
                     const double *cc1, const double *cc2, const int *list221, int n221, const uint32_t *g221);
 // zero-body part (commutator_110 + commutator_220) into *dst (device scalar)
-void launch_zerobody(hipStream_t s, const DevTab &t, const Ops &o, const Seg2 *pphh_segs, int npphh,
-                     double *part, double *dst, int which /*1: 110, 2: 220, 3: both*/);
+void launch_zerobody(hipStream_t s, const DevTab &t, const Ops &o, const Seg2 *pphh_segs, const Tile2 *pphh_tiles,
+                     int ntiles, double *part, double *dst, int which /*1: 110, 2: 220, 3: both*/);
 void launch_calc_cc(hipStream_t s, const DevTab &t, const double *v, int herm, double *cc,
                     const Seg2 *cc_segs, const Tile2 *cc_tiles, int ntiles);
 void launch_white(hipStream_t s, const DevTab &t, const double *rec, int hrec, double *gen,
