@@ -186,10 +186,14 @@ class WarmStart:
                 o.close()
 
 
-def warm_flow(ctx, ws: WarmStart, omega: float, H0, HS, OMEGA, diis: int = 6, max_iter: int = 100):
+def warm_flow(ctx, ws: WarmStart, omega: float, H0, HS, OMEGA, diis: int = 6, max_iter: int = 15):
     """Magnus flow for one frequency from the network's predicted Omega (with
-    the Pulay-accelerated continuation); the cold start (as shipped) is the
-    fallback when the warm flow fails or has not converged within max_iter.
+    the Pulay-accelerated continuation); the cold start with the same
+    acceleration is the fallback when the warm flow fails or has not
+    converged within max_iter steps.  From a good prediction the warm flow
+    takes 6-11 steps; a poor one (a relative error near 1 in Omega) crawls
+    through the plain map's slow tail before the acceleration starts (40
+    steps seen), so the budget cuts it short.
     Returns (result, fallback, iterations spent in the warm attempt)."""
     from . import QDError, magnus_flow
     ws.predict(omega, OMEGA)
@@ -201,5 +205,5 @@ def warm_flow(ctx, ws: WarmStart, omega: float, H0, HS, OMEGA, diis: int = 6, ma
             return r, False, spent
     except QDError:
         spent = max_iter
-    r, _ = magnus_flow(ctx, H0, HS, OMEGA)  # cold: Omega is reset to zero
+    r, _ = magnus_flow(ctx, H0, HS, OMEGA, diis=diis)  # cold: Omega is reset to zero
     return r, True, spent
