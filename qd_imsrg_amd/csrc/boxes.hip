@@ -52,8 +52,10 @@ __device__ __forceinline__ int box_of(const int *pre, int lo, int ng, int e) {
 // distance 0.5 / d of (a + 0.5) / d from the nearest integer
 __device__ __forceinline__ int qdiv(int a, float rd) { return (int)(((float)a + 0.5f) * rd); }
 
-// NT threads per task, grids of up to VC elements, RC row grid entries
-template <int NT, int VC, int RC>
+// NT threads per task, grids of up to VC elements, RC row grid entries;
+// STAGE: the D staging path (TB_DSTAGE) is compiled in; SPLIT: paired boxes
+// write their transposed outputs in a second phase (below)
+template <int NT, int VC, int RC, bool STAGE, bool SPLIT>
 __global__ void __launch_bounds__(NT) k_box2b(DevTab t, Ops o, double *__restrict__ out,
                                                const BTask *__restrict__ tasks, const int *__restrict__ blist,
                                                int flags, const double *__restrict__ w1,
@@ -76,6 +78,7 @@ __global__ void __launch_bounds__(NT) k_box2b(DevTab t, Ops o, double *__restric
   __shared__ int bgo[BX_GMAX];                 // first row grid entry of the box in cp_grid
   __shared__ int bpr[BX_GMAX];                 // the box also writes the transposed box's outputs
   __shared__ int rtot_s;                       // row grid entries of the task
+  __shared__ int anyp_s;                       // some box of the task is paired
   const BTask tk = tasks[blockIdx.x];
   if (tk.block < 0) return;
   const int q = tk.block, mode = tk.mode, ng = tk.ng;
@@ -137,10 +140,12 @@ __global__ void __launch_bounds__(NT) k_box2b(DevTab t, Ops o, double *__restric
       bgo[b] = go + r_off;
       bpr[b] = pr;
     }
+    const bool anyp = __any(pr);
     if (b == ng - 1) {
       pre[ng] = e;
       fsz_s = q34 + fz;
       rtot_s = rz;
+      anyp_s = anyp;
     }
   } else if (dopd && (int)threadIdx.x - 64 < ng) {
     // wave 1, one lane per box: D(x3,x1|x2,x4), D(x3,x2|x1,x4),
@@ -207,10 +212,10 @@ __global__ void __launch_bounds__(NT) k_box2b(DevTab t, Ops o, double *__restric
   // a time, in the grid's LDS before the grids), and each thread keeps the
   // sum for its grid elements e = tid + k NT, in order, in registers
   constexpr int KQ = (VC + NT - 1) / NT;
-  const bool stage_d = dopd && mode == 0 && dsign != 0.0 && (flags & TB_DSTAGE);
-  double pq[KQ];
+  const bool stage_d = STAGE && dopd && mode == 0 && dsign != 0.0 && (flags & TB_DSTAGE);
+  double pq[STAGE ? KQ : 1];
 #pragma unroll
-  for (int k = 0; k < KQ; k++) pq[k] = 0.0;
+  for (int k = 0; k < (STAGE ? KQ : 1); k++) pq[k] = 0.0;
   if (stage_d) {
     double *__restrict__ gd = (double *)gg;  // [2][total]
     for (int h = 0; h < 2; h++) {
@@ -234,7 +239,7 @@ __global__ void __launch_bounds__(NT) k_box2b(DevTab t, Ops o, double *__restric
       __syncthreads();
       int b = 0;
 #pragma unroll
-      for (int k = 0; k < KQ; k++) {
+      for (int k = 0; k < (STAGE ? KQ : 1); k++) {
         const int e = threadIdx.x + k * NT;
         if (e < total) {
           if (b + 1 < ng && pre[b + 1] <= e) b = box_of(pre, b + 1, ng, e);
@@ -328,11 +333,60 @@ __global__ void __launch_bounds__(NT) k_box2b(DevTab t, Ops o, double *__restric
   auto sub_of = [](int ta, int tb) {
     return tb == 0 ? (ta == 0 ? HHHH : (ta == 1 ? PHHH : PPHH)) : (tb == 1 ? (ta == 1 ? PHPH : PPPH) : PPPP);
   };
+  // ladder inputs (signed combinations of the products of matrix_mult222,
+  // src/commutators.f90:999-1127, and the summed products for Vpphh,
+  // Vphhh, Vppph): xa = w1(I,J), xb = the summed product at (I,J) (L for
+  // PPHH and PPPH, whose sums come in two halves; w3 for PHHH), xc =
+  // w2(I,J) for PHHH / PPPH; the square sub-blocks read their whole ladder
+  // term from L.  A value the sub-block does not use is read at element 0
+  // and ignored.  One set per stored output.
+  const bool lad = first && (flags & TB_LADDER);
+  auto ld = [&](int sub, int64_t idx, double &xa, double &xb, double &xc) {
+    const bool sq = sub == HHHH || sub == PPPP || sub == PHPH;
+    const bool usec = sub == PHHH || sub == PPPH;
+    xa = (sq ? L : w1)[sub == PPHH ? 0 : idx];
+    xb = (sub == PHHH ? w3 : L)[sq ? 0 : idx];
+    xc = w2[usec ? idx : 0];
+  };
+  auto finish = [&](int sub, double acc, double xa, double xb, double xc) {
+    if (lad) {
+      switch (sub) {
+        case HHHH:
+        case PPPP:
+        case PHPH: acc += xa; break;
+        case PPHH: acc += xb; break;
+        case PHHH: acc += h1 * xc - h2 * xa + xb; break;
+        case PPPH: acc += -h2 * xa + h1 * xc + xb; break;
+        default: break;
+      }
+    }
+    return acc;
+  };
+  // the bra slabs of a split box add to what the ket slabs wrote
+  const bool rd = first ? ((flags & TB_ACCUM) || (mode == 1 && do122)) : (flags & TB_ACCUM) != 0;
+  // Whole boxes with paired boxes among them write in two phases: the outputs
+  // <I|Z|J> of every grid element here, with consecutive threads on
+  // consecutive rows (consecutive stored rows of a column: few cache lines
+  // per wave); the paired outputs <J|Z|I> = h_Z (...) after all grid elements
+  // are contracted, from the values kept in LDS, with consecutive threads on
+  // consecutive columns, so that their stored rows are consecutive too.
+  // Each thread keeps its values in registers (vq, in order, like pq) until
+  // the grids in LDS are no longer read.
+  // (The narrow form only: in the wide form, one 16-wave task per CU, the
+  // extra barriers cost more than the scattered writes: N=42 R=20 two-body
+  // class 2.30 -> 2.39 ms, while N=20 R=16 gains 0.481 -> 0.461 ms.)
+  const bool split = SPLIT && mode == 0 && anyp_s;
+  constexpr int KV = SPLIT ? KQ : 1;
+  double vq[KV];
+#pragma unroll
+  for (int k = 0; k < KV; k++) vq[k] = 0.0;
   for (int e = threadIdx.x, b = 0; e < total; e += NT) {
-    // this element's staged Pandya sum; the queue moves up by one
+    // this element's staged Pandya sum; the queues move up by one
     const double pdv = pq[0];
 #pragma unroll
-    for (int k = 0; k + 1 < KQ; k++) pq[k] = pq[k + 1];
+    for (int k = 0; k + 1 < (STAGE ? KQ : 1); k++) pq[k] = pq[k + 1];
+#pragma unroll
+    for (int k = 0; k + 1 < KV; k++) vq[k] = vq[k + 1];
     if (b + 1 < ng && pre[b + 1] <= e) b = box_of(pre, b + 1, ng, e);
     const int4 B = bx[b];
     const int le = e - pre[b], c = qdiv(le, brc[b].x), r = le - c * B.y;
@@ -345,9 +399,9 @@ __global__ void __launch_bounds__(NT) k_box2b(DevTab t, Ops o, double *__restric
     // one orientation of a same-channel pair
     if ((S.z && sr) || (same34 && sc)) continue;
     // stored: <I|Z|J> with row type >= column type; in a paired box also
-    // <J|Z|I> (column type >= row type)
-    const bool st1 = tyr >= tyc, st2 = bpr[b] && tyc >= tyr;
-    if (!st1 && !st2) continue;
+    // <J|Z|I> (column type >= row type), here unless written in the second phase
+    const bool st1 = tyr >= tyc, st2 = bpr[b] && tyc >= tyr && !split;
+    if (!st1 && !(bpr[b] && tyc >= tyr)) continue;
     const int s1 = S.x, s2 = S.y;
     const int i1 = qdiv(rr, brc[b].y), i2 = rr - i1 * s2, i3 = qdiv(cc, rs4), i4 = cc - i3 * s4;
     // stored positions (sub-blocks of the flat layout); 0 where not stored
@@ -376,30 +430,13 @@ __global__ void __launch_bounds__(NT) k_box2b(DevTab t, Ops o, double *__restric
       d2 = dg(2, i4 * s2 + i2, i1 * s3 + i3);
       d3 = dg(3, i4 * s1 + i1, i2 * s3 + i3);
     }
-    // ladder inputs (signed combinations of the products of matrix_mult222,
-    // src/commutators.f90:999-1127, and the summed products for Vpphh,
-    // Vphhh, Vppph): xa = w1(I,J), xb = the summed product at (I,J) (L for
-    // PPHH and PPPH, whose sums come in two halves; w3 for PHHH), xc =
-    // w2(I,J) for PHHH / PPPH; the square sub-blocks read their whole ladder
-    // term from L.  A value the sub-block does not use is read at element 0
-    // and ignored.  One set per stored output.
-    const bool lad = first && (flags & TB_LADDER);
     double xa1 = 0.0, xb1 = 0.0, xc1 = 0.0, xa2 = 0.0, xb2 = 0.0, xc2 = 0.0;
     if (lad) {
-      auto ld = [&](int sub, int64_t idx, double &xa, double &xb, double &xc) {
-        const bool sq = sub == HHHH || sub == PPPP || sub == PHPH;
-        const bool usec = sub == PHHH || sub == PPPH;
-        xa = (sq ? L : w1)[sub == PPHH ? 0 : idx];
-        xb = (sub == PHHH ? w3 : L)[sq ? 0 : idx];
-        xc = w2[usec ? idx : 0];
-      };
       ld(sub1, idx1, xa1, xb1, xc1);
-      ld(sub2, idx2, xa2, xb2, xc2);
+      if (!split) ld(sub2, idx2, xa2, xb2, xc2);
     }
-    // the bra slabs of a split box add to what the ket slabs wrote
-    const bool rd = first ? ((flags & TB_ACCUM) || (mode == 1 && do122)) : (flags & TB_ACCUM) != 0;
     const double o1 = rd ? out[idx1] : 0.0;
-    const double o2 = rd ? out[idx2] : 0.0;
+    const double o2 = rd && !split ? out[idx2] : 0.0;
     double val = 0.0;
     if (do122) {
       const double2 *G = gg + B.z;
@@ -435,26 +472,61 @@ __global__ void __launch_bounds__(NT) k_box2b(DevTab t, Ops o, double *__restric
     if (stage_d) val += pdv;
     else if (dopd) val += d0 - d1 + d2 - d3;
     if (sr != sc) val = -val;
+    vq[KV - 1] = val;
     if (!first) {  // ket slabs run before the main pass: the first writer of their outputs
       out[idx1] = o1 + val;
       continue;
     }
-    auto finish = [&](int sub, double acc, double xa, double xb, double xc) {
-      if (lad) {
-        switch (sub) {
-          case HHHH:
-          case PPPP:
-          case PHPH: acc += xa; break;
-          case PPHH: acc += xb; break;
-          case PHHH: acc += h1 * xc - h2 * xa + xb; break;
-          case PPPH: acc += -h2 * xa + h1 * xc + xb; break;
-          default: break;
-        }
-      }
-      return acc;
-    };
     if (st1) out[idx1] = finish(sub1, o1 + val, xa1, xb1, xc1);
     if (st2) out[idx2] = finish(sub2, o2 + hz * val, xa2, xb2, xc2);
+  }
+  if constexpr (!SPLIT) return;
+  if (!split) return;
+  // second phase: the values of this thread's grid elements (iterations j of
+  // the loop above, at vq[KQ - n + j]) into LDS at their element positions,
+  // then the paired outputs with consecutive threads on consecutive columns
+  __syncthreads();  // every contraction has read the grids
+  {
+    double *__restrict__ gv = (double *)gg;
+    const int n = total > (int)threadIdx.x ? (total - 1 - (int)threadIdx.x) / NT + 1 : 0;
+#pragma unroll
+    for (int k = 0; k < KV; k++) {
+      const int j = k - (KV - n);
+      if (j >= 0) gv[threadIdx.x + j * NT] = vq[k];
+    }
+  }
+  __syncthreads();
+  const double *__restrict__ gv = (const double *)gg;
+  const float rnc = 1.0f / (float)nc;
+  // four elements per thread and pass, all their loads in flight together
+  // (an element without a paired output reads element 0 and writes nothing)
+  constexpr int U2 = 4;
+  for (int e0 = threadIdx.x, b = 0; e0 < total; e0 += U2 * NT) {
+    double xa[U2], xb[U2], xc[U2], o2[U2], v[U2];
+    int64_t ix[U2];
+    int sb[U2];
+    bool ok[U2];
+#pragma unroll
+    for (int u = 0; u < U2; u++) {
+      const int e = min(e0 + NT * u, total - 1);
+      if (b + 1 < ng && pre[b + 1] <= e) b = box_of(pre, b + 1, ng, e);
+      const int nr = bx[b].y, le = e - pre[b], r = qdiv(le, rnc), c = le - r * nc;
+      const uint32_t er = rent[bs[b].w + r], ec = cent[c];
+      const int tyr = (int)((er >> 18) & 3u), tyc = (int)((ec >> 18) & 3u);
+      const bool sr = (er >> 16) & 1u, sc = (ec >> 16) & 1u;
+      ok[u] = e0 + NT * u < total && bpr[b] && ((er >> 17) & (ec >> 17) & 1u) && !(bs[b].z && sr) &&
+              !(same34 && sc) && tyc >= tyr;
+      const int Il = (int)(er & 0xffffu) - tstart[tyr], Jl = (int)(ec & 0xffffu) - tstart[tyc];
+      sb[u] = ok[u] ? sub_of(tyc, tyr) : HHHH;
+      ix[u] = ok[u] ? soff[tyc][tyr] + Jl + (int64_t)Il * tcount[tyc] : 0;
+      xa[u] = xb[u] = xc[u] = 0.0;
+      if (lad) ld(sb[u], ix[u], xa[u], xb[u], xc[u]);
+      o2[u] = rd ? out[ix[u]] : 0.0;
+      v[u] = gv[ok[u] ? pre[b] + c * nr + r : 0];
+    }
+#pragma unroll
+    for (int u = 0; u < U2; u++)
+      if (ok[u]) out[ix[u]] = finish(sb[u], o2[u] + hz * v[u], xa[u], xb[u], xc[u]);
   }
 }
 
@@ -462,12 +534,18 @@ void launch_box2b(hipStream_t s, const DevTab &t, const Ops &o, double *out, con
                   const int *blist, int flags, const double *w1, const double *w2, const double *w3,
                   const double *L, const double *F, const double *D, double dsign, bool wide) {
   if (ntasks <= 0) return;
-  if (wide)
-    hipLaunchKernelGGL((k_box2b<BX_WIDE_THREADS, BX_WIDE_VCAP, BX_WIDE_RCAP>), dim3(ntasks), dim3(BX_WIDE_THREADS), 0, s,
-                       t, o, out, tasks, blist, flags, w1, w2, w3, L, F, D, dsign);
-  else
-    hipLaunchKernelGGL((k_box2b<BX_THREADS, BX_VCAP, BX_RCAP>), dim3(ntasks), dim3(BX_THREADS), 0, s, t, o, out, tasks,
-                       blist, flags, w1, w2, w3, L, F, D, dsign);
+#define QD_BOX2B(NT_, VC_, RC_, ST_, SP_)                                                                   \
+  hipLaunchKernelGGL((k_box2b<NT_, VC_, RC_, ST_, SP_>), dim3(ntasks), dim3(NT_), 0, s, t, o, out, tasks, blist, \
+                     flags, w1, w2, w3, L, F, D, dsign)
+  const bool stage = (flags & TB_DSTAGE) && D && dsign != 0.0;
+  if (wide) {
+    if (stage) QD_BOX2B(BX_WIDE_THREADS, BX_WIDE_VCAP, BX_WIDE_RCAP, true, false);
+    else QD_BOX2B(BX_WIDE_THREADS, BX_WIDE_VCAP, BX_WIDE_RCAP, false, false);
+  } else {
+    if (stage) QD_BOX2B(BX_THREADS, BX_VCAP, BX_RCAP, true, true);
+    else QD_BOX2B(BX_THREADS, BX_VCAP, BX_RCAP, false, true);
+  }
+#undef QD_BOX2B
 }
 
 __global__ void __launch_bounds__(256) k_sum_halves(const double *__restrict__ w3, const double *__restrict__ w4,
