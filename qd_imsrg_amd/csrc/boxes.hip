@@ -54,8 +54,9 @@ __device__ __forceinline__ int qdiv(int a, float rd) { return (int)(((float)a + 
 
 // NT threads per task, grids of up to VC elements, RC row grid entries;
 // STAGE: the D staging path (TB_DSTAGE) is compiled in; SPLIT: paired boxes
-// write their transposed outputs in a second phase (below)
-template <int NT, int VC, int RC, bool STAGE, bool SPLIT>
+// write their transposed outputs in a second phase (below); LDIR: the square
+// sub-blocks' ladder terms are formed here from the products (TB_LDIRECT)
+template <int NT, int VC, int RC, bool STAGE, bool SPLIT, bool LDIR>
 __global__ void __launch_bounds__(NT) k_box2b(DevTab t, Ops o, double *__restrict__ out,
                                                const BTask *__restrict__ tasks, const int *__restrict__ blist,
                                                int flags, const double *__restrict__ w1,
@@ -338,9 +339,16 @@ __global__ void __launch_bounds__(NT) k_box2b(DevTab t, Ops o, double *__restric
   // Vphhh, Vppph): xa = w1(I,J), xb = the summed product at (I,J) (L for
   // PPHH and PPPH, whose sums come in two halves; w3 for PHHH), xc =
   // w2(I,J) for PHHH / PPPH; the square sub-blocks read their whole ladder
-  // term from L.  A value the sub-block does not use is read at element 0
-  // and ignored.  One set per stored output.
+  // term from L (without LDIR).  A value the sub-block does not use is read
+  // at element 0 and ignored.  One set per stored output.
   const bool lad = first && (flags & TB_LADDER);
+  // LDIR: the square sub-blocks (HHHH, PHPH, PPPP) read A = w1 and B = w3
+  // (w2 for PHPH) at (I, J) and at (J, I) and form L(I,J) = U(J,I) + W(I,J),
+  // U = h1 A + c B, W = -h2 A - d B (c = h1 h2, d = 1; PHPH c = h1, d = h2),
+  // as k_ladder_sq does; both orientations come from the same four values.
+  // A product of a sub-block type that is zero in r1 (ZB bits in flags >>
+  // TB_NZ_SHIFT) is not read.
+  const uint32_t nz1 = (uint32_t)flags >> TB_NZ_SHIFT;
   auto ld = [&](int sub, int64_t idx, double &xa, double &xb, double &xc) {
     const bool sq = sub == HHHH || sub == PPPP || sub == PHPH;
     const bool usec = sub == PHHH || sub == PPPH;
@@ -431,9 +439,26 @@ __global__ void __launch_bounds__(NT) k_box2b(DevTab t, Ops o, double *__restric
       d3 = dg(3, i4 * s1 + i1, i2 * s3 + i3);
     }
     double xa1 = 0.0, xb1 = 0.0, xc1 = 0.0, xa2 = 0.0, xb2 = 0.0, xc2 = 0.0;
+    const bool sqd = LDIR && tyr == tyc;  // st1 holds, and st2 in a paired box
+    bool za = false, zb = false;
     if (lad) {
-      ld(sub1, idx1, xa1, xb1, xc1);
-      if (!split) ld(sub2, idx2, xa2, xb2, xc2);
+      if constexpr (LDIR) {
+        if (sqd) {
+          za = tyr == 1 && !((nz1 >> PPPH) & 1u);  // w1.phph = r2.ppph^T r1.ppph
+          zb = !((nz1 >> (tyr == 0 ? HHHH : (tyr == 1 ? PHHH : PPPP))) & 1u);
+        }
+        const int64_t it = sqd ? soff[tyr][tyr] + Jl + (int64_t)Il * tcount[tyr] : 0;  // (J, I)
+        const double *Bq = tyr == 1 ? w2 : w3;
+        xa1 = sqd ? w1[za ? 0 : idx1] : 0.0;
+        xb1 = sqd ? Bq[zb ? 0 : idx1] : 0.0;
+        xa2 = sqd ? w1[za ? 0 : it] : 0.0;
+        xb2 = sqd ? Bq[zb ? 0 : it] : 0.0;
+        if (!sqd) ld(sub1, idx1, xa1, xb1, xc1);
+        if (!sqd && !split) ld(sub2, idx2, xa2, xb2, xc2);
+      } else {
+        ld(sub1, idx1, xa1, xb1, xc1);
+        if (!split) ld(sub2, idx2, xa2, xb2, xc2);
+      }
     }
     const double o1 = rd ? out[idx1] : 0.0;
     const double o2 = rd && !split ? out[idx2] : 0.0;
@@ -472,7 +497,15 @@ __global__ void __launch_bounds__(NT) k_box2b(DevTab t, Ops o, double *__restric
     if (stage_d) val += pdv;
     else if (dopd) val += d0 - d1 + d2 - d3;
     if (sr != sc) val = -val;
-    vq[KV - 1] = val;
+    if (LDIR && lad && sqd) {
+      const double a_ij = za ? 0.0 : xa1, b_ij = zb ? 0.0 : xb1, a_ji = za ? 0.0 : xa2, b_ji = zb ? 0.0 : xb2;
+      const double cq = tyr == 1 ? h1 : h1 * h2, dq = tyr == 1 ? h2 : 1.0;
+      xa1 = (h1 * a_ji + cq * b_ji) + (-h2 * a_ij - dq * b_ij);
+      xa2 = (h1 * a_ij + cq * b_ij) + (-h2 * a_ji - dq * b_ji);
+    }
+    // the second phase adds this value times h_Z; a direct square term is
+    // folded in (h_Z = +-1)
+    vq[KV - 1] = LDIR && lad && sqd ? val + hz * xa2 : val;
     if (!first) {  // ket slabs run before the main pass: the first writer of their outputs
       out[idx1] = o1 + val;
       continue;
@@ -520,7 +553,9 @@ __global__ void __launch_bounds__(NT) k_box2b(DevTab t, Ops o, double *__restric
       sb[u] = ok[u] ? sub_of(tyc, tyr) : HHHH;
       ix[u] = ok[u] ? soff[tyc][tyr] + Jl + (int64_t)Il * tcount[tyc] : 0;
       xa[u] = xb[u] = xc[u] = 0.0;
-      if (lad) ld(sb[u], ix[u], xa[u], xb[u], xc[u]);
+      const bool folded = LDIR && tyc == tyr;  // its ladder term is in the value
+      if (lad) ld(folded ? HHHH : sb[u], folded ? 0 : ix[u], xa[u], xb[u], xc[u]);
+      if (folded) xa[u] = 0.0;
       o2[u] = rd ? out[ix[u]] : 0.0;
       v[u] = gv[ok[u] ? pre[b] + c * nr + r : 0];
     }
@@ -534,17 +569,24 @@ void launch_box2b(hipStream_t s, const DevTab &t, const Ops &o, double *out, con
                   const int *blist, int flags, const double *w1, const double *w2, const double *w3,
                   const double *L, const double *F, const double *D, double dsign, bool wide) {
   if (ntasks <= 0) return;
-#define QD_BOX2B(NT_, VC_, RC_, ST_, SP_)                                                                   \
-  hipLaunchKernelGGL((k_box2b<NT_, VC_, RC_, ST_, SP_>), dim3(ntasks), dim3(NT_), 0, s, t, o, out, tasks, blist, \
+#define QD_BOX2B(NT_, VC_, RC_, ST_, SP_, LD_)                                                                  \
+  hipLaunchKernelGGL((k_box2b<NT_, VC_, RC_, ST_, SP_, LD_>), dim3(ntasks), dim3(NT_), 0, s, t, o, out, tasks, blist, \
                      flags, w1, w2, w3, L, F, D, dsign)
+#define QD_BOX2B_L(NT_, VC_, RC_, ST_, SP_)                     \
+  do {                                                           \
+    if (ldir) QD_BOX2B(NT_, VC_, RC_, ST_, SP_, true);           \
+    else QD_BOX2B(NT_, VC_, RC_, ST_, SP_, false);               \
+  } while (0)
   const bool stage = (flags & TB_DSTAGE) && D && dsign != 0.0;
+  const bool ldir = (flags & TB_LDIRECT) != 0;
   if (wide) {
-    if (stage) QD_BOX2B(BX_WIDE_THREADS, BX_WIDE_VCAP, BX_WIDE_RCAP, true, false);
-    else QD_BOX2B(BX_WIDE_THREADS, BX_WIDE_VCAP, BX_WIDE_RCAP, false, false);
+    if (stage) QD_BOX2B_L(BX_WIDE_THREADS, BX_WIDE_VCAP, BX_WIDE_RCAP, true, false);
+    else QD_BOX2B_L(BX_WIDE_THREADS, BX_WIDE_VCAP, BX_WIDE_RCAP, false, false);
   } else {
-    if (stage) QD_BOX2B(BX_THREADS, BX_VCAP, BX_RCAP, true, true);
-    else QD_BOX2B(BX_THREADS, BX_VCAP, BX_RCAP, false, true);
+    if (stage) QD_BOX2B_L(BX_THREADS, BX_VCAP, BX_RCAP, true, true);
+    else QD_BOX2B_L(BX_THREADS, BX_VCAP, BX_RCAP, false, true);
   }
+#undef QD_BOX2B_L
 #undef QD_BOX2B
 }
 

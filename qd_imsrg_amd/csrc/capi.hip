@@ -133,6 +133,9 @@ struct qd_context {
   // form (N=42 R=20 two-body class 2.39 -> 2.28 ms); in the narrow form the
   // extra staging rounds cost more than the reads (N=20 R=14 0.202 -> 0.220)
   bool d_stage = false;
+  // the box pass forms the square sub-blocks' ladder terms from the products
+  // (no k_ladder_sq pass and no round trip through wl); QD_LDIRECT=0: off
+  bool ldirect = true;
   double *fcls = nullptr;                                            // channel blocks of f (2 operands)
   TilePair *ptp = nullptr; int nptp = 0;  // 32x32 tile pairs of the Pandya combine
   TilePair *ftp = nullptr; int nftp = 0;  // 64x64 tile pairs of the fused Pandya product + combine
@@ -1076,6 +1079,7 @@ enum { TB_PART_PRE = 1, TB_PART_MAIN = 2 };
 // already in wl (launched by the caller right after the products)
 constexpr int TB_LSQ_DONE = 16;
 void run_ladder_sq(qd_context *c, const qd_op *r1, const qd_op *r2) {
+  if (c->ldirect) return;  // the box pass forms them from the products
   launch_ladder_sq(c->stream, c->dt, c->w1, c->w2, c->w3, c->wl, c->lsq, c->nlsq, r1->herm, r2->herm,
                    c->nz_on ? c->nz_mask : (uint32_t)ZB_ALL);
 }
@@ -1096,6 +1100,8 @@ int run_twobody(qd_context *c, const qd_op *r1, const qd_op *r2, qd_op *out, int
   if (part & TB_PART_MAIN) {
     if ((flags & TB_LADDER) && !lsq_done) run_ladder_sq(c, r1, r2);
     paired = paired && c->box_pairs;
+    if (c->ldirect)
+      flags |= TB_LDIRECT | (int)((c->nz_on ? c->nz_mask : (uint32_t)ZB_ALL) << TB_NZ_SHIFT);
     launch_box2b(c->stream, c->dt, o, out->d, paired ? c->bt1p : c->bt1, paired ? c->nbt1p : c->nbt1, c->blist,
                  flags | (c->d_stage ? TB_DSTAGE : 0), c->w1, c->w2, c->w3, c->wl, c->fcls,
                  (flags & TB_PANDYA) ? c->P : nullptr, c->d_sign, c->box_wide);
@@ -1443,6 +1449,7 @@ int qd_context_create(int nbody, int nshell, qd_context **out) {
     if (v >= 16 && v <= c->box_cap) c->box_cap = v;
   }
   if (const char *env = getenv("QD_BOX_PAIRS")) c->box_pairs = env[0] != '0';
+  if (const char *env = getenv("QD_LDIRECT")) c->ldirect = env[0] != '0';
   if ((rc = dev_alloc(c, sizeof(double) * ch.M * ch.M, &p))) return bail(rc); c->u1 = (double *)p;
   if ((rc = dev_alloc(c, sizeof(double) * ch.M * ch.M, &p))) return bail(rc); c->u2 = (double *)p;
   if ((rc = dev_alloc(c, sizeof(double) * 1024 * 64, &p))) return bail(rc); c->partials = (double *)p;

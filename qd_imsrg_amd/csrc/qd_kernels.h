@@ -98,7 +98,12 @@ __device__ __forceinline__ double felem(const DevTab &t, const double *__restric
 
 // flags of the fused two-body kernel
 // TB_DSTAGE: whole boxes stage the Pandya sub-blocks of D (run panels) in LDS
-enum { TB_122 = 1, TB_LADDER = 2, TB_PANDYA = 4, TB_ACCUM = 8, TB_DSTAGE = 32 };
+// TB_LDIRECT: the square sub-blocks' ladder terms are formed from the products
+// w1 and w3 (w2 for PHPH) at (I, J) and (J, I) instead of read from L; the
+// ZB bits of r1's nonzero sub-block types go in flags >> TB_NZ_SHIFT (a
+// product of a zero type is not read)
+enum { TB_122 = 1, TB_LADDER = 2, TB_PANDYA = 4, TB_ACCUM = 8, TB_DSTAGE = 32, TB_LDIRECT = 64 };
+constexpr int TB_NZ_SHIFT = 16;
 // flags of the one-body kernel; OB_MV: only the 121 mat-vecs into u1, u2
 // (they read the operands and CC sets only), OB_NOMV: u1, u2 are filled
 enum { OB_111 = 1, OB_121 = 2, OB_221 = 4, OB_ACCUM = 8, OB_MV = 16, OB_NOMV = 32 };
@@ -149,9 +154,9 @@ constexpr int BX_WIDE_RCAP = 768;
 // r1, F[fcls_size, 2 fcls_size) of r2
 void launch_fclass(hipStream_t s, const DevTab &t, const Ops &o, double *F);
 // all stored two-body outputs of the tasks: 1b x 2b (TB_122), ladder
-// combinations of the products w1..w3 and, for the square sub-blocks and
-// the summed products of Vpphh and Vppph, of L
-// (TB_LADDER; launch_ladder_sq has filled L), Pandya term from D
+// combinations of the products w1..w3 and, for the summed products of Vpphh
+// and Vppph and (without TB_LDIRECT) the square sub-blocks, of L
+// (TB_LADDER; launch_sum_halves and launch_ladder_sq have filled L), Pandya term from D
 // (TB_PANDYA); TB_ACCUM adds to out (mode 2 tasks always add).  wide: the
 // tasks were built for the wide form (BX_WIDE_*).  dsign != 0: D is in run
 // panels with only the sub-blocks of runs (a, b), a_start <= b_start, stored

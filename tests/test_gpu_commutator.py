@@ -356,6 +356,40 @@ def test_staged_pandya_reads(monkeypatch, n, r):
         assert np.array_equal(outs["1"][k], outs["0"][k])
 
 
+@pytest.mark.parametrize("wide", ["0", "1"])
+def test_direct_square_ladder_terms(monkeypatch, wide):
+    # QD_LDIRECT=1 (the default): the box pass forms the ladder terms of the
+    # HHHH, PHPH and PPPP sub-blocks from the products at (I, J) and (J, I);
+    # QD_LDIRECT=0: k_ladder_sq forms them into a buffer first.  Both against
+    # the oracle for every herm pair, with paired boxes (commutator_cc; the
+    # narrow form writes the paired outputs in a second phase) and every box
+    # on its own (commutator), and against each other to rounding
+    monkeypatch.setenv("QD_BOX_WIDE", wide)
+    outs = {}
+    for ld in ("1", "0"):
+        monkeypatch.setenv("QD_LDIRECT", ld)
+        c = Case(6, 6)
+        try:
+            res = []
+            for h1, h2 in [(-1, 1), (1, 1), (1, -1), (-1, -1)]:
+                x, y, X, Y = c.ops(37, h1, h2)
+                ref = c.o.commutator(x, h1, y, h2, pandya=True)
+                HCC, ECC = c.ctx.cc(float(h2)), c.ctx.cc(float(h1))
+                for paired in (True, False):
+                    Z = c.ctx.operator(-h1 * h2)
+                    if paired:
+                        qd.commutator_cc(X, Y, Z, HCC, ECC)
+                    else:
+                        qd.commutator(X, Y, Z, HCC, ECC)
+                    assert rel(Z.download(), ref) <= TOL, (h1, h2, paired)
+                    res.append(Z.download())
+            outs[ld] = res
+        finally:
+            c.ctx.close()
+    for a, b in zip(outs["1"], outs["0"]):
+        assert rel(a, b) <= 1e-14
+
+
 def test_without_gather_tables(monkeypatch):
     # QD_GATHER_TABLES=0: calc_cc and the 221 traces resolve the pair tables
     # per element, the path of operators whose flat index needs more than
