@@ -74,7 +74,10 @@ const char *qd_last_error(void);
  * QD_DSTAGE=0/1 whole boxes read their Pandya sub-blocks of D from HBM or
  * stage them in LDS (default: staged in the wide form), QD_ZSKIP=0 the BCH
  * terms run every GEMM product instead of skipping those that read a
- * sub-block type of G that is zero in every block (the same values). */
+ * sub-block type of G that is zero in every block (the same values),
+ * QD_LDIRECT=0 the square two-body sub-blocks' ladder terms go through a
+ * buffer filled by their own pass instead of being formed in the box pass
+ * (the same values to rounding). */
 int qd_context_create(int nbody, int nshell, qd_context **out);
 int qd_context_destroy(qd_context *ctx);
 int qd_context_info(const qd_context *ctx, qd_info *out);
