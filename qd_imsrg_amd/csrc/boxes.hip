@@ -214,9 +214,13 @@ __global__ void __launch_bounds__(NT) k_box2b(DevTab t, Ops o, double *__restric
   // sum for its grid elements e = tid + k NT, in order, in registers
   constexpr int KQ = (VC + NT - 1) / NT;
   const bool stage_d = STAGE && dopd && mode == 0 && dsign != 0.0 && (flags & TB_DSTAGE);
-  double pq[STAGE ? KQ : 1];
+  // One register queue rq serves both the staged Pandya sums (entry k: element
+  // tid + k NT, consumed from the front) and the values kept for the second
+  // write phase (appended at the back as the front is consumed)
+  constexpr int KV = STAGE || SPLIT ? KQ : 1;
+  double rq[KV];
 #pragma unroll
-  for (int k = 0; k < (STAGE ? KQ : 1); k++) pq[k] = 0.0;
+  for (int k = 0; k < KV; k++) rq[k] = 0.0;
   if (stage_d) {
     double *__restrict__ gd = (double *)gg;  // [2][total]
     for (int h = 0; h < 2; h++) {
@@ -240,7 +244,7 @@ __global__ void __launch_bounds__(NT) k_box2b(DevTab t, Ops o, double *__restric
       __syncthreads();
       int b = 0;
 #pragma unroll
-      for (int k = 0; k < (STAGE ? KQ : 1); k++) {
+      for (int k = 0; k < (STAGE ? KV : 1); k++) {
         const int e = threadIdx.x + k * NT;
         if (e < total) {
           if (b + 1 < ng && pre[b + 1] <= e) b = box_of(pre, b + 1, ng, e);
@@ -249,9 +253,9 @@ __global__ void __launch_bounds__(NT) k_box2b(DevTab t, Ops o, double *__restric
           const int i1 = qdiv(r, brc[b].y), i2 = r - i1 * s2, i3 = qdiv(c, rs4), i4 = c - i3 * s4;
           const double *__restrict__ g0 = gd + pre[b], *__restrict__ g1 = gd + total + pre[b];
           if (h == 0)
-            pq[k] = dterm(g0, b, 0, i3 * s1 + i1, i2 * s4 + i4) - dterm(g1, b, 1, i3 * s2 + i2, i1 * s4 + i4);
+            rq[k] = dterm(g0, b, 0, i3 * s1 + i1, i2 * s4 + i4) - dterm(g1, b, 1, i3 * s2 + i2, i1 * s4 + i4);
           else
-            pq[k] = (pq[k] + dterm(g0, b, 2, i4 * s2 + i2, i1 * s3 + i3)) - dterm(g1, b, 3, i4 * s1 + i1, i2 * s3 + i3);
+            rq[k] = (rq[k] + dterm(g0, b, 2, i4 * s2 + i2, i1 * s3 + i3)) - dterm(g1, b, 3, i4 * s1 + i1, i2 * s3 + i3);
         }
       }
       __syncthreads();
@@ -378,23 +382,17 @@ __global__ void __launch_bounds__(NT) k_box2b(DevTab t, Ops o, double *__restric
   // per wave); the paired outputs <J|Z|I> = h_Z (...) after all grid elements
   // are contracted, from the values kept in LDS, with consecutive threads on
   // consecutive columns, so that their stored rows are consecutive too.
-  // Each thread keeps its values in registers (vq, in order, like pq) until
+  // Each thread keeps its values in registers (rq, in order) until
   // the grids in LDS are no longer read.
   // (The narrow form only: in the wide form, one 16-wave task per CU, the
   // extra barriers cost more than the scattered writes: N=42 R=20 two-body
   // class 2.30 -> 2.39 ms, while N=20 R=16 gains 0.481 -> 0.461 ms.)
   const bool split = SPLIT && mode == 0 && anyp_s;
-  constexpr int KV = SPLIT ? KQ : 1;
-  double vq[KV];
-#pragma unroll
-  for (int k = 0; k < KV; k++) vq[k] = 0.0;
   for (int e = threadIdx.x, b = 0; e < total; e += NT) {
     // this element's staged Pandya sum; the queues move up by one
-    const double pdv = pq[0];
+    const double pdv = rq[0];
 #pragma unroll
-    for (int k = 0; k + 1 < (STAGE ? KQ : 1); k++) pq[k] = pq[k + 1];
-#pragma unroll
-    for (int k = 0; k + 1 < KV; k++) vq[k] = vq[k + 1];
+    for (int k = 0; k + 1 < KV; k++) rq[k] = rq[k + 1];
     if (b + 1 < ng && pre[b + 1] <= e) b = box_of(pre, b + 1, ng, e);
     const int4 B = bx[b];
     const int le = e - pre[b], c = qdiv(le, brc[b].x), r = le - c * B.y;
@@ -505,7 +503,7 @@ __global__ void __launch_bounds__(NT) k_box2b(DevTab t, Ops o, double *__restric
     }
     // the second phase adds this value times h_Z; a direct square term is
     // folded in (h_Z = +-1)
-    vq[KV - 1] = LDIR && lad && sqd ? val + hz * xa2 : val;
+    rq[KV - 1] = LDIR && lad && sqd ? val + hz * xa2 : val;
     if (!first) {  // ket slabs run before the main pass: the first writer of their outputs
       out[idx1] = o1 + val;
       continue;
@@ -516,7 +514,7 @@ __global__ void __launch_bounds__(NT) k_box2b(DevTab t, Ops o, double *__restric
   if constexpr (!SPLIT) return;
   if (!split) return;
   // second phase: the values of this thread's grid elements (iterations j of
-  // the loop above, at vq[KQ - n + j]) into LDS at their element positions,
+  // the loop above, at rq[KV - n + j]) into LDS at their element positions,
   // then the paired outputs with consecutive threads on consecutive columns
   __syncthreads();  // every contraction has read the grids
   {
@@ -525,7 +523,7 @@ __global__ void __launch_bounds__(NT) k_box2b(DevTab t, Ops o, double *__restric
 #pragma unroll
     for (int k = 0; k < KV; k++) {
       const int j = k - (KV - n);
-      if (j >= 0) gv[threadIdx.x + j * NT] = vq[k];
+      if (j >= 0) gv[threadIdx.x + j * NT] = rq[k];
     }
   }
   __syncthreads();
