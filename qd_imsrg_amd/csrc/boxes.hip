@@ -504,8 +504,9 @@ __global__ void __launch_bounds__(NT) k_box2b(DevTab t, Ops o, double *__restric
     // the second phase adds this value times h_Z; a direct square term is
     // folded in (h_Z = +-1)
     rq[KV - 1] = LDIR && lad && sqd ? val + hz * xa2 : val;
-    if (!first) {  // ket slabs run before the main pass: the first writer of their outputs
-      out[idx1] = o1 + val;
+    if (!first) {  // ket slabs run before the main pass: the first writers of their outputs
+      if (st1) out[idx1] = o1 + val;
+      if (st2) out[idx2] = o2 + hz * val;
       continue;
     }
     if (st1) out[idx1] = finish(sub1, o1 + val, xa1, xb1, xc1);

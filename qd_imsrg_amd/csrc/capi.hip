@@ -112,8 +112,8 @@ struct qd_context {
   double *u1 = nullptr, *u2 = nullptr;  // cross-coupled mat-vecs of commutator_121 (M*M each)
   double *partials = nullptr, *scal = nullptr;  // reduction partials, 8 device scalars
   double *host_scal = nullptr;                  // pinned mirror of scal
-  BTask *bt1 = nullptr, *bt2 = nullptr, *bt1p = nullptr;              // class-pair box passes
-  int nbt1 = 0, nbt2 = 0, nbt1p = 0;
+  BTask *bt1 = nullptr, *bt2 = nullptr, *bt1p = nullptr, *bt2p = nullptr;  // class-pair box passes
+  int nbt1 = 0, nbt2 = 0, nbt1p = 0, nbt2p = 0;
   int *blist = nullptr;                                              // class pairs of the box tasks
   int box_cap = BX_VCAP;                                             // grid elements per box task
   int box_rcap = BX_RCAP;                                            // row grid entries per box task
@@ -567,8 +567,11 @@ int build_worklists(qd_context *c) {
   // LDS grid both ways, the box (rows a, columns b) also writes the outputs of
   // the box (rows b, columns a): <J|Z|I> = h_Z <I|Z|J> for the 1b x 2b and
   // Pandya parts, so one staged grid and one contraction serve both.  Its
-  // blist entry carries BOX_PAIRED; the box (b, a) is left out.
-  std::vector<std::vector<BTask>> bt1(ch.nblock), bt2(ch.nblock), bt1p(ch.nblock);
+  // blist entry carries BOX_PAIRED; the box (b, a) is left out.  Where both
+  // boxes are too large for one grid, the slabs of (a, b) write both
+  // orientations the same way (column slabs in bt1p, row slabs in bt2p, the
+  // row slabs' ket terms being the first writers of either orientation).
+  std::vector<std::vector<BTask>> bt1(ch.nblock), bt2(ch.nblock), bt1p(ch.nblock), bt2p(ch.nblock);
   std::vector<int> blist;
   {
     const int cap = c->box_cap;
@@ -580,6 +583,32 @@ int build_worklists(qd_context *c) {
     auto pairable = [&](int x, int y) {  // x < y
       return whole_box(x, y) && whole_box(y, x) &&
              (ch.cp_tmax[x] >= ch.cp_tmin[y] || ch.cp_tmax[y] >= ch.cp_tmin[x]);
+    };
+    auto slab_pairable = [&](int x, int y) {  // x < y, both boxes split into slabs
+      return !whole_box(x, y) && !whole_box(y, x) &&
+             (ch.cp_tmax[x] >= ch.cp_tmin[y] || ch.cp_tmax[y] >= ch.cp_tmin[x]);
+    };
+    // column slabs (mode 1, bra terms) and row slabs (mode 2, ket terms) of
+    // the box gr x gc into c1 and c2, with blist entry e
+    auto slabs = [&](int q, int gr, int gc, int e, std::vector<BTask> *c1a, std::vector<BTask> *c1b,
+                     std::vector<BTask> *c2a, std::vector<BTask> *c2b) {
+      const int nr = csize(ch.cp_a[gr]) * csize(ch.cp_b[gr]), nc = csize(ch.cp_a[gc]) * csize(ch.cp_b[gc]);
+      const int ldc = nc | 1;
+      int w = cap / nr;                   // column slab: nr * (w | 1) <= cap
+      if ((w | 1) > cap / nr) w--;
+      for (int lo = 0; lo < nc; lo += w) {
+        const BTask t{q, gc, (int)blist.size(), 1, 1, lo, std::min(w, nc - lo)};
+        if (c1a) c1a->push_back(t);
+        if (c1b) c1b->push_back(t);
+        blist.push_back(e);
+      }
+      const int h = cap / ldc;            // row slab: h * ldc <= cap
+      for (int lo = 0; lo < nr; lo += h) {
+        const BTask t{q, gc, (int)blist.size(), 1, 2, lo, std::min(h, nr - lo)};
+        if (c2a) c2a->push_back(t);
+        if (c2b) c2b->push_back(t);
+        blist.push_back(e);
+      }
     };
     for (int q = 0; q < ch.nblock; q++)
       for (int gc = ch.cp_start[q]; gc < ch.cp_start[q + 1]; gc++) {
@@ -596,22 +625,13 @@ int build_worklists(qd_context *c) {
             return fail(QD_E_LAYOUT, "single-particle channel of %d states too large for the two-body boxes",
                         std::max(std::max(s1, s2), std::max(s3, s4)));
           const bool pair_here = gr < gc && pairable(gr, gc), pair_there = gr > gc && pairable(gc, gr);
+          const bool spl_here = gr < gc && slab_pairable(gr, gc), spl_there = gr > gc && slab_pairable(gc, gr);
           if (pair_here) wholep.push_back({nr * ldc, gr | BOX_PAIRED});
+          if (spl_here) slabs(q, gr, gc, gr | BOX_PAIRED, nullptr, &bt1p[q], nullptr, &bt2p[q]);
           if (ch.cp_tmax[gr] < ch.cp_tmin[gc]) continue;
           if (nr * ldc > cap) {
-            int w = cap / nr;                   // column slab: nr * (w | 1) <= cap
-            if ((w | 1) > cap / nr) w--;
-            for (int lo = 0; lo < nc; lo += w) {
-              const BTask t{q, gc, (int)blist.size(), 1, 1, lo, std::min(w, nc - lo)};
-              bt1[q].push_back(t);
-              bt1p[q].push_back(t);
-              blist.push_back(gr);
-            }
-            const int h = cap / ldc;            // row slab: h * ldc <= cap
-            for (int lo = 0; lo < nr; lo += h) {
-              bt2[q].push_back(BTask{q, gc, (int)blist.size(), 1, 2, lo, std::min(h, nr - lo)});
-              blist.push_back(gr);
-            }
+            const bool pl = spl_here || spl_there;  // the paired lists have these outputs already
+            slabs(q, gr, gc, gr, &bt1[q], pl ? nullptr : &bt1p[q], &bt2[q], pl ? nullptr : &bt2p[q]);
             continue;
           }
           whole.push_back({nr * ldc, gr});
@@ -654,8 +674,8 @@ int build_worklists(qd_context *c) {
     for (int q = 0; q < ch.nblock; q++) order[q] = q;
     std::sort(order.begin(), order.end(), [&](int a, int b) { return work[a] > work[b]; });
     std::vector<double> load(NXCD, 0.0);
-    for (int pass = 0; pass < 3; pass++) {
-      auto &src = pass == 0 ? bt1 : pass == 1 ? bt2 : bt1p;
+    for (int pass = 0; pass < 4; pass++) {
+      auto &src = pass == 0 ? bt1 : pass == 1 ? bt2 : pass == 2 ? bt1p : bt2p;
       std::vector<std::vector<BTask>> bx(NXCD);
       std::fill(load.begin(), load.end(), 0.0);
       for (int q : order) {
@@ -670,8 +690,9 @@ int build_worklists(qd_context *c) {
       for (int x = 0; x < NXCD; x++)
         for (size_t k = 0; k < bx[x].size(); k++) bt[k * NXCD + x] = bx[x][k];
       if (bt.empty()) continue;
-      if ((rc = dev_upload(c, bt, pass == 0 ? &c->bt1 : pass == 1 ? &c->bt2 : &c->bt1p))) return rc;
-      (pass == 0 ? c->nbt1 : pass == 1 ? c->nbt2 : c->nbt1p) = (int)bt.size();
+      if ((rc = dev_upload(c, bt, pass == 0 ? &c->bt1 : pass == 1 ? &c->bt2 : pass == 2 ? &c->bt1p : &c->bt2p)))
+        return rc;
+      (pass == 0 ? c->nbt1 : pass == 1 ? c->nbt2 : pass == 2 ? c->nbt1p : c->nbt2p) = (int)bt.size();
     }
     if (blist.empty()) blist.push_back(0);
     if ((rc = dev_upload(c, blist, &c->blist))) return rc;
@@ -1092,14 +1113,14 @@ int run_twobody(qd_context *c, const qd_op *r1, const qd_op *r2, qd_op *out, int
   Timer tm(c, TC_TWOBODY);
   const bool lsq_done = (flags & TB_LSQ_DONE) != 0;
   flags &= ~TB_LSQ_DONE;
+  paired = paired && c->box_pairs;  // the same in both parts
   if ((part & TB_PART_PRE) && (flags & TB_122)) {
     launch_fclass(c->stream, c->dt, o, c->fcls);
-    launch_box2b(c->stream, c->dt, o, out->d, c->bt2, c->nbt2, c->blist, flags, c->w1, c->w2, c->w3, c->wl,
-                 c->fcls, nullptr, 0.0, c->box_wide);
+    launch_box2b(c->stream, c->dt, o, out->d, paired ? c->bt2p : c->bt2, paired ? c->nbt2p : c->nbt2, c->blist, flags,
+                 c->w1, c->w2, c->w3, c->wl, c->fcls, nullptr, 0.0, c->box_wide);
   }
   if (part & TB_PART_MAIN) {
     if ((flags & TB_LADDER) && !lsq_done) run_ladder_sq(c, r1, r2);
-    paired = paired && c->box_pairs;
     if (c->ldirect)
       flags |= TB_LDIRECT | (int)((c->nz_on ? c->nz_mask : (uint32_t)ZB_ALL) << TB_NZ_SHIFT);
     launch_box2b(c->stream, c->dt, o, out->d, paired ? c->bt1p : c->bt1, paired ? c->nbt1p : c->nbt1, c->blist,
@@ -1194,7 +1215,7 @@ int full_commutator(qd_context *c, const qd_op *r1, const qd_op *r2, qd_op *r3, 
   if (side) {
     HIPC(hipEventRecord(c->ev_cc, c->aux));
     if (!rc) rc = run_acc(c, r1, r2, r3);
-    if (!rc) rc = run_twobody(c, r1, r2, r3, tbf, hcc, ecc, TB_PART_PRE);
+    if (!rc) rc = run_twobody(c, r1, r2, r3, tbf, hcc, ecc, TB_PART_PRE, paired);
     HIPC(hipEventRecord(c->ev_acc, c->aux));
     if (!rc) early_1b();
     on(main);

@@ -157,12 +157,12 @@ void launch_fclass(hipStream_t s, const DevTab &t, const Ops &o, double *F);
 // combinations of the products w1..w3 and, for the summed products of Vpphh
 // and Vppph and (without TB_LDIRECT) the square sub-blocks, of L
 // (TB_LADDER; launch_sum_halves and launch_ladder_sq have filled L), Pandya term from D
-// (TB_PANDYA); TB_ACCUM adds to out (mode 2 tasks always add).  wide: the
+// (TB_PANDYA); TB_ACCUM adds to out (mode 1 tasks add to what the mode 2 tasks wrote).  wide: the
 // tasks were built for the wide form (BX_WIDE_*).  dsign != 0: D is in run
 // panels with only the sub-blocks of runs (a, b), a_start <= b_start, stored
 // (launch_pandya_fused), a lower element (x, y) is read as dsign * D(y, x);
 // dsign = 0: D is whole and column-major
-// Paired boxes (BOX_PAIRED in blist, the bt1p task list) need operators that
+// Paired boxes (BOX_PAIRED in blist, the bt1p and bt2p task lists) need operators that
 // obey their herm flags (square sub-blocks and fhh, fpp (anti)symmetric) and,
 // with the Pandya term, D built from calc_cc of those operators.
 void launch_box2b(hipStream_t s, const DevTab &t, const Ops &o, double *out, const BTask *tasks, int ntasks,
