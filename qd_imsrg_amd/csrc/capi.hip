@@ -1981,15 +1981,14 @@ int qd_magnus_flow(qd_context *c, const qd_op *h0, qd_op *hs, qd_op *omega, cons
                    qd_flow_result *res, double *trace, int trace_len) {
   if (!c || !h0 || !hs || !omega || !p || !res) return fail(QD_E_ARG, "null argument");
   int rc;
-  qd_op *eta, *eta0, *g0, *hs0, *dg;
-  if ((rc = qd_op_create(c, -1, &eta))) return rc;
-  qd_op_create(c, -1, &eta0);
-  qd_op_create(c, -1, &g0);
-  qd_op_create(c, 1, &hs0);
-  qd_op_create(c, -1, &dg);
+  qd_op *eta = nullptr, *eta0 = nullptr, *g0 = nullptr, *hs0 = nullptr;
   qd_cc *hcc = nullptr, *occ = nullptr;
-  qd_cc_create(c, 1.0, &hcc);
-  qd_cc_create(c, -1.0, &occ);
+  if ((rc = qd_op_create(c, -1, &eta)) || (rc = qd_op_create(c, -1, &eta0)) || (rc = qd_op_create(c, -1, &g0)) ||
+      (rc = qd_op_create(c, 1, &hs0)) || (rc = qd_cc_create(c, 1.0, &hcc)) || (rc = qd_cc_create(c, -1.0, &occ))) {
+    qd_op_destroy(eta); qd_op_destroy(eta0); qd_op_destroy(g0); qd_op_destroy(hs0);
+    qd_cc_destroy(hcc); qd_cc_destroy(occ);
+    return rc;
+  }
   if (p->pandya) { hcc->zero = false; occ->zero = false; }  // request calc_cc-filled inputs
   omega->herm = -1;
   memset(res, 0, sizeof *res);
@@ -1997,7 +1996,7 @@ int qd_magnus_flow(qd_context *c, const qd_op *h0, qd_op *hs, qd_op *omega, cons
   int iteration = 0, rejected = 0, nt = 0, rows = 0;
   int64_t terms = 0;
   auto done = [&](int r) {
-    qd_op_destroy(eta); qd_op_destroy(eta0); qd_op_destroy(g0); qd_op_destroy(hs0); qd_op_destroy(dg);
+    qd_op_destroy(eta); qd_op_destroy(eta0); qd_op_destroy(g0); qd_op_destroy(hs0);
     qd_cc_destroy(hcc); qd_cc_destroy(occ);
     return r;
   };
@@ -2150,12 +2149,13 @@ int qd_magnus_flow(qd_context *c, const qd_op *h0, qd_op *hs, qd_op *omega, cons
     }
     count = 0;
     if ((rc = qd_copy_arrays(c, omega, g0))) return done2(rc);
-    if ((rc = qd_magnus_expand(c, dg, omega, eta))) return done2(rc);
-    if ((rc = qd_euler_step(c, omega, dg, &s, stp))) return done2(rc);
+    // MAGNUS_EXPAND is first order (dOmega = eta, src/magnus_IMSRG.f90:345),
+    // so the Euler step takes eta itself
+    if ((rc = qd_euler_step(c, omega, eta, &s, stp))) return done2(rc);
     if ((rc = qd_copy_arrays(c, hs, hs0))) return done2(rc);
     if ((rc = qd_bch_expand(c, hs, omega, h0, hcc, occ, &nt))) return done2(rc);
     terms += nt;
-    if ((rc = qd_copy_arrays(c, eta, eta0))) return done2(rc);
+    std::swap(eta, eta0);  // eta0 keeps this step's generator for a rejection
     if ((rc = qd_build_white(c, eta, hs))) return done2(rc);
     if ((rc = qd_mat_2_norm(c, eta, &nrm2))) return done2(rc);
     if ((rc = read_e0())) return done2(rc);
@@ -2163,9 +2163,8 @@ int qd_magnus_flow(qd_context *c, const qd_op *h0, qd_op *hs, qd_op *omega, cons
     if (nrm1 * 1.5 < nrm2) {
       if (p->verbose) fprintf(p->verbose == 2 ? stderr : stdout, " faillll\n");
       s -= stp;
-      if ((rc = qd_copy_arrays(c, g0, omega)) || (rc = qd_copy_arrays(c, hs0, hs)) ||
-          (rc = qd_copy_arrays(c, eta0, eta)))
-        return done2(rc);
+      if ((rc = qd_copy_arrays(c, g0, omega)) || (rc = qd_copy_arrays(c, hs0, hs))) return done2(rc);
+      std::swap(eta, eta0);
       stp = stp / 2.0 + R4_1EM3;
       rejected++;
       continue;
