@@ -47,6 +47,11 @@ __device__ __forceinline__ int box_of(const int *pre, int lo, int ng, int e) {
   return lo;
 }
 
+// stored sub-block (enum Sub) of (row pair type, column pair type), row >= column
+__device__ __forceinline__ int sub_type(int ta, int tb) {
+  return tb == 0 ? (ta == 0 ? HHHH : (ta == 1 ? PHHH : PPHH)) : (tb == 1 ? (ta == 1 ? PHPH : PPPH) : PPPP);
+}
+
 // floor(a / d) for 0 <= a < 2^22 and 1 <= d < 2^12 from rd = 1.0f / d: the
 // product's relative error (< 2^-23) times (a + 0.5) / d stays below the
 // distance 0.5 / d of (a + 0.5) / d from the nearest integer
@@ -82,6 +87,8 @@ __global__ void __launch_bounds__(NT) k_box2b(DevTab t, Ops o, double *__restric
   __shared__ int anyp_s;                       // some box of the task is paired
   const BTask tk = tasks[blockIdx.x];
   if (tk.block < 0) return;
+  // ZB bits of r1's nonzero sub-block types (all set outside the BCH terms)
+  const uint32_t nz1 = (uint32_t)flags >> TB_NZ_SHIFT;
   const int q = tk.block, mode = tk.mode, ng = tk.ng;
   const int nh = t.nh[q], nb = t.nb[q], np = t.np[q];
   const int tstart[3] = {0, nh, nh + nb}, tcount[3] = {nh, nb, np};
@@ -319,9 +326,11 @@ __global__ void __launch_bounds__(NT) k_box2b(DevTab t, Ops o, double *__restric
         ok[u] = (er >> 17) & (ec >> 17) & 1u;
         const int64_t a = ok[u] ? soff[tA][tB] + (tr ? Jl : Il) + (int64_t)(tr ? Il : Jl) * tcount[tA] : 0;
         const bool neg = ((er ^ ec) >> 16) & 1u;
-        x1[u] = o.v1[a];
+        // r1's sub-block types that are zero in every block are not read
+        const bool z1 = !((nz1 >> sub_type(tA, tB)) & 1u);
+        x1[u] = o.v1[z1 ? 0 : a];
         x2[u] = o.v2[a];
-        g1[u] = (neg ? -1.0 : 1.0) * (tr ? (double)o.h1 : 1.0);
+        g1[u] = z1 ? 0.0 : (neg ? -1.0 : 1.0) * (tr ? (double)o.h1 : 1.0);
         g2[u] = (neg ? -1.0 : 1.0) * (tr ? (double)o.h2 : 1.0);
         dst[u] = B.z + r * ldc + c;
       }
@@ -335,9 +344,7 @@ __global__ void __launch_bounds__(NT) k_box2b(DevTab t, Ops o, double *__restric
   const double h1 = o.h1, h2 = o.h2, hz = -h1 * h2;  // hz: herm of [r1, r2]
   const bool same34 = A3 == A4;
   // stored sub-block of (row type, column type), row type >= column type
-  auto sub_of = [](int ta, int tb) {
-    return tb == 0 ? (ta == 0 ? HHHH : (ta == 1 ? PHHH : PPHH)) : (tb == 1 ? (ta == 1 ? PHPH : PPPH) : PPPP);
-  };
+  auto sub_of = [](int ta, int tb) { return sub_type(ta, tb); };
   // ladder inputs (signed combinations of the products of matrix_mult222,
   // src/commutators.f90:999-1127, and the summed products for Vpphh,
   // Vphhh, Vppph): xa = w1(I,J), xb = the summed product at (I,J) (L for
@@ -352,7 +359,6 @@ __global__ void __launch_bounds__(NT) k_box2b(DevTab t, Ops o, double *__restric
   // as k_ladder_sq does; both orientations come from the same four values.
   // A product of a sub-block type that is zero in r1 (ZB bits in flags >>
   // TB_NZ_SHIFT) is not read.
-  const uint32_t nz1 = (uint32_t)flags >> TB_NZ_SHIFT;
   auto ld = [&](int sub, int64_t idx, double &xa, double &xb, double &xc) {
     const bool sq = sub == HHHH || sub == PPPP || sub == PHPH;
     const bool usec = sub == PHHH || sub == PPPH;

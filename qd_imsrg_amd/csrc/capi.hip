@@ -1114,6 +1114,8 @@ int run_twobody(qd_context *c, const qd_op *r1, const qd_op *r2, qd_op *out, int
   const bool lsq_done = (flags & TB_LSQ_DONE) != 0;
   flags &= ~TB_LSQ_DONE;
   paired = paired && c->box_pairs;  // the same in both parts
+  // r1's sub-block types that are nonzero (the BCH terms' Omega: fph, Vpphh)
+  flags |= (int)((c->nz_on ? c->nz_mask : (uint32_t)ZB_ALL) << TB_NZ_SHIFT);
   if ((part & TB_PART_PRE) && (flags & TB_122)) {
     launch_fclass(c->stream, c->dt, o, c->fcls);
     launch_box2b(c->stream, c->dt, o, out->d, paired ? c->bt2p : c->bt2, paired ? c->nbt2p : c->nbt2, c->blist, flags,
@@ -1121,8 +1123,7 @@ int run_twobody(qd_context *c, const qd_op *r1, const qd_op *r2, qd_op *out, int
   }
   if (part & TB_PART_MAIN) {
     if ((flags & TB_LADDER) && !lsq_done) run_ladder_sq(c, r1, r2);
-    if (c->ldirect)
-      flags |= TB_LDIRECT | (int)((c->nz_on ? c->nz_mask : (uint32_t)ZB_ALL) << TB_NZ_SHIFT);
+    if (c->ldirect) flags |= TB_LDIRECT;
     launch_box2b(c->stream, c->dt, o, out->d, paired ? c->bt1p : c->bt1, paired ? c->nbt1p : c->nbt1, c->blist,
                  flags | (c->d_stage ? TB_DSTAGE : 0), c->w1, c->w2, c->w3, c->wl, c->fcls,
                  (flags & TB_PANDYA) ? c->P : nullptr, c->d_sign, c->box_wide);

@@ -99,9 +99,9 @@ __device__ __forceinline__ double felem(const DevTab &t, const double *__restric
 // flags of the fused two-body kernel
 // TB_DSTAGE: whole boxes stage the Pandya sub-blocks of D (run panels) in LDS
 // TB_LDIRECT: the square sub-blocks' ladder terms are formed from the products
-// w1 and w3 (w2 for PHPH) at (I, J) and (J, I) instead of read from L; the
-// ZB bits of r1's nonzero sub-block types go in flags >> TB_NZ_SHIFT (a
-// product of a zero type is not read)
+// w1 and w3 (w2 for PHPH) at (I, J) and (J, I) instead of read from L.  The
+// ZB bits of r1's nonzero sub-block types go in flags >> TB_NZ_SHIFT: r1's
+// elements and products of a zero type are not read
 enum { TB_122 = 1, TB_LADDER = 2, TB_PANDYA = 4, TB_ACCUM = 8, TB_DSTAGE = 32, TB_LDIRECT = 64 };
 constexpr int TB_NZ_SHIFT = 16;
 // flags of the one-body kernel; OB_MV: only the 121 mat-vecs into u1, u2
