@@ -2033,6 +2033,33 @@ int qd_magnus_flow(qd_context *c, const qd_op *h0, qd_op *hs, qd_op *omega, cons
     crit = rule_deds ? std::fabs(deds) : std::fabs(m2);
     return QD_OK;
   };
+  // the end of a step in one read-back: ||eta|| (rejection test), E0, and
+  // both stopping quantities of the new H(s); crit itself is updated by
+  // set_crit once the step is accepted
+  double m2n = 0.0, dedsn = 0.0;
+  auto step_scalars = [&]() -> int {
+    Timer tm(c, TC_OTHER);
+    launch_norm2(c->stream, eta->d, c->ch.neq, c->partials, c->scal);
+    launch_mbpt2(c->stream, c->dt, hs->d, c->pphh_segs, c->pphh_tiles, c->npphh_tiles, c->partials, c->scal + 1);
+    Ops zo{eta->d, hs->d, eta->herm, hs->herm};
+    launch_zerobody(c->stream, c->dt, zo, c->pphh_segs, c->pphh_tiles, c->npphh_tiles, c->partials, c->scal + 2, 3);
+    HIPC(hipGetLastError());
+    HIPC(hipMemcpyAsync(c->host_scal, c->scal, 3 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    HIPC(hipMemcpyAsync(c->host_scal + 3, hs->d, sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    HIPC(hipStreamSynchronize(c->stream));
+    nrm2 = std::sqrt(c->host_scal[0]);
+    m2n = c->host_scal[1];
+    dedsn = c->host_scal[2];
+    e0 = c->host_scal[3];
+    if (!std::isfinite(e0) || !std::isfinite(nrm2))
+      return fail(QD_E_NUMERIC, "non-finite E0 or ||eta|| at s=%g", s);
+    return QD_OK;
+  };
+  auto set_crit = [&]() {
+    m2 = m2n;
+    deds = dedsn;
+    crit = rule_deds ? std::fabs(deds) : std::fabs(m2);
+  };
   if (p->stop_rule != 0 && p->stop_rule != 1) return done(fail(QD_E_ARG, "stop_rule must be 0 or 1"));
   if ((rc = eval_crit(hs))) return done(rc);
   // write_spec (src/magnus_IMSRG.f90:165-189) into spectrum_path
@@ -2100,14 +2127,6 @@ int qd_magnus_flow(qd_context *c, const qd_op *h0, qd_op *hs, qd_op *omega, cons
       r[0] = iteration; r[1] = s; r[2] = e0; r[3] = nrm1; r[4] = nrm2; r[5] = crit;
     }
   };
-  auto read_e0 = [&]() -> int {
-    HIPC(hipMemcpyAsync(c->host_scal, hs->d, sizeof(double), hipMemcpyDeviceToHost, c->stream));
-    HIPC(hipStreamSynchronize(c->stream));
-    e0 = c->host_scal[0];
-    if (!std::isfinite(e0) || !std::isfinite(nrm2))
-      return fail(QD_E_NUMERIC, "non-finite E0 or ||eta|| at s=%g", s);
-    return QD_OK;
-  };
   while (crit > conv && steps < p->max_iter) {
     steps++;
     if (m > 0 && crit <= DIIS_START) {
@@ -2138,13 +2157,12 @@ int qd_magnus_flow(qd_context *c, const qd_op *h0, qd_op *hs, qd_op *omega, cons
       if ((rc = qd_bch_expand(c, hs, omega, h0, hcc, occ, &nt))) return done2(rc);
       terms += nt;
       if ((rc = qd_build_white(c, eta, hs))) return done2(rc);
-      if ((rc = qd_mat_2_norm(c, eta, &nrm2))) return done2(rc);
-      if ((rc = read_e0())) return done2(rc);
+      if ((rc = step_scalars())) return done2(rc);
       log_step();
       if (count == 1 || nrm2 < best) best = nrm2;
       if (nrm2 > 2.0 * best) count = 0;  // drop the history
       if ((rc = write_spec(e0)) || (rc = write_spec(e0))) return done2(rc);
-      if ((rc = eval_crit(hs))) return done2(rc);
+      set_crit();
       iteration++;
       continue;
     }
@@ -2158,8 +2176,7 @@ int qd_magnus_flow(qd_context *c, const qd_op *h0, qd_op *hs, qd_op *omega, cons
     terms += nt;
     std::swap(eta, eta0);  // eta0 keeps this step's generator for a rejection
     if ((rc = qd_build_white(c, eta, hs))) return done2(rc);
-    if ((rc = qd_mat_2_norm(c, eta, &nrm2))) return done2(rc);
-    if ((rc = read_e0())) return done2(rc);
+    if ((rc = step_scalars())) return done2(rc);
     log_step();
     if (nrm1 * 1.5 < nrm2) {
       if (p->verbose) fprintf(p->verbose == 2 ? stderr : stdout, " faillll\n");
@@ -2171,7 +2188,7 @@ int qd_magnus_flow(qd_context *c, const qd_op *h0, qd_op *hs, qd_op *omega, cons
       continue;
     }
     if ((rc = write_spec(e0)) || (rc = write_spec(e0))) return done2(rc);
-    if ((rc = eval_crit(hs))) return done2(rc);
+    set_crit();
     iteration++;
   }
   free_hist();
