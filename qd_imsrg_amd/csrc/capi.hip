@@ -1234,7 +1234,9 @@ int full_commutator(qd_context *c, const qd_op *r1, const qd_op *r2, qd_op *r3, 
     HIPC(hipGetLastError());
     c->d_sign = -hcc->herm * ecc->herm;
   } else {
-    if (side) HIPC(hipStreamWaitEvent(main, c->ev_cc, 0));
+    // the products read the CC sets only with the Pandya term; wait for the
+    // fills only when there are any
+    if (side && (fh || fe)) HIPC(hipStreamWaitEvent(main, c->ev_cc, 0));
     if ((rc = run_products(c, r1, r2, r3, hcc, ecc))) return rc;
     if (side) run_ladder_sq(c, r1, r2);
   }
