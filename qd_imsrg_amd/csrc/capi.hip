@@ -1179,8 +1179,27 @@ int full_commutator(qd_context *c, const qd_op *r1, const qd_op *r2, qd_op *r3, 
   const bool side = !c->timing;
   hipStream_t main = c->stream;
   auto on = [&](hipStream_t st) { c->stream = st; };
+  if (fh) fh->zero = false;  // host flags; the fills are launched below
+  if (fe) fe->zero = false;
+  const bool pd = pandya_on(hcc, ecc);
+  const bool fused = pd && c->launch_mode == 0 && pandya_fused_prec(c->gemm_prec);
+  const int tbf = TB_122 | TB_LADDER | (pd ? TB_PANDYA : 0);
+  // paired boxes: without the Pandya term, or with CC sets that are calc_cc
+  // of r2 and r1 under the operators' own herm flags
+  const bool paired = !pd || ((fill & FILL_H) && ((fill & FILL_E) || e_from_r1) && hcc->herm == r2->herm &&
+                              ecc->herm == r1->herm);
+  // Per-channel launches (the plain form): the products go first when they
+  // need no CC fills, so that the many small GEMM launches of the main
+  // stream start before the side stream's (N=20 R=16 flow 1.28 -> 1.00 s).
+  // The grouped form keeps the side stream's work first (issued the other
+  // way its flow went 0.121 -> 0.127 s).
+  const bool first_products = side && c->launch_mode != 0 && !(fh || fe);  // (never fused)
+  if (side) HIPC(hipEventRecord(c->ev_fork, main));
+  if (first_products) {
+    if ((rc = run_products(c, r1, r2, r3, hcc, ecc))) return rc;
+    run_ladder_sq(c, r1, r2);
+  }
   if (side) {
-    HIPC(hipEventRecord(c->ev_fork, main));
     HIPC(hipStreamWaitEvent(c->aux, c->ev_fork, 0));
     on(c->aux);
   }
@@ -1190,15 +1209,6 @@ int full_commutator(qd_context *c, const qd_op *r1, const qd_op *r2, qd_op *r3, 
     if (fe) launch_calc_cc(c->stream, c->dt, r1->d, r1->herm, fe->d, c->cc_segs, c->cc_tiles, c->ncc_tiles);
   }
   if (hipGetLastError() != hipSuccess) rc = fail(QD_E_HIP, "calc_cc launch failed");
-  if (fh) fh->zero = false;
-  if (fe) fe->zero = false;
-  const bool pd = pandya_on(hcc, ecc);
-  const bool fused = pd && c->launch_mode == 0 && pandya_fused_prec(c->gemm_prec);
-  const int tbf = TB_122 | TB_LADDER | (pd ? TB_PANDYA : 0);
-  // paired boxes: without the Pandya term, or with CC sets that are calc_cc
-  // of r2 and r1 under the operators' own herm flags
-  const bool paired = !pd || ((fill & FILL_H) && ((fill & FILL_E) || e_from_r1) && hcc->herm == r2->herm &&
-                              ecc->herm == r1->herm);
   // the 121 mat-vecs and the zero-body traces read only the operands and CC
   // sets: on the side stream right after the work the two-body pass waits
   // for, beside the Pandya product (N=20 R=14 step 0.484 -> 0.479 ms, N=20
@@ -1222,18 +1232,19 @@ int full_commutator(qd_context *c, const qd_op *r1, const qd_op *r2, qd_op *r3, 
     on(main);
   }
   if (rc) return rc;
-  // products phase
+  // products phase (the rest of it)
   if (fused) {
-    Slots s = make_slots(c, r1, r2, r3, ecc, hcc);
     Timer tm(c, TC_PRODUCTS);
-    if ((rc = run_gemm_list(c, c->g_all, false, s))) return rc;  // ladder (and one-body) products
-    if (side) run_ladder_sq(c, r1, r2);  // while the CC fills finish on the side stream
+    if (!first_products) {
+      if ((rc = run_gemm_list(c, c->g_all, false, make_slots(c, r1, r2, r3, ecc, hcc)))) return rc;
+      if (side) run_ladder_sq(c, r1, r2);  // while the CC fills finish on the side stream
+    }
     if (side) HIPC(hipStreamWaitEvent(main, c->ev_cc, 0));
     launch_pandya_fused(c->stream, c->dt, ecc->d, hcc->d, c->P, c->ftp, c->nftp, hcc->herm, ecc->herm, c->gemm_prec,
                         CcSlices{c->cs, c->cs_len, c->cs_off, c->cs_units, c->ncs_units});
     HIPC(hipGetLastError());
     c->d_sign = -hcc->herm * ecc->herm;
-  } else {
+  } else if (!first_products) {
     // the products read the CC sets only with the Pandya term; wait for the
     // fills only when there are any
     if (side && (fh || fe)) HIPC(hipStreamWaitEvent(main, c->ev_cc, 0));
