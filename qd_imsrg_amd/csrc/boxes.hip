@@ -460,8 +460,10 @@ __global__ void __launch_bounds__(NT) k_box2b(DevTab t, Ops o, double *__restric
         if (!sqd) ld(sub1, idx1, xa1, xb1, xc1);
         if (!sqd && !split) ld(sub2, idx2, xa2, xb2, xc2);
       } else {
+        // a square sub-block's L is (anti)symmetric, L(J,I) = h_Z L(I,J):
+        // its paired output needs no second read
         ld(sub1, idx1, xa1, xb1, xc1);
-        if (!split) ld(sub2, idx2, xa2, xb2, xc2);
+        if (!split) ld(tyr == tyc ? HHHH : sub2, tyr == tyc ? 0 : idx2, xa2, xb2, xc2);
       }
     }
     const double o1 = rd ? out[idx1] : 0.0;
@@ -501,15 +503,20 @@ __global__ void __launch_bounds__(NT) k_box2b(DevTab t, Ops o, double *__restric
     if (stage_d) val += pdv;
     else if (dopd) val += d0 - d1 + d2 - d3;
     if (sr != sc) val = -val;
-    if (LDIR && lad && sqd) {
-      const double a_ij = za ? 0.0 : xa1, b_ij = zb ? 0.0 : xb1, a_ji = za ? 0.0 : xa2, b_ji = zb ? 0.0 : xb2;
-      const double cq = tyr == 1 ? h1 : h1 * h2, dq = tyr == 1 ? h2 : 1.0;
-      xa1 = (h1 * a_ji + cq * b_ji) + (-h2 * a_ij - dq * b_ij);
-      xa2 = (h1 * a_ij + cq * b_ij) + (-h2 * a_ji - dq * b_ji);
+    const bool sq = tyr == tyc;
+    if (lad && sq) {
+      if constexpr (LDIR) {
+        const double a_ij = za ? 0.0 : xa1, b_ij = zb ? 0.0 : xb1, a_ji = za ? 0.0 : xa2, b_ji = zb ? 0.0 : xb2;
+        const double cq = tyr == 1 ? h1 : h1 * h2, dq = tyr == 1 ? h2 : 1.0;
+        xa1 = (h1 * a_ji + cq * b_ji) + (-h2 * a_ij - dq * b_ij);
+        xa2 = (h1 * a_ij + cq * b_ij) + (-h2 * a_ji - dq * b_ji);
+      } else {
+        xa2 = hz * xa1;
+      }
     }
-    // the second phase adds this value times h_Z; a direct square term is
-    // folded in (h_Z = +-1)
-    rq[KV - 1] = LDIR && lad && sqd ? val + hz * xa2 : val;
+    // the second phase adds this value times h_Z; a square sub-block's
+    // ladder term is folded in (h_Z = +-1)
+    rq[KV - 1] = lad && sq ? val + hz * xa2 : val;
     if (!first) {  // ket slabs run before the main pass: the first writers of their outputs
       if (st1) out[idx1] = o1 + val;
       if (st2) out[idx2] = o2 + hz * val;
@@ -558,7 +565,7 @@ __global__ void __launch_bounds__(NT) k_box2b(DevTab t, Ops o, double *__restric
       sb[u] = ok[u] ? sub_of(tyc, tyr) : HHHH;
       ix[u] = ok[u] ? soff[tyc][tyr] + Jl + (int64_t)Il * tcount[tyc] : 0;
       xa[u] = xb[u] = xc[u] = 0.0;
-      const bool folded = LDIR && tyc == tyr;  // its ladder term is in the value
+      const bool folded = tyc == tyr;  // its ladder term is in the value
       if (lad) ld(folded ? HHHH : sb[u], folded ? 0 : ix[u], xa[u], xb[u], xc[u]);
       if (folded) xa[u] = 0.0;
       o2[u] = rd ? out[ix[u]] : 0.0;
