@@ -950,9 +950,6 @@ k_pandya_fused_bf16(DevTab t, const unsigned short *__restrict__ planes,
   double *__restrict__ Dq = D + t.cc_poff[q];
   const bool diag = pr.ti == pr.tj;
   const int i0 = pr.ti * T, j0 = pr.tj * T;
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  const int lr = lane & 15, lk = lane >> 4;
-  const int m0 = (wave & 1) * 32, n0 = (wave >> 1) * 32;
   d4 a1[2][2], a2[2][2];
 #pragma unroll
   for (int x = 0; x < 2; x++)
