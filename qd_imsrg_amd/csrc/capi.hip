@@ -670,6 +670,7 @@ int build_worklists(qd_context *c) {
       }
   }
   {
+    auto csz = [&](int h) { return ch.ch_start[h + 1] - ch.ch_start[h]; };
     std::vector<int> order(ch.nblock);
     for (int q = 0; q < ch.nblock; q++) order[q] = q;
     std::sort(order.begin(), order.end(), [&](int a, int b) { return work[a] > work[b]; });
@@ -678,11 +679,29 @@ int build_worklists(qd_context *c) {
       auto &src = pass == 0 ? bt1 : pass == 1 ? bt2 : pass == 2 ? bt1p : bt2p;
       std::vector<std::vector<BTask>> bx(NXCD);
       std::fill(load.begin(), load.end(), 0.0);
+      // within a block, the largest tasks (grid elements) first; the blocks
+      // stay contiguous (their operands shared in the XCD's L2)
+      auto grid_of = [&](const BTask &t) {
+        const int nc = csz(ch.cp_a[t.gcol]) * csz(ch.cp_b[t.gcol]);
+        if (t.mode != 0) {
+          const int gr = blist[t.first] & ~BOX_PAIRED;
+          return t.mode == 1 ? csz(ch.cp_a[gr]) * csz(ch.cp_b[gr]) * t.cnt : t.cnt * nc;
+        }
+        int e = 0;
+        for (int b = 0; b < t.ng; b++) {
+          const int gr = blist[t.first + b] & ~BOX_PAIRED;
+          e += csz(ch.cp_a[gr]) * csz(ch.cp_b[gr]) * nc;
+        }
+        return e;
+      };
       for (int q : order) {
         if (work[q] == 0) continue;
         int best = (int)(std::min_element(load.begin(), load.end()) - load.begin());
         load[best] += work[q];
+        const size_t at = bx[best].size();
         bx[best].insert(bx[best].end(), src[q].begin(), src[q].end());
+        std::stable_sort(bx[best].begin() + at, bx[best].end(),
+                         [&](const BTask &u, const BTask &v) { return grid_of(u) > grid_of(v); });
       }
       size_t lng = 0;
       for (auto &l : bx) lng = std::max(lng, l.size());
