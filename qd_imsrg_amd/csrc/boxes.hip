@@ -156,39 +156,15 @@ __global__ void __launch_bounds__(NT) k_box2b(DevTab t, Ops o, double *__restric
       anyp_s = anyp;
     }
   } else if (dopd && (int)threadIdx.x - 64 < ng) {
-    // wave 1, one lane per box: D(x3,x1|x2,x4), D(x3,x2|x1,x4),
-    // D(x4,x2|x1,x3), D(x4,x1|x2,x3): device rows of a CC block come in
-    // runs by the channel of the first state, so each term's box is the
-    // dense sub-block of two runs (rows: channel rc0 with partner rc1,
-    // columns: cc0 with partner cc1).  In run panels (dsign != 0) the
-    // sub-block is contiguous, the stored one being the transposed sub-block
-    // when the row run starts after the column run; whole column-major D
-    // (dsign == 0) has leading dimension r
+    // wave 1, one lane per box: the bases of its four Pandya terms,
+    // D(x3,x1|x2,x4), D(x3,x2|x1,x4), D(x4,x2|x1,x3), D(x4,x1|x2,x3), from
+    // the table built with the task lists (box_pand_terms in capi.hip)
     const int b = threadIdx.x - 64;
-    const int g = blist[tk.first + b] & ~BOX_PAIRED;
-    const int A1 = t.cp_a[g], A2 = t.cp_b[g];
-    const int rc0[4] = {A3, A3, A4, A4}, rc1[4] = {A1, A2, A2, A1}, cc0[4] = {A2, A1, A1, A2},
-              cc1[4] = {A4, A4, A3, A3};
-    auto csz = [&](int h) { return t.ch_start[h + 1] - t.ch_start[h]; };
+    const int64_t e = ((int64_t)(dsign == 0.0 ? 0 : 1) * t.nblist + tk.first + b) * 4;
+#pragma unroll
     for (int k = 0; k < 4; k++) {
-      const int x = t.ch_states[t.ch_start[rc0[k]]], y = t.ch_states[t.ch_start[rc1[k]]];
-      const int K = t.cc_map[x * t.M + y].x, r = t.cc_r[K];
-      const int Lr = t.cc_rs[K * t.nch + rc0[k]], Lc = t.cc_rs[K * t.nch + cc0[k]];
-      const int lr = csz(rc0[k]) * csz(rc1[k]), lc = csz(cc0[k]) * csz(cc1[k]);
-      const int64_t po = t.cc_poff[K];
-      if (t.cc_nb[K] == 0) {
-        pb[b][k] = -1;
-        pl[b][k] = 0;
-      } else if (dsign == 0.0) {
-        pb[b][k] = po + Lr + (int64_t)Lc * r;
-        pl[b][k] = r;
-      } else if (Lr <= Lc) {
-        pb[b][k] = po + (int64_t)Lr * r + (int64_t)Lc * lr;
-        pl[b][k] = lr;
-      } else {
-        pb[b][k] = po + (int64_t)Lc * r + (int64_t)Lr * lc;
-        pl[b][k] = lc | (1 << 30);
-      }
+      pb[b][k] = t.box_pb[e + k];
+      pl[b][k] = t.box_pl[e + k];
     }
   }
   __syncthreads();

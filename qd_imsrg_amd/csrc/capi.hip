@@ -539,6 +539,42 @@ void onebody_products(const Channels &ch, GemmBuilder &g) {
   }
 }
 
+// The Pandya terms of a box (rows: class pair A1 A2, columns: A3 A4), k =
+// D(x3,x1|x2,x4), D(x3,x2|x1,x4), D(x4,x2|x1,x3), D(x4,x1|x2,x3): device rows
+// of a CC block come in runs by the channel of the first state, so each
+// term's box is the dense sub-block of two runs (rows: channel rc0 with
+// partner rc1, columns: cc0 with partner cc1).  In run panels (layout 1) the
+// sub-block is contiguous, the stored one being the transposed sub-block when
+// the row run starts after the column run (flagged 1 << 30 in the leading
+// dimension); whole column-major D (layout 0) has leading dimension r.  pb =
+// -1: the term has no CC block.
+void box_pand_terms(const Channels &ch, int A1, int A2, int A3, int A4, int layout, int64_t *pb, int *pl) {
+  const int rc0[4] = {A3, A3, A4, A4}, rc1[4] = {A1, A2, A2, A1}, cc0[4] = {A2, A1, A1, A2},
+            cc1[4] = {A4, A4, A3, A3};
+  auto csz = [&](int h) { return ch.ch_start[h + 1] - ch.ch_start[h]; };
+  for (int k = 0; k < 4; k++) {
+    const int x = ch.ch_states[ch.ch_start[rc0[k]]], y = ch.ch_states[ch.ch_start[rc1[k]]];
+    const int K = ch.cc_map[2 * ((size_t)x * ch.M + y)];
+    pb[k] = -1;
+    pl[k] = 0;
+    if (K < 0 || ch.cc_nb[K] == 0) continue;
+    const int r = ch.cc_r[K];
+    const int Lr = ch.cc_rs[(size_t)K * ch.nch + rc0[k]], Lc = ch.cc_rs[(size_t)K * ch.nch + cc0[k]];
+    const int lr = csz(rc0[k]) * csz(rc1[k]), lc = csz(cc0[k]) * csz(cc1[k]);
+    const int64_t po = ch.cc_poff[K];
+    if (layout == 0) {
+      pb[k] = po + Lr + (int64_t)Lc * r;
+      pl[k] = r;
+    } else if (Lr <= Lc) {
+      pb[k] = po + (int64_t)Lr * r + (int64_t)Lc * lr;
+      pl[k] = lr;
+    } else {
+      pb[k] = po + (int64_t)Lc * r + (int64_t)Lr * lc;
+      pl[k] = lc | (1 << 30);
+    }
+  }
+}
+
 int build_worklists(qd_context *c) {
   const Channels &ch = c->ch;
   int rc;
@@ -668,6 +704,26 @@ int build_worklists(qd_context *c) {
           }
         }
       }
+  }
+  {  // the Pandya term bases of every box of the first-pass tasks (modes 0, 1)
+    const size_t nbl = blist.size();
+    std::vector<int64_t> pbv(2 * nbl * 4, -1);
+    std::vector<int> plv(2 * nbl * 4, 0);
+    for (auto *lists : {&bt1, &bt1p})
+      for (auto &l : *lists)
+        for (const BTask &t : l) {
+          if (t.mode == 2) continue;
+          for (int b = 0; b < t.ng; b++) {
+            const int g = blist[t.first + b] & ~BOX_PAIRED;
+            for (int layout = 0; layout < 2; layout++) {
+              const size_t e = ((size_t)layout * nbl + t.first + b) * 4;
+              box_pand_terms(ch, ch.cp_a[g], ch.cp_b[g], ch.cp_a[t.gcol], ch.cp_b[t.gcol], layout, &pbv[e], &plv[e]);
+            }
+          }
+        }
+    if ((rc = dev_upload(c, pbv, (int64_t **)&c->dt.box_pb)) || (rc = dev_upload(c, plv, (int **)&c->dt.box_pl)))
+      return rc;
+    c->dt.nblist = (int)nbl;
   }
   {
     auto csz = [&](int h) { return ch.ch_start[h + 1] - ch.ch_start[h]; };

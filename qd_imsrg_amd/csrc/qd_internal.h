@@ -125,6 +125,13 @@ struct DevTab {
   int64_t p_size;  // Pandya product entries; element p_size of D is a zero
   const int *cc_rowblk;  // per CC row (M * M, rows of the blocks in order): its block
   const uint32_t *cc_run;  // per CC row: first row of its run | run length << 16
+  // per entry of the box list (nblist of them), for the four Pandya terms of
+  // the box in its task: the offset in D of the term's sub-block (-1: none)
+  // and its leading dimension | stored transposed << 30; [layout][entry][4],
+  // layout 0 whole column-major D, 1 run panels
+  const int64_t *box_pb;
+  const int *box_pl;
+  int nblist;
 };
 constexpr uint32_t CC_ZERO = 0x3fffffffu;
 
