@@ -106,24 +106,24 @@ __global__ void __launch_bounds__(NT) k_box2b(DevTab t, Ops o, double *__restric
   }
   for (int k = threadIdx.x; k < nc; k += NT) cent[k] = cgrid[k];
   if (threadIdx.x < 64) {
-    // wave 0, one lane per box: the box's tables (three dependent loads for
-    // all boxes at once), then its offsets by exclusive prefix sums over the
-    // lanes (ng <= BX_GMAX <= 64)
+    // wave 0, one lane per box: the box's entries of the table built with
+    // the task lists (one load), then its offsets by exclusive prefix sums
+    // over the lanes (ng <= BX_GMAX <= 64)
     static_assert(BX_GMAX <= 64, "one lane of wave 0 per box");
     const int b = threadIdx.x;
-    int g = 0, A1 = 0, A2 = 0, s1 = 1, s2 = 1, f1 = 0, f2 = 0, go = 0;
+    int g = 0, s1 = 1, s2 = 1, f1 = 0, f2 = 0, go = 0, same12 = 0;
     int pr = 0;
     if (b < ng) {
-      g = blist[tk.first + b];
+      const int4 i0 = t.box_info[2 * (tk.first + b)], i1 = t.box_info[2 * (tk.first + b) + 1];
+      g = i0.x;
       pr = (g & BOX_PAIRED) != 0;
       g &= ~BOX_PAIRED;
-      A1 = t.cp_a[g];
-      A2 = t.cp_b[g];
-      go = t.cp_grid_off[g];
-      s1 = t.ch_start[A1 + 1] - t.ch_start[A1];
-      s2 = t.ch_start[A2 + 1] - t.ch_start[A2];
-      f1 = t.fcls_off[A1];
-      f2 = t.fcls_off[A2];
+      go = i0.y;
+      s1 = i0.z;
+      s2 = i0.w;
+      f1 = i1.x;
+      f2 = i1.y;
+      same12 = i1.z;
     }
     const int nr = b < ng ? (mode == 2 ? tk.cnt : s1 * s2) : 0;
     const int fn = b < ng ? s1 * s1 + s2 * s2 : 0;
@@ -141,7 +141,7 @@ __global__ void __launch_bounds__(NT) k_box2b(DevTab t, Ops o, double *__restric
     const int q34 = s3 * s3 + s4 * s4;
     if (b < ng) {
       bx[b] = make_int4(g, nr, gz - nr * ldc, q34 + fz - fn);
-      bs[b] = make_int4(s1, s2, A1 == A2, rz - nr);
+      bs[b] = make_int4(s1, s2, same12, rz - nr);
       brc[b] = make_float2(1.0f / (float)nr, 1.0f / (float)s2);
       pre[b] = e - nr * nc;
       bfo[b] = make_int2(f1, f2);

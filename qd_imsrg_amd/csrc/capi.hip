@@ -771,6 +771,13 @@ int build_worklists(qd_context *c) {
     }
     if (blist.empty()) blist.push_back(0);
     if ((rc = dev_upload(c, blist, &c->blist))) return rc;
+    std::vector<int4> info(2 * blist.size());
+    for (size_t e = 0; e < blist.size(); e++) {
+      const int g = blist[e] & ~BOX_PAIRED, A1 = ch.cp_a[g], A2 = ch.cp_b[g];
+      info[2 * e] = make_int4(blist[e], ch.cp_grid_off[g], csz(A1), csz(A2));
+      info[2 * e + 1] = make_int4(ch.fcls_off[A1], ch.fcls_off[A2], A1 == A2, 0);
+    }
+    if ((rc = dev_upload(c, info, (int4 **)&c->dt.box_info))) return rc;
   }
   if ((rc = dev_upload(c, pseg, &c->pphh_segs))) return rc;
   if ((rc = dev_upload(c, ptiles, &c->pphh_tiles))) return rc;

@@ -132,6 +132,9 @@ struct DevTab {
   const int64_t *box_pb;
   const int *box_pl;
   int nblist;
+  // per entry of the box list, two int4: (the entry, grid offset in cp_grid,
+  // |A1|, |A2|) and (f offset of A1, f offset of A2, A1 == A2, 0)
+  const int4 *box_info;
 };
 constexpr uint32_t CC_ZERO = 0x3fffffffu;
 
