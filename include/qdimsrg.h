@@ -65,7 +65,7 @@ const char *qd_last_error(void);
  * QD_BOX_CAP grid elements per two-body box task (16..3600, default 3600),
  * QD_GATHER_TABLES=0 per-element calc_cc and 221 paths instead of the gather
  * tables, QD_GEMM_WG_PER_CU work-groups per CU of the persistent GEMM grid
- * (1..16, default 6), QD_BOX_PAIRS=0 every class-pair box of the two-body
+ * (1..16, default 8), QD_BOX_PAIRS=0 every class-pair box of the two-body
  * pass evaluated on its own instead of paired with its transpose (results
  * then differ at rounding level only), QD_BOX_WIDE=0/1 the narrow (512
  * threads, 3600-element grids) or wide (1024 threads, 8000) box tasks instead

@@ -1481,10 +1481,12 @@ int qd_context_create(int nbody, int nshell, qd_context **out) {
     if (hipGetDevice(&dev) == hipSuccess &&
         hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev) == hipSuccess && ncu > 0) {
       // work-groups per CU of the persistent GEMM grid (QD_GEMM_WG_PER_CU, 1..16;
-      // default 6, the FP64 mixed launch's occupancy: at N=20 R=14 steps of
-      // 0.560 / 0.532 / 0.541 ms at 4 / 6 / 8)
+      // default 8: two more than the FP64 mixed launch keeps resident, which
+      // then start as the first ones finish and even out the tail.  N=20
+      // R=14 products class 0.201 / 0.195 ms and accumulating list 0.076 /
+      // 0.079 ms at 6 / 8; N=20 R=16 grouped flow 0.120 / 0.117 s)
       const char *e = getenv("QD_GEMM_WG_PER_CU");
-      const int per = e && atoi(e) >= 1 && atoi(e) <= 16 ? atoi(e) : 6;
+      const int per = e && atoi(e) >= 1 && atoi(e) <= 16 ? atoi(e) : 8;
       c->gemm_grid = per * ncu;
     }
   }
